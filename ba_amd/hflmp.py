@@ -1,0 +1,348 @@
+"""Python binding of the gfx950 lmp platform (libhflmp_hip.so) and the
+structured FE generator (libhflmp_gen.so).
+
+A thin ctypes layer over the C ABI in include/hflmp_hip.h: device buffers,
+the CSR matrix arrays, the BLAS-1 / SpMV / Jacobi entry points and the
+device-resident solvers.  There is no CPU fallback: if the native library
+is missing or no GPU is visible, constructing a Context raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(_HERE, "lib")
+HIP_LIB = os.path.join(LIB_DIR, "libhflmp_hip.so")
+GEN_LIB = os.path.join(LIB_DIR, "libhflmp_gen.so")
+
+HFLMP_OK = 0
+STATUS = {0: "converged", 1: "max iterations", 2: "diverged", 3: "breakdown"}
+
+_hip = None
+_gen = None
+
+
+class HflmpError(RuntimeError):
+    pass
+
+
+class SolveInfo(C.Structure):
+    _fields_ = [("iters", C.c_int), ("status", C.c_int), ("res_init", C.c_double),
+                ("res_final", C.c_double), ("seconds", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def _vp(x):
+    return C.c_void_p(x)
+
+
+def hip_lib():
+    """Load libhflmp_hip.so (raises if it was not built)."""
+    global _hip
+    if _hip is None:
+        if not os.path.exists(HIP_LIB):
+            raise HflmpError("%s not built: run __graft_entry__.build()" % HIP_LIB)
+        L = C.CDLL(HIP_LIB)
+        vp, i, sz, d, f = C.c_void_p, C.c_int, C.c_size_t, C.c_double, C.c_float
+        L.hflmp_ctx_last_error.restype = C.c_char_p
+        L.hflmp_ctx_last_error.argtypes = [vp]
+        L.hflmp_ctx_create.argtypes = [i, C.POINTER(vp)]
+        for n in ("hflmp_ctx_destroy", "hflmp_ctx_sync"):
+            getattr(L, n).argtypes = [vp]
+        L.hflmp_device_count.argtypes = [C.POINTER(i)]
+        L.hflmp_malloc.argtypes = [vp, sz, C.POINTER(vp)]
+        L.hflmp_free.argtypes = [vp, vp]
+        for n in ("h2d", "d2h", "d2d"):
+            getattr(L, "hflmp_memcpy_" + n).argtypes = [vp, vp, vp, sz]
+        L.hflmp_fill_zero.argtypes = [vp, vp, sz]
+        L.hflmp_event_create.argtypes = [vp, C.POINTER(vp)]
+        L.hflmp_event_record.argtypes = [vp, vp]
+        L.hflmp_event_elapsed_ms.argtypes = [vp, vp, vp, C.POINTER(C.c_float)]
+        L.hflmp_event_destroy.argtypes = [vp, vp]
+        L.hflmp_pcg_workspace_bytes.restype = sz
+        L.hflmp_pcg_workspace_bytes.argtypes = [i, i, i]
+        for s, ct in (("f64", d), ("f32", f)):
+            g = lambda n: getattr(L, "hflmp_%s_%s" % (n, s))
+            g("dot").argtypes = [vp, i, vp, vp, C.POINTER(ct)]
+            g("dot_dev").argtypes = [vp, i, vp, vp, vp]
+            for n in ("nrm2", "norm1", "normmax"):
+                g(n).argtypes = [vp, i, vp, C.POINTER(ct)]
+            for n in ("argmax_abs", "argmin_abs", "count_nan"):
+                g(n).argtypes = [vp, i, vp, C.POINTER(i)]
+            for n in ("axpy", "scaleadd"):
+                g(n).argtypes = [vp, i, ct, vp, vp]
+            g("scal").argtypes = [vp, i, ct, vp]
+            for n in ("ewmul", "copy"):
+                g(n).argtypes = [vp, i, vp, vp]
+            g("rot").argtypes = [vp, i, vp, vp, ct, ct]
+            g("jacobi_apply").argtypes = [vp, i, vp, vp, vp]
+            for n in ("gather", "scatter", "scatter_add"):
+                g(n).argtypes = [vp, i, vp, vp, vp]
+            for n in ("csr_spmv", "csr_spmv_add", "csr_spmv_nodiag"):
+                g(n).argtypes = [vp, i, i, i, vp, vp, vp, vp, vp]
+            g("csr_extract_invdiag").argtypes = [vp, i, vp, vp, vp, vp, C.POINTER(i)]
+            g("pcg_jacobi").argtypes = [vp, i, i, vp, vp, vp, vp, vp, vp, i, d, d, d,
+                                        vp, sz, C.POINTER(SolveInfo)]
+        L.hflmp_cast_f64_to_f32.argtypes = [vp, i, vp, vp]
+        L.hflmp_cast_f32_to_f64.argtypes = [vp, i, vp, vp]
+        L.hflmp_cast_csr_f64_to_f32.argtypes = [vp, i, vp, vp]
+        L.hflmp_csr_diag_ptr.argtypes = [vp, i, vp, vp, vp]
+        L.hflmp_csr_zero_rows_f64.argtypes = [vp, i, vp, vp, vp, vp, d]
+        _hip = L
+    return _hip
+
+
+def gen_lib():
+    global _gen
+    if _gen is None:
+        if not os.path.exists(GEN_LIB):
+            raise HflmpError("%s not built: run __graft_entry__.build()" % GEN_LIB)
+        L = C.CDLL(GEN_LIB)
+        L.hflmp_gen_fe_sizes.argtypes = [C.c_int, C.c_int, C.c_int,
+                                         C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        L.hflmp_gen_fe_csr.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p,
+                                       C.c_void_p, C.c_void_p, C.c_void_p]
+        L.hflmp_gen_kappa.argtypes = [C.c_int, C.c_int, C.c_double, C.c_uint64,
+                                      C.c_void_p]
+        L.hflmp_gen_rhs.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_void_p]
+        L.hflmp_gen_set_threads.argtypes = [C.c_int]
+        _gen = L
+    return _gen
+
+
+# ---- structured FE systems (host) ---------------------------------------------
+def fe_sizes(order, dim, n):
+    N, Z = C.c_int64(), C.c_int64()
+    if gen_lib().hflmp_gen_fe_sizes(order, dim, n, C.byref(N), C.byref(Z)):
+        raise ValueError("unsupported element order=%d dim=%d n=%d" % (order, dim, n))
+    return N.value, Z.value
+
+
+def fe_kappa(dim, n, sigma, seed):
+    k = np.empty(n ** dim, np.float64)
+    gen_lib().hflmp_gen_kappa(dim, n, sigma, seed, k.ctypes.data)
+    return k
+
+
+def fe_system(order, dim, n, kappa=None):
+    """(row_ptr, col, val) of the Dirichlet FE stiffness matrix."""
+    N, Z = fe_sizes(order, dim, n)
+    rp = np.empty(N + 1, np.int32)
+    ci = np.empty(Z, np.int32)
+    v = np.empty(Z, np.float64)
+    kp = None
+    if kappa is not None:
+        kappa = np.ascontiguousarray(kappa, np.float64)
+        kp = kappa.ctypes.data
+    rc = gen_lib().hflmp_gen_fe_csr(order, dim, n, kp, rp.ctypes.data, ci.ctypes.data,
+                                    v.ctypes.data)
+    if rc:
+        raise HflmpError("hflmp_gen_fe_csr failed (%d)" % rc)
+    return rp, ci, v
+
+
+def fe_rhs(order, dim, n, seed):
+    N, _ = fe_sizes(order, dim, n)
+    b = np.empty(N, np.float64)
+    gen_lib().hflmp_gen_rhs(order, dim, n, seed, b.ctypes.data)
+    return b
+
+
+# ---- device context and buffers ---------------------------------------------------
+def device_count():
+    n = C.c_int()
+    hip_lib().hflmp_device_count(C.byref(n))
+    return n.value
+
+
+class Context:
+    """One device, one stream (hflmp_ctx)."""
+
+    def __init__(self, device=0):
+        L = hip_lib()
+        h = C.c_void_p()
+        rc = L.hflmp_ctx_create(device, C.byref(h))
+        if rc != HFLMP_OK:
+            raise HflmpError("hflmp_ctx_create(%d) failed (%d): no usable GPU" % (device, rc))
+        self.h = h
+        self.L = L
+
+    def check(self, rc, what=""):
+        if rc != HFLMP_OK:
+            msg = self.L.hflmp_ctx_last_error(self.h)
+            raise HflmpError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else ""))
+
+    def sync(self):
+        self.check(self.L.hflmp_ctx_sync(self.h), "sync")
+
+    def close(self):
+        if self.h:
+            self.L.hflmp_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def event(self):
+        return Event(self)
+
+    def empty(self, n, dtype):
+        return DeviceArray(self, n, dtype)
+
+    def zeros(self, n, dtype):
+        a = DeviceArray(self, n, dtype)
+        if a.nbytes:
+            self.check(self.L.hflmp_fill_zero(self.h, a.ptr, a.nbytes), "fill_zero")
+        return a
+
+    def upload(self, host):
+        host = np.ascontiguousarray(host)
+        a = DeviceArray(self, host.size, host.dtype)
+        if a.nbytes:
+            self.check(self.L.hflmp_memcpy_h2d(self.h, a.ptr, host.ctypes.data, a.nbytes),
+                       "memcpy_h2d")
+        return a
+
+
+class Event:
+    """A HIP event recorded on the context's stream."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        ctx.check(ctx.L.hflmp_event_create(ctx.h, C.byref(self.h)), "event_create")
+
+    def record(self):
+        self.ctx.check(self.ctx.L.hflmp_event_record(self.ctx.h, self.h), "event_record")
+
+    def elapsed_ms(self, start):
+        ms = C.c_float()
+        self.ctx.check(self.ctx.L.hflmp_event_elapsed_ms(self.ctx.h, start.h, self.h,
+                                                         C.byref(ms)), "event_elapsed")
+        return ms.value
+
+    def destroy(self):
+        if self.h:
+            self.ctx.L.hflmp_event_destroy(self.ctx.h, self.h)
+            self.h = None
+
+
+class DeviceArray:
+    """A device allocation holding `n` items of `dtype` (freed on close)."""
+
+    def __init__(self, ctx, n, dtype):
+        self.ctx = ctx
+        self.n = int(n)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = self.n * self.dtype.itemsize
+        p = C.c_void_p()
+        ctx.check(ctx.L.hflmp_malloc(ctx.h, self.nbytes, C.byref(p)), "malloc(%d)" % self.nbytes)
+        self.ptr = p
+
+    def download(self):
+        out = np.empty(self.n, self.dtype)
+        if self.nbytes:
+            self.ctx.check(self.ctx.L.hflmp_memcpy_d2h(self.ctx.h, out.ctypes.data, self.ptr,
+                                                       self.nbytes), "memcpy_d2h")
+        return out
+
+    def copy_from_host(self, host):
+        host = np.ascontiguousarray(host, self.dtype)
+        assert host.size == self.n
+        self.ctx.check(self.ctx.L.hflmp_memcpy_h2d(self.ctx.h, self.ptr, host.ctypes.data,
+                                                   self.nbytes), "memcpy_h2d")
+
+    def free(self):
+        if self.ptr is not None and self.ptr.value:
+            self.ctx.L.hflmp_free(self.ctx.h, self.ptr)
+        self.ptr = C.c_void_p()
+
+
+def _sfx(dtype):
+    return "f64" if np.dtype(dtype) == np.float64 else "f32"
+
+
+class CsrMatrix:
+    """Device CSR arrays (int32 row_ptr/col, float64 or float32 values)."""
+
+    def __init__(self, ctx, row_ptr, col, val, ncol=None):
+        self.ctx = ctx
+        self.nrow = len(row_ptr) - 1
+        self.ncol = self.nrow if ncol is None else ncol
+        self.nnz = len(col)
+        self.rp = ctx.upload(np.asarray(row_ptr, np.int32))
+        self.ci = ctx.upload(np.asarray(col, np.int32))
+        self.val = ctx.upload(np.asarray(val))
+        self.dtype = self.val.dtype
+
+    def astype_f32(self):
+        """A float32 copy of the values sharing row_ptr/col."""
+        m = CsrMatrix.__new__(CsrMatrix)
+        m.ctx, m.nrow, m.ncol, m.nnz = self.ctx, self.nrow, self.ncol, self.nnz
+        m.rp, m.ci = self.rp, self.ci
+        m.val = self.ctx.empty(self.nnz, np.float32)
+        self.ctx.check(self.ctx.L.hflmp_cast_csr_f64_to_f32(self.ctx.h, self.nnz, self.val.ptr,
+                                                            m.val.ptr), "cast_csr")
+        m.dtype = np.dtype(np.float32)
+        return m
+
+    def free(self):
+        for a in (self.rp, self.ci, self.val):
+            a.free()
+
+
+# ---- operations ---------------------------------------------------------------------
+def _call(ctx, name, *args):
+    ctx.check(getattr(ctx.L, name)(ctx.h, *args), name)
+
+
+def spmv(ctx, A, x, y, mode="assign"):
+    name = {"assign": "csr_spmv", "add": "csr_spmv_add", "nodiag": "csr_spmv_nodiag"}[mode]
+    _call(ctx, "hflmp_%s_%s" % (name, _sfx(A.dtype)), A.nrow, A.ncol, A.nnz, A.rp.ptr, A.ci.ptr,
+          A.val.ptr, x.ptr, y.ptr)
+
+
+def invdiag(ctx, A):
+    d = ctx.empty(A.nrow, A.dtype)
+    bad = C.c_int()
+    _call(ctx, "hflmp_csr_extract_invdiag_" + _sfx(A.dtype), A.nrow, A.rp.ptr, A.ci.ptr,
+          A.val.ptr, d.ptr, C.byref(bad))
+    return d
+
+
+def dot(ctx, x, y):
+    ct = C.c_double if x.dtype == np.float64 else C.c_float
+    r = ct()
+    _call(ctx, "hflmp_dot_" + _sfx(x.dtype), x.n, x.ptr, y.ptr, C.byref(r))
+    return r.value
+
+
+def nrm2(ctx, x):
+    ct = C.c_double if x.dtype == np.float64 else C.c_float
+    r = ct()
+    _call(ctx, "hflmp_nrm2_" + _sfx(x.dtype), x.n, x.ptr, C.byref(r))
+    return r.value
+
+
+def pcg_jacobi(ctx, A, invd, b, x, maxit=10000, atol=0.0, rtol=1e-8, dtol=1e6, work=None):
+    """Device-resident Jacobi-PCG; x holds x0 on entry and the solution on exit."""
+    vb = A.dtype.itemsize
+    need = ctx.L.hflmp_pcg_workspace_bytes(A.nrow, A.nnz, vb)
+    own = work is None
+    if own:
+        work = ctx.empty(need, np.uint8)
+    info = SolveInfo()
+    try:
+        _call(ctx, "hflmp_pcg_jacobi_" + _sfx(A.dtype), A.nrow, A.nnz, A.rp.ptr, A.ci.ptr,
+              A.val.ptr, invd.ptr, b.ptr, x.ptr, maxit, atol, rtol, dtol, work.ptr,
+              work.nbytes, C.byref(info))
+    finally:
+        if own:
+            work.free()
+    return info.as_dict()

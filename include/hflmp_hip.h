@@ -1,0 +1,230 @@
+/*
+ * hflmp_hip.h -- C ABI of the gfx950 HIP platform for HiFlow3's local
+ * multi-platform linear algebra ("lmp").
+ *
+ * Everything here is extern "C" with plain pointers and sizes.  Device
+ * pointers are HIP device allocations (hflmp_malloc or any hipMalloc'd
+ * memory on the context's device).  All routines enqueue on the context's
+ * stream; the ones returning a host scalar synchronise that stream.
+ *
+ * Return value: HFLMP_OK (0) or a negative HFLMP_E* code; the message of
+ * the last error is hflmp_ctx_last_error(ctx).  The C++ platform classes
+ * turn a failure into LOG_ERROR + exit, the reference's convention
+ * (linear_algebra/lmp/cuda/cuda_utils.h:25-55, common/macros.h:79-84).
+ *
+ * Each entry point names the reference interface it replaces (paths are
+ * relative to HiFlow3's src/).
+ */
+#ifndef HFLMP_HIP_H
+#define HFLMP_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HFLMP_OK 0
+#define HFLMP_EINVAL -1   /* bad argument (size, null pointer, shape)      */
+#define HFLMP_EHIP -2     /* HIP runtime error                             */
+#define HFLMP_ENOMEM -3   /* allocation failed / admission budget exceeded */
+#define HFLMP_ESTATE -4   /* object not built / wrong state                */
+#define HFLMP_ENUMERIC -5 /* zero diagonal, NaN, breakdown                 */
+
+typedef struct hflmp_ctx hflmp_ctx;
+
+/* ---- platform / context -------------------------------------------------
+ * Replaces init_platform / stop_platform (linear_algebra/lmp/
+ * platform_management.h:32-35, cuda/platform_management_gpu.cu:78-143). */
+int hflmp_device_count(int *count);
+int hflmp_ctx_create(int device, hflmp_ctx **ctx);
+int hflmp_ctx_destroy(hflmp_ctx *ctx);
+/* Synchronise the context's stream (lVector::Sync, lvector.h:266). */
+int hflmp_ctx_sync(hflmp_ctx *ctx);
+const char *hflmp_ctx_last_error(const hflmp_ctx *ctx);
+/* The stream the context enqueues on (a hipStream_t).  set_stream(NULL)
+ * returns to the context's own stream. */
+int hflmp_ctx_get_stream(hflmp_ctx *ctx, void **stream);
+int hflmp_ctx_set_stream(hflmp_ctx *ctx, void *stream);
+int hflmp_ctx_device(const hflmp_ctx *ctx, int *device);
+
+/* HIP events on the context's stream, for timing kernels where they run
+ * (the reference times operations on the host only: common/timer.h:30-100,
+ * tools/ScopedTimer.h:33-135).  elapsed_ms waits for `stop`. */
+int hflmp_event_create(hflmp_ctx *ctx, void **event);
+int hflmp_event_record(hflmp_ctx *ctx, void *event);
+int hflmp_event_elapsed_ms(hflmp_ctx *ctx, void *start, void *stop, float *ms);
+int hflmp_event_destroy(hflmp_ctx *ctx, void *event);
+
+/* ---- memory ----------------------------------------------------------------
+ * Replaces lmp_mem / mem_cuda (cuda/mem_cuda.cu:42-135: memcpy2dev,
+ * memcpy2host, memcpydev, memsetdev). */
+int hflmp_malloc(hflmp_ctx *ctx, size_t bytes, void **dptr);
+int hflmp_free(hflmp_ctx *ctx, void *dptr);
+int hflmp_memcpy_h2d(hflmp_ctx *ctx, void *dst, const void *src, size_t bytes);
+int hflmp_memcpy_d2h(hflmp_ctx *ctx, void *dst, const void *src, size_t bytes);
+int hflmp_memcpy_d2d(hflmp_ctx *ctx, void *dst, const void *src, size_t bytes);
+/* zero fill by a kernel (no hipMemset node, safe under graph capture) */
+int hflmp_fill_zero(hflmp_ctx *ctx, void *dptr, size_t bytes);
+
+/* ---- indexed access (cuda/mem_cuda.h:21-92: kernel_cudasetvalues,
+ *      kernel_cudagetvalues, kernel_cudaaddblockvalues) -------------------- */
+/* dst[k] = src[idx[k]], k < n  (GetIndexedValues, lvector_gpu.cc:231-255) */
+int hflmp_gather_f64(hflmp_ctx *ctx, int n, const int *idx, const double *src,
+                     double *dst);
+int hflmp_gather_f32(hflmp_ctx *ctx, int n, const int *idx, const float *src,
+                     float *dst);
+/* dst[idx[k]] = src[k]  (SetValues / SetIndexedValues, lvector_gpu.cc:125-186) */
+int hflmp_scatter_f64(hflmp_ctx *ctx, int n, const int *idx, const double *src,
+                      double *dst);
+int hflmp_scatter_f32(hflmp_ctx *ctx, int n, const int *idx, const float *src,
+                      float *dst);
+/* dst[idx[k]] += src[k]  (add_values, lvector.h:76-83) */
+int hflmp_scatter_add_f64(hflmp_ctx *ctx, int n, const int *idx,
+                          const double *src, double *dst);
+int hflmp_scatter_add_f32(hflmp_ctx *ctx, int n, const int *idx,
+                          const float *src, float *dst);
+
+/* ---- BLAS-1 (cuda/GPUblas_lvector.cu:75-433, cuBLAS calls) --------------
+ * Scalar results: the plain form writes a HOST scalar and synchronises the
+ * stream (lVector::Dot returns by value); the _dev form writes a DEVICE
+ * scalar and does not synchronise. */
+int hflmp_dot_f64(hflmp_ctx *ctx, int n, const double *x, const double *y,
+                  double *result);
+int hflmp_dot_f32(hflmp_ctx *ctx, int n, const float *x, const float *y,
+                  float *result);
+int hflmp_dot_dev_f64(hflmp_ctx *ctx, int n, const double *x, const double *y,
+                      double *d_result);
+int hflmp_dot_dev_f32(hflmp_ctx *ctx, int n, const float *x, const float *y,
+                      float *d_result);
+/* sqrt(Dot(x,x)) (CPUsimple_lvector.cc:144-153; CoupledVector::Norm2,
+ * coupled_vector.cc:392-395) */
+int hflmp_nrm2_f64(hflmp_ctx *ctx, int n, const double *x, double *result);
+int hflmp_nrm2_f32(hflmp_ctx *ctx, int n, const float *x, float *result);
+int hflmp_norm1_f64(hflmp_ctx *ctx, int n, const double *x, double *result);
+int hflmp_norm1_f32(hflmp_ctx *ctx, int n, const float *x, float *result);
+/* max |x_i| and the first index attaining it (NormMax / ArgMax);
+ * argmin gives the first index of min |x_i| (ArgMin) */
+int hflmp_normmax_f64(hflmp_ctx *ctx, int n, const double *x, double *result);
+int hflmp_normmax_f32(hflmp_ctx *ctx, int n, const float *x, float *result);
+int hflmp_argmax_abs_f64(hflmp_ctx *ctx, int n, const double *x, int *index);
+int hflmp_argmax_abs_f32(hflmp_ctx *ctx, int n, const float *x, int *index);
+int hflmp_argmin_abs_f64(hflmp_ctx *ctx, int n, const double *x, int *index);
+int hflmp_argmin_abs_f32(hflmp_ctx *ctx, int n, const float *x, int *index);
+/* number of NaN entries (ContainsNaN, lvector_cpu.h:142-145) */
+int hflmp_count_nan_f64(hflmp_ctx *ctx, int n, const double *x, int *count);
+int hflmp_count_nan_f32(hflmp_ctx *ctx, int n, const float *x, int *count);
+/* y += a x  (Axpy) */
+int hflmp_axpy_f64(hflmp_ctx *ctx, int n, double a, const double *x, double *y);
+int hflmp_axpy_f32(hflmp_ctx *ctx, int n, float a, const float *x, float *y);
+/* y = a y + x  (ScaleAdd, lvector.h:282-286) */
+int hflmp_scaleadd_f64(hflmp_ctx *ctx, int n, double a, const double *x,
+                       double *y);
+int hflmp_scaleadd_f32(hflmp_ctx *ctx, int n, float a, const float *x, float *y);
+/* y *= a  (Scale) */
+int hflmp_scal_f64(hflmp_ctx *ctx, int n, double a, double *y);
+int hflmp_scal_f32(hflmp_ctx *ctx, int n, float a, float *y);
+/* y[i] *= x[i]  (ElementWiseMult, kernel_cudamultvalues mem_cuda.h:83) */
+int hflmp_ewmul_f64(hflmp_ctx *ctx, int n, const double *x, double *y);
+int hflmp_ewmul_f32(hflmp_ctx *ctx, int n, const float *x, float *y);
+/* y = x */
+int hflmp_copy_f64(hflmp_ctx *ctx, int n, const double *x, double *y);
+int hflmp_copy_f32(hflmp_ctx *ctx, int n, const float *x, float *y);
+/* (Rot, lvector.h:306) x' = c x + s y, y' = c y - s x */
+int hflmp_rot_f64(hflmp_ctx *ctx, int n, double *x, double *y, double c,
+                  double s);
+int hflmp_rot_f32(hflmp_ctx *ctx, int n, float *x, float *y, float c, float s);
+/* precision casts (CastFrom/CastTo, kernel_cudacast* mem_cuda.h:94-132) */
+int hflmp_cast_f64_to_f32(hflmp_ctx *ctx, int n, const double *x, float *y);
+int hflmp_cast_f32_to_f64(hflmp_ctx *ctx, int n, const float *x, double *y);
+
+/* ---- CSR matrix (zero-based, int32 row_ptr[nrow+1], col[nnz]) -----------
+ * Replaces GPUscalar_CSR_lMatrix::VectorMult / VectorMultAdd /
+ * VectorMultNoDiag (cuda/GPUscalar_csr_lmatrix.cu:67-210) and
+ * GPUcublas2_CSR_lMatrix::VectorMult (cuSPARSE csrmv).  Each row is summed
+ * in ascending storage order from 0 with rounded products, the order of
+ * CPUsimple_CSR_lMatrix::VectorMult (CPUsimple_csr_lmatrix.cc:76-111). */
+int hflmp_csr_spmv_f64(hflmp_ctx *ctx, int nrow, int ncol, int nnz,
+                       const int *row_ptr, const int *col, const double *val,
+                       const double *x, double *y);
+int hflmp_csr_spmv_f32(hflmp_ctx *ctx, int nrow, int ncol, int nnz,
+                       const int *row_ptr, const int *col, const float *val,
+                       const float *x, float *y);
+/* y += A x (VectorMultAdd, lmatrix_csr_cpu.cc:1424-1450) */
+int hflmp_csr_spmv_add_f64(hflmp_ctx *ctx, int nrow, int ncol, int nnz,
+                           const int *row_ptr, const int *col,
+                           const double *val, const double *x, double *y);
+int hflmp_csr_spmv_add_f32(hflmp_ctx *ctx, int nrow, int ncol, int nnz,
+                           const int *row_ptr, const int *col, const float *val,
+                           const float *x, float *y);
+/* y = (A - D) x (kernel_spmvnd_csr_scalar, cuda_spmv_kernel_csr_scalar.h:48) */
+int hflmp_csr_spmv_nodiag_f64(hflmp_ctx *ctx, int nrow, int ncol, int nnz,
+                              const int *row_ptr, const int *col,
+                              const double *val, const double *x, double *y);
+int hflmp_csr_spmv_nodiag_f32(hflmp_ctx *ctx, int nrow, int ncol, int nnz,
+                              const int *row_ptr, const int *col,
+                              const float *val, const float *x, float *y);
+/* invdiag[i] = 1/a_ii (extract_invdiagelements, lmatrix_csr_cpu.cc:
+ * 2420-2442); *n_bad (host) = rows whose diagonal is missing or zero. */
+int hflmp_csr_extract_invdiag_f64(hflmp_ctx *ctx, int nrow, const int *row_ptr,
+                                  const int *col, const double *val,
+                                  double *invdiag, int *n_bad);
+int hflmp_csr_extract_invdiag_f32(hflmp_ctx *ctx, int nrow, const int *row_ptr,
+                                  const int *col, const float *val,
+                                  float *invdiag, int *n_bad);
+/* diag_ptr[i] = position of a_ii, -1 if absent (get_diag_ptr,
+ * lmatrix_formats.h:489-532) */
+int hflmp_csr_diag_ptr(hflmp_ctx *ctx, int nrow, const int *row_ptr,
+                       const int *col, int *diag_ptr);
+/* Dirichlet rows: val = 0 in listed rows, a_ii = alpha (ZeroRows,
+ * lmatrix_csr_cpu.cc:908-927) */
+int hflmp_csr_zero_rows_f64(hflmp_ctx *ctx, int nrows_zero, const int *rows,
+                            const int *row_ptr, const int *col, double *val,
+                            double alpha);
+/* val_f32 = (float) val_f64 (CastFrom on matrices, lmatrix.h) */
+int hflmp_cast_csr_f64_to_f32(hflmp_ctx *ctx, int nnz, const double *val,
+                              float *val32);
+
+/* ---- preconditioners ------------------------------------------------------
+ * Jacobi apply z = invdiag .* r fused into one pass
+ * (lPreconditioner_Jacobi::ApplylPreconditioner, lpreconditioner.cc:184-200:
+ * CopyFrom + ElementWiseMult). */
+int hflmp_jacobi_apply_f64(hflmp_ctx *ctx, int n, const double *invdiag,
+                           const double *r, double *z);
+int hflmp_jacobi_apply_f32(hflmp_ctx *ctx, int n, const float *invdiag,
+                           const float *r, float *z);
+
+/* ---- device-resident Krylov solves ----------------------------------------
+ * Jacobi-preconditioned CG with the arithmetic of CG::SolvePrecond
+ * (linear_solver/cg.h:165-246) and the stopping rule of IterateControl::
+ * Check (common/iterate_control.cc:80-107) on the recursive residual, with
+ * every scalar kept on the device: one iteration is three kernels (SpMV
+ * fused with p.Ap; x/r/z update fused with r.r and r.z; p update) and the
+ * host reads the state only every `poll` iterations.  x holds x0 on entry.
+ * Workspace: hflmp_pcg_workspace_bytes() bytes of device memory. */
+typedef struct {
+  int iters;          /* iterations taken                                  */
+  int status;         /* 0 converged (abs or rel), 1 max iterations,
+                         2 divergence, 3 breakdown (p.Ap <= 0 or NaN)     */
+  double res_init;    /* ||b - A x0||                                      */
+  double res_final;   /* recursive residual at exit                        */
+  double seconds;     /* wall time of the iteration loop                   */
+} hflmp_solve_info;
+
+size_t hflmp_pcg_workspace_bytes(int nrow, int nnz, int value_bytes);
+int hflmp_pcg_jacobi_f64(hflmp_ctx *ctx, int nrow, int nnz, const int *row_ptr,
+                         const int *col, const double *val,
+                         const double *invdiag, const double *b, double *x,
+                         int maxit, double atol, double rtol, double dtol,
+                         void *work, size_t work_bytes, hflmp_solve_info *info);
+int hflmp_pcg_jacobi_f32(hflmp_ctx *ctx, int nrow, int nnz, const int *row_ptr,
+                         const int *col, const float *val,
+                         const float *invdiag, const float *b, float *x,
+                         int maxit, double atol, double rtol, double dtol,
+                         void *work, size_t work_bytes, hflmp_solve_info *info);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

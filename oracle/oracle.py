@@ -1,0 +1,337 @@
+"""ctypes front end of the CPU oracle (oracle/lib/libhflmp_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: the correctness checker and the timed CPU
+baseline.  Only the tests, the smoke check and the CPU-baseline leg of the
+benchmark import this module; the product never does.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "lib", "libhflmp_oracle.so")
+_lib = None
+
+PREC_NONE, PREC_JACOBI, PREC_SGS, PREC_MCSGS = 0, 1, 2, 3
+
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_vp = C.c_void_p
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-B", "-C", _HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        L.or_splitmix64_at.restype = C.c_uint64
+        L.or_splitmix64_at.argtypes = [C.c_uint64, C.c_uint64]
+        L.or_uniform_pm1.restype = C.c_double
+        L.or_uniform_pm1.argtypes = [C.c_uint64, C.c_uint64]
+        L.or_gauss.restype = C.c_double
+        L.or_gauss.argtypes = [C.c_uint64, C.c_uint64]
+        L.or_fe_sizes.argtypes = [C.c_int, C.c_int, C.c_int,
+                                  C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        L.or_fe_assemble.argtypes = [C.c_int, C.c_int, C.c_int, _vp,
+                                     _i32p, _i32p, _f64p]
+        L.or_fe_kappa.argtypes = [C.c_int, C.c_int, C.c_double, C.c_uint64, _f64p]
+        L.or_fe_rhs.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint64, _f64p]
+        L.or_fe_boundary_mask.argtypes = [C.c_int, C.c_int, C.c_int, _u8p]
+        L.or_coo_to_csr_d.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _f64p,
+                                      _i32p, _i32p, _f64p]
+        L.or_csr_diag_ptr.argtypes = [C.c_int, _i32p, _i32p, _i32p]
+        L.or_csr_zero_rows_d.argtypes = [_i32p, C.c_int, _i32p, _i32p, _f64p,
+                                         C.c_double]
+        L.or_multicoloring.argtypes = [C.c_int, _i32p, _i32p, C.POINTER(C.c_int),
+                                       _i32p, _i32p, _i32p]
+        L.or_cast_d2s.argtypes = [C.c_int, _f64p, _f32p]
+        L.or_cast_s2d.argtypes = [C.c_int, _f32p, _f64p]
+        for s, fp, ct in (("d", _f64p, C.c_double), ("s", _f32p, C.c_float)):
+            g = lambda n: getattr(L, "or_%s_%s" % (n, s))
+            for n in ("spmv", "spmv_add", "spmv_nodiag"):
+                g(n).argtypes = [C.c_int, _i32p, _i32p, fp, fp, fp]
+            g("spmv_omp").argtypes = [C.c_int, _i32p, _i32p, fp, fp, fp, C.c_int]
+            g("dot").argtypes = [C.c_int, fp, fp]
+            g("dot").restype = ct
+            for n in ("nrm2", "norm1", "normmax"):
+                g(n).argtypes = [C.c_int, fp]
+                g(n).restype = ct
+            for n in ("axpy", "scaleadd"):
+                g(n).argtypes = [C.c_int, ct, fp, fp]
+            g("scale").argtypes = [C.c_int, ct, fp]
+            g("ewmult").argtypes = [C.c_int, fp, fp]
+            g("invdiag").argtypes = [C.c_int, _i32p, _i32p, fp, fp]
+            g("jacobi_apply").argtypes = [C.c_int, fp, fp, fp]
+            g("sgs_apply").argtypes = [C.c_int, _i32p, _i32p, fp, fp, fp]
+            g("mcsgs_apply").argtypes = [C.c_int, _i32p, _i32p, fp, C.c_int,
+                                         _i32p, _i32p, _i32p, ct, fp, fp]
+            g("cg").argtypes = [C.c_int, _i32p, _i32p, fp, C.c_int, _vp,
+                                C.c_int, _vp, _vp, _vp, fp, fp, C.c_int,
+                                C.c_double, C.c_double, C.c_double,
+                                C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                C.POINTER(C.c_double), _vp]
+            g("gmres").argtypes = [C.c_int, _i32p, _i32p, fp, C.c_int, _vp,
+                                   C.c_int, _vp, _vp, _vp, C.c_int, C.c_int,
+                                   fp, fp, C.c_int, C.c_double, C.c_double,
+                                   C.c_double, C.POINTER(C.c_int),
+                                   C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.or_mpir_ds.argtypes = [C.c_int, _i32p, _i32p, _f64p, _f32p, _f64p,
+                                 _f64p, C.c_int, C.c_double, C.c_double,
+                                 C.c_int, C.c_double, C.POINTER(C.c_int),
+                                 C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                 C.POINTER(C.c_double)]
+        _lib = L
+    return _lib
+
+
+def _sfx(a):
+    return "d" if a.dtype == np.float64 else "s"
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+# ---- generator ------------------------------------------------------------
+def fe_sizes(order, dim, n):
+    N, Z = C.c_int64(), C.c_int64()
+    if lib().or_fe_sizes(order, dim, n, C.byref(N), C.byref(Z)):
+        raise ValueError("unsupported element")
+    return N.value, Z.value
+
+
+def fe_kappa(dim, n, sigma, seed):
+    k = np.empty(n ** dim, np.float64)
+    lib().or_fe_kappa(dim, n, sigma, seed, k)
+    return k
+
+
+def fe_system(order, dim, n, kappa=None):
+    N, Z = fe_sizes(order, dim, n)
+    rp = np.empty(N + 1, np.int32)
+    ci = np.empty(Z, np.int32)
+    v = np.empty(Z, np.float64)
+    kap = None if kappa is None else np.ascontiguousarray(kappa, np.float64)
+    rc = lib().or_fe_assemble(order, dim, n, _ptr(kap), rp, ci, v)
+    if rc:
+        raise RuntimeError("or_fe_assemble failed %d" % rc)
+    return rp, ci, v
+
+
+def fe_rhs(order, dim, n, seed):
+    N, _ = fe_sizes(order, dim, n)
+    b = np.empty(N, np.float64)
+    lib().or_fe_rhs(order, dim, n, seed, b)
+    return b
+
+
+def fe_boundary_mask(order, dim, n):
+    N, _ = fe_sizes(order, dim, n)
+    m = np.empty(N, np.uint8)
+    lib().or_fe_boundary_mask(order, dim, n, m)
+    return m.astype(bool)
+
+
+def splitmix64_at(seed, i):
+    return lib().or_splitmix64_at(seed, i)
+
+
+def uniform_pm1(seed, i):
+    return lib().or_uniform_pm1(seed, i)
+
+
+# ---- CSR utilities ----------------------------------------------------------
+def coo_to_csr(nrow, ri, ci, v):
+    ri = np.ascontiguousarray(ri, np.int32)
+    ci = np.ascontiguousarray(ci, np.int32)
+    v = np.ascontiguousarray(v, np.float64)
+    rp = np.empty(nrow + 1, np.int32)
+    col = np.empty(len(ci), np.int32)
+    val = np.empty(len(ci), np.float64)
+    lib().or_coo_to_csr_d(nrow, len(ci), ri, ci, v, rp, col, val)
+    return rp, col, val
+
+
+def diag_ptr(rp, ci):
+    d = np.empty(len(rp) - 1, np.int32)
+    missing = lib().or_csr_diag_ptr(len(rp) - 1, rp, ci, d)
+    return d, missing
+
+
+def zero_rows(rows, rp, ci, v, alpha=1.0):
+    rows = np.ascontiguousarray(rows, np.int32)
+    lib().or_csr_zero_rows_d(rows, len(rows), rp, ci, v, alpha)
+
+
+def multicoloring(rp, ci):
+    n = len(rp) - 1
+    maxlen = int(np.max(np.diff(rp))) if n else 0
+    csz = np.zeros(maxlen + 2, np.int32)
+    perm = np.empty(n, np.int32)
+    color = np.empty(n, np.int32)
+    nc = C.c_int()
+    lib().or_multicoloring(n, rp, ci, C.byref(nc), csz, perm, color)
+    return nc.value, csz[: nc.value].copy(), perm, color
+
+
+# ---- kernels -----------------------------------------------------------------
+def spmv(rp, ci, v, x):
+    y = np.empty(len(rp) - 1, v.dtype)
+    getattr(lib(), "or_spmv_" + _sfx(v))(len(rp) - 1, rp, ci, v, x, y)
+    return y
+
+
+def spmv_add(rp, ci, v, x, y):
+    getattr(lib(), "or_spmv_add_" + _sfx(v))(len(rp) - 1, rp, ci, v, x, y)
+    return y
+
+
+def spmv_nodiag(rp, ci, v, x):
+    y = np.empty(len(rp) - 1, v.dtype)
+    getattr(lib(), "or_spmv_nodiag_" + _sfx(v))(len(rp) - 1, rp, ci, v, x, y)
+    return y
+
+
+def spmv_omp(rp, ci, v, x, y, nthreads):
+    getattr(lib(), "or_spmv_omp_" + _sfx(v))(len(rp) - 1, rp, ci, v, x, y,
+                                             nthreads)
+    return y
+
+
+def dot(x, y):
+    return getattr(lib(), "or_dot_" + _sfx(x))(len(x), x, y)
+
+
+def nrm2(x):
+    return getattr(lib(), "or_nrm2_" + _sfx(x))(len(x), x)
+
+
+def norm1(x):
+    return getattr(lib(), "or_norm1_" + _sfx(x))(len(x), x)
+
+
+def normmax(x):
+    return getattr(lib(), "or_normmax_" + _sfx(x))(len(x), x)
+
+
+def axpy(a, x, y):
+    getattr(lib(), "or_axpy_" + _sfx(x))(len(x), a, x, y)
+    return y
+
+
+def scaleadd(a, x, y):
+    getattr(lib(), "or_scaleadd_" + _sfx(x))(len(x), a, x, y)
+    return y
+
+
+def scale(a, y):
+    getattr(lib(), "or_scale_" + _sfx(y))(len(y), a, y)
+    return y
+
+
+def ewmult(x, y):
+    getattr(lib(), "or_ewmult_" + _sfx(x))(len(x), x, y)
+    return y
+
+
+def invdiag(rp, ci, v):
+    d = np.empty(len(rp) - 1, v.dtype)
+    getattr(lib(), "or_invdiag_" + _sfx(v))(len(rp) - 1, rp, ci, v, d)
+    return d
+
+
+def jacobi_apply(invd, r):
+    z = np.empty_like(r)
+    getattr(lib(), "or_jacobi_apply_" + _sfx(r))(len(r), invd, r, z)
+    return z
+
+
+def sgs_apply(rp, ci, v, r):
+    z = np.empty_like(r)
+    getattr(lib(), "or_sgs_apply_" + _sfx(v))(len(r), rp, ci, v, r, z)
+    return z
+
+
+def mcsgs_apply(rp, ci, v, coloring, r, omega=1.0):
+    nc, csz, perm, color = coloring
+    z = np.empty_like(r)
+    getattr(lib(), "or_mcsgs_apply_" + _sfx(v))(len(r), rp, ci, v, nc, csz,
+                                                perm, color, omega, r, z)
+    return z
+
+
+def cast_d2s(x):
+    y = np.empty(len(x), np.float32)
+    lib().or_cast_d2s(len(x), x, y)
+    return y
+
+
+def cast_s2d(x):
+    y = np.empty(len(x), np.float64)
+    lib().or_cast_s2d(len(x), x, y)
+    return y
+
+
+# ---- solvers -------------------------------------------------------------------
+def _prec_args(rp, ci, v, prec, coloring):
+    invd = invdiag(rp, ci, v) if prec == PREC_JACOBI else None
+    if prec == PREC_MCSGS:
+        if coloring is None:
+            coloring = multicoloring(rp, ci)
+        nc, csz, perm, color = coloring
+        return invd, nc, csz, perm, color
+    return invd, 0, None, None, None
+
+
+def cg(rp, ci, v, b, x0=None, prec=PREC_JACOBI, maxit=10000, atol=0.0,
+       rtol=1e-8, dtol=1e6, coloring=None, history=False):
+    """CG::Solve with the reference semantics; returns (x, info dict)."""
+    n = len(rp) - 1
+    x = np.zeros(n, v.dtype) if x0 is None else np.array(x0, v.dtype)
+    invd, nc, csz, perm, color = _prec_args(rp, ci, v, prec, coloring)
+    it, r0, rf = C.c_int(), C.c_double(), C.c_double()
+    hist = np.zeros(maxit + 1, np.float64) if history else None
+    st = getattr(lib(), "or_cg_" + _sfx(v))(
+        n, rp, ci, v, prec, _ptr(invd), nc, _ptr(csz), _ptr(perm), _ptr(color),
+        np.ascontiguousarray(b, v.dtype), x, maxit, atol, rtol, dtol,
+        C.byref(it), C.byref(r0), C.byref(rf), _ptr(hist))
+    info = dict(status=st, iters=it.value, res_init=r0.value, res_final=rf.value)
+    if history:
+        info["history"] = hist[: it.value + 1].copy()
+    return x, info
+
+
+def gmres(rp, ci, v, b, x0=None, prec=PREC_NONE, basis=100, flexible=False,
+          maxit=10000, atol=0.0, rtol=1e-8, dtol=1e6, coloring=None):
+    n = len(rp) - 1
+    x = np.zeros(n, v.dtype) if x0 is None else np.array(x0, v.dtype)
+    invd, nc, csz, perm, color = _prec_args(rp, ci, v, prec, coloring)
+    it, r0, rf = C.c_int(), C.c_double(), C.c_double()
+    st = getattr(lib(), "or_gmres_" + _sfx(v))(
+        n, rp, ci, v, prec, _ptr(invd), nc, _ptr(csz), _ptr(perm), _ptr(color),
+        basis, int(flexible), np.ascontiguousarray(b, v.dtype), x, maxit, atol,
+        rtol, dtol, C.byref(it), C.byref(r0), C.byref(rf))
+    return x, dict(status=st, iters=it.value, res_init=r0.value,
+                   res_final=rf.value)
+
+
+def mpir(rp, ci, vd, b, x0=None, max_outer=50, atol=0.0, rtol=1e-10,
+         inner_maxit=10000, inner_rtol=1e-4):
+    n = len(rp) - 1
+    x = np.zeros(n, np.float64) if x0 is None else np.array(x0, np.float64)
+    vs = vd.astype(np.float32)
+    oi, ii, r0, rf = C.c_int(), C.c_int(), C.c_double(), C.c_double()
+    st = lib().or_mpir_ds(n, rp, ci, vd, vs, np.ascontiguousarray(b), x,
+                          max_outer, atol, rtol, inner_maxit, inner_rtol,
+                          C.byref(oi), C.byref(ii), C.byref(r0), C.byref(rf))
+    return x, dict(status=st, outer=oi.value, inner_total=ii.value,
+                   res_init=r0.value, res_final=rf.value)

@@ -1,0 +1,297 @@
+"""HIP kernels through the C ABI against the CPU oracle.
+
+SpMV is bit-identical to the oracle's row-order loop (rounded products
+summed in ascending storage order), which is stronger than the required
+|y - y_ref|_i <= 1e-13 (|A||x|)_i; both are asserted.  Reductions sum in a
+different order than the sequential oracle and are checked to
+1e-13 * sum|x_i y_i|.
+"""
+import numpy as np
+import pytest
+
+from ba_amd import hflmp
+from oracle import oracle as O
+from tests.helpers import (assert_spmv_close, q1_sine_eigenvalue, random_csr, sine_mode,
+                           spmv_abs)
+
+pytestmark = pytest.mark.gpu
+
+
+def _spmv_dev(ctx, rp, ci, v, x, mode="assign", y0=None):
+    A = hflmp.CsrMatrix(ctx, rp, ci, v, ncol=len(x))
+    dx = ctx.upload(x)
+    dy = ctx.upload(y0) if y0 is not None else ctx.zeros(len(rp) - 1, v.dtype)
+    hflmp.spmv(ctx, A, dx, dy, mode)
+    y = dy.download()
+    A.free(); dx.free(); dy.free()
+    return y
+
+
+# ---- known-answer tests of the reference ------------------------------------
+def test_kat_cuda_test_dense_csr(ctx):
+    # tests/cuda_test.cc:43-78: 3x3 stored densely (explicit zeros) times
+    # [1.2,-2.3,0.7] -> [4.7,-6.5,3.7] to 1e-12
+    rp = np.array([0, 3, 6, 9], np.int32)
+    ci = np.array([0, 1, 2] * 3, np.int32)
+    v = np.array([2, -1, 0, -1, 2, -1, 0, -1, 2], np.float64)
+    y = _spmv_dev(ctx, rp, ci, v, np.array([1.2, -2.3, 0.7]))
+    np.testing.assert_allclose(y, [4.7, -6.5, 3.7], rtol=0, atol=1e-12)
+
+
+def test_kat_lmp_test_coo(ctx):
+    # tests/lmp_test.cc:48-83: TransformFromCOO of the 7-entry tridiagonal
+    # times [1,-2,0] -> [4,-5,2] exactly
+    rp, ci, v = O.coo_to_csr(3, [0, 0, 1, 1, 1, 2, 2], [0, 1, 0, 1, 2, 1, 2],
+                             [2, -1, -1, 2, -1, -1, 2])
+    y = _spmv_dev(ctx, rp, ci, v, np.array([1.0, -2.0, 0.0]))
+    assert list(y) == [4.0, -5.0, 2.0]
+
+
+def test_kat_parallel_matrix_tridiagonal(ctx):
+    # tests/parallel_matrix_test.cc:922-1048: tridiagonal [-1,2,-1] block
+    # (ends 2,-1) times ones -> [1,0,...,0,1]
+    n = 10
+    ri, cj, vv = [], [], []
+    for i in range(n):
+        for j, a in ((i - 1, -1.0), (i, 2.0), (i + 1, -1.0)):
+            if 0 <= j < n:
+                ri.append(i); cj.append(j); vv.append(a)
+    rp, ci, v = O.coo_to_csr(n, ri, cj, vv)
+    y = _spmv_dev(ctx, rp, ci, v, np.ones(n))
+    assert list(y) == [1.0] + [0.0] * (n - 2) + [1.0]
+
+
+# ---- SpMV parity ---------------------------------------------------------------------
+@pytest.mark.parametrize("order,dim,n", [(1, 2, 256), (1, 3, 48), (2, 2, 96), (1, 2, 1)])
+def test_spmv_fe_bit_exact(ctx, order, dim, n):
+    kappa = hflmp.fe_kappa(dim, n, 1.0, 5)
+    rp, ci, v = hflmp.fe_system(order, dim, n, kappa)
+    x = np.random.default_rng(11).uniform(-1, 1, len(rp) - 1)
+    y = _spmv_dev(ctx, rp, ci, v, x)
+    yref = O.spmv(rp, ci, v, x)
+    assert_spmv_close(y, yref, spmv_abs(rp, ci, v, x))
+    assert np.array_equal(y, yref)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_spmv_ragged_and_long_rows(ctx, dtype):
+    # empty rows, short rows and rows longer than one LDS window (4096
+    # doubles / 8192 floats), so the multi-window path runs
+    rng = np.random.default_rng(3)
+    nrow, ncol = 3000, 20000
+    lengths = rng.integers(0, 40, nrow)
+    lengths[::7] = 0
+    lengths[17] = 9000
+    lengths[1500] = 12000
+    lengths[-1] = 5
+    rp, ci, v = random_csr(nrow, ncol, lengths, seed=4, dtype=dtype)
+    x = rng.uniform(-1, 1, ncol).astype(dtype)
+    y = _spmv_dev(ctx, rp, ci, v, x)
+    yref = O.spmv(rp, ci, v, x)
+    assert np.array_equal(y, yref)
+
+
+def test_spmv_add_and_nodiag(ctx):
+    rp, ci, v = hflmp.fe_system(1, 3, 20, hflmp.fe_kappa(3, 20, 1.0, 9))
+    rng = np.random.default_rng(2)
+    x = rng.uniform(-1, 1, len(rp) - 1)
+    y0 = rng.uniform(-1, 1, len(rp) - 1)
+    y = _spmv_dev(ctx, rp, ci, v, x, "add", y0)
+    assert np.array_equal(y, O.spmv_add(rp, ci, v, x, y0.copy()))
+    y = _spmv_dev(ctx, rp, ci, v, x, "nodiag")
+    yref = O.spmv_nodiag(rp, ci, v, x)
+    assert_spmv_close(y, yref, spmv_abs(rp, ci, v, x))
+
+
+def test_spmv_empty_matrix(ctx):
+    rp = np.zeros(6, np.int32)
+    ci = np.zeros(0, np.int32)
+    v = np.zeros(0)
+    A = hflmp.CsrMatrix(ctx, rp, ci, v, ncol=5)
+    dy = ctx.upload(np.full(5, 7.0))
+    dx = ctx.upload(np.ones(5))
+    hflmp.spmv(ctx, A, dx, dy)
+    assert list(dy.download()) == [0.0] * 5
+
+
+def test_spmv_128cube_full_size(ctx):
+    # the 128^3 Q1 system at full size: bit-exact against the oracle, and
+    # the sine-mode eigen relation A v = lambda v on interior rows
+    n = 128
+    rp, ci, v = hflmp.fe_system(1, 3, n)
+    assert len(rp) - 1 == 2146689 and len(ci) == 57066625
+    A = hflmp.CsrMatrix(ctx, rp, ci, v)
+    x = np.random.default_rng(1).uniform(-1, 1, A.nrow)
+    dx, dy = ctx.upload(x), ctx.empty(A.nrow, np.float64)
+    hflmp.spmv(ctx, A, dx, dy)
+    assert np.array_equal(dy.download(), O.spmv(rp, ci, v, x))
+    modes = (1, 2, 3)
+    s = sine_mode(1, 3, n, modes)
+    dx.copy_from_host(s)
+    hflmp.spmv(ctx, A, dx, dy)
+    y = dy.download()
+    interior = ~O.fe_boundary_mask(1, 3, n)
+    lam = q1_sine_eigenvalue(3, n, modes)
+    scale = spmv_abs(rp, ci, v, s)
+    err = np.abs(y - lam * s)[interior]
+    assert np.all(err <= 1e-13 * scale[interior] + 1e-300)
+    A.free(); dx.free(); dy.free()
+
+
+# ---- BLAS-1 -------------------------------------------------------------------------------
+@pytest.mark.parametrize("n", [1, 255, 1000003])
+def test_blas1_against_oracle(ctx, n):
+    rng = np.random.default_rng(n)
+    x = rng.uniform(-1, 1, n)
+    y = rng.uniform(-1, 1, n)
+    dx, dy = ctx.upload(x), ctx.upload(y)
+    d = hflmp.dot(ctx, dx, dy)
+    assert abs(d - O.dot(x, y)) <= 1e-13 * np.sum(np.abs(x * y)) + 1e-300
+    assert abs(hflmp.nrm2(ctx, dx) - O.nrm2(x)) <= 1e-13 * O.nrm2(x)
+    L, h = ctx.L, ctx.h
+    ctx.check(L.hflmp_axpy_f64(h, n, 0.75, dx.ptr, dy.ptr))
+    ref = O.axpy(0.75, x, y.copy())
+    np.testing.assert_allclose(dy.download(), ref, rtol=1e-15, atol=1e-15)
+    dy.copy_from_host(y)
+    ctx.check(L.hflmp_scaleadd_f64(h, n, -1.5, dx.ptr, dy.ptr))
+    np.testing.assert_allclose(dy.download(), O.scaleadd(-1.5, x, y.copy()), rtol=1e-15,
+                               atol=1e-15)
+    dy.copy_from_host(y)
+    ctx.check(L.hflmp_scal_f64(h, n, 3.0, dy.ptr))
+    assert np.array_equal(dy.download(), O.scale(3.0, y.copy()))
+    dy.copy_from_host(y)
+    ctx.check(L.hflmp_ewmul_f64(h, n, dx.ptr, dy.ptr))
+    assert np.array_equal(dy.download(), O.ewmult(x, y.copy()))
+    d32 = ctx.empty(n, np.float32)
+    ctx.check(L.hflmp_cast_f64_to_f32(h, n, dx.ptr, d32.ptr))
+    assert np.array_equal(d32.download(), O.cast_d2s(x))
+    ctx.check(L.hflmp_cast_f32_to_f64(h, n, d32.ptr, dy.ptr))
+    assert np.array_equal(dy.download(), O.cast_s2d(O.cast_d2s(x)))
+    import ctypes as C
+    r, idx = C.c_double(), C.c_int()
+    ctx.check(L.hflmp_normmax_f64(h, n, dx.ptr, C.byref(r)))
+    assert r.value == O.normmax(x)
+    ctx.check(L.hflmp_argmax_abs_f64(h, n, dx.ptr, C.byref(idx)))
+    assert idx.value == int(np.argmax(np.abs(x)))
+    ctx.check(L.hflmp_argmin_abs_f64(h, n, dx.ptr, C.byref(idx)))
+    assert idx.value == int(np.argmin(np.abs(x)))
+    ctx.check(L.hflmp_norm1_f64(h, n, dx.ptr, C.byref(r)))
+    assert abs(r.value - O.norm1(x)) <= 1e-13 * O.norm1(x)
+    for a in (dx, dy, d32):
+        a.free()
+
+
+def test_blas1_kats_parallel_vector(ctx):
+    # tests/parallel_vector_test.cc: v_i = i, w_i = i + 2 for i < 10
+    import ctypes as C
+    n = 10
+    x = np.arange(n, dtype=np.float64)
+    y = x + 2
+    dx, dy = ctx.upload(x), ctx.upload(y)
+    N = float(n - 1)
+    assert abs(hflmp.dot(ctx, dx, dy) - (N * (N + 1) * (2 * N + 1) / 6 + N * (N + 1))) < 1e-10
+    ctx.check(ctx.L.hflmp_axpy_f64(ctx.h, n, 4.0, dy.ptr, dx.ptr))   # axpy1
+    assert list(dx.download()) == list(x + 4.0 * y)
+    dx.copy_from_host(x)
+    ctx.check(ctx.L.hflmp_scal_f64(ctx.h, n, 0.5, dx.ptr))           # scale
+    assert list(dx.download()) == list(0.5 * x)
+    dx.copy_from_host(x)
+    ctx.check(ctx.L.hflmp_scaleadd_f64(ctx.h, n, 4.0, dy.ptr, dx.ptr))  # scale_add
+    assert list(dx.download()) == list(4.0 * x + y)
+    cnt = C.c_int()
+    z = x.copy(); z[3] = np.nan; z[7] = np.nan
+    dz = ctx.upload(z)
+    ctx.check(ctx.L.hflmp_count_nan_f64(ctx.h, n, dz.ptr, C.byref(cnt)))
+    assert cnt.value == 2
+
+
+def test_dot_f32(ctx):
+    rng = np.random.default_rng(8)
+    x = rng.uniform(-1, 1, 500001).astype(np.float32)
+    y = rng.uniform(-1, 1, 500001).astype(np.float32)
+    d = hflmp.dot(ctx, ctx.upload(x), ctx.upload(y))
+    exact = float(np.dot(x.astype(np.float64), y.astype(np.float64)))
+    assert abs(d - exact) <= 1e-5 * float(np.sum(np.abs(x.astype(np.float64) * y)))
+
+
+def test_gather_scatter(ctx):
+    rng = np.random.default_rng(5)
+    src = rng.uniform(-1, 1, 1000)
+    idx = rng.permutation(1000)[:300].astype(np.int32)
+    ds, di, dd = ctx.upload(src), ctx.upload(idx), ctx.zeros(300, np.float64)
+    ctx.check(ctx.L.hflmp_gather_f64(ctx.h, 300, di.ptr, ds.ptr, dd.ptr))
+    assert np.array_equal(dd.download(), src[idx])
+    dst = ctx.zeros(1000, np.float64)
+    ctx.check(ctx.L.hflmp_scatter_f64(ctx.h, 300, di.ptr, dd.ptr, dst.ptr))
+    ref = np.zeros(1000); ref[idx] = src[idx]
+    assert np.array_equal(dst.download(), ref)
+    ctx.check(ctx.L.hflmp_scatter_add_f64(ctx.h, 300, di.ptr, dd.ptr, dst.ptr))
+    assert np.array_equal(dst.download(), 2 * ref)
+
+
+def test_invdiag_and_zero_diag(ctx):
+    rp, ci, v = hflmp.fe_system(2, 2, 17, hflmp.fe_kappa(2, 17, 1.0, 3))
+    A = hflmp.CsrMatrix(ctx, rp, ci, v)
+    d = hflmp.invdiag(ctx, A)
+    assert np.array_equal(d.download(), O.invdiag(rp, ci, v))
+    v2 = v.copy()
+    dp, _ = O.diag_ptr(rp, ci)
+    v2[dp[5]] = 0.0
+    B = hflmp.CsrMatrix(ctx, rp, ci, v2)
+    with pytest.raises(hflmp.HflmpError):
+        hflmp.invdiag(ctx, B)
+
+
+# ---- device-resident Jacobi-PCG ------------------------------------------------------
+@pytest.mark.parametrize("order,dim,n,sigma", [(1, 2, 256, 0.0), (1, 3, 32, 1.0),
+                                               (2, 2, 64, 0.0)])
+def test_pcg_jacobi_f64_vs_oracle(ctx, order, dim, n, sigma):
+    kappa = hflmp.fe_kappa(dim, n, sigma, 21) if sigma else None
+    rp, ci, v = hflmp.fe_system(order, dim, n, kappa)
+    b = hflmp.fe_rhs(order, dim, n, 1234)
+    xo, info_o = O.cg(rp, ci, v, b, prec=O.PREC_JACOBI, rtol=1e-8, maxit=20000)
+    A = hflmp.CsrMatrix(ctx, rp, ci, v)
+    invd = hflmp.invdiag(ctx, A)
+    db, dx = ctx.upload(b), ctx.zeros(A.nrow, np.float64)
+    info = hflmp.pcg_jacobi(ctx, A, invd, db, dx, maxit=20000, rtol=1e-8)
+    x = dx.download()
+    assert info["status"] == 0 and info_o["status"] == 0
+    assert abs(info["iters"] - info_o["iters"]) <= 2, (info, info_o)
+    assert np.linalg.norm(x - xo) <= 1e-8 * np.linalg.norm(xo)
+    assert abs(info["res_init"] - info_o["res_init"]) <= 1e-12 * info_o["res_init"]
+    # true residual of the GPU solution
+    r = b - O.spmv(rp, ci, v, x)
+    assert np.linalg.norm(r) <= 1.01e-8 * info["res_init"] * 10
+
+
+def test_pcg_jacobi_f32_vs_oracle(ctx):
+    rp, ci, v = hflmp.fe_system(1, 2, 64)
+    b = hflmp.fe_rhs(1, 2, 64, 4)
+    v32, b32 = v.astype(np.float32), b.astype(np.float32)
+    xo, info_o = O.cg(rp, ci, v32, b32, prec=O.PREC_JACOBI, rtol=1e-5, maxit=5000)
+    A = hflmp.CsrMatrix(ctx, rp, ci, v32)
+    invd = hflmp.invdiag(ctx, A)
+    db, dx = ctx.upload(b32), ctx.zeros(A.nrow, np.float32)
+    info = hflmp.pcg_jacobi(ctx, A, invd, db, dx, maxit=5000, rtol=1e-5)
+    x = dx.download()
+    assert info["status"] == 0
+    assert abs(info["iters"] - info_o["iters"]) <= 2, (info, info_o)
+    assert np.linalg.norm(x - xo) <= 1e-4 * np.linalg.norm(xo)
+
+
+def test_pcg_maxit_and_atol(ctx):
+    rp, ci, v = hflmp.fe_system(1, 2, 64)
+    b = hflmp.fe_rhs(1, 2, 64, 4)
+    A = hflmp.CsrMatrix(ctx, rp, ci, v)
+    invd = hflmp.invdiag(ctx, A)
+    db, dx = ctx.upload(b), ctx.zeros(A.nrow, np.float64)
+    info = hflmp.pcg_jacobi(ctx, A, invd, db, dx, maxit=7, rtol=1e-12)
+    assert info["status"] == 1 and info["iters"] == 7
+    xo, io = O.cg(rp, ci, v, b, prec=O.PREC_JACOBI, rtol=1e-12, maxit=7)
+    assert io["status"] == 1 and io["iters"] == 7
+    np.testing.assert_allclose(dx.download(), xo, rtol=1e-10, atol=1e-12)
+    # zero right-hand side: converged at iteration 0 by the absolute test
+    db.copy_from_host(np.zeros_like(b))
+    dx.copy_from_host(np.zeros_like(b))
+    info = hflmp.pcg_jacobi(ctx, A, invd, db, dx, maxit=100, atol=1e-30, rtol=1e-8)
+    assert info["status"] == 0 and info["iters"] == 0
