@@ -126,9 +126,9 @@ __global__ void __launch_bounds__(kBlock) reduce_kernel(int n, const T *__restri
 
 // out = sum of the partials; FINISH = 1 takes the square root.
 template <typename T, int FINISH>
-__global__ void __launch_bounds__(kBlock) reduce_final_kernel(const T *__restrict__ partials,
-                                                              int count, T *out) {
-  __shared__ T lds[kBlock / kWave];
+__global__ void __launch_bounds__(kFinalBlock)
+    reduce_final_kernel(const T *__restrict__ partials, int count, T *out) {
+  __shared__ T lds[kFinalBlock / kWave];
   const T tot = sum_partials(partials, count, 1, 0, lds);
   if (threadIdx.x == 0) out[0] = FINISH ? sqrt(tot) : tot;
 }
@@ -202,7 +202,7 @@ int launch_reduce(hflmp_ctx *ctx, int n, const T *x, const T *y, T *d_out) {
   const int g = red_blocks(ctx, n);
   T *partials = static_cast<T *>(ctx->red_partials);
   reduce_kernel<T, OP><<<g, kBlock, 0, ctx->stream>>>(n, x, y, partials);
-  reduce_final_kernel<T, FINISH><<<1, kBlock, 0, ctx->stream>>>(partials, g, d_out);
+  reduce_final_kernel<T, FINISH><<<1, kFinalBlock, 0, ctx->stream>>>(partials, g, d_out);
   HFLMP_LAUNCH_CHECK(ctx);
   return HFLMP_OK;
 }

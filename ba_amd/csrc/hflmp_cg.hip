@@ -50,9 +50,9 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kFinalBlock)
     cg_init_final(CgState<T> *st, const T *__restrict__ partials, int count) {
-  __shared__ T lds[kBlock / kWave];
+  __shared__ T lds[kFinalBlock / kWave];
   const T rr = sum_partials(partials, count, 2, 0, lds);
   const T rz = sum_partials(partials, count, 2, 1, lds);
   if (threadIdx.x == 0) {
@@ -88,9 +88,9 @@ __global__ void __launch_bounds__(kBlock)
 
 // F1: alpha = (r.z) / (p.Ap)
 template <typename T>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kFinalBlock)
     cg_alpha_final(CgState<T> *st, const T *__restrict__ partials, int count) {
-  __shared__ T lds[kBlock / kWave];
+  __shared__ T lds[kFinalBlock / kWave];
   if (st->status != kIterate) return;
   const T pAp = sum_partials(partials, count, 1, 0, lds);
   if (threadIdx.x == 0) {
@@ -124,9 +124,9 @@ __global__ void __launch_bounds__(kBlock)
 
 // F2: res = sqrt(r.r), stopping test, beta = gamma / beta
 template <typename T>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kFinalBlock)
     cg_beta_final(CgState<T> *st, const T *__restrict__ partials, int count) {
-  __shared__ T lds[kBlock / kWave];
+  __shared__ T lds[kFinalBlock / kWave];
   if (st->status != kIterate) return;
   const T rr = sum_partials(partials, count, 2, 0, lds);
   const T rz = sum_partials(partials, count, 2, 1, lds);
@@ -183,14 +183,14 @@ int pcg_jacobi(hflmp_ctx *ctx, int nrow, int nnz, const int *rp, const int *ci, 
   init_state_kernel<T><<<1, kBlock, 0, s>>>(st, maxit, atol, rtol, dtol);
   cg_init_kernel<T><<<L.grid1, kBlock, 0, s>>>(nrow, L.R, rp, ci, val, invd, b, x, r, z, p,
                                                partials);
-  cg_init_final<T><<<1, kBlock, 0, s>>>(st, partials, L.grid1);
+  cg_init_final<T><<<1, kFinalBlock, 0, s>>>(st, partials, L.grid1);
   HFLMP_LAUNCH_CHECK(ctx);
 
   auto iteration = [&]() {
     cg_spmv_kernel<T><<<L.grid1, kBlock, 0, s>>>(nrow, L.R, rp, ci, val, p, Ap, st, partials);
-    cg_alpha_final<T><<<1, kBlock, 0, s>>>(st, partials, L.grid1);
+    cg_alpha_final<T><<<1, kFinalBlock, 0, s>>>(st, partials, L.grid1);
     cg_update_kernel<T><<<g2, kBlock, 0, s>>>(nrow, invd, p, Ap, x, r, z, st, partials);
-    cg_beta_final<T><<<1, kBlock, 0, s>>>(st, partials, g2);
+    cg_beta_final<T><<<1, kFinalBlock, 0, s>>>(st, partials, g2);
     cg_p_kernel<T><<<g3, kBlock, 0, s>>>(nrow, z, p, st);
   };
   int rc = run_polled(ctx, maxit, &st->status, iteration);

@@ -116,13 +116,36 @@ __device__ __forceinline__ T block_sum(T v, T *lds) {
 // visible; no device-scope fence (an L2 write-back on gfx950's per-XCD L2s)
 // is issued per workgroup, and the result does not depend on timing.
 
-// Sum of `count` strided partials src[k*stride + off] over one workgroup;
-// valid in thread 0.
+// Threads of the one-workgroup kernels that sum partials.
+constexpr int kFinalBlock = 1024;
+
+// Sum of `count` strided partials src[k*stride + off] over one workgroup of
+// kFinalBlock threads; valid in thread 0.  Four independent accumulators
+// keep four loads in flight per thread; the order is fixed, so the result
+// is deterministic.  `lds` holds kFinalBlock / kWave values.
 template <typename T>
 __device__ __forceinline__ T sum_partials(const T *src, int count, int stride, int off, T *lds) {
-  T acc = 0;
-  for (int k = threadIdx.x; k < count; k += kBlock) acc += src[(size_t)k * stride + off];
-  return block_sum(acc, lds);
+  T a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+  int k = threadIdx.x;
+  for (; k + 3 * kFinalBlock < count; k += 4 * kFinalBlock) {
+    a0 += src[(size_t)k * stride + off];
+    a1 += src[(size_t)(k + kFinalBlock) * stride + off];
+    a2 += src[(size_t)(k + 2 * kFinalBlock) * stride + off];
+    a3 += src[(size_t)(k + 3 * kFinalBlock) * stride + off];
+  }
+  for (; k < count; k += kFinalBlock) a0 += src[(size_t)k * stride + off];
+  T v = wave_sum((a0 + a1) + (a2 + a3));
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wid = threadIdx.x / kWave;
+  __syncthreads();  // a previous call may still be reading lds
+  if (lane == 0) lds[wid] = v;
+  __syncthreads();
+  T s = 0;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int w = 0; w < kFinalBlock / kWave; ++w) s += lds[w];
+  }
+  return s;
 }
 
 // XCD-aware tile index: workgroup b runs on XCD b % 8; give each XCD a

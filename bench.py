@@ -78,6 +78,35 @@ def cpu_baseline(rp, ci, v, x, budget_s):
                        "semantics" % (nreps, el1)})
 
 
+def measured_traffic(n):
+    """HBM bytes per SpMV launch measured with rocprofv3 FETCH_SIZE/WRITE_SIZE
+    (scaled by a 1 GiB copy), recorded in profiles/; None if not measured
+    for this size."""
+    path = os.path.join(ROOT, "profiles", "spmv_128cube_traffic.json")
+    if n != 128 or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return round(json.load(f)["traffic_bytes"])
+
+
+def copy_calibration(hflmp, reps):
+    """y = x over 2^27 doubles: 1 GiB read + 1 GiB written per launch."""
+    n = 1 << 27
+    ctx = hflmp.Context(0)
+    x, y = ctx.zeros(n, np.float64), ctx.empty(n, np.float64)
+    for _ in range(2):
+        ctx.check(ctx.L.hflmp_copy_f64(ctx.h, n, x.ptr, y.ptr), "copy")
+    e0, e1 = ctx.event(), ctx.event()
+    e0.record()
+    for _ in range(reps):
+        ctx.check(ctx.L.hflmp_copy_f64(ctx.h, n, x.ptr, y.ptr), "copy")
+    e1.record()
+    ms = e1.elapsed_ms(e0) / reps
+    print(json.dumps({"copy_bytes_read": 8 * n, "copy_bytes_written": 8 * n,
+                      "ms_per_copy": ms, "GBps_rw": 16 * n / ms / 1e6}), flush=True)
+    x.free(); y.free(); ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,6 +117,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cg", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device copy (hflmp_copy_f64), the reference "
+                         "kernel for scaling the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X and has no "
@@ -95,6 +127,9 @@ def main():
         sys.exit(2)
 
     from ba_amd import hflmp
+    if args.copy_calibration:
+        copy_calibration(hflmp, args.steps)
+        return
     n = args.n
     t0 = time.perf_counter()
     rp, ci, v = hflmp.fe_system(1, 3, n)
@@ -141,7 +176,7 @@ def main():
                    "nnz": nnz, "bytes_per_spmv": nb, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
-                     "traffic": None},
+                     "traffic": measured_traffic(n)},
         "spmv_us": round(per_launch_s * 1e6, 2),
         "gen_seconds": round(t_gen, 3),
     }

@@ -1,0 +1,413 @@
+"""The CPU oracle pinned against the reference's known-answer tests, closed
+forms, independent constructions and library routines (CPU only).
+
+Each check is built so that a plausible slip in the oracle (wrong sign,
+index, dropped term, transposed operand) fails it: the FE matrices are
+rebuilt from 1D Kronecker products and from Gauss quadrature of the shape
+functions, the preconditioners from dense matrices, the solvers against
+scipy and dense solves.
+"""
+import numpy as np
+import pytest
+import scipy.linalg as sla
+import scipy.sparse as sp
+import scipy.sparse.linalg as spl
+
+from oracle import oracle as O
+from tests.helpers import q1_sine_eigenvalue, random_csr, sine_mode, spmv_abs
+
+
+def dense(rp, ci, v, ncol=None):
+    n = len(rp) - 1
+    return sp.csr_matrix((v, ci, rp), shape=(n, ncol or n)).toarray()
+
+
+# ---- random streams ------------------------------------------------------------
+def test_splitmix64_published_vector():
+    # SplitMix64 (Steele, Lea, Flood 2014) from state 0: the first outputs
+    # are the widely published 0xe220a8397b1dcdaf, 0x6e789e6aa1b965f4,
+    # 0x06c45d188009454f.
+    assert O.splitmix64_at(0, 0) == 0xE220A8397B1DCDAF
+    assert O.splitmix64_at(0, 1) == 0x6E789E6AA1B965F4
+    assert O.splitmix64_at(0, 2) == 0x06C45D188009454F
+
+
+def test_uniform_and_gauss_statistics():
+    u = np.array([O.uniform_pm1(99, i) for i in range(20000)])
+    assert u.min() >= -1 and u.max() < 1
+    assert abs(u.mean()) < 0.02 and abs(u.var() - 1 / 3) < 0.01
+    k = O.fe_kappa(2, 100, 1.0, 5)   # 10^4 log-normal cells
+    g = np.log(k)
+    assert abs(g.mean()) < 0.03 and abs(g.std() - 1.0) < 0.03
+
+
+# ---- reference known-answer tests --------------------------------------------------
+def test_kat_lmp_test():
+    # tests/lmp_test.cc:48-83
+    rp, ci, v = O.coo_to_csr(3, [0, 0, 1, 1, 1, 2, 2], [0, 1, 0, 1, 2, 1, 2],
+                             [2, -1, -1, 2, -1, -1, 2])
+    assert list(rp) == [0, 2, 5, 7] and list(ci) == [0, 1, 0, 1, 2, 1, 2]
+    assert list(O.spmv(rp, ci, v, np.array([1.0, -2.0, 0.0]))) == [4.0, -5.0, 2.0]
+    y = np.empty(3)
+    assert list(O.spmv_omp(rp, ci, v, np.array([1.0, -2.0, 0.0]), y, 2)) == [4.0, -5.0, 2.0]
+
+
+def test_kat_coo_unsorted_input_is_sorted():
+    # coo_2_csr sorts the columns of each row (lmatrix_formats.h:404-433)
+    rp, ci, v = O.coo_to_csr(2, [1, 0, 1, 0], [1, 1, 0, 0], [4.0, 2.0, 3.0, 1.0])
+    assert list(ci) == [0, 1, 0, 1] and list(v) == [1.0, 2.0, 3.0, 4.0]
+
+
+def test_kat_cuda_test():
+    # tests/cuda_test.cc:43-78, explicit zeros kept in the CSR pattern
+    rp = np.array([0, 3, 6, 9], np.int32)
+    ci = np.array([0, 1, 2] * 3, np.int32)
+    v = np.array([2, -1, 0, -1, 2, -1, 0, -1, 2], np.float64)
+    np.testing.assert_allclose(O.spmv(rp, ci, v, np.array([1.2, -2.3, 0.7])),
+                               [4.7, -6.5, 3.7], rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("prec", [O.PREC_NONE, O.PREC_JACOBI])
+def test_kat_cg_test(prec):
+    # tests/cg_test.cc:47-94: x = [-0.25, -1.5, -0.75] to 1e-12
+    rp, ci, v = O.coo_to_csr(3, [0, 0, 1, 1, 1, 2, 2], [0, 1, 0, 1, 2, 1, 2],
+                             [2, -1, -1, 2, -1, -1, 2])
+    x, info = O.cg(rp, ci, v, np.array([1.0, -2.0, 0.0]), prec=prec, rtol=1e-14,
+                   atol=1e-15, maxit=100)
+    np.testing.assert_allclose(x, [-0.25, -1.5, -0.75], rtol=0, atol=1e-12)
+    assert info["status"] == 0 and info["iters"] <= 3
+
+
+def test_kat_parallel_vector_ops():
+    # tests/parallel_vector_test.cc:402-660 with one rank, 10 entries
+    v = np.arange(10, dtype=np.float64)
+    w = v + 2
+    N = 9.0
+    assert abs(O.dot(v, w) - (N * (N + 1) * (2 * N + 1) / 6 + N * (N + 1))) < 1e-10
+    assert list(O.axpy(4.0, w, v.copy())) == list(v + 4.0 * w)          # axpy1
+    assert list(O.scale(0.5, v.copy())) == list(0.5 * v)                 # scale
+    assert list(O.scaleadd(4.0, w, v.copy())) == list(4.0 * v + w)       # scale_add
+    assert O.nrm2(v) == np.sqrt(O.dot(v, v))
+    assert O.normmax(-v) == 9.0 and O.norm1(-v) == 45.0
+
+
+def test_kat_parallel_matrix_tridiagonal():
+    # tests/parallel_matrix_test.cc:922-1048
+    n = 10
+    ri = [i for i in range(n) for j in (i - 1, i, i + 1) if 0 <= j < n]
+    cj = [j for i in range(n) for j in (i - 1, i, i + 1) if 0 <= j < n]
+    vv = [2.0 if i == j else -1.0 for i, j in zip(ri, cj)]
+    rp, ci, v = O.coo_to_csr(n, ri, cj, vv)
+    assert list(O.spmv(rp, ci, v, np.ones(n))) == [1.0] + [0.0] * (n - 2) + [1.0]
+
+
+def test_zero_rows_keeps_columns():
+    # ZeroRows (lmatrix_csr_cpu.cc:908-927): row -> 0, diag -> alpha,
+    # columns untouched (the matrix becomes non-symmetric)
+    rp, ci, v = O.coo_to_csr(3, [0, 0, 1, 1, 1, 2, 2], [0, 1, 0, 1, 2, 1, 2],
+                             [2, -1, -1, 2, -1, -1, 2])
+    O.zero_rows([0], rp, ci, v, 1.0)
+    A = dense(rp, ci, v)
+    assert list(A[0]) == [1.0, 0.0, 0.0] and A[1, 0] == -1.0
+    d, missing = O.diag_ptr(rp, ci)
+    assert missing == 0 and list(d) == [0, 3, 6]
+
+
+# ---- FE assembly: independent constructions --------------------------------------------
+def _assemble_1d(order, n):
+    """Assembled 1D stiffness and mass on n cells of [0,1]."""
+    h = 1.0 / n
+    if order == 1:
+        Ke = np.array([[1, -1], [-1, 1]]) / h
+        Me = np.array([[2, 1], [1, 2]]) * h / 6
+    else:
+        Ke = np.array([[7, -8, 1], [-8, 16, -8], [1, -8, 7]]) / (3 * h)
+        Me = np.array([[4, 2, -1], [2, 16, 2], [-1, 2, 4]]) * h / 30
+    p = order * n + 1
+    K, M = np.zeros((p, p)), np.zeros((p, p))
+    for c in range(n):
+        idx = np.arange(order * c, order * c + order + 1)
+        K[np.ix_(idx, idx)] += Ke
+        M[np.ix_(idx, idx)] += Me
+    return K, M
+
+
+def _dirichlet(A, mask):
+    A = A.copy()
+    A[mask] = 0.0
+    A[mask, mask] = 1.0
+    return A
+
+
+@pytest.mark.parametrize("order,dim,n", [(1, 2, 5), (1, 3, 4), (2, 2, 4), (1, 2, 1)])
+def test_assembly_matches_kronecker_construction(order, dim, n):
+    K, M = _assemble_1d(order, n)
+    if dim == 2:
+        A = np.kron(M, K) + np.kron(K, M)
+    else:
+        A = np.kron(M, np.kron(M, K)) + np.kron(M, np.kron(K, M)) + np.kron(K, np.kron(M, M))
+    mask = O.fe_boundary_mask(order, dim, n)
+    A = _dirichlet(A, mask)
+    rp, ci, v = O.fe_system(order, dim, n)
+    N, Z = O.fe_sizes(order, dim, n)
+    assert len(rp) - 1 == N and len(ci) == Z
+    Ad = dense(rp, ci, v)
+    np.testing.assert_allclose(Ad, A, rtol=0, atol=1e-14 * np.abs(A).max())
+    # the pattern is exactly the node pairs sharing a cell: compare with the
+    # nonzero pattern of the unconstrained Kronecker matrices of ones
+    Kp = (np.abs(K) + np.abs(M)) > 0
+    P = np.kron(Kp, Kp) if dim == 2 else np.kron(Kp, np.kron(Kp, Kp))
+    S = np.zeros_like(Ad, dtype=bool)
+    S[np.repeat(np.arange(N), np.diff(rp)), ci] = True
+    assert np.array_equal(S, P)
+    assert all(np.all(np.diff(ci[rp[i]:rp[i + 1]]) > 0) for i in range(N))
+
+
+def _gauss(m):
+    x, w = np.polynomial.legendre.leggauss(m)
+    return (x + 1) / 2, w / 2
+
+
+def _shape_1d(order, t):
+    if order == 1:
+        return np.array([1 - t, t]), np.array([-1.0 + 0 * t, 1.0 + 0 * t])
+    return (np.array([2 * t * t - 3 * t + 1, 4 * t * (1 - t), 2 * t * t - t]),
+            np.array([4 * t - 3, 4 - 8 * t, 4 * t - 1]))
+
+
+def _element_stiffness_quadrature(order, dim, h):
+    """int grad(phi_a).grad(phi_b) over a cell of side h by tensor Gauss
+    quadrature of the Lagrange shape functions (lexicographic local nodes)."""
+    q = order + 1
+    tg, wg = _gauss(order + 1)
+    phi, dphi = _shape_1d(order, tg)       # [q, ng]
+    nl = q ** dim
+    E = np.zeros((nl, nl))
+    import itertools
+    for pts in itertools.product(range(len(tg)), repeat=dim):
+        w = np.prod([wg[p] for p in pts]) * h ** dim
+        grads = np.zeros((nl, dim))
+        for a in range(nl):
+            la = [(a // q ** d) % q for d in range(dim)]
+            for d in range(dim):
+                g = 1.0
+                for e in range(dim):
+                    g *= (dphi[la[e], pts[e]] / h) if e == d else phi[la[e], pts[e]]
+                grads[a, d] = g
+        E += w * grads @ grads.T
+    return E
+
+
+@pytest.mark.parametrize("order,dim,n", [(1, 2, 4), (1, 3, 3), (2, 2, 3)])
+def test_assembly_variable_kappa_matches_quadrature(order, dim, n):
+    kappa = O.fe_kappa(dim, n, 1.0, 17)
+    h = 1.0 / n
+    E = _element_stiffness_quadrature(order, dim, h)
+    q = order + 1
+    p = order * n + 1
+    N = p ** dim
+    A = np.zeros((N, N))
+    import itertools
+    for cell in itertools.product(range(n), repeat=dim):   # (cx, cy[, cz])
+        cidx = sum(cell[d] * n ** d for d in range(dim))
+        nodes = []
+        for a in range(q ** dim):
+            la = [(a // q ** d) % q for d in range(dim)]
+            nodes.append(sum((order * cell[d] + la[d]) * p ** d for d in range(dim)))
+        A[np.ix_(nodes, nodes)] += kappa[cidx] * E
+    A = _dirichlet(A, O.fe_boundary_mask(order, dim, n))
+    rp, ci, v = O.fe_system(order, dim, n, kappa)
+    np.testing.assert_allclose(dense(rp, ci, v), A, rtol=0, atol=1e-13 * np.abs(A).max())
+
+
+def test_stencil_values_q1():
+    # 2D: centre 8/3, all eight neighbours -1/3; 3D (times 1/h): centre 8/3,
+    # faces 0 (stored), edges -1/6, corners -1/12
+    rp, ci, v = O.fe_system(1, 2, 4)
+    i = 2 + 5 * 2
+    row = dict(zip(ci[rp[i]:rp[i + 1]], v[rp[i]:rp[i + 1]]))
+    assert len(row) == 9 and abs(row[i] - 8 / 3) < 1e-15
+    assert all(abs(row[j] + 1 / 3) < 1e-15 for j in row if j != i)
+    n = 4
+    rp, ci, v = O.fe_system(1, 3, n)
+    p = n + 1
+    i = 2 + p * (2 + p * 2)
+    cols, vals = ci[rp[i]:rp[i + 1]], v[rp[i]:rp[i + 1]] * n
+    assert len(cols) == 27
+    for c, a in zip(cols, vals):
+        dx, dy, dz = c % p - 2, (c // p) % p - 2, c // (p * p) - 2
+        k = abs(dx) + abs(dy) + abs(dz)
+        assert abs(a - {0: 8 / 3, 1: 0.0, 2: -1 / 6, 3: -1 / 12}[k]) < 1e-14
+
+
+@pytest.mark.parametrize("dim,n,modes", [(2, 32, (1, 3)), (3, 12, (2, 1, 3))])
+def test_sine_mode_eigen_relation(dim, n, modes):
+    rp, ci, v = O.fe_system(1, dim, n)
+    s = sine_mode(1, dim, n, modes)
+    y = O.spmv(rp, ci, v, s)
+    interior = ~O.fe_boundary_mask(1, dim, n)
+    lam = q1_sine_eigenvalue(dim, n, modes)
+    scale = spmv_abs(rp, ci, v, s)
+    assert np.all(np.abs(y - lam * s)[interior] <= 1e-13 * scale[interior] + 1e-300)
+    assert np.all(y[~interior] == s[~interior])   # unit Dirichlet rows
+
+
+def test_rhs_boundary_zero_and_seeded():
+    b = O.fe_rhs(1, 2, 8, 3)
+    m = O.fe_boundary_mask(1, 2, 8)
+    assert np.all(b[m] == 0) and np.all(b[~m] != 0)
+    assert np.array_equal(b, O.fe_rhs(1, 2, 8, 3))
+    assert not np.array_equal(b, O.fe_rhs(1, 2, 8, 4))
+
+
+# ---- SpMV family against scipy -------------------------------------------------------
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_spmv_vs_scipy(dtype):
+    rng = np.random.default_rng(0)
+    lengths = rng.integers(0, 30, 400)
+    rp, ci, v = random_csr(400, 300, lengths, seed=1, dtype=dtype)
+    x = rng.uniform(-1, 1, 300).astype(dtype)
+    A = sp.csr_matrix((v.astype(np.float64), ci, rp), shape=(400, 300))
+    ref = A @ x.astype(np.float64)
+    tol = (1e-13 if dtype == np.float64 else 1e-5) * spmv_abs(rp, ci, v, x) + 1e-300
+    assert np.all(np.abs(O.spmv(rp, ci, v, x) - ref) <= tol)
+    y0 = rng.uniform(-1, 1, 400).astype(dtype)
+    assert np.all(np.abs(O.spmv_add(rp, ci, v, x, y0.copy()) - (ref + y0)) <= tol + 2e-7)
+    D = sp.diags(A.diagonal(), shape=(400, 300))
+    assert np.all(np.abs(O.spmv_nodiag(rp, ci, v, x) - (A - D) @ x.astype(np.float64)) <= tol)
+
+
+def test_spmv_naive_and_openmp_identical():
+    rp, ci, v = O.fe_system(1, 3, 10, O.fe_kappa(3, 10, 1.0, 2))
+    x = np.random.default_rng(3).uniform(-1, 1, len(rp) - 1)
+    y = np.empty_like(x)
+    assert np.array_equal(O.spmv(rp, ci, v, x), O.spmv_omp(rp, ci, v, x, y, 4))
+
+
+def test_invdiag_and_jacobi():
+    rp, ci, v = O.fe_system(2, 2, 5, O.fe_kappa(2, 5, 1.0, 8))
+    A = dense(rp, ci, v)
+    d = O.invdiag(rp, ci, v)
+    assert np.array_equal(d, 1.0 / np.diag(A))
+    r = np.random.default_rng(0).uniform(-1, 1, len(d))
+    assert np.array_equal(O.jacobi_apply(d, r) * np.diag(A), r * d * np.diag(A))
+    np.testing.assert_allclose(O.jacobi_apply(d, r) * np.diag(A), r, rtol=1e-15)
+
+
+# ---- Gauss-Seidel family against dense matrices ------------------------------------------
+def test_sgs_matches_dense_factorisation():
+    # Psgauss_seidel: M = (D+L) D^-1 (D+U), z = M^-1 r
+    rp, ci, v = O.fe_system(1, 2, 6, O.fe_kappa(2, 6, 1.0, 4))
+    A = dense(rp, ci, v)
+    D = np.diag(np.diag(A))
+    L, U = np.tril(A, -1), np.triu(A, 1)
+    M = (D + L) @ np.linalg.inv(D) @ (D + U)
+    r = np.random.default_rng(1).uniform(-1, 1, A.shape[0])
+    z = O.sgs_apply(rp, ci, v, r)
+    np.testing.assert_allclose(M @ z, r, rtol=0, atol=1e-12 * np.abs(r).max())
+
+
+def test_multicoloring_first_fit():
+    rp, ci, v = O.fe_system(1, 3, 5)
+    nc, csz, perm, color = O.multicoloring(rp, ci)
+    N = len(rp) - 1
+    assert csz.sum() == N and sorted(perm) == list(range(N))
+    # grouped by colour, ascending inside a colour
+    off = np.concatenate([[0], np.cumsum(csz)])
+    for c in range(nc):
+        grp = perm[off[c]:off[c + 1]]
+        assert np.all(color[grp] == c) and np.all(np.diff(grp) > 0)
+    # valid colouring and natural-order first fit: the colour of node j is
+    # the smallest not held by a stored neighbour of lower index
+    for j in range(N):
+        nb = [c for c in ci[rp[j]:rp[j + 1]] if c != j]
+        assert all(color[c] != color[j] for c in nb)
+        lower = {color[c] for c in nb if c < j}
+        assert color[j] == min(set(range(nc + 1)) - lower)
+    assert nc == 8  # trilinear 27-point coupling: 2 x 2 x 2 colours
+
+
+@pytest.mark.parametrize("omega", [1.0, 1.3])
+def test_mcsgs_matches_dense_permuted_factorisation(omega):
+    # M = w(2-w) (D + wL_p) D^-1 (D + wU_p) in the colour-permuted order
+    # (the relaxation of lpreconditioner_mc_sgs.cc:44-90)
+    rp, ci, v = O.fe_system(1, 2, 7, O.fe_kappa(2, 7, 1.0, 6))
+    col = O.multicoloring(rp, ci)
+    perm = col[2]
+    A = dense(rp, ci, v)
+    Ap = A[np.ix_(perm, perm)]
+    D = np.diag(np.diag(Ap))
+    L, U = np.tril(Ap, -1), np.triu(Ap, 1)
+    M = omega * (2 - omega) * (D + omega * L) @ np.linalg.inv(D) @ (D + omega * U)
+    r = np.random.default_rng(2).uniform(-1, 1, A.shape[0])
+    z = O.mcsgs_apply(rp, ci, v, col, r, omega)
+    np.testing.assert_allclose(M @ z[perm], r[perm], rtol=0, atol=1e-12)
+
+
+# ---- solvers ----------------------------------------------------------------------------
+@pytest.mark.parametrize("prec", [O.PREC_NONE, O.PREC_JACOBI])
+def test_cg_vs_scipy_and_dense(prec):
+    rp, ci, v = O.fe_system(1, 2, 40)
+    b = O.fe_rhs(1, 2, 40, 11)
+    A = sp.csr_matrix((v, ci, rp))
+    x, info = O.cg(rp, ci, v, b, prec=prec, rtol=1e-8, history=True)
+    xd = spl.spsolve(A.tocsc(), b)
+    assert info["status"] == 0
+    assert np.linalg.norm(x - xd) <= 1e-7 * np.linalg.norm(xd)
+    its = [0]
+    M = sp.diags(1 / A.diagonal()) if prec == O.PREC_JACOBI else None
+    spl.cg(A, b, rtol=1e-8, atol=0, M=M, callback=lambda xk: its.__setitem__(0, its[0] + 1))
+    assert abs(info["iters"] - its[0]) <= 1
+    h = info["history"]
+    assert h[0] == info["res_init"] and h[-1] < 1e-8 * h[0] <= h[-2]
+    # recursive residual tracks the true residual
+    assert abs(np.linalg.norm(b - A @ x) - info["res_final"]) <= 1e-9 * info["res_init"]
+
+
+def test_cg_stopping_rules():
+    rp, ci, v = O.fe_system(1, 2, 16)
+    b = O.fe_rhs(1, 2, 16, 2)
+    x, info = O.cg(rp, ci, v, b, maxit=5, rtol=1e-12)
+    assert info["status"] == 1 and info["iters"] == 5
+    x, info = O.cg(rp, ci, v, np.zeros_like(b), maxit=5, rtol=1e-12)
+    assert info["status"] == 0 and info["iters"] == 0
+    x, info = O.cg(rp, ci, v, b, atol=1e-3, rtol=0.0)
+    assert info["status"] == 0 and info["res_final"] <= 1e-3
+
+
+@pytest.mark.parametrize("prec,flexible", [(O.PREC_NONE, False), (O.PREC_JACOBI, False),
+                                           (O.PREC_SGS, True), (O.PREC_MCSGS, True)])
+def test_gmres_family(prec, flexible):
+    rp, ci, v = O.fe_system(1, 2, 24, O.fe_kappa(2, 24, 1.0, 3))
+    b = O.fe_rhs(1, 2, 24, 5)
+    A = sp.csr_matrix((v, ci, rp))
+    x, info = O.gmres(rp, ci, v, b, prec=prec, basis=30, flexible=flexible, rtol=1e-9,
+                      maxit=5000)
+    xd = spl.spsolve(A.tocsc(), b)
+    assert info["status"] == 0
+    assert np.linalg.norm(x - xd) <= 1e-6 * np.linalg.norm(xd)
+    assert np.linalg.norm(b - A @ x) <= 1.2e-9 * info["res_init"] * 10
+
+
+def test_gmres_finite_termination():
+    # unrestarted GMRES on N unknowns reaches the exact solution in <= N steps
+    rng = np.random.default_rng(4)
+    n = 12
+    Ad = rng.uniform(-1, 1, (n, n)) + 4 * np.eye(n)
+    A = sp.csr_matrix(Ad)
+    b = rng.uniform(-1, 1, n)
+    x, info = O.gmres(A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data, b,
+                      basis=n, rtol=1e-13, maxit=n)
+    assert info["iters"] <= n
+    np.testing.assert_allclose(x, np.linalg.solve(Ad, b), rtol=1e-10, atol=1e-12)
+
+
+def test_mpir_reaches_fp64_tolerance():
+    rp, ci, v = O.fe_system(1, 3, 10, O.fe_kappa(3, 10, 1.0, 7))
+    b = O.fe_rhs(1, 3, 10, 8)
+    A = sp.csr_matrix((v, ci, rp))
+    x, info = O.mpir(rp, ci, v, b, rtol=1e-10, inner_rtol=1e-4)
+    assert info["status"] == 0 and info["outer"] >= 2
+    assert np.linalg.norm(b - A @ x) <= 1e-10 * info["res_init"]
+    xd = spl.spsolve(A.tocsc(), b)
+    assert np.linalg.norm(x - xd) <= 1e-8 * np.linalg.norm(xd)
