@@ -148,15 +148,20 @@ int hflmp_csr_diag_ptr(hflmp_ctx *ctx, int nrow, const int *rp, const int *ci, i
   return HFLMP_OK;
 }
 
-int hflmp_csr_zero_rows_f64(hflmp_ctx *ctx, int m, const int *rows, const int *rp, const int *ci,
-                            double *val, double alpha) {
-  if (!ctx) return HFLMP_EINVAL;
-  HFLMP_REQUIRE(ctx, m >= 0, "zero_rows: negative size");
-  if (m == 0) return HFLMP_OK;
-  HFLMP_REQUIRE(ctx, rows && rp && ci && val, "zero_rows: null pointer");
-  zero_rows_kernel<double><<<grid_for(m, 1), kBlock, 0, ctx->stream>>>(m, rows, rp, ci, val, alpha);
-  HFLMP_LAUNCH_CHECK(ctx);
-  return HFLMP_OK;
-}
+#define DEFINE_ZERO_ROWS(T, S)                                                              \
+  int hflmp_csr_zero_rows_##S(hflmp_ctx *ctx, int m, const int *rows, const int *rp,        \
+                              const int *ci, T *val, T alpha) {                             \
+    if (!ctx) return HFLMP_EINVAL;                                                          \
+    HFLMP_REQUIRE(ctx, m >= 0, "zero_rows: negative size");                                \
+    if (m == 0) return HFLMP_OK;                                                            \
+    HFLMP_REQUIRE(ctx, rows && rp && ci && val, "zero_rows: null pointer");                \
+    zero_rows_kernel<T><<<grid_for(m, 1), kBlock, 0, ctx->stream>>>(m, rows, rp, ci, val,   \
+                                                                    alpha);                 \
+    HFLMP_LAUNCH_CHECK(ctx);                                                                \
+    return HFLMP_OK;                                                                        \
+  }
+
+DEFINE_ZERO_ROWS(double, f64)
+DEFINE_ZERO_ROWS(float, f32)
 
 }  // extern "C"

@@ -15,6 +15,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_DIR = os.path.join(_HERE, "lib")
 HIP_LIB = os.path.join(LIB_DIR, "libhflmp_hip.so")
 GEN_LIB = os.path.join(LIB_DIR, "libhflmp_gen.so")
+LMP_LIB = os.path.join(LIB_DIR, "libhflmp_lmp.so")
 
 HFLMP_OK = 0
 STATUS = {0: "converged", 1: "max iterations", 2: "diverged", 3: "breakdown"}
@@ -86,6 +87,10 @@ def hip_lib():
             g("csr_extract_invdiag").argtypes = [vp, i, vp, vp, vp, vp, C.POINTER(i)]
             g("pcg_jacobi").argtypes = [vp, i, i, vp, vp, vp, vp, vp, vp, i, d, d, d,
                                         vp, sz, C.POINTER(SolveInfo)]
+        L.hflmp_mcsgs_create_f64.argtypes = [vp, i, i, vp, vp, vp, i, vp, vp, d,
+                                             C.POINTER(vp)]
+        L.hflmp_mcsgs_apply_f64.argtypes = [vp, vp, vp, vp]
+        L.hflmp_mcsgs_destroy.argtypes = [vp, vp]
         L.hflmp_cast_f64_to_f32.argtypes = [vp, i, vp, vp]
         L.hflmp_cast_f32_to_f64.argtypes = [vp, i, vp, vp]
         L.hflmp_cast_csr_f64_to_f32.argtypes = [vp, i, vp, vp]
@@ -346,3 +351,106 @@ def pcg_jacobi(ctx, A, invd, b, x, maxit=10000, atol=0.0, rtol=1e-8, dtol=1e6, w
         if own:
             work.free()
     return info.as_dict()
+
+
+class McSgs:
+    """Multicoloured symmetric Gauss-Seidel on a device CSR matrix
+    (hflmp_mcsgs_*); `coloring` = (ncolors, color_sizes, perm) on the host."""
+
+    def __init__(self, ctx, A, coloring, omega=1.0):
+        nc, csz, perm = coloring[:3]
+        csz = np.ascontiguousarray(csz, np.int32)
+        perm = np.ascontiguousarray(perm, np.int32)
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        _call(ctx, "hflmp_mcsgs_create_f64", A.nrow, A.nnz, A.rp.ptr, A.ci.ptr, A.val.ptr,
+              int(nc), csz.ctypes.data, perm.ctypes.data, float(omega), C.byref(self.h))
+
+    def apply(self, r, z):
+        _call(self.ctx, "hflmp_mcsgs_apply_f64", self.h, r.ptr, z.ptr)
+
+    def destroy(self):
+        if self.h:
+            self.ctx.L.hflmp_mcsgs_destroy(self.ctx.h, self.h)
+            self.h = None
+
+
+# ---- C++ lmp layer (libhflmp_lmp.so) ---------------------------------------------------
+LMP_CG, LMP_GMRES, LMP_FGMRES, LMP_MPIR = 0, 1, 2, 3
+LMP_PREC_NONE, LMP_PREC_JACOBI, LMP_PREC_MCSGS = 0, 1, 2
+
+
+class LmpOptions(C.Structure):
+    _fields_ = [("solver", C.c_int), ("precond", C.c_int), ("precision", C.c_int),
+                ("maxit", C.c_int), ("atol", C.c_double), ("rtol", C.c_double),
+                ("dtol", C.c_double), ("basis", C.c_int), ("omega", C.c_double),
+                ("mpir_max_outer", C.c_int), ("mpir_atol", C.c_double),
+                ("mpir_rtol", C.c_double), ("solver_inner", C.c_int)]
+
+
+class LmpResult(C.Structure):
+    _fields_ = [("iters", C.c_int), ("status", C.c_int), ("outer", C.c_int),
+                ("res_init", C.c_double), ("res_final", C.c_double), ("seconds", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lmp = None
+
+
+def lmp_lib():
+    global _lmp
+    if _lmp is None:
+        if not os.path.exists(LMP_LIB):
+            raise HflmpError("%s not built: run __graft_entry__.build()" % LMP_LIB)
+        L = C.CDLL(LMP_LIB)
+        L.hflmp_lmp_default_options.argtypes = [C.POINTER(LmpOptions)]
+        L.hflmp_lmp_solve.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                      C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.POINTER(LmpOptions), C.POINTER(LmpResult)]
+        L.hflmp_lmp_vector_mult.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                            C.c_void_p, C.c_void_p, C.c_void_p]
+        L.hflmp_lmp_last_error.restype = C.c_char_p
+        _lmp = L
+    return _lmp
+
+
+def lmp_options(**kw):
+    o = LmpOptions()
+    lmp_lib().hflmp_lmp_default_options(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def lmp_solve(rp, ci, v, b, x0=None, device=0, **kw):
+    """Solve with the C++ solver templates over CoupledMatrix/CoupledVector on
+    the HIP platform; returns (x, result dict)."""
+    rp = np.ascontiguousarray(rp, np.int32)
+    ci = np.ascontiguousarray(ci, np.int32)
+    v = np.ascontiguousarray(v, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    x = np.zeros(len(rp) - 1) if x0 is None else np.array(x0, np.float64)
+    o = lmp_options(**kw)
+    r = LmpResult()
+    rc = lmp_lib().hflmp_lmp_solve(device, len(rp) - 1, len(ci), rp.ctypes.data,
+                                   ci.ctypes.data, v.ctypes.data, b.ctypes.data,
+                                   x.ctypes.data, C.byref(o), C.byref(r))
+    if rc:
+        raise HflmpError("hflmp_lmp_solve: " + lmp_lib().hflmp_lmp_last_error().decode())
+    return x, r.as_dict()
+
+
+def lmp_vector_mult(rp, ci, v, x, device=0):
+    rp = np.ascontiguousarray(rp, np.int32)
+    ci = np.ascontiguousarray(ci, np.int32)
+    v = np.ascontiguousarray(v, np.float64)
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.empty(len(rp) - 1)
+    rc = lmp_lib().hflmp_lmp_vector_mult(device, len(rp) - 1, len(ci), rp.ctypes.data,
+                                         ci.ctypes.data, v.ctypes.data, x.ctypes.data,
+                                         y.ctypes.data)
+    if rc:
+        raise HflmpError("hflmp_lmp_vector_mult: " + lmp_lib().hflmp_lmp_last_error().decode())
+    return y

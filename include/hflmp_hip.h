@@ -182,6 +182,9 @@ int hflmp_csr_diag_ptr(hflmp_ctx *ctx, int nrow, const int *row_ptr,
 int hflmp_csr_zero_rows_f64(hflmp_ctx *ctx, int nrows_zero, const int *rows,
                             const int *row_ptr, const int *col, double *val,
                             double alpha);
+int hflmp_csr_zero_rows_f32(hflmp_ctx *ctx, int nrows_zero, const int *rows,
+                            const int *row_ptr, const int *col, float *val,
+                            float alpha);
 /* val_f32 = (float) val_f64 (CastFrom on matrices, lmatrix.h) */
 int hflmp_cast_csr_f64_to_f32(hflmp_ctx *ctx, int nnz, const double *val,
                               float *val32);
@@ -194,6 +197,26 @@ int hflmp_jacobi_apply_f64(hflmp_ctx *ctx, int n, const double *invdiag,
                            const double *r, double *z);
 int hflmp_jacobi_apply_f32(hflmp_ctx *ctx, int n, const float *invdiag,
                            const float *r, float *z);
+
+/* Multicoloured symmetric Gauss-Seidel
+ * (lPreconditioner_MultiColoring_SymmetricGaussSeidel,
+ *  lpreconditioner_mc.cc:147-723, lpreconditioner_mc_sgs.cc:44-90;
+ *  colouring CPU_CSR_lMatrix::Multicoloring lmatrix_csr_cpu.cc:2195-2300).
+ * create() takes the HOST colouring (h_perm: rows grouped by colour,
+ * h_color_sizes[ncolors]) and the DEVICE matrix, checks that no two rows of
+ * one colour couple, and builds the per-colour split; apply() computes
+ * z = M^-1 r with M = w(2-w) (D + wL) D^-1 (D + wU) in the colour order.
+ * The object keeps pointers to col/val, which must outlive it; new values
+ * in val are picked up without a rebuild. */
+typedef struct hflmp_mcsgs hflmp_mcsgs;
+int hflmp_mcsgs_create_f64(hflmp_ctx *ctx, int nrow, int nnz,
+                           const int *row_ptr, const int *col,
+                           const double *val, int ncolors,
+                           const int *h_color_sizes, const int *h_perm,
+                           double omega, hflmp_mcsgs **out);
+int hflmp_mcsgs_apply_f64(hflmp_ctx *ctx, const hflmp_mcsgs *m,
+                          const double *r, double *z);
+int hflmp_mcsgs_destroy(hflmp_ctx *ctx, hflmp_mcsgs *m);
 
 /* ---- device-resident Krylov solves ----------------------------------------
  * Jacobi-preconditioned CG with the arithmetic of CG::SolvePrecond

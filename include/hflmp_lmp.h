@@ -1,0 +1,62 @@
+/*
+ * hflmp_lmp.h -- C entry points into the C++ lmp layer (libhflmp_lmp.so).
+ *
+ * These drive the unchanged solver templates (CG, GMRES, FGMRES, MPIR over
+ * CoupledMatrix / CoupledVector) on the HIP platform for a system given as
+ * host CSR arrays.  They exist for embedding and testing: the reference's
+ * own solver code talks to the lmp classes directly.
+ * Errors inside the C++ layer are returned as negative codes with a
+ * message (hflmp_lmp_last_error) instead of ending the process.
+ */
+#ifndef HFLMP_LMP_H
+#define HFLMP_LMP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HFLMP_LMP_CG 0
+#define HFLMP_LMP_GMRES 1
+#define HFLMP_LMP_FGMRES 2
+#define HFLMP_LMP_MPIR 3 /* FP64 refinement around an FP32 inner solver */
+
+#define HFLMP_LMP_PREC_NONE 0
+#define HFLMP_LMP_PREC_JACOBI 1
+#define HFLMP_LMP_PREC_MCSGS 2
+
+typedef struct {
+  int solver;         /* HFLMP_LMP_* (CG, GMRES, FGMRES, MPIR)             */
+  int precond;        /* HFLMP_LMP_PREC_*                                  */
+  int precision;      /* 64 or 32: value type of CG/GMRES/FGMRES           */
+  int maxit;
+  double atol, rtol, dtol;
+  int basis;          /* GMRES/FGMRES basis size                           */
+  double omega;       /* MC-SGS relaxation                                 */
+  int mpir_max_outer; /* MPIR: outer corrections; inner solver = `solver_inner` */
+  double mpir_atol, mpir_rtol;
+  int solver_inner;   /* MPIR inner solver: HFLMP_LMP_CG/GMRES/FGMRES      */
+} hflmp_lmp_options;
+
+typedef struct {
+  int iters;       /* iterations (MPIR: accumulated inner iterations)       */
+  int status;      /* 0 success, 1 not converged, 2 solver error            */
+  int outer;       /* MPIR corrections                                      */
+  double res_init; /* initial residual norm                                 */
+  double res_final;
+  double seconds;  /* Solve() wall time                                     */
+} hflmp_lmp_result;
+
+void hflmp_lmp_default_options(hflmp_lmp_options *opt);
+/* Solve A x = b; x holds x0 on entry.  Host arrays, FP64. */
+int hflmp_lmp_solve(int device, int nrow, int nnz, const int *row_ptr, const int *col,
+                    const double *val, const double *b, double *x,
+                    const hflmp_lmp_options *opt, hflmp_lmp_result *res);
+/* y = A x through CoupledMatrix::VectorMult on the HIP platform. */
+int hflmp_lmp_vector_mult(int device, int nrow, int nnz, const int *row_ptr, const int *col,
+                          const double *val, const double *x, double *y);
+const char *hflmp_lmp_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,321 @@
+// The reference's lmp known-answer tests, run on the HIP platform through
+// the factory, the Coupled layer and the unchanged solver templates:
+//   tests/lmp_test.cc:42-90, tests/cuda_test.cc:29-108, tests/cg_test.cc:
+//   30-139, tests/parallel_vector_test.cc:402-660,
+//   tests/parallel_matrix_test.cc:922-1048,1187+ (one process).
+// Prints one line per case and exits non-zero on the first failure.
+#include <cmath>
+#include <cstdio>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "init_vec_mat.h"
+#include "la_descriptor.h"
+#include "linear_solver/cg.h"
+#include "linear_solver/gmres.h"
+#include "linear_solver/mpir.h"
+#include "linear_solver/preconditioner_bjacobi_standard.h"
+#include "lmp_log.h"
+#include "platform_management.h"
+
+using namespace hiflow;
+using namespace hiflow::la;
+
+static int g_fail = 0;
+#define CHECK_CLOSE(a, b, tol)                                                         \
+  do {                                                                                 \
+    const double a_ = (a), b_ = (b);                                                   \
+    if (!(std::fabs(a_ - b_) <= (tol))) {                                              \
+      std::printf("  FAIL %s:%d: %s = %.17g, expected %.17g\n", __FILE__, __LINE__, #a, \
+                  a_, b_);                                                             \
+      ++g_fail;                                                                        \
+    }                                                                                  \
+  } while (0)
+#define CHECK(c)                                                                \
+  do {                                                                          \
+    if (!(c)) {                                                                 \
+      std::printf("  FAIL %s:%d: %s\n", __FILE__, __LINE__, #c);                \
+      ++g_fail;                                                                 \
+    }                                                                           \
+  } while (0)
+
+static void run(const char *name, void (*f)()) {
+  const int before = g_fail;
+  f();
+  std::printf("%s %s\n", g_fail == before ? "ok  " : "FAIL", name);
+  std::fflush(stdout);
+}
+
+// lmp_test.cc: TransformFromCOO of [[2,-1,0],[-1,2,-1],[0,-1,2]] times [1,-2,0]
+static void test_lmp_vectormult() {
+  std::unique_ptr<lMatrix<double>> A(init_matrix<double>(7, 3, 3, "sm", HIP, HIPKERNEL, CSR));
+  const int r[] = {0, 0, 1, 1, 1, 2, 2}, c[] = {0, 1, 0, 1, 2, 1, 2};
+  const double v[] = {2, -1, -1, 2, -1, -1, 2};
+  A->TransformFromCOO(r, c, v, 3, 3, 7);
+  std::unique_ptr<lVector<double>> x(init_vector<double>(3, "svi", HIP, HIPKERNEL));
+  std::unique_ptr<lVector<double>> y(init_vector<double>(3, "svo", HIP, HIPKERNEL));
+  x->add_value(0, 1.0);
+  x->add_value(1, -2.0);
+  x->add_value(2, 0.0);
+  A->VectorMult(*x, y.get());
+  CHECK(y->GetValue(0) == 4.0);
+  CHECK(y->GetValue(1) == -5.0);
+  CHECK(y->GetValue(2) == 2.0);
+}
+
+// cuda_test.cc: dense 9-entry CSR (explicit zeros) times [1.2,-2.3,0.7]
+static void test_dense_csr_vectormult() {
+  std::unique_ptr<lMatrix<double>> A(init_matrix<double>(9, 3, 3, "A", HIP, HIPKERNEL, CSR));
+  std::vector<int> r, c;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      r.push_back(i);
+      c.push_back(j);
+    }
+  A->init_structure(r.data(), c.data());
+  const double vals[3][3] = {{2, -1, 0}, {-1, 2, -1}, {0, -1, 2}};
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) A->add_value(i, j, vals[i][j]);
+  CHECK(A->get_nnz() == 9);
+  std::unique_ptr<lVector<double>> x(init_vector<double>(3, "x", HIP, HIPKERNEL));
+  std::unique_ptr<lVector<double>> y(x->CloneWithoutContent());
+  const int idx[] = {0, 1, 2};
+  const double xv[] = {1.2, -2.3, 0.7};
+  x->SetValues(idx, 3, xv);
+  y->Zeros();
+  A->VectorMult(*x, y.get());
+  double out[3];
+  y->GetValues(idx, 3, out);
+  CHECK_CLOSE(out[0], 4.7, 1e-12);
+  CHECK_CLOSE(out[1], -6.5, 1e-12);
+  CHECK_CLOSE(out[2], 3.7, 1e-12);
+}
+
+static void make_3x3(CoupledMatrix<double> &A) {
+  LaCouplings cp(3);
+  A.Init(cp);
+  const int r[] = {0, 0, 1, 1, 1, 2, 2}, c[] = {0, 1, 0, 1, 2, 1, 2};
+  A.InitStructure(r, c, 7, nullptr, nullptr, 0);
+  const double v[] = {2, -1, -1, 2, -1, -1, 2};
+  for (int k = 0; k < 7; ++k) A.Add(r[k], c[k], v[k]);
+}
+
+// cg_test.cc: A x = [1,-2,0] -> x = [-0.25,-1.5,-0.75] to 1e-12
+template <class LAD>
+static void cg_3x3(bool precond) {
+  CoupledMatrix<double> A;
+  make_3x3(A);
+  CoupledVector<double> b, x;
+  b.Init(3);
+  x.Init(3);
+  const int idx[] = {0, 1, 2};
+  const double bv[] = {1, -2, 0};
+  b.SetValues(idx, 3, bv);
+  CG<LAD> cg;
+  cg.InitControl(100, 1e-15, 1e-14, 1e6);
+  PreconditionerBlockJacobiStand<LAD> pre;
+  if (precond) {
+    pre.Init_Jacobi();
+    cg.SetupPreconditioner(pre);
+    cg.InitParameter("Preconditioning");
+  } else {
+    cg.InitParameter("NoPreconditioning");
+  }
+  cg.SetupOperator(A);
+  CHECK(cg.Solve(b, &x) == kSolverSuccess);
+  CHECK_CLOSE(x.GetValue(0), -0.25, 1e-12);
+  CHECK_CLOSE(x.GetValue(1), -1.5, 1e-12);
+  CHECK_CLOSE(x.GetValue(2), -0.75, 1e-12);
+  CHECK(cg.iter() <= 3);
+}
+static void test_cg_noprecond() { cg_3x3<LADescriptorCoupledD>(false); }
+static void test_cg_jacobi() { cg_3x3<LADescriptorCoupledD>(true); }
+
+// parallel_vector_test.cc: v_i = i, w_i = i + 2, i < 10
+static void test_vector_ops() {
+  CoupledVector<double> v, w;
+  v.Init(10);
+  w.CloneFromWithoutContent(v);
+  CHECK(w.size_global() == 10 && w.ghost().get_size() == 0);
+  for (int i = 0; i < 10; ++i) {
+    v.Add(i, double(i));
+    w.Add(i, double(i + 2));
+  }
+  const double N = 9.0;
+  CHECK_CLOSE(v.Dot(w), N * (N + 1) * (2 * N + 1) / 6 + N * (N + 1), 1e-10);
+  CoupledVector<double> t;
+  t.CloneFrom(v);
+  t.Axpy(w, 4.0);  // axpy1
+  for (int i = 0; i < 10; ++i) CHECK_CLOSE(t.GetValue(i), i + 4.0 * (i + 2), 1e-12);
+  t.CopyFrom(v);
+  t.Scale(0.5);  // scale
+  for (int i = 0; i < 10; ++i) CHECK_CLOSE(t.GetValue(i), 0.5 * i, 1e-12);
+  t.CopyFrom(v);
+  t.ScaleAdd(w, 4.0);  // scale_add: t = 4 t + w
+  for (int i = 0; i < 10; ++i) CHECK_CLOSE(t.GetValue(i), 4.0 * i + (i + 2), 1e-12);
+  CHECK_CLOSE(v.Norm2(), std::sqrt(N * (N + 1) * (2 * N + 1) / 6), 1e-12);
+  CHECK_CLOSE(v.NormMax(), 9.0, 0);
+  CHECK_CLOSE(v.Norm1(), 45.0, 1e-12);
+  CHECK(v.interior().ArgMax() == 9 && v.interior().ArgMin() == 0);
+  CoupledVector<float> f;
+  f.Init(10);
+  f.CastInteriorFrom(w);
+  CHECK_CLOSE(f.GetValue(7), 9.0, 0);
+  CHECK(!v.interior().ContainsNaN());
+}
+
+// parallel_matrix_test.cc: tridiagonal block times ones -> [1,0,...,0,1];
+// diagonalize_rows keeps the columns
+static void test_matrix_tridiagonal() {
+  const int n = 10;
+  std::vector<int> r, c;
+  for (int i = 0; i < n; ++i)
+    for (int j = i - 1; j <= i + 1; ++j)
+      if (j >= 0 && j < n) {
+        r.push_back(i);
+        c.push_back(j);
+      }
+  CoupledMatrix<double> A;
+  A.Init(LaCouplings(n));
+  A.InitStructure(r.data(), c.data(), (int)r.size(), nullptr, nullptr, 0);
+  std::vector<int> rows(n);
+  for (int i = 0; i < n; ++i) rows[i] = i;
+  std::vector<double> val(n * n, 0.0);
+  for (int i = 0; i < n; ++i)
+    for (int j = i - 1; j <= i + 1; ++j)
+      if (j >= 0 && j < n) val[i * n + j] = (i == j) ? 2.0 : -1.0;
+  // full row x column block: entries outside the pattern are skipped by the
+  // reference's SetValues only when zero; set the pattern entries one by one
+  for (int k = 0; k < (int)r.size(); ++k) A.SetValues(&r[k], 1, &c[k], 1, &val[r[k] * n + c[k]]);
+  CoupledVector<double> in, out;
+  in.Init(n);
+  out.Init(n);
+  std::vector<double> ones(n, 1.0);
+  in.Add(rows.data(), n, ones.data());
+  A.VectorMult(in, &out);
+  CHECK_CLOSE(out.GetValue(0), 1.0, 1e-14);
+  for (int i = 1; i < n - 1; ++i) CHECK_CLOSE(out.GetValue(i), 0.0, 1e-14);
+  CHECK_CLOSE(out.GetValue(n - 1), 1.0, 1e-14);
+  const int drow[] = {0, 4};
+  A.diagonalize_rows(drow, 2, 3.0);
+  double g;
+  A.diagonal().get_value(4, 4, &g);
+  CHECK(g == 3.0);
+  A.diagonal().get_value(4, 3, &g);
+  CHECK(g == 0.0);
+  A.diagonal().get_value(3, 4, &g);
+  CHECK(g == -1.0);
+}
+
+// GMRES / FGMRES / MPIR on a 1D Poisson system through the same templates
+static void poisson_1d(int n, CoupledMatrix<double> &A, CoupledVector<double> &b) {
+  std::vector<int> r, c;
+  std::vector<double> v;
+  for (int i = 0; i < n; ++i)
+    for (int j = i - 1; j <= i + 1; ++j)
+      if (j >= 0 && j < n) {
+        r.push_back(i);
+        c.push_back(j);
+        v.push_back(i == j ? 2.0 : -1.0);
+      }
+  A.Init(LaCouplings(n));
+  A.InitStructure(r.data(), c.data(), (int)r.size(), nullptr, nullptr, 0);
+  for (size_t k = 0; k < r.size(); ++k) A.Add(r[k], c[k], v[k]);
+  b.Init(n);
+  for (int i = 0; i < n; ++i) b.Add(i, std::sin(0.37 * i) + 0.1);
+}
+
+static double residual(CoupledMatrix<double> &A, CoupledVector<double> &b,
+                       CoupledVector<double> &x) {
+  CoupledVector<double> r;
+  r.CloneFromWithoutContent(b);
+  A.VectorMult(x, &r);
+  r.ScaleAdd(b, -1.0);
+  return r.Norm2() / b.Norm2();
+}
+
+static void test_gmres_fgmres_mpir() {
+  CoupledMatrix<double> A;
+  CoupledVector<double> b;
+  poisson_1d(60, A, b);
+  {
+    CoupledVector<double> x;
+    x.CloneFromWithoutContent(b);
+    GMRES<LADescriptorCoupledD> gm;
+    gm.InitControl(1000, 0.0, 1e-10, 1e6);
+    gm.InitParameter(60, "NoPreconditioning");
+    gm.SetupOperator(A);
+    CHECK(gm.Solve(b, &x) == kSolverSuccess);
+    CHECK(residual(A, b, x) < 1e-9);
+  }
+  {
+    CoupledVector<double> x;
+    x.CloneFromWithoutContent(b);
+    FGMRES<LADescriptorCoupledD> fg;
+    PreconditionerBlockJacobiStand<LADescriptorCoupledD> pre;
+    pre.Init_MultiColoring_SymmetricGaussSeidel(1.0);
+    fg.InitControl(1000, 0.0, 1e-10, 1e6);
+    fg.InitParameter(20, "RightPreconditioning");
+    fg.SetupPreconditioner(pre);
+    fg.SetupOperator(A);
+    CHECK(fg.Solve(b, &x) == kSolverSuccess);
+    CHECK(residual(A, b, x) < 1e-9);
+  }
+  {
+    CoupledVector<double> x;
+    x.CloneFromWithoutContent(b);
+    MPIR<LADescriptorCoupledD, LADescriptorCoupledS> mpir;
+    CG<LADescriptorCoupledS> inner;
+    PreconditionerBlockJacobiStand<LADescriptorCoupledS> pre;
+    pre.Init_Jacobi();
+    inner.InitControl(5000, 0.0, 1e-4, 1e6);
+    inner.InitParameter("Preconditioning");
+    inner.SetupPreconditioner(pre);
+    mpir.SetMpirParameters(30, 0.0, 1e-11, 1e300);
+    mpir.SetupOperator(A);
+    mpir.SetupErrorCorrectionSolverLow(inner);
+    CHECK(mpir.Solve(b, &x) == kSolverSuccess);
+    CHECK(mpir.num_corrections() >= 2);
+    CHECK(residual(A, b, x) < 1e-11);
+  }
+}
+
+static void test_factory_errors() {
+  set_fatal_handler([](const std::string &) {});
+  bool threw = false;
+  try {
+    std::unique_ptr<lVector<double>> v(init_vector<double>(4, "cpu", CPU, NAIVE));
+  } catch (const LmpError &) {
+    threw = true;
+  }
+  CHECK(threw);
+  threw = false;
+  try {
+    CoupledMatrix<double> A;
+    make_3x3(A);
+    A.Add(0, 2, 1.0);  // (0,2) is outside the pattern
+  } catch (const LmpError &) {
+    threw = true;
+  }
+  CHECK(threw);
+  set_fatal_handler(nullptr);
+}
+
+int main() {
+  SYSTEM sys;
+  sys.Platform = HIP;
+  sys.device = 0;
+  init_platform(sys);
+  run("lmp_test VectorMult (TransformFromCOO)", test_lmp_vectormult);
+  run("cuda_test VectorMult (dense CSR)", test_dense_csr_vectormult);
+  run("cg_test NoPreconditioning", test_cg_noprecond);
+  run("cg_test Jacobi", test_cg_jacobi);
+  run("parallel_vector_test dot/axpy/scale/scale_add", test_vector_ops);
+  run("parallel_matrix_test vector_mult_diag + diagonalize_rows", test_matrix_tridiagonal);
+  run("GMRES / FGMRES(MC-SGS) / MPIR(FP32 CG)", test_gmres_fgmres_mpir);
+  run("factory and pattern errors", test_factory_errors);
+  stop_platform(sys);
+  std::printf("%s: %d failure(s)\n", g_fail ? "FAILED" : "PASSED", g_fail);
+  return g_fail ? 1 : 0;
+}
