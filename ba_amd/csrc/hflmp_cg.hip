@@ -67,17 +67,17 @@ __global__ void __launch_bounds__(kFinalBlock)
   }
 }
 
-// K1: Ap = A p, partials of p.Ap
-template <typename T>
+// K1: Ap = A p, partials of p.Ap (IT: int32 columns or int16 tile offsets)
+template <typename T, typename IT = int>
 __global__ void __launch_bounds__(kBlock)
-    cg_spmv_kernel(int nrow, int R, const int *__restrict__ rp, const int *__restrict__ ci,
+    cg_spmv_kernel(int nrow, int R, const int *__restrict__ rp, const IT *__restrict__ ci,
                    const T *__restrict__ val, const T *__restrict__ p, T *__restrict__ Ap,
                    const CgState<T> *st, T *__restrict__ partials) {
   __shared__ T prod[TileCap<T>::value];
   if (st->status != kIterate) return;
   const long long tile = xcd_tile(blockIdx.x, gridDim.x);
   TileRow tr;
-  const T s = tile_row_sum<T, false>(tile, R, nrow, rp, ci, val, p, T(0), tr, prod);
+  const T s = tile_row_sum<T, false, IT>(tile, R, nrow, rp, ci, val, p, T(0), tr, prod);
   T part = 0;
   if (tr.has_row) {
     Ap[tr.row] = s;
@@ -207,6 +207,233 @@ int pcg_jacobi(hflmp_ctx *ctx, int nrow, int nnz, const int *rp, const int *ci, 
   return HFLMP_OK;
 }
 
+// ---- FP32-inner / FP64-refined CG -----------------------------------------------
+// The refinement of MPIR (linear_solver/mpir.cc:52-296) with the FP32
+// Jacobi-PCG above as the error-correction solver, every step on the device:
+//   r = b - A x (FP64 SpMV and residual, FP64 norm)  ->  r32 = (float) r
+//   FP32 PCG on A32 c = r32 from c = 0                ->  x += (double) c
+// The outer relative test uses the initial residual throughout.
+
+struct MpState {
+  double res0, res;
+  int status;
+  unsigned int bad16;  // entries whose column does not fit an int16 tile offset
+};
+
+// c16[j] = col[j] - r0(tile of row(j)) as int16, for the FP32 inner SpMV;
+// counts the entries that do not fit.
+__global__ void __launch_bounds__(kBlock)
+    tile_offsets16_kernel(int nrow, int R, const int *__restrict__ rp,
+                          const int *__restrict__ ci, short *__restrict__ c16,
+                          unsigned int *bad) {
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < nrow; i += gridDim.x * kBlock) {
+    const int base = (i / R) * R;
+    for (int j = rp[i]; j < rp[i + 1]; ++j) {
+      int d = ci[j] - base;
+      if (d < -32768 || d > 32767) {
+        atomicAdd(bad, 1u);
+        d = 0;
+      }
+      c16[j] = (short)d;
+    }
+  }
+}
+
+// r = b - A x in FP64; r32 = (float) r; start of the FP32 PCG from c = 0:
+// z = D32^-1 r32, p = z.  Partials: FP64 r.r; FP32 r32.r32 and r32.z.
+__global__ void __launch_bounds__(kBlock)
+    mp_residual_kernel(int nrow, int R, const int *__restrict__ rp, const int *__restrict__ ci,
+                       const double *__restrict__ val, const double *__restrict__ b,
+                       const double *__restrict__ x, const float *__restrict__ invd32,
+                       float *__restrict__ r32, float *__restrict__ z32, float *__restrict__ p32,
+                       double *__restrict__ part64, float *__restrict__ part32) {
+  __shared__ double prod[TileCap<double>::value];
+  const long long tile = xcd_tile(blockIdx.x, gridDim.x);
+  TileRow tr;
+  const double s = tile_row_sum<double, false>(tile, R, nrow, rp, ci, val, x, 0.0, tr, prod);
+  double rr = 0;
+  float rr32 = 0, rz32 = 0;
+  if (tr.has_row) {
+    const int i = tr.row;
+    const double ri = -1.0 * s + b[i];
+    const float rf = (float)ri;
+    const float zf = rf * invd32[i];
+    r32[i] = rf;
+    z32[i] = zf;
+    p32[i] = zf;
+    rr = ri * ri;
+    rr32 = rf * rf;
+    rz32 = rf * zf;
+  }
+  block_partial1(rr, part64);
+  block_partial2(rr32, rz32, part32);
+}
+
+// Outer test on the FP64 residual, then the FP32 PCG's initial state.  The
+// inner relative tolerance is inner_rtol, relaxed to what the outer target
+// still needs: max(inner_rtol, 0.5 * rtol * res_0 / res), so the last
+// correction does not solve further than the outer test requires.
+__global__ void __launch_bounds__(kFinalBlock)
+    mp_residual_final(MpState *ms, CgState<float> *st, const double *__restrict__ part64,
+                      const float *__restrict__ part32, int count, int outer, int max_outer,
+                      double atol, double rtol, double inner_rtol) {
+  __shared__ double lds64[kFinalBlock / kWave];
+  __shared__ float lds32[kFinalBlock / kWave];
+  const double rr = sum_partials(part64, count, 1, 0, lds64);
+  const float rr32 = sum_partials(part32, count, 2, 0, lds32);
+  const float rz32 = sum_partials(part32, count, 2, 1, lds32);
+  if (threadIdx.x == 0) {
+    const double res = sqrt(rr);
+    if (outer == 0) ms->res0 = res;
+    ms->res = res;
+    const double res0 = ms->res0;
+    int status = kIterate;
+    if (res <= atol || res < rtol * res0) status = kConverged;
+    else if (!(res == res)) status = kBreakdown;
+    else if (outer >= max_outer) status = kMaxIts;
+    ms->status = status;
+    const double need = res > 0 ? 0.5 * rtol * res0 / res : 0.0;
+    const double tol = need > inner_rtol ? (need < 0.5 ? need : 0.5) : inner_rtol;
+    st->rtol = tol;
+    const float r0 = sqrtf(rr32);
+    st->rr = rr32;
+    st->rz = rz32;
+    st->res0 = r0;
+    st->res = r0;
+    st->iter = 0;
+    st->status = status == kIterate ? check_state(0, (double)r0, (double)r0, st->maxit,
+                                                  st->atol, st->rtol, st->dtol)
+                                    : kConverged;
+  }
+}
+
+// x += (double) c
+__global__ void __launch_bounds__(kBlock)
+    mp_correct_kernel(int n, const float *__restrict__ c, double *__restrict__ x) {
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+    x[i] += (double)c[i];
+}
+
+struct MpLayout {
+  size_t ms, st, part64, part32, vec, vstride, c16, total;
+  int R64, grid64, R32, grid32, grid2;
+};
+
+inline MpLayout mp_layout(int nrow, int nnz) {
+  MpLayout L;
+  const SolverLayout L64 = solver_layout(nrow, nnz, 8, 0);
+  const SolverLayout L32 = solver_layout(nrow, nnz, 4, 0);
+  L.R64 = L64.R;
+  L.grid64 = L64.grid1;
+  L.R32 = L32.R;
+  L.grid32 = L32.grid1;
+  L.grid2 = L32.grid2;
+  int gmax = L.grid64 > L.grid32 ? L.grid64 : L.grid32;
+  if (L.grid2 > gmax) gmax = L.grid2;
+  L.ms = 0;
+  L.st = 256;
+  L.part64 = 512;
+  L.part32 = align256(L.part64 + (size_t)gmax * 8);
+  L.vec = align256(L.part32 + 2 * (size_t)gmax * 4);
+  L.vstride = align256((size_t)nrow * 4);
+  L.c16 = L.vec + 5 * L.vstride;  // after r32, p32, z32, Ap32, c32
+  L.total = L.c16 + align256((size_t)nnz * 2);
+  return L;
+}
+
+int mpcg_jacobi(hflmp_ctx *ctx, int nrow, int nnz, const int *rp, const int *ci,
+                const double *val, const float *val32, const float *invd32, const double *b,
+                double *x, int max_outer, double atol, double rtol, int inner_maxit,
+                double inner_rtol, int flags, void *work, size_t work_bytes,
+                hflmp_mp_info *info) {
+  if (!ctx) return HFLMP_EINVAL;
+  HFLMP_REQUIRE(ctx, nrow > 0 && nnz > 0, "mpcg: empty system");
+  HFLMP_REQUIRE(ctx, rp && ci && val && val32 && invd32 && b && x && work && info,
+                "mpcg: null pointer");
+  HFLMP_REQUIRE(ctx, max_outer > 0 && inner_maxit > 0 && atol >= 0 && rtol >= 0 &&
+                         inner_rtol > 0 && inner_rtol < 1,
+                "mpcg: bad control parameters");
+  const MpLayout L = mp_layout(nrow, nnz);
+  HFLMP_REQUIRE(ctx, work_bytes >= L.total, "mpcg: workspace %zu < %zu bytes", work_bytes,
+                L.total);
+  char *w = static_cast<char *>(work);
+  auto *ms = reinterpret_cast<MpState *>(w + L.ms);
+  auto *st = reinterpret_cast<CgState<float> *>(w + L.st);
+  double *part64 = reinterpret_cast<double *>(w + L.part64);
+  float *part32 = reinterpret_cast<float *>(w + L.part32);
+  float *r = reinterpret_cast<float *>(w + L.vec);
+  float *p = reinterpret_cast<float *>(w + L.vec + L.vstride);
+  float *z = reinterpret_cast<float *>(w + L.vec + 2 * L.vstride);
+  float *Ap = reinterpret_cast<float *>(w + L.vec + 3 * L.vstride);
+  float *c = reinterpret_cast<float *>(w + L.vec + 4 * L.vstride);
+  short *c16 = reinterpret_cast<short *>(w + L.c16);
+  hipStream_t s = ctx->stream;
+  const unsigned g3 = grid_for(nrow, 4);
+  int *host_status = reinterpret_cast<int *>(static_cast<char *>(ctx->host_pinned) + 2048);
+
+  const auto t0 = std::chrono::steady_clock::now();
+  // FP32 inner operator: int16 tile offsets when the band fits
+  bool use16 = !(flags & HFLMP_MP_INT32_COLS);
+  if (use16) {
+    int rc = hflmp_fill_zero(ctx, ms, sizeof(MpState));
+    if (rc) return rc;
+    tile_offsets16_kernel<<<grid_for(nrow, 1), kBlock, 0, s>>>(nrow, L.R32, rp, ci, c16,
+                                                               &ms->bad16);
+    HFLMP_LAUNCH_CHECK(ctx);
+    HFLMP_HIP_CHECK(ctx, hipMemcpyAsync(host_status, &ms->bad16, sizeof(int),
+                                        hipMemcpyDeviceToHost, s));
+    HFLMP_HIP_CHECK(ctx, hipStreamSynchronize(s));
+    use16 = host_status[0] == 0;
+  }
+  int outer = 0, inner_total = 0;
+  for (;; ++outer) {
+    init_state_kernel<float><<<1, kBlock, 0, s>>>(st, inner_maxit, 0.0, inner_rtol, 1e30);
+    mp_residual_kernel<<<L.grid64, kBlock, 0, s>>>(nrow, L.R64, rp, ci, val, b, x, invd32, r, z,
+                                                   p, part64, part32);
+    mp_residual_final<<<1, kFinalBlock, 0, s>>>(ms, st, part64, part32, L.grid64, outer,
+                                                max_outer, atol, rtol, inner_rtol);
+    HFLMP_LAUNCH_CHECK(ctx);
+    HFLMP_HIP_CHECK(ctx, hipMemcpyAsync(host_status, &ms->status, sizeof(int),
+                                        hipMemcpyDeviceToHost, s));
+    HFLMP_HIP_CHECK(ctx, hipStreamSynchronize(s));
+    if (host_status[0] != kIterate) break;
+    int rc = hflmp_fill_zero(ctx, c, sizeof(float) * (size_t)nrow);
+    if (rc) return rc;
+    auto iteration = [&]() {
+      if (use16)
+        cg_spmv_kernel<float, short><<<L.grid32, kBlock, 0, s>>>(nrow, L.R32, rp, c16, val32, p,
+                                                                 Ap, st, part32);
+      else
+        cg_spmv_kernel<float><<<L.grid32, kBlock, 0, s>>>(nrow, L.R32, rp, ci, val32, p, Ap, st,
+                                                          part32);
+      cg_alpha_final<float><<<1, kFinalBlock, 0, s>>>(st, part32, L.grid32);
+      cg_update_kernel<float><<<L.grid2, kBlock, 0, s>>>(nrow, invd32, p, Ap, c, r, z, st,
+                                                         part32);
+      cg_beta_final<float><<<1, kFinalBlock, 0, s>>>(st, part32, L.grid2);
+      cg_p_kernel<float><<<g3, kBlock, 0, s>>>(nrow, z, p, st);
+    };
+    rc = run_polled(ctx, inner_maxit, &st->status, iteration);
+    if (rc) return rc;
+    mp_correct_kernel<<<grid_for(nrow, 4), kBlock, 0, s>>>(nrow, c, x);
+    HFLMP_LAUNCH_CHECK(ctx);
+    CgState<float> h;
+    HFLMP_HIP_CHECK(ctx, hipMemcpyAsync(&h, st, sizeof(h), hipMemcpyDeviceToHost, s));
+    HFLMP_HIP_CHECK(ctx, hipStreamSynchronize(s));
+    inner_total += h.iter;
+  }
+  MpState hm;
+  HFLMP_HIP_CHECK(ctx, hipMemcpyAsync(&hm, ms, sizeof(hm), hipMemcpyDeviceToHost, s));
+  HFLMP_HIP_CHECK(ctx, hipStreamSynchronize(s));
+  const auto t1 = std::chrono::steady_clock::now();
+  info->outer_iters = outer;
+  info->inner_iters = inner_total;
+  info->status = hm.status == kIterate ? kMaxIts : hm.status;
+  info->res_init = hm.res0;
+  info->res_final = hm.res;
+  info->seconds = std::chrono::duration<double>(t1 - t0).count();
+  return HFLMP_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -222,6 +449,20 @@ int hflmp_pcg_jacobi_f64(hflmp_ctx *ctx, int nrow, int nnz, const int *rp, const
                          size_t work_bytes, hflmp_solve_info *info) {
   return pcg_jacobi<double>(ctx, nrow, nnz, rp, ci, val, invd, b, x, maxit, atol, rtol, dtol,
                             work, work_bytes, info);
+}
+
+size_t hflmp_mpcg_workspace_bytes(int nrow, int nnz) {
+  if (nrow <= 0) return 0;
+  return mp_layout(nrow, nnz).total;
+}
+
+int hflmp_mpcg_jacobi(hflmp_ctx *ctx, int nrow, int nnz, const int *rp, const int *ci,
+                      const double *val, const float *val32, const float *invdiag32,
+                      const double *b, double *x, int max_outer, double atol, double rtol,
+                      int inner_maxit, double inner_rtol, int flags, void *work,
+                      size_t work_bytes, hflmp_mp_info *info) {
+  return mpcg_jacobi(ctx, nrow, nnz, rp, ci, val, val32, invdiag32, b, x, max_outer, atol, rtol,
+                     inner_maxit, inner_rtol, flags, work, work_bytes, info);
 }
 
 int hflmp_pcg_jacobi_f32(hflmp_ctx *ctx, int nrow, int nnz, const int *rp, const int *ci,

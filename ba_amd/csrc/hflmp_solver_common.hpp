@@ -116,10 +116,12 @@ inline SolverLayout solver_layout(int nrow, int nnz, int vb, int nvec) {
 
 // Queue `poll` iterations per chunk and keep one chunk queued ahead of the
 // one whose state the host reads.  `status` points at the device state's
-// status word; every kernel of an iteration returns at once once it is no
-// longer kIterate, so chunks queued past convergence cost only launches.
+// status word, which holds `iterate_value` while the solve continues; every
+// kernel of an iteration returns at once once it changes, so chunks queued
+// past convergence cost only launches.
 template <typename F>
-int run_polled(hflmp_ctx *ctx, int maxit, const int *status, F &&iteration, int poll = 16) {
+int run_polled(hflmp_ctx *ctx, int maxit, const int *status, F &&iteration, int poll = 16,
+               int iterate_value = kIterate) {
   hipStream_t s = ctx->stream;
   hipEvent_t ev[2];
   HFLMP_HIP_CHECK(ctx, hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
@@ -148,7 +150,7 @@ int run_polled(hflmp_ctx *ctx, int maxit, const int *status, F &&iteration, int 
     }
     // Once every iteration is queued the caller's final synchronisation
     // waits for the rest (the state cannot stay "iterate" past maxit).
-    if (slot[cur][0] != kIterate || launched >= maxit) break;
+    if (slot[cur][0] != iterate_value || launched >= maxit) break;
     ++chunk;
   }
   (void)hipEventDestroy(ev[0]);

@@ -12,19 +12,24 @@
 // the result is bit-identical to that CPU loop.
 //
 // Tiles are sized on the host from the mean row length so that a tile's
-// products normally fit one 16 KiB LDS window, which keeps eight
-// workgroups (all 32 wave slots) resident per CU.  That case runs a fully
-// unrolled single-window path.  A tile whose range is longer than the
+// products normally fit one 2048-entry LDS window (16 KiB in FP64, 8 KiB in
+// FP32), which keeps eight workgroups (all 32 wave slots) resident per CU
+// with eight loads of col and val in flight per thread.  That case runs a
+// fully unrolled single-window path.  A tile whose range is longer than the
 // window (long rows) is processed in consecutive windows; each row keeps
 // its running sum across windows, so the order is unchanged.
+//
+// Column indices are int32, or int16 offsets from the tile's first row
+// (IT = short, col = r0 + offset) for matrices whose band fits; the FP32
+// inner solve of the mixed-precision CG uses the latter.
 #pragma once
 
 #include "hflmp_internal.hpp"
 
-// Products staged per window: 16 KiB of LDS per workgroup.
+// Products staged per window.
 template <typename T>
 struct TileCap {
-  static constexpr int value = 16384 / sizeof(T);
+  static constexpr int value = 2048;
 };
 
 // Rows per tile: the mean row length times 1.05 must fit one window.
@@ -62,14 +67,15 @@ __device__ __forceinline__ T lds_seq_sum(const T *__restrict__ prod, int lo, int
 }
 
 // Returns s + sum_j val[j] * x[col[j]] over this thread's row of the tile.
-// SKIP_DIAG drops j with col[j] == row (y = (A - D) x).
+// SKIP_DIAG drops j with col[j] == row (y = (A - D) x).  IT = short reads
+// 16-bit offsets from the tile's first row.
 // All threads of the workgroup must call this (it synchronises).
-template <typename T, bool SKIP_DIAG>
+template <typename T, bool SKIP_DIAG, typename IT = int>
 __device__ __forceinline__ T tile_row_sum(long long tile, int rows_per_tile, int nrow,
                                           const int *__restrict__ rp,
-                                          const int *__restrict__ ci,
+                                          const IT *__restrict__ ci,
                                           const T *__restrict__ val,
-                                          const T *__restrict__ x, T s, TileRow &tr,
+                                          const T *__restrict__ x_in, T s, TileRow &tr,
                                           T *__restrict__ prod) {
 #pragma clang fp contract(off)
   constexpr int CAP = TileCap<T>::value;
@@ -81,6 +87,9 @@ __device__ __forceinline__ T tile_row_sum(long long tile, int rows_per_tile, int
   if (r0l >= nrow) return s;  // uniform across the workgroup
   const int r0 = (int)r0l;
   const int r1 = min(r0 + rows_per_tile, nrow);
+  // x indexed by the stored column: absolute for int32, tile-relative for int16
+  const T *__restrict__ x = std::is_same<IT, int>::value ? x_in : x_in + r0;
+  const int cbase = std::is_same<IT, int>::value ? 0 : r0;
   const int nz0 = rp[r0];
   const int nz1 = rp[r1];
   tr.has_row = t < r1 - r0;
@@ -112,7 +121,7 @@ __device__ __forceinline__ T tile_row_sum(long long tile, int rows_per_tile, int
     if (tr.has_row) {
       if (SKIP_DIAG) {
         for (int k = lo; k < hi; ++k)
-          if (ci[nz0 + k] != tr.row) s += prod[k];
+          if (cbase + (int)ci[nz0 + k] != tr.row) s += prod[k];
       } else {
         s = lds_seq_sum(prod, lo, hi, s);
       }
@@ -127,7 +136,7 @@ __device__ __forceinline__ T tile_row_sum(long long tile, int rows_per_tile, int
   }
   for (int base = nz0; base < nz1; base += CAP) {
     const int cnt = min(CAP, nz1 - base);
-    const int *__restrict__ cb = ci + base;
+    const IT *__restrict__ cb = ci + base;
     const T *__restrict__ vb = val + base;
     for (int k0 = t; k0 < cnt; k0 += kBlock * 4) {
       int c[4];
@@ -152,7 +161,7 @@ __device__ __forceinline__ T tile_row_sum(long long tile, int rows_per_tile, int
       const int hi = min(my_hi, base + cnt) - base;
       if (SKIP_DIAG) {
         for (int k = lo; k < hi; ++k)
-          if (cb[k] != tr.row) s += prod[k];
+          if (cbase + (int)cb[k] != tr.row) s += prod[k];
       } else {
         s = lds_seq_sum(prod, lo, hi, s);
       }

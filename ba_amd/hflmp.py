@@ -36,6 +36,14 @@ class SolveInfo(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class MpInfo(C.Structure):
+    _fields_ = [("outer_iters", C.c_int), ("inner_iters", C.c_int), ("status", C.c_int),
+                ("res_init", C.c_double), ("res_final", C.c_double), ("seconds", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 def _vp(x):
     return C.c_void_p(x)
 
@@ -87,6 +95,17 @@ def hip_lib():
             g("csr_extract_invdiag").argtypes = [vp, i, vp, vp, vp, vp, C.POINTER(i)]
             g("pcg_jacobi").argtypes = [vp, i, i, vp, vp, vp, vp, vp, vp, i, d, d, d,
                                         vp, sz, C.POINTER(SolveInfo)]
+        for s in ("f64", "f32"):
+            getattr(L, "hflmp_csr_spmm_" + s).argtypes = [vp, i, i, i, i, vp, vp, vp, vp, vp]
+            getattr(L, "hflmp_batched_pcg_jacobi_" + s).argtypes = [
+                vp, i, i, i, vp, vp, vp, vp, vp, vp, i, d, d, d, vp, sz, vp, vp, vp, vp,
+                C.POINTER(d)]
+        L.hflmp_batched_pcg_workspace_bytes.restype = sz
+        L.hflmp_batched_pcg_workspace_bytes.argtypes = [vp, i, i, i]
+        L.hflmp_mpcg_workspace_bytes.restype = sz
+        L.hflmp_mpcg_workspace_bytes.argtypes = [i, i]
+        L.hflmp_mpcg_jacobi.argtypes = [vp, i, i, vp, vp, vp, vp, vp, vp, vp, i, d, d, i, d,
+                                        i, vp, sz, C.POINTER(MpInfo)]
         L.hflmp_mcsgs_create_f64.argtypes = [vp, i, i, vp, vp, vp, i, vp, vp, d,
                                              C.POINTER(vp)]
         L.hflmp_mcsgs_apply_f64.argtypes = [vp, vp, vp, vp]
@@ -347,6 +366,60 @@ def pcg_jacobi(ctx, A, invd, b, x, maxit=10000, atol=0.0, rtol=1e-8, dtol=1e6, w
         _call(ctx, "hflmp_pcg_jacobi_" + _sfx(A.dtype), A.nrow, A.nnz, A.rp.ptr, A.ci.ptr,
               A.val.ptr, invd.ptr, b.ptr, x.ptr, maxit, atol, rtol, dtol, work.ptr,
               work.nbytes, C.byref(info))
+    finally:
+        if own:
+            work.free()
+    return info.as_dict()
+
+
+def spmm(ctx, A, X, Y, nrhs):
+    """Y = A X for row-major blocks of nrhs <= 64 vectors (device arrays)."""
+    _call(ctx, "hflmp_csr_spmm_" + _sfx(A.dtype), A.nrow, A.ncol, A.nnz, nrhs, A.rp.ptr,
+          A.ci.ptr, A.val.ptr, X.ptr, Y.ptr)
+
+
+def batched_pcg_jacobi(ctx, A, invd, B, X, nrhs, maxit=10000, atol=0.0, rtol=1e-8, dtol=1e6,
+                       work=None):
+    """Lockstep Jacobi-PCG on nrhs right-hand sides (row-major device blocks
+    B, X); returns per-column iterations/status/residuals and the time."""
+    need = ctx.L.hflmp_batched_pcg_workspace_bytes(ctx.h, A.nrow, nrhs, A.dtype.itemsize)
+    own = work is None
+    if own:
+        work = ctx.empty(need, np.uint8)
+    it = np.zeros(nrhs, np.int32)
+    st = np.zeros(nrhs, np.int32)
+    r0 = np.zeros(nrhs)
+    rf = np.zeros(nrhs)
+    sec = C.c_double()
+    try:
+        _call(ctx, "hflmp_batched_pcg_jacobi_" + _sfx(A.dtype), A.nrow, A.nnz, nrhs, A.rp.ptr,
+              A.ci.ptr, A.val.ptr, invd.ptr, B.ptr, X.ptr, maxit, atol, rtol, dtol, work.ptr,
+              work.nbytes, it.ctypes.data, st.ctypes.data, r0.ctypes.data, rf.ctypes.data,
+              C.byref(sec))
+    finally:
+        if own:
+            work.free()
+    return dict(iters=it, status=st, res_init=r0, res_final=rf, seconds=sec.value)
+
+
+MP_INT32_COLS = 1
+
+
+def mpcg_jacobi(ctx, A, A32, invd32, b, x, max_outer=50, atol=0.0, rtol=1e-10,
+                inner_maxit=20000, inner_rtol=1e-5, flags=0, work=None):
+    """FP32-inner / FP64-refined Jacobi-CG on the device (hflmp_mpcg_jacobi);
+    A is the FP64 matrix, A32 its FP32 copy (A.astype_f32()), invd32 the FP32
+    inverse diagonal; x holds x0 and receives the solution.  flags
+    MP_INT32_COLS keeps int32 columns in the FP32 operator."""
+    need = ctx.L.hflmp_mpcg_workspace_bytes(A.nrow, A.nnz)
+    own = work is None
+    if own:
+        work = ctx.empty(need, np.uint8)
+    info = MpInfo()
+    try:
+        _call(ctx, "hflmp_mpcg_jacobi", A.nrow, A.nnz, A.rp.ptr, A.ci.ptr, A.val.ptr,
+              A32.val.ptr, invd32.ptr, b.ptr, x.ptr, max_outer, atol, rtol, inner_maxit,
+              inner_rtol, flags, work.ptr, work.nbytes, C.byref(info))
     finally:
         if own:
             work.free()

@@ -247,6 +247,76 @@ int hflmp_pcg_jacobi_f32(hflmp_ctx *ctx, int nrow, int nnz, const int *row_ptr,
                          int maxit, double atol, double rtol, double dtol,
                          void *work, size_t work_bytes, hflmp_solve_info *info);
 
+/* ---- several right-hand sides (no reference counterpart: the reference
+ * solves one right-hand side per CG::Solve, linear_solver/cg.h:165-246) ---
+ * Blocks of nrhs <= 64 vectors are row-major: X[i*nrhs + c] is entry i of
+ * column c.  Y = A X sums every column in ascending storage order, so
+ * column c is bit-identical to hflmp_csr_spmv of column c. */
+int hflmp_csr_spmm_f64(hflmp_ctx *ctx, int nrow, int ncol, int nnz, int nrhs,
+                       const int *row_ptr, const int *col, const double *val,
+                       const double *X, double *Y);
+int hflmp_csr_spmm_f32(hflmp_ctx *ctx, int nrow, int ncol, int nnz, int nrhs,
+                       const int *row_ptr, const int *col, const float *val,
+                       const float *X, float *Y);
+/* Jacobi-PCG on nrhs systems with the same matrix, advanced in lockstep:
+ * every column follows the single-vector iteration with its own alpha,
+ * beta and stopping test and stops updating when it converges.  Per-column
+ * outputs (host arrays of nrhs, any may be NULL): iterations, status (as
+ * hflmp_solve_info), initial and final recursive residuals; *seconds is the
+ * wall time.  Workspace: hflmp_batched_pcg_workspace_bytes(). */
+size_t hflmp_batched_pcg_workspace_bytes(hflmp_ctx *ctx, int nrow, int nrhs,
+                                         int value_bytes);
+int hflmp_batched_pcg_jacobi_f64(hflmp_ctx *ctx, int nrow, int nnz, int nrhs,
+                                 const int *row_ptr, const int *col,
+                                 const double *val, const double *invdiag,
+                                 const double *B, double *X, int maxit,
+                                 double atol, double rtol, double dtol,
+                                 void *work, size_t work_bytes, int *iters,
+                                 int *status, double *res_init,
+                                 double *res_final, double *seconds);
+int hflmp_batched_pcg_jacobi_f32(hflmp_ctx *ctx, int nrow, int nnz, int nrhs,
+                                 const int *row_ptr, const int *col,
+                                 const float *val, const float *invdiag,
+                                 const float *B, float *X, int maxit,
+                                 double atol, double rtol, double dtol,
+                                 void *work, size_t work_bytes, int *iters,
+                                 int *status, double *res_init,
+                                 double *res_final, double *seconds);
+
+/* FP32-inner / FP64-refined Jacobi-CG: the refinement of
+ * MPIR<LADescriptorCoupledD, LADescriptorCoupledS> (linear_solver/mpir.cc:
+ * 52-296) with every step on the device.  Per outer step: r = b - A x and
+ * ||r|| in FP64 (one fused SpMV kernel that also writes r32 = (float) r and
+ * starts the inner solve), then Jacobi-PCG in FP32 on val32 / invdiag32
+ * (FP32 values, vectors and reductions) from c = 0 to ||r32 - A32 c|| <
+ * inner_rtol ||r32|| or inner_maxit, then x += (double) c.  Stops when
+ * ||r|| <= atol or ||r|| < rtol ||r_0|| (r_0 the initial residual, kept for
+ * the whole run; the reference resets it every step, mpir.cc:240), or after
+ * max_outer corrections.  val32 shares row_ptr/col with val.
+ * Each correction's inner tolerance is max(inner_rtol, rtol ||r_0|| /
+ * (2 ||r||)): no solve goes further than the outer target needs.
+ * The FP32 operator reads its columns as int16 offsets from each row
+ * tile's first row (built in the workspace at the start of the call) when
+ * every offset fits, else as int32; flags HFLMP_MP_INT32_COLS forces int32.
+ * The FP64 residual always uses the int32 CSR arrays. */
+#define HFLMP_MP_INT32_COLS 1
+typedef struct {
+  int outer_iters;  /* corrections applied                                */
+  int inner_iters;  /* FP32 CG iterations over all corrections           */
+  int status;       /* 0 converged, 1 max_outer reached, 3 breakdown      */
+  double res_init;  /* ||b - A x0|| (FP64)                                 */
+  double res_final; /* ||b - A x|| (FP64, true residual)                   */
+  double seconds;
+} hflmp_mp_info;
+
+size_t hflmp_mpcg_workspace_bytes(int nrow, int nnz);
+int hflmp_mpcg_jacobi(hflmp_ctx *ctx, int nrow, int nnz, const int *row_ptr,
+                      const int *col, const double *val, const float *val32,
+                      const float *invdiag32, const double *b, double *x,
+                      int max_outer, double atol, double rtol, int inner_maxit,
+                      double inner_rtol, int flags, void *work, size_t work_bytes,
+                      hflmp_mp_info *info);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,82 @@
+"""Several right-hand sides at once against the CPU oracle run one column at
+a time: the SpMM is bit-identical to the single-vector SpMV of each column,
+and every column of the lockstep Jacobi-PCG matches the oracle's CG on that
+right-hand side (iterations within 2, solution within 1e-8)."""
+import numpy as np
+import pytest
+
+from ba_amd import hflmp
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("k", [64, 7, 1])
+def test_spmm_columns_bit_exact(ctx, k, dtype):
+    rp, ci, v = hflmp.fe_system(2, 2, 40, hflmp.fe_kappa(2, 40, 1.0, 3))
+    v = v.astype(dtype)
+    N = len(rp) - 1
+    X = np.random.default_rng(k).uniform(-1, 1, (N, k)).astype(dtype)
+    A = hflmp.CsrMatrix(ctx, rp, ci, v)
+    dX, dY = ctx.upload(X.ravel()), ctx.empty(N * k, dtype)
+    hflmp.spmm(ctx, A, dX, dY, k)
+    Y = dY.download().reshape(N, k)
+    for c in range(k):
+        assert np.array_equal(Y[:, c], O.spmv(rp, ci, v, np.ascontiguousarray(X[:, c])))
+
+
+def test_spmm_long_rows(ctx):
+    # rows longer than a wave (64) take several broadcast rounds
+    from tests.helpers import random_csr
+    rng = np.random.default_rng(0)
+    lengths = rng.integers(0, 200, 300)
+    lengths[5] = 0
+    rp, ci, v = random_csr(300, 500, lengths, seed=1)
+    k = 33
+    X = rng.uniform(-1, 1, (500, k))
+    A = hflmp.CsrMatrix(ctx, rp, ci, v, ncol=500)
+    dX, dY = ctx.upload(X.ravel()), ctx.empty(300 * k, np.float64)
+    hflmp.spmm(ctx, A, dX, dY, k)
+    Y = dY.download().reshape(300, k)
+    for c in (0, 13, 32):
+        assert np.array_equal(Y[:, c], O.spmv(rp, ci, v, np.ascontiguousarray(X[:, c])))
+
+
+@pytest.mark.parametrize("k", [64, 5])
+def test_batched_pcg_vs_per_column_oracle(ctx, k):
+    order, dim, n = 2, 2, 48
+    rp, ci, v = hflmp.fe_system(order, dim, n)
+    N = len(rp) - 1
+    B = np.stack([hflmp.fe_rhs(order, dim, n, 100 + c) for c in range(k)], axis=1)
+    B[:, 3] = 0.0  # a zero right-hand side converges at iteration 0
+    A = hflmp.CsrMatrix(ctx, rp, ci, v)
+    invd = hflmp.invdiag(ctx, A)
+    dB, dX = ctx.upload(B.ravel()), ctx.zeros(N * k, np.float64)
+    info = hflmp.batched_pcg_jacobi(ctx, A, invd, dB, dX, k, rtol=1e-8, maxit=5000)
+    X = dX.download().reshape(N, k)
+    for c in range(k):
+        b = np.ascontiguousarray(B[:, c])
+        xo, io = O.cg(rp, ci, v, b, prec=O.PREC_JACOBI, rtol=1e-8, maxit=5000)
+        assert info["status"][c] == 0
+        assert abs(info["iters"][c] - io["iters"]) <= 2, (c, info["iters"][c], io["iters"])
+        nrm = np.linalg.norm(xo)
+        assert np.linalg.norm(X[:, c] - xo) <= 1e-8 * max(nrm, 1e-300) or nrm == 0
+    assert info["iters"][3] == 0
+
+
+def test_batched_pcg_fp32(ctx):
+    rp, ci, v = hflmp.fe_system(1, 2, 40)
+    N = len(rp) - 1
+    k = 16
+    B = np.stack([hflmp.fe_rhs(1, 2, 40, c) for c in range(k)], axis=1).astype(np.float32)
+    v32 = v.astype(np.float32)
+    A = hflmp.CsrMatrix(ctx, rp, ci, v32)
+    invd = hflmp.invdiag(ctx, A)
+    dB, dX = ctx.upload(B.ravel()), ctx.zeros(N * k, np.float32)
+    info = hflmp.batched_pcg_jacobi(ctx, A, invd, dB, dX, k, rtol=1e-5, maxit=5000)
+    X = dX.download().reshape(N, k)
+    for c in (0, 7, 15):
+        xo, io = O.cg(rp, ci, v32, np.ascontiguousarray(B[:, c]), prec=O.PREC_JACOBI, rtol=1e-5)
+        assert abs(info["iters"][c] - io["iters"]) <= 2
+        assert np.linalg.norm(X[:, c] - xo) <= 1e-4 * np.linalg.norm(xo)
