@@ -16,8 +16,14 @@
 // Every kernel returns at once when the state is no longer "iterate", so
 // the host queues iterations ahead and reads the state every `poll`
 // iterations only.
+//
+// The SpMV kernels take the operator as a value (CsrTileOp over the CSR
+// arrays or SellOp over the SELL-64 layout, hflmp_sell.hpp); both give each
+// thread of a workgroup one row and the same products, so the iterates do
+// not depend on the layout.
 #include <chrono>
 
+#include "hflmp_sell.hpp"
 #include "hflmp_solver_common.hpp"
 
 namespace {
@@ -25,16 +31,15 @@ namespace {
 using namespace hflmp_solver;
 
 // r = b - A x (ScaleAdd(b,-1) of A x), z = D^-1 r, p = z; partials r.r, r.z
-template <typename T>
+template <typename Op, typename T = typename Op::value_type>
 __global__ void __launch_bounds__(kBlock)
-    cg_init_kernel(int nrow, int R, const int *__restrict__ rp, const int *__restrict__ ci,
-                   const T *__restrict__ val, const T *__restrict__ invd,
-                   const T *__restrict__ b, const T *__restrict__ x, T *__restrict__ r,
-                   T *__restrict__ z, T *__restrict__ p, T *__restrict__ partials) {
-  __shared__ T prod[TileCap<T>::value];
+    cg_init_kernel(Op op, const T *__restrict__ invd, const T *__restrict__ b,
+                   const T *__restrict__ x, T *__restrict__ r, T *__restrict__ z,
+                   T *__restrict__ p, T *__restrict__ partials) {
+  __shared__ T prod[Op::kLds];
   const long long tile = xcd_tile(blockIdx.x, gridDim.x);
   TileRow tr;
-  const T s = tile_row_sum<T, false>(tile, R, nrow, rp, ci, val, x, T(0), tr, prod);
+  const T s = op.row_sum(tile, x, T(0), tr, prod);
   T rr = 0, rz = 0;
   if (tr.has_row) {
     const int i = tr.row;
@@ -67,17 +72,16 @@ __global__ void __launch_bounds__(kFinalBlock)
   }
 }
 
-// K1: Ap = A p, partials of p.Ap (IT: int32 columns or int16 tile offsets)
-template <typename T, typename IT = int>
+// K1: Ap = A p, partials of p.Ap
+template <typename Op, typename T = typename Op::value_type>
 __global__ void __launch_bounds__(kBlock)
-    cg_spmv_kernel(int nrow, int R, const int *__restrict__ rp, const IT *__restrict__ ci,
-                   const T *__restrict__ val, const T *__restrict__ p, T *__restrict__ Ap,
-                   const CgState<T> *st, T *__restrict__ partials) {
-  __shared__ T prod[TileCap<T>::value];
+    cg_spmv_kernel(Op op, const T *__restrict__ p, T *__restrict__ Ap, const CgState<T> *st,
+                   T *__restrict__ partials) {
+  __shared__ T prod[Op::kLds];
   if (st->status != kIterate) return;
   const long long tile = xcd_tile(blockIdx.x, gridDim.x);
   TileRow tr;
-  const T s = tile_row_sum<T, false, IT>(tile, R, nrow, rp, ci, val, p, T(0), tr, prod);
+  const T s = op.row_sum(tile, p, T(0), tr, prod);
   T part = 0;
   if (tr.has_row) {
     Ap[tr.row] = s;
@@ -156,16 +160,20 @@ __global__ void __launch_bounds__(kBlock)
     p[i] = beta * p[i] + z[i];
 }
 
-template <typename T>
-int pcg_jacobi(hflmp_ctx *ctx, int nrow, int nnz, const int *rp, const int *ci, const T *val,
-               const T *invd, const T *b, T *x, int maxit, double atol, double rtol,
-               double dtol, void *work, size_t work_bytes, hflmp_solve_info *info) {
-  if (!ctx) return HFLMP_EINVAL;
+// The partial-sum area of the workspace is sized for the CSR tile grid,
+// which is never smaller than the SELL grid (R <= 256 rows per tile vs 256
+// rows per SELL workgroup), so one workspace size serves both operators.
+template <typename Op, typename T = typename Op::value_type>
+int pcg_jacobi(hflmp_ctx *ctx, const Op &op, int nnz, const T *invd, const T *b, T *x,
+               int maxit, double atol, double rtol, double dtol, void *work,
+               size_t work_bytes, hflmp_solve_info *info) {
+  const int nrow = op.nrow;
   HFLMP_REQUIRE(ctx, nrow > 0 && nnz > 0, "pcg: empty system");
-  HFLMP_REQUIRE(ctx, rp && ci && val && invd && b && x && work && info, "pcg: null pointer");
+  HFLMP_REQUIRE(ctx, invd && b && x && work && info, "pcg: null pointer");
   HFLMP_REQUIRE(ctx, maxit > 0 && atol >= 0 && rtol >= 0 && dtol > 1,
                 "pcg: bad control parameters (maxit>0, atol>=0, rtol>=0, dtol>1)");
   const SolverLayout L = solver_layout(nrow, nnz, sizeof(T), 4);
+  const int grid1 = op_grid(op);
   HFLMP_REQUIRE(ctx, work_bytes >= L.total, "pcg: workspace %zu < %zu bytes", work_bytes,
                 L.total);
   char *w = static_cast<char *>(work);
@@ -181,14 +189,13 @@ int pcg_jacobi(hflmp_ctx *ctx, int nrow, int nnz, const int *rp, const int *ci, 
 
   const auto t0 = std::chrono::steady_clock::now();
   init_state_kernel<T><<<1, kBlock, 0, s>>>(st, maxit, atol, rtol, dtol);
-  cg_init_kernel<T><<<L.grid1, kBlock, 0, s>>>(nrow, L.R, rp, ci, val, invd, b, x, r, z, p,
-                                               partials);
-  cg_init_final<T><<<1, kFinalBlock, 0, s>>>(st, partials, L.grid1);
+  cg_init_kernel<Op><<<grid1, kBlock, 0, s>>>(op, invd, b, x, r, z, p, partials);
+  cg_init_final<T><<<1, kFinalBlock, 0, s>>>(st, partials, grid1);
   HFLMP_LAUNCH_CHECK(ctx);
 
   auto iteration = [&]() {
-    cg_spmv_kernel<T><<<L.grid1, kBlock, 0, s>>>(nrow, L.R, rp, ci, val, p, Ap, st, partials);
-    cg_alpha_final<T><<<1, kFinalBlock, 0, s>>>(st, partials, L.grid1);
+    cg_spmv_kernel<Op><<<grid1, kBlock, 0, s>>>(op, p, Ap, st, partials);
+    cg_alpha_final<T><<<1, kFinalBlock, 0, s>>>(st, partials, grid1);
     cg_update_kernel<T><<<g2, kBlock, 0, s>>>(nrow, invd, p, Ap, x, r, z, st, partials);
     cg_beta_final<T><<<1, kFinalBlock, 0, s>>>(st, partials, g2);
     cg_p_kernel<T><<<g3, kBlock, 0, s>>>(nrow, z, p, st);
@@ -241,16 +248,16 @@ __global__ void __launch_bounds__(kBlock)
 
 // r = b - A x in FP64; r32 = (float) r; start of the FP32 PCG from c = 0:
 // z = D32^-1 r32, p = z.  Partials: FP64 r.r; FP32 r32.r32 and r32.z.
+template <typename Op64>
 __global__ void __launch_bounds__(kBlock)
-    mp_residual_kernel(int nrow, int R, const int *__restrict__ rp, const int *__restrict__ ci,
-                       const double *__restrict__ val, const double *__restrict__ b,
-                       const double *__restrict__ x, const float *__restrict__ invd32,
-                       float *__restrict__ r32, float *__restrict__ z32, float *__restrict__ p32,
+    mp_residual_kernel(Op64 op, const double *__restrict__ b, const double *__restrict__ x,
+                       const float *__restrict__ invd32, float *__restrict__ r32,
+                       float *__restrict__ z32, float *__restrict__ p32,
                        double *__restrict__ part64, float *__restrict__ part32) {
-  __shared__ double prod[TileCap<double>::value];
+  __shared__ double prod[Op64::kLds];
   const long long tile = xcd_tile(blockIdx.x, gridDim.x);
   TileRow tr;
-  const double s = tile_row_sum<double, false>(tile, R, nrow, rp, ci, val, x, 0.0, tr, prod);
+  const double s = op.row_sum(tile, x, 0.0, tr, prod);
   double rr = 0;
   float rr32 = 0, rz32 = 0;
   if (tr.has_row) {
@@ -314,22 +321,26 @@ __global__ void __launch_bounds__(kBlock)
     x[i] += (double)c[i];
 }
 
+// Partials are sized for the larger of the grids that can run: the CSR
+// tile grids (FP64 and FP32 windows), the elementwise grid, and the SELL
+// grid (never larger than a CSR tile grid, and bounded by ceil(nrow/256)
+// rounded to 8 when there are no CSR arrays).
 struct MpLayout {
   size_t ms, st, part64, part32, vec, vstride, c16, total;
-  int R64, grid64, R32, grid32, grid2;
+  int grid2;
 };
 
-inline MpLayout mp_layout(int nrow, int nnz) {
+inline MpLayout mp_layout(int nrow, int nnz, bool csr) {
   MpLayout L;
-  const SolverLayout L64 = solver_layout(nrow, nnz, 8, 0);
-  const SolverLayout L32 = solver_layout(nrow, nnz, 4, 0);
-  L.R64 = L64.R;
-  L.grid64 = L64.grid1;
-  L.R32 = L32.R;
-  L.grid32 = L32.grid1;
-  L.grid2 = L32.grid2;
-  int gmax = L.grid64 > L.grid32 ? L.grid64 : L.grid32;
+  int gmax = sell_grid(nrow);
+  L.grid2 = solver_layout(nrow, 0, 4, 0).grid2;
   if (L.grid2 > gmax) gmax = L.grid2;
+  if (csr) {
+    const SolverLayout L64 = solver_layout(nrow, nnz, 8, 0);
+    const SolverLayout L32 = solver_layout(nrow, nnz, 4, 0);
+    if (L64.grid1 > gmax) gmax = L64.grid1;
+    if (L32.grid1 > gmax) gmax = L32.grid1;
+  }
   L.ms = 0;
   L.st = 256;
   L.part64 = 512;
@@ -337,25 +348,19 @@ inline MpLayout mp_layout(int nrow, int nnz) {
   L.vec = align256(L.part32 + 2 * (size_t)gmax * 4);
   L.vstride = align256((size_t)nrow * 4);
   L.c16 = L.vec + 5 * L.vstride;  // after r32, p32, z32, Ap32, c32
-  L.total = L.c16 + align256((size_t)nnz * 2);
+  L.total = L.c16 + (csr ? align256((size_t)nnz * 2) : 0);
   return L;
 }
 
-int mpcg_jacobi(hflmp_ctx *ctx, int nrow, int nnz, const int *rp, const int *ci,
-                const double *val, const float *val32, const float *invd32, const double *b,
-                double *x, int max_outer, double atol, double rtol, int inner_maxit,
-                double inner_rtol, int flags, void *work, size_t work_bytes,
-                hflmp_mp_info *info) {
-  if (!ctx) return HFLMP_EINVAL;
-  HFLMP_REQUIRE(ctx, nrow > 0 && nnz > 0, "mpcg: empty system");
-  HFLMP_REQUIRE(ctx, rp && ci && val && val32 && invd32 && b && x && work && info,
-                "mpcg: null pointer");
-  HFLMP_REQUIRE(ctx, max_outer > 0 && inner_maxit > 0 && atol >= 0 && rtol >= 0 &&
-                         inner_rtol > 0 && inner_rtol < 1,
-                "mpcg: bad control parameters");
-  const MpLayout L = mp_layout(nrow, nnz);
-  HFLMP_REQUIRE(ctx, work_bytes >= L.total, "mpcg: workspace %zu < %zu bytes", work_bytes,
-                L.total);
+// The refinement loop over an FP64 operator (residual) and an FP32
+// operator (inner CG); see the comment above MpState.
+template <typename Op64, typename Op32>
+int mpcg_run(hflmp_ctx *ctx, const Op64 &op64, const Op32 &op32, const MpLayout &L,
+             const float *invd32, const double *b, double *x, int max_outer, double atol,
+             double rtol, int inner_maxit, double inner_rtol, void *work,
+             hflmp_mp_info *info, std::chrono::steady_clock::time_point t0) {
+  const int nrow = op64.nrow;
+  const int grid64 = op_grid(op64), grid32 = op_grid(op32);
   char *w = static_cast<char *>(work);
   auto *ms = reinterpret_cast<MpState *>(w + L.ms);
   auto *st = reinterpret_cast<CgState<float> *>(w + L.st);
@@ -366,31 +371,15 @@ int mpcg_jacobi(hflmp_ctx *ctx, int nrow, int nnz, const int *rp, const int *ci,
   float *z = reinterpret_cast<float *>(w + L.vec + 2 * L.vstride);
   float *Ap = reinterpret_cast<float *>(w + L.vec + 3 * L.vstride);
   float *c = reinterpret_cast<float *>(w + L.vec + 4 * L.vstride);
-  short *c16 = reinterpret_cast<short *>(w + L.c16);
   hipStream_t s = ctx->stream;
   const unsigned g3 = grid_for(nrow, 4);
   int *host_status = reinterpret_cast<int *>(static_cast<char *>(ctx->host_pinned) + 2048);
-
-  const auto t0 = std::chrono::steady_clock::now();
-  // FP32 inner operator: int16 tile offsets when the band fits
-  bool use16 = !(flags & HFLMP_MP_INT32_COLS);
-  if (use16) {
-    int rc = hflmp_fill_zero(ctx, ms, sizeof(MpState));
-    if (rc) return rc;
-    tile_offsets16_kernel<<<grid_for(nrow, 1), kBlock, 0, s>>>(nrow, L.R32, rp, ci, c16,
-                                                               &ms->bad16);
-    HFLMP_LAUNCH_CHECK(ctx);
-    HFLMP_HIP_CHECK(ctx, hipMemcpyAsync(host_status, &ms->bad16, sizeof(int),
-                                        hipMemcpyDeviceToHost, s));
-    HFLMP_HIP_CHECK(ctx, hipStreamSynchronize(s));
-    use16 = host_status[0] == 0;
-  }
   int outer = 0, inner_total = 0;
   for (;; ++outer) {
     init_state_kernel<float><<<1, kBlock, 0, s>>>(st, inner_maxit, 0.0, inner_rtol, 1e30);
-    mp_residual_kernel<<<L.grid64, kBlock, 0, s>>>(nrow, L.R64, rp, ci, val, b, x, invd32, r, z,
-                                                   p, part64, part32);
-    mp_residual_final<<<1, kFinalBlock, 0, s>>>(ms, st, part64, part32, L.grid64, outer,
+    mp_residual_kernel<Op64><<<grid64, kBlock, 0, s>>>(op64, b, x, invd32, r, z, p, part64,
+                                                       part32);
+    mp_residual_final<<<1, kFinalBlock, 0, s>>>(ms, st, part64, part32, grid64, outer,
                                                 max_outer, atol, rtol, inner_rtol);
     HFLMP_LAUNCH_CHECK(ctx);
     HFLMP_HIP_CHECK(ctx, hipMemcpyAsync(host_status, &ms->status, sizeof(int),
@@ -400,13 +389,8 @@ int mpcg_jacobi(hflmp_ctx *ctx, int nrow, int nnz, const int *rp, const int *ci,
     int rc = hflmp_fill_zero(ctx, c, sizeof(float) * (size_t)nrow);
     if (rc) return rc;
     auto iteration = [&]() {
-      if (use16)
-        cg_spmv_kernel<float, short><<<L.grid32, kBlock, 0, s>>>(nrow, L.R32, rp, c16, val32, p,
-                                                                 Ap, st, part32);
-      else
-        cg_spmv_kernel<float><<<L.grid32, kBlock, 0, s>>>(nrow, L.R32, rp, ci, val32, p, Ap, st,
-                                                          part32);
-      cg_alpha_final<float><<<1, kFinalBlock, 0, s>>>(st, part32, L.grid32);
+      cg_spmv_kernel<Op32><<<grid32, kBlock, 0, s>>>(op32, p, Ap, st, part32);
+      cg_alpha_final<float><<<1, kFinalBlock, 0, s>>>(st, part32, grid32);
       cg_update_kernel<float><<<L.grid2, kBlock, 0, s>>>(nrow, invd32, p, Ap, c, r, z, st,
                                                          part32);
       cg_beta_final<float><<<1, kFinalBlock, 0, s>>>(st, part32, L.grid2);
@@ -434,6 +418,121 @@ int mpcg_jacobi(hflmp_ctx *ctx, int nrow, int nnz, const int *rp, const int *ci,
   return HFLMP_OK;
 }
 
+#define HFLMP_MP_CHECK_CONTROL(ctx)                                                     \
+  HFLMP_REQUIRE(ctx, max_outer > 0 && inner_maxit > 0 && atol >= 0 && rtol >= 0 &&      \
+                         inner_rtol > 0 && inner_rtol < 1,                              \
+                "mpcg: bad control parameters")
+
+int mpcg_jacobi(hflmp_ctx *ctx, int nrow, int nnz, const int *rp, const int *ci,
+                const double *val, const float *val32, const float *invd32, const double *b,
+                double *x, int max_outer, double atol, double rtol, int inner_maxit,
+                double inner_rtol, int flags, void *work, size_t work_bytes,
+                hflmp_mp_info *info) {
+  if (!ctx) return HFLMP_EINVAL;
+  HFLMP_REQUIRE(ctx, nrow > 0 && nnz > 0, "mpcg: empty system");
+  HFLMP_REQUIRE(ctx, rp && ci && val && val32 && invd32 && b && x && work && info,
+                "mpcg: null pointer");
+  HFLMP_MP_CHECK_CONTROL(ctx);
+  const MpLayout L = mp_layout(nrow, nnz, true);
+  HFLMP_REQUIRE(ctx, work_bytes >= L.total, "mpcg: workspace %zu < %zu bytes", work_bytes,
+                L.total);
+  char *w = static_cast<char *>(work);
+  auto *ms = reinterpret_cast<MpState *>(w + L.ms);
+  short *c16 = reinterpret_cast<short *>(w + L.c16);
+  hipStream_t s = ctx->stream;
+  int *host_status = reinterpret_cast<int *>(static_cast<char *>(ctx->host_pinned) + 2048);
+
+  const auto t0 = std::chrono::steady_clock::now();
+  const CsrTileOp<double> op64 = csr_view<double>(nrow, nnz, rp, ci, val);
+  const CsrTileOp<float> op32 = csr_view<float>(nrow, nnz, rp, ci, val32);
+  // FP32 inner operator: int16 tile offsets when the band fits
+  bool use16 = !(flags & HFLMP_MP_INT32_COLS);
+  if (use16) {
+    int rc = hflmp_fill_zero(ctx, ms, sizeof(MpState));
+    if (rc) return rc;
+    tile_offsets16_kernel<<<grid_for(nrow, 1), kBlock, 0, s>>>(nrow, op32.R, rp, ci, c16,
+                                                               &ms->bad16);
+    HFLMP_LAUNCH_CHECK(ctx);
+    HFLMP_HIP_CHECK(ctx, hipMemcpyAsync(host_status, &ms->bad16, sizeof(int),
+                                        hipMemcpyDeviceToHost, s));
+    HFLMP_HIP_CHECK(ctx, hipStreamSynchronize(s));
+    use16 = host_status[0] == 0;
+  }
+  if (use16) {
+    CsrTileOp<float, short> op16;
+    op16.nrow = nrow;
+    op16.R = op32.R;
+    op16.rp = rp;
+    op16.ci = c16;
+    op16.val = val32;
+    return mpcg_run(ctx, op64, op16, L, invd32, b, x, max_outer, atol, rtol, inner_maxit,
+                    inner_rtol, work, info, t0);
+  }
+  return mpcg_run(ctx, op64, op32, L, invd32, b, x, max_outer, atol, rtol, inner_maxit,
+                  inner_rtol, work, info, t0);
+}
+
+int mpcg_jacobi_sell(hflmp_ctx *ctx, const hflmp_sell *A64, const hflmp_sell *A32,
+                     const float *invd32, const double *b, double *x, int max_outer,
+                     double atol, double rtol, int inner_maxit, double inner_rtol, void *work,
+                     size_t work_bytes, hflmp_mp_info *info) {
+  if (!ctx) return HFLMP_EINVAL;
+  HFLMP_REQUIRE(ctx, A64 && A32 && invd32 && b && x && work && info, "mpcg: null pointer");
+  HFLMP_REQUIRE(ctx, A64->value_bytes == 8 && A32->value_bytes == 4,
+                "mpcg: A64 must hold FP64 values and A32 FP32 values");
+  HFLMP_REQUIRE(ctx, A64->nrow == A32->nrow && A64->nrow > 0 && A64->nnz > 0,
+                "mpcg: operators differ in size or are empty");
+  HFLMP_REQUIRE(ctx, A64->device == ctx->device && A32->device == ctx->device,
+                "mpcg: matrix on another device");
+  HFLMP_MP_CHECK_CONTROL(ctx);
+  const MpLayout L = mp_layout(A64->nrow, 0, false);
+  HFLMP_REQUIRE(ctx, work_bytes >= L.total, "mpcg: workspace %zu < %zu bytes", work_bytes,
+                L.total);
+  const auto t0 = std::chrono::steady_clock::now();
+  // four instantiations: int32 or int16 columns in each precision
+  if (A64->col_bytes == 2) {
+    const auto o64 = sell_view<double, short>(A64);
+    if (A32->col_bytes == 2)
+      return mpcg_run(ctx, o64, sell_view<float, short>(A32), L, invd32, b, x, max_outer,
+                        atol, rtol, inner_maxit, inner_rtol, work, info, t0);
+    return mpcg_run(ctx, o64, sell_view<float, int>(A32), L, invd32, b, x, max_outer, atol,
+                      rtol, inner_maxit, inner_rtol, work, info, t0);
+  }
+  const auto o64 = sell_view<double, int>(A64);
+  if (A32->col_bytes == 2)
+    return mpcg_run(ctx, o64, sell_view<float, short>(A32), L, invd32, b, x, max_outer, atol,
+                      rtol, inner_maxit, inner_rtol, work, info, t0);
+  return mpcg_run(ctx, o64, sell_view<float, int>(A32), L, invd32, b, x, max_outer, atol,
+                    rtol, inner_maxit, inner_rtol, work, info, t0);
+}
+
+template <typename T>
+int pcg_jacobi_sell(hflmp_ctx *ctx, const hflmp_sell *A, const T *invd, const T *b, T *x,
+                    int maxit, double atol, double rtol, double dtol, void *work,
+                    size_t work_bytes, hflmp_solve_info *info) {
+  if (!ctx) return HFLMP_EINVAL;
+  HFLMP_REQUIRE(ctx, A != nullptr, "pcg: null matrix");
+  HFLMP_REQUIRE(ctx, A->value_bytes == (int)sizeof(T), "pcg: value type mismatch");
+  HFLMP_REQUIRE(ctx, A->device == ctx->device, "pcg: matrix on another device");
+  if (A->col_bytes == 2)
+    return pcg_jacobi(ctx, sell_view<T, short>(A), A->nnz, invd, b, x, maxit, atol, rtol, dtol,
+                      work, work_bytes, info);
+  return pcg_jacobi(ctx, sell_view<T, int>(A), A->nnz, invd, b, x, maxit, atol, rtol, dtol,
+                    work, work_bytes, info);
+}
+
+template <typename T>
+int pcg_jacobi_csr(hflmp_ctx *ctx, int nrow, int nnz, const int *rp, const int *ci,
+                   const T *val, const T *invd, const T *b, T *x, int maxit, double atol,
+                   double rtol, double dtol, void *work, size_t work_bytes,
+                   hflmp_solve_info *info) {
+  if (!ctx) return HFLMP_EINVAL;
+  HFLMP_REQUIRE(ctx, rp && ci && val, "pcg: null pointer");
+  HFLMP_REQUIRE(ctx, nrow > 0 && nnz > 0, "pcg: empty system");
+  return pcg_jacobi(ctx, csr_view<T>(nrow, nnz, rp, ci, val), nnz, invd, b, x, maxit, atol,
+                    rtol, dtol, work, work_bytes, info);
+}
+
 }  // namespace
 
 extern "C" {
@@ -447,13 +546,42 @@ int hflmp_pcg_jacobi_f64(hflmp_ctx *ctx, int nrow, int nnz, const int *rp, const
                          const double *val, const double *invd, const double *b, double *x,
                          int maxit, double atol, double rtol, double dtol, void *work,
                          size_t work_bytes, hflmp_solve_info *info) {
-  return pcg_jacobi<double>(ctx, nrow, nnz, rp, ci, val, invd, b, x, maxit, atol, rtol, dtol,
-                            work, work_bytes, info);
+  return pcg_jacobi_csr<double>(ctx, nrow, nnz, rp, ci, val, invd, b, x, maxit, atol, rtol,
+                                dtol, work, work_bytes, info);
+}
+
+int hflmp_pcg_jacobi_sell_f64(hflmp_ctx *ctx, const hflmp_sell *A, const double *invd,
+                              const double *b, double *x, int maxit, double atol, double rtol,
+                              double dtol, void *work, size_t work_bytes,
+                              hflmp_solve_info *info) {
+  return pcg_jacobi_sell<double>(ctx, A, invd, b, x, maxit, atol, rtol, dtol, work, work_bytes,
+                                 info);
+}
+
+int hflmp_pcg_jacobi_sell_f32(hflmp_ctx *ctx, const hflmp_sell *A, const float *invd,
+                              const float *b, float *x, int maxit, double atol, double rtol,
+                              double dtol, void *work, size_t work_bytes,
+                              hflmp_solve_info *info) {
+  return pcg_jacobi_sell<float>(ctx, A, invd, b, x, maxit, atol, rtol, dtol, work, work_bytes,
+                                info);
 }
 
 size_t hflmp_mpcg_workspace_bytes(int nrow, int nnz) {
   if (nrow <= 0) return 0;
-  return mp_layout(nrow, nnz).total;
+  return mp_layout(nrow, nnz, true).total;
+}
+
+size_t hflmp_mpcg_sell_workspace_bytes(int nrow) {
+  if (nrow <= 0) return 0;
+  return mp_layout(nrow, 0, false).total;
+}
+
+int hflmp_mpcg_jacobi_sell(hflmp_ctx *ctx, const hflmp_sell *A64, const hflmp_sell *A32,
+                           const float *invdiag32, const double *b, double *x, int max_outer,
+                           double atol, double rtol, int inner_maxit, double inner_rtol,
+                           void *work, size_t work_bytes, hflmp_mp_info *info) {
+  return mpcg_jacobi_sell(ctx, A64, A32, invdiag32, b, x, max_outer, atol, rtol, inner_maxit,
+                          inner_rtol, work, work_bytes, info);
 }
 
 int hflmp_mpcg_jacobi(hflmp_ctx *ctx, int nrow, int nnz, const int *rp, const int *ci,
@@ -469,8 +597,8 @@ int hflmp_pcg_jacobi_f32(hflmp_ctx *ctx, int nrow, int nnz, const int *rp, const
                          const float *val, const float *invd, const float *b, float *x,
                          int maxit, double atol, double rtol, double dtol, void *work,
                          size_t work_bytes, hflmp_solve_info *info) {
-  return pcg_jacobi<float>(ctx, nrow, nnz, rp, ci, val, invd, b, x, maxit, atol, rtol, dtol,
-                           work, work_bytes, info);
+  return pcg_jacobi_csr<float>(ctx, nrow, nnz, rp, ci, val, invd, b, x, maxit, atol, rtol,
+                               dtol, work, work_bytes, info);
 }
 
 }  // extern "C"

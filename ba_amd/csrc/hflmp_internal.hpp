@@ -121,21 +121,35 @@ __device__ __forceinline__ T block_sum(T v, T *lds) {
 constexpr int kFinalBlock = 1024;
 
 // Sum of `count` strided partials src[k*stride + off] over one workgroup of
-// kFinalBlock threads; valid in thread 0.  Four independent accumulators
-// keep four loads in flight per thread; the order is fixed, so the result
-// is deterministic.  `lds` holds kFinalBlock / kWave values.
+// kFinalBlock threads; valid in thread 0.  Eight independent accumulators
+// keep eight loads in flight per thread (one round trip for up to 8192
+// partials); the order is fixed, so the result is deterministic.  `lds` holds kFinalBlock / kWave values.
 template <typename T>
 __device__ __forceinline__ T sum_partials(const T *src, int count, int stride, int off, T *lds) {
-  T a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+  constexpr int F = 8;
+  T a[F];
+#pragma unroll
+  for (int f = 0; f < F; ++f) a[f] = 0;
   int k = threadIdx.x;
-  for (; k + 3 * kFinalBlock < count; k += 4 * kFinalBlock) {
-    a0 += src[(size_t)k * stride + off];
-    a1 += src[(size_t)(k + kFinalBlock) * stride + off];
-    a2 += src[(size_t)(k + 2 * kFinalBlock) * stride + off];
-    a3 += src[(size_t)(k + 3 * kFinalBlock) * stride + off];
+  for (; k + (F - 1) * kFinalBlock < count; k += F * kFinalBlock) {
+    T v[F];
+#pragma unroll
+    for (int f = 0; f < F; ++f) v[f] = src[(size_t)(k + f * kFinalBlock) * stride + off];
+#pragma unroll
+    for (int f = 0; f < F; ++f) a[f] += v[f];
   }
-  for (; k < count; k += kFinalBlock) a0 += src[(size_t)k * stride + off];
-  T v = wave_sum((a0 + a1) + (a2 + a3));
+  // the remainder: up to F - 1 loads, all issued before any add
+  {
+    T v[F];
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+      const int kk = k + f * kFinalBlock;
+      v[f] = kk < count ? src[(size_t)kk * stride + off] : T(0);
+    }
+#pragma unroll
+    for (int f = 0; f < F; ++f) a[f] += v[f];
+  }
+  T v = wave_sum(((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7])));
   const int lane = threadIdx.x & (kWave - 1);
   const int wid = threadIdx.x / kWave;
   __syncthreads();  // a previous call may still be reading lds

@@ -1,0 +1,185 @@
+// Sliced-ELL execution layout of a CSR matrix (SELL-64), and the two
+// "operator" views the fused solver kernels are written against.
+//
+// SELL-64: rows are cut into slices of 64 consecutive rows, one wave64 per
+// slice and one lane per row.  Slice s stores its rows' entries padded to
+// the slice's longest row (rounded up to P), lane-interleaved in chunks of
+// P consecutive entries of a row:
+//     entry j of row 64 s + l  is at  off[s] + 64 P (j / P) + P l + (j mod P)
+// so each load instruction of a wave reads one contiguous 64 P-element
+// block (FP64: P = 2, 16 B per lane; FP32: P = 4, 16 B per lane), and the
+// x gathers of a step j hit x at 64 neighbouring rows' j-th columns, which
+// for an FE band is a nearly contiguous range.  Each lane adds its row's
+// products in ascending storage order starting from zero (or from y[i] for
+// y += A x), i.e. the summation order of CPUsimple_CSR_lMatrix::VectorMult
+// (CPUsimple_csr_lmatrix.cc:76-111): the result is bit-identical to the
+// sequential CPU loop.  Padding is never read: a lane stops at its row's
+// length (a byte per row).
+//
+// Columns are int32, or int16 offsets from the slice's first row when the
+// band fits (|col - 64 s| < 32768), which cuts a nonzero from 12 to 10 B
+// (FP64) or from 8 to 6 B (FP32).
+#pragma once
+
+#include "hflmp_spmv_tile.hpp"
+
+struct hflmp_sell {
+  int device;
+  int nrow, ncol, nnz, nslice;
+  int chunk;        // P
+  int col_bytes;    // 4 (absolute int32) or 2 (int16 offset from the slice's first row)
+  int value_bytes;  // 8 or 4
+  long long entries;
+  long long *soff;      // nslice + 1 entry offsets
+  int *slen;            // padded length of each slice (a multiple of P)
+  unsigned char *rlen;  // row lengths
+  void *col;
+  void *val;
+};
+
+template <typename T>
+struct SellChunk;
+template <>
+struct SellChunk<double> {
+  static constexpr int P = 2;  // entries per lane load
+  static constexpr int U = 7;  // lane loads in flight per step
+};
+template <>
+struct SellChunk<float> {
+  static constexpr int P = 4;
+  static constexpr int U = 3;
+};
+
+template <typename T, int P>
+struct VecOf {
+  typedef T type __attribute__((ext_vector_type(P)));
+};
+
+constexpr int kSellRows = 64;                         // rows per slice (one wave)
+constexpr int kSellSlicesPerBlock = kBlock / kWave;   // 4 slices per workgroup
+
+// Workgroups for an operator over SELL-64: 256 consecutive rows each,
+// rounded up to a multiple of the XCD count.
+inline int sell_grid(int nrow) {
+  const long long units = ((long long)nrow + kBlock - 1) / kBlock;
+  return (int)(((units + kXcds - 1) / kXcds) * kXcds);
+}
+
+// CSR row tiles (hflmp_spmv_tile.hpp): unit = tile of R rows.
+template <typename T, typename IT = int>
+struct CsrTileOp {
+  using value_type = T;
+  static constexpr int kLds = TileCap<T>::value;
+  int nrow, R;
+  const int *rp;
+  const IT *ci;
+  const T *val;
+
+  __device__ __forceinline__ long long thread_row(long long unit) const {
+    return threadIdx.x < (unsigned)R ? unit * R + threadIdx.x : -1;
+  }
+  __device__ __forceinline__ T row_sum(long long unit, const T *x, T s, TileRow &tr,
+                                       T *lds) const {
+    return tile_row_sum<T, false, IT>(unit, R, nrow, rp, ci, val, x, s, tr, lds);
+  }
+};
+
+// SELL-64: unit = four slices (256 consecutive rows), wave w of the
+// workgroup runs slice 4 unit + w.  No LDS and no barrier.
+template <typename T, typename IT = int>
+struct SellOp {
+  using value_type = T;
+  static constexpr int kLds = 1;
+  static constexpr int P = SellChunk<T>::P;
+  static constexpr int U = SellChunk<T>::U;
+  int nrow, nslice;
+  const long long *soff;
+  const int *slen;
+  const unsigned char *rlen;
+  const IT *col;
+  const T *val;
+
+  __device__ __forceinline__ long long thread_row(long long unit) const {
+    return unit * kBlock + threadIdx.x;
+  }
+  __device__ __forceinline__ T row_sum(long long unit, const T *x_in, T s, TileRow &tr,
+                                       T *) const {
+#pragma clang fp contract(off)
+    typedef typename VecOf<T, P>::type TV;
+    typedef typename VecOf<IT, P>::type IV;
+    tr.has_row = false;
+    tr.row = 0;
+    const long long sl = unit * kSellSlicesPerBlock + (threadIdx.x >> 6);
+    if (sl >= nslice) return s;  // uniform across the wave
+    const int sid = __builtin_amdgcn_readfirstlane((int)sl);
+    const int lane = threadIdx.x & (kWave - 1);
+    const int row = sid * kSellRows + lane;
+    tr.row = row;
+    tr.has_row = row < nrow;
+    const int len = tr.has_row ? (int)rlen[row] : 0;
+    const long long base = soff[sid];
+    const int smax = slen[sid];
+    const IV *__restrict__ c = reinterpret_cast<const IV *>(col + base) + lane;
+    const TV *__restrict__ v = reinterpret_cast<const TV *>(val + base) + lane;
+    // int16 columns are offsets from the slice's first row
+    const T *__restrict__ x = std::is_same<IT, int>::value ? x_in : x_in + sid * kSellRows;
+    for (int j = 0; j < smax; j += P * U) {
+      IV cc[U];
+      TV vv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (j + u * P < len) {
+          cc[u] = __builtin_nontemporal_load(c + (j / P + u) * kWave);
+          vv[u] = __builtin_nontemporal_load(v + (j / P + u) * kWave);
+        }
+      }
+      T xx[U][P];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int q = 0; q < P; ++q)
+          if (j + u * P + q < len) xx[u][q] = x[(int)cc[u][q]];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int q = 0; q < P; ++q)
+          if (j + u * P + q < len) s += vv[u][q] * xx[u][q];
+    }
+    return s;
+  }
+};
+
+template <typename T, typename IT>
+inline SellOp<T, IT> sell_view(const hflmp_sell *m) {
+  SellOp<T, IT> op;
+  op.nrow = m->nrow;
+  op.nslice = m->nslice;
+  op.soff = m->soff;
+  op.slen = m->slen;
+  op.rlen = m->rlen;
+  op.col = static_cast<const IT *>(m->col);
+  op.val = static_cast<const T *>(m->val);
+  return op;
+}
+
+template <typename T>
+inline CsrTileOp<T, int> csr_view(int nrow, int nnz, const int *rp, const int *ci,
+                                  const T *val) {
+  CsrTileOp<T, int> op;
+  op.nrow = nrow;
+  op.R = tile_rows(nrow, nnz, TileCap<T>::value);
+  op.rp = rp;
+  op.ci = ci;
+  op.val = val;
+  return op;
+}
+
+template <typename T, typename IT>
+inline int op_grid(const CsrTileOp<T, IT> &op) {
+  return tile_grid(((long long)op.nrow + op.R - 1) / op.R);
+}
+
+template <typename T, typename IT>
+inline int op_grid(const SellOp<T, IT> &op) {
+  return sell_grid(op.nrow);
+}

@@ -100,6 +100,19 @@ def hip_lib():
             getattr(L, "hflmp_batched_pcg_jacobi_" + s).argtypes = [
                 vp, i, i, i, vp, vp, vp, vp, vp, vp, i, d, d, d, vp, sz, vp, vp, vp, vp,
                 C.POINTER(d)]
+        for s in ("f64", "f32"):
+            getattr(L, "hflmp_sell_create_" + s).argtypes = [vp, i, i, i, vp, vp, vp, i, vp]
+            getattr(L, "hflmp_sell_set_values_" + s).argtypes = [vp, vp, vp, vp]
+            for nm in ("hflmp_sell_spmv_", "hflmp_sell_spmv_add_"):
+                getattr(L, nm + s).argtypes = [vp, vp, vp, vp]
+            getattr(L, "hflmp_pcg_jacobi_sell_" + s).argtypes = [vp, vp, vp, vp, vp, i, d, d, d,
+                                                                  vp, sz, vp]
+        L.hflmp_sell_get_info.argtypes = [vp, vp]
+        L.hflmp_sell_destroy.argtypes = [vp]
+        L.hflmp_sell_destroy.restype = None
+        L.hflmp_mpcg_sell_workspace_bytes.restype = sz
+        L.hflmp_mpcg_sell_workspace_bytes.argtypes = [i]
+        L.hflmp_mpcg_jacobi_sell.argtypes = [vp, vp, vp, vp, vp, vp, i, d, d, i, d, vp, sz, vp]
         L.hflmp_batched_pcg_workspace_bytes.restype = sz
         L.hflmp_batched_pcg_workspace_bytes.argtypes = [vp, i, i, i]
         L.hflmp_mpcg_workspace_bytes.restype = sz
@@ -197,7 +210,9 @@ class Context:
     def check(self, rc, what=""):
         if rc != HFLMP_OK:
             msg = self.L.hflmp_ctx_last_error(self.h)
-            raise HflmpError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else ""))
+            err = HflmpError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else ""))
+            err.code = rc
+            raise err
 
     def sync(self):
         self.check(self.L.hflmp_ctx_sync(self.h), "sync")
@@ -420,6 +435,87 @@ def mpcg_jacobi(ctx, A, A32, invd32, b, x, max_outer=50, atol=0.0, rtol=1e-10,
         _call(ctx, "hflmp_mpcg_jacobi", A.nrow, A.nnz, A.rp.ptr, A.ci.ptr, A.val.ptr,
               A32.val.ptr, invd32.ptr, b.ptr, x.ptr, max_outer, atol, rtol, inner_maxit,
               inner_rtol, flags, work.ptr, work.nbytes, C.byref(info))
+    finally:
+        if own:
+            work.free()
+    return info.as_dict()
+
+
+class SellInfo(C.Structure):
+    _fields_ = [("nrow", C.c_int), ("ncol", C.c_int), ("nnz", C.c_int), ("chunk", C.c_int),
+                ("col_bytes", C.c_int), ("value_bytes", C.c_int), ("entries", C.c_longlong),
+                ("device_bytes", C.c_size_t)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+SELL_COL16 = 1
+EFORMAT = -6
+
+
+class SellMatrix:
+    """SELL-64 device layout of a device CSR matrix (hflmp_sell_*), built on
+    the device; flags SELL_COL16 asks for int16 slice-offset columns."""
+
+    def __init__(self, ctx, A, flags=0):
+        self.ctx = ctx
+        self.nrow, self.ncol, self.nnz = A.nrow, A.ncol, A.nnz
+        self.dtype = A.dtype
+        self.h = C.c_void_p()
+        _call(ctx, "hflmp_sell_create_" + _sfx(A.dtype), A.nrow, A.ncol, A.nnz, A.rp.ptr,
+              A.ci.ptr, A.val.ptr, int(flags), C.byref(self.h))
+
+    def info(self):
+        inf = SellInfo()
+        rc = self.ctx.L.hflmp_sell_get_info(self.h, C.byref(inf))
+        if rc:
+            raise HflmpError("hflmp_sell_get_info failed (%d)" % rc)
+        return inf.as_dict()
+
+    def spmv(self, x, y, add=False):
+        name = "hflmp_sell_spmv_add_" if add else "hflmp_sell_spmv_"
+        _call(self.ctx, name + _sfx(self.dtype), self.h, x.ptr, y.ptr)
+
+    def set_values(self, A):
+        _call(self.ctx, "hflmp_sell_set_values_" + _sfx(self.dtype), self.h, A.rp.ptr,
+              A.val.ptr)
+
+    def free(self):
+        if self.h:
+            self.ctx.L.hflmp_sell_destroy(self.h)
+            self.h = C.c_void_p()
+
+
+def pcg_jacobi_sell(ctx, S, invd, b, x, maxit=10000, atol=0.0, rtol=1e-8, dtol=1e6,
+                    work=None):
+    """Device-resident Jacobi-PCG over a SellMatrix."""
+    need = ctx.L.hflmp_pcg_workspace_bytes(S.nrow, S.nnz, S.dtype.itemsize)
+    own = work is None
+    if own:
+        work = ctx.empty(need, np.uint8)
+    info = SolveInfo()
+    try:
+        _call(ctx, "hflmp_pcg_jacobi_sell_" + _sfx(S.dtype), S.h, invd.ptr, b.ptr, x.ptr, maxit,
+              atol, rtol, dtol, work.ptr, work.nbytes, C.byref(info))
+    finally:
+        if own:
+            work.free()
+    return info.as_dict()
+
+
+def mpcg_jacobi_sell(ctx, S64, S32, invd32, b, x, max_outer=50, atol=0.0, rtol=1e-10,
+                     inner_maxit=20000, inner_rtol=1e-5, work=None):
+    """FP32-inner / FP64-refined Jacobi-CG with both operators in SELL-64
+    layout (S64: FP64 values, S32: FP32 values)."""
+    need = ctx.L.hflmp_mpcg_sell_workspace_bytes(S64.nrow)
+    own = work is None
+    if own:
+        work = ctx.empty(need, np.uint8)
+    info = MpInfo()
+    try:
+        _call(ctx, "hflmp_mpcg_jacobi_sell", S64.h, S32.h, invd32.ptr, b.ptr, x.ptr, max_outer,
+              atol, rtol, inner_maxit, inner_rtol, work.ptr, work.nbytes, C.byref(info))
     finally:
         if own:
             work.free()

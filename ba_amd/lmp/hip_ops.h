@@ -79,6 +79,19 @@ struct HipOps;
                          T *v, T alpha) {                                                    \
       return hflmp_csr_zero_rows_##S(c, m, rows, rp, ci, v, alpha);                          \
     }                                                                                        \
+    static int sell_create(hflmp_ctx *c, int nr, int nc, int nnz, const int *rp,             \
+                           const int *ci, const T *v, int flags, hflmp_sell **m) {           \
+      return hflmp_sell_create_##S(c, nr, nc, nnz, rp, ci, v, flags, m);                     \
+    }                                                                                        \
+    static int sell_set_values(hflmp_ctx *c, hflmp_sell *m, const int *rp, const T *v) {     \
+      return hflmp_sell_set_values_##S(c, m, rp, v);                                         \
+    }                                                                                        \
+    static int sell_spmv(hflmp_ctx *c, const hflmp_sell *m, const T *x, T *y) {              \
+      return hflmp_sell_spmv_##S(c, m, x, y);                                                \
+    }                                                                                        \
+    static int sell_spmv_add(hflmp_ctx *c, const hflmp_sell *m, const T *x, T *y) {          \
+      return hflmp_sell_spmv_add_##S(c, m, x, y);                                            \
+    }                                                                                        \
   };
 
 HFLMP_OPS(double, f64)

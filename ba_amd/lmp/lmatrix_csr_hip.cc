@@ -54,7 +54,44 @@ HIP_CSR_lMatrix<T>::~HIP_CSR_lMatrix() {
 }
 
 template <typename T>
+void HIP_CSR_lMatrix<T>::drop_layout() {
+  if (sell_) hflmp_sell_destroy(sell_);
+  sell_ = nullptr;
+  sell_refused_ = false;
+  sell_stale_ = false;
+}
+
+template <typename T>
+void HIP_CSR_lMatrix<T>::set_use_sell(bool on) {
+  use_sell_ = on;
+  if (!on) drop_layout();
+}
+
+template <typename T>
+const hflmp_sell *HIP_CSR_lMatrix<T>::sell_layout() const {
+  if (!use_sell_ || sell_refused_ || this->nnz_ == 0 || this->num_row_ == 0 ||
+      this->num_col_ == 0)
+    return nullptr;
+  if (!sell_) {
+    const int rc = HipOps<T>::sell_create(hip_ctx(), this->num_row_, this->num_col_,
+                                          this->nnz_, row_, col_, val_, 0, &sell_);
+    if (rc == HFLMP_EFORMAT) {
+      sell_ = nullptr;
+      sell_refused_ = true;
+      return nullptr;
+    }
+    hip_check(rc, "SELL-64 layout");
+    sell_stale_ = false;
+  } else if (sell_stale_) {
+    hip_check(HipOps<T>::sell_set_values(hip_ctx(), sell_, row_, val_), "SELL-64 values");
+    sell_stale_ = false;
+  }
+  return sell_;
+}
+
+template <typename T>
 void HIP_CSR_lMatrix<T>::allocate(int nnz, int nrow, int ncol) {
+  drop_layout();
   dev_free(row_);
   dev_free(col_);
   dev_free(val_);
@@ -81,6 +118,7 @@ void HIP_CSR_lMatrix<T>::Init(const int init_nnz, const int init_num_row, const 
 
 template <typename T>
 void HIP_CSR_lMatrix<T>::Clear() {
+  drop_layout();
   dev_free(row_);
   dev_free(col_);
   dev_free(val_);
@@ -91,11 +129,13 @@ void HIP_CSR_lMatrix<T>::Clear() {
 
 template <typename T>
 void HIP_CSR_lMatrix<T>::Zeros() {
+  values_changed();
   if (this->nnz_) hip_check(hflmp_fill_zero(hip_ctx(), val_, sizeof(T) * (size_t)this->nnz_), "Zeros");
 }
 
 template <typename T>
 void HIP_CSR_lMatrix<T>::upload_structure() {
+  drop_layout();
   hip_check(hflmp_memcpy_h2d(hip_ctx(), row_, h_row_.data(), sizeof(int) * h_row_.size()),
             "row_ptr upload");
   if (this->nnz_)
@@ -197,6 +237,7 @@ template <typename T>
 void HIP_CSR_lMatrix<T>::scatter_values(const std::vector<int> &pos, const std::vector<T> &v,
                                         bool add) {
   if (pos.empty()) return;
+  values_changed();
   int *dp = dev_alloc<int>(pos.size(), "scatter positions");
   T *dv = dev_alloc<T>(v.size(), "scatter values");
   hip_check(hflmp_memcpy_h2d(hip_ctx(), dp, pos.data(), sizeof(int) * pos.size()), "upload");
@@ -264,6 +305,7 @@ void HIP_CSR_lMatrix<T>::ZeroRows(const int *index_set, const int size, const T 
   for (int k = 0; k < size; ++k)
     if (index_set[k] < 0 || index_set[k] >= this->num_row_)
       LMP_FATAL("HIP_CSR_lMatrix::ZeroRows: row " << index_set[k] << " out of range");
+  values_changed();
   int *d = dev_alloc<int>((size_t)size, "ZeroRows");
   hip_check(hflmp_memcpy_h2d(hip_ctx(), d, index_set, sizeof(int) * (size_t)size), "upload");
   hip_check(HipOps<T>::zero_rows(hip_ctx(), size, d, row_, col_, val_, alpha), "ZeroRows");
@@ -289,6 +331,7 @@ void HIP_CSR_lMatrix<T>::CopyFrom(const lMatrix<T> &mat) {
   if (h == this) return;
   if (h->nnz_ != this->nnz_ || h->num_row_ != this->num_row_)
     LMP_FATAL("HIP_CSR_lMatrix::CopyFrom: structure differs (call CopyStructureFrom first)");
+  values_changed();
   if (this->nnz_)
     hip_check(hflmp_memcpy_d2d(hip_ctx(), val_, h->val_, sizeof(T) * (size_t)this->nnz_),
               "CopyFrom");
@@ -349,6 +392,11 @@ void HIP_CSR_lMatrix<T>::check_vectors(const lVector<T> &in, const lVector<T> *o
 template <typename T>
 void HIP_CSR_lMatrix<T>::VectorMult(const lVector<T> &invec, lVector<T> *outvec) const {
   check_vectors(invec, outvec, "VectorMult");
+  if (const hflmp_sell *s = sell_layout()) {
+    hip_check(HipOps<T>::sell_spmv(hip_ctx(), s, invec.GetBuffer(), outvec->GetBuffer()),
+              "VectorMult");
+    return;
+  }
   hip_check(HipOps<T>::spmv(hip_ctx(), this->num_row_, this->num_col_, this->nnz_, row_, col_,
                             val_, invec.GetBuffer(), outvec->GetBuffer()),
             "VectorMult");
@@ -358,6 +406,11 @@ template <typename T>
 void HIP_CSR_lMatrix<T>::VectorMultAdd(const lVector<T> &invec, lVector<T> *outvec) const {
   check_vectors(invec, outvec, "VectorMultAdd");
   if (this->nnz_ == 0) return;  // empty off-diagonal block: nothing to add
+  if (const hflmp_sell *s = sell_layout()) {
+    hip_check(HipOps<T>::sell_spmv_add(hip_ctx(), s, invec.GetBuffer(), outvec->GetBuffer()),
+              "VectorMultAdd");
+    return;
+  }
   hip_check(HipOps<T>::spmv_add(hip_ctx(), this->num_row_, this->num_col_, this->nnz_, row_, col_,
                                 val_, invec.GetBuffer(), outvec->GetBuffer()),
             "VectorMultAdd");

@@ -3,10 +3,19 @@
 // cuda/lmatrix_csr_gpu.h:39-160).  row_ptr/col/val live on the device;
 // a host copy of the pattern serves assembly (add_values finds positions
 // on the host, then scatters on the device) and colouring.
+//
+// VectorMult and VectorMultAdd run through a SELL-64 copy of the matrix
+// (hflmp_sell_*: slices of 64 rows, one wave each, lane per row), built on
+// the device at the first product and refreshed after any value change.
+// Its products are bit-identical to the CSR kernels'; a pattern that does
+// not suit it (a row longer than 255 entries, heavy padding) stays on the
+// CSR kernels, and set_use_sell(false) forces them.  The copy costs about
+// the matrix's own device memory again.
 #pragma once
 
 #include <vector>
 
+#include "hflmp_hip.h"
 #include "lmatrix.h"
 
 namespace hiflow {
@@ -56,12 +65,19 @@ class HIP_CSR_lMatrix : public lMatrix<ValueType> {
   // device arrays
   const int *row_ptr() const { return row_; }
   const int *col_ind() const { return col_; }
-  ValueType *values() const { return val_; }
+  const ValueType *values() const { return val_; }
   // host copy of the pattern
   const std::vector<int> &host_row_ptr() const { return h_row_; }
   const std::vector<int> &host_col_ind() const { return h_col_; }
 
+  // SELL-64 execution layout for the products (on by default)
+  void set_use_sell(bool on);
+  // the layout the next product will use: nullptr means the CSR kernels
+  const hflmp_sell *sell_layout() const;
+
  private:
+  void values_changed() { sell_stale_ = true; }
+  void drop_layout();
   void allocate(int nnz, int nrow, int ncol);
   void upload_structure();
   int position(int row, int col) const;  // -1 if (row,col) is not in the pattern
@@ -73,6 +89,10 @@ class HIP_CSR_lMatrix : public lMatrix<ValueType> {
   int *col_ = nullptr;
   ValueType *val_ = nullptr;
   std::vector<int> h_row_, h_col_;
+  mutable hflmp_sell *sell_ = nullptr;
+  mutable bool sell_refused_ = false;  // the pattern does not suit SELL-64
+  mutable bool sell_stale_ = false;    // values changed since the copy was made
+  bool use_sell_ = true;
 };
 
 }  // namespace la

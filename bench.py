@@ -1,13 +1,24 @@
 #!/usr/bin/env python3
-"""Benchmark of the gfx950 lmp platform on the 3D Q1 Poisson system
-(128^3 hex cells, 129^3 = 2,146,689 DoFs, 57,066,625 nonzeros, FP64).
+"""Benchmark of the gfx950 lmp platform.
 
-A step is one FP64 CSR SpMV y = A x of that matrix with A, x, y resident in
-HBM.  `value` is the SpMV throughput in GB/s of algorithmic bytes
-(12 nnz + 20 N + 4 per SpMV: int32 col + FP64 val per nonzero, int32
-row_ptr, x read once, y written once).  The same run also times
-Jacobi-preconditioned CG to a relative residual of 1e-8 on the same system
-and the CPU restatement of the reference SpMV on the host cores.
+Headline (the timed steps): FP64 SpMV y = A x of the 3D Q1 Poisson matrix
+(128^3 hex cells, 129^3 = 2,146,689 DoFs, 57,066,625 nonzeros) with A, x, y
+resident in HBM.  A is assembled in CSR and executed through its SELL-64
+device layout (built once on the device from the CSR arrays; products
+bit-identical to the sequential CSR loop).  `value` is GB/s of CSR
+algorithmic bytes: 12 nnz + 20 N + 4 per SpMV (int32 col + FP64 val per
+nonzero, int32 row_ptr, x read once, y written once).
+
+Reported beside it, each measured in the same run:
+  * the SpMV of the CSR-array row-tile kernel and of the int16-column SELL
+    layout (compressed: same CSR bytes, reported separately);
+  * parity of both layouts against the oracle (bit-exact);
+  * Jacobi-CG to 1e-8 on the same system (CSR and SELL operators);
+  * mixed precision: log-normal diffusivity 128^3, FP32-inner / FP64-refined
+    CG vs plain FP64 CG, both to a relative residual of 1e-10;
+  * batched: 2D Q2 512^2 with 64 right-hand sides to 1e-8 vs one-at-a-time
+    CG, FP64 and FP32;
+  * the oracle's CPU SpMV on the host cores.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 128]
 """
@@ -24,6 +35,9 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
+KAPPA_SEED, RHS_SEED = 20241203, 20241204  # mixed-precision system
+BATCH_SEED0 = 1000  # right-hand sides of the batched system: seeds 1000..1063
+
 
 def spmv_bytes(N, nnz, vb=8):
     return (4 + vb) * nnz + 4 * (N + 1) + 2 * vb * N
@@ -32,6 +46,10 @@ def spmv_bytes(N, nnz, vb=8):
 def cg_iter_bytes(N, nnz, vb=8):
     # SpMV + 13 N-vector streams of the fused iteration
     return (4 + vb) * nnz + 4 * (N + 1) + 13 * vb * N
+
+
+def spmm_bytes(N, nnz, k, vb=8):
+    return (4 + vb) * nnz + 4 * (N + 1) + 2 * k * vb * N
 
 
 def host_threads():
@@ -78,15 +96,16 @@ def cpu_baseline(rp, ci, v, x, budget_s):
                        "semantics" % (nreps, el1)})
 
 
-def measured_traffic(n):
-    """HBM bytes per SpMV launch measured with rocprofv3 FETCH_SIZE/WRITE_SIZE
+def measured_traffic(n, kernel):
+    """HBM bytes per launch measured with rocprofv3 FETCH_SIZE/WRITE_SIZE
     (scaled by a 1 GiB copy), recorded in profiles/; None if not measured
-    for this size."""
+    for this size and kernel."""
     path = os.path.join(ROOT, "profiles", "spmv_128cube_traffic.json")
     if n != 128 or not os.path.exists(path):
         return None
     with open(path) as f:
-        return round(json.load(f)["traffic_bytes"])
+        rec = json.load(f).get(kernel)
+    return round(rec["traffic_bytes"]) if rec else None
 
 
 def copy_calibration(hflmp, reps):
@@ -107,6 +126,150 @@ def copy_calibration(hflmp, reps):
     x.free(); y.free(); ctx.close()
 
 
+def time_launches(ctx, fn, reps, warmup=5):
+    """Average duration of `fn` over `reps` back-to-back launches, by HIP
+    events on the context's stream (the stream the kernels run on)."""
+    for _ in range(warmup):
+        fn()
+    e0, e1 = ctx.event(), ctx.event()
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    ctx.sync()
+    return e1.elapsed_ms(e0) / 1e3 / reps
+
+
+def cg_record(info, N, nnz, vb=8):
+    it = max(info["iters"], 1)
+    from ba_amd import hflmp
+    return {"iters": info["iters"], "status": hflmp.STATUS.get(info["status"], "?"),
+            "seconds_to_tol": round(info["seconds"], 5),
+            "ms_per_iter": round(info["seconds"] * 1e3 / it, 5),
+            "GBps_per_iter": round(cg_iter_bytes(N, nnz, vb) * it / info["seconds"] / 1e9, 2),
+            "res_final_rel": info["res_final"] / info["res_init"]}
+
+
+def bench_cg(hflmp, ctx, A, S, b):
+    """Jacobi-CG to 1e-8 through both operators (second solve timed)."""
+    N, nnz = A.nrow, A.nnz
+    invd = hflmp.invdiag(ctx, A)
+    db = ctx.upload(b)
+    res = {"precond": "jacobi", "rtol": 1e-8}
+    for name, run in (("sell", lambda dx: hflmp.pcg_jacobi_sell(ctx, S, invd, db, dx,
+                                                                maxit=20000, rtol=1e-8)),
+                      ("csr", lambda dx: hflmp.pcg_jacobi(ctx, A, invd, db, dx, maxit=20000,
+                                                          rtol=1e-8))):
+        for rep in range(2):
+            dx = ctx.zeros(N, np.float64)
+            info = run(dx)
+            dx.free()
+        res[name] = cg_record(info, N, nnz)
+    db.free(); invd.free()
+    return res
+
+
+def bench_mixed(hflmp, n):
+    """C3: log-normal diffusivity (sigma = 1), 128^3 Q1, relative residual
+    1e-10: plain FP64 CG vs FP32-inner / FP64-refined CG, both through the
+    SELL-64 operators (FP32 with int16 columns)."""
+    from oracle import oracle as O
+    kap = hflmp.fe_kappa(3, n, 1.0, KAPPA_SEED)
+    rp, ci, v = hflmp.fe_system(1, 3, n, kap)
+    b = hflmp.fe_rhs(1, 3, n, RHS_SEED)
+    N, nnz = len(rp) - 1, len(ci)
+    ctx = hflmp.Context(0)
+    A = hflmp.CsrMatrix(ctx, rp, ci, v)
+    A32 = A.astype_f32()
+    invd, invd32 = hflmp.invdiag(ctx, A), hflmp.invdiag(ctx, A32)
+    S64 = hflmp.SellMatrix(ctx, A, 0)
+    S32 = hflmp.SellMatrix(ctx, A32, hflmp.SELL_COL16)
+    db = ctx.upload(b)
+    out = {"workload": "3D Q1 log-normal diffusivity (sigma 1) %d^3, Jacobi, rtol 1e-10" % n,
+           "seeds": {"kappa": KAPPA_SEED, "rhs": RHS_SEED},
+           "fp32_operator": "SELL-64, int16 columns" if S32.info()["col_bytes"] == 2
+           else "SELL-64, int32 columns"}
+    for rep in range(2):
+        dx64 = ctx.zeros(N, np.float64)
+        i64 = hflmp.pcg_jacobi_sell(ctx, S64, invd, db, dx64, maxit=50000, rtol=1e-10)
+        if rep == 0:
+            dx64.free()
+    x64 = dx64.download()
+    for rep in range(2):
+        dxm = ctx.zeros(N, np.float64)
+        im = hflmp.mpcg_jacobi_sell(ctx, S64, S32, invd32, db, dxm, rtol=1e-10,
+                                    inner_rtol=1e-4)
+        if rep == 0:
+            dxm.free()
+    xm = dxm.download()
+    bn = float(np.linalg.norm(b))
+    out["fp64_cg"] = {"iters": i64["iters"], "seconds": round(i64["seconds"], 5),
+                      "status": hflmp.STATUS.get(i64["status"], "?"),
+                      "true_rel_residual": float(np.linalg.norm(b - O.spmv(rp, ci, v, x64)))
+                      / bn}
+    out["mixed"] = {"outer": im["outer_iters"], "inner_iters": im["inner_iters"],
+                    "inner_rtol": 1e-4, "seconds": round(im["seconds"], 5),
+                    "status": hflmp.STATUS.get(im["status"], "?"),
+                    "true_rel_residual": float(np.linalg.norm(b - O.spmv(rp, ci, v, xm))) / bn,
+                    "rel_error_vs_fp64_gpu": float(np.linalg.norm(xm - x64)
+                                                   / np.linalg.norm(x64))}
+    out["speedup_mixed_vs_fp64"] = round(i64["seconds"] / im["seconds"], 3)
+    for a in (dx64, dxm, db, invd, invd32):
+        a.free()
+    S64.free(); S32.free(); A32.val.free(); A.free()
+    ctx.close()
+    return out
+
+
+def bench_batched(hflmp, k=64, n=512):
+    """C4: 2D Q2 512^2, 64 right-hand sides at once (lockstep Jacobi-PCG,
+    rtol 1e-8) vs the same right-hand sides solved one at a time; FP64 and
+    FP32 values / vectors."""
+    rp, ci, v = hflmp.fe_system(2, 2, n)
+    N, nnz = len(rp) - 1, len(ci)
+    B = np.stack([hflmp.fe_rhs(2, 2, n, BATCH_SEED0 + c) for c in range(k)], axis=1)
+    ctx = hflmp.Context(0)
+    out = {"workload": "2D Q2 Poisson %d^2, %d right-hand sides, Jacobi, rtol 1e-8" % (n, k),
+           "nrow": N, "nnz": nnz}
+    for dt, name in ((np.float64, "fp64"), (np.float32, "fp32")):
+        rtol = 1e-8 if dt == np.float64 else 1e-5
+        A = hflmp.CsrMatrix(ctx, rp, ci, v.astype(dt))
+        invd = hflmp.invdiag(ctx, A)
+        Bd = B.astype(dt)
+        dB = ctx.upload(Bd.ravel())
+        dX = ctx.zeros(N * k, dt)
+        vb = np.dtype(dt).itemsize
+        t_mm = time_launches(ctx, lambda: hflmp.spmm(ctx, A, dB, dX, k), 20)
+        ctx.check(ctx.L.hflmp_fill_zero(ctx.h, dX.ptr, dX.nbytes), "fill")
+        info = hflmp.batched_pcg_jacobi(ctx, A, invd, dB, dX, k, rtol=rtol, maxit=50000)
+        # one at a time: the same columns through the single-vector solver
+        S = hflmp.SellMatrix(ctx, A, 0)
+        singles, t_single = [], 0.0
+        ncheck = 8
+        for c in range(ncheck):
+            db = ctx.upload(np.ascontiguousarray(Bd[:, c]))
+            dx = ctx.zeros(N, dt)
+            si = hflmp.pcg_jacobi_sell(ctx, S, invd, db, dx, rtol=rtol, maxit=50000)
+            singles.append(si["iters"])
+            t_single += si["seconds"]
+            db.free(); dx.free()
+        t_single_all = t_single / ncheck * k
+        out[name] = {"rtol": rtol, "spmm_us": round(t_mm * 1e6, 2),
+                     "spmm_GBps": round(spmm_bytes(N, nnz, k, vb) / t_mm / 1e9, 2),
+                     "iters_max": int(info["iters"].max()), "iters_min": int(info["iters"].min()),
+                     "all_converged": bool((info["status"] == 0).all()),
+                     "seconds_batched": round(info["seconds"], 4),
+                     "solves_per_s_batched": round(k / info["seconds"], 3),
+                     "seconds_one_at_a_time_est": round(t_single_all, 4),
+                     "one_at_a_time_sample": "%d of the %d columns solved singly "
+                                             "(SELL operator), scaled to %d" % (ncheck, k, k),
+                     "single_iters": singles,
+                     "speedup_batched": round(t_single_all / info["seconds"], 3)}
+        S.free(); dB.free(); dX.free(); invd.free(); A.free()
+    ctx.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,6 +280,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cg", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (hflmp_copy_f64), the reference "
                          "kernel for scaling the FETCH_SIZE / WRITE_SIZE counters")
@@ -140,27 +305,32 @@ def main():
 
     ctx = hflmp.Context(0)
     A = hflmp.CsrMatrix(ctx, rp, ci, v)
+    t0 = time.perf_counter()
+    S = hflmp.SellMatrix(ctx, A, 0)
+    ctx.sync()
+    t_build = time.perf_counter() - t0
+    sinfo = S.info()
     dx = ctx.upload(x)
     dy = ctx.empty(N, np.float64)
     nb = spmv_bytes(N, nnz)
 
+    # ---- the timed steps: SELL-64 FP64 SpMV
     for _ in range(args.warmup):
-        hflmp.spmv(ctx, A, dx, dy)
+        S.spmv(dx, dy)
     ev0, ev1 = ctx.event(), ctx.event()
     ctx.sync()
     w0 = time.perf_counter()
     ev0.record()
     for _ in range(args.steps):
-        hflmp.spmv(ctx, A, dx, dy)
+        S.spmv(dx, dy)
     ev1.record()
     ctx.sync()
     wall = time.perf_counter() - w0
-    ms = ev1.elapsed_ms(ev0)
-    per_launch_s = ms / 1e3 / args.steps
+    per_launch_s = ev1.elapsed_ms(ev0) / 1e3 / args.steps
     gbs = nb / per_launch_s / 1e9
 
     out = {
-        "metric": "FP64 CSR SpMV GB/s (algorithmic bytes), 3D Q1 Poisson 128^3",
+        "metric": "FP64 SpMV GB/s (CSR algorithmic bytes), 3D Q1 Poisson 128^3",
         "value": round(gbs, 2),
         "unit": "GB/s",
         "n_gpus": 1,
@@ -172,47 +342,64 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic: structured 3D Q1 FE stiffness, Dirichlet rows, random x",
-        "config": {"workload": "3D Q1 Poisson %d^3 cells, CSR FP64 SpMV" % n, "nrow": N,
-                   "nnz": nnz, "bytes_per_spmv": nb, "parallelism": "single GPU"},
+        "config": {"workload": "3D Q1 Poisson %d^3 cells, FP64 SpMV" % n, "nrow": N,
+                   "nnz": nnz, "bytes_per_spmv": nb, "parallelism": "single GPU",
+                   "operator_layout": "SELL-64 (int32 columns, P=2) built on the device from "
+                                      "the CSR arrays; products bit-identical to the CSR loop",
+                   "layout_build_ms": round(t_build * 1e3, 2),
+                   "layout_entries": sinfo["entries"],
+                   "layout_device_bytes": sinfo["device_bytes"]},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(n)},
+                     "traffic": measured_traffic(n, "sell_spmv_f64")},
         "spmv_us": round(per_launch_s * 1e6, 2),
         "gen_seconds": round(t_gen, 3),
     }
 
+    # ---- other layouts of the same product, reported separately
+    dz = ctx.empty(N, np.float64)
+    t_csr = time_launches(ctx, lambda: hflmp.spmv(ctx, A, dx, dz), max(args.steps // 2, 1))
+    S16 = hflmp.SellMatrix(ctx, A, hflmp.SELL_COL16)
+    dw = ctx.empty(N, np.float64)
+    t_16 = time_launches(ctx, lambda: S16.spmv(dx, dw), max(args.steps // 2, 1))
+    out["spmv_csr_tiles"] = {"us": round(t_csr * 1e6, 2), "GBps": round(nb / t_csr / 1e9, 2),
+                             "frac": round(nb / t_csr / 1e9 / HBM_PEAK_GBS, 4),
+                             "traffic": measured_traffic(n, "csr_spmv_f64"),
+                             "note": "row-tile kernel reading the CSR arrays directly"}
+    out["spmv_sell_int16"] = {"us": round(t_16 * 1e6, 2), "GBps": round(nb / t_16 / 1e9, 2),
+                              "frac": round(nb / t_16 / 1e9 / HBM_PEAK_GBS, 4),
+                              "traffic": measured_traffic(n, "sell_spmv_f64_int16"),
+                              "note": "compressed (int16 column offsets, 10 B per nonzero); "
+                                      "scored with the CSR bytes"}
+
     if not args.no_parity:
         from oracle import oracle as O
-        hflmp.spmv(ctx, A, dx, dy)
-        y = dy.download()
         yref = O.spmv(rp, ci, v, x)
+        S.spmv(dx, dy)
+        hflmp.spmv(ctx, A, dx, dz)
+        S16.spmv(dx, dw)
+        y, z, w = dy.download(), dz.download(), dw.download()
         out["parity"] = {"spmv_bit_exact": bool(np.array_equal(y, yref)),
+                         "csr_tiles_bit_exact": bool(np.array_equal(z, yref)),
+                         "sell_int16_bit_exact": bool(np.array_equal(w, yref)),
                          "max_abs_diff": float(np.max(np.abs(y - yref)))}
+    S16.free(); dz.free(); dw.free()
 
     if not args.no_cg:
-        invd = hflmp.invdiag(ctx, A)
-        db, dxs = ctx.upload(b), ctx.zeros(N, np.float64)
-        info = hflmp.pcg_jacobi(ctx, A, invd, db, dxs, maxit=20000, rtol=1e-8)  # warm
-        dxs.free()
-        dxs = ctx.zeros(N, np.float64)
-        info = hflmp.pcg_jacobi(ctx, A, invd, db, dxs, maxit=20000, rtol=1e-8)
-        it = max(info["iters"], 1)
-        out["cg"] = {"precond": "jacobi", "rtol": 1e-8, "iters": info["iters"],
-                     "status": hflmp.STATUS.get(info["status"], "?"),
-                     "seconds_to_tol": round(info["seconds"], 5),
-                     "ms_per_iter": round(info["seconds"] * 1e3 / it, 5),
-                     "GBps_per_iter": round(cg_iter_bytes(N, nnz) * it / info["seconds"] / 1e9,
-                                            2),
-                     "res_final_rel": info["res_final"] / info["res_init"]}
-        dxs.free(); db.free(); invd.free()
+        out["cg"] = bench_cg(hflmp, ctx, A, S, b)
+    S.free(); A.free(); dx.free(); dy.free()
+    ctx.close()
+
+    if not args.no_mixed:
+        out["mixed_precision"] = bench_mixed(hflmp, n)
+    if not args.no_batched:
+        out["batched"] = bench_batched(hflmp)
 
     if not args.no_cpu:
         base, naive = cpu_baseline(rp, ci, v, x, args.cpu_seconds)
         out["cpu_baseline"] = base
         out["cpu_baseline_naive"] = naive
     print(json.dumps(out), flush=True)
-    A.free(); dx.free(); dy.free()
-    ctx.close()
 
 
 if __name__ == "__main__":

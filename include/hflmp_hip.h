@@ -31,6 +31,7 @@ extern "C" {
 #define HFLMP_ENOMEM -3   /* allocation failed / admission budget exceeded */
 #define HFLMP_ESTATE -4   /* object not built / wrong state                */
 #define HFLMP_ENUMERIC -5 /* zero diagonal, NaN, breakdown                 */
+#define HFLMP_EFORMAT -6  /* matrix does not suit the requested layout     */
 
 typedef struct hflmp_ctx hflmp_ctx;
 
@@ -189,6 +190,55 @@ int hflmp_csr_zero_rows_f32(hflmp_ctx *ctx, int nrows_zero, const int *rows,
 int hflmp_cast_csr_f64_to_f32(hflmp_ctx *ctx, int nnz, const double *val,
                               float *val32);
 
+/* ---- SELL-64 execution layout ---------------------------------------------
+ * A device copy of a CSR matrix in sliced ELL: slices of 64 rows (one
+ * wave64, lane = row), each padded to its longest row and interleaved in
+ * chunks of P entries per lane (P = 2 for FP64, 4 for FP32), so every
+ * lane-load instruction reads one contiguous block.  It fills the
+ * reference's ELL slot (la_global.h:56, MATRIX_FORMAT::ELL; parsed in
+ * init_vec_mat.cc:135-137) for matrices the caller assembles in CSR.
+ * Products are bit-identical to hflmp_csr_spmv_* (each row summed in
+ * ascending storage order from zero; VectorMult, CPUsimple_csr_lmatrix.cc:
+ * 76-111, and VectorMultAdd, lmatrix_csr_cpu.cc:1424-1450).
+ * create() builds on the device from the DEVICE CSR arrays (which it does
+ * not keep).  flags HFLMP_SELL_COL16 stores columns as int16 offsets from
+ * the slice's first row when every offset fits (else int32).  Returns
+ * HFLMP_EFORMAT if a row has more than 255 entries or the padding would
+ * exceed twice the nonzeros; use the CSR kernels then.  Padding is never
+ * read: each lane stops at its row's length. */
+#define HFLMP_SELL_COL16 1
+typedef struct hflmp_sell hflmp_sell;
+typedef struct {
+  int nrow, ncol, nnz;
+  int chunk;          /* P: entries per lane load                         */
+  int col_bytes;      /* 4 (int32) or 2 (int16 slice offsets)             */
+  int value_bytes;    /* 8 or 4                                           */
+  long long entries;  /* stored entries including padding                 */
+  size_t device_bytes;
+} hflmp_sell_info;
+int hflmp_sell_create_f64(hflmp_ctx *ctx, int nrow, int ncol, int nnz,
+                          const int *row_ptr, const int *col,
+                          const double *val, int flags, hflmp_sell **out);
+int hflmp_sell_create_f32(hflmp_ctx *ctx, int nrow, int ncol, int nnz,
+                          const int *row_ptr, const int *col, const float *val,
+                          int flags, hflmp_sell **out);
+/* New values for the same pattern (row_ptr, val: device CSR arrays) */
+int hflmp_sell_set_values_f64(hflmp_ctx *ctx, hflmp_sell *m,
+                              const int *row_ptr, const double *val);
+int hflmp_sell_set_values_f32(hflmp_ctx *ctx, hflmp_sell *m,
+                              const int *row_ptr, const float *val);
+int hflmp_sell_get_info(const hflmp_sell *m, hflmp_sell_info *info);
+void hflmp_sell_destroy(hflmp_sell *m);
+/* y = A x, y += A x */
+int hflmp_sell_spmv_f64(hflmp_ctx *ctx, const hflmp_sell *m, const double *x,
+                        double *y);
+int hflmp_sell_spmv_f32(hflmp_ctx *ctx, const hflmp_sell *m, const float *x,
+                        float *y);
+int hflmp_sell_spmv_add_f64(hflmp_ctx *ctx, const hflmp_sell *m,
+                            const double *x, double *y);
+int hflmp_sell_spmv_add_f32(hflmp_ctx *ctx, const hflmp_sell *m,
+                            const float *x, float *y);
+
 /* ---- preconditioners ------------------------------------------------------
  * Jacobi apply z = invdiag .* r fused into one pass
  * (lPreconditioner_Jacobi::ApplylPreconditioner, lpreconditioner.cc:184-200:
@@ -246,6 +296,20 @@ int hflmp_pcg_jacobi_f32(hflmp_ctx *ctx, int nrow, int nnz, const int *row_ptr,
                          const float *invdiag, const float *b, float *x,
                          int maxit, double atol, double rtol, double dtol,
                          void *work, size_t work_bytes, hflmp_solve_info *info);
+/* The same solve with the operator in SELL-64 layout.  The products are
+ * identical; the dot products group their per-workgroup partials over 256
+ * rows instead of one CSR tile, so iterates agree to rounding.  Workspace
+ * as hflmp_pcg_workspace_bytes(nrow, nnz, value_bytes). */
+int hflmp_pcg_jacobi_sell_f64(hflmp_ctx *ctx, const hflmp_sell *A,
+                              const double *invdiag, const double *b, double *x,
+                              int maxit, double atol, double rtol, double dtol,
+                              void *work, size_t work_bytes,
+                              hflmp_solve_info *info);
+int hflmp_pcg_jacobi_sell_f32(hflmp_ctx *ctx, const hflmp_sell *A,
+                              const float *invdiag, const float *b, float *x,
+                              int maxit, double atol, double rtol, double dtol,
+                              void *work, size_t work_bytes,
+                              hflmp_solve_info *info);
 
 /* ---- several right-hand sides (no reference counterpart: the reference
  * solves one right-hand side per CG::Solve, linear_solver/cg.h:165-246) ---
@@ -316,6 +380,16 @@ int hflmp_mpcg_jacobi(hflmp_ctx *ctx, int nrow, int nnz, const int *row_ptr,
                       int max_outer, double atol, double rtol, int inner_maxit,
                       double inner_rtol, int flags, void *work, size_t work_bytes,
                       hflmp_mp_info *info);
+/* The same refinement with both operators in SELL-64 layout: A64 (FP64
+ * values) for the residual and A32 (FP32 values, int16 columns where they
+ * fit) for the inner solve.  Workspace: hflmp_mpcg_sell_workspace_bytes. */
+size_t hflmp_mpcg_sell_workspace_bytes(int nrow);
+int hflmp_mpcg_jacobi_sell(hflmp_ctx *ctx, const hflmp_sell *A64,
+                           const hflmp_sell *A32, const float *invdiag32,
+                           const double *b, double *x, int max_outer,
+                           double atol, double rtol, int inner_maxit,
+                           double inner_rtol, void *work, size_t work_bytes,
+                           hflmp_mp_info *info);
 
 #ifdef __cplusplus
 }

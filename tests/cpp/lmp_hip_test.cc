@@ -16,6 +16,7 @@
 #include "linear_solver/gmres.h"
 #include "linear_solver/mpir.h"
 #include "linear_solver/preconditioner_bjacobi_standard.h"
+#include "lmatrix_csr_hip.h"
 #include "lmp_log.h"
 #include "platform_management.h"
 
@@ -281,6 +282,99 @@ static void test_gmres_fgmres_mpir() {
   }
 }
 
+// The SELL-64 copy behind VectorMult / VectorMultAdd against the CSR
+// kernels (set_use_sell(false)) bit for bit, through value changes that
+// must refresh the copy: ZeroRows, add_values, Zeros + set_values, CopyFrom.
+static void sell_products_equal(const HIP_CSR_lMatrix<double> &A,
+                                const HIP_CSR_lMatrix<double> &B, int n) {
+  std::unique_ptr<lVector<double>> x(init_vector<double>(n, "x", HIP, HIPKERNEL));
+  std::unique_ptr<lVector<double>> ya(init_vector<double>(n, "ya", HIP, HIPKERNEL));
+  std::unique_ptr<lVector<double>> yb(init_vector<double>(n, "yb", HIP, HIPKERNEL));
+  std::vector<double> hx(n), a(n), b(n);
+  for (int i = 0; i < n; ++i) hx[i] = std::sin(0.37 * i + 0.1) * (1.0 + (i % 7));
+  x->SetBlockValues(0, n, hx.data());
+  A.VectorMult(*x, ya.get());
+  B.VectorMult(*x, yb.get());
+  ya->GetBlockValues(0, n, a.data());
+  yb->GetBlockValues(0, n, b.data());
+  int diff = 0;
+  for (int i = 0; i < n; ++i) diff += a[i] != b[i];
+  CHECK(diff == 0);
+  A.VectorMultAdd(*x, ya.get());
+  B.VectorMultAdd(*x, yb.get());
+  ya->GetBlockValues(0, n, a.data());
+  yb->GetBlockValues(0, n, b.data());
+  diff = 0;
+  for (int i = 0; i < n; ++i) diff += a[i] != b[i];
+  CHECK(diff == 0);
+}
+
+static void test_sell_layout() {
+  // 2D 5-point pattern on a 37 x 41 grid (a partial last slice)
+  const int nx = 37, ny = 41, n = nx * ny;
+  std::vector<int> rp(1, 0), ci;
+  std::vector<double> v;
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) {
+      const int r = j * nx + i;
+      const int nb[5] = {r - nx, r - 1, r, r + 1, r + nx};
+      const bool ok[5] = {j > 0, i > 0, true, i < nx - 1, j < ny - 1};
+      for (int k = 0; k < 5; ++k)
+        if (ok[k]) {
+          ci.push_back(nb[k]);
+          v.push_back(nb[k] == r ? 4.0 + 0.01 * (r % 13) : -1.0 - 0.001 * (r % 5));
+        }
+      rp.push_back((int)ci.size());
+    }
+  const int nnz = (int)ci.size();
+  HIP_CSR_lMatrix<double> A(nnz, n, n, "A"), B(nnz, n, n, "B");
+  A.SetCSR(rp.data(), ci.data(), v.data());
+  B.SetCSR(rp.data(), ci.data(), v.data());
+  B.set_use_sell(false);
+  CHECK(B.sell_layout() == nullptr);
+  sell_products_equal(A, B, n);
+  CHECK(A.sell_layout() != nullptr);
+  const int zr[] = {0, 5, 700, n - 1};
+  A.ZeroRows(zr, 4, 2.5);
+  B.ZeroRows(zr, 4, 2.5);
+  sell_products_equal(A, B, n);
+  const int ar[] = {100, 101}, ac[] = {100, 101};
+  const double av[] = {0.5, -0.25, 0.125, 3.0};
+  A.add_values(ar, 2, ac, 2, av);
+  B.add_values(ar, 2, ac, 2, av);
+  sell_products_equal(A, B, n);
+  A.Zeros();
+  B.Zeros();
+  const int sr[] = {7}, sc[] = {7};
+  const double sv[] = {9.0};
+  A.set_values(sr, 1, sc, 1, sv);
+  B.set_values(sr, 1, sc, 1, sv);
+  sell_products_equal(A, B, n);
+  HIP_CSR_lMatrix<double> C(nnz, n, n, "C");
+  C.SetCSR(rp.data(), ci.data(), v.data());
+  sell_products_equal(C, C, n);  // builds C's copy
+  C.CopyFrom(B);                  // B's values (CSR kernels) into C
+  sell_products_equal(C, B, n);
+  // a row of 300 entries: the layout is refused and the CSR kernels run
+  const int m = 400;
+  std::vector<int> rp2(1, 0), ci2;
+  std::vector<double> v2;
+  for (int r = 0; r < m; ++r) {
+    if (r == 3)
+      for (int c = 0; c < 300; ++c) ci2.push_back(c);
+    else
+      ci2.push_back(r);
+    rp2.push_back((int)ci2.size());
+  }
+  for (size_t k = 0; k < ci2.size(); ++k) v2.push_back(1.0 + 0.5 * (double)(k % 11));
+  HIP_CSR_lMatrix<double> D((int)ci2.size(), m, m, "D"), E((int)ci2.size(), m, m, "E");
+  D.SetCSR(rp2.data(), ci2.data(), v2.data());
+  E.SetCSR(rp2.data(), ci2.data(), v2.data());
+  E.set_use_sell(false);
+  sell_products_equal(D, E, m);
+  CHECK(D.sell_layout() == nullptr);
+}
+
 static void test_factory_errors() {
   set_fatal_handler([](const std::string &) {});
   bool threw = false;
@@ -314,6 +408,7 @@ int main() {
   run("parallel_vector_test dot/axpy/scale/scale_add", test_vector_ops);
   run("parallel_matrix_test vector_mult_diag + diagonalize_rows", test_matrix_tridiagonal);
   run("GMRES / FGMRES(MC-SGS) / MPIR(FP32 CG)", test_gmres_fgmres_mpir);
+  run("SELL-64 products vs CSR kernels through value changes", test_sell_layout);
   run("factory and pattern errors", test_factory_errors);
   stop_platform(sys);
   std::printf("%s: %d failure(s)\n", g_fail ? "FAILED" : "PASSED", g_fail);
