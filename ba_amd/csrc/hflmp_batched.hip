@@ -73,7 +73,20 @@ __device__ __forceinline__ T spmm_row(long long i, int k, const int *__restrict_
       my_c = ld_stream(ci + base + lane);
       my_v = ld_stream(val + base + lane);
     }
-    for (int e = 0; e < cnt; ++e) {
+    // eight X-row gathers in flight, then their products added in order
+    int e = 0;
+    for (; e + 8 <= cnt; e += 8) {
+      T xv[8], vv[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int c = bcast(my_c, e + u);
+        vv[u] = bcast(my_v, e + u);
+        xv[u] = lane < k ? X[(size_t)c * k + lane] : T(0);
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) s += vv[u] * xv[u];
+    }
+    for (; e < cnt; ++e) {
       const int c = bcast(my_c, e);
       const T v = bcast(my_v, e);
       if (lane < k) s += v * X[(size_t)c * k + lane];
@@ -105,11 +118,13 @@ struct BState {
   T rz[kMaxRhs], alpha[kMaxRhs], beta[kMaxRhs], res[kMaxRhs], res0[kMaxRhs];
   int iter[kMaxRhs], status[kMaxRhs];
   int any_iterate;  // polled by the host: 1 while some column iterates, else 0
+  int n_live;       // columns still iterating; the one that stops the last clears any_iterate
   int maxit;
   double atol, rtol, dtol;
 };
 
-// per-column partial sums of one or two quantities, [block][k] each
+// per-column partial sums of one or two quantities, stored [q][column][block]
+// so that each column's partials are contiguous for its finalize workgroup
 template <typename T, int NQ>
 __device__ __forceinline__ void block_col_partials(const T (&a)[NQ], int k, T *__restrict__ part) {
   __shared__ T lds[NQ][kWavesPerBlock][kWave];
@@ -123,31 +138,40 @@ __device__ __forceinline__ void block_col_partials(const T (&a)[NQ], int k, T *_
       T s = 0;
 #pragma unroll
       for (int w = 0; w < kWavesPerBlock; ++w) s += lds[q][w][lane];
-      part[((size_t)q * gridDim.x + blockIdx.x) * k + lane] = s;
+      part[((size_t)q * k + lane) * gridDim.x + blockIdx.x] = s;
     }
   }
 }
 
-// one workgroup of kFinalBlock threads: column c's total of quantity q over
-// `count` workgroups, for c < k (16 threads per column, fixed order)
+// The finalize kernels run one workgroup of kBlock threads per column
+// (blockIdx.x = column).  col_total: the column's total of quantity q over
+// `count` workgroup partials, eight loads in flight per thread, fixed order;
+// valid in thread 0.
 template <typename T>
-__device__ __forceinline__ void col_totals(const T *__restrict__ part, int count, int k, int q,
-                                           T *__restrict__ out /* lds[k] */) {
-  __shared__ T red[kFinalBlock];
-  const int c = threadIdx.x % kMaxRhs, g = threadIdx.x / kMaxRhs;  // 16 groups
-  T s = 0;
-  if (c < k)
-    for (int b = g; b < count; b += kFinalBlock / kMaxRhs)
-      s += part[((size_t)q * count + b) * k + c];
-  __syncthreads();
-  red[threadIdx.x] = s;
-  __syncthreads();
-  if (threadIdx.x < kMaxRhs && c < k) {
-    T t = 0;
-    for (int gg = 0; gg < kFinalBlock / kMaxRhs; ++gg) t += red[gg * kMaxRhs + c];
-    out[c] = t;
+__device__ __forceinline__ T col_total(const T *__restrict__ part, int count, int k, int q) {
+  __shared__ T lds[kBlock / kWave];
+  const T *__restrict__ src = part + ((size_t)q * k + blockIdx.x) * count;
+  T a[8];
+#pragma unroll
+  for (int f = 0; f < 8; ++f) a[f] = 0;
+  for (int b0 = threadIdx.x; b0 < count; b0 += 8 * kBlock) {
+    T v[8];
+#pragma unroll
+    for (int f = 0; f < 8; ++f) {
+      const int b = b0 + f * kBlock;
+      v[f] = b < count ? src[b] : T(0);
+    }
+#pragma unroll
+    for (int f = 0; f < 8; ++f) a[f] += v[f];
   }
-  __syncthreads();
+  const T t = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+  return block_sum(t, lds);
+}
+
+// A column leaves the iteration: the last one to leave clears any_iterate.
+template <typename T>
+__device__ __forceinline__ void column_stops(BState<T> *st) {
+  if (atomicSub(&st->n_live, 1) == 1) st->any_iterate = 0;
 }
 
 template <typename T>
@@ -161,6 +185,7 @@ __global__ void binit_state_kernel(BState<T> *st, int k, int maxit, double atol,
   }
   if (c == 0) {
     st->any_iterate = 1;
+    st->n_live = k;
     st->maxit = maxit;
     st->atol = atol;
     st->rtol = rtol;
@@ -195,26 +220,21 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kFinalBlock)
+__global__ void __launch_bounds__(kBlock)
     bcg_init_final(BState<T> *st, const T *__restrict__ part, int count, int k) {
-  __shared__ T rr[kMaxRhs], rz[kMaxRhs];
-  col_totals(part, count, k, 0, rr);
-  col_totals(part, count, k, 1, rz);
-  if (threadIdx.x < k) {
-    const int c = threadIdx.x;
-    const T r0 = sqrt(rr[c]);
-    st->rz[c] = rz[c];
+  const T rr = col_total(part, count, k, 0);
+  const T rz = col_total(part, count, k, 1);
+  if (threadIdx.x == 0) {
+    const int c = blockIdx.x;
+    const T r0 = sqrt(rr);
+    st->rz[c] = rz;
     st->res0[c] = r0;
     st->res[c] = r0;
     st->iter[c] = 0;
-    st->status[c] = check_state(0, (double)r0, (double)r0, st->maxit, st->atol, st->rtol,
-                                st->dtol);
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int any = 0;
-    for (int c = 0; c < k; ++c) any |= st->status[c] == kIterate;
-    st->any_iterate = any;
+    const int s = check_state(0, (double)r0, (double)r0, st->maxit, st->atol, st->rtol,
+                              st->dtol);
+    st->status[c] = s;
+    if (s != kIterate) column_stops(st);
   }
 }
 
@@ -240,21 +260,17 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kFinalBlock)
+__global__ void __launch_bounds__(kBlock)
     bcg_alpha_final(BState<T> *st, const T *__restrict__ part, int count, int k) {
-  __shared__ T pap[kMaxRhs];
-  if (!st->any_iterate) return;
-  col_totals(part, count, k, 0, pap);
-  if (threadIdx.x < k) {
-    const int c = threadIdx.x;
-    if (st->status[c] == kIterate) {
-      st->alpha[c] = st->rz[c] / pap[c];
-      if (pap[c] == T(0) || !(pap[c] == pap[c])) {
-        st->status[c] = kBreakdown;
-        st->alpha[c] = T(0);
-      }
-    } else {
+  const int c = blockIdx.x;
+  if (!st->any_iterate || st->status[c] != kIterate) return;  // uniform per workgroup
+  const T pap = col_total(part, count, k, 0);
+  if (threadIdx.x == 0) {
+    st->alpha[c] = st->rz[c] / pap;
+    if (pap == T(0) || !(pap == pap)) {
+      st->status[c] = kBreakdown;
       st->alpha[c] = T(0);
+      column_stops(st);
     }
   }
 }
@@ -294,33 +310,25 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kFinalBlock)
+__global__ void __launch_bounds__(kBlock)
     bcg_beta_final(BState<T> *st, const T *__restrict__ part, int count, int k) {
-  __shared__ T rr[kMaxRhs], rz[kMaxRhs];
-  if (!st->any_iterate) return;
-  col_totals(part, count, k, 0, rr);
-  col_totals(part, count, k, 1, rz);
-  if (threadIdx.x < k) {
-    const int c = threadIdx.x;
-    if (st->status[c] == kIterate) {
-      const int it = st->iter[c] + 1;
-      const T res = sqrt(rr[c]);
-      st->iter[c] = it;
-      st->res[c] = res;
-      const int s = check_state(it, (double)res, (double)st->res0[c], st->maxit, st->atol,
-                                st->rtol, st->dtol);
-      if (s == kIterate) {
-        st->beta[c] = rz[c] / st->rz[c];
-        st->rz[c] = rz[c];
-      }
-      st->status[c] = s;
-    }
-  }
-  __syncthreads();
+  const int c = blockIdx.x;
+  if (!st->any_iterate || st->status[c] != kIterate) return;  // uniform per workgroup
+  const T rr = col_total(part, count, k, 0);
+  const T rz = col_total(part, count, k, 1);
   if (threadIdx.x == 0) {
-    int any = 0;
-    for (int c = 0; c < k; ++c) any |= st->status[c] == kIterate;
-    st->any_iterate = any;
+    const int it = st->iter[c] + 1;
+    const T res = sqrt(rr);
+    st->iter[c] = it;
+    st->res[c] = res;
+    const int s = check_state(it, (double)res, (double)st->res0[c], st->maxit, st->atol,
+                              st->rtol, st->dtol);
+    if (s == kIterate) {
+      st->beta[c] = rz / st->rz[c];
+      st->rz[c] = rz;
+    }
+    st->status[c] = s;
+    if (s != kIterate) column_stops(st);
   }
 }
 
@@ -389,13 +397,13 @@ int batched_pcg(hflmp_ctx *ctx, int nrow, int nnz, int k, const int *rp, const i
   const auto t0 = std::chrono::steady_clock::now();
   binit_state_kernel<T><<<1, kMaxRhs, 0, s>>>(st, k, maxit, atol, rtol, dtol);
   bcg_init_kernel<T><<<g, kBlock, 0, s>>>(nrow, k, rp, ci, val, invd, B, X, R, Z, P, part);
-  bcg_init_final<T><<<1, kFinalBlock, 0, s>>>(st, part, g, k);
+  bcg_init_final<T><<<k, kBlock, 0, s>>>(st, part, g, k);
   HFLMP_LAUNCH_CHECK(ctx);
   auto iteration = [&]() {
     bcg_spmm_kernel<T><<<g, kBlock, 0, s>>>(nrow, k, rp, ci, val, P, AP, st, part);
-    bcg_alpha_final<T><<<1, kFinalBlock, 0, s>>>(st, part, g, k);
+    bcg_alpha_final<T><<<k, kBlock, 0, s>>>(st, part, g, k);
     bcg_update_kernel<T><<<g, kBlock, 0, s>>>(nrow, k, invd, P, AP, X, R, Z, st, part);
-    bcg_beta_final<T><<<1, kFinalBlock, 0, s>>>(st, part, g, k);
+    bcg_beta_final<T><<<k, kBlock, 0, s>>>(st, part, g, k);
     bcg_p_kernel<T><<<g, kBlock, 0, s>>>(nrow, k, Z, P, st);
   };
   // the poll reads any_iterate: stop when it drops to 0 (kConverged)

@@ -113,6 +113,16 @@ def hip_lib():
         L.hflmp_mpcg_sell_workspace_bytes.restype = sz
         L.hflmp_mpcg_sell_workspace_bytes.argtypes = [i]
         L.hflmp_mpcg_jacobi_sell.argtypes = [vp, vp, vp, vp, vp, vp, i, d, d, i, d, vp, sz, vp]
+        L.hflmp_server_create.argtypes = [vp, vp, vp]
+        L.hflmp_server_destroy.argtypes = [vp]
+        L.hflmp_server_destroy.restype = None
+        L.hflmp_server_footprint.restype = sz
+        L.hflmp_server_footprint.argtypes = [i, i, vp]
+        L.hflmp_server_submit.argtypes = [vp, i, i, vp, vp, vp, vp, vp, d, d, i, vp]
+        L.hflmp_server_step.argtypes = [vp, vp]
+        L.hflmp_server_poll.argtypes = [vp, C.c_longlong, vp]
+        L.hflmp_server_release.argtypes = [vp, C.c_longlong]
+        L.hflmp_server_stats.argtypes = [vp, vp]
         L.hflmp_batched_pcg_workspace_bytes.restype = sz
         L.hflmp_batched_pcg_workspace_bytes.argtypes = [vp, i, i, i]
         L.hflmp_mpcg_workspace_bytes.restype = sz
@@ -520,6 +530,97 @@ def mpcg_jacobi_sell(ctx, S64, S32, invd32, b, x, max_outer=50, atol=0.0, rtol=1
         if own:
             work.free()
     return info.as_dict()
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget", C.c_size_t), ("max_active", C.c_int), ("chunk_iters", C.c_int),
+                ("use_graphs", C.c_int)]
+
+
+class ServerResult(C.Structure):
+    _fields_ = [("state", C.c_int), ("iters", C.c_int), ("status", C.c_int),
+                ("res_init", C.c_double), ("res_final", C.c_double),
+                ("zero_diagonals", C.c_int), ("footprint", C.c_size_t),
+                ("queued_seconds", C.c_double), ("latency_seconds", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("queued", C.c_int), ("active", C.c_int), ("active_units", C.c_int),
+                ("arena_free", C.c_size_t), ("arena_largest_free", C.c_size_t),
+                ("steps", C.c_longlong), ("graph_captures", C.c_longlong)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def server_footprint(rp, nnz):
+    rp = np.ascontiguousarray(rp, np.int32)
+    return hip_lib().hflmp_server_footprint(len(rp) - 1, int(nnz), rp.ctypes.data)
+
+
+class Server:
+    """Device-resident solve server (hflmp_server_*): submit host CSR
+    systems, step, collect x.  Keeps each request's host arrays alive until
+    its result is taken."""
+
+    def __init__(self, ctx, hbm_budget, max_active=64, chunk_iters=16, use_graphs=True):
+        self.ctx = ctx
+        cfg = ServerConfig(int(hbm_budget), int(max_active), int(chunk_iters),
+                           1 if use_graphs else 0)
+        self.h = C.c_void_p()
+        ctx.check(ctx.L.hflmp_server_create(ctx.h, C.byref(cfg), C.byref(self.h)),
+                  "server_create")
+        self.keep = {}
+
+    def _check(self, rc, what):
+        if rc:
+            self.ctx.check(rc, what)
+
+    def submit(self, rp, ci, v, b, rtol=1e-8, atol=0.0, maxit=100000):
+        rp = np.ascontiguousarray(rp, np.int32)
+        ci = np.ascontiguousarray(ci, np.int32)
+        v = np.ascontiguousarray(v, np.float64)
+        b = np.ascontiguousarray(b, np.float64)
+        x = np.zeros(len(rp) - 1)
+        t = C.c_longlong()
+        self._check(self.ctx.L.hflmp_server_submit(
+            self.h, len(rp) - 1, len(ci), rp.ctypes.data, ci.ctypes.data, v.ctypes.data,
+            b.ctypes.data, x.ctypes.data, float(atol), float(rtol), int(maxit), C.byref(t)),
+            "server_submit")
+        self.keep[t.value] = (rp, ci, v, b, x)
+        return t.value
+
+    def step(self):
+        n = C.c_int()
+        self._check(self.ctx.L.hflmp_server_step(self.h, C.byref(n)), "server_step")
+        return n.value
+
+    def poll(self, ticket):
+        r = ServerResult()
+        self._check(self.ctx.L.hflmp_server_poll(self.h, int(ticket), C.byref(r)), "server_poll")
+        return r.as_dict()
+
+    def take(self, ticket):
+        """x and the result record of a done ticket (then forgotten)."""
+        info = self.poll(ticket)
+        if info["state"] != 2:
+            raise HflmpError("ticket %d is not done" % ticket)
+        x = self.keep.pop(ticket)[4]
+        self._check(self.ctx.L.hflmp_server_release(self.h, int(ticket)), "server_release")
+        return x, info
+
+    def stats(self):
+        s = ServerStats()
+        self._check(self.ctx.L.hflmp_server_stats(self.h, C.byref(s)), "server_stats")
+        return s.as_dict()
+
+    def close(self):
+        if self.h:
+            self.ctx.L.hflmp_server_destroy(self.h)
+            self.h = C.c_void_p()
 
 
 class McSgs:

@@ -18,6 +18,9 @@ Reported beside it, each measured in the same run:
     CG vs plain FP64 CG, both to a relative residual of 1e-10;
   * batched: 2D Q2 512^2 with 64 right-hand sides to 1e-8 vs one-at-a-time
     CG, FP64 and FP32;
+  * serving: a stream of 2D/3D Q1 solves of uneven sizes (FP64 Jacobi-CG to
+    1e-8) through the solve server vs one at a time; solves/s and p50/p99
+    latency under Poisson arrivals;
   * the oracle's CPU SpMV on the host cores.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 128]
@@ -270,6 +273,158 @@ def bench_batched(hflmp, k=64, n=512):
     return out
 
 
+def serve_stream(nreq, seed):
+    """The serving mix: 70% 2D Q1 with n (cells per side) log-uniform in
+    [16, 1024], 30% 3D Q1 with n log-uniform in [16, 96] (n rounded), each
+    with its own log-normal diffusivity (sigma 1) and uniform RHS seed, and
+    unit-rate exponential inter-arrival gaps (scaled by 1/rate later)."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for k in range(nreq):
+        dim, lo, hi = (2, 16, 1024) if rng.uniform() < 0.7 else (3, 16, 96)
+        n = int(round(float(np.exp(rng.uniform(np.log(lo), np.log(hi))))))
+        specs.append((dim, n, seed * 100003 + 2 * k, seed * 100003 + 2 * k + 1))
+    gaps = rng.exponential(1.0, nreq)
+    return specs, gaps
+
+
+def make_request(hflmp, spec):
+    dim, n, ks, bs = spec
+    kap = hflmp.fe_kappa(dim, n, 1.0, ks)
+    rp, ci, v = hflmp.fe_system(1, dim, n, kap)
+    return rp, ci, v, hflmp.fe_rhs(1, dim, n, bs)
+
+
+def fifo_latencies(arrivals, service):
+    """Completion - arrival of a one-at-a-time server taking requests in
+    arrival order with the measured service times."""
+    t, out = 0.0, []
+    for a, s in zip(arrivals, service):
+        t = max(t, a) + s
+        out.append(t - a)
+    return np.array(out)
+
+
+def run_server(hflmp, ctx, reqs, arrivals, cfg, rtol):
+    """Replay requests into a server at their arrival times (all at t = 0
+    when arrivals is None); returns per-request (x, info, latency)."""
+    sv = hflmp.Server(ctx, cfg["hbm_budget"], cfg["max_active"], cfg["chunk_iters"],
+                      cfg["use_graphs"])
+    try:
+        n = len(reqs)
+        tickets, lat, out = [None] * n, [0.0] * n, [None] * n
+        by_ticket = {}
+        outstanding = set()
+        nxt = 0
+        t0 = time.perf_counter()
+        while nxt < n or outstanding:
+            now = time.perf_counter() - t0
+            while nxt < n and (arrivals is None or arrivals[nxt] <= now):
+                t = sv.submit(*reqs[nxt], rtol=rtol)
+                tickets[nxt] = t
+                by_ticket[t] = nxt
+                outstanding.add(t)
+                nxt += 1
+            if not outstanding:
+                time.sleep(max(0.0, min(arrivals[nxt] - now, 0.01)))
+                continue
+            if sv.step():
+                now = time.perf_counter() - t0
+                for t in list(outstanding):
+                    if sv.poll(t)["state"] == 2:
+                        k = by_ticket[t]
+                        x, info = sv.take(t)
+                        out[k] = (x, info)
+                        lat[k] = now - (0.0 if arrivals is None else arrivals[k])
+                        outstanding.discard(t)
+        makespan = time.perf_counter() - t0
+        stats = sv.stats()
+    finally:
+        sv.close()
+    return out, np.array(lat), makespan, stats
+
+
+def bench_serve(hflmp, nreq, seed, rate, ref_style):
+    """C5: a stream of FE solves (FP64 Jacobi-CG to 1e-8) through the solve
+    server vs one-at-a-time device-resident CG (and the reference's
+    host-driven CG<LAD> on this platform), throughput with every request
+    present at t = 0, latency under Poisson arrivals."""
+    from concurrent.futures import ThreadPoolExecutor
+    rtol = 1e-8
+    specs, gaps = serve_stream(nreq, seed)
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(8) as ex:
+        reqs = list(ex.map(lambda s: make_request(hflmp, s), specs))
+    t_gen = time.perf_counter() - t0
+    total_nnz = int(sum(len(r[1]) for r in reqs))
+    ctx = hflmp.Context(0)
+    # one at a time, device-resident CG (upload, SELL-64 build, Jacobi, solve, download)
+    service, singles = [], []
+    t0 = time.perf_counter()
+    for rp, ci, v, b in reqs:
+        ts = time.perf_counter()
+        A = hflmp.CsrMatrix(ctx, rp, ci, v)
+        S = hflmp.SellMatrix(ctx, A, 0)
+        invd = hflmp.invdiag(ctx, A)
+        db, dx = ctx.upload(b), ctx.zeros(A.nrow, np.float64)
+        info = hflmp.pcg_jacobi_sell(ctx, S, invd, db, dx, rtol=rtol, maxit=100000)
+        x = dx.download()
+        S.free(); A.free(); invd.free(); db.free(); dx.free()
+        service.append(time.perf_counter() - ts)
+        singles.append((x, info))
+    t_single = time.perf_counter() - t0
+    cfg = {"hbm_budget": 32 << 30, "max_active": 256, "chunk_iters": 16, "use_graphs": True}
+    out, _, makespan, stats = run_server(hflmp, ctx, reqs, None, cfg, rtol)
+    errs = [float(np.linalg.norm(x - xs) / max(np.linalg.norm(xs), 1e-300))
+            for (x, _), (xs, _) in zip(out, singles)]
+    itd = [abs(i["iters"] - s["iters"]) for (_, i), (_, s) in zip(out, singles)]
+    res = {"stream": {"requests": nreq, "seed": seed, "rtol": rtol, "precision": "fp64",
+                      "mix": "70% 2D Q1 n log-uniform [16,1024], 30% 3D Q1 n log-uniform "
+                             "[16,96]; log-normal diffusivity sigma 1; uniform RHS",
+                      "total_nnz": total_nnz, "gen_seconds": round(t_gen, 2)},
+           "server": {"config": cfg, "solves_per_s": round(nreq / makespan, 2),
+                      "makespan_s": round(makespan, 3), "steps": stats["steps"],
+                      "graph_captures": stats["graph_captures"],
+                      "all_converged": all(i["status"] == 0 for _, i in out)},
+           "one_at_a_time_device": {"solves_per_s": round(nreq / t_single, 2),
+                                    "makespan_s": round(t_single, 3),
+                                    "all_converged": all(s["status"] == 0 for _, s in singles)},
+           "server_vs_single": {"max_rel_diff_x": max(errs), "max_iter_diff": max(itd)}}
+    res["speedup_vs_one_at_a_time_device"] = round(t_single / makespan, 3)
+    if ref_style:
+        t0 = time.perf_counter()
+        ok = True
+        rservice = []
+        for rp, ci, v, b in reqs:
+            ts = time.perf_counter()
+            _, r = hflmp.lmp_solve(rp, ci, v, b, solver=0, precond=1, rtol=rtol, maxit=100000)
+            rservice.append(time.perf_counter() - ts)
+            ok = ok and r["status"] == 0
+        t_ref = time.perf_counter() - t0
+        res["one_at_a_time_reference_style"] = {
+            "solves_per_s": round(nreq / t_ref, 2), "makespan_s": round(t_ref, 3),
+            "all_converged": ok,
+            "what": "CG<LAD>::SolvePrecond with the block-Jacobi preconditioner over "
+                    "CoupledMatrix/CoupledVector on the HIP platform (host-driven, one "
+                    "synchronising dot per reduction)"}
+        res["speedup_vs_one_at_a_time_reference_style"] = round(t_ref / makespan, 3)
+    # latency under Poisson arrivals at `rate` (default: 60% of the server's capacity)
+    lam = rate if rate else 0.6 * nreq / makespan
+    arrivals = np.cumsum(gaps) / lam
+    _, lat, mk2, _ = run_server(hflmp, ctx, reqs, arrivals, cfg, rtol)
+    base = fifo_latencies(arrivals, service)
+    res["latency"] = {"rate_per_s": round(lam, 2),
+                      "server_p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
+                      "server_p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2),
+                      "server_solves_per_s": round(nreq / mk2, 2),
+                      "one_at_a_time_p50_ms": round(float(np.percentile(base, 50)) * 1e3, 2),
+                      "one_at_a_time_p99_ms": round(float(np.percentile(base, 99)) * 1e3, 2),
+                      "note": "server measured live; one-at-a-time latencies from its measured "
+                              "per-request service times in arrival order (FIFO)"}
+    ctx.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -282,6 +437,13 @@ def main():
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=200)
+    ap.add_argument("--serve-seed", type=int, default=20241205)
+    ap.add_argument("--serve-rate", type=float, default=0.0,
+                    help="Poisson arrival rate for the latency run (default: 60%% of the "
+                         "server's measured capacity)")
+    ap.add_argument("--no-serve-reference", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (hflmp_copy_f64), the reference "
                          "kernel for scaling the FETCH_SIZE / WRITE_SIZE counters")
@@ -394,6 +556,9 @@ def main():
         out["mixed_precision"] = bench_mixed(hflmp, n)
     if not args.no_batched:
         out["batched"] = bench_batched(hflmp)
+    if not args.no_serve:
+        out["serving"] = bench_serve(hflmp, args.serve_requests, args.serve_seed,
+                                     args.serve_rate, not args.no_serve_reference)
 
     if not args.no_cpu:
         base, naive = cpu_baseline(rp, ci, v, x, args.cpu_seconds)

@@ -391,6 +391,60 @@ int hflmp_mpcg_jacobi_sell(hflmp_ctx *ctx, const hflmp_sell *A64,
                            double inner_rtol, void *work, size_t work_bytes,
                            hflmp_mp_info *info);
 
+/* ---- solve server (no reference counterpart: the reference runs one
+ * CG::Solve at a time, linear_solver/cg.h:165-246) -------------------------
+ * Many FP64 Jacobi-PCG solves of different sizes advanced together on one
+ * device.  One arena of hbm_budget bytes holds the admitted systems; a
+ * submitted system waits in a first-come-first-served queue until a slot
+ * is free and its footprint (hflmp_server_footprint: CSR, SELL-64 copy,
+ * inverse diagonal, b, x and the CG vectors) fits.  hflmp_server_step
+ * admits what fits, forms new systems' initial residuals, advances every
+ * active system by chunk_iters iterations in shared launches (one launch
+ * per kernel for all systems, captured as a hipGraph per active set when
+ * use_graphs), and retires the systems that stopped: their x is copied to
+ * the caller's buffer.  Each system keeps its own scalars and partial sums
+ * in a fixed order, so its result is bit-identical alone and in any mix.
+ * Solves start from x0 = 0 and stop as IterateControl::Check
+ * (common/iterate_control.cc:80-107).  The caller's host arrays (row_ptr,
+ * col, val, b, x) must stay valid until the ticket is done.  Rows may hold
+ * 0..255 entries. */
+typedef struct hflmp_server hflmp_server;
+typedef struct {
+  size_t hbm_budget; /* device bytes for admitted systems (>= 1 MiB)       */
+  int max_active;    /* systems advanced together, 1..4096                */
+  int chunk_iters;   /* CG iterations per step, 1..256                    */
+  int use_graphs;    /* 1: replay each step's launches as a hipGraph      */
+} hflmp_server_config;
+typedef struct {
+  int state;              /* 0 queued, 1 active, 2 done                   */
+  int iters;
+  int status;             /* as hflmp_solve_info.status once done, else -1 */
+  double res_init, res_final;
+  int zero_diagonals;     /* rows whose diagonal is missing or zero        */
+  size_t footprint;       /* arena bytes while active                      */
+  double queued_seconds;  /* submit -> admission (-1 while queued)         */
+  double latency_seconds; /* submit -> x in the caller's buffer (-1 before) */
+} hflmp_server_result;
+typedef struct {
+  int queued, active, active_units;
+  size_t arena_free, arena_largest_free;
+  long long steps, graph_captures;
+} hflmp_server_stats_t;
+int hflmp_server_create(hflmp_ctx *ctx, const hflmp_server_config *cfg,
+                        hflmp_server **out);
+void hflmp_server_destroy(hflmp_server *sv);
+size_t hflmp_server_footprint(int nrow, int nnz, const int *row_ptr);
+int hflmp_server_submit(hflmp_server *sv, int nrow, int nnz, const int *row_ptr,
+                        const int *col, const double *val, const double *b,
+                        double *x, double atol, double rtol, int maxit,
+                        long long *ticket);
+int hflmp_server_step(hflmp_server *sv, int *n_done);
+int hflmp_server_poll(hflmp_server *sv, long long ticket,
+                      hflmp_server_result *res);
+/* forget a done ticket */
+int hflmp_server_release(hflmp_server *sv, long long ticket);
+int hflmp_server_stats(const hflmp_server *sv, hflmp_server_stats_t *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,141 @@
+"""The solve server: every request against the oracle's Jacobi-CG (iterations
+within 2, solution within 1e-8), each result bit-identical alone and in a
+mix (with and without graph replay), admission by HBM footprint (requests
+wait, then run, with the same results), and its edge cases."""
+import numpy as np
+import pytest
+
+from ba_amd import hflmp
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+MB = 1 << 20
+
+
+def request(dim, n, seed):
+    kap = hflmp.fe_kappa(dim, n, 1.0, seed)
+    rp, ci, v = hflmp.fe_system(1, dim, n, kap)
+    b = hflmp.fe_rhs(1, dim, n, seed + 1)
+    return rp, ci, v, b
+
+
+MIX = [(2, 8, 11), (2, 40, 12), (3, 6, 13), (2, 17, 14), (3, 11, 15), (2, 64, 16), (2, 3, 17),
+       (3, 16, 18)]
+
+
+def run_all(server, reqs, submit_every=0, rtol=1e-8):
+    """Submit the requests (all at once, or one per `submit_every` steps
+    while stepping) and step until all are done."""
+    tickets, pending = [], list(reqs)
+    steps = 0
+    while pending or any(server.poll(t)["state"] != 2 for t in tickets):
+        if pending and (submit_every == 0 or steps % submit_every == 0):
+            batch = pending if submit_every == 0 else pending[:1]
+            for r in batch:
+                tickets.append(server.submit(*r, rtol=rtol))
+            pending = [] if submit_every == 0 else pending[1:]
+        server.step()
+        steps += 1
+        assert steps < 100000
+    return [server.take(t) for t in tickets]
+
+
+def test_serve_vs_oracle(ctx):
+    reqs = [request(*m) for m in MIX]
+    sv = hflmp.Server(ctx, 256 * MB, max_active=8, chunk_iters=8)
+    try:
+        out = run_all(sv, reqs)
+    finally:
+        sv.close()
+    for (rp, ci, v, b), (x, info) in zip(reqs, out):
+        xo, io = O.cg(rp, ci, v, b, prec=O.PREC_JACOBI, rtol=1e-8, maxit=100000)
+        assert info["status"] == 0
+        assert abs(info["iters"] - io["iters"]) <= 2, (info["iters"], io["iters"])
+        assert np.linalg.norm(x - xo) <= 1e-8 * np.linalg.norm(xo)
+        assert info["latency_seconds"] >= info["queued_seconds"] >= 0
+
+
+@pytest.mark.parametrize("graphs", [True, False])
+def test_serve_alone_equals_mix(ctx, graphs):
+    reqs = [request(*m) for m in MIX]
+    probe = request(3, 12, 99)
+    sv = hflmp.Server(ctx, 256 * MB, max_active=1, chunk_iters=5, use_graphs=graphs)
+    try:
+        (x_alone, i_alone), = run_all(sv, [probe])
+    finally:
+        sv.close()
+    # the probe arrives in the middle of a staggered mix, with another chunk
+    # length and more slots
+    mixed = reqs[:4] + [probe] + reqs[4:]
+    sv = hflmp.Server(ctx, 256 * MB, max_active=6, chunk_iters=7, use_graphs=not graphs)
+    try:
+        out = run_all(sv, mixed, submit_every=3)
+        st = sv.stats()
+    finally:
+        sv.close()
+    x_mix, i_mix = out[4]
+    assert i_mix["iters"] == i_alone["iters"]
+    assert np.array_equal(x_mix, x_alone)
+    assert st["active"] == 0 and st["queued"] == 0
+
+
+def test_serve_admission_by_footprint(ctx):
+    reqs = [request(2, 100, 40 + k) for k in range(6)]
+    foot = hflmp.server_footprint(reqs[0][0], len(reqs[0][1]))
+    # room for two systems at a time
+    budget = max(2 * foot + foot // 2, MB)
+    sv = hflmp.Server(ctx, budget, max_active=16, chunk_iters=16)
+    try:
+        tickets = [sv.submit(*r) for r in reqs]
+        sv.step()
+        st = sv.stats()
+        assert st["active"] == 2 and st["queued"] == 4
+        while any(sv.poll(t)["state"] != 2 for t in tickets):
+            sv.step()
+            assert sv.stats()["active"] <= 2
+        out = [sv.take(t) for t in tickets]
+    finally:
+        sv.close()
+    ref = hflmp.Server(ctx, 64 * budget, max_active=16, chunk_iters=16)
+    try:
+        out2 = run_all(ref, reqs)
+    finally:
+        ref.close()
+    for (x, i), (x2, i2) in zip(out, out2):
+        assert np.array_equal(x, x2) and i["iters"] == i2["iters"]
+        assert i["footprint"] == foot
+    # a system larger than the whole budget is refused at submission
+    big = request(2, 300, 7)
+    small = hflmp.Server(ctx, 4 * MB, max_active=4, chunk_iters=4)
+    try:
+        with pytest.raises(hflmp.HflmpError):
+            small.submit(*big)
+    finally:
+        small.close()
+
+
+def test_serve_edge_cases(ctx):
+    rp, ci, v, b = request(2, 10, 3)
+    zero = np.zeros_like(b)
+    sv = hflmp.Server(ctx, 64 * MB, max_active=4, chunk_iters=4)
+    try:
+        (x0, i0), (x1, i1) = run_all(sv, [(rp, ci, v, zero), (rp, ci, v, b)])
+        assert i0["iters"] == 0 and i0["status"] == 0 and not x0.any()
+        assert i1["status"] == 0 and i1["iters"] > 0
+        # an iteration cap: status "max iterations"
+        t = sv.submit(rp, ci, v, b, rtol=1e-14, maxit=3)
+        while sv.poll(t)["state"] != 2:
+            sv.step()
+        x, info = sv.take(t)
+        assert info["status"] == 1 and info["iters"] == 3
+        with pytest.raises(hflmp.HflmpError):
+            sv.poll(t)  # released
+        # a row longer than 255 entries is refused
+        n = 300
+        rp2 = np.array([0, n] + list(range(n + 1, 2 * n)), np.int32)
+        ci2 = np.concatenate([np.arange(n), np.arange(1, n)]).astype(np.int32)
+        with pytest.raises(hflmp.HflmpError):
+            sv.submit(rp2, ci2, np.ones(len(ci2)), np.ones(n))
+    finally:
+        sv.close()
