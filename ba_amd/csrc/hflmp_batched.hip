@@ -103,7 +103,8 @@ __global__ void __launch_bounds__(kBlock)
   const RowWalk w = row_walk(nrow);
   for (long long i = w.first; i < w.end; i += w.step) {
     const T s = spmm_row(i, k, rp, ci, val, X);
-    if (lane < k) Y[(size_t)i * k + lane] = s;
+    // streamed past L2, which keeps the X rows the band reuses
+    if (lane < k) __builtin_nontemporal_store(s, Y + (size_t)i * k + lane);
   }
 }
 
@@ -252,7 +253,7 @@ __global__ void __launch_bounds__(kBlock)
     const T s = spmm_row(i, k, rp, ci, val, P);
     if (lane < k) {
       const size_t e = (size_t)i * k + lane;
-      AP[e] = s;
+      __builtin_nontemporal_store(s, AP + e);
       acc[0] += s * P[e];
     }
   }

@@ -1,28 +1,33 @@
-// Device-resident solve server: many independent FP64 Jacobi-PCG solves of
-// different sizes advanced together.  No reference counterpart (the
-// reference runs one CG::Solve at a time, linear_solver/cg.h:165-246); each
-// system follows that arithmetic with its own scalars and IterateControl
-// test (common/iterate_control.cc:80-107).
+// Device-resident solve server: many independent Jacobi-PCG solves of
+// different sizes advanced together, each in FP64 or in the FP32-inner /
+// FP64-refined mode.  No reference counterpart (the reference runs one
+// CG::Solve or MPIR::Solve at a time, linear_solver/cg.h:165-246,
+// linear_solver/mpir.cc:52-296); each system follows that arithmetic with its
+// own scalars and IterateControl test (common/iterate_control.cc:80-107).
 //
 // Memory.  One device arena of the HBM budget is allocated at creation.  A
 // request is admitted (first come, first served) when a slot is free and
-// its footprint fits in the arena; its CSR arrays, SELL-64 layout, inverse
-// diagonal, right-hand side and CG vectors then occupy one arena block
+// its footprint fits in the arena; its CSR arrays, SELL-64 layout(s), inverse
+// diagonal, right-hand side and Krylov vectors then occupy one arena block
 // until its result has been copied back.
 //
 // Batching.  The rows of every active system are cut into units of 256
 // consecutive rows (one workgroup, four SELL-64 slices).  One launch of each
 // iteration kernel covers the units of all active systems (unit -> system
-// through a device table), and the two scalar steps run one workgroup per
-// system.  So an iteration of S systems is five launches, as for one.  A
-// system's partial sums are indexed by its own units and added in a fixed
-// order, so its iterates do not depend on which other systems share the
-// launches: a result is bit-identical alone and in any mix.
+// through a device table); each workgroup branches on its system's precision.
+// The two scalar steps run one workgroup per system.  So an iteration of S
+// systems is five launches, as for one.  A system's partial sums are
+// indexed by its own units and added in a fixed order, so its iterates do
+// not depend on which other systems share the launches: a result is
+// bit-identical alone and in any mix.
 //
-// Steps.  hflmp_server_step admits what fits, forms the initial residuals
-// of new systems, runs `chunk` iterations (captured once per active set as
-// a hipGraph and replayed while the set is unchanged), reads the states,
-// and retires the systems that stopped (x copied to the caller's buffer).
+// Steps.  hflmp_server_step admits what fits, then runs one step: the
+// initial residuals of new FP64 systems, the FP64 outer step (x += c,
+// r = b - A x, r32 = (float) r, outer test, next inner start) of the mixed
+// systems whose inner solve has stopped, and `chunk` iterations -- captured
+// once per active set as a hipGraph and replayed while the set is unchanged.
+// It then reads the states and retires the systems that stopped (x copied
+// to the caller's buffer).
 #include <algorithm>
 #include <chrono>
 #include <cstring>
@@ -38,21 +43,45 @@ namespace {
 using namespace hflmp_solver;
 
 constexpr int kFresh = -2;  // admitted; initial residual not yet formed
+constexpr int kModeFp64 = 64, kModeMixed = 32;
+constexpr int kMaxOuter = 50;  // refinement steps of a mixed system
+
+struct SlotState {
+  CgState<double> s64;  // FP64 systems
+  CgState<float> s32;   // mixed systems: the inner FP32 CG
+  double res0, res;     // mixed: FP64 true residual, initial and latest
+  double atol, rtol, inner_rtol;
+  int mode;
+  int status;  // mixed: kFresh, kIterate or the final status
+  int phase;   // mixed: 1 while the inner CG runs, 0 when an outer step is due
+  int outer, max_outer, inner_total, inner_maxit;
+};
 
 struct SysDesc {
   int nrow, nslice, nunits, unit0;
+  int mode, col16;  // mixed: FP32 columns are int16 offsets from the slice start
+  // FP64 SELL-64 (P = 2): the operator of FP64 systems and of the residuals
   const long long *soff;
   const int *slen;
   const unsigned char *rlen;
   const int *col;
   const double *sval;
+  // FP32 SELL-64 (P = 4): the inner operator of mixed systems
+  const long long *soff32;
+  const int *slen32;
+  const void *col32;
+  const float *sval32;
   const double *invd;
+  const float *invd32;
   const double *b;
-  double *x, *r, *p, *z, *Ap, *part;
-  CgState<double> *st;
+  double *x;
+  double *r, *p, *z, *Ap;               // FP64 systems
+  float *r32, *p32, *z32, *Ap32, *c32;  // mixed systems
+  double *part;                         // 16 B per unit
+  SlotState *st;
 };
 
-__device__ __forceinline__ SellOp<double, int> sys_op(const SysDesc &d) {
+__device__ __forceinline__ SellOp<double, int> op64(const SysDesc &d) {
   SellOp<double, int> op;
   op.nrow = d.nrow;
   op.nslice = d.nslice;
@@ -64,30 +93,46 @@ __device__ __forceinline__ SellOp<double, int> sys_op(const SysDesc &d) {
   return op;
 }
 
+template <typename IT>
+__device__ __forceinline__ SellOp<float, IT> op32(const SysDesc &d) {
+  SellOp<float, IT> op;
+  op.nrow = d.nrow;
+  op.nslice = d.nslice;
+  op.soff = d.soff32;
+  op.slen = d.slen32;
+  op.rlen = d.rlen;
+  op.col = static_cast<const IT *>(d.col32);
+  op.val = d.sval32;
+  return op;
+}
+
 // sum of src[stride*u + off], u < count, over one workgroup of kBlock
 // threads, eight loads in flight per thread; valid in thread 0
-__device__ __forceinline__ double units_sum(const double *__restrict__ src, int count, int stride,
-                                            int off) {
-  __shared__ double lds[kBlock / kWave];
-  double a[8];
+template <typename T>
+__device__ __forceinline__ T units_sum(const T *__restrict__ src, int count, int stride,
+                                       int off) {
+  __shared__ T lds[kBlock / kWave];
+  T a[8];
 #pragma unroll
   for (int f = 0; f < 8; ++f) a[f] = 0;
   for (int u0 = threadIdx.x; u0 < count; u0 += 8 * kBlock) {
-    double v[8];
+    T v[8];
 #pragma unroll
     for (int f = 0; f < 8; ++f) {
       const int u = u0 + f * kBlock;
-      v[f] = u < count ? src[(size_t)u * stride + off] : 0.0;
+      v[f] = u < count ? src[(size_t)u * stride + off] : T(0);
     }
 #pragma unroll
     for (int f = 0; f < 8; ++f) a[f] += v[f];
   }
-  const double t = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+  const T t = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
   return block_sum(t, lds);
 }
 
-__device__ __forceinline__ void unit_partial2(double a, double b, double *part, int j) {
-  __shared__ double lds[2][kBlock / kWave];
+// workgroup sums of a and b stored at part[2j], part[2j+1]
+template <typename T>
+__device__ __forceinline__ void unit_partial2(T a, T b, T *part, int j) {
+  __shared__ T lds[2][kBlock / kWave];
   a = wave_sum(a);
   b = wave_sum(b);
   const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
@@ -97,7 +142,7 @@ __device__ __forceinline__ void unit_partial2(double a, double b, double *part, 
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    double sa = 0, sb = 0;
+    T sa = 0, sb = 0;
 #pragma unroll
     for (int w = 0; w < kBlock / kWave; ++w) {
       sa += lds[0][w];
@@ -106,6 +151,18 @@ __device__ __forceinline__ void unit_partial2(double a, double b, double *part, 
     part[2 * j] = sa;
     part[2 * j + 1] = sb;
   }
+}
+
+template <typename T>
+__device__ __forceinline__ void unit_partial1(T a, T *part, int j) {
+  __shared__ T lds[kBlock / kWave];
+  const T t = block_sum(a, lds);
+  if (threadIdx.x == 0) part[j] = t;
+}
+
+// float view of a system's partials (FP32 sums of mixed systems)
+__device__ __forceinline__ float *part32(const SysDesc &d) {
+  return reinterpret_cast<float *>(d.part);
 }
 
 // Unit of the launch -> (system descriptor, local unit).  Units of the
@@ -120,13 +177,24 @@ __device__ __forceinline__ bool unit_of(const SysDesc *__restrict__ sys,
   return true;
 }
 
+__device__ __forceinline__ bool mixed_outer_due(const SlotState *st) {
+  return st->mode == kModeMixed &&
+         (st->status == kFresh || (st->status == kIterate && st->phase == 0));
+}
+
+__device__ __forceinline__ bool mixed_inner_live(const SlotState *st) {
+  return st->mode == kModeMixed && st->status == kIterate && st->phase == 1 &&
+         st->s32.status == kIterate;
+}
+
+// ---- FP64 systems: start -----------------------------------------------------
 // Fresh systems: x = 0, r = b, z = D^-1 b, p = z; partials r.r, r.z
 __global__ void __launch_bounds__(kBlock)
     grp_init_kernel(const SysDesc *__restrict__ sys, const int *__restrict__ unit_sys, int U) {
   SysDesc d;
   int j;
   if (!unit_of(sys, unit_sys, U, d, j)) return;
-  if (d.st->status != kFresh) return;
+  if (d.mode != kModeFp64 || d.st->s64.status != kFresh) return;
   const long long i = (long long)j * kBlock + threadIdx.x;
   double rr = 0, rz = 0;
   if (i < d.nrow) {
@@ -144,8 +212,8 @@ __global__ void __launch_bounds__(kBlock)
 
 __global__ void __launch_bounds__(kBlock) grp_init_final(const SysDesc *__restrict__ sys) {
   const SysDesc d = sys[blockIdx.x];
-  CgState<double> *st = d.st;
-  if (st->status != kFresh) return;
+  CgState<double> *st = &d.st->s64;
+  if (d.mode != kModeFp64 || st->status != kFresh) return;
   const double rr = units_sum(d.part, d.nunits, 2, 0);
   const double rz = units_sum(d.part, d.nunits, 2, 1);
   if (threadIdx.x == 0) {
@@ -159,80 +227,235 @@ __global__ void __launch_bounds__(kBlock) grp_init_final(const SysDesc *__restri
   }
 }
 
+// ---- mixed systems: the FP64 outer step --------------------------------------
+// x = 0 for a fresh system, else x += (double) c
+__global__ void __launch_bounds__(kBlock)
+    grp_mix_correct_kernel(const SysDesc *__restrict__ sys, const int *__restrict__ unit_sys,
+                           int U) {
+  SysDesc d;
+  int j;
+  if (!unit_of(sys, unit_sys, U, d, j)) return;
+  if (!mixed_outer_due(d.st)) return;
+  const bool fresh = d.st->status == kFresh;
+  const long long i = (long long)j * kBlock + threadIdx.x;
+  if (i < d.nrow) d.x[i] = fresh ? 0.0 : d.x[i] + (double)d.c32[i];
+}
+
+// r = b - A x in FP64; r32 = (float) r; the inner start from c = 0:
+// z32 = D32^-1 r32, p32 = z32.  Partials: FP64 r.r; FP32 r32.r32 and r32.z32.
+__global__ void __launch_bounds__(kBlock)
+    grp_mix_residual_kernel(const SysDesc *__restrict__ sys, const int *__restrict__ unit_sys,
+                            int U) {
+  SysDesc d;
+  int j;
+  if (!unit_of(sys, unit_sys, U, d, j)) return;
+  if (!mixed_outer_due(d.st)) return;
+  TileRow tr;
+  const double s = op64(d).row_sum(j, d.x, 0.0, tr, nullptr);
+  double rr = 0;
+  float rr32 = 0, rz32 = 0;
+  if (tr.has_row) {
+    const int i = tr.row;
+    const double ri = -1.0 * s + d.b[i];
+    const float rf = (float)ri;
+    const float zf = rf * d.invd32[i];
+    d.r32[i] = rf;
+    d.z32[i] = zf;
+    d.p32[i] = zf;
+    d.c32[i] = 0.0f;
+    rr = ri * ri;
+    rr32 = rf * rf;
+    rz32 = rf * zf;
+  }
+  // part[2j] = FP64 r.r; the two FP32 sums fill the unit's second 8 bytes
+  // (floats 4j+2 and 4j+3)
+  unit_partial1(rr, d.part, 2 * j);
+  unit_partial2(rr32, rz32, part32(d), 2 * j + 1);
+}
+
+// Outer test on the FP64 residual (initial residual kept as r_0), then the
+// inner start with tolerance max(inner_rtol, 0.5 rtol r_0 / r), capped at 0.5.
+__global__ void __launch_bounds__(kBlock) grp_mix_residual_final(const SysDesc *__restrict__ sys) {
+  const SysDesc d = sys[blockIdx.x];
+  SlotState *st = d.st;
+  if (!mixed_outer_due(st)) return;
+  const double rr = units_sum(d.part, d.nunits, 2, 0);
+  const float rr32 = units_sum(part32(d), d.nunits, 4, 2);
+  const float rz32 = units_sum(part32(d), d.nunits, 4, 3);
+  if (threadIdx.x == 0) {
+    const bool fresh = st->status == kFresh;
+    if (!fresh) st->outer += 1;
+    const double res = sqrt(rr);
+    if (fresh) st->res0 = res;
+    st->res = res;
+    const double res0 = st->res0;
+    int status = kIterate;
+    if (res <= st->atol || res < st->rtol * res0) status = kConverged;
+    else if (!(res == res)) status = kBreakdown;
+    else if (st->outer >= st->max_outer) status = kMaxIts;
+    st->status = status;
+    if (status == kIterate) {
+      const double need = res > 0 ? 0.5 * st->rtol * res0 / res : 0.0;
+      const double tol = need > st->inner_rtol ? (need < 0.5 ? need : 0.5) : st->inner_rtol;
+      CgState<float> &s = st->s32;
+      const float r0 = sqrtf(rr32);
+      s.rz = rz32;
+      s.rr = rr32;
+      s.res0 = r0;
+      s.res = r0;
+      s.alpha = s.beta = s.pAp = 0.0f;
+      s.iter = 0;
+      s.maxit = st->inner_maxit;
+      s.atol = 0.0;
+      s.rtol = tol;
+      s.dtol = 1e30;
+      s.status = check_state(0, (double)r0, (double)r0, s.maxit, s.atol, s.rtol, s.dtol);
+      st->phase = s.status == kIterate ? 1 : 0;
+    }
+  }
+}
+
+// ---- the iteration, both precisions -------------------------------------------
 // K1: Ap = A p, partial p.Ap per unit
 __global__ void __launch_bounds__(kBlock)
     grp_spmv_kernel(const SysDesc *__restrict__ sys, const int *__restrict__ unit_sys, int U) {
   SysDesc d;
   int j;
   if (!unit_of(sys, unit_sys, U, d, j)) return;
-  if (d.st->status != kIterate) return;
-  const SellOp<double, int> op = sys_op(d);
   TileRow tr;
-  const double s = op.row_sum(j, d.p, 0.0, tr, nullptr);
-  double part = 0;
-  if (tr.has_row) {
-    d.Ap[tr.row] = s;
-    part = s * d.p[tr.row];
+  if (d.mode == kModeFp64) {
+    if (d.st->s64.status != kIterate) return;
+    const double s = op64(d).row_sum(j, d.p, 0.0, tr, nullptr);
+    double part = 0;
+    if (tr.has_row) {
+      d.Ap[tr.row] = s;
+      part = s * d.p[tr.row];
+    }
+    unit_partial1(part, d.part, j);
+  } else {
+    if (!mixed_inner_live(d.st)) return;
+    const float s = d.col16 ? op32<short>(d).row_sum(j, d.p32, 0.0f, tr, nullptr)
+                            : op32<int>(d).row_sum(j, d.p32, 0.0f, tr, nullptr);
+    float part = 0;
+    if (tr.has_row) {
+      d.Ap32[tr.row] = s;
+      part = s * d.p32[tr.row];
+    }
+    unit_partial1(part, part32(d), j);
   }
-  __shared__ double lds[kBlock / kWave];
-  const double t = block_sum(part, lds);
-  if (threadIdx.x == 0) d.part[j] = t;
 }
 
 // F1: alpha = r.z / p.Ap, one workgroup per system
 __global__ void __launch_bounds__(kBlock) grp_alpha_final(const SysDesc *__restrict__ sys) {
   const SysDesc d = sys[blockIdx.x];
-  CgState<double> *st = d.st;
-  if (st->status != kIterate) return;
-  const double pAp = units_sum(d.part, d.nunits, 1, 0);
-  if (threadIdx.x == 0) {
-    st->pAp = pAp;
-    st->alpha = st->rz / pAp;
-    if (pAp == 0.0 || !(pAp == pAp)) st->status = kBreakdown;
+  if (d.mode == kModeFp64) {
+    CgState<double> *st = &d.st->s64;
+    if (st->status != kIterate) return;
+    const double pAp = units_sum(d.part, d.nunits, 1, 0);
+    if (threadIdx.x == 0) {
+      st->pAp = pAp;
+      st->alpha = st->rz / pAp;
+      if (pAp == 0.0 || !(pAp == pAp)) st->status = kBreakdown;
+    }
+  } else {
+    if (!mixed_inner_live(d.st)) return;
+    CgState<float> *st = &d.st->s32;
+    const float pAp = units_sum(part32(d), d.nunits, 1, 0);
+    if (threadIdx.x == 0) {
+      st->pAp = pAp;
+      st->alpha = st->rz / pAp;
+      if (pAp == 0.0f || !(pAp == pAp)) {
+        st->status = kBreakdown;
+        d.st->inner_total += st->iter;
+        d.st->phase = 0;
+      }
+    }
   }
 }
 
 // K2: x += alpha p; r -= alpha Ap; z = D^-1 r; partials r.r, r.z
+// (mixed systems: c in place of x, all FP32)
 __global__ void __launch_bounds__(kBlock)
     grp_update_kernel(const SysDesc *__restrict__ sys, const int *__restrict__ unit_sys, int U) {
   SysDesc d;
   int j;
   if (!unit_of(sys, unit_sys, U, d, j)) return;
-  if (d.st->status != kIterate) return;
-  const double alpha = d.st->alpha, malpha = -alpha;
   const long long i = (long long)j * kBlock + threadIdx.x;
-  double rr = 0, rz = 0;
-  if (i < d.nrow) {
-    d.x[i] += alpha * d.p[i];
-    const double ri = d.r[i] + malpha * d.Ap[i];
-    const double zi = ri * d.invd[i];
-    d.r[i] = ri;
-    d.z[i] = zi;
-    rr = ri * ri;
-    rz = ri * zi;
+  if (d.mode == kModeFp64) {
+    if (d.st->s64.status != kIterate) return;
+    const double alpha = d.st->s64.alpha, malpha = -alpha;
+    double rr = 0, rz = 0;
+    if (i < d.nrow) {
+      d.x[i] += alpha * d.p[i];
+      const double ri = d.r[i] + malpha * d.Ap[i];
+      const double zi = ri * d.invd[i];
+      d.r[i] = ri;
+      d.z[i] = zi;
+      rr = ri * ri;
+      rz = ri * zi;
+    }
+    unit_partial2(rr, rz, d.part, j);
+  } else {
+    if (!mixed_inner_live(d.st)) return;
+    const float alpha = d.st->s32.alpha, malpha = -alpha;
+    float rr = 0, rz = 0;
+    if (i < d.nrow) {
+      d.c32[i] += alpha * d.p32[i];
+      const float ri = d.r32[i] + malpha * d.Ap32[i];
+      const float zi = ri * d.invd32[i];
+      d.r32[i] = ri;
+      d.z32[i] = zi;
+      rr = ri * ri;
+      rz = ri * zi;
+    }
+    unit_partial2(rr, rz, part32(d), j);
   }
-  unit_partial2(rr, rz, d.part, j);
 }
 
-// F2: res, stopping test, beta; one workgroup per system
+// F2: res, stopping test, beta; one workgroup per system.  A mixed system
+// whose inner solve stops is marked for the outer step of the next step.
 __global__ void __launch_bounds__(kBlock) grp_beta_final(const SysDesc *__restrict__ sys) {
   const SysDesc d = sys[blockIdx.x];
-  CgState<double> *st = d.st;
-  if (st->status != kIterate) return;
-  const double rr = units_sum(d.part, d.nunits, 2, 0);
-  const double rz = units_sum(d.part, d.nunits, 2, 1);
-  if (threadIdx.x == 0) {
-    const int iter = st->iter + 1;
-    const double res = sqrt(rr);
-    st->iter = iter;
-    st->rr = rr;
-    st->res = res;
-    const int s = check_state(iter, res, st->res0, st->maxit, st->atol, st->rtol, st->dtol);
-    if (s == kIterate) {
-      st->beta = rz / st->rz;
-      st->rz = rz;
+  if (d.mode == kModeFp64) {
+    CgState<double> *st = &d.st->s64;
+    if (st->status != kIterate) return;
+    const double rr = units_sum(d.part, d.nunits, 2, 0);
+    const double rz = units_sum(d.part, d.nunits, 2, 1);
+    if (threadIdx.x == 0) {
+      const int iter = st->iter + 1;
+      const double res = sqrt(rr);
+      st->iter = iter;
+      st->rr = rr;
+      st->res = res;
+      const int s = check_state(iter, res, st->res0, st->maxit, st->atol, st->rtol, st->dtol);
+      if (s == kIterate) {
+        st->beta = rz / st->rz;
+        st->rz = rz;
+      }
+      st->status = s;
     }
-    st->status = s;
+  } else {
+    if (!mixed_inner_live(d.st)) return;
+    CgState<float> *st = &d.st->s32;
+    const float rr = units_sum(part32(d), d.nunits, 2, 0);
+    const float rz = units_sum(part32(d), d.nunits, 2, 1);
+    if (threadIdx.x == 0) {
+      const int iter = st->iter + 1;
+      const float res = sqrtf(rr);
+      st->iter = iter;
+      st->rr = rr;
+      st->res = res;
+      const int s = check_state(iter, (double)res, (double)st->res0, st->maxit, st->atol,
+                                st->rtol, st->dtol);
+      if (s == kIterate) {
+        st->beta = rz / st->rz;
+        st->rz = rz;
+      } else {
+        d.st->inner_total += iter;
+        d.st->phase = 0;
+      }
+      st->status = s;
+    }
   }
 }
 
@@ -242,40 +465,60 @@ __global__ void __launch_bounds__(kBlock)
   SysDesc d;
   int j;
   if (!unit_of(sys, unit_sys, U, d, j)) return;
-  if (d.st->status != kIterate) return;
-  const double beta = d.st->beta;
   const long long i = (long long)j * kBlock + threadIdx.x;
-  if (i < d.nrow) d.p[i] = beta * d.p[i] + d.z[i];
+  if (d.mode == kModeFp64) {
+    if (d.st->s64.status != kIterate) return;
+    const double beta = d.st->s64.beta;
+    if (i < d.nrow) d.p[i] = beta * d.p[i] + d.z[i];
+  } else {
+    if (!mixed_inner_live(d.st)) return;
+    const float beta = d.st->s32.beta;
+    if (i < d.nrow) d.p32[i] = beta * d.p32[i] + d.z32[i];
+  }
 }
 
-// CSR -> SELL-64 of one system (thread per row), and its inverse diagonal
-// (1/a_ii formed in double as extract_invdiagelements does,
-// lmatrix_csr_cpu.cc:2419-2442; 0 where the diagonal is missing or zero,
-// counted in *bad).
+// CSR -> SELL-64 layouts of one system (thread per row) and its inverse
+// diagonal: 1/a_ii formed in double (extract_invdiagelements,
+// lmatrix_csr_cpu.cc:2419-2442); for the FP32 operator from the FP32 value,
+// as the cast matrix's own extraction gives.  0 where the diagonal is
+// missing or zero, counted in *bad.
+template <typename IT32>
 __global__ void __launch_bounds__(kBlock)
     sys_fill_kernel(int nrow, const int *__restrict__ rp, const int *__restrict__ ci,
                     const double *__restrict__ val, const long long *__restrict__ soff,
                     int *__restrict__ col, double *__restrict__ sval, double *__restrict__ invd,
-                    unsigned int *bad) {
-  constexpr int P = SellChunk<double>::P;
+                    const long long *__restrict__ soff32, IT32 *__restrict__ col32,
+                    float *__restrict__ sval32, float *__restrict__ invd32, unsigned int *bad) {
+  constexpr int P = SellChunk<double>::P, P32 = SellChunk<float>::P;
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < nrow; i += gridDim.x * kBlock) {
     const int s = i / kSellRows, lane = i % kSellRows;
     const long long base = soff[s] + (long long)P * lane;
+    const long long base32 = soff32 ? soff32[s] + (long long)P32 * lane : 0;
     const int k0 = rp[i], len = rp[i + 1] - k0;
     double dg = 0;
     for (int j = 0; j < len; ++j) {
-      const long long pos = base + (long long)kSellRows * P * (j / P) + (j % P);
       const int c = ci[k0 + j];
       const double v = val[k0 + j];
+      const long long pos = base + (long long)kSellRows * P * (j / P) + (j % P);
       col[pos] = c;
       sval[pos] = v;
+      if (soff32) {
+        const long long p32 = base32 + (long long)kSellRows * P32 * (j / P32) + (j % P32);
+        col32[p32] = (IT32)(std::is_same<IT32, int>::value ? c : c - s * kSellRows);
+        sval32[p32] = (float)v;
+      }
       if (c == i && dg == 0.0) dg = v;
     }
     if (dg == 0.0) {
-      invd[i] = 0.0;
+      if (invd) invd[i] = 0.0;
+      if (invd32) invd32[i] = 0.0f;
       atomicAdd(bad, 1u);
     } else {
-      invd[i] = 1.0 / dg;
+      if (invd) invd[i] = 1.0 / dg;
+      if (invd32) {
+        const float d32 = (float)dg;
+        invd32[i] = d32 == 0.0f ? 0.0f : (float)(1.0 / (double)d32);
+      }
     }
   }
 }
@@ -334,78 +577,119 @@ class Arena {
 
 enum JobState { kQueued = 0, kActive = 1, kDone = 2 };
 
+// SELL-64 slice plan for chunk P from the host row_ptr
+struct SlicePlan {
+  std::vector<long long> soff;
+  std::vector<int> slen;
+  long long entries = 0;
+};
+
+void plan_slices(int nrow, const int *rp, int P, SlicePlan &sp) {
+  const int ns = (nrow + kSellRows - 1) / kSellRows;
+  sp.slen.assign(ns, 0);
+  sp.soff.assign((size_t)ns + 1, 0);
+  long long tot = 0;
+  for (int s = 0; s < ns; ++s) {
+    int m = 0;
+    for (int l = 0; l < kSellRows; ++l) {
+      const int r = s * kSellRows + l;
+      if (r >= nrow) break;
+      m = std::max(m, rp[r + 1] - rp[r]);
+    }
+    sp.slen[s] = (m + P - 1) / P * P;
+    sp.soff[s] = tot;
+    tot += (long long)kSellRows * sp.slen[s];
+  }
+  sp.soff[ns] = tot;
+  sp.entries = tot;
+}
+
 struct Job {
   long long ticket = 0;
-  int nrow = 0, nnz = 0, nslice = 0, nunits = 0;
+  int nrow = 0, nnz = 0, nslice = 0, nunits = 0, mode = kModeFp64, col16 = 0;
   const int *rp = nullptr, *ci = nullptr;
   const double *val = nullptr, *b = nullptr;
   double *x_out = nullptr;
   double atol = 0, rtol = 0;
   int maxit = 0;
-  std::vector<long long> soff;
-  std::vector<int> slen;
+  SlicePlan sp64, sp32;
   std::vector<unsigned char> rlen;
-  long long entries = 0;
   size_t bytes = 0, off = 0;
   int slot = -1;
   int state = kQueued;
   SysDesc d{};
-  // device-side block layout (offsets inside the job's arena block)
-  size_t o_rp = 0, o_ci = 0, o_val = 0, o_soff = 0, o_slen = 0, o_rlen = 0, o_col = 0,
-         o_sval = 0, o_invd = 0, o_b = 0, o_x = 0, o_r = 0, o_p = 0, o_z = 0, o_Ap = 0,
-         o_part = 0;
+  // device-side block layout (offsets inside the job's arena block; 0 = absent)
+  size_t o_rp = 0, o_ci = 0, o_val = 0, o_rlen = 0, o_b = 0, o_x = 0, o_part = 0;
+  size_t o_soff = 0, o_slen = 0, o_col = 0, o_sval = 0, o_invd = 0, o_r = 0, o_p = 0, o_z = 0,
+         o_Ap = 0;
+  size_t o_soff32 = 0, o_slen32 = 0, o_col32 = 0, o_sval32 = 0, o_invd32 = 0, o_r32 = 0,
+         o_p32 = 0, o_z32 = 0, o_Ap32 = 0, o_c32 = 0;
   std::chrono::steady_clock::time_point t_submit, t_admit, t_done;
-  int iters = 0, status = 0, bad_diag = 0;
+  int iters = 0, status = 0, outer = 0, bad_diag = 0;
   double res_init = 0, res_final = 0;
 };
 
 // Lays out the job's arrays and computes its footprint (256-byte aligned).
 void plan_job(Job &jb) {
-  constexpr int P = SellChunk<double>::P;
   jb.nslice = (jb.nrow + kSellRows - 1) / kSellRows;
   jb.nunits = (jb.nrow + kBlock - 1) / kBlock;
-  jb.slen.assign(jb.nslice, 0);
   jb.rlen.assign(jb.nrow, 0);
-  jb.soff.assign((size_t)jb.nslice + 1, 0);
-  long long tot = 0;
-  for (int s = 0; s < jb.nslice; ++s) {
-    int m = 0;
-    for (int l = 0; l < kSellRows; ++l) {
-      const int r = s * kSellRows + l;
-      if (r >= jb.nrow) break;
-      const int len = jb.rp[r + 1] - jb.rp[r];
-      jb.rlen[r] = (unsigned char)len;
-      m = std::max(m, len);
+  for (int r = 0; r < jb.nrow; ++r) jb.rlen[r] = (unsigned char)(jb.rp[r + 1] - jb.rp[r]);
+  plan_slices(jb.nrow, jb.rp, SellChunk<double>::P, jb.sp64);
+  jb.col16 = 0;
+  if (jb.mode == kModeMixed) {
+    plan_slices(jb.nrow, jb.rp, SellChunk<float>::P, jb.sp32);
+    bool fits = jb.ci != nullptr;
+    for (int r = 0; r < jb.nrow && fits; ++r) {
+      const int base = (r / kSellRows) * kSellRows;
+      for (int k = jb.rp[r]; k < jb.rp[r + 1]; ++k) {
+        const int dlt = jb.ci[k] - base;
+        if (dlt < -32768 || dlt > 32767) {
+          fits = false;
+          break;
+        }
+      }
     }
-    jb.slen[s] = (m + P - 1) / P * P;
-    jb.soff[s] = tot;
-    tot += (long long)kSellRows * jb.slen[s];
+    jb.col16 = fits ? 1 : 0;
   }
-  jb.soff[jb.nslice] = tot;
-  jb.entries = tot;
   size_t o = 0;
   auto take = [&](size_t bytes) {
     const size_t at = o;
-    o = align256(o + bytes);
+    o = align256(o + (bytes ? bytes : 1));
     return at;
   };
   const size_t n = (size_t)jb.nrow;
+  const size_t e64 = (size_t)jb.sp64.entries;
   jb.o_rp = take(4 * (n + 1));
   jb.o_ci = take(4 * (size_t)jb.nnz);
   jb.o_val = take(8 * (size_t)jb.nnz);
-  jb.o_soff = take(8 * ((size_t)jb.nslice + 1));
-  jb.o_slen = take(4 * (size_t)jb.nslice);
   jb.o_rlen = take(n);
-  jb.o_col = take(4 * (size_t)tot);
-  jb.o_sval = take(8 * (size_t)tot);
-  jb.o_invd = take(8 * n);
   jb.o_b = take(8 * n);
   jb.o_x = take(8 * n);
-  jb.o_r = take(8 * n);
-  jb.o_p = take(8 * n);
-  jb.o_z = take(8 * n);
-  jb.o_Ap = take(8 * n);
   jb.o_part = take(16 * (size_t)jb.nunits);
+  jb.o_soff = take(8 * ((size_t)jb.nslice + 1));
+  jb.o_slen = take(4 * (size_t)jb.nslice);
+  jb.o_col = take(4 * e64);
+  jb.o_sval = take(8 * e64);
+  if (jb.mode == kModeFp64) {
+    jb.o_invd = take(8 * n);
+    jb.o_r = take(8 * n);
+    jb.o_p = take(8 * n);
+    jb.o_z = take(8 * n);
+    jb.o_Ap = take(8 * n);
+  } else {
+    const size_t e32 = (size_t)jb.sp32.entries;
+    jb.o_soff32 = take(8 * ((size_t)jb.nslice + 1));
+    jb.o_slen32 = take(4 * (size_t)jb.nslice);
+    jb.o_col32 = take((jb.col16 ? 2 : 4) * e32);
+    jb.o_sval32 = take(4 * e32);
+    jb.o_invd32 = take(4 * n);
+    jb.o_r32 = take(4 * n);
+    jb.o_p32 = take(4 * n);
+    jb.o_z32 = take(4 * n);
+    jb.o_Ap32 = take(4 * n);
+    jb.o_c32 = take(4 * n);
+  }
   jb.bytes = o;
 }
 
@@ -416,10 +700,11 @@ struct hflmp_server {
   size_t budget = 0;
   char *arena = nullptr;
   Arena alloc;
-  int max_active = 0, chunk = 0, use_graph = 1;
-  CgState<double> *d_states = nullptr;  // [max_active]
-  CgState<double> *h_states = nullptr;  // pinned mirror
-  SysDesc *d_sys = nullptr;             // [max_active]
+  int max_active = 0, chunk = 0, use_graph = 1, mixed_min_rows = 0;
+  double inner_rtol = 1e-4;
+  SlotState *d_states = nullptr;  // [max_active]
+  SlotState *h_states = nullptr;  // pinned mirror
+  SysDesc *d_sys = nullptr;       // [max_active]
   int *d_unit_sys = nullptr;
   size_t unit_cap = 0;
   unsigned int *d_bad = nullptr;
@@ -448,13 +733,16 @@ void drop_graph(hflmp_server *sv) {
   sv->graph = nullptr;
 }
 
-// The work of one step on the stream: initial residuals of fresh systems,
-// then `chunk` iterations.
+// The work of one step on the stream: starts of fresh FP64 systems, outer
+// steps of mixed systems, then `chunk` iterations.
 void enqueue_chunk(hflmp_server *sv, hipStream_t s) {
   const unsigned gU = (unsigned)(((sv->U + kXcds - 1) / kXcds) * kXcds);
   const unsigned gS = (unsigned)sv->S;
   grp_init_kernel<<<gU, kBlock, 0, s>>>(sv->d_sys, sv->d_unit_sys, sv->U);
   grp_init_final<<<gS, kBlock, 0, s>>>(sv->d_sys);
+  grp_mix_correct_kernel<<<gU, kBlock, 0, s>>>(sv->d_sys, sv->d_unit_sys, sv->U);
+  grp_mix_residual_kernel<<<gU, kBlock, 0, s>>>(sv->d_sys, sv->d_unit_sys, sv->U);
+  grp_mix_residual_final<<<gS, kBlock, 0, s>>>(sv->d_sys);
   for (int it = 0; it < sv->chunk; ++it) {
     grp_spmv_kernel<<<gU, kBlock, 0, s>>>(sv->d_sys, sv->d_unit_sys, sv->U);
     grp_alpha_final<<<gS, kBlock, 0, s>>>(sv->d_sys);
@@ -480,9 +768,12 @@ int rebuild_tables(hflmp_server *sv) {
   sv->U = unit0;
   if ((size_t)sv->U > sv->unit_cap) {
     if (sv->d_unit_sys) (void)hipFree(sv->d_unit_sys);
+    sv->d_unit_sys = nullptr;
     sv->unit_cap = std::max((size_t)sv->U, 2 * sv->unit_cap);
-    if (hipMalloc(&sv->d_unit_sys, sizeof(int) * sv->unit_cap) != hipSuccess)
+    if (hipMalloc(&sv->d_unit_sys, sizeof(int) * sv->unit_cap) != hipSuccess) {
+      sv->unit_cap = 0;
       return set_error(ctx, HFLMP_ENOMEM, "server: unit table allocation failed");
+    }
   }
   if (sv->S) {
     HFLMP_HIP_CHECK(ctx, hipMemcpyAsync(sv->d_sys, sv->h_sys.data(), sizeof(SysDesc) * sv->S,
@@ -500,64 +791,116 @@ int admit(hflmp_server *sv, Job &jb, int slot) {
   hflmp_ctx *ctx = sv->ctx;
   hipStream_t s = ctx->stream;
   char *base = sv->arena + jb.off;
-  auto dp = [&](size_t o) { return base + o; };
+  auto dp = [&](size_t o) { return static_cast<void *>(base + o); };
   const size_t n = (size_t)jb.nrow;
-  HFLMP_HIP_CHECK(ctx, hipMemcpyAsync(dp(jb.o_rp), jb.rp, 4 * (n + 1), hipMemcpyHostToDevice, s));
-  if (jb.nnz) {
-    HFLMP_HIP_CHECK(ctx, hipMemcpyAsync(dp(jb.o_ci), jb.ci, 4 * (size_t)jb.nnz,
-                                        hipMemcpyHostToDevice, s));
-    HFLMP_HIP_CHECK(ctx, hipMemcpyAsync(dp(jb.o_val), jb.val, 8 * (size_t)jb.nnz,
-                                        hipMemcpyHostToDevice, s));
+  const bool mixed = jb.mode == kModeMixed;
+  auto up = [&](size_t o, const void *src, size_t bytes) {
+    return bytes ? hipMemcpyAsync(dp(o), src, bytes, hipMemcpyHostToDevice, s) : hipSuccess;
+  };
+  HFLMP_HIP_CHECK(ctx, up(jb.o_rp, jb.rp, 4 * (n + 1)));
+  HFLMP_HIP_CHECK(ctx, up(jb.o_ci, jb.ci, 4 * (size_t)jb.nnz));
+  HFLMP_HIP_CHECK(ctx, up(jb.o_val, jb.val, 8 * (size_t)jb.nnz));
+  HFLMP_HIP_CHECK(ctx, up(jb.o_rlen, jb.rlen.data(), n));
+  HFLMP_HIP_CHECK(ctx, up(jb.o_b, jb.b, 8 * n));
+  HFLMP_HIP_CHECK(ctx, up(jb.o_soff, jb.sp64.soff.data(), 8 * jb.sp64.soff.size()));
+  HFLMP_HIP_CHECK(ctx, up(jb.o_slen, jb.sp64.slen.data(), 4 * jb.sp64.slen.size()));
+  int rc = hflmp_fill_zero(ctx, dp(jb.o_col), 4 * (size_t)jb.sp64.entries);
+  if (!rc) rc = hflmp_fill_zero(ctx, dp(jb.o_sval), 8 * (size_t)jb.sp64.entries);
+  if (!rc && mixed) {
+    HFLMP_HIP_CHECK(ctx, up(jb.o_soff32, jb.sp32.soff.data(), 8 * jb.sp32.soff.size()));
+    HFLMP_HIP_CHECK(ctx, up(jb.o_slen32, jb.sp32.slen.data(), 4 * jb.sp32.slen.size()));
+    rc = hflmp_fill_zero(ctx, dp(jb.o_col32), (jb.col16 ? 2 : 4) * (size_t)jb.sp32.entries);
+    if (!rc) rc = hflmp_fill_zero(ctx, dp(jb.o_sval32), 4 * (size_t)jb.sp32.entries);
   }
-  HFLMP_HIP_CHECK(ctx, hipMemcpyAsync(dp(jb.o_soff), jb.soff.data(), 8 * jb.soff.size(),
-                                      hipMemcpyHostToDevice, s));
-  HFLMP_HIP_CHECK(ctx, hipMemcpyAsync(dp(jb.o_slen), jb.slen.data(), 4 * jb.slen.size(),
-                                      hipMemcpyHostToDevice, s));
-  HFLMP_HIP_CHECK(ctx, hipMemcpyAsync(dp(jb.o_rlen), jb.rlen.data(), n, hipMemcpyHostToDevice, s));
-  HFLMP_HIP_CHECK(ctx, hipMemcpyAsync(dp(jb.o_b), jb.b, 8 * n, hipMemcpyHostToDevice, s));
-  int rc = hflmp_fill_zero(ctx, dp(jb.o_col), 4 * (size_t)jb.entries);
-  if (!rc) rc = hflmp_fill_zero(ctx, dp(jb.o_sval), 8 * (size_t)jb.entries);
   if (rc) return rc;
-  sys_fill_kernel<<<grid_for(jb.nrow, 1), kBlock, 0, s>>>(
-      jb.nrow, reinterpret_cast<const int *>(dp(jb.o_rp)),
-      reinterpret_cast<const int *>(dp(jb.o_ci)), reinterpret_cast<const double *>(dp(jb.o_val)),
-      reinterpret_cast<const long long *>(dp(jb.o_soff)), reinterpret_cast<int *>(dp(jb.o_col)),
-      reinterpret_cast<double *>(dp(jb.o_sval)), reinterpret_cast<double *>(dp(jb.o_invd)),
-      sv->d_bad + slot);
+  const unsigned g = grid_for(jb.nrow, 1);
+  const int *rp = static_cast<const int *>(dp(jb.o_rp));
+  const int *ci = static_cast<const int *>(dp(jb.o_ci));
+  const double *val = static_cast<const double *>(dp(jb.o_val));
+  const long long *soff = static_cast<const long long *>(dp(jb.o_soff));
+  int *col = static_cast<int *>(dp(jb.o_col));
+  double *sval = static_cast<double *>(dp(jb.o_sval));
+  if (!mixed) {
+    sys_fill_kernel<int><<<g, kBlock, 0, s>>>(jb.nrow, rp, ci, val, soff, col, sval,
+                                              static_cast<double *>(dp(jb.o_invd)), nullptr,
+                                              nullptr, nullptr, nullptr, sv->d_bad + slot);
+  } else if (jb.col16) {
+    sys_fill_kernel<short><<<g, kBlock, 0, s>>>(
+        jb.nrow, rp, ci, val, soff, col, sval, nullptr,
+        static_cast<const long long *>(dp(jb.o_soff32)), static_cast<short *>(dp(jb.o_col32)),
+        static_cast<float *>(dp(jb.o_sval32)), static_cast<float *>(dp(jb.o_invd32)),
+        sv->d_bad + slot);
+  } else {
+    sys_fill_kernel<int><<<g, kBlock, 0, s>>>(
+        jb.nrow, rp, ci, val, soff, col, sval, nullptr,
+        static_cast<const long long *>(dp(jb.o_soff32)), static_cast<int *>(dp(jb.o_col32)),
+        static_cast<float *>(dp(jb.o_sval32)), static_cast<float *>(dp(jb.o_invd32)),
+        sv->d_bad + slot);
+  }
   HFLMP_LAUNCH_CHECK(ctx);
-  CgState<double> st{};
-  st.status = kFresh;
-  st.maxit = jb.maxit;
+  SlotState st;
+  std::memset(&st, 0, sizeof(st));
+  st.mode = jb.mode;
   st.atol = jb.atol;
   st.rtol = jb.rtol;
-  st.dtol = 1e300;
+  st.inner_rtol = sv->inner_rtol;
+  st.inner_maxit = jb.maxit;
+  st.max_outer = kMaxOuter;
+  st.status = mixed ? kFresh : kIterate;
+  st.s64.status = mixed ? kConverged : kFresh;
+  st.s64.maxit = jb.maxit;
+  st.s64.atol = jb.atol;
+  st.s64.rtol = jb.rtol;
+  st.s64.dtol = 1e300;
+  st.s32.status = kConverged;
   // the pinned mirror slot is free (the slot was retired) and is only read
-  // after the next state copy
+  // after the next state copy, which the stream orders after this one
   sv->h_states[slot] = st;
   HFLMP_HIP_CHECK(ctx, hipMemcpyAsync(sv->d_states + slot, sv->h_states + slot, sizeof(st),
                                       hipMemcpyHostToDevice, s));
   SysDesc &d = jb.d;
+  std::memset(&d, 0, sizeof(d));
   d.nrow = jb.nrow;
   d.nslice = jb.nslice;
   d.nunits = jb.nunits;
-  d.soff = reinterpret_cast<const long long *>(dp(jb.o_soff));
-  d.slen = reinterpret_cast<const int *>(dp(jb.o_slen));
-  d.rlen = reinterpret_cast<const unsigned char *>(dp(jb.o_rlen));
-  d.col = reinterpret_cast<const int *>(dp(jb.o_col));
-  d.sval = reinterpret_cast<const double *>(dp(jb.o_sval));
-  d.invd = reinterpret_cast<const double *>(dp(jb.o_invd));
-  d.b = reinterpret_cast<const double *>(dp(jb.o_b));
-  d.x = reinterpret_cast<double *>(dp(jb.o_x));
-  d.r = reinterpret_cast<double *>(dp(jb.o_r));
-  d.p = reinterpret_cast<double *>(dp(jb.o_p));
-  d.z = reinterpret_cast<double *>(dp(jb.o_z));
-  d.Ap = reinterpret_cast<double *>(dp(jb.o_Ap));
-  d.part = reinterpret_cast<double *>(dp(jb.o_part));
+  d.mode = jb.mode;
+  d.col16 = jb.col16;
+  d.soff = soff;
+  d.slen = static_cast<const int *>(dp(jb.o_slen));
+  d.rlen = static_cast<const unsigned char *>(dp(jb.o_rlen));
+  d.col = col;
+  d.sval = sval;
+  d.b = static_cast<const double *>(dp(jb.o_b));
+  d.x = static_cast<double *>(dp(jb.o_x));
+  d.part = static_cast<double *>(dp(jb.o_part));
+  if (!mixed) {
+    d.invd = static_cast<const double *>(dp(jb.o_invd));
+    d.r = static_cast<double *>(dp(jb.o_r));
+    d.p = static_cast<double *>(dp(jb.o_p));
+    d.z = static_cast<double *>(dp(jb.o_z));
+    d.Ap = static_cast<double *>(dp(jb.o_Ap));
+  } else {
+    d.soff32 = static_cast<const long long *>(dp(jb.o_soff32));
+    d.slen32 = static_cast<const int *>(dp(jb.o_slen32));
+    d.col32 = dp(jb.o_col32);
+    d.sval32 = static_cast<const float *>(dp(jb.o_sval32));
+    d.invd32 = static_cast<const float *>(dp(jb.o_invd32));
+    d.r32 = static_cast<float *>(dp(jb.o_r32));
+    d.p32 = static_cast<float *>(dp(jb.o_p32));
+    d.z32 = static_cast<float *>(dp(jb.o_z32));
+    d.Ap32 = static_cast<float *>(dp(jb.o_Ap32));
+    d.c32 = static_cast<float *>(dp(jb.o_c32));
+  }
   d.st = sv->d_states + slot;
   jb.slot = slot;
   jb.state = kActive;
   jb.t_admit = Clock::now();
   return HFLMP_OK;
+}
+
+bool slot_running(const SlotState &st) {
+  if (st.mode == kModeMixed) return st.status == kIterate || st.status == kFresh;
+  return st.s64.status == kIterate || st.s64.status == kFresh;
 }
 
 }  // namespace
@@ -570,25 +913,29 @@ int hflmp_server_create(hflmp_ctx *ctx, const hflmp_server_config *cfg, hflmp_se
   *out = nullptr;
   HFLMP_REQUIRE(ctx, cfg->hbm_budget >= (1u << 20) && cfg->max_active >= 1 &&
                          cfg->max_active <= 4096 && cfg->chunk_iters >= 1 &&
-                         cfg->chunk_iters <= 256,
+                         cfg->chunk_iters <= 256 && cfg->mixed_min_rows >= 0 &&
+                         cfg->inner_rtol >= 0 && cfg->inner_rtol < 1,
                 "server_create: need hbm_budget >= 1 MiB, 1 <= max_active <= 4096, "
-                "1 <= chunk_iters <= 256");
+                "1 <= chunk_iters <= 256, mixed_min_rows >= 0, 0 <= inner_rtol < 1");
   auto *sv = new hflmp_server();
   sv->ctx = ctx;
   sv->budget = cfg->hbm_budget;
   sv->max_active = cfg->max_active;
   sv->chunk = cfg->chunk_iters;
   sv->use_graph = cfg->use_graphs ? 1 : 0;
+  sv->mixed_min_rows = cfg->mixed_min_rows;
+  sv->inner_rtol = cfg->inner_rtol > 0 ? cfg->inner_rtol : 1e-4;
   auto fail = [&](const char *what) {
     hflmp_server_destroy(sv);
     return set_error(ctx, HFLMP_ENOMEM, "server_create: %s", what);
   };
   if (hipMalloc(&sv->arena, sv->budget) != hipSuccess) return fail("arena allocation failed");
-  if (hipMalloc(&sv->d_states, sizeof(CgState<double>) * sv->max_active) != hipSuccess ||
+  if (hipMalloc(&sv->d_states, sizeof(SlotState) * sv->max_active) != hipSuccess ||
       hipMalloc(&sv->d_sys, sizeof(SysDesc) * sv->max_active) != hipSuccess ||
       hipMalloc(&sv->d_bad, sizeof(unsigned int) * sv->max_active) != hipSuccess ||
-      hipHostMalloc(&sv->h_states, sizeof(CgState<double>) * sv->max_active) != hipSuccess)
+      hipHostMalloc(&sv->h_states, sizeof(SlotState) * sv->max_active) != hipSuccess)
     return fail("table allocation failed");
+  std::memset(sv->h_states, 0, sizeof(SlotState) * sv->max_active);
   sv->alloc.reset(sv->budget);
   sv->slot_job.assign(sv->max_active, -1);
   if (hflmp_fill_zero(ctx, sv->d_bad, sizeof(unsigned int) * sv->max_active))
@@ -610,25 +957,31 @@ void hflmp_server_destroy(hflmp_server *sv) {
   delete sv;
 }
 
-size_t hflmp_server_footprint(int nrow, int nnz, const int *row_ptr) {
-  if (nrow <= 0 || nnz < 0 || !row_ptr) return 0;
+size_t hflmp_server_footprint(int nrow, int nnz, const int *row_ptr, const int *col,
+                              int precision) {
+  if (nrow <= 0 || nnz < 0 || !row_ptr || (precision != 64 && precision != 32)) return 0;
   Job jb;
   jb.nrow = nrow;
   jb.nnz = nnz;
   jb.rp = row_ptr;
+  jb.ci = col;
+  jb.mode = precision;
   plan_job(jb);
   return jb.bytes;
 }
 
 int hflmp_server_submit(hflmp_server *sv, int nrow, int nnz, const int *row_ptr,
                         const int *col, const double *val, const double *b, double *x,
-                        double atol, double rtol, int maxit, long long *ticket) {
+                        double atol, double rtol, int maxit, int precision,
+                        long long *ticket) {
   if (!sv) return HFLMP_EINVAL;
   hflmp_ctx *ctx = sv->ctx;
   HFLMP_REQUIRE(ctx, nrow > 0 && nnz >= 0 && row_ptr && b && x && ticket &&
                          (nnz == 0 || (col && val)),
                 "server_submit: bad system");
   HFLMP_REQUIRE(ctx, maxit > 0 && atol >= 0 && rtol >= 0, "server_submit: bad tolerances");
+  HFLMP_REQUIRE(ctx, precision == 0 || precision == 64 || precision == 32,
+                "server_submit: precision must be 0 (by size), 64 or 32");
   HFLMP_REQUIRE(ctx, row_ptr[0] == 0 && row_ptr[nrow] == nnz,
                 "server_submit: row_ptr does not span nnz");
   for (int r = 0; r < nrow; ++r)
@@ -647,6 +1000,9 @@ int hflmp_server_submit(hflmp_server *sv, int nrow, int nnz, const int *row_ptr,
   jb.atol = atol;
   jb.rtol = rtol;
   jb.maxit = maxit;
+  if (precision == 0)
+    precision = (sv->mixed_min_rows > 0 && nrow >= sv->mixed_min_rows) ? 32 : 64;
+  jb.mode = precision == 32 ? kModeMixed : kModeFp64;
   plan_job(jb);
   HFLMP_REQUIRE(ctx, jb.bytes <= sv->budget,
                 "server_submit: footprint %zu B exceeds the HBM budget %zu B", jb.bytes,
@@ -675,7 +1031,10 @@ int hflmp_server_step(hflmp_server *sv, int *n_done) {
     Job &jb = sv->jobs[sv->queue.front()];
     if (!sv->alloc.alloc(jb.bytes, &jb.off)) break;
     int rc = admit(sv, jb, slot);
-    if (rc) return rc;
+    if (rc) {
+      sv->alloc.release(jb.off, jb.bytes);
+      return rc;
+    }
     sv->slot_job[slot] = jb.ticket;
     sv->order.push_back(slot);
     sv->queue.pop_front();
@@ -701,15 +1060,15 @@ int hflmp_server_step(hflmp_server *sv, int *n_done) {
     }
     ++sv->steps;
     HFLMP_HIP_CHECK(ctx, hipMemcpyAsync(sv->h_states, sv->d_states,
-                                        sizeof(CgState<double>) * sv->max_active,
+                                        sizeof(SlotState) * sv->max_active,
                                         hipMemcpyDeviceToHost, s));
     HFLMP_HIP_CHECK(ctx, hipStreamSynchronize(s));
     // retire the systems that stopped
     std::vector<int> keep;
     for (int slot : sv->order) {
-      const CgState<double> &st = sv->h_states[slot];
+      const SlotState &st = sv->h_states[slot];
       Job &jb = sv->jobs[sv->slot_job[slot]];
-      if (st.status == kIterate || st.status == kFresh) {
+      if (slot_running(st)) {
         keep.push_back(slot);
         continue;
       }
@@ -721,10 +1080,19 @@ int hflmp_server_step(hflmp_server *sv, int *n_done) {
         const int rc = hflmp_fill_zero(ctx, sv->d_bad + slot, sizeof(bad));
         if (rc) return rc;
       }
-      jb.iters = st.iter;
-      jb.status = st.status;
-      jb.res_init = st.res0;
-      jb.res_final = st.res;
+      if (st.mode == kModeMixed) {
+        jb.iters = st.inner_total;
+        jb.status = st.status;
+        jb.outer = st.outer;
+        jb.res_init = st.res0;
+        jb.res_final = st.res;
+      } else {
+        jb.iters = st.s64.iter;
+        jb.status = st.s64.status;
+        jb.outer = 0;
+        jb.res_init = st.s64.res0;
+        jb.res_final = st.s64.res;
+      }
       jb.bad_diag = (int)bad;
       jb.t_done = Clock::now();
       jb.state = kDone;
@@ -757,6 +1125,8 @@ int hflmp_server_poll(hflmp_server *sv, long long ticket, hflmp_server_result *r
   };
   res->queued_seconds = jb.state >= kActive ? sec(jb.t_submit, jb.t_admit) : -1.0;
   res->latency_seconds = jb.state == kDone ? sec(jb.t_submit, jb.t_done) : -1.0;
+  res->precision = jb.mode;
+  res->corrections = jb.outer;
   return HFLMP_OK;
 }
 

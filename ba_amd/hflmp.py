@@ -117,8 +117,8 @@ def hip_lib():
         L.hflmp_server_destroy.argtypes = [vp]
         L.hflmp_server_destroy.restype = None
         L.hflmp_server_footprint.restype = sz
-        L.hflmp_server_footprint.argtypes = [i, i, vp]
-        L.hflmp_server_submit.argtypes = [vp, i, i, vp, vp, vp, vp, vp, d, d, i, vp]
+        L.hflmp_server_footprint.argtypes = [i, i, vp, vp, i]
+        L.hflmp_server_submit.argtypes = [vp, i, i, vp, vp, vp, vp, vp, d, d, i, i, vp]
         L.hflmp_server_step.argtypes = [vp, vp]
         L.hflmp_server_poll.argtypes = [vp, C.c_longlong, vp]
         L.hflmp_server_release.argtypes = [vp, C.c_longlong]
@@ -534,14 +534,15 @@ def mpcg_jacobi_sell(ctx, S64, S32, invd32, b, x, max_outer=50, atol=0.0, rtol=1
 
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget", C.c_size_t), ("max_active", C.c_int), ("chunk_iters", C.c_int),
-                ("use_graphs", C.c_int)]
+                ("use_graphs", C.c_int), ("mixed_min_rows", C.c_int), ("inner_rtol", C.c_double)]
 
 
 class ServerResult(C.Structure):
     _fields_ = [("state", C.c_int), ("iters", C.c_int), ("status", C.c_int),
                 ("res_init", C.c_double), ("res_final", C.c_double),
                 ("zero_diagonals", C.c_int), ("footprint", C.c_size_t),
-                ("queued_seconds", C.c_double), ("latency_seconds", C.c_double)]
+                ("queued_seconds", C.c_double), ("latency_seconds", C.c_double),
+                ("precision", C.c_int), ("corrections", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -556,9 +557,11 @@ class ServerStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
-def server_footprint(rp, nnz):
+def server_footprint(rp, ci, precision=64):
     rp = np.ascontiguousarray(rp, np.int32)
-    return hip_lib().hflmp_server_footprint(len(rp) - 1, int(nnz), rp.ctypes.data)
+    ci = np.ascontiguousarray(ci, np.int32)
+    return hip_lib().hflmp_server_footprint(len(rp) - 1, len(ci), rp.ctypes.data,
+                                            ci.ctypes.data, int(precision))
 
 
 class Server:
@@ -566,10 +569,11 @@ class Server:
     systems, step, collect x.  Keeps each request's host arrays alive until
     its result is taken."""
 
-    def __init__(self, ctx, hbm_budget, max_active=64, chunk_iters=16, use_graphs=True):
+    def __init__(self, ctx, hbm_budget, max_active=64, chunk_iters=16, use_graphs=True,
+                 mixed_min_rows=0, inner_rtol=1e-4):
         self.ctx = ctx
         cfg = ServerConfig(int(hbm_budget), int(max_active), int(chunk_iters),
-                           1 if use_graphs else 0)
+                           1 if use_graphs else 0, int(mixed_min_rows), float(inner_rtol))
         self.h = C.c_void_p()
         ctx.check(ctx.L.hflmp_server_create(ctx.h, C.byref(cfg), C.byref(self.h)),
                   "server_create")
@@ -579,7 +583,9 @@ class Server:
         if rc:
             self.ctx.check(rc, what)
 
-    def submit(self, rp, ci, v, b, rtol=1e-8, atol=0.0, maxit=100000):
+    def submit(self, rp, ci, v, b, rtol=1e-8, atol=0.0, maxit=100000, precision=64):
+        """precision 64 (FP64 CG), 32 (FP32-inner / FP64-refined) or 0 (by
+        size: mixed from mixed_min_rows rows)."""
         rp = np.ascontiguousarray(rp, np.int32)
         ci = np.ascontiguousarray(ci, np.int32)
         v = np.ascontiguousarray(v, np.float64)
@@ -588,7 +594,8 @@ class Server:
         t = C.c_longlong()
         self._check(self.ctx.L.hflmp_server_submit(
             self.h, len(rp) - 1, len(ci), rp.ctypes.data, ci.ctypes.data, v.ctypes.data,
-            b.ctypes.data, x.ctypes.data, float(atol), float(rtol), int(maxit), C.byref(t)),
+            b.ctypes.data, x.ctypes.data, float(atol), float(rtol), int(maxit), int(precision),
+            C.byref(t)),
             "server_submit")
         self.keep[t.value] = (rp, ci, v, b, x)
         return t.value

@@ -305,11 +305,11 @@ def fifo_latencies(arrivals, service):
     return np.array(out)
 
 
-def run_server(hflmp, ctx, reqs, arrivals, cfg, rtol):
+def run_server(hflmp, ctx, reqs, arrivals, cfg, rtol, precision=64):
     """Replay requests into a server at their arrival times (all at t = 0
     when arrivals is None); returns per-request (x, info, latency)."""
     sv = hflmp.Server(ctx, cfg["hbm_budget"], cfg["max_active"], cfg["chunk_iters"],
-                      cfg["use_graphs"])
+                      cfg["use_graphs"], cfg.get("mixed_min_rows", 0), cfg.get("inner_rtol", 1e-4))
     try:
         n = len(reqs)
         tickets, lat, out = [None] * n, [0.0] * n, [None] * n
@@ -320,7 +320,7 @@ def run_server(hflmp, ctx, reqs, arrivals, cfg, rtol):
         while nxt < n or outstanding:
             now = time.perf_counter() - t0
             while nxt < n and (arrivals is None or arrivals[nxt] <= now):
-                t = sv.submit(*reqs[nxt], rtol=rtol)
+                t = sv.submit(*reqs[nxt], rtol=rtol, precision=precision)
                 tickets[nxt] = t
                 by_ticket[t] = nxt
                 outstanding.add(t)
@@ -344,22 +344,29 @@ def run_server(hflmp, ctx, reqs, arrivals, cfg, rtol):
     return out, np.array(lat), makespan, stats
 
 
-def bench_serve(hflmp, nreq, seed, rate, ref_style):
-    """C5: a stream of FE solves (FP64 Jacobi-CG to 1e-8) through the solve
-    server vs one-at-a-time device-resident CG (and the reference's
+def bench_serve(hflmp, nreq, seed, rate, ref_style, graphs=True, ntune=32):
+    """C5: a stream of FE solves (Jacobi-CG to 1e-8) through the solve
+    server vs one-at-a-time device-resident FP64 CG (and the reference's
     host-driven CG<LAD> on this platform), throughput with every request
-    present at t = 0, latency under Poisson arrivals."""
+    present at t = 0, latency under Poisson arrivals.  The server's
+    precision per size class and its step length come from the autotuner
+    run on a separate tuning sample; an FP64-only server run is reported
+    beside it."""
     from concurrent.futures import ThreadPoolExecutor
+    from ba_amd import serve_tune
+    from oracle import oracle as O
     rtol = 1e-8
     specs, gaps = serve_stream(nreq, seed)
+    tspecs, _ = serve_stream(ntune, seed + 1)
     t0 = time.perf_counter()
     with ThreadPoolExecutor(8) as ex:
         reqs = list(ex.map(lambda s: make_request(hflmp, s), specs))
+        treqs = list(ex.map(lambda s: make_request(hflmp, s), tspecs))
     t_gen = time.perf_counter() - t0
     total_nnz = int(sum(len(r[1]) for r in reqs))
     ctx = hflmp.Context(0)
     # one at a time, device-resident CG (upload, SELL-64 build, Jacobi, solve, download)
-    service, singles = [], []
+    service, singles, solve_s = [], [], 0.0
     t0 = time.perf_counter()
     for rp, ci, v, b in reqs:
         ts = time.perf_counter()
@@ -368,29 +375,57 @@ def bench_serve(hflmp, nreq, seed, rate, ref_style):
         invd = hflmp.invdiag(ctx, A)
         db, dx = ctx.upload(b), ctx.zeros(A.nrow, np.float64)
         info = hflmp.pcg_jacobi_sell(ctx, S, invd, db, dx, rtol=rtol, maxit=100000)
+        solve_s += info["seconds"]
         x = dx.download()
         S.free(); A.free(); invd.free(); db.free(); dx.free()
         service.append(time.perf_counter() - ts)
         singles.append((x, info))
     t_single = time.perf_counter() - t0
-    cfg = {"hbm_budget": 32 << 30, "max_active": 256, "chunk_iters": 16, "use_graphs": True}
-    out, _, makespan, stats = run_server(hflmp, ctx, reqs, None, cfg, rtol)
+    base = {"hbm_budget": 32 << 30, "max_active": 256, "chunk_iters": 16, "use_graphs": graphs,
+            "mixed_min_rows": 0, "inner_rtol": 1e-4}
+    t0 = time.perf_counter()
+    cfg, tune_report = serve_tune.tune(ctx, treqs, base, rtol)
+    t_tune = time.perf_counter() - t0
+    out64, _, makespan64, _ = run_server(hflmp, ctx, reqs, None, base, rtol, 64)
+    out, _, makespan, stats = run_server(hflmp, ctx, reqs, None, cfg, rtol, 0)
+    true_res = [float(np.linalg.norm(b - O.spmv(rp, ci, v, x)) / max(np.linalg.norm(b), 1e-300))
+                for (rp, ci, v, b), (x, _) in zip(reqs, out)]
+    err_auto = [float(np.linalg.norm(x - xs) / max(np.linalg.norm(xs), 1e-300))
+                for (x, _), (xs, _) in zip(out, singles)]
     errs = [float(np.linalg.norm(x - xs) / max(np.linalg.norm(xs), 1e-300))
-            for (x, _), (xs, _) in zip(out, singles)]
-    itd = [abs(i["iters"] - s["iters"]) for (_, i), (_, s) in zip(out, singles)]
+            for (x, _), (xs, _) in zip(out64, singles)]
+    itd = [abs(i["iters"] - s["iters"]) for (_, i), (_, s) in zip(out64, singles)]
+    # algorithmic bytes of every FP64 CG iteration of the stream (fused iteration)
+    moved = sum(i["iters"] * cg_iter_bytes(len(r[0]) - 1, len(r[1]))
+                for (_, i), r in zip(out64, reqs))
+    nmixed = sum(1 for _, i in out if i["precision"] == 32)
     res = {"stream": {"requests": nreq, "seed": seed, "rtol": rtol, "precision": "fp64",
                       "mix": "70% 2D Q1 n log-uniform [16,1024], 30% 3D Q1 n log-uniform "
                              "[16,96]; log-normal diffusivity sigma 1; uniform RHS",
-                      "total_nnz": total_nnz, "gen_seconds": round(t_gen, 2)},
+                      "total_nnz": total_nnz, "gen_seconds": round(t_gen, 2),
+                      "fp64_iterations_total": int(sum(i["iters"] for _, i in out64)),
+                      "fp64_cg_bytes_total": int(moved)},
+           "autotune": {"sample": "%d requests of the same mix, seed %d" % (ntune, seed + 1),
+                        "seconds": round(t_tune, 2), "report": tune_report},
            "server": {"config": cfg, "solves_per_s": round(nreq / makespan, 2),
                       "makespan_s": round(makespan, 3), "steps": stats["steps"],
                       "graph_captures": stats["graph_captures"],
-                      "all_converged": all(i["status"] == 0 for _, i in out)},
+                      "mixed_requests": nmixed,
+                      "all_converged": all(i["status"] == 0 for _, i in out),
+                      "max_true_rel_residual": max(true_res),
+                      "max_rel_diff_x_vs_fp64_single": max(err_auto)},
+           "server_fp64_only": {"config": base, "solves_per_s": round(nreq / makespan64, 2),
+                                "makespan_s": round(makespan64, 3),
+                                "effective_GBps": round(moved / makespan64 / 1e9, 1),
+                                "all_converged": all(i["status"] == 0 for _, i in out64)},
            "one_at_a_time_device": {"solves_per_s": round(nreq / t_single, 2),
                                     "makespan_s": round(t_single, 3),
+                                    "solve_loop_seconds": round(solve_s, 3),
+                                    "effective_GBps": round(moved / t_single / 1e9, 1),
                                     "all_converged": all(s["status"] == 0 for _, s in singles)},
-           "server_vs_single": {"max_rel_diff_x": max(errs), "max_iter_diff": max(itd)}}
+           "server_fp64_vs_single": {"max_rel_diff_x": max(errs), "max_iter_diff": max(itd)}}
     res["speedup_vs_one_at_a_time_device"] = round(t_single / makespan, 3)
+    res["speedup_fp64_server_vs_one_at_a_time_device"] = round(t_single / makespan64, 3)
     if ref_style:
         t0 = time.perf_counter()
         ok = True
@@ -411,7 +446,7 @@ def bench_serve(hflmp, nreq, seed, rate, ref_style):
     # latency under Poisson arrivals at `rate` (default: 60% of the server's capacity)
     lam = rate if rate else 0.6 * nreq / makespan
     arrivals = np.cumsum(gaps) / lam
-    _, lat, mk2, _ = run_server(hflmp, ctx, reqs, arrivals, cfg, rtol)
+    _, lat, mk2, _ = run_server(hflmp, ctx, reqs, arrivals, cfg, rtol, 0)
     base = fifo_latencies(arrivals, service)
     res["latency"] = {"rate_per_s": round(lam, 2),
                       "server_p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
@@ -444,6 +479,8 @@ def main():
                     help="Poisson arrival rate for the latency run (default: 60%% of the "
                          "server's measured capacity)")
     ap.add_argument("--no-serve-reference", action="store_true")
+    ap.add_argument("--serve-no-graphs", action="store_true",
+                    help="run the server's steps as direct launches instead of graph replay")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (hflmp_copy_f64), the reference "
                          "kernel for scaling the FETCH_SIZE / WRITE_SIZE counters")
@@ -558,7 +595,8 @@ def main():
         out["batched"] = bench_batched(hflmp)
     if not args.no_serve:
         out["serving"] = bench_serve(hflmp, args.serve_requests, args.serve_seed,
-                                     args.serve_rate, not args.no_serve_reference)
+                                     args.serve_rate, not args.no_serve_reference,
+                                     not args.serve_no_graphs)
 
     if not args.no_cpu:
         base, naive = cpu_baseline(rp, ci, v, x, args.cpu_seconds)

@@ -393,19 +393,24 @@ int hflmp_mpcg_jacobi_sell(hflmp_ctx *ctx, const hflmp_sell *A64,
 
 /* ---- solve server (no reference counterpart: the reference runs one
  * CG::Solve at a time, linear_solver/cg.h:165-246) -------------------------
- * Many FP64 Jacobi-PCG solves of different sizes advanced together on one
- * device.  One arena of hbm_budget bytes holds the admitted systems; a
+ * Many Jacobi-PCG solves of different sizes advanced together on one
+ * device, each in FP64 or in the FP32-inner / FP64-refined mode of
+ * hflmp_mpcg_jacobi (MPIR, linear_solver/mpir.cc:52-296: FP64 residual,
+ * correction and outer test; FP32 inner CG; at most 50 corrections).  One arena of hbm_budget bytes holds the admitted systems; a
  * submitted system waits in a first-come-first-served queue until a slot
  * is free and its footprint (hflmp_server_footprint: CSR, SELL-64 copy,
- * inverse diagonal, b, x and the CG vectors) fits.  hflmp_server_step
+ * inverse diagonal(s), b, x and the Krylov vectors) fits.  hflmp_server_step
  * admits what fits, forms new systems' initial residuals, advances every
  * active system by chunk_iters iterations in shared launches (one launch
  * per kernel for all systems, captured as a hipGraph per active set when
  * use_graphs), and retires the systems that stopped: their x is copied to
  * the caller's buffer.  Each system keeps its own scalars and partial sums
  * in a fixed order, so its result is bit-identical alone and in any mix.
- * Solves start from x0 = 0 and stop as IterateControl::Check
- * (common/iterate_control.cc:80-107).  The caller's host arrays (row_ptr,
+ * Solves start from x0 = 0.  FP64 solves stop as IterateControl::Check
+ * (common/iterate_control.cc:80-107) on the recursive residual; mixed
+ * solves stop on the FP64 true residual ||b - A x|| <= atol or
+ * < rtol ||b||, with maxit bounding each inner solve.  precision 0 at
+ * submission picks the mixed mode for nrow >= mixed_min_rows (0: never).  The caller's host arrays (row_ptr,
  * col, val, b, x) must stay valid until the ticket is done.  Rows may hold
  * 0..255 entries. */
 typedef struct hflmp_server hflmp_server;
@@ -414,6 +419,8 @@ typedef struct {
   int max_active;    /* systems advanced together, 1..4096                */
   int chunk_iters;   /* CG iterations per step, 1..256                    */
   int use_graphs;    /* 1: replay each step's launches as a hipGraph      */
+  int mixed_min_rows;   /* precision 0: mixed from this many rows (0: off) */
+  double inner_rtol;    /* mixed: inner relative tolerance (0: 1e-4)      */
 } hflmp_server_config;
 typedef struct {
   int state;              /* 0 queued, 1 active, 2 done                   */
@@ -424,6 +431,8 @@ typedef struct {
   size_t footprint;       /* arena bytes while active                      */
   double queued_seconds;  /* submit -> admission (-1 while queued)         */
   double latency_seconds; /* submit -> x in the caller's buffer (-1 before) */
+  int precision;          /* 64 or 32 (FP32-inner / FP64-refined)          */
+  int corrections;        /* mixed: outer corrections applied              */
 } hflmp_server_result;
 typedef struct {
   int queued, active, active_units;
@@ -433,11 +442,13 @@ typedef struct {
 int hflmp_server_create(hflmp_ctx *ctx, const hflmp_server_config *cfg,
                         hflmp_server **out);
 void hflmp_server_destroy(hflmp_server *sv);
-size_t hflmp_server_footprint(int nrow, int nnz, const int *row_ptr);
+/* footprint for precision 64 or 32 (col may be NULL: int32 columns assumed) */
+size_t hflmp_server_footprint(int nrow, int nnz, const int *row_ptr,
+                              const int *col, int precision);
 int hflmp_server_submit(hflmp_server *sv, int nrow, int nnz, const int *row_ptr,
                         const int *col, const double *val, const double *b,
                         double *x, double atol, double rtol, int maxit,
-                        long long *ticket);
+                        int precision, long long *ticket);
 int hflmp_server_step(hflmp_server *sv, int *n_done);
 int hflmp_server_poll(hflmp_server *sv, long long ticket,
                       hflmp_server_result *res);

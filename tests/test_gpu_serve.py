@@ -1,7 +1,9 @@
-"""The solve server: every request against the oracle's Jacobi-CG (iterations
-within 2, solution within 1e-8), each result bit-identical alone and in a
-mix (with and without graph replay), admission by HBM footprint (requests
-wait, then run, with the same results), and its edge cases."""
+"""The solve server: every FP64 request against the oracle's Jacobi-CG
+(iterations within 2, solution within 1e-8); every FP32-inner / FP64-refined
+request against its true FP64 residual and the oracle's tight solve; each
+result bit-identical alone and in a mix (with and without graph replay, in
+both precisions); admission by HBM footprint (requests wait, then run, with
+the same results); and its edge cases."""
 import numpy as np
 import pytest
 
@@ -24,7 +26,7 @@ MIX = [(2, 8, 11), (2, 40, 12), (3, 6, 13), (2, 17, 14), (3, 11, 15), (2, 64, 16
        (3, 16, 18)]
 
 
-def run_all(server, reqs, submit_every=0, rtol=1e-8):
+def run_all(server, reqs, submit_every=0, rtol=1e-8, precision=64):
     """Submit the requests (all at once, or one per `submit_every` steps
     while stepping) and step until all are done."""
     tickets, pending = [], list(reqs)
@@ -33,7 +35,7 @@ def run_all(server, reqs, submit_every=0, rtol=1e-8):
         if pending and (submit_every == 0 or steps % submit_every == 0):
             batch = pending if submit_every == 0 else pending[:1]
             for r in batch:
-                tickets.append(server.submit(*r, rtol=rtol))
+                tickets.append(server.submit(*r, rtol=rtol, precision=precision))
             pending = [] if submit_every == 0 else pending[1:]
         server.step()
         steps += 1
@@ -82,7 +84,7 @@ def test_serve_alone_equals_mix(ctx, graphs):
 
 def test_serve_admission_by_footprint(ctx):
     reqs = [request(2, 100, 40 + k) for k in range(6)]
-    foot = hflmp.server_footprint(reqs[0][0], len(reqs[0][1]))
+    foot = hflmp.server_footprint(reqs[0][0], reqs[0][1])
     # room for two systems at a time
     budget = max(2 * foot + foot // 2, MB)
     sv = hflmp.Server(ctx, budget, max_active=16, chunk_iters=16)
@@ -139,3 +141,56 @@ def test_serve_edge_cases(ctx):
             sv.submit(rp2, ci2, np.ones(len(ci2)), np.ones(n))
     finally:
         sv.close()
+
+
+def test_serve_mixed_vs_oracle(ctx):
+    reqs = [request(*m) for m in MIX]
+    sv = hflmp.Server(ctx, 256 * MB, max_active=8, chunk_iters=8)
+    try:
+        out = run_all(sv, reqs, precision=32)
+    finally:
+        sv.close()
+    for (rp, ci, v, b), (x, info) in zip(reqs, out):
+        assert info["status"] == 0 and info["precision"] == 32
+        bn = np.linalg.norm(b)
+        # the stopping test is on the FP64 true residual
+        assert np.linalg.norm(b - O.spmv(rp, ci, v, x)) < 1e-8 * bn
+        # as accurate as FP64 CG at the same tolerance, against a tight solve
+        # (plus 1e-8 relative: on the smallest systems FP64 CG is exact)
+        xt, _ = O.cg(rp, ci, v, b, prec=O.PREC_JACOBI, rtol=1e-13, maxit=200000)
+        x8, _ = O.cg(rp, ci, v, b, prec=O.PREC_JACOBI, rtol=1e-8, maxit=200000)
+        tn = np.linalg.norm(xt)
+        assert np.linalg.norm(x - xt) <= 10 * np.linalg.norm(x8 - xt) + 1e-8 * tn
+        assert info["corrections"] >= 1
+
+
+@pytest.mark.parametrize("graphs", [True, False])
+def test_serve_mixed_alone_equals_mix(ctx, graphs):
+    reqs = [request(*m) for m in MIX]
+    probe = request(2, 50, 77)
+    sv = hflmp.Server(ctx, 256 * MB, max_active=1, chunk_iters=6, use_graphs=graphs)
+    try:
+        (x_alone, i_alone), = run_all(sv, [probe], precision=32)
+    finally:
+        sv.close()
+    sv = hflmp.Server(ctx, 256 * MB, max_active=5, chunk_iters=9, use_graphs=not graphs,
+                      mixed_min_rows=500)
+    try:
+        # precision 0: the mix is FP64 below 500 rows and mixed above
+        mixed = reqs[:3] + [probe] + reqs[3:]
+        tickets, out = [], []
+        for k, r in enumerate(mixed):
+            tickets.append(sv.submit(*r, precision=32 if k == 3 else 0))
+            sv.step()
+        while any(sv.poll(t)["state"] != 2 for t in tickets):
+            sv.step()
+        out = [sv.take(t) for t in tickets]
+    finally:
+        sv.close()
+    x_mix, i_mix = out[3]
+    assert i_mix["iters"] == i_alone["iters"]
+    assert i_mix["corrections"] == i_alone["corrections"]
+    assert np.array_equal(x_mix, x_alone)
+    for (rp, ci, v, b), (x, info) in zip(mixed, out):
+        assert info["status"] == 0
+        assert info["precision"] == (32 if len(rp) - 1 >= 500 else 64)
