@@ -247,13 +247,14 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // r = b - A x in FP64; r32 = (float) r; start of the FP32 PCG from c = 0:
-// z = D32^-1 r32, p = z.  Partials: FP64 r.r; FP32 r32.r32 and r32.z.
+// z = D32^-1 r32, p = z (keep_p: p is kept for p = z + beta p, the reliable
+// update).  Partials: FP64 r.r; FP32 r32.r32 and r32.z.
 template <typename Op64>
 __global__ void __launch_bounds__(kBlock)
     mp_residual_kernel(Op64 op, const double *__restrict__ b, const double *__restrict__ x,
                        const float *__restrict__ invd32, float *__restrict__ r32,
                        float *__restrict__ z32, float *__restrict__ p32,
-                       double *__restrict__ part64, float *__restrict__ part32) {
+                       double *__restrict__ part64, float *__restrict__ part32, int keep_p) {
   __shared__ double prod[Op64::kLds];
   const long long tile = xcd_tile(blockIdx.x, gridDim.x);
   TileRow tr;
@@ -267,7 +268,7 @@ __global__ void __launch_bounds__(kBlock)
     const float zf = rf * invd32[i];
     r32[i] = rf;
     z32[i] = zf;
-    p32[i] = zf;
+    if (!keep_p) p32[i] = zf;
     rr = ri * ri;
     rr32 = rf * rf;
     rz32 = rf * zf;
@@ -279,11 +280,13 @@ __global__ void __launch_bounds__(kBlock)
 // Outer test on the FP64 residual, then the FP32 PCG's initial state.  The
 // inner relative tolerance is inner_rtol, relaxed to what the outer target
 // still needs: max(inner_rtol, 0.5 * rtol * res_0 / res), so the last
-// correction does not solve further than the outer test requires.
+// correction does not solve further than the outer test requires.  keep_p:
+// the inner CG continues from the replaced residual with
+// beta = (r32.z) / (r.z of its last iteration) instead of restarting.
 __global__ void __launch_bounds__(kFinalBlock)
     mp_residual_final(MpState *ms, CgState<float> *st, const double *__restrict__ part64,
                       const float *__restrict__ part32, int count, int outer, int max_outer,
-                      double atol, double rtol, double inner_rtol) {
+                      double atol, double rtol, double inner_rtol, int keep_p) {
   __shared__ double lds64[kFinalBlock / kWave];
   __shared__ float lds32[kFinalBlock / kWave];
   const double rr = sum_partials(part64, count, 1, 0, lds64);
@@ -303,6 +306,7 @@ __global__ void __launch_bounds__(kFinalBlock)
     const double tol = need > inner_rtol ? (need < 0.5 ? need : 0.5) : inner_rtol;
     st->rtol = tol;
     const float r0 = sqrtf(rr32);
+    if (keep_p) st->beta = st->rz > 0.0f ? rz32 / st->rz : 0.0f;
     st->rr = rr32;
     st->rz = rz32;
     st->res0 = r0;
@@ -354,10 +358,13 @@ inline MpLayout mp_layout(int nrow, int nnz, bool csr) {
 
 // The refinement loop over an FP64 operator (residual) and an FP32
 // operator (inner CG); see the comment above MpState.
+// reliable: the outer steps after the first replace the inner CG's residual
+// and keep its search direction (no restart) unless the inner solve broke
+// down.
 template <typename Op64, typename Op32>
 int mpcg_run(hflmp_ctx *ctx, const Op64 &op64, const Op32 &op32, const MpLayout &L,
              const float *invd32, const double *b, double *x, int max_outer, double atol,
-             double rtol, int inner_maxit, double inner_rtol, void *work,
+             double rtol, int inner_maxit, double inner_rtol, bool reliable, void *work,
              hflmp_mp_info *info, std::chrono::steady_clock::time_point t0) {
   const int nrow = op64.nrow;
   const int grid64 = op_grid(op64), grid32 = op_grid(op32);
@@ -375,12 +382,16 @@ int mpcg_run(hflmp_ctx *ctx, const Op64 &op64, const Op32 &op32, const MpLayout 
   const unsigned g3 = grid_for(nrow, 4);
   int *host_status = reinterpret_cast<int *>(static_cast<char *>(ctx->host_pinned) + 2048);
   int outer = 0, inner_total = 0;
+  bool last_breakdown = false;
   for (;; ++outer) {
-    init_state_kernel<float><<<1, kBlock, 0, s>>>(st, inner_maxit, 0.0, inner_rtol, 1e30);
+    const int keep_p = reliable && outer > 0 && !last_breakdown;
+    if (!keep_p)
+      init_state_kernel<float><<<1, kBlock, 0, s>>>(st, inner_maxit, 0.0, inner_rtol, 1e30);
     mp_residual_kernel<Op64><<<grid64, kBlock, 0, s>>>(op64, b, x, invd32, r, z, p, part64,
-                                                       part32);
+                                                       part32, keep_p);
     mp_residual_final<<<1, kFinalBlock, 0, s>>>(ms, st, part64, part32, grid64, outer,
-                                                max_outer, atol, rtol, inner_rtol);
+                                                max_outer, atol, rtol, inner_rtol, keep_p);
+    if (keep_p) cg_p_kernel<float><<<g3, kBlock, 0, s>>>(nrow, z, p, st);
     HFLMP_LAUNCH_CHECK(ctx);
     HFLMP_HIP_CHECK(ctx, hipMemcpyAsync(host_status, &ms->status, sizeof(int),
                                         hipMemcpyDeviceToHost, s));
@@ -404,6 +415,7 @@ int mpcg_run(hflmp_ctx *ctx, const Op64 &op64, const Op32 &op32, const MpLayout 
     HFLMP_HIP_CHECK(ctx, hipMemcpyAsync(&h, st, sizeof(h), hipMemcpyDeviceToHost, s));
     HFLMP_HIP_CHECK(ctx, hipStreamSynchronize(s));
     inner_total += h.iter;
+    last_breakdown = h.status == kBreakdown;
   }
   MpState hm;
   HFLMP_HIP_CHECK(ctx, hipMemcpyAsync(&hm, ms, sizeof(hm), hipMemcpyDeviceToHost, s));
@@ -466,16 +478,16 @@ int mpcg_jacobi(hflmp_ctx *ctx, int nrow, int nnz, const int *rp, const int *ci,
     op16.ci = c16;
     op16.val = val32;
     return mpcg_run(ctx, op64, op16, L, invd32, b, x, max_outer, atol, rtol, inner_maxit,
-                    inner_rtol, work, info, t0);
+                    inner_rtol, (flags & HFLMP_MP_RELIABLE) != 0, work, info, t0);
   }
   return mpcg_run(ctx, op64, op32, L, invd32, b, x, max_outer, atol, rtol, inner_maxit,
-                  inner_rtol, work, info, t0);
+                  inner_rtol, (flags & HFLMP_MP_RELIABLE) != 0, work, info, t0);
 }
 
 int mpcg_jacobi_sell(hflmp_ctx *ctx, const hflmp_sell *A64, const hflmp_sell *A32,
                      const float *invd32, const double *b, double *x, int max_outer,
-                     double atol, double rtol, int inner_maxit, double inner_rtol, void *work,
-                     size_t work_bytes, hflmp_mp_info *info) {
+                     double atol, double rtol, int inner_maxit, double inner_rtol, int flags,
+                     void *work, size_t work_bytes, hflmp_mp_info *info) {
   if (!ctx) return HFLMP_EINVAL;
   HFLMP_REQUIRE(ctx, A64 && A32 && invd32 && b && x && work && info, "mpcg: null pointer");
   HFLMP_REQUIRE(ctx, A64->value_bytes == 8 && A32->value_bytes == 4,
@@ -488,22 +500,24 @@ int mpcg_jacobi_sell(hflmp_ctx *ctx, const hflmp_sell *A64, const hflmp_sell *A3
   const MpLayout L = mp_layout(A64->nrow, 0, false);
   HFLMP_REQUIRE(ctx, work_bytes >= L.total, "mpcg: workspace %zu < %zu bytes", work_bytes,
                 L.total);
+  HFLMP_REQUIRE(ctx, (flags & ~HFLMP_MP_RELIABLE) == 0, "mpcg: unknown flags %d", flags);
+  const bool rel = (flags & HFLMP_MP_RELIABLE) != 0;
   const auto t0 = std::chrono::steady_clock::now();
   // four instantiations: int32 or int16 columns in each precision
   if (A64->col_bytes == 2) {
     const auto o64 = sell_view<double, short>(A64);
     if (A32->col_bytes == 2)
       return mpcg_run(ctx, o64, sell_view<float, short>(A32), L, invd32, b, x, max_outer,
-                        atol, rtol, inner_maxit, inner_rtol, work, info, t0);
+                        atol, rtol, inner_maxit, inner_rtol, rel, work, info, t0);
     return mpcg_run(ctx, o64, sell_view<float, int>(A32), L, invd32, b, x, max_outer, atol,
-                      rtol, inner_maxit, inner_rtol, work, info, t0);
+                      rtol, inner_maxit, inner_rtol, rel, work, info, t0);
   }
   const auto o64 = sell_view<double, int>(A64);
   if (A32->col_bytes == 2)
     return mpcg_run(ctx, o64, sell_view<float, short>(A32), L, invd32, b, x, max_outer, atol,
-                      rtol, inner_maxit, inner_rtol, work, info, t0);
+                      rtol, inner_maxit, inner_rtol, rel, work, info, t0);
   return mpcg_run(ctx, o64, sell_view<float, int>(A32), L, invd32, b, x, max_outer, atol,
-                    rtol, inner_maxit, inner_rtol, work, info, t0);
+                    rtol, inner_maxit, inner_rtol, rel, work, info, t0);
 }
 
 template <typename T>
@@ -579,9 +593,9 @@ size_t hflmp_mpcg_sell_workspace_bytes(int nrow) {
 int hflmp_mpcg_jacobi_sell(hflmp_ctx *ctx, const hflmp_sell *A64, const hflmp_sell *A32,
                            const float *invdiag32, const double *b, double *x, int max_outer,
                            double atol, double rtol, int inner_maxit, double inner_rtol,
-                           void *work, size_t work_bytes, hflmp_mp_info *info) {
+                           int flags, void *work, size_t work_bytes, hflmp_mp_info *info) {
   return mpcg_jacobi_sell(ctx, A64, A32, invdiag32, b, x, max_outer, atol, rtol, inner_maxit,
-                          inner_rtol, work, work_bytes, info);
+                          inner_rtol, flags, work, work_bytes, info);
 }
 
 int hflmp_mpcg_jacobi(hflmp_ctx *ctx, int nrow, int nnz, const int *rp, const int *ci,

@@ -23,11 +23,19 @@
 //
 // Steps.  hflmp_server_step admits what fits, then runs one step: the
 // initial residuals of new FP64 systems, the FP64 outer step (x += c,
-// r = b - A x, r32 = (float) r, outer test, next inner start) of the mixed
-// systems whose inner solve has stopped, and `chunk` iterations -- captured
-// once per active set as a hipGraph and replayed while the set is unchanged.
-// It then reads the states and retires the systems that stopped (x copied
-// to the caller's buffer).
+// r = b - A x, r32 = (float) r, outer test) of the mixed systems whose inner
+// solve has reached its tolerance, and `chunk` iterations -- captured once
+// per active set as a hipGraph and replayed while the set is unchanged.  It
+// then reads the states and retires the systems that stopped (x copied to
+// the caller's buffer).
+//
+// Mixed systems use reliable updates: the outer step replaces the inner
+// CG's recursive residual by the FP64 true residual and the correction c
+// is folded into x, but the search direction is kept (beta formed from the
+// replaced residual, p = z + beta p), so the Krylov space built so far is
+// not thrown away as a restart would (the reference's MPIR restarts its
+// inner solver).  Iteration counts stay close to FP64 CG's while every
+// inner iteration moves FP32 bytes.
 #include <algorithm>
 #include <chrono>
 #include <cstring>
@@ -53,7 +61,7 @@ struct SlotState {
   double atol, rtol, inner_rtol;
   int mode;
   int status;  // mixed: kFresh, kIterate or the final status
-  int phase;   // mixed: 1 while the inner CG runs, 0 when an outer step is due
+  int phase;   // mixed: 0 outer step due, 1 inner CG running, 2 its p to refresh first
   int outer, max_outer, inner_total, inner_maxit;
 };
 
@@ -183,7 +191,7 @@ __device__ __forceinline__ bool mixed_outer_due(const SlotState *st) {
 }
 
 __device__ __forceinline__ bool mixed_inner_live(const SlotState *st) {
-  return st->mode == kModeMixed && st->status == kIterate && st->phase == 1 &&
+  return st->mode == kModeMixed && st->status == kIterate && st->phase >= 1 &&
          st->s32.status == kIterate;
 }
 
@@ -241,8 +249,9 @@ __global__ void __launch_bounds__(kBlock)
   if (i < d.nrow) d.x[i] = fresh ? 0.0 : d.x[i] + (double)d.c32[i];
 }
 
-// r = b - A x in FP64; r32 = (float) r; the inner start from c = 0:
-// z32 = D32^-1 r32, p32 = z32.  Partials: FP64 r.r; FP32 r32.r32 and r32.z32.
+// r = b - A x in FP64; r32 = (float) r, z32 = D32^-1 r32, c = 0; a fresh
+// system starts with p32 = z32 (later steps keep p32 for the refresh).
+// Partials: FP64 r.r; FP32 r32.r32 and r32.z32.
 __global__ void __launch_bounds__(kBlock)
     grp_mix_residual_kernel(const SysDesc *__restrict__ sys, const int *__restrict__ unit_sys,
                             int U) {
@@ -250,6 +259,7 @@ __global__ void __launch_bounds__(kBlock)
   int j;
   if (!unit_of(sys, unit_sys, U, d, j)) return;
   if (!mixed_outer_due(d.st)) return;
+  const bool fresh = d.st->status == kFresh;
   TileRow tr;
   const double s = op64(d).row_sum(j, d.x, 0.0, tr, nullptr);
   double rr = 0;
@@ -261,7 +271,7 @@ __global__ void __launch_bounds__(kBlock)
     const float zf = rf * d.invd32[i];
     d.r32[i] = rf;
     d.z32[i] = zf;
-    d.p32[i] = zf;
+    if (fresh) d.p32[i] = zf;
     d.c32[i] = 0.0f;
     rr = ri * ri;
     rr32 = rf * rf;
@@ -274,7 +284,10 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // Outer test on the FP64 residual (initial residual kept as r_0), then the
-// inner start with tolerance max(inner_rtol, 0.5 rtol r_0 / r), capped at 0.5.
+// inner CG continues from the replaced residual with tolerance
+// max(inner_rtol, 0.5 rtol r_0 / r), capped at 0.5, relative to it:
+// beta = (r32.z32) / (r.z of the last inner iteration), or 0 (a restart,
+// p = z) for a fresh system or after an inner breakdown.
 __global__ void __launch_bounds__(kBlock) grp_mix_residual_final(const SysDesc *__restrict__ sys) {
   const SysDesc d = sys[blockIdx.x];
   SlotState *st = d.st;
@@ -299,20 +312,36 @@ __global__ void __launch_bounds__(kBlock) grp_mix_residual_final(const SysDesc *
       const double tol = need > st->inner_rtol ? (need < 0.5 ? need : 0.5) : st->inner_rtol;
       CgState<float> &s = st->s32;
       const float r0 = sqrtf(rr32);
+      const bool restart = fresh || s.status == kBreakdown || !(s.rz > 0.0f);
+      s.beta = restart ? 0.0f : rz32 / s.rz;
       s.rz = rz32;
       s.rr = rr32;
       s.res0 = r0;
       s.res = r0;
-      s.alpha = s.beta = s.pAp = 0.0f;
+      s.alpha = s.pAp = 0.0f;
       s.iter = 0;
       s.maxit = st->inner_maxit;
       s.atol = 0.0;
       s.rtol = tol;
       s.dtol = 1e30;
       s.status = check_state(0, (double)r0, (double)r0, s.maxit, s.atol, s.rtol, s.dtol);
-      st->phase = s.status == kIterate ? 1 : 0;
+      st->phase = s.status == kIterate ? (fresh ? 1 : 2) : 0;
     }
   }
+}
+
+// p32 = beta p32 + z32 after an outer step (p32 = z32 on a restart)
+__global__ void __launch_bounds__(kBlock)
+    grp_mix_prefresh_kernel(const SysDesc *__restrict__ sys, const int *__restrict__ unit_sys,
+                            int U) {
+  SysDesc d;
+  int j;
+  if (!unit_of(sys, unit_sys, U, d, j)) return;
+  const SlotState *st = d.st;
+  if (st->mode != kModeMixed || st->status != kIterate || st->phase != 2) return;
+  const float beta = st->s32.beta;
+  const long long i = (long long)j * kBlock + threadIdx.x;
+  if (i < d.nrow) d.p32[i] = beta == 0.0f ? d.z32[i] : beta * d.p32[i] + d.z32[i];
 }
 
 // ---- the iteration, both precisions -------------------------------------------
@@ -362,6 +391,7 @@ __global__ void __launch_bounds__(kBlock) grp_alpha_final(const SysDesc *__restr
     CgState<float> *st = &d.st->s32;
     const float pAp = units_sum(part32(d), d.nunits, 1, 0);
     if (threadIdx.x == 0) {
+      d.st->phase = 1;  // the refreshed p has been used
       st->pAp = pAp;
       st->alpha = st->rz / pAp;
       if (pAp == 0.0f || !(pAp == pAp)) {
@@ -700,8 +730,8 @@ struct hflmp_server {
   size_t budget = 0;
   char *arena = nullptr;
   Arena alloc;
-  int max_active = 0, chunk = 0, use_graph = 1, mixed_min_rows = 0;
-  double inner_rtol = 1e-4;
+  int max_active = 0, chunk = 0, use_graph = 1, mixed_min_nnz = 0;
+  double inner_rtol = 1e-2;
   SlotState *d_states = nullptr;  // [max_active]
   SlotState *h_states = nullptr;  // pinned mirror
   SysDesc *d_sys = nullptr;       // [max_active]
@@ -743,6 +773,7 @@ void enqueue_chunk(hflmp_server *sv, hipStream_t s) {
   grp_mix_correct_kernel<<<gU, kBlock, 0, s>>>(sv->d_sys, sv->d_unit_sys, sv->U);
   grp_mix_residual_kernel<<<gU, kBlock, 0, s>>>(sv->d_sys, sv->d_unit_sys, sv->U);
   grp_mix_residual_final<<<gS, kBlock, 0, s>>>(sv->d_sys);
+  grp_mix_prefresh_kernel<<<gU, kBlock, 0, s>>>(sv->d_sys, sv->d_unit_sys, sv->U);
   for (int it = 0; it < sv->chunk; ++it) {
     grp_spmv_kernel<<<gU, kBlock, 0, s>>>(sv->d_sys, sv->d_unit_sys, sv->U);
     grp_alpha_final<<<gS, kBlock, 0, s>>>(sv->d_sys);
@@ -913,18 +944,21 @@ int hflmp_server_create(hflmp_ctx *ctx, const hflmp_server_config *cfg, hflmp_se
   *out = nullptr;
   HFLMP_REQUIRE(ctx, cfg->hbm_budget >= (1u << 20) && cfg->max_active >= 1 &&
                          cfg->max_active <= 4096 && cfg->chunk_iters >= 1 &&
-                         cfg->chunk_iters <= 256 && cfg->mixed_min_rows >= 0 &&
+                         cfg->chunk_iters <= 256 && cfg->mixed_min_nnz >= 0 &&
                          cfg->inner_rtol >= 0 && cfg->inner_rtol < 1,
                 "server_create: need hbm_budget >= 1 MiB, 1 <= max_active <= 4096, "
-                "1 <= chunk_iters <= 256, mixed_min_rows >= 0, 0 <= inner_rtol < 1");
+                "1 <= chunk_iters <= 256, mixed_min_nnz >= 0, 0 <= inner_rtol < 1");
   auto *sv = new hflmp_server();
   sv->ctx = ctx;
   sv->budget = cfg->hbm_budget;
   sv->max_active = cfg->max_active;
   sv->chunk = cfg->chunk_iters;
   sv->use_graph = cfg->use_graphs ? 1 : 0;
-  sv->mixed_min_rows = cfg->mixed_min_rows;
-  sv->inner_rtol = cfg->inner_rtol > 0 ? cfg->inner_rtol : 1e-4;
+  sv->mixed_min_nnz = cfg->mixed_min_nnz;
+  // FP32 CG driven far below its starting residual drifts from the true
+  // residual on badly conditioned systems; frequent replacement keeps the
+  // reliable updates effective
+  sv->inner_rtol = cfg->inner_rtol > 0 ? cfg->inner_rtol : 1e-2;
   auto fail = [&](const char *what) {
     hflmp_server_destroy(sv);
     return set_error(ctx, HFLMP_ENOMEM, "server_create: %s", what);
@@ -1001,7 +1035,7 @@ int hflmp_server_submit(hflmp_server *sv, int nrow, int nnz, const int *row_ptr,
   jb.rtol = rtol;
   jb.maxit = maxit;
   if (precision == 0)
-    precision = (sv->mixed_min_rows > 0 && nrow >= sv->mixed_min_rows) ? 32 : 64;
+    precision = (sv->mixed_min_nnz > 0 && nnz >= sv->mixed_min_nnz) ? 32 : 64;
   jb.mode = precision == 32 ? kModeMixed : kModeFp64;
   plan_job(jb);
   HFLMP_REQUIRE(ctx, jb.bytes <= sv->budget,

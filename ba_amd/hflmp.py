@@ -112,7 +112,7 @@ def hip_lib():
         L.hflmp_sell_destroy.restype = None
         L.hflmp_mpcg_sell_workspace_bytes.restype = sz
         L.hflmp_mpcg_sell_workspace_bytes.argtypes = [i]
-        L.hflmp_mpcg_jacobi_sell.argtypes = [vp, vp, vp, vp, vp, vp, i, d, d, i, d, vp, sz, vp]
+        L.hflmp_mpcg_jacobi_sell.argtypes = [vp, vp, vp, vp, vp, vp, i, d, d, i, d, i, vp, sz, vp]
         L.hflmp_server_create.argtypes = [vp, vp, vp]
         L.hflmp_server_destroy.argtypes = [vp]
         L.hflmp_server_destroy.restype = None
@@ -428,6 +428,7 @@ def batched_pcg_jacobi(ctx, A, invd, B, X, nrhs, maxit=10000, atol=0.0, rtol=1e-
 
 
 MP_INT32_COLS = 1
+MP_RELIABLE = 2
 
 
 def mpcg_jacobi(ctx, A, A32, invd32, b, x, max_outer=50, atol=0.0, rtol=1e-10,
@@ -515,9 +516,10 @@ def pcg_jacobi_sell(ctx, S, invd, b, x, maxit=10000, atol=0.0, rtol=1e-8, dtol=1
 
 
 def mpcg_jacobi_sell(ctx, S64, S32, invd32, b, x, max_outer=50, atol=0.0, rtol=1e-10,
-                     inner_maxit=20000, inner_rtol=1e-5, work=None):
+                     inner_maxit=20000, inner_rtol=1e-5, flags=0, work=None):
     """FP32-inner / FP64-refined Jacobi-CG with both operators in SELL-64
-    layout (S64: FP64 values, S32: FP32 values)."""
+    layout (S64: FP64 values, S32: FP32 values); flags MP_RELIABLE keeps the
+    inner search direction across corrections."""
     need = ctx.L.hflmp_mpcg_sell_workspace_bytes(S64.nrow)
     own = work is None
     if own:
@@ -525,7 +527,8 @@ def mpcg_jacobi_sell(ctx, S64, S32, invd32, b, x, max_outer=50, atol=0.0, rtol=1
     info = MpInfo()
     try:
         _call(ctx, "hflmp_mpcg_jacobi_sell", S64.h, S32.h, invd32.ptr, b.ptr, x.ptr, max_outer,
-              atol, rtol, inner_maxit, inner_rtol, work.ptr, work.nbytes, C.byref(info))
+              atol, rtol, inner_maxit, inner_rtol, int(flags), work.ptr, work.nbytes,
+              C.byref(info))
     finally:
         if own:
             work.free()
@@ -534,7 +537,7 @@ def mpcg_jacobi_sell(ctx, S64, S32, invd32, b, x, max_outer=50, atol=0.0, rtol=1
 
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget", C.c_size_t), ("max_active", C.c_int), ("chunk_iters", C.c_int),
-                ("use_graphs", C.c_int), ("mixed_min_rows", C.c_int), ("inner_rtol", C.c_double)]
+                ("use_graphs", C.c_int), ("mixed_min_nnz", C.c_int), ("inner_rtol", C.c_double)]
 
 
 class ServerResult(C.Structure):
@@ -570,10 +573,10 @@ class Server:
     its result is taken."""
 
     def __init__(self, ctx, hbm_budget, max_active=64, chunk_iters=16, use_graphs=True,
-                 mixed_min_rows=0, inner_rtol=1e-4):
+                 mixed_min_nnz=0, inner_rtol=1e-2):
         self.ctx = ctx
         cfg = ServerConfig(int(hbm_budget), int(max_active), int(chunk_iters),
-                           1 if use_graphs else 0, int(mixed_min_rows), float(inner_rtol))
+                           1 if use_graphs else 0, int(mixed_min_nnz), float(inner_rtol))
         self.h = C.c_void_p()
         ctx.check(ctx.L.hflmp_server_create(ctx.h, C.byref(cfg), C.byref(self.h)),
                   "server_create")
@@ -585,7 +588,7 @@ class Server:
 
     def submit(self, rp, ci, v, b, rtol=1e-8, atol=0.0, maxit=100000, precision=64):
         """precision 64 (FP64 CG), 32 (FP32-inner / FP64-refined) or 0 (by
-        size: mixed from mixed_min_rows rows)."""
+        size: mixed from mixed_min_nnz nonzeros)."""
         rp = np.ascontiguousarray(rp, np.int32)
         ci = np.ascontiguousarray(ci, np.int32)
         v = np.ascontiguousarray(v, np.float64)

@@ -364,6 +364,12 @@ int hflmp_batched_pcg_jacobi_f32(hflmp_ctx *ctx, int nrow, int nnz, int nrhs,
  * every offset fits, else as int32; flags HFLMP_MP_INT32_COLS forces int32.
  * The FP64 residual always uses the int32 CSR arrays. */
 #define HFLMP_MP_INT32_COLS 1
+/* Reliable updates: after the first outer step the inner CG is not
+ * restarted; its recursive residual is replaced by the FP64 true residual
+ * and its search direction kept (beta from the replaced residual), which
+ * keeps the Krylov space built so far.  Without the flag each correction
+ * restarts the inner solve, as MPIR's inner solver does. */
+#define HFLMP_MP_RELIABLE 2
 typedef struct {
   int outer_iters;  /* corrections applied                                */
   int inner_iters;  /* FP32 CG iterations over all corrections           */
@@ -382,22 +388,24 @@ int hflmp_mpcg_jacobi(hflmp_ctx *ctx, int nrow, int nnz, const int *row_ptr,
                       hflmp_mp_info *info);
 /* The same refinement with both operators in SELL-64 layout: A64 (FP64
  * values) for the residual and A32 (FP32 values, int16 columns where they
- * fit) for the inner solve.  Workspace: hflmp_mpcg_sell_workspace_bytes. */
+ * fit) for the inner solve; flags: HFLMP_MP_RELIABLE.  Workspace:
+ * hflmp_mpcg_sell_workspace_bytes. */
 size_t hflmp_mpcg_sell_workspace_bytes(int nrow);
 int hflmp_mpcg_jacobi_sell(hflmp_ctx *ctx, const hflmp_sell *A64,
                            const hflmp_sell *A32, const float *invdiag32,
                            const double *b, double *x, int max_outer,
                            double atol, double rtol, int inner_maxit,
-                           double inner_rtol, void *work, size_t work_bytes,
-                           hflmp_mp_info *info);
+                           double inner_rtol, int flags, void *work,
+                           size_t work_bytes, hflmp_mp_info *info);
 
 /* ---- solve server (no reference counterpart: the reference runs one
  * CG::Solve at a time, linear_solver/cg.h:165-246) -------------------------
  * Many Jacobi-PCG solves of different sizes advanced together on one
  * device, each in FP64 or in the FP32-inner / FP64-refined mode of
  * hflmp_mpcg_jacobi (MPIR, linear_solver/mpir.cc:52-296: FP64 residual,
- * correction and outer test; FP32 inner CG; at most 50 corrections).  One arena of hbm_budget bytes holds the admitted systems; a
- * submitted system waits in a first-come-first-served queue until a slot
+ * correction and outer test; FP32 inner CG; at most 50 corrections).
+ * One arena of hbm_budget bytes holds the admitted systems; a submitted
+ * system waits in a first-come-first-served queue until a slot
  * is free and its footprint (hflmp_server_footprint: CSR, SELL-64 copy,
  * inverse diagonal(s), b, x and the Krylov vectors) fits.  hflmp_server_step
  * admits what fits, forms new systems' initial residuals, advances every
@@ -410,17 +418,17 @@ int hflmp_mpcg_jacobi_sell(hflmp_ctx *ctx, const hflmp_sell *A64,
  * (common/iterate_control.cc:80-107) on the recursive residual; mixed
  * solves stop on the FP64 true residual ||b - A x|| <= atol or
  * < rtol ||b||, with maxit bounding each inner solve.  precision 0 at
- * submission picks the mixed mode for nrow >= mixed_min_rows (0: never).  The caller's host arrays (row_ptr,
- * col, val, b, x) must stay valid until the ticket is done.  Rows may hold
- * 0..255 entries. */
+ * submission picks the mixed mode for nnz >= mixed_min_nnz (0: never).
+ * The caller's host arrays (row_ptr, col, val, b, x) must stay valid until
+ * the ticket is done.  Rows may hold 0..255 entries. */
 typedef struct hflmp_server hflmp_server;
 typedef struct {
   size_t hbm_budget; /* device bytes for admitted systems (>= 1 MiB)       */
   int max_active;    /* systems advanced together, 1..4096                */
   int chunk_iters;   /* CG iterations per step, 1..256                    */
   int use_graphs;    /* 1: replay each step's launches as a hipGraph      */
-  int mixed_min_rows;   /* precision 0: mixed from this many rows (0: off) */
-  double inner_rtol;    /* mixed: inner relative tolerance (0: 1e-4)      */
+  int mixed_min_nnz;    /* precision 0: mixed from this many nonzeros (0: off) */
+  double inner_rtol;    /* mixed: residual drop between replacements (0: 1e-2) */
 } hflmp_server_config;
 typedef struct {
   int state;              /* 0 queued, 1 active, 2 done                   */

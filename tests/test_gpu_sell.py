@@ -147,7 +147,8 @@ def test_pcg_sell_vs_oracle(ctx, dtype, rtol):
     S.free()
 
 
-def test_mpcg_sell_reaches_target(ctx):
+@pytest.mark.parametrize("flags", [0, hflmp.MP_RELIABLE])
+def test_mpcg_sell_reaches_target(ctx, flags):
     n = 24
     rp, ci, v = hflmp.fe_system(1, 3, n, hflmp.fe_kappa(3, n, 1.0, 9))
     b = hflmp.fe_rhs(1, 3, n, 10)
@@ -158,7 +159,8 @@ def test_mpcg_sell_reaches_target(ctx):
     assert S32.info()["col_bytes"] == 2
     invd32 = hflmp.invdiag(ctx, A32)
     db, dx = ctx.upload(b), ctx.zeros(A.nrow, np.float64)
-    info = hflmp.mpcg_jacobi_sell(ctx, S64, S32, invd32, db, dx, rtol=1e-10, inner_rtol=1e-4)
+    info = hflmp.mpcg_jacobi_sell(ctx, S64, S32, invd32, db, dx, rtol=1e-10, inner_rtol=1e-4,
+                                  flags=flags)
     assert info["status"] == 0
     x = dx.download()
     res = np.linalg.norm(b - O.spmv(rp, ci, v, x))

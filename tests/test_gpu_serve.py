@@ -174,9 +174,9 @@ def test_serve_mixed_alone_equals_mix(ctx, graphs):
     finally:
         sv.close()
     sv = hflmp.Server(ctx, 256 * MB, max_active=5, chunk_iters=9, use_graphs=not graphs,
-                      mixed_min_rows=500)
+                      mixed_min_nnz=5000)
     try:
-        # precision 0: the mix is FP64 below 500 rows and mixed above
+        # precision 0: the mix is FP64 below 5000 nonzeros and mixed above
         mixed = reqs[:3] + [probe] + reqs[3:]
         tickets, out = [], []
         for k, r in enumerate(mixed):
@@ -193,4 +193,4 @@ def test_serve_mixed_alone_equals_mix(ctx, graphs):
     assert np.array_equal(x_mix, x_alone)
     for (rp, ci, v, b), (x, info) in zip(mixed, out):
         assert info["status"] == 0
-        assert info["precision"] == (32 if len(rp) - 1 >= 500 else 64)
+        assert info["precision"] == (32 if len(ci) >= 5000 else 64)
