@@ -13,8 +13,8 @@
 // products in ascending storage order starting from zero (or from y[i] for
 // y += A x), i.e. the summation order of CPUsimple_CSR_lMatrix::VectorMult
 // (CPUsimple_csr_lmatrix.cc:76-111): the result is bit-identical to the
-// sequential CPU loop.  Padding is never read: a lane stops at its row's
-// length (a byte per row).
+// sequential CPU loop.  A lane stops at its row's length (a byte per row),
+// so only the padding inside a row's last chunk of P entries is loaded.
 //
 // Columns are int32, or int16 offsets from the slice's first row when the
 // band fits (|col - 64 s| < 32768), which cuts a nonzero from 12 to 10 B
