@@ -84,7 +84,7 @@ __global__ void __launch_bounds__(kBlock)
   const T s = op.row_sum(tile, p, T(0), tr, prod);
   T part = 0;
   if (tr.has_row) {
-    Ap[tr.row] = s;
+    __builtin_nontemporal_store(s, Ap + tr.row);
     part = s * p[tr.row];
   }
   block_partial1(part, partials);

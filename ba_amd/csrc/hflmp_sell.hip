@@ -122,7 +122,8 @@ __global__ void __launch_bounds__(kBlock)
   }
   TileRow tr;
   s = op.row_sum(unit, x, s, tr, nullptr);
-  if (tr.has_row) y[tr.row] = s;
+  // y is not read again by this launch: stream it past the caches
+  if (tr.has_row) __builtin_nontemporal_store(s, y + tr.row);
 }
 
 void sell_free(hflmp_sell *m) {

@@ -24,7 +24,7 @@ __global__ void __launch_bounds__(kBlock)
   }
   TileRow tr;
   s = tile_row_sum<T, MODE == kNoDiag>(tile, rows_per_tile, nrow, rp, ci, val, x, s, tr, prod);
-  if (tr.has_row) y[tr.row] = s;
+  if (tr.has_row) __builtin_nontemporal_store(s, y + tr.row);
 }
 
 template <typename T>

@@ -203,8 +203,9 @@ int hflmp_cast_csr_f64_to_f32(hflmp_ctx *ctx, int nnz, const double *val,
  * create() builds on the device from the DEVICE CSR arrays (which it does
  * not keep).  flags HFLMP_SELL_COL16 stores columns as int16 offsets from
  * the slice's first row when every offset fits (else int32).  Returns
- * HFLMP_EFORMAT if a row has more than 255 entries or the padding would
- * exceed twice the nonzeros; use the CSR kernels then.  Each lane stops at
+ * HFLMP_EFORMAT if a row has more than 255 entries or the stored entries
+ * (nonzeros plus padding) would exceed 3 nnz + 64 P; use the CSR kernels
+ * then.  Each lane stops at
  * its row's length: padding past a row's last chunk of P is never read. */
 #define HFLMP_SELL_COL16 1
 typedef struct hflmp_sell hflmp_sell;
