@@ -730,7 +730,7 @@ struct hflmp_server {
   size_t budget = 0;
   char *arena = nullptr;
   Arena alloc;
-  int max_active = 0, chunk = 0, use_graph = 1, mixed_min_nnz = 0;
+  int max_active = 0, chunk = 0, use_graph = 1, mixed_min_nnz = 0, mixed_max_nnz = 0;
   double inner_rtol = 1e-2;
   SlotState *d_states = nullptr;  // [max_active]
   SlotState *h_states = nullptr;  // pinned mirror
@@ -945,9 +945,11 @@ int hflmp_server_create(hflmp_ctx *ctx, const hflmp_server_config *cfg, hflmp_se
   HFLMP_REQUIRE(ctx, cfg->hbm_budget >= (1u << 20) && cfg->max_active >= 1 &&
                          cfg->max_active <= 4096 && cfg->chunk_iters >= 1 &&
                          cfg->chunk_iters <= 256 && cfg->mixed_min_nnz >= 0 &&
-                         cfg->inner_rtol >= 0 && cfg->inner_rtol < 1,
+                         cfg->mixed_max_nnz >= 0 && cfg->inner_rtol >= 0 &&
+                         cfg->inner_rtol < 1,
                 "server_create: need hbm_budget >= 1 MiB, 1 <= max_active <= 4096, "
-                "1 <= chunk_iters <= 256, mixed_min_nnz >= 0, 0 <= inner_rtol < 1");
+                "1 <= chunk_iters <= 256, mixed_min_nnz >= 0, mixed_max_nnz >= 0, "
+                "0 <= inner_rtol < 1");
   auto *sv = new hflmp_server();
   sv->ctx = ctx;
   sv->budget = cfg->hbm_budget;
@@ -955,6 +957,7 @@ int hflmp_server_create(hflmp_ctx *ctx, const hflmp_server_config *cfg, hflmp_se
   sv->chunk = cfg->chunk_iters;
   sv->use_graph = cfg->use_graphs ? 1 : 0;
   sv->mixed_min_nnz = cfg->mixed_min_nnz;
+  sv->mixed_max_nnz = cfg->mixed_max_nnz;
   // FP32 CG driven far below its starting residual drifts from the true
   // residual on badly conditioned systems; frequent replacement keeps the
   // reliable updates effective
@@ -1035,7 +1038,10 @@ int hflmp_server_submit(hflmp_server *sv, int nrow, int nnz, const int *row_ptr,
   jb.rtol = rtol;
   jb.maxit = maxit;
   if (precision == 0)
-    precision = (sv->mixed_min_nnz > 0 && nnz >= sv->mixed_min_nnz) ? 32 : 64;
+    precision = (sv->mixed_min_nnz > 0 && nnz >= sv->mixed_min_nnz &&
+                 (sv->mixed_max_nnz == 0 || nnz < sv->mixed_max_nnz))
+                    ? 32
+                    : 64;
   jb.mode = precision == 32 ? kModeMixed : kModeFp64;
   plan_job(jb);
   HFLMP_REQUIRE(ctx, jb.bytes <= sv->budget,

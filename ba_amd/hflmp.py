@@ -537,7 +537,8 @@ def mpcg_jacobi_sell(ctx, S64, S32, invd32, b, x, max_outer=50, atol=0.0, rtol=1
 
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget", C.c_size_t), ("max_active", C.c_int), ("chunk_iters", C.c_int),
-                ("use_graphs", C.c_int), ("mixed_min_nnz", C.c_int), ("inner_rtol", C.c_double)]
+                ("use_graphs", C.c_int), ("mixed_min_nnz", C.c_int), ("inner_rtol", C.c_double),
+                ("mixed_max_nnz", C.c_int)]
 
 
 class ServerResult(C.Structure):
@@ -573,10 +574,11 @@ class Server:
     its result is taken."""
 
     def __init__(self, ctx, hbm_budget, max_active=64, chunk_iters=16, use_graphs=True,
-                 mixed_min_nnz=0, inner_rtol=1e-2):
+                 mixed_min_nnz=0, inner_rtol=1e-2, mixed_max_nnz=0):
         self.ctx = ctx
         cfg = ServerConfig(int(hbm_budget), int(max_active), int(chunk_iters),
-                           1 if use_graphs else 0, int(mixed_min_nnz), float(inner_rtol))
+                           1 if use_graphs else 0, int(mixed_min_nnz), float(inner_rtol),
+                           int(mixed_max_nnz))
         self.h = C.c_void_p()
         ctx.check(ctx.L.hflmp_server_create(ctx.h, C.byref(cfg), C.byref(self.h)),
                   "server_create")
@@ -588,7 +590,7 @@ class Server:
 
     def submit(self, rp, ci, v, b, rtol=1e-8, atol=0.0, maxit=100000, precision=64):
         """precision 64 (FP64 CG), 32 (FP32-inner / FP64-refined) or 0 (by
-        size: mixed from mixed_min_nnz nonzeros)."""
+        size: mixed for mixed_min_nnz <= nnz < mixed_max_nnz)."""
         rp = np.ascontiguousarray(rp, np.int32)
         ci = np.ascontiguousarray(ci, np.int32)
         v = np.ascontiguousarray(v, np.float64)

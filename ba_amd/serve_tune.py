@@ -2,8 +2,9 @@
 
 Per size class (nonzeros of the system) it chooses the precision mode --
 FP64 CG or the FP32-inner / FP64-refined mode -- by timing the class's
-systems from a tuning sample through the server in both modes; the classes
-where the mixed mode wins from some size upwards give `mixed_min_nnz`.
+systems from a tuning sample through the server in both modes; the run of consecutive
+classes where the mixed mode wins that saves the most time gives the window
+[`mixed_min_nnz`, `mixed_max_nnz`).
 Nonzeros rather than rows separate the classes because the mixed mode's
 gain follows the bytes per iteration: a 3D Q1 row (27 entries) gains more
 than a 2D one (9 entries), and large 3D systems need few iterations, so
@@ -34,7 +35,7 @@ def size_class(nnz):
 def _run(ctx, reqs, cfg, precision, rtol):
     sv = hflmp.Server(ctx, cfg["hbm_budget"], cfg["max_active"], cfg["chunk_iters"],
                       cfg["use_graphs"], cfg.get("mixed_min_nnz", 0),
-                      cfg.get("inner_rtol", 1e-2))
+                      cfg.get("inner_rtol", 1e-2), cfg.get("mixed_max_nnz", 0))
     try:
         t0 = time.perf_counter()
         tickets = [sv.submit(*r, rtol=rtol, precision=precision) for r in reqs]
@@ -53,6 +54,21 @@ def _run(ctx, reqs, cfg, precision, rtol):
 
 
 MARGIN = 0.05  # a candidate replaces the default only when this much faster
+
+
+def mixed_window(wins, gain):
+    """(lo, hi): the run of consecutive size classes lo..hi, all with
+    wins[k], whose summed gain[k] (seconds saved by the mixed mode) is
+    largest; None when no class wins."""
+    best, best_gain = None, 0.0
+    for lo in sorted(wins):
+        total, k = 0.0, lo
+        while wins.get(k):
+            total += gain[k]
+            if best is None or total > best_gain:
+                best, best_gain = (lo, k), total
+            k += 1
+    return best
 
 
 def _best(ctx, reqs, cfg, precision, rtol):
@@ -81,16 +97,14 @@ def tune(ctx, sample, base_cfg, rtol=1e-8, chunks=(8, 16, 32)):
                                 "systems": len(reqs), "fp64_s": round(t64, 4),
                                 "mixed_s": round(t32, 4),
                                 "choice": "mixed" if mixed_wins[k] else "fp64"}
-    # the smallest class from which the mixed mode wins for every larger class
-    start = None
-    for k in sorted(classes, reverse=True):
-        if mixed_wins[k]:
-            start = k
-        else:
-            break
-    cfg["mixed_min_nnz"] = CLASS_BOUNDS[start] if start is not None else 0
-    if cfg["mixed_min_nnz"] == 0 and start is not None:
-        cfg["mixed_min_nnz"] = 1
+    lo_hi = mixed_window(mixed_wins, {k: report["classes"][k]["fp64_s"] -
+                                      report["classes"][k]["mixed_s"] for k in classes})
+    if lo_hi is None:
+        cfg["mixed_min_nnz"], cfg["mixed_max_nnz"] = 0, 0
+    else:
+        lo, hi = lo_hi
+        cfg["mixed_min_nnz"] = max(CLASS_BOUNDS[lo], 1)
+        cfg["mixed_max_nnz"] = 0 if hi == len(CLASS_BOUNDS) - 2 else CLASS_BOUNDS[hi + 1]
     report["chunks"] = {}
     times = {}
     for c in sorted(set(chunks) | {cfg["chunk_iters"]}):
@@ -104,5 +118,6 @@ def tune(ctx, sample, base_cfg, rtol=1e-8, chunks=(8, 16, 32)):
         if t_def is None or times[c_best] < (1 - MARGIN) * t_def:
             cfg["chunk_iters"] = c_best
     report["mixed_min_nnz"] = cfg["mixed_min_nnz"]
+    report["mixed_max_nnz"] = cfg["mixed_max_nnz"]
     report["chunk_iters"] = cfg["chunk_iters"]
     return cfg, report

@@ -322,7 +322,8 @@ def run_server(hflmp, ctx, reqs, arrivals, cfg, rtol, precision=64):
     """Replay requests into a server at their arrival times (all at t = 0
     when arrivals is None); returns per-request (x, info, latency)."""
     sv = hflmp.Server(ctx, cfg["hbm_budget"], cfg["max_active"], cfg["chunk_iters"],
-                      cfg["use_graphs"], cfg.get("mixed_min_nnz", 0), cfg.get("inner_rtol", 1e-2))
+                      cfg["use_graphs"], cfg.get("mixed_min_nnz", 0), cfg.get("inner_rtol", 1e-2),
+                      cfg.get("mixed_max_nnz", 0))
     try:
         n = len(reqs)
         tickets, lat, out = [None] * n, [0.0] * n, [None] * n
@@ -395,7 +396,7 @@ def bench_serve(hflmp, nreq, seed, rate, ref_style, graphs=True, ntune=32):
         singles.append((x, info))
     t_single = time.perf_counter() - t0
     base = {"hbm_budget": 32 << 30, "max_active": 256, "chunk_iters": 16, "use_graphs": graphs,
-            "mixed_min_nnz": 0, "inner_rtol": 1e-2}
+            "mixed_min_nnz": 0, "inner_rtol": 1e-2, "mixed_max_nnz": 0}
     t0 = time.perf_counter()
     cfg, tune_report = serve_tune.tune(ctx, treqs, base, rtol)
     t_tune = time.perf_counter() - t0

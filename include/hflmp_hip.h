@@ -419,7 +419,8 @@ int hflmp_mpcg_jacobi_sell(hflmp_ctx *ctx, const hflmp_sell *A64,
  * (common/iterate_control.cc:80-107) on the recursive residual; mixed
  * solves stop on the FP64 true residual ||b - A x|| <= atol or
  * < rtol ||b||, with maxit bounding each inner solve.  precision 0 at
- * submission picks the mixed mode for nnz >= mixed_min_nnz (0: never).
+ * submission picks the mixed mode for mixed_min_nnz <= nnz < mixed_max_nnz
+ * (mixed_min_nnz 0: never; mixed_max_nnz 0: no upper bound).
  * The caller's host arrays (row_ptr, col, val, b, x) must stay valid until
  * the ticket is done.  Rows may hold 0..255 entries. */
 typedef struct hflmp_server hflmp_server;
@@ -430,6 +431,7 @@ typedef struct {
   int use_graphs;    /* 1: replay each step's launches as a hipGraph      */
   int mixed_min_nnz;    /* precision 0: mixed from this many nonzeros (0: off) */
   double inner_rtol;    /* mixed: residual drop between replacements (0: 1e-2) */
+  int mixed_max_nnz;    /* precision 0: mixed below this many nonzeros (0: no bound) */
 } hflmp_server_config;
 typedef struct {
   int state;              /* 0 queued, 1 active, 2 done                   */

@@ -194,3 +194,24 @@ def test_serve_mixed_alone_equals_mix(ctx, graphs):
     for (rp, ci, v, b), (x, info) in zip(mixed, out):
         assert info["status"] == 0
         assert info["precision"] == (32 if len(ci) >= 5000 else 64)
+
+
+def test_serve_precision_window(ctx):
+    # precision 0 picks the mixed mode for 5000 <= nnz < 20000 only
+    reqs = [request(*m) for m in MIX]
+    sv = hflmp.Server(ctx, 256 * MB, max_active=8, chunk_iters=8, mixed_min_nnz=5000,
+                      mixed_max_nnz=20000)
+    try:
+        tickets = [sv.submit(*r, precision=0) for r in reqs]
+        while any(sv.poll(t)["state"] != 2 for t in tickets):
+            sv.step()
+        out = [sv.take(t) for t in tickets]
+    finally:
+        sv.close()
+    modes = [info["precision"] for _, info in out]
+    assert modes == [32 if 5000 <= len(r[1]) < 20000 else 64 for r in reqs]
+    assert 32 in modes and 64 in modes
+    for (rp, ci, v, b), (x, info) in zip(reqs, out):
+        assert info["status"] == 0
+        # FP64 stops on the recursive residual, mixed on the true one
+        assert np.linalg.norm(b - O.spmv(rp, ci, v, x)) < 1.1e-8 * np.linalg.norm(b)

@@ -83,3 +83,17 @@ def test_bench_refuses_multi_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode != 0 and "single" in r.stderr
+
+
+def test_serve_tune_mixed_window():
+    from ba_amd import serve_tune as T
+    # the larger of two winning runs is chosen; a losing class splits runs
+    wins = {0: True, 1: False, 2: True, 3: True, 4: False}
+    gain = {0: 0.5, 1: -0.1, 2: 0.2, 3: 0.4, 4: -0.3}
+    assert T.mixed_window(wins, gain) == (2, 3)
+    # a winning class with a small gain at the edge still joins the run
+    assert T.mixed_window({1: True, 2: True}, {1: 0.01, 2: 0.3}) == (1, 2)
+    assert T.mixed_window({0: False, 1: False}, {0: -1, 1: -1}) is None
+    # a class missing from the sample ends a run
+    assert T.mixed_window({0: True, 2: True}, {0: 0.1, 2: 0.3}) == (2, 2)
+    assert [T.size_class(n) for n in (0, 65535, 65536, 1 << 20, 9 << 20)] == [0, 0, 1, 2, 3]
