@@ -18,13 +18,14 @@
 // iterations only.
 //
 // The SpMV kernels take the operator as a value (CsrTileOp over the CSR
-// arrays or SellOp over the SELL-64 layout, hflmp_sell.hpp); both give each
-// thread of a workgroup one row and the same products, so the iterates do
-// not depend on the layout.
+// arrays or SellOp over the SELL-64 layout, hflmp_sell.hpp, or the
+// matrix-free StencilOp, hflmp_stencil.hpp); each gives every thread of a
+// workgroup one row.  The CSR and SELL-64 products are identical.
 #include <chrono>
 
 #include "hflmp_sell.hpp"
 #include "hflmp_solver_common.hpp"
+#include "hflmp_stencil.hpp"
 
 namespace {
 
@@ -176,6 +177,8 @@ int pcg_jacobi(hflmp_ctx *ctx, const Op &op, int nnz, const T *invd, const T *b,
   const int grid1 = op_grid(op);
   HFLMP_REQUIRE(ctx, work_bytes >= L.total, "pcg: workspace %zu < %zu bytes", work_bytes,
                 L.total);
+  HFLMP_REQUIRE(ctx, grid1 <= L.grid1 || grid1 <= L.grid2,
+                "pcg: operator grid %d exceeds the partial-sum area", grid1);
   char *w = static_cast<char *>(work);
   auto *st = reinterpret_cast<CgState<T> *>(w + L.state);
   T *partials = reinterpret_cast<T *>(w + L.partials);
@@ -535,6 +538,25 @@ int pcg_jacobi_sell(hflmp_ctx *ctx, const hflmp_sell *A, const T *invd, const T 
                     work, work_bytes, info);
 }
 
+// One-entry rows give 256-row tiles in solver_layout, the stencil grid.
+inline int stencil_layout_nnz(long long nnode) { return (int)nnode; }
+
+template <typename T>
+int pcg_jacobi_stencil(hflmp_ctx *ctx, const hflmp_stencil *A, const T *invd, const T *b, T *x,
+                       int maxit, double atol, double rtol, double dtol, void *work,
+                       size_t work_bytes, hflmp_solve_info *info) {
+  if (!ctx) return HFLMP_EINVAL;
+  HFLMP_REQUIRE(ctx, A != nullptr, "pcg: null operator");
+  HFLMP_REQUIRE(ctx, A->value_bytes == (int)sizeof(T), "pcg: value type mismatch");
+  HFLMP_REQUIRE(ctx, A->device == ctx->device, "pcg: operator on another device");
+  int rc = HFLMP_OK;
+  stencil_dispatch<T>(A, [&](auto op) {
+    rc = pcg_jacobi(ctx, op, stencil_layout_nnz(A->nnode), invd, b, x, maxit, atol, rtol, dtol,
+                    work, work_bytes, info);
+  });
+  return rc;
+}
+
 template <typename T>
 int pcg_jacobi_csr(hflmp_ctx *ctx, int nrow, int nnz, const int *rp, const int *ci,
                    const T *val, const T *invd, const T *b, T *x, int maxit, double atol,
@@ -578,6 +600,27 @@ int hflmp_pcg_jacobi_sell_f32(hflmp_ctx *ctx, const hflmp_sell *A, const float *
                               hflmp_solve_info *info) {
   return pcg_jacobi_sell<float>(ctx, A, invd, b, x, maxit, atol, rtol, dtol, work, work_bytes,
                                 info);
+}
+
+size_t hflmp_pcg_stencil_workspace_bytes(long long nnode, int value_bytes) {
+  if (nnode <= 0 || nnode > 0x7fffffffLL || (value_bytes != 4 && value_bytes != 8)) return 0;
+  return solver_layout((int)nnode, stencil_layout_nnz(nnode), value_bytes, 4).total;
+}
+
+int hflmp_pcg_jacobi_stencil_f64(hflmp_ctx *ctx, const hflmp_stencil *A, const double *invd,
+                                 const double *b, double *x, int maxit, double atol,
+                                 double rtol, double dtol, void *work, size_t work_bytes,
+                                 hflmp_solve_info *info) {
+  return pcg_jacobi_stencil<double>(ctx, A, invd, b, x, maxit, atol, rtol, dtol, work,
+                                    work_bytes, info);
+}
+
+int hflmp_pcg_jacobi_stencil_f32(hflmp_ctx *ctx, const hflmp_stencil *A, const float *invd,
+                                 const float *b, float *x, int maxit, double atol, double rtol,
+                                 double dtol, void *work, size_t work_bytes,
+                                 hflmp_solve_info *info) {
+  return pcg_jacobi_stencil<float>(ctx, A, invd, b, x, maxit, atol, rtol, dtol, work,
+                                   work_bytes, info);
 }
 
 size_t hflmp_mpcg_workspace_bytes(int nrow, int nnz) {

@@ -178,6 +178,15 @@ int hflmp_gen_fe_csr(int order, int dim, int n, const double *kappa, int *row_pt
   return 0;
 }
 
+int hflmp_gen_fe_element(int order, int dim, int n, double *E) {
+  int64_t N, Z;
+  if (hflmp_gen_fe_sizes(order, dim, n, &N, &Z)) return -1;
+  Element el;
+  make_element(order, dim, n, el);
+  for (int k = 0; k < el.nloc * el.nloc; ++k) E[k] = el.E[k];
+  return el.nloc;
+}
+
 void hflmp_gen_kappa(int dim, int n, double sigma, uint64_t seed, double *kappa) {
   int64_t nc = 1;
   for (int d = 0; d < dim; ++d) nc *= n;

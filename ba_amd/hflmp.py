@@ -108,6 +108,21 @@ def hip_lib():
             getattr(L, "hflmp_pcg_jacobi_sell_" + s).argtypes = [vp, vp, vp, vp, vp, i, d, d, d,
                                                                   vp, sz, vp]
         L.hflmp_sell_get_info.argtypes = [vp, vp]
+        for s in ("f64", "f32"):
+            getattr(L, "hflmp_stencil_create_" + s).argtypes = [vp, vp, vp, vp, vp, i, vp, vp]
+            getattr(L, "hflmp_stencil_set_factors_" + s).argtypes = [vp, vp, vp]
+            getattr(L, "hflmp_stencil_apply_" + s).argtypes = [vp, vp, vp, vp]
+            ct = d if s == "f64" else f
+            getattr(L, "hflmp_stencil_apply_add_" + s).argtypes = [vp, vp, ct, vp, ct, vp]
+            getattr(L, "hflmp_stencil_extract_invdiag_" + s).argtypes = [vp, vp, vp,
+                                                                         C.POINTER(i)]
+            getattr(L, "hflmp_pcg_jacobi_stencil_" + s).argtypes = [vp, vp, vp, vp, vp, i, d, d,
+                                                                     d, vp, sz, vp]
+        L.hflmp_stencil_get_info.argtypes = [vp, vp]
+        L.hflmp_stencil_destroy.argtypes = [vp]
+        L.hflmp_stencil_destroy.restype = None
+        L.hflmp_pcg_stencil_workspace_bytes.restype = sz
+        L.hflmp_pcg_stencil_workspace_bytes.argtypes = [C.c_longlong, i]
         L.hflmp_sell_destroy.argtypes = [vp]
         L.hflmp_sell_destroy.restype = None
         L.hflmp_mpcg_sell_workspace_bytes.restype = sz
@@ -156,6 +171,7 @@ def gen_lib():
                                       C.c_void_p]
         L.hflmp_gen_rhs.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_void_p]
         L.hflmp_gen_set_threads.argtypes = [C.c_int]
+        L.hflmp_gen_fe_element.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p]
         _gen = L
     return _gen
 
@@ -189,6 +205,28 @@ def fe_system(order, dim, n, kappa=None):
     if rc:
         raise HflmpError("hflmp_gen_fe_csr failed (%d)" % rc)
     return rp, ci, v
+
+
+def fe_element(order, dim, n):
+    """Element stiffness (nloc x nloc, h included) entered per cell with
+    factor kappa: the stencil operator's local matrix."""
+    q = order + 1
+    E = np.empty((q ** dim) ** 2, np.float64)
+    if gen_lib().hflmp_gen_fe_element(order, dim, n, E.ctypes.data) < 0:
+        raise ValueError("unsupported element order=%d dim=%d n=%d" % (order, dim, n))
+    return E.reshape(q ** dim, q ** dim)
+
+
+def fe_fixed(order, dim, n):
+    """Indices of the boundary (Dirichlet) nodes, ascending."""
+    p = order * n + 1
+    g = np.arange(p)
+    edge = (g == 0) | (g == p - 1)
+    if dim == 2:
+        m = edge[None, :] | edge[:, None]
+    else:
+        m = edge[None, None, :] | edge[None, :, None] | edge[:, None, None]
+    return np.nonzero(m.ravel())[0].astype(np.int32)
 
 
 def fe_rhs(order, dim, n, seed):
@@ -509,6 +547,107 @@ def pcg_jacobi_sell(ctx, S, invd, b, x, maxit=10000, atol=0.0, rtol=1e-8, dtol=1
     try:
         _call(ctx, "hflmp_pcg_jacobi_sell_" + _sfx(S.dtype), S.h, invd.ptr, b.ptr, x.ptr, maxit,
               atol, rtol, dtol, work.ptr, work.nbytes, C.byref(info))
+    finally:
+        if own:
+            work.free()
+    return info.as_dict()
+
+
+class StencilDesc(C.Structure):
+    _fields_ = [("order", C.c_int), ("dim", C.c_int), ("ncell", C.c_int * 3),
+                ("nmat", C.c_int)]
+
+
+class StencilInfo(C.Structure):
+    _fields_ = [("order", C.c_int), ("dim", C.c_int), ("nloc", C.c_int), ("nmat", C.c_int),
+                ("ncell", C.c_int * 3), ("nnode", C.c_longlong), ("ncell_total", C.c_longlong),
+                ("nfixed", C.c_int), ("value_bytes", C.c_int), ("device_bytes", C.c_size_t)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["ncell"] = list(self.ncell)
+        return d
+
+
+class StencilOperator:
+    """Matrix-free stencil operator on a structured Q1/Q2 mesh
+    (hflmp_stencil_*): local_matrices (host, nmat x nloc x nloc), cell
+    factors and the optional per-cell matrix index and fixed dof list are
+    host arrays uploaded here."""
+
+    def __init__(self, ctx, order, dim, ncell, local_matrices, cell_factor, fixed=None,
+                 cell_lm=None, dtype=np.float64):
+        self.ctx = ctx
+        self.dtype = np.dtype(dtype)
+        E = np.ascontiguousarray(local_matrices, self.dtype)
+        q = order + 1
+        nloc = q ** dim
+        nmat = E.size // (nloc * nloc)
+        nc = list(ncell) + [1] * (3 - len(ncell))
+        desc = StencilDesc(order, dim, (C.c_int * 3)(*nc), nmat)
+        f = ctx.upload(np.ascontiguousarray(cell_factor, self.dtype))
+        fx = None if fixed is None or len(fixed) == 0 else \
+            ctx.upload(np.ascontiguousarray(fixed, np.int32))
+        lm = None if cell_lm is None else ctx.upload(np.ascontiguousarray(cell_lm, np.uint8))
+        self.h = C.c_void_p()
+        try:
+            _call(ctx, "hflmp_stencil_create_" + _sfx(self.dtype), C.byref(desc),
+                  E.ctypes.data, None if lm is None else lm.ptr, f.ptr,
+                  0 if fx is None else len(fixed), None if fx is None else fx.ptr,
+                  C.byref(self.h))
+        finally:
+            for a in (f, fx, lm):
+                if a is not None:
+                    a.free()
+        inf = self.info()
+        self.nrow = inf["nnode"]
+
+    def info(self):
+        inf = StencilInfo()
+        rc = self.ctx.L.hflmp_stencil_get_info(self.h, C.byref(inf))
+        if rc:
+            raise HflmpError("hflmp_stencil_get_info failed (%d)" % rc)
+        return inf.as_dict()
+
+    def apply(self, x, y):
+        _call(self.ctx, "hflmp_stencil_apply_" + _sfx(self.dtype), self.h, x.ptr, y.ptr)
+
+    def apply_add(self, alpha, x, beta, y):
+        _call(self.ctx, "hflmp_stencil_apply_add_" + _sfx(self.dtype), self.h, alpha, x.ptr,
+              beta, y.ptr)
+
+    def invdiag(self):
+        d = self.ctx.empty(self.nrow, self.dtype)
+        bad = C.c_int()
+        _call(self.ctx, "hflmp_stencil_extract_invdiag_" + _sfx(self.dtype), self.h, d.ptr,
+              C.byref(bad))
+        return d
+
+    def set_factors(self, cell_factor):
+        f = self.ctx.upload(np.ascontiguousarray(cell_factor, self.dtype))
+        try:
+            _call(self.ctx, "hflmp_stencil_set_factors_" + _sfx(self.dtype), self.h, f.ptr)
+            self.ctx.sync()
+        finally:
+            f.free()
+
+    def free(self):
+        if self.h:
+            self.ctx.L.hflmp_stencil_destroy(self.h)
+            self.h = C.c_void_p()
+
+
+def pcg_jacobi_stencil(ctx, S, invd, b, x, maxit=10000, atol=0.0, rtol=1e-8, dtol=1e6,
+                       work=None):
+    """Device-resident Jacobi-PCG over a StencilOperator."""
+    need = ctx.L.hflmp_pcg_stencil_workspace_bytes(S.nrow, S.dtype.itemsize)
+    own = work is None
+    if own:
+        work = ctx.empty(need, np.uint8)
+    info = SolveInfo()
+    try:
+        _call(ctx, "hflmp_pcg_jacobi_stencil_" + _sfx(S.dtype), S.h, invd.ptr, b.ptr, x.ptr,
+              maxit, atol, rtol, dtol, work.ptr, work.nbytes, C.byref(info))
     finally:
         if own:
             work.free()

@@ -172,6 +172,63 @@ def bench_cg(hflmp, ctx, A, S, b):
     return res
 
 
+def stencil_bytes(N, ncell, vb=8):
+    """Matrix-free product: x read, y written, one factor per cell, a bit
+    per node for the fixed dofs."""
+    return 2 * vb * N + vb * ncell + 4 * ((N + 31) // 32)
+
+
+def bench_stencil(hflmp, ctx, n, x, b, steps, sell_cg=None, assembled=None):
+    """Matrix-free StencilOperator on the same 3D Q1 128^3 system (constant
+    coefficient): product time and roofline, bit-exactness against the
+    oracle's cell loop, agreement with the assembled product, and Jacobi-CG
+    to 1e-8 through it."""
+    from oracle import oracle as O
+    E = hflmp.fe_element(1, 3, n)
+    fixed = hflmp.fe_fixed(1, 3, n)
+    ones = np.ones(n ** 3)
+    St = hflmp.StencilOperator(ctx, 1, 3, [n] * 3, E, ones, fixed)
+    N = St.nrow
+    dx = ctx.upload(x)
+    dy = ctx.empty(N, np.float64)
+    t = time_launches(ctx, lambda: St.apply(dx, dy), steps)
+    nb = stencil_bytes(N, n ** 3)
+    out = {"workload": "3D Q1 Poisson %d^3, FP64, matrix-free (element matrix x cell factor)" % n,
+           "apply_us": round(t * 1e6, 2), "bytes_per_apply": nb,
+           "roofline": {"bound": "hbm", "achieved": round(nb / t / 1e9, 2),
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(nb / t / 1e9 / HBM_PEAK_GBS, 4)},
+           "device_bytes": St.info()["device_bytes"]}
+    St.apply(dx, dy)
+    y = dy.download()
+    yref = O.stencil_vmadd(1, 3, [n] * 3, E, ones, fixed, x)
+    out["parity"] = {"bit_exact_vs_oracle_cell_loop": bool(np.array_equal(y, yref))}
+    if assembled is not None:
+        S, rp, ci, v = assembled
+        S.spmv(dx, dy)
+        ya = dy.download()
+        import scipy.sparse as sp
+        scale = sp.csr_matrix((np.abs(v), ci, rp), shape=(N, N)) @ np.abs(x)
+        out["parity"]["max_diff_vs_assembled_over_absA_absx"] = float(
+            np.max(np.abs(y - ya) / np.maximum(scale, 1e-300)))
+    invd = St.invdiag()
+    db = ctx.upload(b)
+    for rep in range(2):
+        dxs = ctx.zeros(N, np.float64)
+        info = hflmp.pcg_jacobi_stencil(ctx, St, invd, db, dxs, maxit=20000, rtol=1e-8)
+        dxs.free()
+    it = max(info["iters"], 1)
+    out["cg"] = {"rtol": 1e-8, "iters": info["iters"],
+                 "status": hflmp.STATUS.get(info["status"], "?"),
+                 "seconds_to_tol": round(info["seconds"], 5),
+                 "ms_per_iter": round(info["seconds"] * 1e3 / it, 5)}
+    if sell_cg:
+        out["cg"]["speedup_vs_sell_cg"] = round(sell_cg["seconds_to_tol"] / info["seconds"], 3)
+        out["cg"]["iters_sell_cg"] = sell_cg["iters"]
+    St.free(); dx.free(); dy.free(); invd.free(); db.free()
+    return out
+
+
 def bench_mixed(hflmp, n):
     """C3: log-normal diffusivity (sigma = 1), 128^3 Q1, relative residual
     1e-10: plain FP64 CG vs FP32-inner / FP64-refined CG, both through the
@@ -486,6 +543,7 @@ def main():
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-stencil", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=200)
     ap.add_argument("--serve-seed", type=int, default=20241205)
@@ -600,6 +658,10 @@ def main():
 
     if not args.no_cg:
         out["cg"] = bench_cg(hflmp, ctx, A, S, b)
+    if not args.no_stencil:
+        out["matrix_free"] = bench_stencil(hflmp, ctx, n, x, b, max(args.steps // 2, 1),
+                                           out.get("cg", {}).get("sell"),
+                                           None if args.no_parity else (S, rp, ci, v))
     S.free(); A.free(); dx.free(); dy.free()
     ctx.close()
 

@@ -37,6 +37,10 @@ int hflmp_gen_fe_csr(int order, int dim, int n, const double *kappa,
 void hflmp_gen_kappa(int dim, int n, double sigma, uint64_t seed,
                      double *kappa);
 void hflmp_gen_rhs(int order, int dim, int n, uint64_t seed, double *b);
+/* the element stiffness E (nloc x nloc, row-major, local nodes
+ * lexicographic, h included) that cell c enters with factor kappa_c: the
+ * local matrix of the matrix-free stencil operator; returns nloc, or -1 */
+int hflmp_gen_fe_element(int order, int dim, int n, double *E);
 /* number of host threads the generator uses (0 = all) */
 void hflmp_gen_set_threads(int nthreads);
 

@@ -240,6 +240,77 @@ int hflmp_sell_spmv_add_f64(hflmp_ctx *ctx, const hflmp_sell *m,
 int hflmp_sell_spmv_add_f32(hflmp_ctx *ctx, const hflmp_sell *m,
                             const float *x, float *y);
 
+/* ---- matrix-free stencil operator ------------------------------------------
+ * StencilOperator (linear_algebra/stencil_operator.h:43-330) on a
+ * structured mesh of ncell[0] x ncell[1] (x ncell[2]) cells of Q1 (2D, 3D)
+ * or Q2 (2D) elements: nodes numbered lexicographically (x fastest), local
+ * nodes of a cell likewise.  Cell c contributes f_c E_m (E_m: one of nmat
+ * nloc x nloc local matrices, m = cell_lm[c]; the reference's
+ * CellMatrixHandler with one matrix per cell); fixed dofs get y_d = x_d
+ * (stencil_operator.h:319-325).  Products follow the reference's cell loop
+ * (per cell tmp = E_m x_c summed from 0 over ascending local columns, lhs =
+ * 0 + (alpha f_c) tmp, added into y in ascending cell order) and are
+ * bit-identical to it; no matrix is stored.
+ * create() copies the local matrices (HOST array, nmat * nloc * nloc
+ * values, row-major), the cell factors and cell_lm (DEVICE arrays, one
+ * entry per cell, lexicographic; cell_lm may be NULL when nmat == 1) and
+ * the fixed dof list (DEVICE int array).  Vectors and the cell array stay
+ * below 2 GiB each (nnode and ncell times value_bytes < 2^31).  Interface
+ * ("cross") matrices of discontinuous elements are not supported. */
+typedef struct hflmp_stencil hflmp_stencil;
+typedef struct {
+  int order;     /* 1 or 2                                                 */
+  int dim;       /* 2 or 3 (order 2: 2)                                    */
+  int ncell[3];  /* cells per direction (ncell[2] unused in 2D)            */
+  int nmat;      /* local matrices, 1..256                                 */
+} hflmp_stencil_desc;
+typedef struct {
+  int order, dim, nloc, nmat;
+  int ncell[3];
+  long long nnode, ncell_total;
+  int nfixed;
+  int value_bytes;
+  size_t device_bytes;
+} hflmp_stencil_info;
+int hflmp_stencil_create_f64(hflmp_ctx *ctx, const hflmp_stencil_desc *desc,
+                             const double *local_matrices,
+                             const unsigned char *cell_lm,
+                             const double *cell_factor, int nfixed,
+                             const int *fixed, hflmp_stencil **out);
+int hflmp_stencil_create_f32(hflmp_ctx *ctx, const hflmp_stencil_desc *desc,
+                             const float *local_matrices,
+                             const unsigned char *cell_lm,
+                             const float *cell_factor, int nfixed,
+                             const int *fixed, hflmp_stencil **out);
+/* New cell factors (DEVICE array, one per cell) */
+int hflmp_stencil_set_factors_f64(hflmp_ctx *ctx, hflmp_stencil *m,
+                                  const double *cell_factor);
+int hflmp_stencil_set_factors_f32(hflmp_ctx *ctx, hflmp_stencil *m,
+                                  const float *cell_factor);
+int hflmp_stencil_get_info(const hflmp_stencil *m, hflmp_stencil_info *info);
+void hflmp_stencil_destroy(hflmp_stencil *m);
+/* y = A x (VectorMult, stencil_operator.h:87-91; y is not read) */
+int hflmp_stencil_apply_f64(hflmp_ctx *ctx, const hflmp_stencil *m,
+                            const double *x, double *y);
+int hflmp_stencil_apply_f32(hflmp_ctx *ctx, const hflmp_stencil *m,
+                            const float *x, float *y);
+/* y = beta y + alpha A x, fixed dofs y_d = x_d (VectorMultAdd,
+ * stencil_operator.h:153-330); x and y must differ */
+int hflmp_stencil_apply_add_f64(hflmp_ctx *ctx, const hflmp_stencil *m,
+                                double alpha, const double *x, double beta,
+                                double *y);
+int hflmp_stencil_apply_add_f32(hflmp_ctx *ctx, const hflmp_stencil *m,
+                                float alpha, const float *x, float beta,
+                                float *y);
+/* invdiag[i] = 1 / sum_c f_c E_m[l][l] over the node's cells in ascending
+ * order (the diagonal cell-by-cell assembly produces), 1 on fixed dofs;
+ * the Jacobi preconditioner of the operator (lpreconditioner.cc:124-181 on
+ * the assembled matrix).  *n_bad (host) = zero diagonals. */
+int hflmp_stencil_extract_invdiag_f64(hflmp_ctx *ctx, const hflmp_stencil *m,
+                                      double *invdiag, int *n_bad);
+int hflmp_stencil_extract_invdiag_f32(hflmp_ctx *ctx, const hflmp_stencil *m,
+                                      float *invdiag, int *n_bad);
+
 /* ---- preconditioners ------------------------------------------------------
  * Jacobi apply z = invdiag .* r fused into one pass
  * (lPreconditioner_Jacobi::ApplylPreconditioner, lpreconditioner.cc:184-200:
@@ -311,6 +382,20 @@ int hflmp_pcg_jacobi_sell_f32(hflmp_ctx *ctx, const hflmp_sell *A,
                               int maxit, double atol, double rtol, double dtol,
                               void *work, size_t work_bytes,
                               hflmp_solve_info *info);
+/* The same solve with the matrix-free stencil operator (CG<LAD> over a
+ * StencilOperator, stencil_operator.h:43-330): Ap from the cells, every
+ * other step as above.  Workspace: hflmp_pcg_stencil_workspace_bytes(). */
+size_t hflmp_pcg_stencil_workspace_bytes(long long nnode, int value_bytes);
+int hflmp_pcg_jacobi_stencil_f64(hflmp_ctx *ctx, const hflmp_stencil *A,
+                                 const double *invdiag, const double *b,
+                                 double *x, int maxit, double atol,
+                                 double rtol, double dtol, void *work,
+                                 size_t work_bytes, hflmp_solve_info *info);
+int hflmp_pcg_jacobi_stencil_f32(hflmp_ctx *ctx, const hflmp_stencil *A,
+                                 const float *invdiag, const float *b, float *x,
+                                 int maxit, double atol, double rtol,
+                                 double dtol, void *work, size_t work_bytes,
+                                 hflmp_solve_info *info);
 
 /* ---- several right-hand sides (no reference counterpart: the reference
  * solves one right-hand side per CG::Solve, linear_solver/cg.h:165-246) ---

@@ -31,6 +31,8 @@
  *   MPIR          linear_solver/mpir.cc:52-296 (res0 kept at the initial
  *                 residual; the reference resets it after every correction,
  *                 mpir.cc:240)
+ *   stencil       linear_algebra/stencil_operator.h:153-330 (matrix-free
+ *                 VectorMultAdd, cell by cell)
  *
  * Problem generator: structured Q1 (2D/3D) and Q2 (2D) Poisson/diffusion on
  * the unit square/cube, assembled element by element in lexicographic cell
@@ -60,6 +62,9 @@ int or_fe_assemble(int order, int dim, int n, const double *kappa,
 void or_fe_kappa(int dim, int n, double sigma, uint64_t seed, double *kappa);
 void or_fe_rhs(int order, int dim, int n, uint64_t seed, double *b);
 void or_fe_boundary_mask(int order, int dim, int n, uint8_t *mask);
+/* element stiffness of the generator (nloc x nloc, local nodes
+ * lexicographic, h included; the cell factor is kappa) */
+void or_fe_element(int order, int dim, int n, double *E);
 
 /* ---- CSR utilities ----------------------------------------------------- */
 void or_coo_to_csr_d(int nrow, int nnz, const int *ri, const int *ci,
@@ -108,7 +113,11 @@ int or_multicoloring(int nrow, const int *rowptr, const int *col,
                    const int *perm, const int *color, int basis,              \
                    int flexible, const T *b, T *x, int maxit, double atol,    \
                    double rtol, double dtol, int *iters, double *res_init,    \
-                   double *res_final);
+                   double *res_final);                                        \
+  void or_stencil_vmadd_##S(int order, int dim, const int *nc, const T *E,    \
+                            const uint8_t *cell_lm, const T *cell_factor,     \
+                            int nfixed, const int *fixed, T alpha,            \
+                            const T *in, T beta, T *out);
 
 OR_DECL(double, d)
 OR_DECL(float, s)

@@ -46,6 +46,7 @@ def lib():
         L.or_fe_kappa.argtypes = [C.c_int, C.c_int, C.c_double, C.c_uint64, _f64p]
         L.or_fe_rhs.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint64, _f64p]
         L.or_fe_boundary_mask.argtypes = [C.c_int, C.c_int, C.c_int, _u8p]
+        L.or_fe_element.argtypes = [C.c_int, C.c_int, C.c_int, _f64p]
         L.or_coo_to_csr_d.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _f64p,
                                       _i32p, _i32p, _f64p]
         L.or_csr_diag_ptr.argtypes = [C.c_int, _i32p, _i32p, _i32p]
@@ -79,6 +80,8 @@ def lib():
                                 C.c_double, C.c_double, C.c_double,
                                 C.POINTER(C.c_int), C.POINTER(C.c_double),
                                 C.POINTER(C.c_double), _vp]
+            g("stencil_vmadd").argtypes = [C.c_int, C.c_int, _i32p, fp, _vp, fp, C.c_int,
+                                           _vp, ct, fp, ct, fp]
             g("gmres").argtypes = [C.c_int, _i32p, _i32p, fp, C.c_int, _vp,
                                    C.c_int, _vp, _vp, _vp, C.c_int, C.c_int,
                                    fp, fp, C.c_int, C.c_double, C.c_double,
@@ -139,6 +142,34 @@ def fe_boundary_mask(order, dim, n):
     m = np.empty(N, np.uint8)
     lib().or_fe_boundary_mask(order, dim, n, m)
     return m.astype(bool)
+
+
+def fe_element(order, dim, n):
+    """The generator's element stiffness (h included; the cell factor is
+    kappa), local nodes lexicographic, x fastest."""
+    q = order + 1
+    nloc = q ** dim
+    E = np.empty(nloc * nloc, np.float64)
+    lib().or_fe_element(order, dim, n, E)
+    return E.reshape(nloc, nloc)
+
+
+def stencil_vmadd(order, dim, nc, E, cell_factor, fixed, x, y=None, alpha=1.0, beta=0.0,
+                  cell_lm=None):
+    """out = beta out + alpha A x by the matrix-free cell loop of
+    StencilOperator::VectorMultAdd (stencil_operator.h:153-330); E holds one
+    or more nloc x nloc local matrices, cell_lm picks one per cell."""
+    dt = np.asarray(x).dtype
+    nc3 = np.array(list(nc) + [1] * (3 - len(nc)), np.int32)
+    E = np.ascontiguousarray(E, dt).ravel()
+    f = np.ascontiguousarray(cell_factor, dt)
+    fx = np.ascontiguousarray(fixed, np.int32)
+    lm = None if cell_lm is None else np.ascontiguousarray(cell_lm, np.uint8)
+    x = np.ascontiguousarray(x, dt)
+    out = np.zeros_like(x) if y is None else np.array(y, dt, copy=True)
+    getattr(lib(), "or_stencil_vmadd_" + _sfx(x))(order, dim, nc3, E, _ptr(lm), f, len(fx),
+                                                  _ptr(fx), alpha, x, beta, out)
+    return out
 
 
 def splitmix64_at(seed, i):

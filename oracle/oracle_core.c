@@ -193,6 +193,10 @@ int or_fe_assemble(int order, int dim, int n, const double *kappa,
   return 0;
 }
 
+void or_fe_element(int order, int dim, int n, double *E) {
+  element_matrix(order, dim, n, E);
+}
+
 void or_fe_boundary_mask(int order, int dim, int n, uint8_t *mask) {
   int64_t N, Z;
   or_fe_sizes(order, dim, n, &N, &Z);

@@ -411,3 +411,71 @@ def test_mpir_reaches_fp64_tolerance():
     assert np.linalg.norm(b - A @ x) <= 1e-10 * info["res_init"]
     xd = spl.spsolve(A.tocsc(), b)
     assert np.linalg.norm(x - xd) <= 1e-8 * np.linalg.norm(xd)
+
+
+# ---- matrix-free stencil operator ----------------------------------------------
+def _dense_from_cells(order, dim, nc, mats, factor, cell_lm, fixed):
+    """Dense operator assembled from the cell matrices with its own node map
+    (grid coordinates -> np.ravel_multi_index), unit rows on fixed dofs."""
+    q = order + 1
+    shape = tuple(order * nc[d] + 1 for d in range(dim))     # nodes per direction (x first)
+    N = int(np.prod(shape))
+    A = np.zeros((N, N))
+    loc = np.array(np.meshgrid(*[np.arange(q)] * dim, indexing="ij")).reshape(dim, -1)
+    loc = loc[::-1]                                          # local index = x fastest
+    cells = np.array(np.meshgrid(*[np.arange(nc[d]) for d in range(dim)][::-1],
+                                 indexing="ij")).reshape(dim, -1)[::-1]
+    for k in range(cells.shape[1]):
+        g = order * cells[:, k:k + 1] + loc                  # (dim, nloc) grid coords
+        dofs = np.ravel_multi_index(tuple(g[::-1]), shape[::-1])
+        m = 0 if cell_lm is None else cell_lm[k]
+        A[np.ix_(dofs, dofs)] += factor[k] * mats[m]
+    A[fixed, :] = 0
+    A[fixed, fixed] = 1
+    return A
+
+
+@pytest.mark.parametrize("order,dim,n", [(1, 2, 9), (1, 3, 6), (2, 2, 7)])
+def test_stencil_oracle_matches_assembled_csr(order, dim, n):
+    kap = O.fe_kappa(dim, n, 1.0, 17)
+    rp, ci, v = O.fe_system(order, dim, n, kap)
+    E = O.fe_element(order, dim, n)
+    fixed = np.nonzero(O.fe_boundary_mask(order, dim, n))[0]
+    x = np.random.default_rng(3).uniform(-1, 1, len(rp) - 1)
+    y = O.stencil_vmadd(order, dim, [n] * dim, E, kap, fixed, x)
+    assert_close = spmv_abs(rp, ci, v, x)
+    assert np.all(np.abs(y - O.spmv(rp, ci, v, x)) <= 1e-13 * assert_close + 1e-300)
+    # the diagonal cell-by-cell assembly puts on each row is the stencil's
+    assert np.array_equal(O.invdiag(rp, ci, v)[fixed], np.ones(len(fixed)))
+
+
+@pytest.mark.parametrize("order,dim,nc", [(1, 2, (4, 3)), (1, 3, (3, 2, 4)), (2, 2, (2, 3))])
+def test_stencil_oracle_nonsquare_multi_matrix(order, dim, nc):
+    # two local matrices picked per cell, non-cubic mesh, scaled update
+    rng = np.random.default_rng(5)
+    nloc = (order + 1) ** dim
+    mats = rng.uniform(-1, 1, (2, nloc, nloc))
+    ncell = int(np.prod(nc))
+    f = rng.uniform(0.5, 2, ncell)
+    lm = rng.integers(0, 2, ncell).astype(np.uint8)
+    N = int(np.prod([order * c + 1 for c in nc]))
+    fixed = np.array(sorted(rng.choice(N, N // 5, replace=False)), np.int32)
+    A = _dense_from_cells(order, dim, nc, mats, f, lm, fixed)
+    x, y0 = rng.uniform(-1, 1, N), rng.uniform(-1, 1, N)
+    y = O.stencil_vmadd(order, dim, nc, mats, f, fixed, x, y0, alpha=0.75, beta=-1.5,
+                        cell_lm=lm)
+    ref = -1.5 * y0 + 0.75 * (A @ x)
+    ref[fixed] = x[fixed]
+    scale = np.abs(y0) * 1.5 + 0.75 * (np.abs(A) @ np.abs(x))
+    assert np.all(np.abs(y - ref) <= 1e-13 * scale + 1e-300)
+
+
+@pytest.mark.parametrize("dim,n,modes", [(2, 24, (2, 5)), (3, 10, (1, 2, 3))])
+def test_stencil_oracle_sine_mode(dim, n, modes):
+    s = sine_mode(1, dim, n, modes)
+    fixed = np.nonzero(O.fe_boundary_mask(1, dim, n))[0]
+    y = O.stencil_vmadd(1, dim, [n] * dim, O.fe_element(1, dim, n), np.ones(n ** dim), fixed, s)
+    lam = q1_sine_eigenvalue(dim, n, modes)
+    inner = np.setdiff1d(np.arange(len(s)), fixed)
+    assert np.all(np.abs(y - lam * s)[inner] <= 1e-13 * np.abs(lam) * (np.abs(s).max()))
+    assert np.array_equal(y[fixed], s[fixed])
