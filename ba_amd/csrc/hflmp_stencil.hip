@@ -22,7 +22,10 @@ __global__ void __launch_bounds__(kBlock)
   T s = 0;
   if (MODE == kStScaleAdd) {
     const long long r = op.thread_row(unit);
-    if (r < op.nrow) s = y[r] * beta;
+    // + 0 turns a -0 into +0: the running sum never holds -0 (the
+    // reference's sum differs from it only in the sign of that zero,
+    // which the first cell's term overwrites either way)
+    if (r < op.nrow) s = y[r] * beta + T(0);
   }
   TileRow tr;
   s = op.row_sum_scaled(unit, x, s, MODE == kStScaleAdd ? alpha : T(1), tr);

@@ -64,6 +64,24 @@ __device__ __forceinline__ unsigned before_start(unsigned off) {
   return (int)off < 0 ? 0x80000000u : off;
 }
 
+// Division of n < 2^31 by a fixed d >= 1 as a multiply-high (Granlund and
+// Montgomery, PLDI 1994): with s = ceil(log2 d) and M = floor(2^(32+s) / d)
+// + 1 < 2^33, n / d = floor(n M / 2^(32+s)); the error n (M - 2^(32+s)/d) /
+// 2^(32+s) < 2^31 / 2^(32+s) <= 1 / (2 d) never crosses an integer.
+struct DivMagic {
+  unsigned long long m;
+  int shift;
+  __host__ void init(unsigned d) {
+    int s = 0;
+    while ((1ull << s) < d) ++s;
+    shift = 32 + s;
+    m = ((1ull << shift) / d) + 1;
+  }
+  __device__ __forceinline__ int div(int n) const {
+    return (int)(((unsigned long long)(unsigned)n * m) >> shift);
+  }
+};
+
 // Operator view for the fused kernels: unit = 256 consecutive nodes.
 // ORDER, DIM fixed at compile time; MULTI: per-cell matrix index.
 template <typename T, int DIM, int ORDER, bool MULTI>
@@ -77,6 +95,7 @@ struct StencilOp {
   static constexpr int WZ = DIM == 3 ? W : 1;
   int nrow;
   int ncx, ncy, ncz, px, py, pz;
+  DivMagic dpx, dpy;  // division by px and py
   const T *E;
   const T *f;
   const unsigned char *lm;
@@ -87,7 +106,8 @@ struct StencilOp {
   }
 
   // s + sum over the node's cells (ascending) of 0 + (alpha f_c) (E_m x_c)[local],
-  // or x[row] on a fixed node.
+  // or x[row] on a fixed node.  s must not be -0 (callers start from +0 or
+  // add +0 to a scaled y).
   __device__ __forceinline__ T row_sum_scaled(long long unit, const T *__restrict__ x, T s,
                                               T alpha, TileRow &tr) const {
 #pragma clang fp contract(off)
@@ -96,10 +116,10 @@ struct StencilOp {
     tr.row = tr.has_row ? (int)r : 0;
     if (!tr.has_row) return s;
     const int i = (int)r;
-    const int gx = i % px;
-    const int t = i / px;
-    const int gy = t % py;
-    const int gz = DIM == 3 ? t / py : 0;
+    const int t = dpx.div(i);
+    const int gx = i - t * px;
+    const int gz = DIM == 3 ? dpy.div(t) : 0;
+    const int gy = t - gz * py;
     // Every load is issued before the arithmetic: the x neighbourhood, the
     // factors (and matrix indices) of the node's cells, the fixed-dof word.
     const int pxy = px * py;
@@ -188,8 +208,11 @@ struct StencilOp {
                                                           Q * (az > ORDER ? ORDER : az));
           const T *Er = Em + li * NLOC;
           // local column j = o + a - ORDER for neighbour offset o: ascending o
-          // is ascending j, the order of the reference's row sum
-          T tmp = 0;
+          // is ascending j, the order of the reference's row sum.  It starts
+          // from -0, the identity of addition, so tmp is the plain sum of
+          // the products; the reference's start from +0 differs only in the
+          // sign of an all-zero sum, which the two steps below do not see.
+          T tmp = T(-0.0);
 #pragma unroll
           for (int oz = 0; oz < WZ; ++oz)
 #pragma unroll
@@ -202,9 +225,12 @@ struct StencilOp {
                   continue;
                 tmp += Er[jx + Q * (jy + Q * jz)] * xn[oz][oy][ox];
               }
+          // The reference adds lhs = 0 + (alpha f_c) tmp.  Adding (alpha f_c)
+          // tmp itself gives the same sum: the two differ only for a product
+          // of -0, and a running sum that starts from +0 never holds -0, so
+          // adding -0 or +0 leaves it unchanged.
           const T cf = alpha * fc[kz][ky][kx];
-          const T lhs = T(0) + cf * tmp;
-          const T sn = s + lhs;
+          const T sn = s + cf * tmp;
           s = ok ? sn : s;
         }
     if ((fixed_word >> (i & 31)) & 1u) s = xn[DIM == 3 ? ORDER : 0][ORDER][ORDER];
@@ -227,6 +253,8 @@ inline StencilOp<T, DIM, ORDER, MULTI> stencil_view(const hflmp_stencil *m) {
   op.px = m->np[0];
   op.py = m->np[1];
   op.pz = m->np[2];
+  op.dpx.init((unsigned)m->np[0]);
+  op.dpy.init((unsigned)m->np[1]);
   op.E = static_cast<const T *>(m->E);
   op.f = static_cast<const T *>(m->factor);
   op.lm = m->cell_lm;

@@ -833,6 +833,13 @@ def lmp_lib():
         L.hflmp_lmp_vector_mult.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
                                             C.c_void_p, C.c_void_p, C.c_void_p]
         L.hflmp_lmp_last_error.restype = C.c_char_p
+        ip = C.POINTER(C.c_int)
+        L.hflmp_lmp_mm_read_coo.argtypes = [C.c_char_p, ip, ip, ip, C.c_void_p, C.c_void_p,
+                                            C.c_void_p, C.c_int]
+        L.hflmp_lmp_mm_write_csr.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_void_p,
+                                             C.c_void_p, C.c_void_p]
+        L.hflmp_lmp_file_spmv.argtypes = [C.c_int, C.c_char_p, C.c_char_p, C.c_char_p,
+                                          C.c_char_p, ip, ip]
         _lmp = L
     return _lmp
 
@@ -875,3 +882,46 @@ def lmp_vector_mult(rp, ci, v, x, device=0):
     if rc:
         raise HflmpError("hflmp_lmp_vector_mult: " + lmp_lib().hflmp_lmp_last_error().decode())
     return y
+
+
+def _lmp_check(rc, what):
+    if rc:
+        raise HflmpError(what + ": " + lmp_lib().hflmp_lmp_last_error().decode())
+
+
+def mm_read_coo(path):
+    """MatrixMarket coordinate file as the lmp reader sees it: (nrow, ncol,
+    rows, cols, vals), 0-based, in file order, mirrors of a symmetric file
+    after their entry."""
+    L = lmp_lib()
+    m, n, k = C.c_int(), C.c_int(), C.c_int()
+    p = os.fsencode(path)
+    _lmp_check(L.hflmp_lmp_mm_read_coo(p, C.byref(m), C.byref(n), C.byref(k), None, None, None,
+                                       0), "mm_read_coo")
+    r = np.empty(k.value, np.int32)
+    c = np.empty(k.value, np.int32)
+    v = np.empty(k.value, np.float64)
+    _lmp_check(L.hflmp_lmp_mm_read_coo(p, C.byref(m), C.byref(n), C.byref(k), r.ctypes.data,
+                                       c.ctypes.data, v.ctypes.data, k.value), "mm_read_coo")
+    return m.value, n.value, r, c, v
+
+
+def mm_write_csr(path, rp, ci, v, ncol=None):
+    rp = np.ascontiguousarray(rp, np.int32)
+    ci = np.ascontiguousarray(ci, np.int32)
+    v = np.ascontiguousarray(v, np.float64)
+    nrow = len(rp) - 1
+    _lmp_check(lmp_lib().hflmp_lmp_mm_write_csr(os.fsencode(path), nrow,
+                                                nrow if ncol is None else ncol, rp.ctypes.data,
+                                                ci.ctypes.data, v.ctypes.data), "mm_write_csr")
+
+
+def lmp_file_spmv(matrix_file, x_file, y_file, matrix_out=None, device=0):
+    """y = A x on the HIP platform with A, x read from files (ReadFile) and y
+    written (WriteFile); returns (nrow, nnz)."""
+    nrow, nnz = C.c_int(), C.c_int()
+    _lmp_check(lmp_lib().hflmp_lmp_file_spmv(
+        device, os.fsencode(matrix_file), os.fsencode(x_file), os.fsencode(y_file),
+        None if matrix_out is None else os.fsencode(matrix_out), C.byref(nrow), C.byref(nnz)),
+        "lmp_file_spmv")
+    return nrow.value, nnz.value

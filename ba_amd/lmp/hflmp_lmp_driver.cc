@@ -8,6 +8,9 @@
 
 #include "hflmp_lmp.h"
 #include "la_descriptor.h"
+#include "lmatrix_csr_hip.h"
+#include "lmp_mmio.h"
+#include "lvector_hip.h"
 #include "linear_solver/cg.h"
 #include "linear_solver/gmres.h"
 #include "linear_solver/mpir.h"
@@ -203,6 +206,78 @@ int hflmp_lmp_vector_mult(int device, int nrow, int nnz, const int *rp, const in
     vx.interior().SetBlockValues(0, nrow, x);
     A.VectorMult(vx, &vy);
     vy.interior().GetBlockValues(0, nrow, y);
+  } catch (const std::exception &e) {
+    g_last_error = e.what();
+    return -1;
+  }
+  return 0;
+}
+
+int hflmp_lmp_mm_read_coo(const char *path, int *nrow, int *ncol, int *nentries, int *rows,
+                          int *cols, double *vals, int capacity) {
+  if (!path || !nrow || !ncol || !nentries) {
+    g_last_error = "hflmp_lmp_mm_read_coo: null argument";
+    return -1;
+  }
+  MmCoo coo;
+  const std::string err = mm_read_coo(path, coo);
+  if (!err.empty()) {
+    g_last_error = err;
+    return -1;
+  }
+  *nrow = coo.nrow;
+  *ncol = coo.ncol;
+  *nentries = (int)coo.val.size();
+  if (!rows || !cols || !vals) return 0;
+  if (capacity < (int)coo.val.size()) {
+    g_last_error = "hflmp_lmp_mm_read_coo: capacity below the number of entries";
+    return -1;
+  }
+  std::memcpy(rows, coo.row.data(), sizeof(int) * coo.row.size());
+  std::memcpy(cols, coo.col.data(), sizeof(int) * coo.col.size());
+  std::memcpy(vals, coo.val.data(), sizeof(double) * coo.val.size());
+  return 0;
+}
+
+int hflmp_lmp_mm_write_csr(const char *path, int nrow, int ncol, const int *rp, const int *ci,
+                           const double *val) {
+  if (!path || !rp || nrow < 0 || ncol < 0 || (rp[nrow] > 0 && (!ci || !val))) {
+    g_last_error = "hflmp_lmp_mm_write_csr: bad arguments";
+    return -1;
+  }
+  const std::string err = mm_write_csr(path, nrow, ncol, rp, ci, val);
+  if (!err.empty()) {
+    g_last_error = err;
+    return -1;
+  }
+  return 0;
+}
+
+int hflmp_lmp_file_spmv(int device, const char *matrix_file, const char *x_file,
+                        const char *y_file, const char *matrix_out, int *nrow, int *nnz) {
+  ThrowOnFatal guard;
+  if (!matrix_file || !x_file || !y_file) {
+    g_last_error = "hflmp_lmp_file_spmv: null file name";
+    return -1;
+  }
+  try {
+    SYSTEM sys;
+    sys.Platform = HIP;
+    sys.device = device;
+    init_platform(sys);
+    HIP_CSR_lMatrix<double> A;
+    A.ReadFile(matrix_file);
+    HIP_lVector<double> x, y;
+    x.ReadFile(x_file);
+    if (x.get_size() != A.get_num_col())
+      LMP_FATAL("x has " << x.get_size() << " entries, the matrix " << A.get_num_col()
+                         << " columns");
+    y.Init(A.get_num_row(), "y");
+    A.VectorMult(x, &y);
+    y.WriteFile(y_file);
+    if (matrix_out) A.WriteFile(matrix_out);
+    if (nrow) *nrow = A.get_num_row();
+    if (nnz) *nnz = A.get_nnz();
   } catch (const std::exception &e) {
     g_last_error = e.what();
     return -1;

@@ -7,7 +7,11 @@
 #include <iostream>
 #include <string>
 
+#include <vector>
+
 #include "la_global.h"
+#include "lmp_log.h"
+#include "lmp_mmio.h"
 #include "lvector.h"
 
 namespace hiflow {
@@ -37,6 +41,11 @@ class lMatrix {
                     const std::string init_name) = 0;
   virtual void Clear() = 0;
   virtual void Zeros() = 0;
+  // MatrixMarket coordinate file (lmatrix.h:344-348; CSR through COO,
+  // lmatrix_csr_cpu.cc:1781-1810): read replaces structure and values
+  // (duplicate entries are added), write stores the matrix in CSR order.
+  virtual void ReadFile(const char *filename);
+  virtual void WriteFile(const char *filename) const;
 
   // structure from (row, col) pairs; duplicates are merged, columns sorted
   // ascending per row (init_structure / coo_2_csr, lmatrix_formats.h:320-456)
@@ -92,6 +101,23 @@ void lMatrix<ValueType>::print(std::ostream &out) const {
       << " cols=" << num_col_ << " precision=" << 8 * sizeof(ValueType)
       << "bit format=" << get_matrix_format_name() << " platform=" << get_platform_name()
       << " implementation=" << get_implementation_name() << std::endl;
+}
+
+template <typename ValueType>
+void lMatrix<ValueType>::ReadFile(const char *filename) {
+  MmCoo coo;
+  const std::string err = mm_read_coo(filename, coo);
+  if (!err.empty()) LMP_FATAL("lMatrix::ReadFile(" << filename << "): " << err);
+  std::vector<ValueType> v(coo.val.begin(), coo.val.end());
+  this->Clear();
+  this->Init((int)v.size(), coo.nrow, coo.ncol, filename);
+  this->TransformFromCOO(coo.row.data(), coo.col.data(), v.data(), coo.nrow, coo.ncol,
+                         (int)v.size());
+}
+
+template <typename ValueType>
+void lMatrix<ValueType>::WriteFile(const char *filename) const {
+  LMP_FATAL("lMatrix::WriteFile() does not support the format " << get_matrix_format_name());
 }
 
 template <typename ValueType>

@@ -226,6 +226,19 @@ void HIP_CSR_lMatrix<T>::SetCSR(const int *row_ptr, const int *col, const T *val
 }
 
 template <typename T>
+void HIP_CSR_lMatrix<T>::WriteFile(const char *filename) const {
+  std::vector<T> v((size_t)this->nnz_);
+  if (this->nnz_)
+    hip_check(hflmp_memcpy_d2h(hip_ctx(), v.data(), val_, sizeof(T) * v.size()), "val download");
+  std::vector<double> d(v.begin(), v.end());
+  const std::vector<int> rp = h_row_.empty() ? std::vector<int>((size_t)this->num_row_ + 1, 0)
+                                             : h_row_;
+  const std::string err = mm_write_csr(filename, this->num_row_, this->num_col_, rp.data(),
+                                       h_col_.data(), d.data());
+  if (!err.empty()) LMP_FATAL("HIP_CSR_lMatrix::WriteFile(" << filename << "): " << err);
+}
+
+template <typename T>
 int HIP_CSR_lMatrix<T>::position(int row, int col) const {
   if (row < 0 || row >= this->num_row_) return -1;
   const auto b = h_col_.begin() + h_row_[row], e = h_col_.begin() + h_row_[row + 1];

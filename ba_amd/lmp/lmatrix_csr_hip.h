@@ -39,6 +39,8 @@ class HIP_CSR_lMatrix : public lMatrix<ValueType> {
                         const int num_nonzeros) override;
   // CSR arrays given directly (host), columns sorted ascending in each row
   void SetCSR(const int *row_ptr, const int *col, const ValueType *val);
+  // MatrixMarket "coordinate real general", entries in CSR order
+  void WriteFile(const char *filename) const override;
 
   void add_value(const int row, const int col, const ValueType val) override;
   void add_values(const int *rows, int num_rows, const int *cols, int num_cols,

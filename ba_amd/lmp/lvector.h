@@ -9,8 +9,11 @@
 
 #include <iostream>
 #include <string>
+#include <vector>
 
 #include "la_global.h"
+#include "lmp_log.h"
+#include "lmp_mmio.h"
 
 namespace hiflow {
 namespace la {
@@ -35,6 +38,11 @@ class lVector {
   virtual void Init(const int size, const std::string name) = 0;
   virtual void Clear() = 0;
   virtual void Zeros() = 0;
+  // plain text, one value per line (lvector_cpu.cc:496-565): read sizes
+  // the vector to the values in the file; write stores 17 significant
+  // digits, so a round trip is exact
+  virtual void ReadFile(const char *filename);
+  virtual void WriteFile(const char *filename);
 
   // element access (host arrays)
   virtual void add_value(const int i, ValueType val) = 0;
@@ -100,6 +108,26 @@ class lVector {
   IMPLEMENTATION implementation_id_ = HIPKERNEL;
   PLATFORM platform_id_ = HIP;
 };
+
+template <typename ValueType>
+void lVector<ValueType>::ReadFile(const char *filename) {
+  std::vector<double> v;
+  const std::string err = vec_read(filename, v);
+  if (!err.empty()) LMP_FATAL("lVector::ReadFile(" << filename << "): " << err);
+  std::vector<ValueType> t(v.begin(), v.end());
+  this->Clear();
+  this->Init((int)t.size(), filename);
+  if (!t.empty()) this->SetBlockValues(0, (int)t.size(), t.data());
+}
+
+template <typename ValueType>
+void lVector<ValueType>::WriteFile(const char *filename) {
+  std::vector<ValueType> t((size_t)size_);
+  if (size_ > 0) this->GetBlockValues(0, size_, t.data());
+  std::vector<double> v(t.begin(), t.end());
+  const std::string err = vec_write(filename, v.data(), (int)v.size());
+  if (!err.empty()) LMP_FATAL("lVector::WriteFile(" << filename << "): " << err);
+}
 
 template <typename ValueType>
 void lVector<ValueType>::print(std::ostream &out) const {

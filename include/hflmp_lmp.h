@@ -54,6 +54,21 @@ int hflmp_lmp_solve(int device, int nrow, int nnz, const int *row_ptr, const int
 /* y = A x through CoupledMatrix::VectorMult on the HIP platform. */
 int hflmp_lmp_vector_mult(int device, int nrow, int nnz, const int *row_ptr, const int *col,
                           const double *val, const double *x, double *y);
+/* MatrixMarket coordinate files (lMatrix::ReadFile / WriteFile,
+ * lmatrix_csr_cpu.cc:1781-1810 through lmatrix_coo_cpu.cc:123-279), host
+ * only.  read_coo: the entries as the reader produces them (0-based, file
+ * order, a symmetric file's mirrors after their entry); with null arrays it
+ * only reports the sizes.  write_csr: "coordinate real general", entries in
+ * CSR order, 17 significant digits. */
+int hflmp_lmp_mm_read_coo(const char *path, int *nrow, int *ncol, int *nentries, int *rows,
+                          int *cols, double *vals, int capacity);
+int hflmp_lmp_mm_write_csr(const char *path, int nrow, int ncol, const int *row_ptr,
+                           const int *col, const double *val);
+/* y = A x with A read by HIP_CSR_lMatrix::ReadFile and x by
+ * lVector::ReadFile (one value per line), y written by lVector::WriteFile;
+ * matrix_out (optional) receives HIP_CSR_lMatrix::WriteFile. */
+int hflmp_lmp_file_spmv(int device, const char *matrix_file, const char *x_file,
+                        const char *y_file, const char *matrix_out, int *nrow, int *nnz);
 const char *hflmp_lmp_last_error(void);
 
 #ifdef __cplusplus

@@ -366,3 +366,51 @@ def mpir(rp, ci, vd, b, x0=None, max_outer=50, atol=0.0, rtol=1e-10,
                           C.byref(oi), C.byref(ii), C.byref(r0), C.byref(rf))
     return x, dict(status=st, outer=oi.value, inner_total=ii.value,
                    res_init=r0.value, res_final=rf.value)
+
+
+# ---- MatrixMarket files ----------------------------------------------------------
+def mm_read(path):
+    """CPU_COO_lMatrix::ReadFile (lmatrix_coo_cpu.cc:123-243): banner, size
+    line, 1-based "row col value" entries; a symmetric file adds the mirror
+    of every off-diagonal entry right after it.  Skew-symmetric mirrors get
+    the opposite sign (the MatrixMarket definition; the reference keeps only
+    the stored triangle).  Returns (nrow, ncol, rows, cols, vals)."""
+    with open(path) as f:
+        banner = f.readline().split()
+        if len(banner) != 5 or banner[0].lower() != "%%matrixmarket":
+            raise ValueError("no MatrixMarket banner")
+        obj, fmt, field, sym = (t.lower() for t in banner[1:])
+        if obj != "matrix" or fmt != "coordinate" or field not in ("real", "integer", "double"):
+            raise ValueError("unsupported MatrixMarket type")
+        if sym not in ("general", "symmetric", "skew-symmetric"):
+            raise ValueError("unsupported symmetry")
+        lines = (ln for ln in f if ln.strip() and not ln.lstrip().startswith("%"))
+        m, n, nz = (int(t) for t in next(lines).split()[:3])
+        if sym != "general" and m != n:
+            raise ValueError("a symmetric matrix must be square")
+        rows, cols, vals = [], [], []
+        for _ in range(nz):
+            t = next(lines).split()
+            i, j, v = int(t[0]) - 1, int(t[1]) - 1, float(t[2])
+            if not (0 <= i < m and 0 <= j < n):
+                raise ValueError("entry out of range")
+            rows.append(i); cols.append(j); vals.append(v)
+            if sym != "general" and i != j:
+                rows.append(j); cols.append(i); vals.append(-v if sym == "skew-symmetric" else v)
+    return (m, n, np.array(rows, np.int32), np.array(cols, np.int32), np.array(vals, np.float64))
+
+
+def coo_to_csr_merge(nrow, rows, cols, vals):
+    """Sorted CSR with duplicates summed in their order of appearance (the
+    product's structure build; coo_2_csr, lmatrix_formats.h:320-456,
+    sorts the columns of each row)."""
+    order = np.lexsort((np.arange(len(rows)), cols, rows))
+    r, c, v = rows[order], cols[order], vals[order]
+    rp = np.zeros(nrow + 1, np.int32)
+    ci, vv = [], []
+    for k in range(len(r)):
+        if k > 0 and r[k] == r[k - 1] and c[k] == c[k - 1]:
+            vv[-1] = vv[-1] + v[k]
+        else:
+            ci.append(c[k]); vv.append(v[k]); rp[r[k] + 1] += 1
+    return np.cumsum(rp).astype(np.int32), np.array(ci, np.int32), np.array(vv, np.float64)

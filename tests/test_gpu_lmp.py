@@ -116,3 +116,44 @@ def test_mcsgs_rejects_bad_colouring(ctx):
     bad = (1, np.array([N], np.int32), np.arange(N, dtype=np.int32))  # one colour: coupled rows
     with pytest.raises(hflmp.HflmpError):
         hflmp.McSgs(ctx, A, bad)
+
+
+@pytest.mark.parametrize("symmetric", [False, True])
+def test_file_spmv_unstructured(tmp_path, symmetric):
+    # an FE matrix renumbered at random (irregular band), its entries
+    # shuffled in the file and some split into two exact halves: ReadFile
+    # sorts and merges, VectorMult (SELL-64) gives the sequential CSR bits
+    rng = np.random.default_rng(8)
+    rp, ci, v = O.fe_system(1, 3, 9, O.fe_kappa(3, 9, 1.0, 4))
+    n = len(rp) - 1
+    perm = rng.permutation(n)
+    A = sp.csr_matrix((v, ci, rp), shape=(n, n))[perm][:, perm].tocoo()
+    r, c, vals = A.row + 1, A.col + 1, A.data
+    if symmetric:
+        keep = r >= c
+        r, c, vals = r[keep], c[keep], vals[keep]
+    # split about a third of the entries into two halves (exact in binary)
+    split = rng.random(len(vals)) < 0.3
+    r = np.concatenate([r, r[split]])
+    c = np.concatenate([c, c[split]])
+    vals = np.concatenate([np.where(split, vals * 0.5, vals), vals[split] * 0.5])
+    order = rng.permutation(len(vals))
+    path = tmp_path / "A.mtx"
+    with open(path, "w") as f:
+        f.write("%%%%MatrixMarket matrix coordinate real %s\n%d %d %d\n"
+                % ("symmetric" if symmetric else "general", n, n, len(vals)))
+        for k in order:
+            f.write("%d %d %.17g\n" % (r[k], c[k], vals[k]))
+    x = rng.uniform(-1, 1, n)
+    xf, yf, outf = tmp_path / "x.txt", tmp_path / "y.txt", tmp_path / "B.mtx"
+    np.savetxt(xf, x, fmt="%.17g")
+    nrow, nnz = hflmp.lmp_file_spmv(str(path), str(xf), str(yf), str(outf))
+    m, _, rr, cc, vv = O.mm_read(str(path))
+    rp2, ci2, v2 = O.coo_to_csr_merge(m, rr, cc, vv)
+    assert nrow == n and nnz == len(ci2)
+    y = np.loadtxt(yf)
+    assert np.array_equal(y, O.spmv(rp2, ci2, v2, x))
+    # WriteFile gives the merged CSR back exactly
+    _, _, r3, c3, v3 = hflmp.mm_read_coo(str(outf))
+    rp3, ci3, v4 = O.coo_to_csr_merge(m, r3, c3, v3)
+    assert np.array_equal(rp3, rp2) and np.array_equal(ci3, ci2) and np.array_equal(v4, v2)
