@@ -119,6 +119,13 @@ def hip_lib():
             getattr(L, "hflmp_pcg_jacobi_stencil_" + s).argtypes = [vp, vp, vp, vp, vp, i, d, d,
                                                                      d, vp, sz, vp]
         L.hflmp_stencil_get_info.argtypes = [vp, vp]
+        L.hflmp_fe_mesh_create.argtypes = [vp, i, i, i, vp, vp]
+        L.hflmp_fe_mesh_nnz.argtypes = [vp, C.POINTER(C.c_longlong)]
+        L.hflmp_fe_mesh_pattern.argtypes = [vp, vp, vp, vp]
+        for s in ("f64", "f32"):
+            getattr(L, "hflmp_fe_assemble_" + s).argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
+        L.hflmp_fe_mesh_destroy.argtypes = [vp]
+        L.hflmp_fe_mesh_destroy.restype = None
         L.hflmp_stencil_destroy.argtypes = [vp]
         L.hflmp_stencil_destroy.restype = None
         L.hflmp_pcg_stencil_workspace_bytes.restype = sz
@@ -227,6 +234,24 @@ def fe_fixed(order, dim, n):
     else:
         m = edge[None, None, :] | edge[None, :, None] | edge[:, None, None]
     return np.nonzero(m.ravel())[0].astype(np.int32)
+
+
+def fe_cell_dofs(order, dim, n):
+    """Cell -> dof table of the structured mesh: cells and their local
+    nodes lexicographic (x fastest), int32 array ncell x (order+1)^dim."""
+    p = order * n + 1
+    loc = np.arange(order + 1)
+    c = np.arange(n)
+    if dim == 2:
+        cy, cx = np.meshgrid(c, c, indexing="ij")
+        ly, lx = np.meshgrid(loc, loc, indexing="ij")
+        g = [order * cx.reshape(-1, 1) + lx.reshape(1, -1),
+             order * cy.reshape(-1, 1) + ly.reshape(1, -1)]
+        return (g[0] + p * g[1]).astype(np.int32)
+    cz, cy, cx = np.meshgrid(c, c, c, indexing="ij")
+    lz, ly, lx = np.meshgrid(loc, loc, loc, indexing="ij")
+    g = [order * a.reshape(-1, 1) + b.reshape(1, -1) for a, b in ((cx, lx), (cy, ly), (cz, lz))]
+    return (g[0] + p * (g[1] + p * g[2])).astype(np.int32)
 
 
 def fe_rhs(order, dim, n, seed):
@@ -551,6 +576,53 @@ def pcg_jacobi_sell(ctx, S, invd, b, x, maxit=10000, atol=0.0, rtol=1e-8, dtol=1
         if own:
             work.free()
     return info.as_dict()
+
+
+class FeMesh:
+    """Device assembly of a general mesh (hflmp_fe_*): cell_dofs is a host
+    ncell x nloc array of node indices."""
+
+    def __init__(self, ctx, nnode, cell_dofs):
+        self.ctx = ctx
+        dofs = np.ascontiguousarray(cell_dofs, np.int32)
+        self.nnode = int(nnode)
+        self.ncell, self.nloc = dofs.shape
+        d = ctx.upload(dofs)
+        self.h = C.c_void_p()
+        try:
+            _call(ctx, "hflmp_fe_mesh_create", self.nnode, self.ncell, self.nloc, d.ptr,
+                  C.byref(self.h))
+        finally:
+            d.free()
+        z = C.c_longlong()
+        ctx.check(ctx.L.hflmp_fe_mesh_nnz(self.h, C.byref(z)), "fe_mesh_nnz")
+        self.nnz = z.value
+
+    def pattern(self):
+        rp = self.ctx.empty(self.nnode + 1, np.int32)
+        ci = self.ctx.empty(max(self.nnz, 1), np.int32)
+        _call(self.ctx, "hflmp_fe_mesh_pattern", self.h, rp.ptr, ci.ptr)
+        return rp, ci
+
+    def assemble(self, rp, ci, E=None, factor=None, cell_mats=None, dtype=np.float64):
+        dt = np.dtype(dtype)
+        up = [None if a is None else self.ctx.upload(np.ascontiguousarray(a, dt))
+              for a in (cell_mats, E, factor)]
+        val = self.ctx.empty(max(self.nnz, 1), dt)
+        try:
+            _call(self.ctx, "hflmp_fe_assemble_" + _sfx(dt), self.h,
+                  *[None if a is None else a.ptr for a in up], rp.ptr, ci.ptr, val.ptr)
+            self.ctx.sync()
+        finally:
+            for a in up:
+                if a is not None:
+                    a.free()
+        return val
+
+    def free(self):
+        if self.h:
+            self.ctx.L.hflmp_fe_mesh_destroy(self.h)
+            self.h = C.c_void_p()
 
 
 class StencilDesc(C.Structure):

@@ -26,6 +26,7 @@ Reported beside it, each measured in the same run:
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 128]
 """
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -227,6 +228,54 @@ def bench_stencil(hflmp, ctx, n, x, b, steps, sell_cg=None, assembled=None):
         out["cg"]["iters_sell_cg"] = sell_cg["iters"]
     St.free(); dx.free(); dy.free(); invd.free(); db.free()
     return out
+
+
+def bench_assembly(hflmp, n, kappa_seed=KAPPA_SEED):
+    """Device assembly of the 3D Q1 n^3 system from its cell -> dof table
+    (pattern, then values cell by cell), with the Dirichlet rows applied,
+    against the host generator's CSR of the same system (bit-identical)."""
+    kap = hflmp.fe_kappa(3, n, 1.0, kappa_seed)
+    t0 = time.perf_counter()
+    rp_h, ci_h, v_h = hflmp.fe_system(1, 3, n, kap)
+    t_host = time.perf_counter() - t0
+    dofs = hflmp.fe_cell_dofs(1, 3, n)
+    E = hflmp.fe_element(1, 3, n)
+    N = (n + 1) ** 3
+    fixed = hflmp.fe_fixed(1, 3, n)
+    ctx = hflmp.Context(0)
+    dd, dE, df, dfix = ctx.upload(dofs), ctx.upload(E), ctx.upload(kap), ctx.upload(fixed)
+    times = []
+    for rep in range(3):
+        ctx.sync()
+        t0 = time.perf_counter()
+        h = C.c_void_p()
+        ctx.check(ctx.L.hflmp_fe_mesh_create(ctx.h, N, dofs.shape[0], dofs.shape[1], dd.ptr,
+                                             C.byref(h)), "fe_mesh_create")
+        z = C.c_longlong()
+        ctx.check(ctx.L.hflmp_fe_mesh_nnz(h, C.byref(z)), "fe_mesh_nnz")
+        rp, ci, val = ctx.empty(N + 1, np.int32), ctx.empty(z.value, np.int32), \
+            ctx.empty(z.value, np.float64)
+        ctx.check(ctx.L.hflmp_fe_mesh_pattern(ctx.h, h, rp.ptr, ci.ptr), "fe_mesh_pattern")
+        ctx.check(ctx.L.hflmp_fe_assemble_f64(ctx.h, h, None, dE.ptr, df.ptr, rp.ptr, ci.ptr,
+                                              val.ptr), "fe_assemble")
+        ctx.check(ctx.L.hflmp_csr_zero_rows_f64(ctx.h, len(fixed), dfix.ptr, rp.ptr, ci.ptr,
+                                                val.ptr, 1.0), "zero_rows")
+        ctx.sync()
+        times.append(time.perf_counter() - t0)
+        ctx.L.hflmp_fe_mesh_destroy(h)
+        if rep < 2:
+            rp.free(); ci.free(); val.free()
+    same = (np.array_equal(rp.download(), rp_h) and np.array_equal(ci.download(), ci_h)
+            and np.array_equal(val.download(), v_h))
+    for a in (rp, ci, val, dd, dE, df, dfix):
+        a.free()
+    ctx.close()
+    return {"workload": "3D Q1 %d^3 log-normal diffusivity: cell -> dof table to CSR with "
+                        "Dirichlet rows" % n, "nrow": N, "nnz": int(len(ci_h)),
+            "device_seconds": round(min(times), 4),
+            "host_generator_seconds": round(t_host, 4),
+            "host_generator_threads": host_threads(),
+            "bit_identical_to_host_csr": bool(same)}
 
 
 def bench_mixed(hflmp, n):
@@ -544,6 +593,7 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-stencil", action="store_true")
+    ap.add_argument("--no-assembly", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=200)
     ap.add_argument("--serve-seed", type=int, default=20241205)
@@ -665,6 +715,8 @@ def main():
     S.free(); A.free(); dx.free(); dy.free()
     ctx.close()
 
+    if not args.no_assembly:
+        out["assembly"] = bench_assembly(hflmp, n)
     if not args.no_mixed:
         out["mixed_precision"] = bench_mixed(hflmp, n)
     if not args.no_batched:

@@ -311,6 +311,38 @@ int hflmp_stencil_extract_invdiag_f64(hflmp_ctx *ctx, const hflmp_stencil *m,
 int hflmp_stencil_extract_invdiag_f32(hflmp_ctx *ctx, const hflmp_stencil *m,
                                       float *invdiag, int *n_bad);
 
+/* ---- finite-element assembly into CSR ---------------------------------------
+ * The reference assembles on the host: the pattern of every dof pair that
+ * shares a cell, columns ascending (compute_sparsity_structure,
+ * assembly/assembly_utils.cc:144; coo_2_csr, lmp/lmatrix_formats.h:
+ * 320-456), then the values cell by cell in ascending cell order into a
+ * zeroed matrix (add_values, lmp/lmatrix_csr_cpu.cc:1102-1153).  These do
+ * both on the device for any mesh given as a cell -> dof table (DEVICE
+ * int array, ncell x nloc, nloc <= 32), with values bit-identical to that
+ * cell loop.
+ *   mesh_create: node -> cell incidence and the pattern's size;
+ *   mesh_pattern: row_ptr[nnode + 1] and col[nnz] (DEVICE);
+ *   assemble: val[nnz] (DEVICE) = sum over cells c of the entries
+ *     factor[c] * E[a][b] (E: one nloc x nloc DEVICE matrix, factor NULL
+ *     for 1) or cell_mats[c][a][b] (ncell x nloc x nloc, when not NULL),
+ *     added in ascending cell order from 0.
+ * Dirichlet rows afterwards: hflmp_csr_zero_rows_*. */
+typedef struct hflmp_fe_mesh hflmp_fe_mesh;
+int hflmp_fe_mesh_create(hflmp_ctx *ctx, int nnode, int ncell, int nloc,
+                         const int *cell_dofs, hflmp_fe_mesh **out);
+int hflmp_fe_mesh_nnz(const hflmp_fe_mesh *m, long long *nnz);
+int hflmp_fe_mesh_pattern(hflmp_ctx *ctx, const hflmp_fe_mesh *m, int *row_ptr,
+                          int *col);
+int hflmp_fe_assemble_f64(hflmp_ctx *ctx, const hflmp_fe_mesh *m,
+                          const double *cell_mats, const double *E,
+                          const double *factor, const int *row_ptr,
+                          const int *col, double *val);
+int hflmp_fe_assemble_f32(hflmp_ctx *ctx, const hflmp_fe_mesh *m,
+                          const float *cell_mats, const float *E,
+                          const float *factor, const int *row_ptr,
+                          const int *col, float *val);
+void hflmp_fe_mesh_destroy(hflmp_fe_mesh *m);
+
 /* ---- preconditioners ------------------------------------------------------
  * Jacobi apply z = invdiag .* r fused into one pass
  * (lPreconditioner_Jacobi::ApplylPreconditioner, lpreconditioner.cc:184-200:

@@ -33,6 +33,8 @@
  *                 mpir.cc:240)
  *   stencil       linear_algebra/stencil_operator.h:153-330 (matrix-free
  *                 VectorMultAdd, cell by cell)
+ *   assembly      assembly/assembly_utils.cc:144 (pattern),
+ *                 lmatrix_csr_cpu.cc:1102-1153 (add_values, cell by cell)
  *
  * Problem generator: structured Q1 (2D/3D) and Q2 (2D) Poisson/diffusion on
  * the unit square/cube, assembled element by element in lexicographic cell
@@ -117,7 +119,11 @@ int or_multicoloring(int nrow, const int *rowptr, const int *col,
   void or_stencil_vmadd_##S(int order, int dim, const int *nc, const T *E,    \
                             const uint8_t *cell_lm, const T *cell_factor,     \
                             int nfixed, const int *fixed, T alpha,            \
-                            const T *in, T beta, T *out);
+                            const T *in, T beta, T *out);                  \
+  int or_fe_assemble_cells_##S(int nnode, int ncell, int nloc,                \
+                               const int *dofs, const T *cell_mats,           \
+                               const T *E, const T *factor, int *rp,          \
+                               int *col, T *val);
 
 OR_DECL(double, d)
 OR_DECL(float, s)

@@ -80,6 +80,8 @@ def lib():
                                 C.c_double, C.c_double, C.c_double,
                                 C.POINTER(C.c_int), C.POINTER(C.c_double),
                                 C.POINTER(C.c_double), _vp]
+            g("fe_assemble_cells").argtypes = [C.c_int, C.c_int, C.c_int, _i32p, _vp, _vp,
+                                               _vp, _i32p, _vp, _vp]
             g("stencil_vmadd").argtypes = [C.c_int, C.c_int, _i32p, fp, _vp, fp, C.c_int,
                                            _vp, ct, fp, ct, fp]
             g("gmres").argtypes = [C.c_int, _i32p, _i32p, fp, C.c_int, _vp,
@@ -414,3 +416,43 @@ def coo_to_csr_merge(nrow, rows, cols, vals):
         else:
             ci.append(c[k]); vv.append(v[k]); rp[r[k] + 1] += 1
     return np.cumsum(rp).astype(np.int32), np.array(ci, np.int32), np.array(vv, np.float64)
+
+
+def fe_assemble_cells(nnode, dofs, E=None, factor=None, cell_mats=None, dtype=np.float64):
+    """CSR of a general mesh (cell -> dof table `dofs`, ncell x nloc): the
+    host assembly of the reference (pattern, then add_values cell by
+    cell)."""
+    dofs = np.ascontiguousarray(dofs, np.int32)
+    ncell, nloc = dofs.shape
+    fn = getattr(lib(), "or_fe_assemble_cells_" + ("d" if dtype == np.float64 else "s"))
+    cm = None if cell_mats is None else np.ascontiguousarray(cell_mats, dtype)
+    e = None if E is None else np.ascontiguousarray(E, dtype)
+    f = None if factor is None else np.ascontiguousarray(factor, dtype)
+    rp = np.zeros(nnode + 1, np.int32)
+    if fn(nnode, ncell, nloc, dofs, _ptr(cm), _ptr(e), _ptr(f), rp, None, None):
+        raise ValueError("dof outside the mesh")
+    ci = np.empty(rp[-1], np.int32)
+    v = np.empty(rp[-1], dtype)
+    fn(nnode, ncell, nloc, dofs, _ptr(cm), _ptr(e), _ptr(f), rp, _ptr(ci), _ptr(v))
+    return rp, ci, v
+
+
+def fe_cell_dofs(order, dim, n):
+    """Cell -> dof table of the structured mesh (cells and their local
+    nodes lexicographic, x fastest)."""
+    p = order * n + 1
+    q = order + 1
+    c = np.arange(n)
+    loc = np.arange(q)
+    if dim == 2:
+        cy, cx = np.meshgrid(c, c, indexing="ij")
+        ly, lx = np.meshgrid(loc, loc, indexing="ij")
+        gx = order * cx.reshape(-1, 1) + lx.reshape(1, -1)
+        gy = order * cy.reshape(-1, 1) + ly.reshape(1, -1)
+        return (gx + p * gy).astype(np.int32)
+    cz, cy, cx = np.meshgrid(c, c, c, indexing="ij")
+    lz, ly, lx = np.meshgrid(loc, loc, loc, indexing="ij")
+    gx = order * cx.reshape(-1, 1) + lx.reshape(1, -1)
+    gy = order * cy.reshape(-1, 1) + ly.reshape(1, -1)
+    gz = order * cz.reshape(-1, 1) + lz.reshape(1, -1)
+    return (gx + p * (gy + p * gz)).astype(np.int32)

@@ -18,6 +18,11 @@ enum {
   OR_FAIL_MAXIT = 4
 };
 
+static int cmp_int_or(const void *a, const void *b) {
+  const int x = *(const int *)a, y = *(const int *)b;
+  return (x > y) - (x < y);
+}
+
 #define T double
 #define S d
 #include "oracle_real.inc"

@@ -479,3 +479,36 @@ def test_stencil_oracle_sine_mode(dim, n, modes):
     inner = np.setdiff1d(np.arange(len(s)), fixed)
     assert np.all(np.abs(y - lam * s)[inner] <= 1e-13 * np.abs(lam) * (np.abs(s).max()))
     assert np.array_equal(y[fixed], s[fixed])
+
+
+# ---- general-mesh assembly -----------------------------------------------------
+@pytest.mark.parametrize("order,dim,n", [(1, 2, 6), (1, 3, 4), (2, 2, 5)])
+def test_cell_assembly_equals_structured_assembly(order, dim, n):
+    kap = O.fe_kappa(dim, n, 1.0, 9)
+    N = (order * n + 1) ** dim
+    rp, ci, v = O.fe_assemble_cells(N, O.fe_cell_dofs(order, dim, n), E=O.fe_element(order, dim, n),
+                                    factor=kap)
+    fixed = np.nonzero(O.fe_boundary_mask(order, dim, n))[0].astype(np.int32)
+    O.zero_rows(fixed, rp, ci, v)
+    rq, cq, vq = O.fe_system(order, dim, n, kap)
+    assert np.array_equal(rp, rq) and np.array_equal(ci, cq) and np.array_equal(v, vq)
+
+
+def test_cell_assembly_matches_dense_scatter():
+    rng = np.random.default_rng(3)
+    nnode, ncell, nloc = 30, 45, 5
+    dofs = np.stack([rng.choice(nnode, nloc, replace=False) for _ in range(ncell)])
+    mats = rng.uniform(-1, 1, (ncell, nloc, nloc))
+    A = np.zeros((nnode, nnode))
+    P = np.zeros((nnode, nnode), bool)
+    for c in range(ncell):
+        A[np.ix_(dofs[c], dofs[c])] += mats[c]
+        P[np.ix_(dofs[c], dofs[c])] = True
+    rp, ci, v = O.fe_assemble_cells(nnode, dofs, cell_mats=mats)
+    D = dense(rp, ci, v)
+    assert np.allclose(D, A, rtol=0, atol=1e-14)
+    pat = np.zeros((nnode, nnode), bool)
+    for i in range(nnode):
+        pat[i, ci[rp[i]:rp[i + 1]]] = True
+    assert np.array_equal(pat, P)
+    assert all(np.all(np.diff(ci[rp[i]:rp[i + 1]]) > 0) for i in range(nnode))
