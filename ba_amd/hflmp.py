@@ -869,7 +869,7 @@ class McSgs:
 
 
 # ---- C++ lmp layer (libhflmp_lmp.so) ---------------------------------------------------
-LMP_CG, LMP_GMRES, LMP_FGMRES, LMP_MPIR = 0, 1, 2, 3
+LMP_CG, LMP_GMRES, LMP_FGMRES, LMP_MPIR, LMP_BICGSTAB = 0, 1, 2, 3, 4
 LMP_PREC_NONE, LMP_PREC_JACOBI, LMP_PREC_MCSGS = 0, 1, 2
 
 

@@ -11,6 +11,7 @@
 #include "lmatrix_csr_hip.h"
 #include "lmp_mmio.h"
 #include "lvector_hip.h"
+#include "linear_solver/bicgstab.h"
 #include "linear_solver/cg.h"
 #include "linear_solver/gmres.h"
 #include "linear_solver/mpir.h"
@@ -41,6 +42,11 @@ std::unique_ptr<LinearSolver<LAD>> make_solver(int kind, const hflmp_lmp_options
     auto *g = new GMRES<LAD>();
     g->InitParameter(o.basis, o.precond == HFLMP_LMP_PREC_NONE ? "NoPreconditioning"
                                                                 : "RightPreconditioning");
+    s.reset(g);
+  } else if (kind == HFLMP_LMP_BICGSTAB) {
+    auto *g = new BiCGSTAB<LAD>();
+    g->InitParameter(o.precond == HFLMP_LMP_PREC_NONE ? "NoPreconditioning"
+                                                      : "RightPreconditioning");
     s.reset(g);
   } else if (kind == HFLMP_LMP_FGMRES) {
     auto *g = new FGMRES<LAD>();

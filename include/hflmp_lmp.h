@@ -19,13 +19,14 @@ extern "C" {
 #define HFLMP_LMP_GMRES 1
 #define HFLMP_LMP_FGMRES 2
 #define HFLMP_LMP_MPIR 3 /* FP64 refinement around an FP32 inner solver */
+#define HFLMP_LMP_BICGSTAB 4 /* linear_solver/bicgstab.h:127-483 */
 
 #define HFLMP_LMP_PREC_NONE 0
 #define HFLMP_LMP_PREC_JACOBI 1
 #define HFLMP_LMP_PREC_MCSGS 2
 
 typedef struct {
-  int solver;         /* HFLMP_LMP_* (CG, GMRES, FGMRES, MPIR)             */
+  int solver;         /* HFLMP_LMP_* (CG, GMRES, FGMRES, MPIR, BICGSTAB)   */
   int precond;        /* HFLMP_LMP_PREC_*                                  */
   int precision;      /* 64 or 32: value type of CG/GMRES/FGMRES           */
   int maxit;
@@ -34,7 +35,7 @@ typedef struct {
   double omega;       /* MC-SGS relaxation                                 */
   int mpir_max_outer; /* MPIR: outer corrections; inner solver = `solver_inner` */
   double mpir_atol, mpir_rtol;
-  int solver_inner;   /* MPIR inner solver: HFLMP_LMP_CG/GMRES/FGMRES      */
+  int solver_inner;   /* MPIR inner solver: CG/GMRES/FGMRES/BICGSTAB       */
 } hflmp_lmp_options;
 
 typedef struct {

@@ -33,6 +33,7 @@
  *                 mpir.cc:240)
  *   stencil       linear_algebra/stencil_operator.h:153-330 (matrix-free
  *                 VectorMultAdd, cell by cell)
+ *   BiCGSTAB      linear_solver/bicgstab.h:127-483
  *   assembly      assembly/assembly_utils.cc:144 (pattern),
  *                 lmatrix_csr_cpu.cc:1102-1153 (add_values, cell by cell)
  *
@@ -123,7 +124,12 @@ int or_multicoloring(int nrow, const int *rowptr, const int *col,
   int or_fe_assemble_cells_##S(int nnode, int ncell, int nloc,                \
                                const int *dofs, const T *cell_mats,           \
                                const T *E, const T *factor, int *rp,          \
-                               int *col, T *val);
+                               int *col, T *val);                             \
+  int or_bicgstab_##S(int nrow, const int *rp, const int *ci, const T *v,     \
+                      int prec, const T *invd, int ncolors, const int *csz,   \
+                      const int *perm, const int *color, const T *b, T *x,    \
+                      int maxit, double atol, double rtol, double dtol,       \
+                      int *iters, double *res_init, double *res_final);
 
 OR_DECL(double, d)
 OR_DECL(float, s)

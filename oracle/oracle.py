@@ -80,6 +80,10 @@ def lib():
                                 C.c_double, C.c_double, C.c_double,
                                 C.POINTER(C.c_int), C.POINTER(C.c_double),
                                 C.POINTER(C.c_double), _vp]
+            g("bicgstab").argtypes = [C.c_int, _i32p, _i32p, fp, C.c_int, _vp, C.c_int, _vp,
+                                      _vp, _vp, fp, fp, C.c_int, C.c_double, C.c_double,
+                                      C.c_double, C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                      C.POINTER(C.c_double)]
             g("fe_assemble_cells").argtypes = [C.c_int, C.c_int, C.c_int, _i32p, _vp, _vp,
                                                _vp, _i32p, _vp, _vp]
             g("stencil_vmadd").argtypes = [C.c_int, C.c_int, _i32p, fp, _vp, fp, C.c_int,
@@ -341,6 +345,21 @@ def cg(rp, ci, v, b, x0=None, prec=PREC_JACOBI, maxit=10000, atol=0.0,
     if history:
         info["history"] = hist[: it.value + 1].copy()
     return x, info
+
+
+def bicgstab(rp, ci, v, b, x0=None, prec=PREC_NONE, maxit=10000, atol=0.0, rtol=1e-8,
+             dtol=1e6, coloring=None):
+    """BiCGSTAB<LAD>::Solve with the reference semantics (right
+    preconditioning); returns (x, info dict)."""
+    n = len(rp) - 1
+    x = np.zeros(n, v.dtype) if x0 is None else np.array(x0, v.dtype)
+    invd, nc, csz, perm, color = _prec_args(rp, ci, v, prec, coloring)
+    it, r0, rf = C.c_int(), C.c_double(), C.c_double()
+    st = getattr(lib(), "or_bicgstab_" + _sfx(v))(
+        n, rp, ci, v, prec, _ptr(invd), nc, _ptr(csz), _ptr(perm), _ptr(color),
+        np.ascontiguousarray(b, v.dtype), x, maxit, atol, rtol, dtol, C.byref(it),
+        C.byref(r0), C.byref(rf))
+    return x, dict(status=st, iters=it.value, res_init=r0.value, res_final=rf.value)
 
 
 def gmres(rp, ci, v, b, x0=None, prec=PREC_NONE, basis=100, flexible=False,

@@ -157,3 +157,34 @@ def test_file_spmv_unstructured(tmp_path, symmetric):
     _, _, r3, c3, v3 = hflmp.mm_read_coo(str(outf))
     rp3, ci3, v4 = O.coo_to_csr_merge(m, r3, c3, v3)
     assert np.array_equal(rp3, rp2) and np.array_equal(ci3, ci2) and np.array_equal(v4, v2)
+
+
+@pytest.mark.parametrize("prec,oprec", [(hflmp.LMP_PREC_NONE, O.PREC_NONE),
+                                        (hflmp.LMP_PREC_JACOBI, O.PREC_JACOBI)])
+def test_lmp_bicgstab_vs_oracle(prec, oprec):
+    # BiCGSTAB<LAD> over CoupledMatrix/CoupledVector on the HIP platform, on
+    # a nonsymmetric system (FE stiffness plus a skew part)
+    rp, ci, v = hflmp.fe_system(1, 2, 48, hflmp.fe_kappa(2, 48, 1.0, 6))
+    n = len(rp) - 1
+    rng = np.random.default_rng(5)
+    A = sp.csr_matrix((v, ci, rp), shape=(n, n))
+    S = sp.csr_matrix((rng.uniform(-0.3, 0.3, len(v)), ci, rp), shape=(n, n))
+    M = (A + S - S.T).tocsr()
+    M.sort_indices()
+    rp2, ci2, v2 = M.indptr.astype(np.int32), M.indices.astype(np.int32), M.data.copy()
+    b = hflmp.fe_rhs(1, 2, 48, 3)
+    # the same iterates as the oracle over the first steps
+    for k in (1, 2, 4, 8):
+        xk, _ = hflmp.lmp_solve(rp2, ci2, v2, b, solver=hflmp.LMP_BICGSTAB, precond=prec,
+                                rtol=1e-14, maxit=k)
+        xok, _ = O.bicgstab(rp2, ci2, v2, b, prec=oprec, rtol=1e-14, maxit=k)
+        assert np.linalg.norm(xk - xok) <= 1e-9 * np.linalg.norm(xok), k
+    x, info = hflmp.lmp_solve(rp2, ci2, v2, b, solver=hflmp.LMP_BICGSTAB, precond=prec,
+                              rtol=1e-10, maxit=20000)
+    xo, io = O.bicgstab(rp2, ci2, v2, b, prec=oprec, rtol=1e-10, maxit=20000)
+    assert info["status"] == 0 and io["status"] == 0
+    # BiCGSTAB's residual is not monotone: the device's dot products (a tree
+    # of partial sums, not the sequential loop) move the stopping iteration
+    # by a few, so the count is held to 5%, the iterates above to 1e-9
+    assert abs(info["iters"] - io["iters"]) <= max(2, 0.05 * io["iters"]), (info, io)
+    assert np.linalg.norm(x - xo) <= 1e-8 * np.linalg.norm(xo)

@@ -512,3 +512,48 @@ def test_cell_assembly_matches_dense_scatter():
         pat[i, ci[rp[i]:rp[i + 1]]] = True
     assert np.array_equal(pat, P)
     assert all(np.all(np.diff(ci[rp[i]:rp[i + 1]]) > 0) for i in range(nnode))
+
+
+# ---- BiCGSTAB ------------------------------------------------------------------
+def _nonsymmetric_fe(n=30, seed=1):
+    rp, ci, v = O.fe_system(1, 2, n, O.fe_kappa(2, n, 1.0, 3))
+    N = len(rp) - 1
+    rng = np.random.default_rng(seed)
+    A = sp.csr_matrix((v, ci, rp), shape=(N, N))
+    S = sp.csr_matrix((rng.uniform(-0.3, 0.3, len(v)), ci, rp), shape=(N, N))
+    M = (A + S - S.T).tocsr()
+    M.sort_indices()
+    return M.indptr.astype(np.int32), M.indices.astype(np.int32), M.data.copy(), M
+
+
+@pytest.mark.parametrize("prec", [O.PREC_NONE, O.PREC_JACOBI])
+def test_bicgstab_vs_scipy_and_direct(prec):
+    rp, ci, v, M = _nonsymmetric_fe()
+    b = O.fe_rhs(1, 2, 30, 4)
+    Minv = None
+    if prec == O.PREC_JACOBI:
+        d = 1.0 / M.diagonal()
+        Minv = spl.LinearOperator(M.shape, matvec=lambda r: d * r)
+    # the same iterates as scipy's BiCGSTAB (van der Vorst's algorithm, right
+    # preconditioning) over the first steps, before rounding separates them
+    for k in (1, 2, 3, 5, 8):
+        xk, _ = O.bicgstab(rp, ci, v, b, prec=prec, rtol=1e-14, maxit=k)
+        xs, _ = spl.bicgstab(M, b, rtol=1e-14, atol=0.0, M=Minv, maxiter=k)
+        assert np.linalg.norm(xk - xs) <= 1e-10 * np.linalg.norm(xs), k
+    x, io = O.bicgstab(rp, ci, v, b, prec=prec, rtol=1e-10)
+    assert io["status"] == 0
+    xd = spl.spsolve(M.tocsc(), b)
+    assert np.linalg.norm(x - xd) <= 1e-7 * np.linalg.norm(xd)
+    # the recursive residual tracks the true one
+    assert np.linalg.norm(b - M @ x) <= 1e-9 * io["res_init"]
+
+
+def test_bicgstab_small_s_branch_identity():
+    # A = I: s = r - A p = 0 after the first step, so the alpha-only update
+    # (bicgstab.h:251-270) ends the solve at x = b in one iteration
+    n = 7
+    rp = np.arange(n + 1, dtype=np.int32)
+    ci = np.arange(n, dtype=np.int32)
+    b = np.linspace(-1, 2, n)
+    x, io = O.bicgstab(rp, ci, np.ones(n), b, rtol=1e-12)
+    assert io["iters"] == 1 and io["status"] == 0 and np.array_equal(x, b)
