@@ -25,6 +25,7 @@ struct hflmp_fe_mesh {
   int *inc_off;       // nnode + 1
   long long *inc;     // ncell * nloc keys: cell * 32 + local index, sorted per node
   int *row_len;       // nnode pattern row lengths
+  long long *tile_sum; // scan workspace, one value per kScanTile nodes
   long long nnz;
 };
 
@@ -46,17 +47,33 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
-// exclusive scan of cnt[0..n) into off[0..n], one workgroup
+// Exclusive scan of cnt[0..n) into off[0..n] (off[n] = total), in three
+// launches: per-tile sums (kScanTile values per workgroup), a one-workgroup
+// scan of the tile sums, and a rescan of each tile from its offset.
+constexpr int kScanPer = 16;                    // values per thread
+constexpr int kScanTile = kBlock * kScanPer;    // values per workgroup
+
+__global__ void __launch_bounds__(kBlock)
+    scan_tiles_kernel(int n, const int *__restrict__ cnt, long long *__restrict__ tile_sum) {
+  __shared__ long long lds[kBlock / kWave];
+  const long long base = (long long)blockIdx.x * kScanTile + (long long)threadIdx.x * kScanPer;
+  long long s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanPer; ++k)
+    if (base + k < n) s += cnt[base + k];
+  const long long t = block_sum(s, lds);
+  if (threadIdx.x == 0) tile_sum[blockIdx.x] = t;
+}
+
 __global__ void __launch_bounds__(kFinalBlock)
-    fe_scan_kernel(int n, const int *__restrict__ cnt, int *__restrict__ off,
-                   long long *total) {
+    scan_offsets_kernel(int ntiles, long long *__restrict__ tile_sum, long long *total) {
   __shared__ long long tot[kFinalBlock];
   const int t = threadIdx.x;
-  const int per = (n + kFinalBlock - 1) / kFinalBlock;
-  const int lo = t * per < n ? t * per : n;
-  const int hi = lo + per < n ? lo + per : n;
+  const int per = (ntiles + kFinalBlock - 1) / kFinalBlock;
+  const int lo = t * per < ntiles ? t * per : ntiles;
+  const int hi = lo + per < ntiles ? lo + per : ntiles;
   long long s = 0;
-  for (int k = lo; k < hi; ++k) s += cnt[k];
+  for (int k = lo; k < hi; ++k) s += tile_sum[k];
   tot[t] = s;
   __syncthreads();
   for (int o = 1; o < kFinalBlock; o <<= 1) {
@@ -67,13 +84,48 @@ __global__ void __launch_bounds__(kFinalBlock)
   }
   long long run = tot[t] - s;
   for (int k = lo; k < hi; ++k) {
-    off[k] = (int)run;
-    run += cnt[k];
+    const long long v = tile_sum[k];
+    tile_sum[k] = run;  // exclusive offset of tile k
+    run += v;
   }
-  if (t == kFinalBlock - 1) {
-    off[n] = (int)tot[t];
-    if (total) *total = tot[t];
+  if (t == kFinalBlock - 1 && total) *total = tot[t];
+}
+
+__global__ void __launch_bounds__(kBlock)
+    scan_write_kernel(int n, const int *__restrict__ cnt, const long long *__restrict__ tile_off,
+                      int *__restrict__ off) {
+  __shared__ long long part[kBlock];
+  const long long base = (long long)blockIdx.x * kScanTile + (long long)threadIdx.x * kScanPer;
+  long long s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanPer; ++k)
+    if (base + k < n) s += cnt[base + k];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 1; o < kBlock; o <<= 1) {
+    const long long add = threadIdx.x >= (unsigned)o ? part[threadIdx.x - o] : 0;
+    __syncthreads();
+    part[threadIdx.x] += add;
+    __syncthreads();
   }
+  long long run = tile_off[blockIdx.x] + part[threadIdx.x] - s;
+#pragma unroll
+  for (int k = 0; k < kScanPer; ++k)
+    if (base + k < n) {
+      off[base + k] = (int)run;
+      run += cnt[base + k];
+    }
+  if (base + kScanPer >= n && base < n) off[n] = (int)run;  // the last value's thread
+}
+
+// off[0..n] = exclusive scan of cnt, *total (device, optional) = the sum;
+// tile_sum holds ceil(n / kScanTile) values
+void scan_exclusive(hipStream_t s, int n, const int *cnt, int *off, long long *tile_sum,
+                    long long *total) {
+  const int ntiles = (n + kScanTile - 1) / kScanTile;
+  scan_tiles_kernel<<<ntiles, kBlock, 0, s>>>(n, cnt, tile_sum);
+  scan_offsets_kernel<<<1, kFinalBlock, 0, s>>>(ntiles, tile_sum, total);
+  scan_write_kernel<<<ntiles, kBlock, 0, s>>>(n, cnt, tile_sum, off);
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -185,6 +237,7 @@ void mesh_free(hflmp_fe_mesh *m) {
   (void)hipFree(m->inc_off);
   (void)hipFree(m->inc);
   (void)hipFree(m->row_len);
+  (void)hipFree(m->tile_sum);
   if (cur >= 0 && cur != m->device) (void)hipSetDevice(cur);
   delete m;
 }
@@ -235,6 +288,9 @@ int hflmp_fe_mesh_create(hflmp_ctx *ctx, int nnode, int ncell, int nloc, const i
       hipMalloc(&m->inc_off, sizeof(int) * ((size_t)nnode + 1)) != hipSuccess ||
       hipMalloc(&m->inc, sizeof(long long) * (size_t)n) != hipSuccess ||
       hipMalloc(&m->row_len, sizeof(int) * (size_t)nnode) != hipSuccess ||
+      hipMalloc(&m->tile_sum,
+                sizeof(long long) * (size_t)((nnode + kScanTile - 1) / kScanTile)) !=
+          hipSuccess ||
       hipMalloc(&cnt, sizeof(int) * (size_t)nnode) != hipSuccess ||
       hipMalloc(&total, 2 * sizeof(long long)) != hipSuccess)
     return fail2(HFLMP_ENOMEM, "out of device memory");
@@ -256,7 +312,7 @@ int hflmp_fe_mesh_create(hflmp_ctx *ctx, int nnode, int ncell, int nloc, const i
     return fail2(HFLMP_EHIP, "HIP error while counting");
   std::memcpy(&hbad, ctx->host_pinned, sizeof(hbad));
   if (hbad) return fail2(HFLMP_EINVAL, "a cell dof is outside [0, nnode)");
-  fe_scan_kernel<<<1, kFinalBlock, 0, s>>>(nnode, cnt, m->inc_off, nullptr);
+  scan_exclusive(s, nnode, cnt, m->inc_off, m->tile_sum, nullptr);
   if ((rc = hflmp_fill_zero(ctx, cnt, sizeof(int) * (size_t)nnode)) != HFLMP_OK)
     return fail2(rc, "fill");
   fe_incidence_kernel<<<grid_n(n), kBlock, 0, s>>>(n, nloc, m->dofs, m->inc_off, cnt, m->inc);
@@ -268,7 +324,7 @@ int hflmp_fe_mesh_create(hflmp_ctx *ctx, int nnode, int ncell, int nloc, const i
   int *scratch_off = nullptr;
   if (hipMalloc(&scratch_off, sizeof(int) * ((size_t)nnode + 1)) != hipSuccess)
     return fail2(HFLMP_ENOMEM, "out of device memory");
-  fe_scan_kernel<<<1, kFinalBlock, 0, s>>>(nnode, m->row_len, scratch_off, total);
+  scan_exclusive(s, nnode, m->row_len, scratch_off, m->tile_sum, total);
   long long nnz = 0;
   const bool ok = hipGetLastError() == hipSuccess &&
                   hipMemcpyAsync(&nnz, total, sizeof(nnz), hipMemcpyDeviceToHost, s) ==
@@ -295,7 +351,7 @@ int hflmp_fe_mesh_pattern(hflmp_ctx *ctx, const hflmp_fe_mesh *m, int *row_ptr, 
   HFLMP_REQUIRE(ctx, m && row_ptr && (m->nnz == 0 || col), "fe_mesh_pattern: null pointer");
   HFLMP_REQUIRE(ctx, m->device == ctx->device, "fe_mesh_pattern: mesh on another device");
   hipStream_t s = ctx->stream;
-  fe_scan_kernel<<<1, kFinalBlock, 0, s>>>(m->nnode, m->row_len, row_ptr, nullptr);
+  scan_exclusive(s, m->nnode, m->row_len, row_ptr, m->tile_sum, nullptr);
   fe_pattern_kernel<true><<<grid_n(m->nnode), kBlock, 0, s>>>(
       m->nnode, m->nloc, m->dofs, m->inc_off, m->inc, nullptr, row_ptr, col);
   HFLMP_LAUNCH_CHECK(ctx);

@@ -198,7 +198,8 @@ def bench_stencil(hflmp, ctx, n, x, b, steps, sell_cg=None, assembled=None):
            "apply_us": round(t * 1e6, 2), "bytes_per_apply": nb,
            "roofline": {"bound": "hbm", "achieved": round(nb / t / 1e9, 2),
                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(nb / t / 1e9 / HBM_PEAK_GBS, 4)},
+                        "frac": round(nb / t / 1e9 / HBM_PEAK_GBS, 4),
+                        "traffic": measured_traffic(n, "stencil_f64")},
            "device_bytes": St.info()["device_bytes"]}
     St.apply(dx, dy)
     y = dy.download()
