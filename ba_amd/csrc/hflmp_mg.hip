@@ -1,0 +1,544 @@
+// Geometric multigrid on the device: V-cycles over a hierarchy of CSR
+// operators with CSR restriction and prolongation, as a preconditioner of
+// CG and as a solver.  The reference's GMG (linear_solver/gmg.h:979-1230,
+// GMGBase::SolveImpl / SolveImplLevel) runs pre-smoothing, the residual,
+// restriction, the cycle on the coarser level, prolongation and
+// post-smoothing; its GPU kernels (lmp/cuda/cuda_gmg_kernel.h) compute the
+// residual as b minus the row's terms one by one, the restriction as a
+// product from zero and the prolongation as the terms added one by one
+// onto the solution.  The smoother is damped Jacobi with the arithmetic of
+// kernel_async_inner_damped_iter_no_offdiag (cuda_async_iter_kernels.h:
+// 116-144), run synchronously (every sweep reads the previous iterate).
+//
+// Every operator is a SELL-64 copy; each row is processed in ascending
+// storage order, so every kernel is bit-identical to a sequential loop over
+// the CSR rows.  One cycle from level l (right-hand side b_l):
+//   x = (w invd) b, then pre-1 sweeps x' = (w invd (b - (A-D) x)) + ((1-w) x)
+//   r = b - A x;  b_{l+1} = R r;  cycle on level l+1;  x = x + P x_{l+1};
+//   post sweeps.  Level L-1: the first step and coarse-1 sweeps.
+// The first step is a sweep from x = 0 without the product with zeros.
+// With pre = post and R = P^T the cycle is a symmetric operator, so it can
+// precondition CG.  Dirichlet rows are unit rows whose right-hand side is
+// zero on every level (R has zero rows for coarse boundary nodes, P zero
+// rows for fine ones), so they stay zero without fixed-dof handling.
+#include <chrono>
+#include <utility>
+
+#include "hflmp_sell.hpp"
+#include "hflmp_solver_common.hpp"
+
+namespace {
+
+using namespace hflmp_solver;
+using Op = SellOp<double, int>;
+constexpr int kMaxLevels = 16;
+
+__device__ __forceinline__ bool stopped(const int *status) {
+  return status && *status != kIterate;
+}
+
+// x = (w invd) b: the sweep from x = 0
+__global__ void __launch_bounds__(kBlock)
+    mg_first_kernel(int n, double w, const double *__restrict__ invd,
+                    const double *__restrict__ b, double *__restrict__ x, const int *status) {
+#pragma clang fp contract(off)
+  if (stopped(status)) return;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+    x[i] = w * invd[i] * b[i];
+}
+
+// xo = (w invd (b - (A - D) xi)) + ((1 - w) xi)
+__global__ void __launch_bounds__(kBlock)
+    mg_sweep_kernel(Op A, double w, double one_minus_w, const double *__restrict__ invd,
+                    const double *__restrict__ b, const double *__restrict__ xi,
+                    double *__restrict__ xo, const int *status) {
+#pragma clang fp contract(off)
+  if (stopped(status)) return;
+  const long long unit = xcd_tile(blockIdx.x, gridDim.x);
+  const long long r = A.thread_row(unit);
+  const double b0 = r < A.nrow ? b[r] : 0.0;
+  TileRow tr;
+  const double t = A.row_sum<true, true>(unit, xi, b0, tr, nullptr);
+  if (tr.has_row) {
+    const int i = tr.row;
+    xo[i] = (w * invd[i] * t) + (one_minus_w * xi[i]);
+  }
+}
+
+// r = b - A x, term by term
+__global__ void __launch_bounds__(kBlock)
+    mg_residual_kernel(Op A, const double *__restrict__ b, const double *__restrict__ x,
+                       double *__restrict__ r, const int *status) {
+#pragma clang fp contract(off)
+  if (stopped(status)) return;
+  const long long unit = xcd_tile(blockIdx.x, gridDim.x);
+  const long long row = A.thread_row(unit);
+  const double b0 = row < A.nrow ? b[row] : 0.0;
+  TileRow tr;
+  const double s = A.row_sum<true>(unit, x, b0, tr, nullptr);
+  if (tr.has_row) r[tr.row] = s;
+}
+
+// y = M x from zero (ADD: y = y + M x, term by term onto y)
+template <bool ADD>
+__global__ void __launch_bounds__(kBlock)
+    mg_apply_kernel(Op M, const double *__restrict__ x, double *__restrict__ y, const int *status) {
+  if (stopped(status)) return;
+  const long long unit = xcd_tile(blockIdx.x, gridDim.x);
+  double s = 0;
+  if (ADD) {
+    const long long r = M.thread_row(unit);
+    if (r < M.nrow) s = y[r];
+  }
+  TileRow tr;
+  s = M.row_sum(unit, x, s, tr, nullptr);
+  if (tr.has_row) y[tr.row] = s;
+}
+
+// ---- CG with the cycle as preconditioner (CG::SolvePrecond, cg.h:165-246) --
+// init: r = b - (A x) (VectorMult, then ScaleAdd(b, -1)), z = cycle(r), then
+// the r.r and r.z partials
+__global__ void __launch_bounds__(kBlock)
+    mgcg_residual_kernel(Op A, const double *__restrict__ b, const double *__restrict__ x,
+                         double *__restrict__ r) {
+#pragma clang fp contract(off)
+  const long long unit = xcd_tile(blockIdx.x, gridDim.x);
+  TileRow tr;
+  const double s = A.row_sum(unit, x, 0.0, tr, nullptr);
+  if (tr.has_row) r[tr.row] = b[tr.row] - s;
+}
+
+__global__ void __launch_bounds__(kBlock)
+    mgcg_dots_kernel(int n, const double *__restrict__ r, const double *__restrict__ z,
+                     double *__restrict__ partials, const int *status) {
+  if (stopped(status)) return;
+  double rr = 0, rz = 0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    rr += r[i] * r[i];
+    rz += r[i] * z[i];
+  }
+  block_partial2(rr, rz, partials);
+}
+
+__global__ void __launch_bounds__(kFinalBlock)
+    mgcg_init_final(CgState<double> *st, const double *__restrict__ partials, int count) {
+  __shared__ double lds[kFinalBlock / kWave];
+  const double rr = sum_partials(partials, count, 2, 0, lds);
+  const double rz = sum_partials(partials, count, 2, 1, lds);
+  if (threadIdx.x == 0) {
+    const double res0 = sqrt(rr);
+    st->rr = rr;
+    st->rz = rz;
+    st->res0 = res0;
+    st->res = res0;
+    st->iter = 0;
+    st->status = check_state(0, res0, res0, st->maxit, st->atol, st->rtol, st->dtol);
+  }
+}
+
+// Ap = A p, partials of p.Ap
+__global__ void __launch_bounds__(kBlock)
+    mgcg_spmv_kernel(Op A, const double *__restrict__ p, double *__restrict__ Ap,
+                     double *__restrict__ partials, const int *status) {
+  if (stopped(status)) return;
+  const long long unit = xcd_tile(blockIdx.x, gridDim.x);
+  TileRow tr;
+  const double s = A.row_sum(unit, p, 0.0, tr, nullptr);
+  double part = 0;
+  if (tr.has_row) {
+    Ap[tr.row] = s;
+    part = s * p[tr.row];
+  }
+  block_partial1(part, partials);
+}
+
+__global__ void __launch_bounds__(kFinalBlock)
+    mgcg_alpha_final(CgState<double> *st, const double *__restrict__ partials, int count) {
+  __shared__ double lds[kFinalBlock / kWave];
+  if (st->status != kIterate) return;
+  const double pAp = sum_partials(partials, count, 1, 0, lds);
+  if (threadIdx.x == 0) {
+    st->pAp = pAp;
+    st->alpha = st->rz / pAp;
+    if (pAp == 0.0 || !(pAp == pAp)) st->status = kBreakdown;
+  }
+}
+
+// x += alpha p; r -= alpha Ap; partials of r.r
+__global__ void __launch_bounds__(kBlock)
+    mgcg_update_kernel(int n, const double *__restrict__ p, const double *__restrict__ Ap,
+                       double *__restrict__ x, double *__restrict__ r, const CgState<double> *st,
+                       double *__restrict__ partials) {
+  if (st->status != kIterate) return;
+  const double alpha = st->alpha, malpha = -alpha;
+  double rr = 0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    x[i] += alpha * p[i];
+    const double ri = r[i] + malpha * Ap[i];
+    r[i] = ri;
+    rr += ri * ri;
+  }
+  block_partial1(rr, partials);
+}
+
+// res = sqrt(r.r), IterateControl::Check
+__global__ void __launch_bounds__(kFinalBlock)
+    mgcg_check_final(CgState<double> *st, const double *__restrict__ partials, int count) {
+  __shared__ double lds[kFinalBlock / kWave];
+  if (st->status != kIterate) return;
+  const double rr = sum_partials(partials, count, 1, 0, lds);
+  if (threadIdx.x == 0) {
+    const int iter = st->iter + 1;
+    const double res = sqrt(rr);
+    st->iter = iter;
+    st->rr = rr;
+    st->res = res;
+    st->status = check_state(iter, res, st->res0, st->maxit, st->atol, st->rtol, st->dtol);
+  }
+}
+
+// partials of r.z
+__global__ void __launch_bounds__(kBlock)
+    mgcg_rz_kernel(int n, const double *__restrict__ r, const double *__restrict__ z,
+                   const CgState<double> *st, double *__restrict__ partials) {
+  if (st->status != kIterate) return;
+  double rz = 0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+    rz += r[i] * z[i];
+  block_partial1(rz, partials);
+}
+
+// beta = gamma / beta
+__global__ void __launch_bounds__(kFinalBlock)
+    mgcg_beta_final(CgState<double> *st, const double *__restrict__ partials, int count) {
+  __shared__ double lds[kFinalBlock / kWave];
+  if (st->status != kIterate) return;
+  const double rz = sum_partials(partials, count, 1, 0, lds);
+  if (threadIdx.x == 0) {
+    st->beta = rz / st->rz;
+    st->rz = rz;
+  }
+}
+
+// p = beta p + z (init: p = z)
+__global__ void __launch_bounds__(kBlock)
+    mgcg_p_kernel(int n, const double *__restrict__ z, double *__restrict__ p,
+                  const CgState<double> *st, int first) {
+  if (st->status != kIterate) return;
+  const double beta = st->beta;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+    p[i] = first ? z[i] : beta * p[i] + z[i];
+}
+
+// ---- the multigrid solver: r = b - A x, check ||r||, x += cycle(r) ---------
+__global__ void __launch_bounds__(kBlock)
+    mgs_residual_kernel(Op A, const double *__restrict__ b, const double *__restrict__ x,
+                        double *__restrict__ r, double *__restrict__ partials, const int *status) {
+#pragma clang fp contract(off)
+  if (stopped(status)) return;
+  const long long unit = xcd_tile(blockIdx.x, gridDim.x);
+  const long long row = A.thread_row(unit);
+  const double b0 = row < A.nrow ? b[row] : 0.0;
+  TileRow tr;
+  const double ri = A.row_sum<true>(unit, x, b0, tr, nullptr);
+  double rr = 0;
+  if (tr.has_row) {
+    r[tr.row] = ri;
+    rr = ri * ri;
+  }
+  block_partial1(rr, partials);
+}
+
+__global__ void __launch_bounds__(kFinalBlock)
+    mgs_check_final(CgState<double> *st, const double *__restrict__ partials, int count,
+                    int first) {
+  __shared__ double lds[kFinalBlock / kWave];
+  if (st->status != kIterate) return;
+  const double rr = sum_partials(partials, count, 1, 0, lds);
+  if (threadIdx.x == 0) {
+    const double res = sqrt(rr);
+    if (first) {
+      st->res0 = res;
+      st->iter = 0;
+    } else {
+      st->iter = st->iter + 1;
+    }
+    st->res = res;
+    st->status = check_state(st->iter, res, st->res0, st->maxit, st->atol, st->rtol, st->dtol);
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+    mgs_add_kernel(int n, const double *__restrict__ z, double *__restrict__ x, const int *status) {
+  if (stopped(status)) return;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) x[i] += z[i];
+}
+
+}  // namespace
+
+struct hflmp_mg {
+  int device = 0;
+  int nlevels = 0;
+  hflmp_mg_params prm;
+  struct Level {
+    int n = 0;
+    hflmp_sell *A = nullptr, *R = nullptr, *P = nullptr;
+    double *invd = nullptr, *b = nullptr, *x = nullptr, *x2 = nullptr, *r = nullptr;
+  } lv[kMaxLevels];
+  int grid0 = 0;  // workgroups of a level-0 operator launch
+};
+
+namespace {
+
+void mg_free(hflmp_mg *m) {
+  if (!m) return;
+  int cur = -1;
+  (void)hipGetDevice(&cur);
+  if (cur != m->device) (void)hipSetDevice(m->device);
+  for (int l = 0; l < m->nlevels; ++l) {
+    auto &L = m->lv[l];
+    hflmp_sell_destroy(L.A);
+    hflmp_sell_destroy(L.R);
+    hflmp_sell_destroy(L.P);
+    (void)hipFree(L.invd);
+    (void)hipFree(L.b);
+    (void)hipFree(L.x);
+    (void)hipFree(L.x2);
+    (void)hipFree(L.r);
+  }
+  if (cur >= 0 && cur != m->device) (void)hipSetDevice(cur);
+  delete m;
+}
+
+// One cycle from level l with right-hand side b; returns the buffer that
+// holds the result.  `out` (level 0) is the buffer the result must end in:
+// the ping-pong pair starts so that the last write lands there.  Every
+// launch returns at once when *status is set and not "iterate".
+double *cycle(hipStream_t s, const hflmp_mg *m, int l, const double *b, double *out,
+              const int *status) {
+  const auto &L = m->lv[l];
+  const bool coarsest = l == m->nlevels - 1;
+  const double w = coarsest ? m->prm.coarse_omega : m->prm.omega;
+  const double omw = 1.0 - w;
+  const Op A = sell_view<double, int>(L.A);
+  const unsigned ga = sell_grid(L.n), gv = grid_for(L.n, 4);
+  const int first = coarsest ? m->prm.coarse_sweeps : m->prm.pre_smooth;
+  const int swaps = (first - 1) + (coarsest ? 0 : m->prm.post_smooth);
+  double *x = L.x, *y = L.x2;
+  if (out) {
+    x = (swaps & 1) ? L.x2 : out;
+    y = (swaps & 1) ? out : L.x2;
+  }
+  mg_first_kernel<<<gv, kBlock, 0, s>>>(L.n, w, L.invd, b, x, status);
+  for (int k = 1; k < first; ++k) {
+    mg_sweep_kernel<<<ga, kBlock, 0, s>>>(A, w, omw, L.invd, b, x, y, status);
+    std::swap(x, y);
+  }
+  if (coarsest) return x;
+  const auto &C = m->lv[l + 1];
+  mg_residual_kernel<<<ga, kBlock, 0, s>>>(A, b, x, L.r, status);
+  mg_apply_kernel<false><<<sell_grid(C.n), kBlock, 0, s>>>(sell_view<double, int>(L.R), L.r,
+                                                            C.b, status);
+  const double *xc = cycle(s, m, l + 1, C.b, nullptr, status);
+  mg_apply_kernel<true><<<ga, kBlock, 0, s>>>(sell_view<double, int>(L.P), xc, x, status);
+  for (int k = 0; k < m->prm.post_smooth; ++k) {
+    mg_sweep_kernel<<<ga, kBlock, 0, s>>>(A, w, omw, L.invd, b, x, y, status);
+    std::swap(x, y);
+  }
+  return x;
+}
+
+// z = M r: one cycle from level 0 (z must not alias r)
+void apply(hipStream_t s, const hflmp_mg *m, const double *r, double *z, const int *status) {
+  (void)cycle(s, m, 0, r, z, status);
+}
+
+}  // namespace
+
+extern "C" {
+
+int hflmp_mg_create_f64(hflmp_ctx *ctx, int nlevels, const hflmp_csr_f64 *A,
+                        const hflmp_csr_f64 *R, const hflmp_csr_f64 *P,
+                        const hflmp_mg_params *prm, hflmp_mg **out) {
+  if (!ctx) return HFLMP_EINVAL;
+  HFLMP_REQUIRE(ctx, A && out && prm && (nlevels == 1 || (R && P)), "mg_create: null pointer");
+  *out = nullptr;
+  HFLMP_REQUIRE(ctx, nlevels >= 1 && nlevels <= kMaxLevels, "mg_create: 1 <= nlevels <= %d",
+                kMaxLevels);
+  HFLMP_REQUIRE(ctx, prm->pre_smooth >= 1 && prm->post_smooth >= 0 && prm->coarse_sweeps >= 1 &&
+                         prm->omega > 0 && prm->coarse_omega > 0,
+                "mg_create: need pre_smooth >= 1, post_smooth >= 0, coarse_sweeps >= 1, "
+                "omega > 0, coarse_omega > 0");
+  for (int l = 0; l < nlevels; ++l) {
+    HFLMP_REQUIRE(ctx, A[l].nrow > 0 && A[l].ncol == A[l].nrow && A[l].row_ptr,
+                  "mg_create: operator %d must be square and non-empty", l);
+    if (l + 1 < nlevels) {
+      HFLMP_REQUIRE(ctx, R[l].nrow == A[l + 1].nrow && R[l].ncol == A[l].nrow &&
+                             P[l].nrow == A[l].nrow && P[l].ncol == A[l + 1].nrow,
+                    "mg_create: R[%d] must be n%d x n%d and P[%d] n%d x n%d", l, l + 1, l, l,
+                    l, l + 1);
+    }
+  }
+  auto *m = new hflmp_mg();
+  m->device = ctx->device;
+  m->nlevels = nlevels;
+  m->prm = *prm;
+  int rc = HFLMP_OK;
+  auto sell_of = [&](const hflmp_csr_f64 &c, hflmp_sell **o) {
+    return hflmp_sell_create_f64(ctx, c.nrow, c.ncol, c.nnz, c.row_ptr, c.col, c.val,
+                                 HFLMP_SELL_ANY_PADDING, o);
+  };
+  for (int l = 0; l < nlevels && rc == HFLMP_OK; ++l) {
+    auto &L = m->lv[l];
+    L.n = A[l].nrow;
+    const size_t bytes = sizeof(double) * (size_t)L.n;
+    if ((rc = sell_of(A[l], &L.A)) != HFLMP_OK) break;
+    if (l + 1 < nlevels && ((rc = sell_of(R[l], &L.R)) != HFLMP_OK ||
+                            (rc = sell_of(P[l], &L.P)) != HFLMP_OK))
+      break;
+    if (hipMalloc(&L.invd, bytes) != hipSuccess || hipMalloc(&L.b, bytes) != hipSuccess ||
+        hipMalloc(&L.x, bytes) != hipSuccess || hipMalloc(&L.x2, bytes) != hipSuccess ||
+        hipMalloc(&L.r, bytes) != hipSuccess) {
+      rc = set_error(ctx, HFLMP_ENOMEM, "mg_create: out of device memory");
+      break;
+    }
+    int bad = 0;
+    if ((rc = hflmp_csr_extract_invdiag_f64(ctx, L.n, A[l].row_ptr, A[l].col, A[l].val, L.invd,
+                                            &bad)) != HFLMP_OK)
+      break;
+    if (bad) {
+      rc = set_error(ctx, HFLMP_ENUMERIC, "mg_create: %d rows of operator %d have no diagonal",
+                     bad, l);
+      break;
+    }
+  }
+  if (rc != HFLMP_OK) {
+    mg_free(m);
+    return rc;
+  }
+  m->grid0 = sell_grid(m->lv[0].n);
+  *out = m;
+  return HFLMP_OK;
+}
+
+void hflmp_mg_destroy(hflmp_mg *m) { mg_free(m); }
+
+int hflmp_mg_apply_f64(hflmp_ctx *ctx, const hflmp_mg *m, const double *r, double *z) {
+  if (!ctx) return HFLMP_EINVAL;
+  HFLMP_REQUIRE(ctx, m && r && z, "mg_apply: null pointer");
+  HFLMP_REQUIRE(ctx, r != z, "mg_apply: r and z must be different vectors");
+  HFLMP_REQUIRE(ctx, m->device == ctx->device, "mg_apply: hierarchy on another device");
+  apply(ctx->stream, m, r, z, nullptr);
+  HFLMP_LAUNCH_CHECK(ctx);
+  return HFLMP_OK;
+}
+
+size_t hflmp_mg_workspace_bytes(const hflmp_mg *m) {
+  if (!m) return 0;
+  const size_t n = (size_t)m->lv[0].n;
+  const size_t g = (size_t)(m->grid0 > kMaxRedBlocks ? m->grid0 : kMaxRedBlocks);
+  return 256 + align256(2 * g * sizeof(double)) + 4 * align256(n * sizeof(double));
+}
+
+int hflmp_pcg_mg_f64(hflmp_ctx *ctx, const hflmp_mg *m, const double *b, double *x, int maxit,
+                     double atol, double rtol, double dtol, void *work, size_t work_bytes,
+                     hflmp_solve_info *info) {
+  if (!ctx) return HFLMP_EINVAL;
+  HFLMP_REQUIRE(ctx, m && b && x && work && info, "pcg_mg: null pointer");
+  HFLMP_REQUIRE(ctx, maxit > 0 && atol >= 0 && rtol >= 0 && dtol > 1,
+                "pcg_mg: bad control parameters (maxit>0, atol>=0, rtol>=0, dtol>1)");
+  HFLMP_REQUIRE(ctx, work_bytes >= hflmp_mg_workspace_bytes(m), "pcg_mg: workspace %zu < %zu",
+                work_bytes, hflmp_mg_workspace_bytes(m));
+  const int n = m->lv[0].n;
+  const size_t g = (size_t)(m->grid0 > kMaxRedBlocks ? m->grid0 : kMaxRedBlocks);
+  char *w = static_cast<char *>(work);
+  auto *st = reinterpret_cast<CgState<double> *>(w);
+  double *partials = reinterpret_cast<double *>(w + 256);
+  const size_t vec0 = 256 + align256(2 * g * sizeof(double)), vs = align256(n * sizeof(double));
+  double *r = reinterpret_cast<double *>(w + vec0);
+  double *p = reinterpret_cast<double *>(w + vec0 + vs);
+  double *z = reinterpret_cast<double *>(w + vec0 + 2 * vs);
+  double *Ap = reinterpret_cast<double *>(w + vec0 + 3 * vs);
+  hipStream_t s = ctx->stream;
+  const Op A = sell_view<double, int>(m->lv[0].A);
+  const unsigned ga = (unsigned)m->grid0;
+  const int gd = grid_for(n, 4) < kMaxRedBlocks ? (int)grid_for(n, 4) : kMaxRedBlocks;
+  const int *status = &st->status;
+  const auto t0 = std::chrono::steady_clock::now();
+  init_state_kernel<double><<<1, kBlock, 0, s>>>(st, maxit, atol, rtol, dtol);
+  mgcg_residual_kernel<<<ga, kBlock, 0, s>>>(A, b, x, r);
+  apply(s, m, r, z, nullptr);
+  mgcg_dots_kernel<<<gd, kBlock, 0, s>>>(n, r, z, partials, nullptr);
+  mgcg_init_final<<<1, kFinalBlock, 0, s>>>(st, partials, gd);
+  mgcg_p_kernel<<<gd, kBlock, 0, s>>>(n, z, p, st, 1);
+  HFLMP_LAUNCH_CHECK(ctx);
+  auto iteration = [&]() {
+    mgcg_spmv_kernel<<<ga, kBlock, 0, s>>>(A, p, Ap, partials, status);
+    mgcg_alpha_final<<<1, kFinalBlock, 0, s>>>(st, partials, (int)ga);
+    mgcg_update_kernel<<<gd, kBlock, 0, s>>>(n, p, Ap, x, r, st, partials);
+    mgcg_check_final<<<1, kFinalBlock, 0, s>>>(st, partials, gd);
+    apply(s, m, r, z, status);
+    mgcg_rz_kernel<<<gd, kBlock, 0, s>>>(n, r, z, st, partials);
+    mgcg_beta_final<<<1, kFinalBlock, 0, s>>>(st, partials, gd);
+    mgcg_p_kernel<<<gd, kBlock, 0, s>>>(n, z, p, st, 0);
+  };
+  int rc = run_polled(ctx, maxit, &st->status, iteration, 2);
+  if (rc) return rc;
+  CgState<double> h;
+  HFLMP_HIP_CHECK(ctx, hipMemcpyAsync(&h, st, sizeof(h), hipMemcpyDeviceToHost, s));
+  HFLMP_HIP_CHECK(ctx, hipStreamSynchronize(s));
+  info->iters = h.iter;
+  info->status = h.status == kIterate ? kMaxIts : h.status;
+  info->res_init = h.res0;
+  info->res_final = h.res;
+  info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return HFLMP_OK;
+}
+
+int hflmp_mg_solve_f64(hflmp_ctx *ctx, const hflmp_mg *m, const double *b, double *x, int maxit,
+                       double atol, double rtol, double dtol, void *work, size_t work_bytes,
+                       hflmp_solve_info *info) {
+  if (!ctx) return HFLMP_EINVAL;
+  HFLMP_REQUIRE(ctx, m && b && x && work && info, "mg_solve: null pointer");
+  HFLMP_REQUIRE(ctx, maxit > 0 && atol >= 0 && rtol >= 0 && dtol > 1,
+                "mg_solve: bad control parameters (maxit>0, atol>=0, rtol>=0, dtol>1)");
+  HFLMP_REQUIRE(ctx, work_bytes >= hflmp_mg_workspace_bytes(m), "mg_solve: workspace %zu < %zu",
+                work_bytes, hflmp_mg_workspace_bytes(m));
+  const int n = m->lv[0].n;
+  const size_t g = (size_t)(m->grid0 > kMaxRedBlocks ? m->grid0 : kMaxRedBlocks);
+  char *w = static_cast<char *>(work);
+  auto *st = reinterpret_cast<CgState<double> *>(w);
+  double *partials = reinterpret_cast<double *>(w + 256);
+  const size_t vec0 = 256 + align256(2 * g * sizeof(double)), vs = align256(n * sizeof(double));
+  double *r = reinterpret_cast<double *>(w + vec0);
+  double *z = reinterpret_cast<double *>(w + vec0 + vs);
+  hipStream_t s = ctx->stream;
+  const Op A = sell_view<double, int>(m->lv[0].A);
+  const unsigned ga = (unsigned)m->grid0;
+  const int *status = &st->status;
+  const auto t0 = std::chrono::steady_clock::now();
+  init_state_kernel<double><<<1, kBlock, 0, s>>>(st, maxit, atol, rtol, dtol);
+  mgs_residual_kernel<<<ga, kBlock, 0, s>>>(A, b, x, r, partials, nullptr);
+  mgs_check_final<<<1, kFinalBlock, 0, s>>>(st, partials, (int)ga, 1);
+  HFLMP_LAUNCH_CHECK(ctx);
+  // the reference's GMG::SolveImpl (gmg.h:979-1066): cycle, new residual, check
+  auto iteration = [&]() {
+    apply(s, m, r, z, status);
+    mgs_add_kernel<<<grid_for(n, 4), kBlock, 0, s>>>(n, z, x, status);
+    mgs_residual_kernel<<<ga, kBlock, 0, s>>>(A, b, x, r, partials, status);
+    mgs_check_final<<<1, kFinalBlock, 0, s>>>(st, partials, (int)ga, 0);
+  };
+  int rc = run_polled(ctx, maxit, &st->status, iteration, 2);
+  if (rc) return rc;
+  CgState<double> h;
+  HFLMP_HIP_CHECK(ctx, hipMemcpyAsync(&h, st, sizeof(h), hipMemcpyDeviceToHost, s));
+  HFLMP_HIP_CHECK(ctx, hipStreamSynchronize(s));
+  info->iters = h.iter;
+  info->status = h.status == kIterate ? kMaxIts : h.status;
+  info->res_init = h.res0;
+  info->res_final = h.res;
+  info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return HFLMP_OK;
+}
+
+}  // extern "C"

@@ -148,7 +148,8 @@ int sell_create(hflmp_ctx *ctx, int nrow, int ncol, int nnz, const int *rp, cons
   *out = nullptr;
   HFLMP_REQUIRE(ctx, nrow > 0 && ncol > 0 && nnz >= 0, "sell_create: bad sizes");
   HFLMP_REQUIRE(ctx, rp && (nnz == 0 || (ci && val)), "sell_create: null pointer");
-  HFLMP_REQUIRE(ctx, (flags & ~HFLMP_SELL_COL16) == 0, "sell_create: unknown flags %d", flags);
+  HFLMP_REQUIRE(ctx, (flags & ~(HFLMP_SELL_COL16 | HFLMP_SELL_ANY_PADDING)) == 0,
+                "sell_create: unknown flags %d", flags);
   hipStream_t s = ctx->stream;
   auto *m = new hflmp_sell();
   m->device = ctx->device;
@@ -193,7 +194,7 @@ int sell_create(hflmp_ctx *ctx, int nrow, int ncol, int nnz, const int *rp, cons
   // Padding is never loaded (a lane stops at its row's length), but it
   // costs memory and partly-used cache lines; refuse more than twice the
   // nonzeros of it.
-  if (entries > 3LL * nnz + (long long)kSellRows * m->chunk)
+  if (!(flags & HFLMP_SELL_ANY_PADDING) && entries > 3LL * nnz + (long long)kSellRows * m->chunk)
     return fail(HFLMP_EFORMAT, "row lengths vary too much within slices (use the CSR kernels)");
   m->entries = entries;
   if ((flags & HFLMP_SELL_COL16) && hs.bad16 == 0) m->col_bytes = 2;

@@ -102,6 +102,9 @@ struct SellOp {
   __device__ __forceinline__ long long thread_row(long long unit) const {
     return unit * kBlock + threadIdx.x;
   }
+  // s + sum of a_ij x_j over the row in ascending storage order; SUB:
+  // s - a_ij x_j instead; NODIAG: the diagonal entry (column == row) skipped.
+  template <bool SUB = false, bool NODIAG = false>
   __device__ __forceinline__ T row_sum(long long unit, const T *x_in, T s, TileRow &tr,
                                        T *) const {
 #pragma clang fp contract(off)
@@ -123,6 +126,7 @@ struct SellOp {
     const TV *__restrict__ v = reinterpret_cast<const TV *>(val + base) + lane;
     // int16 columns are offsets from the slice's first row
     const T *__restrict__ x = std::is_same<IT, int>::value ? x_in : x_in + sid * kSellRows;
+    const int dcol = std::is_same<IT, int>::value ? row : lane;
     for (int j = 0; j < smax; j += P * U) {
       IV cc[U];
       TV vv[U];
@@ -143,7 +147,12 @@ struct SellOp {
       for (int u = 0; u < U; ++u)
 #pragma unroll
         for (int q = 0; q < P; ++q)
-          if (j + u * P + q < len) s += vv[u][q] * xx[u][q];
+          if (j + u * P + q < len && (!NODIAG || (int)cc[u][q] != dcol)) {
+            if (SUB)
+              s -= vv[u][q] * xx[u][q];
+            else
+              s += vv[u][q] * xx[u][q];
+          }
     }
     return s;
   }

@@ -160,6 +160,14 @@ def hip_lib():
         L.hflmp_cast_csr_f64_to_f32.argtypes = [vp, i, vp, vp]
         L.hflmp_csr_diag_ptr.argtypes = [vp, i, vp, vp, vp]
         L.hflmp_csr_zero_rows_f64.argtypes = [vp, i, vp, vp, vp, vp, d]
+        L.hflmp_mg_create_f64.argtypes = [vp, i, vp, vp, vp, vp, vp]
+        L.hflmp_mg_destroy.argtypes = [vp]
+        L.hflmp_mg_destroy.restype = None
+        L.hflmp_mg_apply_f64.argtypes = [vp, vp, vp, vp]
+        L.hflmp_mg_workspace_bytes.argtypes = [vp]
+        L.hflmp_mg_workspace_bytes.restype = sz
+        for n in ("hflmp_pcg_mg_f64", "hflmp_mg_solve_f64"):
+            getattr(L, n).argtypes = [vp, vp, vp, vp, i, d, d, d, vp, sz, vp]
         _hip = L
     return _hip
 
@@ -525,6 +533,8 @@ class SellInfo(C.Structure):
 
 
 SELL_COL16 = 1
+SELL_ANY_PADDING = 2
+EINVAL = -1
 EFORMAT = -6
 
 
@@ -623,6 +633,113 @@ class FeMesh:
         if self.h:
             self.ctx.L.hflmp_fe_mesh_destroy(self.h)
             self.h = C.c_void_p()
+
+
+def csr_from_device(ctx, nrow, ncol, nnz, rp, ci, val):
+    """A CsrMatrix over device arrays that it takes ownership of."""
+    m = CsrMatrix.__new__(CsrMatrix)
+    m.ctx, m.nrow, m.ncol, m.nnz = ctx, nrow, ncol, int(nnz)
+    m.rp, m.ci, m.val = rp, ci, val
+    m.dtype = val.dtype
+    return m
+
+
+def assemble_q1(ctx, dim, n, cell_mats):
+    """Device assembly of the structured Q1 mesh of n^dim cells from one
+    matrix per cell (ncell x nloc x nloc, cells and local nodes
+    lexicographic), with unit Dirichlet rows on the boundary: the operator
+    of a multigrid level."""
+    nnode = (n + 1) ** dim
+    mesh = FeMesh(ctx, nnode, fe_cell_dofs(1, dim, n))
+    rp = ci = val = fx = None
+    try:
+        rp, ci = mesh.pattern()
+        val = mesh.assemble(rp, ci, cell_mats=np.ascontiguousarray(cell_mats, np.float64))
+        fixed = fe_fixed(1, dim, n)
+        fx = ctx.upload(fixed)
+        _call(ctx, "hflmp_csr_zero_rows_f64", len(fixed), fx.ptr, rp.ptr, ci.ptr, val.ptr, 1.0)
+        ctx.sync()
+    except Exception:
+        for a in (rp, ci, val):
+            if a is not None:
+                a.free()
+        raise
+    finally:
+        mesh.free()
+        if fx is not None:
+            fx.free()
+    return csr_from_device(ctx, nnode, nnode, mesh.nnz, rp, ci, val)
+
+
+class MgParams(C.Structure):
+    _fields_ = [("pre_smooth", C.c_int), ("post_smooth", C.c_int), ("coarse_sweeps", C.c_int),
+                ("omega", C.c_double), ("coarse_omega", C.c_double)]
+
+
+class CsrDesc(C.Structure):
+    _fields_ = [("nrow", C.c_int), ("ncol", C.c_int), ("nnz", C.c_int),
+                ("row_ptr", C.c_void_p), ("col", C.c_void_p), ("val", C.c_void_p)]
+
+    @classmethod
+    def of(cls, A):
+        return cls(A.nrow, A.ncol, A.nnz, A.rp.ptr, A.ci.ptr, A.val.ptr)
+
+
+class Multigrid:
+    """Device V-cycle over the levels A[0] (finest) .. A[L-1] with
+    restrictions R[l] and prolongations P[l] (device CsrMatrix objects,
+    float64; copied at creation).  `owned` matrices are freed with it."""
+
+    def __init__(self, ctx, A, R, P, pre_smooth=1, post_smooth=1, coarse_sweeps=4, omega=0.7,
+                 coarse_omega=0.7, owned=()):
+        self.ctx = ctx
+        self.nlevels = len(A)
+        self.n = A[0].nrow
+        self.params = MgParams(pre_smooth, post_smooth, coarse_sweeps, omega, coarse_omega)
+        a = (CsrDesc * len(A))(*[CsrDesc.of(m) for m in A])
+        r = (CsrDesc * max(len(R), 1))(*[CsrDesc.of(m) for m in R])
+        p = (CsrDesc * max(len(P), 1))(*[CsrDesc.of(m) for m in P])
+        self.h = C.c_void_p()
+        self.owned = list(owned)
+        _call(ctx, "hflmp_mg_create_f64", self.nlevels, a, r, p, C.byref(self.params),
+              C.byref(self.h))
+        self.level_rows = [m.nrow for m in A]
+
+    def apply(self, r, z):
+        """z = M r (one V-cycle from zero); r and z device arrays."""
+        _call(self.ctx, "hflmp_mg_apply_f64", self.h, r.ptr, z.ptr)
+
+    def workspace_bytes(self):
+        return self.ctx.L.hflmp_mg_workspace_bytes(self.h)
+
+    def _run(self, name, b, x, maxit, atol, rtol, dtol, work):
+        own = work is None
+        if own:
+            work = self.ctx.empty(self.workspace_bytes(), np.uint8)
+        info = SolveInfo()
+        try:
+            _call(self.ctx, name, self.h, b.ptr, x.ptr, maxit, atol, rtol, dtol, work.ptr,
+                  work.nbytes, C.byref(info))
+        finally:
+            if own:
+                work.free()
+        return info.as_dict()
+
+    def pcg(self, b, x, maxit=10000, atol=0.0, rtol=1e-8, dtol=1e6, work=None):
+        """CG preconditioned by one V-cycle per iteration, device-resident."""
+        return self._run("hflmp_pcg_mg_f64", b, x, maxit, atol, rtol, dtol, work)
+
+    def solve(self, b, x, maxit=1000, atol=0.0, rtol=1e-8, dtol=1e6, work=None):
+        """Multigrid iteration x = x + M (b - A x) until the residual test."""
+        return self._run("hflmp_mg_solve_f64", b, x, maxit, atol, rtol, dtol, work)
+
+    def free(self):
+        if self.h:
+            self.ctx.L.hflmp_mg_destroy(self.h)
+            self.h = C.c_void_p()
+        for m in self.owned:
+            m.free()
+        self.owned = []
 
 
 class StencilDesc(C.Structure):

@@ -231,6 +231,65 @@ def bench_stencil(hflmp, ctx, n, x, b, steps, sell_cg=None, assembled=None):
     return out
 
 
+def bench_mg(hflmp, ctx, n, steps, sell_cg=None):
+    """Geometric multigrid (V-cycle, damped Jacobi 1+1, Galerkin coarse
+    operators down to 2^3 cells) as the CG preconditioner at n^3: time of
+    one V-cycle, and CG to 1e-8 against Jacobi-CG on the constant-
+    coefficient system and on the log-normal (sigma 1) one.  Second solve
+    timed; the hierarchy is built once per system (setup reported)."""
+    from ba_amd import mg
+    E = hflmp.fe_element(1, 3, n)
+    b = hflmp.fe_rhs(1, 3, n, 20241202)
+    out = {"workload": "3D Q1 %d^3, FP64, CG preconditioned by one V-cycle, rtol 1e-8" % n,
+           "cycle": "V(1,1), damped Jacobi w 0.8, 8 coarse sweeps",
+           "levels": mg.level_sizes(n)}
+    db = ctx.upload(b)
+    for name, kap in (("poisson", np.ones(n ** 3)),
+                      ("lognormal_sigma1", hflmp.fe_kappa(3, n, 1.0, KAPPA_SEED))):
+        rp, ci, v = hflmp.fe_system(1, 3, n, None if name == "poisson" else kap)
+        A = hflmp.CsrMatrix(ctx, rp, ci, v)
+        t0 = time.perf_counter()
+        M = mg.build_structured(ctx, 3, n, E, kap, A, pre_smooth=1, post_smooth=1,
+                                coarse_sweeps=8, omega=0.8, coarse_omega=0.8)
+        ctx.sync()
+        setup = time.perf_counter() - t0
+        rec = {"setup_s": round(setup, 3)}
+        dr = ctx.upload(b)
+        dz = ctx.empty(A.nrow, np.float64)
+        rec["vcycle_us"] = round(time_launches(ctx, lambda: M.apply(dr, dz), steps) * 1e6, 2)
+        for rep in range(2):
+            dx = ctx.zeros(A.nrow, np.float64)
+            info = M.pcg(db, dx, maxit=2000, rtol=1e-8)
+            if rep == 1:
+                x = dx.download()
+            dx.free()
+        rec.update({"iters": info["iters"], "status": hflmp.STATUS.get(info["status"], "?"),
+                    "seconds_to_tol": round(info["seconds"], 5),
+                    "ms_per_iter": round(info["seconds"] * 1e3 / max(info["iters"], 1), 4),
+                    "true_res_rel": None})
+        dxr = ctx.upload(x)
+        hflmp.spmv(ctx, A, dxr, dz)
+        rec["true_res_rel"] = float(np.linalg.norm(b - dz.download()) / np.linalg.norm(b))
+        dxr.free()
+        jac = sell_cg if name == "poisson" else None
+        if jac is None:
+            S = hflmp.SellMatrix(ctx, A, 0)
+            invd = hflmp.invdiag(ctx, A)
+            for rep in range(2):
+                dx = ctx.zeros(A.nrow, np.float64)
+                ji = hflmp.pcg_jacobi_sell(ctx, S, invd, db, dx, maxit=20000, rtol=1e-8)
+                dx.free()
+            jac = {"iters": ji["iters"], "seconds_to_tol": ji["seconds"]}
+            S.free(); invd.free()
+        rec["jacobi_sell_cg"] = {"iters": jac["iters"],
+                                 "seconds_to_tol": round(jac["seconds_to_tol"], 5)}
+        rec["speedup_vs_jacobi_sell_cg"] = round(jac["seconds_to_tol"] / info["seconds"], 3)
+        out[name] = rec
+        M.free(); A.free(); dr.free(); dz.free()
+    db.free()
+    return out
+
+
 def bench_assembly(hflmp, n, kappa_seed=KAPPA_SEED):
     """Device assembly of the 3D Q1 n^3 system from its cell -> dof table
     (pattern, then values cell by cell), with the Dirichlet rows applied,
@@ -595,6 +654,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-stencil", action="store_true")
     ap.add_argument("--no-assembly", action="store_true")
+    ap.add_argument("--no-mg", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=200)
     ap.add_argument("--serve-seed", type=int, default=20241205)
@@ -713,6 +773,9 @@ def main():
         out["matrix_free"] = bench_stencil(hflmp, ctx, n, x, b, max(args.steps // 2, 1),
                                            out.get("cg", {}).get("sell"),
                                            None if args.no_parity else (S, rp, ci, v))
+    if not args.no_mg:
+        out["multigrid"] = bench_mg(hflmp, ctx, n, max(args.steps // 10, 5),
+                                    out.get("cg", {}).get("sell"))
     S.free(); A.free(); dx.free(); dy.free()
     ctx.close()
 

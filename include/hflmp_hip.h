@@ -204,10 +204,11 @@ int hflmp_cast_csr_f64_to_f32(hflmp_ctx *ctx, int nnz, const double *val,
  * not keep).  flags HFLMP_SELL_COL16 stores columns as int16 offsets from
  * the slice's first row when every offset fits (else int32).  Returns
  * HFLMP_EFORMAT if a row has more than 255 entries or the stored entries
- * (nonzeros plus padding) would exceed 3 nnz + 64 P; use the CSR kernels
- * then.  Each lane stops at
+ * (nonzeros plus padding) would exceed 3 nnz + 64 P (unless flags has
+ * HFLMP_SELL_ANY_PADDING); use the CSR kernels then.  Each lane stops at
  * its row's length: padding past a row's last chunk of P is never read. */
 #define HFLMP_SELL_COL16 1
+#define HFLMP_SELL_ANY_PADDING 2
 typedef struct hflmp_sell hflmp_sell;
 typedef struct {
   int nrow, ncol, nnz;
@@ -428,6 +429,64 @@ int hflmp_pcg_jacobi_stencil_f32(hflmp_ctx *ctx, const hflmp_stencil *A,
                                  int maxit, double atol, double rtol,
                                  double dtol, void *work, size_t work_bytes,
                                  hflmp_solve_info *info);
+
+/* ---- geometric multigrid ----------------------------------------------------
+ * GMG (linear_solver/gmg.h:979-1230, GMGBase::SolveImpl / SolveImplLevel)
+ * over nlevels CSR operators A[0] (finest) .. A[nlevels-1] with
+ * restrictions R[l] (n_{l+1} x n_l) and prolongations P[l] (n_l x n_{l+1});
+ * all arrays are DEVICE CSR arrays, copied at create() into SELL-64
+ * layouts (any padding accepted).  One V-cycle z = M r from level l with
+ * right-hand side b_l (b_0 = r):
+ *   x_l = w D^-1 b_l, then pre_smooth-1 damped Jacobi sweeps
+ *     x = (w D^-1 (b - (A - D) x)) + ((1 - w) x)
+ *     (kernel_async_inner_damped_iter_no_offdiag, cuda_async_iter_kernels.h:
+ *     116-144; the diagonal is skipped, D^-1 from hflmp_csr_extract_invdiag);
+ *   r_l = b_l - A x_l, subtracted term by term in row order
+ *     (kernel_gmg_compute_residual_no_offdiag, cuda_gmg_kernel.h:46-64);
+ *   b_{l+1} = R r_l (kernel_gmg_compute_restriction_no_offdiag, :89-108);
+ *   the cycle from level l+1;
+ *   x_l = x_l + P x_{l+1}, added term by term onto x_l
+ *     (kernel_gmg_compute_updated_solution_no_offdiag, :137-157);
+ *   post_smooth sweeps.
+ * The coarsest level takes coarse_sweeps sweeps with coarse_omega.  With
+ * pre_smooth == post_smooth and R = P^T the cycle is a symmetric operator
+ * and preconditions CG.  Dirichlet (unit) rows with zero right-hand side
+ * stay zero on every level when R has zero rows for coarse Dirichlet nodes
+ * and P zero rows for fine ones. */
+typedef struct {
+  int pre_smooth;       /* sweeps before the coarse correction, >= 1       */
+  int post_smooth;      /* sweeps after it, >= 0                           */
+  int coarse_sweeps;    /* sweeps on the coarsest level, >= 1              */
+  double omega;         /* damping on the other levels, > 0                */
+  double coarse_omega;  /* damping on the coarsest level, > 0              */
+} hflmp_mg_params;
+typedef struct {
+  int nrow, ncol, nnz;
+  const int *row_ptr, *col;
+  const double *val;
+} hflmp_csr_f64;
+typedef struct hflmp_mg hflmp_mg;
+int hflmp_mg_create_f64(hflmp_ctx *ctx, int nlevels, const hflmp_csr_f64 *A,
+                        const hflmp_csr_f64 *R, const hflmp_csr_f64 *P,
+                        const hflmp_mg_params *prm, hflmp_mg **out);
+void hflmp_mg_destroy(hflmp_mg *m);
+/* z = M r (one V-cycle from z = 0) */
+int hflmp_mg_apply_f64(hflmp_ctx *ctx, const hflmp_mg *m, const double *r,
+                       double *z);
+/* CG::SolvePrecond (linear_solver/cg.h:165-246) with z = M r, every scalar
+ * on the device as in hflmp_pcg_jacobi_f64; A[0] is the operator. */
+size_t hflmp_mg_workspace_bytes(const hflmp_mg *m);
+int hflmp_pcg_mg_f64(hflmp_ctx *ctx, const hflmp_mg *m, const double *b,
+                     double *x, int maxit, double atol, double rtol,
+                     double dtol, void *work, size_t work_bytes,
+                     hflmp_solve_info *info);
+/* Multigrid as the solver (GMGBase::SolveImpl, gmg.h:979-1066): r = b - A x,
+ * then x = x + M r and a new residual per iteration until IterateControl
+ * stops on ||r||. */
+int hflmp_mg_solve_f64(hflmp_ctx *ctx, const hflmp_mg *m, const double *b,
+                       double *x, int maxit, double atol, double rtol,
+                       double dtol, void *work, size_t work_bytes,
+                       hflmp_solve_info *info);
 
 /* ---- several right-hand sides (no reference counterpart: the reference
  * solves one right-hand side per CG::Solve, linear_solver/cg.h:165-246) ---

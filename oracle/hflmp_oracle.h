@@ -36,6 +36,10 @@
  *   BiCGSTAB      linear_solver/bicgstab.h:127-483
  *   assembly      assembly/assembly_utils.cc:144 (pattern),
  *                 lmatrix_csr_cpu.cc:1102-1153 (add_values, cell by cell)
+ *   GMG residual  linear_algebra/lmp/cuda/cuda_gmg_kernel.h:46-64
+ *   damped Jacobi linear_algebra/lmp/cuda/cuda_async_iter_kernels.h:116-144
+ *                 (one synchronous sweep); the V-cycle itself is in
+ *                 oracle.py (linear_solver/gmg.h:979-1187)
  *
  * Problem generator: structured Q1 (2D/3D) and Q2 (2D) Poisson/diffusion on
  * the unit square/cube, assembled element by element in lexicographic cell
@@ -88,6 +92,11 @@ int or_multicoloring(int nrow, const int *rowptr, const int *col,
                           const T *v, const T *x, T *y);                      \
   void or_spmv_omp_##S(int nrow, const int *rp, const int *ci, const T *v,    \
                        const T *x, T *y, int nthreads);                       \
+  void or_residual_sub_##S(int nrow, const int *rp, const int *ci,            \
+                           const T *v, const T *b, const T *x, T *r);         \
+  void or_jacobi_damped_##S(int nrow, const int *rp, const int *ci,           \
+                            const T *v, const T *invd, T w, const T *b,       \
+                            const T *xi, T *xo);                              \
   T or_dot_##S(int n, const T *x, const T *y);                                \
   T or_nrm2_##S(int n, const T *x);                                           \
   T or_norm1_##S(int n, const T *x);                                          \

@@ -61,6 +61,8 @@ def lib():
             for n in ("spmv", "spmv_add", "spmv_nodiag"):
                 g(n).argtypes = [C.c_int, _i32p, _i32p, fp, fp, fp]
             g("spmv_omp").argtypes = [C.c_int, _i32p, _i32p, fp, fp, fp, C.c_int]
+            g("residual_sub").argtypes = [C.c_int, _i32p, _i32p, fp, fp, fp, fp]
+            g("jacobi_damped").argtypes = [C.c_int, _i32p, _i32p, fp, fp, ct, fp, fp, fp]
             g("dot").argtypes = [C.c_int, fp, fp]
             g("dot").restype = ct
             for n in ("nrm2", "norm1", "normmax"):
@@ -475,3 +477,130 @@ def fe_cell_dofs(order, dim, n):
     gy = order * cy.reshape(-1, 1) + ly.reshape(1, -1)
     gz = order * cz.reshape(-1, 1) + lz.reshape(1, -1)
     return (gx + p * (gy + p * gz)).astype(np.int32)
+
+
+# ---- geometric multigrid -------------------------------------------------------------
+def residual_sub(rp, ci, v, b, x):
+    """kernel_gmg_compute_residual_no_offdiag (cuda_gmg_kernel.h:46-64)."""
+    r = np.empty(len(rp) - 1, v.dtype)
+    getattr(lib(), "or_residual_sub_" + _sfx(v))(len(rp) - 1, rp, ci, v, b, x, r)
+    return r
+
+
+def jacobi_damped(rp, ci, v, invd, w, b, x):
+    """One synchronous sweep of kernel_async_inner_damped_iter_no_offdiag
+    (cuda_async_iter_kernels.h:116-144)."""
+    out = np.empty(len(rp) - 1, v.dtype)
+    getattr(lib(), "or_jacobi_damped_" + _sfx(v))(len(rp) - 1, rp, ci, v, invd, w, b, x, out)
+    return out
+
+
+def mg_cycle(levels, b, pre=1, post=1, coarse=4, omega=0.7, coarse_omega=0.7, l=0):
+    """One V-cycle from zero (GMGBase::SolveImplLevel, gmg.h:1070-1187, with
+    the GPU kernels of cuda_gmg_kernel.h).  levels[l] is a dict with
+    A=(rp, ci, v), invd and, above the coarsest level, R and P as
+    (rp, ci, v).  The first sweep from x = 0 is x = (w invd) b."""
+    L = levels[l]
+    last = l == len(levels) - 1
+    w = coarse_omega if last else omega
+    rp, ci, v = L["A"]
+    invd = L["invd"]
+    x = w * invd * b
+    for _ in range((coarse if last else pre) - 1):
+        x = jacobi_damped(rp, ci, v, invd, w, b, x)
+    if last:
+        return x
+    r = residual_sub(rp, ci, v, b, x)
+    bc = spmv(*L["R"], r)
+    xc = mg_cycle(levels, bc, pre, post, coarse, omega, coarse_omega, l + 1)
+    spmv_add(*L["P"], xc, x)
+    for _ in range(post):
+        x = jacobi_damped(rp, ci, v, invd, w, b, x)
+    return x
+
+
+def cg_prec(rp, ci, v, b, prec, x0=None, maxit=10000, atol=0.0, rtol=1e-8, dtol=1e6):
+    """The preconditioned branch of or_cg (CG::SolvePrecond, cg.h:165-246)
+    with z = prec(r) for any callable preconditioner."""
+    n = len(rp) - 1
+    x = np.zeros(n, v.dtype) if x0 is None else np.array(x0, v.dtype)
+    r = spmv(rp, ci, v, x)
+    scaleadd(-1.0, np.ascontiguousarray(b, v.dtype), r)
+    res0 = res = float(np.sqrt(dot(r, r)))
+    hist = [res]
+
+    def check(it, res):
+        if res <= atol or res < rtol * res0:
+            return 0
+        if res > dtol * res0:
+            return 2
+        if it >= maxit:
+            return 1
+        return -1
+
+    it = 0
+    conv = check(0, res)
+    z = prec(r)
+    p = z.copy()
+    beta = dot(r, z)
+    while conv == -1:
+        it += 1
+        Ap = spmv(rp, ci, v, p)
+        alpha = beta / dot(Ap, p)
+        axpy(alpha, p, x)
+        axpy(-alpha, Ap, r)
+        res = float(np.sqrt(dot(r, r)))
+        hist.append(res)
+        conv = check(it, res)
+        if conv != -1:
+            break
+        z = prec(r)
+        gamma = dot(r, z)
+        scaleadd(gamma / beta, z, p)
+        beta = gamma
+    return x, dict(status=conv, iters=it, res_init=res0, res_final=res, history=hist)
+
+
+def mg_solve(levels, b, x0=None, maxit=1000, atol=0.0, rtol=1e-8, dtol=1e6, **cycle):
+    """Multigrid as the solver (GMGBase::SolveImpl, gmg.h:979-1066) in the
+    correction form x = x + cycle(r), r = b - A x term by term; the
+    stopping test IterateControl::Check on ||r|| with res0 = ||b - A x0||."""
+    rp, ci, v = levels[0]["A"]
+    x = np.zeros(len(b)) if x0 is None else np.array(x0, np.float64)
+    r = residual_sub(rp, ci, v, b, x)
+    res0 = res = float(np.sqrt(dot(r, r)))
+
+    def check(it, res):
+        if res <= atol or res < rtol * res0:
+            return 0
+        if res > dtol * res0:
+            return 2
+        if it >= maxit:
+            return 1
+        return -1
+
+    it = 0
+    conv = check(0, res)
+    while conv == -1:
+        it += 1
+        x = x + mg_cycle(levels, r, **cycle)
+        r = residual_sub(rp, ci, v, b, x)
+        res = float(np.sqrt(dot(r, r)))
+        conv = check(it, res)
+    return x, dict(status=conv, iters=it, res_init=res0, res_final=res)
+
+
+def mg_levels(A, R, P, cell_mats, cell_dofs, fixed):
+    """Host hierarchy for mg_cycle: level 0 is A = (rp, ci, v); level l > 0
+    is assembled cell by cell (fe_assemble_cells) from cell_mats[l-1] over
+    the cell -> dof table cell_dofs[l-1], with unit Dirichlet rows at
+    fixed[l-1]; R[l], P[l] as (rp, ci, v)."""
+    levels = [dict(A=A, invd=invdiag(*A))]
+    for l in range(1, len(R) + 1):
+        rpc, cic, vc = fe_assemble_cells(len(R[l - 1][0]) - 1, cell_dofs[l - 1],
+                                         cell_mats=cell_mats[l - 1])
+        zero_rows(fixed[l - 1], rpc, cic, vc)
+        levels[l - 1]["R"] = R[l - 1]
+        levels[l - 1]["P"] = P[l - 1]
+        levels.append(dict(A=(rpc, cic, vc), invd=invdiag(rpc, cic, vc)))
+    return levels

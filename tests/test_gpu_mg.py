@@ -1,0 +1,165 @@
+"""Geometric multigrid on the device against the CPU oracle: the V-cycle
+z = M r bit-identical to the oracle's cycle (2D and 3D, several depths and
+smoother settings, one level), CG preconditioned by it and multigrid as
+the solver within 2 iterations and 1e-8 of the oracle's, and the
+refusals."""
+import numpy as np
+import pytest
+
+from ba_amd import hflmp, mg
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+SMOOTH = [dict(pre_smooth=1, post_smooth=1, coarse_sweeps=8, omega=0.8, coarse_omega=0.8),
+          dict(pre_smooth=2, post_smooth=3, coarse_sweeps=1, omega=0.6, coarse_omega=1.0)]
+
+
+def _oracle_kw(p):
+    return dict(pre=p["pre_smooth"], post=p["post_smooth"], coarse=p["coarse_sweeps"],
+                omega=p["omega"], coarse_omega=p["coarse_omega"])
+
+
+def _setup(ctx, dim, n, nlevels=None, params=SMOOTH[0], seed=7):
+    kap = hflmp.fe_kappa(dim, n, 1.0, seed)
+    E = hflmp.fe_element(1, dim, n)
+    A = hflmp.fe_system(1, dim, n, kap)
+    dA = hflmp.CsrMatrix(ctx, *A)
+    M = mg.build_structured(ctx, dim, n, E, kap, dA, nlevels=nlevels, **params)
+    sizes = mg.level_sizes(n, nlevels)
+    mats = mg.coarse_cell_mats(dim, n, E, kap, sizes)
+    levels = O.mg_levels(A, [mg.restriction(dim, nc) for nc in sizes[1:]],
+                         [mg.prolongation(dim, nc) for nc in sizes[1:]], mats,
+                         [hflmp.fe_cell_dofs(1, dim, nc) for nc in sizes[1:]],
+                         [hflmp.fe_fixed(1, dim, nc) for nc in sizes[1:]])
+    return A, dA, M, levels
+
+
+@pytest.mark.parametrize("p", range(len(SMOOTH)))
+@pytest.mark.parametrize("case", [(2, 32, None), (3, 16, None), (3, 12, None), (3, 16, 2)])
+def test_vcycle_bit_exact(ctx, case, p):
+    dim, n, nl = case
+    print("vcycle", case, p, flush=True)
+    A, dA, M, levels = _setup(ctx, dim, n, nl, SMOOTH[p])
+    try:
+        N = dA.nrow
+        inner = np.ones(N, bool)
+        inner[hflmp.fe_fixed(1, dim, n)] = False
+        r = np.random.default_rng(n).uniform(-1, 1, N) * inner
+        dr, dz = ctx.upload(r), ctx.zeros(N, np.float64)
+        M.apply(dr, dz)
+        z = dz.download()
+        assert np.array_equal(z, O.mg_cycle(levels, r, **_oracle_kw(SMOOTH[p])))
+        M.apply(dr, dz)     # the level buffers keep no state between cycles
+        assert np.array_equal(dz.download(), z)
+        dr.free(), dz.free()
+    finally:
+        M.free()
+        dA.free()
+
+
+def test_single_level_is_damped_jacobi(ctx):
+    rp, ci, v = hflmp.fe_system(1, 2, 9)
+    dA = hflmp.CsrMatrix(ctx, rp, ci, v)
+    M = hflmp.Multigrid(ctx, [dA], [], [], coarse_sweeps=5, coarse_omega=0.55)
+    try:
+        b = hflmp.fe_rhs(1, 2, 9, 2)
+        db, dz = ctx.upload(b), ctx.zeros(len(b), np.float64)
+        M.apply(db, dz)
+        lv = [dict(A=(rp, ci, v), invd=O.invdiag(rp, ci, v))]
+        assert np.array_equal(dz.download(), O.mg_cycle(lv, b, coarse=5, coarse_omega=0.55))
+        db.free(), dz.free()
+    finally:
+        M.free()
+        dA.free()
+
+
+@pytest.mark.parametrize("case", [(2, 64), (3, 16), (3, 32), (3, 64)])
+def test_pcg_mg_vs_oracle(ctx, case):
+    dim, n = case
+    print("pcg_mg", case, flush=True)
+    A, dA, M, levels = _setup(ctx, dim, n)
+    try:
+        b = hflmp.fe_rhs(1, dim, n, 5)
+        db, dx = ctx.upload(b), ctx.zeros(len(b), np.float64)
+        info = M.pcg(db, dx, rtol=1e-8)
+        x = dx.download()
+        xo, io = O.cg_prec(*A, b, lambda r: O.mg_cycle(levels, r, **_oracle_kw(SMOOTH[0])),
+                           rtol=1e-8)
+        assert info["status"] == 0 and io["status"] == 0
+        assert abs(info["iters"] - io["iters"]) <= 2
+        assert abs(info["res_init"] - io["res_init"]) <= 1e-12 * io["res_init"]
+        assert np.linalg.norm(x - xo) <= 1e-8 * np.linalg.norm(xo)
+        # the true residual
+        res = b - O.spmv(*A, x)
+        assert np.linalg.norm(res) <= 2e-8 * np.linalg.norm(b)
+        _, jac = O.cg(*A, b)
+        assert info["iters"] * 3 < jac["iters"]
+        db.free(), dx.free()
+    finally:
+        M.free()
+        dA.free()
+
+
+@pytest.mark.parametrize("case", [(2, 32), (3, 16)])
+def test_mg_solve_vs_oracle(ctx, case):
+    dim, n = case
+    A, dA, M, levels = _setup(ctx, dim, n)
+    try:
+        b = hflmp.fe_rhs(1, dim, n, 6)
+        x0 = np.random.default_rng(1).uniform(-1, 1, len(b))
+        x0[hflmp.fe_fixed(1, dim, n)] = 0.0
+        db, dx = ctx.upload(b), ctx.upload(x0)
+        info = M.solve(db, dx, rtol=1e-9, maxit=200)
+        xo, io = O.mg_solve(levels, b, x0, rtol=1e-9, maxit=200, **_oracle_kw(SMOOTH[0]))
+        assert info["status"] == 0 and io["status"] == 0
+        assert abs(info["iters"] - io["iters"]) <= 1
+        assert info["res_init"] == io["res_init"] or \
+            abs(info["res_init"] - io["res_init"]) <= 1e-12 * io["res_init"]
+        assert np.linalg.norm(dx.download() - xo) <= 1e-8 * np.linalg.norm(xo)
+        # maxit reached: status 1 and exactly maxit cycles
+        dx2 = ctx.upload(x0)
+        i2 = M.solve(db, dx2, rtol=1e-30, maxit=3)
+        assert i2["status"] == 1 and i2["iters"] == 3
+        db.free(), dx.free(), dx2.free()
+    finally:
+        M.free()
+        dA.free()
+
+
+def test_mg_refusals(ctx):
+    rp, ci, v = hflmp.fe_system(1, 2, 8)
+    dA = hflmp.CsrMatrix(ctx, rp, ci, v)
+    rr, rc, rv = mg.restriction(2, 4)
+    pr, pc, pv = mg.prolongation(2, 4)
+    R = hflmp.CsrMatrix(ctx, rr, rc, rv, ncol=81)
+    P = hflmp.CsrMatrix(ctx, pr, pc, pv, ncol=25)
+    Ac = hflmp.CsrMatrix(ctx, *hflmp.fe_system(1, 2, 4))
+    try:
+        for kw in (dict(pre_smooth=0), dict(post_smooth=-1), dict(coarse_sweeps=0),
+                   dict(omega=0.0), dict(coarse_omega=-1.0)):
+            with pytest.raises(hflmp.HflmpError) as e:
+                hflmp.Multigrid(ctx, [dA, Ac], [R], [P], **kw)
+            assert e.value.code == hflmp.EINVAL
+        with pytest.raises(hflmp.HflmpError):      # R and P swapped
+            hflmp.Multigrid(ctx, [dA, Ac], [P], [R])
+        with pytest.raises(hflmp.HflmpError):      # coarse operator of the wrong size
+            hflmp.Multigrid(ctx, [dA, dA], [R], [P])
+        M = hflmp.Multigrid(ctx, [dA, Ac], [R], [P])
+        try:
+            r = ctx.zeros(dA.nrow, np.float64)
+            with pytest.raises(hflmp.HflmpError):  # z aliasing r
+                M.apply(r, r)
+            small = ctx.empty(16, np.uint8)
+            with pytest.raises(hflmp.HflmpError):
+                M.pcg(r, ctx.zeros(dA.nrow, np.float64), work=small)
+            small.free(), r.free()
+        finally:
+            M.free()
+        # the context stays usable
+        z = ctx.zeros(4, np.float64)
+        assert np.array_equal(z.download(), np.zeros(4))
+        z.free()
+    finally:
+        for m in (dA, R, P, Ac):
+            m.free()
