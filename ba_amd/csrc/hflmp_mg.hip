@@ -26,6 +26,7 @@
 
 #include "hflmp_sell.hpp"
 #include "hflmp_solver_common.hpp"
+#include "hflmp_stencil.hpp"
 
 namespace {
 
@@ -95,11 +96,74 @@ __global__ void __launch_bounds__(kBlock)
   if (tr.has_row) y[tr.row] = s;
 }
 
+// ---- the finest level given matrix-free (a StencilOperator) --------------
+// An operator without stored entries has no off-diagonal product, so its
+// smoother is the reference's Richardson iteration with the Jacobi
+// preconditioner (linear_solver/richardson.h:136-251, SolveImplFullResidual):
+//   r = b; A.VectorMultAdd(-1, x, 1, r);  z = r .* invd;  x = x + w z
+// and its residual is that VectorMultAdd (gmg.h:1110-1119).  The products
+// have the stencil's VectorMultAdd arithmetic: each row starts from
+// b * 1 + 0 and adds (-1 f_c) (E x)_c cell by cell.
+
+// x = w (b .* invd): the sweep from x = 0
+__global__ void __launch_bounds__(kBlock)
+    mg_st_first_kernel(int n, double w, const double *__restrict__ invd,
+                       const double *__restrict__ b, double *__restrict__ x, const int *status) {
+#pragma clang fp contract(off)
+  if (stopped(status)) return;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+    x[i] = w * (b[i] * invd[i]);
+}
+
+template <class SOp>
+__device__ __forceinline__ double st_residual(const SOp &A, long long unit,
+                                              const double *__restrict__ b,
+                                              const double *__restrict__ x, TileRow &tr) {
+#pragma clang fp contract(off)
+  const long long row = A.thread_row(unit);
+  const double s = row < A.nrow ? b[row] * 1.0 + 0.0 : 0.0;
+  return A.row_sum_scaled(unit, x, s, -1.0, tr);
+}
+
+// xo = xi + w ((b - A xi) .* invd)
+template <class SOp>
+__global__ void __launch_bounds__(kBlock)
+    mg_st_sweep_kernel(SOp A, double w, const double *__restrict__ invd,
+                       const double *__restrict__ b, const double *__restrict__ xi,
+                       double *__restrict__ xo, const int *status) {
+#pragma clang fp contract(off)
+  if (stopped(status)) return;
+  const long long unit = xcd_tile(blockIdx.x, gridDim.x);
+  TileRow tr;
+  const double r = st_residual(A, unit, b, xi, tr);
+  if (tr.has_row) xo[tr.row] = xi[tr.row] + w * (r * invd[tr.row]);
+}
+
+// r = b - A x (PART: and the partials of r.r)
+template <class SOp, bool PART>
+__global__ void __launch_bounds__(kBlock)
+    mg_st_residual_kernel(SOp A, const double *__restrict__ b, const double *__restrict__ x,
+                          double *__restrict__ r, double *__restrict__ partials,
+                          const int *status) {
+#pragma clang fp contract(off)
+  if (stopped(status)) return;
+  const long long unit = xcd_tile(blockIdx.x, gridDim.x);
+  TileRow tr;
+  const double ri = st_residual(A, unit, b, x, tr);
+  double rr = 0;
+  if (tr.has_row) {
+    r[tr.row] = ri;
+    rr = ri * ri;
+  }
+  if (PART) block_partial1(rr, partials);
+}
+
 // ---- CG with the cycle as preconditioner (CG::SolvePrecond, cg.h:165-246) --
 // init: r = b - (A x) (VectorMult, then ScaleAdd(b, -1)), z = cycle(r), then
 // the r.r and r.z partials
+template <class AOp>
 __global__ void __launch_bounds__(kBlock)
-    mgcg_residual_kernel(Op A, const double *__restrict__ b, const double *__restrict__ x,
+    mgcg_residual_kernel(AOp A, const double *__restrict__ b, const double *__restrict__ x,
                          double *__restrict__ r) {
 #pragma clang fp contract(off)
   const long long unit = xcd_tile(blockIdx.x, gridDim.x);
@@ -137,8 +201,9 @@ __global__ void __launch_bounds__(kFinalBlock)
 }
 
 // Ap = A p, partials of p.Ap
+template <class AOp>
 __global__ void __launch_bounds__(kBlock)
-    mgcg_spmv_kernel(Op A, const double *__restrict__ p, double *__restrict__ Ap,
+    mgcg_spmv_kernel(AOp A, const double *__restrict__ p, double *__restrict__ Ap,
                      double *__restrict__ partials, const int *status) {
   if (stopped(status)) return;
   const long long unit = xcd_tile(blockIdx.x, gridDim.x);
@@ -280,6 +345,7 @@ struct hflmp_mg {
   int device = 0;
   int nlevels = 0;
   hflmp_mg_params prm;
+  const hflmp_stencil *st = nullptr;  // level-0 operator when matrix-free
   struct Level {
     int n = 0;
     hflmp_sell *A = nullptr, *R = nullptr, *P = nullptr;
@@ -310,6 +376,54 @@ void mg_free(hflmp_mg *m) {
   delete m;
 }
 
+// Calls fn(op) with the level-0 operator view: the stencil's or SELL-64.
+template <typename Fn>
+void with_fine_op(const hflmp_mg *m, Fn &&fn) {
+  if (m->st)
+    (void)stencil_dispatch<double>(m->st, fn);
+  else
+    fn(sell_view<double, int>(m->lv[0].A));
+}
+
+// The level's smoothing steps and residual.
+void smooth_first(hipStream_t s, const hflmp_mg *m, int l, double w, const double *b, double *x,
+                  const int *status) {
+  const auto &L = m->lv[l];
+  const unsigned gv = grid_for(L.n, 4);
+  if (l == 0 && m->st)
+    mg_st_first_kernel<<<gv, kBlock, 0, s>>>(L.n, w, L.invd, b, x, status);
+  else
+    mg_first_kernel<<<gv, kBlock, 0, s>>>(L.n, w, L.invd, b, x, status);
+}
+
+void smooth_sweep(hipStream_t s, const hflmp_mg *m, int l, double w, const double *b,
+                  const double *xi, double *xo, const int *status) {
+  const auto &L = m->lv[l];
+  const unsigned ga = sell_grid(L.n);
+  if (l == 0 && m->st) {
+    (void)stencil_dispatch<double>(m->st, [&](auto op) {
+      mg_st_sweep_kernel<<<ga, kBlock, 0, s>>>(op, w, L.invd, b, xi, xo, status);
+    });
+  } else {
+    mg_sweep_kernel<<<ga, kBlock, 0, s>>>(sell_view<double, int>(L.A), w, 1.0 - w, L.invd, b,
+                                          xi, xo, status);
+  }
+}
+
+void residual(hipStream_t s, const hflmp_mg *m, int l, const double *b, const double *x,
+              double *r, const int *status) {
+  const auto &L = m->lv[l];
+  const unsigned ga = sell_grid(L.n);
+  if (l == 0 && m->st) {
+    (void)stencil_dispatch<double>(m->st, [&](auto op) {
+      mg_st_residual_kernel<decltype(op), false>
+          <<<ga, kBlock, 0, s>>>(op, b, x, r, nullptr, status);
+    });
+  } else {
+    mg_residual_kernel<<<ga, kBlock, 0, s>>>(sell_view<double, int>(L.A), b, x, r, status);
+  }
+}
+
 // One cycle from level l with right-hand side b; returns the buffer that
 // holds the result.  `out` (level 0) is the buffer the result must end in:
 // the ping-pong pair starts so that the last write lands there.  Every
@@ -319,9 +433,7 @@ double *cycle(hipStream_t s, const hflmp_mg *m, int l, const double *b, double *
   const auto &L = m->lv[l];
   const bool coarsest = l == m->nlevels - 1;
   const double w = coarsest ? m->prm.coarse_omega : m->prm.omega;
-  const double omw = 1.0 - w;
-  const Op A = sell_view<double, int>(L.A);
-  const unsigned ga = sell_grid(L.n), gv = grid_for(L.n, 4);
+  const unsigned ga = sell_grid(L.n);
   const int first = coarsest ? m->prm.coarse_sweeps : m->prm.pre_smooth;
   const int swaps = (first - 1) + (coarsest ? 0 : m->prm.post_smooth);
   double *x = L.x, *y = L.x2;
@@ -329,20 +441,20 @@ double *cycle(hipStream_t s, const hflmp_mg *m, int l, const double *b, double *
     x = (swaps & 1) ? L.x2 : out;
     y = (swaps & 1) ? out : L.x2;
   }
-  mg_first_kernel<<<gv, kBlock, 0, s>>>(L.n, w, L.invd, b, x, status);
+  smooth_first(s, m, l, w, b, x, status);
   for (int k = 1; k < first; ++k) {
-    mg_sweep_kernel<<<ga, kBlock, 0, s>>>(A, w, omw, L.invd, b, x, y, status);
+    smooth_sweep(s, m, l, w, b, x, y, status);
     std::swap(x, y);
   }
   if (coarsest) return x;
   const auto &C = m->lv[l + 1];
-  mg_residual_kernel<<<ga, kBlock, 0, s>>>(A, b, x, L.r, status);
+  residual(s, m, l, b, x, L.r, status);
   mg_apply_kernel<false><<<sell_grid(C.n), kBlock, 0, s>>>(sell_view<double, int>(L.R), L.r,
                                                             C.b, status);
   const double *xc = cycle(s, m, l + 1, C.b, nullptr, status);
   mg_apply_kernel<true><<<ga, kBlock, 0, s>>>(sell_view<double, int>(L.P), xc, x, status);
   for (int k = 0; k < m->prm.post_smooth; ++k) {
-    mg_sweep_kernel<<<ga, kBlock, 0, s>>>(A, w, omw, L.invd, b, x, y, status);
+    smooth_sweep(s, m, l, w, b, x, y, status);
     std::swap(x, y);
   }
   return x;
@@ -353,13 +465,9 @@ void apply(hipStream_t s, const hflmp_mg *m, const double *r, double *z, const i
   (void)cycle(s, m, 0, r, z, status);
 }
 
-}  // namespace
-
-extern "C" {
-
-int hflmp_mg_create_f64(hflmp_ctx *ctx, int nlevels, const hflmp_csr_f64 *A,
-                        const hflmp_csr_f64 *R, const hflmp_csr_f64 *P,
-                        const hflmp_mg_params *prm, hflmp_mg **out) {
+int mg_build(hflmp_ctx *ctx, const hflmp_stencil *S, int nlevels, const hflmp_csr_f64 *A,
+             const hflmp_csr_f64 *R, const hflmp_csr_f64 *P, const hflmp_mg_params *prm,
+             hflmp_mg **out) {
   if (!ctx) return HFLMP_EINVAL;
   HFLMP_REQUIRE(ctx, A && out && prm && (nlevels == 1 || (R && P)), "mg_create: null pointer");
   *out = nullptr;
@@ -369,12 +477,19 @@ int hflmp_mg_create_f64(hflmp_ctx *ctx, int nlevels, const hflmp_csr_f64 *A,
                          prm->omega > 0 && prm->coarse_omega > 0,
                 "mg_create: need pre_smooth >= 1, post_smooth >= 0, coarse_sweeps >= 1, "
                 "omega > 0, coarse_omega > 0");
+  if (S) {
+    HFLMP_REQUIRE(ctx, S->value_bytes == 8 && S->device == ctx->device,
+                  "mg_create_stencil: the operator must be FP64 and on this device");
+    HFLMP_REQUIRE(ctx, S->nnode < (1LL << 31), "mg_create_stencil: too many nodes");
+  }
+  auto rows = [&](int l) -> long long { return (l == 0 && S) ? S->nnode : A[l].nrow; };
   for (int l = 0; l < nlevels; ++l) {
-    HFLMP_REQUIRE(ctx, A[l].nrow > 0 && A[l].ncol == A[l].nrow && A[l].row_ptr,
-                  "mg_create: operator %d must be square and non-empty", l);
+    if (!(l == 0 && S))
+      HFLMP_REQUIRE(ctx, A[l].nrow > 0 && A[l].ncol == A[l].nrow && A[l].row_ptr,
+                    "mg_create: operator %d must be square and non-empty", l);
     if (l + 1 < nlevels) {
-      HFLMP_REQUIRE(ctx, R[l].nrow == A[l + 1].nrow && R[l].ncol == A[l].nrow &&
-                             P[l].nrow == A[l].nrow && P[l].ncol == A[l + 1].nrow,
+      HFLMP_REQUIRE(ctx, R[l].nrow == rows(l + 1) && R[l].ncol == rows(l) &&
+                             P[l].nrow == rows(l) && P[l].ncol == rows(l + 1),
                     "mg_create: R[%d] must be n%d x n%d and P[%d] n%d x n%d", l, l + 1, l, l,
                     l, l + 1);
     }
@@ -383,6 +498,7 @@ int hflmp_mg_create_f64(hflmp_ctx *ctx, int nlevels, const hflmp_csr_f64 *A,
   m->device = ctx->device;
   m->nlevels = nlevels;
   m->prm = *prm;
+  m->st = S;
   int rc = HFLMP_OK;
   auto sell_of = [&](const hflmp_csr_f64 &c, hflmp_sell **o) {
     return hflmp_sell_create_f64(ctx, c.nrow, c.ncol, c.nnz, c.row_ptr, c.col, c.val,
@@ -390,9 +506,10 @@ int hflmp_mg_create_f64(hflmp_ctx *ctx, int nlevels, const hflmp_csr_f64 *A,
   };
   for (int l = 0; l < nlevels && rc == HFLMP_OK; ++l) {
     auto &L = m->lv[l];
-    L.n = A[l].nrow;
+    const bool mf = l == 0 && S;
+    L.n = (int)rows(l);
     const size_t bytes = sizeof(double) * (size_t)L.n;
-    if ((rc = sell_of(A[l], &L.A)) != HFLMP_OK) break;
+    if (!mf && (rc = sell_of(A[l], &L.A)) != HFLMP_OK) break;
     if (l + 1 < nlevels && ((rc = sell_of(R[l], &L.R)) != HFLMP_OK ||
                             (rc = sell_of(P[l], &L.P)) != HFLMP_OK))
       break;
@@ -403,9 +520,10 @@ int hflmp_mg_create_f64(hflmp_ctx *ctx, int nlevels, const hflmp_csr_f64 *A,
       break;
     }
     int bad = 0;
-    if ((rc = hflmp_csr_extract_invdiag_f64(ctx, L.n, A[l].row_ptr, A[l].col, A[l].val, L.invd,
-                                            &bad)) != HFLMP_OK)
-      break;
+    rc = mf ? hflmp_stencil_extract_invdiag_f64(ctx, S, L.invd, &bad)
+            : hflmp_csr_extract_invdiag_f64(ctx, L.n, A[l].row_ptr, A[l].col, A[l].val, L.invd,
+                                            &bad);
+    if (rc != HFLMP_OK) break;
     if (bad) {
       rc = set_error(ctx, HFLMP_ENUMERIC, "mg_create: %d rows of operator %d have no diagonal",
                      bad, l);
@@ -419,6 +537,59 @@ int hflmp_mg_create_f64(hflmp_ctx *ctx, int nlevels, const hflmp_csr_f64 *A,
   m->grid0 = sell_grid(m->lv[0].n);
   *out = m;
   return HFLMP_OK;
+}
+
+struct MgWork {
+  CgState<double> *st;
+  double *partials, *v[4];
+};
+
+MgWork mg_work(const hflmp_mg *m, void *work) {
+  const size_t n = (size_t)m->lv[0].n;
+  const size_t g = (size_t)(m->grid0 > kMaxRedBlocks ? m->grid0 : kMaxRedBlocks);
+  char *w = static_cast<char *>(work);
+  MgWork k;
+  k.st = reinterpret_cast<CgState<double> *>(w);
+  k.partials = reinterpret_cast<double *>(w + 256);
+  const size_t vec0 = 256 + align256(2 * g * sizeof(double)), vs = align256(n * sizeof(double));
+  for (int i = 0; i < 4; ++i) k.v[i] = reinterpret_cast<double *>(w + vec0 + i * vs);
+  return k;
+}
+
+int finish(hflmp_ctx *ctx, CgState<double> *st, std::chrono::steady_clock::time_point t0,
+           hflmp_solve_info *info) {
+  CgState<double> h;
+  HFLMP_HIP_CHECK(ctx, hipMemcpyAsync(&h, st, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+  HFLMP_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  info->iters = h.iter;
+  info->status = h.status == kIterate ? kMaxIts : h.status;
+  info->res_init = h.res0;
+  info->res_final = h.res;
+  info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return HFLMP_OK;
+}
+
+int check_solve(hflmp_ctx *ctx, const hflmp_mg *m, const double *b, double *x, int maxit,
+                double atol, double rtol, double dtol, void *work, size_t work_bytes,
+                hflmp_solve_info *info, const char *what);
+
+}  // namespace
+
+extern "C" {
+
+int hflmp_mg_create_f64(hflmp_ctx *ctx, int nlevels, const hflmp_csr_f64 *A,
+                        const hflmp_csr_f64 *R, const hflmp_csr_f64 *P,
+                        const hflmp_mg_params *prm, hflmp_mg **out) {
+  return mg_build(ctx, nullptr, nlevels, A, R, P, prm, out);
+}
+
+int hflmp_mg_create_stencil_f64(hflmp_ctx *ctx, const hflmp_stencil *S, int nlevels,
+                                const hflmp_csr_f64 *A, const hflmp_csr_f64 *R,
+                                const hflmp_csr_f64 *P, const hflmp_mg_params *prm,
+                                hflmp_mg **out) {
+  if (!ctx) return HFLMP_EINVAL;
+  HFLMP_REQUIRE(ctx, S != nullptr, "mg_create_stencil: null operator");
+  return mg_build(ctx, S, nlevels, A, R, P, prm, out);
 }
 
 void hflmp_mg_destroy(hflmp_mg *m) { mg_free(m); }
@@ -443,37 +614,28 @@ size_t hflmp_mg_workspace_bytes(const hflmp_mg *m) {
 int hflmp_pcg_mg_f64(hflmp_ctx *ctx, const hflmp_mg *m, const double *b, double *x, int maxit,
                      double atol, double rtol, double dtol, void *work, size_t work_bytes,
                      hflmp_solve_info *info) {
-  if (!ctx) return HFLMP_EINVAL;
-  HFLMP_REQUIRE(ctx, m && b && x && work && info, "pcg_mg: null pointer");
-  HFLMP_REQUIRE(ctx, maxit > 0 && atol >= 0 && rtol >= 0 && dtol > 1,
-                "pcg_mg: bad control parameters (maxit>0, atol>=0, rtol>=0, dtol>1)");
-  HFLMP_REQUIRE(ctx, work_bytes >= hflmp_mg_workspace_bytes(m), "pcg_mg: workspace %zu < %zu",
-                work_bytes, hflmp_mg_workspace_bytes(m));
+  int rc = check_solve(ctx, m, b, x, maxit, atol, rtol, dtol, work, work_bytes, info, "pcg_mg");
+  if (rc) return rc;
   const int n = m->lv[0].n;
-  const size_t g = (size_t)(m->grid0 > kMaxRedBlocks ? m->grid0 : kMaxRedBlocks);
-  char *w = static_cast<char *>(work);
-  auto *st = reinterpret_cast<CgState<double> *>(w);
-  double *partials = reinterpret_cast<double *>(w + 256);
-  const size_t vec0 = 256 + align256(2 * g * sizeof(double)), vs = align256(n * sizeof(double));
-  double *r = reinterpret_cast<double *>(w + vec0);
-  double *p = reinterpret_cast<double *>(w + vec0 + vs);
-  double *z = reinterpret_cast<double *>(w + vec0 + 2 * vs);
-  double *Ap = reinterpret_cast<double *>(w + vec0 + 3 * vs);
+  MgWork k = mg_work(m, work);
+  CgState<double> *st = k.st;
+  double *partials = k.partials, *r = k.v[0], *p = k.v[1], *z = k.v[2], *Ap = k.v[3];
   hipStream_t s = ctx->stream;
-  const Op A = sell_view<double, int>(m->lv[0].A);
   const unsigned ga = (unsigned)m->grid0;
   const int gd = grid_for(n, 4) < kMaxRedBlocks ? (int)grid_for(n, 4) : kMaxRedBlocks;
   const int *status = &st->status;
   const auto t0 = std::chrono::steady_clock::now();
   init_state_kernel<double><<<1, kBlock, 0, s>>>(st, maxit, atol, rtol, dtol);
-  mgcg_residual_kernel<<<ga, kBlock, 0, s>>>(A, b, x, r);
+  with_fine_op(m, [&](auto op) { mgcg_residual_kernel<<<ga, kBlock, 0, s>>>(op, b, x, r); });
   apply(s, m, r, z, nullptr);
   mgcg_dots_kernel<<<gd, kBlock, 0, s>>>(n, r, z, partials, nullptr);
   mgcg_init_final<<<1, kFinalBlock, 0, s>>>(st, partials, gd);
   mgcg_p_kernel<<<gd, kBlock, 0, s>>>(n, z, p, st, 1);
   HFLMP_LAUNCH_CHECK(ctx);
   auto iteration = [&]() {
-    mgcg_spmv_kernel<<<ga, kBlock, 0, s>>>(A, p, Ap, partials, status);
+    with_fine_op(m, [&](auto op) {
+      mgcg_spmv_kernel<<<ga, kBlock, 0, s>>>(op, p, Ap, partials, status);
+    });
     mgcg_alpha_final<<<1, kFinalBlock, 0, s>>>(st, partials, (int)ga);
     mgcg_update_kernel<<<gd, kBlock, 0, s>>>(n, p, Ap, x, r, st, partials);
     mgcg_check_final<<<1, kFinalBlock, 0, s>>>(st, partials, gd);
@@ -482,63 +644,68 @@ int hflmp_pcg_mg_f64(hflmp_ctx *ctx, const hflmp_mg *m, const double *b, double 
     mgcg_beta_final<<<1, kFinalBlock, 0, s>>>(st, partials, gd);
     mgcg_p_kernel<<<gd, kBlock, 0, s>>>(n, z, p, st, 0);
   };
-  int rc = run_polled(ctx, maxit, &st->status, iteration, 2);
+  rc = run_polled(ctx, maxit, &st->status, iteration, 2);
   if (rc) return rc;
-  CgState<double> h;
-  HFLMP_HIP_CHECK(ctx, hipMemcpyAsync(&h, st, sizeof(h), hipMemcpyDeviceToHost, s));
-  HFLMP_HIP_CHECK(ctx, hipStreamSynchronize(s));
-  info->iters = h.iter;
-  info->status = h.status == kIterate ? kMaxIts : h.status;
-  info->res_init = h.res0;
-  info->res_final = h.res;
-  info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  return HFLMP_OK;
+  return finish(ctx, st, t0, info);
 }
 
 int hflmp_mg_solve_f64(hflmp_ctx *ctx, const hflmp_mg *m, const double *b, double *x, int maxit,
                        double atol, double rtol, double dtol, void *work, size_t work_bytes,
                        hflmp_solve_info *info) {
-  if (!ctx) return HFLMP_EINVAL;
-  HFLMP_REQUIRE(ctx, m && b && x && work && info, "mg_solve: null pointer");
-  HFLMP_REQUIRE(ctx, maxit > 0 && atol >= 0 && rtol >= 0 && dtol > 1,
-                "mg_solve: bad control parameters (maxit>0, atol>=0, rtol>=0, dtol>1)");
-  HFLMP_REQUIRE(ctx, work_bytes >= hflmp_mg_workspace_bytes(m), "mg_solve: workspace %zu < %zu",
-                work_bytes, hflmp_mg_workspace_bytes(m));
+  int rc = check_solve(ctx, m, b, x, maxit, atol, rtol, dtol, work, work_bytes, info,
+                       "mg_solve");
+  if (rc) return rc;
   const int n = m->lv[0].n;
-  const size_t g = (size_t)(m->grid0 > kMaxRedBlocks ? m->grid0 : kMaxRedBlocks);
-  char *w = static_cast<char *>(work);
-  auto *st = reinterpret_cast<CgState<double> *>(w);
-  double *partials = reinterpret_cast<double *>(w + 256);
-  const size_t vec0 = 256 + align256(2 * g * sizeof(double)), vs = align256(n * sizeof(double));
-  double *r = reinterpret_cast<double *>(w + vec0);
-  double *z = reinterpret_cast<double *>(w + vec0 + vs);
+  MgWork k = mg_work(m, work);
+  CgState<double> *st = k.st;
+  double *partials = k.partials, *r = k.v[0], *z = k.v[1];
   hipStream_t s = ctx->stream;
-  const Op A = sell_view<double, int>(m->lv[0].A);
   const unsigned ga = (unsigned)m->grid0;
   const int *status = &st->status;
+  // r = b - A x with the partials of r.r: the level-0 residual of the cycle
+  auto resid = [&](const int *stat) {
+    if (m->st) {
+      (void)stencil_dispatch<double>(m->st, [&](auto op) {
+        mg_st_residual_kernel<decltype(op), true>
+            <<<ga, kBlock, 0, s>>>(op, b, x, r, partials, stat);
+      });
+    } else {
+      mgs_residual_kernel<<<ga, kBlock, 0, s>>>(sell_view<double, int>(m->lv[0].A), b, x, r,
+                                                partials, stat);
+    }
+  };
   const auto t0 = std::chrono::steady_clock::now();
   init_state_kernel<double><<<1, kBlock, 0, s>>>(st, maxit, atol, rtol, dtol);
-  mgs_residual_kernel<<<ga, kBlock, 0, s>>>(A, b, x, r, partials, nullptr);
+  resid(nullptr);
   mgs_check_final<<<1, kFinalBlock, 0, s>>>(st, partials, (int)ga, 1);
   HFLMP_LAUNCH_CHECK(ctx);
   // the reference's GMG::SolveImpl (gmg.h:979-1066): cycle, new residual, check
   auto iteration = [&]() {
     apply(s, m, r, z, status);
     mgs_add_kernel<<<grid_for(n, 4), kBlock, 0, s>>>(n, z, x, status);
-    mgs_residual_kernel<<<ga, kBlock, 0, s>>>(A, b, x, r, partials, status);
+    resid(status);
     mgs_check_final<<<1, kFinalBlock, 0, s>>>(st, partials, (int)ga, 0);
   };
-  int rc = run_polled(ctx, maxit, &st->status, iteration, 2);
+  rc = run_polled(ctx, maxit, &st->status, iteration, 2);
   if (rc) return rc;
-  CgState<double> h;
-  HFLMP_HIP_CHECK(ctx, hipMemcpyAsync(&h, st, sizeof(h), hipMemcpyDeviceToHost, s));
-  HFLMP_HIP_CHECK(ctx, hipStreamSynchronize(s));
-  info->iters = h.iter;
-  info->status = h.status == kIterate ? kMaxIts : h.status;
-  info->res_init = h.res0;
-  info->res_final = h.res;
-  info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  return HFLMP_OK;
+  return finish(ctx, st, t0, info);
 }
 
 }  // extern "C"
+
+namespace {
+
+int check_solve(hflmp_ctx *ctx, const hflmp_mg *m, const double *b, double *x, int maxit,
+                double atol, double rtol, double dtol, void *work, size_t work_bytes,
+                hflmp_solve_info *info, const char *what) {
+  if (!ctx) return HFLMP_EINVAL;
+  HFLMP_REQUIRE(ctx, m && b && x && work && info, "%s: null pointer", what);
+  HFLMP_REQUIRE(ctx, m->device == ctx->device, "%s: hierarchy on another device", what);
+  HFLMP_REQUIRE(ctx, maxit > 0 && atol >= 0 && rtol >= 0 && dtol > 1,
+                "%s: bad control parameters (maxit>0, atol>=0, rtol>=0, dtol>1)", what);
+  HFLMP_REQUIRE(ctx, work_bytes >= hflmp_mg_workspace_bytes(m), "%s: workspace %zu < %zu", what,
+                work_bytes, hflmp_mg_workspace_bytes(m));
+  return HFLMP_OK;
+}
+
+}  // namespace

@@ -161,6 +161,7 @@ def hip_lib():
         L.hflmp_csr_diag_ptr.argtypes = [vp, i, vp, vp, vp]
         L.hflmp_csr_zero_rows_f64.argtypes = [vp, i, vp, vp, vp, vp, d]
         L.hflmp_mg_create_f64.argtypes = [vp, i, vp, vp, vp, vp, vp]
+        L.hflmp_mg_create_stencil_f64.argtypes = [vp, vp, i, vp, vp, vp, vp, vp]
         L.hflmp_mg_destroy.argtypes = [vp]
         L.hflmp_mg_destroy.restype = None
         L.hflmp_mg_apply_f64.argtypes = [vp, vp, vp, vp]
@@ -688,22 +689,31 @@ class CsrDesc(C.Structure):
 class Multigrid:
     """Device V-cycle over the levels A[0] (finest) .. A[L-1] with
     restrictions R[l] and prolongations P[l] (device CsrMatrix objects,
-    float64; copied at creation).  `owned` matrices are freed with it."""
+    float64; copied at creation).  With `stencil` (an FP64
+    StencilOperator, kept referenced) the finest operator is matrix-free
+    and A[0] may be None.  `owned` matrices are freed with it."""
 
     def __init__(self, ctx, A, R, P, pre_smooth=1, post_smooth=1, coarse_sweeps=4, omega=0.7,
-                 coarse_omega=0.7, owned=()):
+                 coarse_omega=0.7, owned=(), stencil=None):
         self.ctx = ctx
         self.nlevels = len(A)
-        self.n = A[0].nrow
+        self.stencil = stencil
+        self.n = stencil.nrow if stencil is not None else A[0].nrow
         self.params = MgParams(pre_smooth, post_smooth, coarse_sweeps, omega, coarse_omega)
-        a = (CsrDesc * len(A))(*[CsrDesc.of(m) for m in A])
+        descs = [CsrDesc(self.n, self.n, 0, None, None, None) if (l == 0 and m is None)
+                 else CsrDesc.of(m) for l, m in enumerate(A)]
+        a = (CsrDesc * len(A))(*descs)
         r = (CsrDesc * max(len(R), 1))(*[CsrDesc.of(m) for m in R])
         p = (CsrDesc * max(len(P), 1))(*[CsrDesc.of(m) for m in P])
         self.h = C.c_void_p()
         self.owned = list(owned)
-        _call(ctx, "hflmp_mg_create_f64", self.nlevels, a, r, p, C.byref(self.params),
-              C.byref(self.h))
-        self.level_rows = [m.nrow for m in A]
+        if stencil is not None:
+            _call(ctx, "hflmp_mg_create_stencil_f64", stencil.h, self.nlevels, a, r, p,
+                  C.byref(self.params), C.byref(self.h))
+        else:
+            _call(ctx, "hflmp_mg_create_f64", self.nlevels, a, r, p, C.byref(self.params),
+                  C.byref(self.h))
+        self.level_rows = [d.nrow for d in descs]
 
     def apply(self, r, z):
         """z = M r (one V-cycle from zero); r and z device arrays."""

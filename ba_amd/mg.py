@@ -158,11 +158,14 @@ def coarse_cell_mats(dim, n, E, kappa, sizes):
     return out
 
 
-def build_structured(ctx, dim, n, E, kappa, A, nlevels=None, min_cells=2, **params):
+def build_structured(ctx, dim, n, E, kappa, A, nlevels=None, min_cells=2, stencil=None,
+                     **params):
     """Multigrid over the structured Q1 mesh of n^dim cells whose finest
     operator A (a device CsrMatrix) was assembled from kappa_c E with unit
-    Dirichlet rows on the boundary.  Returns an hflmp.Multigrid that owns
-    the coarse operators, R and P."""
+    Dirichlet rows on the boundary, or is given matrix-free as `stencil`
+    (a StencilOperator with local matrix E and cell factors kappa; A is
+    then None).  Returns an hflmp.Multigrid that owns the coarse
+    operators, R and P."""
     sizes = level_sizes(n, nlevels, min_cells)
     mats = coarse_cell_mats(dim, n, E, kappa, sizes)
     As, Rs, Ps = [A], [], []
@@ -176,7 +179,7 @@ def build_structured(ctx, dim, n, E, kappa, A, nlevels=None, min_cells=2, **para
             owned += [Rs[-1], Ps[-1]]
             As.append(hflmp.assemble_q1(ctx, dim, nc, mats[l]))
             owned.append(As[-1])
-        return hflmp.Multigrid(ctx, As, Rs, Ps, owned=owned, **params)
+        return hflmp.Multigrid(ctx, As, Rs, Ps, owned=owned, stencil=stencil, **params)
     except Exception:
         for m in owned:
             m.free()

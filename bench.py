@@ -284,8 +284,33 @@ def bench_mg(hflmp, ctx, n, steps, sell_cg=None):
         rec["jacobi_sell_cg"] = {"iters": jac["iters"],
                                  "seconds_to_tol": round(jac["seconds_to_tol"], 5)}
         rec["speedup_vs_jacobi_sell_cg"] = round(jac["seconds_to_tol"] / info["seconds"], 3)
+        M.free(); A.free()
+        # the same with the finest level matrix-free (StencilOperator)
+        St = hflmp.StencilOperator(ctx, 1, 3, [n] * 3, E, kap, hflmp.fe_fixed(1, 3, n))
+        t0 = time.perf_counter()
+        M = mg.build_structured(ctx, 3, n, E, kap, None, stencil=St, pre_smooth=1,
+                                post_smooth=1, coarse_sweeps=8, omega=0.8, coarse_omega=0.8)
+        ctx.sync()
+        mf = {"setup_s": round(time.perf_counter() - t0, 3),
+              "vcycle_us": round(time_launches(ctx, lambda: M.apply(dr, dz), steps) * 1e6, 2)}
+        for rep in range(2):
+            dx = ctx.zeros(St.nrow, np.float64)
+            info = M.pcg(db, dx, maxit=2000, rtol=1e-8)
+            dx.free()
+        invd = St.invdiag()
+        for rep in range(2):
+            dx = ctx.zeros(St.nrow, np.float64)
+            ji = hflmp.pcg_jacobi_stencil(ctx, St, invd, db, dx, maxit=20000, rtol=1e-8)
+            dx.free()
+        mf.update({"iters": info["iters"], "status": hflmp.STATUS.get(info["status"], "?"),
+                   "seconds_to_tol": round(info["seconds"], 5),
+                   "jacobi_stencil_cg": {"iters": ji["iters"],
+                                         "seconds_to_tol": round(ji["seconds"], 5)},
+                   "speedup_vs_jacobi_sell_cg": round(jac["seconds_to_tol"] / info["seconds"], 3),
+                   "speedup_vs_jacobi_stencil_cg": round(ji["seconds"] / info["seconds"], 3)})
+        rec["matrix_free_fine_level"] = mf
         out[name] = rec
-        M.free(); A.free(); dr.free(); dz.free()
+        M.free(); St.free(); invd.free(); dr.free(); dz.free()
     db.free()
     return out
 

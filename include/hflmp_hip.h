@@ -469,6 +469,17 @@ typedef struct hflmp_mg hflmp_mg;
 int hflmp_mg_create_f64(hflmp_ctx *ctx, int nlevels, const hflmp_csr_f64 *A,
                         const hflmp_csr_f64 *R, const hflmp_csr_f64 *P,
                         const hflmp_mg_params *prm, hflmp_mg **out);
+/* The same with the finest operator matrix-free: S (FP64 stencil, which
+ * must outlive the hierarchy) replaces A[0] (not read) on level 0, and
+ * D^-1 comes from S.  Level 0 then smooths with the reference's Richardson
+ * iteration with the Jacobi preconditioner (linear_solver/richardson.h:
+ * 136-251, SolveImplFullResidual): r = b, S.VectorMultAdd(-1, x, 1, r),
+ * x = x + w (r .* D^-1); its residual is that VectorMultAdd, the first step
+ * x = w (b .* D^-1). */
+int hflmp_mg_create_stencil_f64(hflmp_ctx *ctx, const hflmp_stencil *S,
+                                int nlevels, const hflmp_csr_f64 *A,
+                                const hflmp_csr_f64 *R, const hflmp_csr_f64 *P,
+                                const hflmp_mg_params *prm, hflmp_mg **out);
 void hflmp_mg_destroy(hflmp_mg *m);
 /* z = M r (one V-cycle from z = 0) */
 int hflmp_mg_apply_f64(hflmp_ctx *ctx, const hflmp_mg *m, const double *r,

@@ -495,37 +495,61 @@ def jacobi_damped(rp, ci, v, invd, w, b, x):
     return out
 
 
+def _lv_residual(L, b, x):
+    """Level residual: the GMG kernel's term-by-term form for a CSR level;
+    VectorMultAdd(-1, x, 1, r = b) of the stencil for a matrix-free one
+    (gmg.h:1110-1119)."""
+    if "S" in L:
+        return stencil_vmadd(*L["S"], x, y=b, alpha=-1.0, beta=1.0)
+    return residual_sub(*L["A"], b, x)
+
+
+def _lv_sweep(L, w, b, x):
+    """Damped Jacobi (cuda_async_iter_kernels.h:116-144) on a CSR level;
+    Richardson with the Jacobi preconditioner (richardson.h:136-251,
+    x + w (r .* invd)) on a matrix-free one."""
+    if "S" in L:
+        return x + w * (_lv_residual(L, b, x) * L["invd"])
+    return jacobi_damped(*L["A"], L["invd"], w, b, x)
+
+
+def _lv_first(L, w, b):
+    """The sweep from x = 0 without the product with zeros."""
+    return w * (b * L["invd"]) if "S" in L else w * L["invd"] * b
+
+
 def mg_cycle(levels, b, pre=1, post=1, coarse=4, omega=0.7, coarse_omega=0.7, l=0):
     """One V-cycle from zero (GMGBase::SolveImplLevel, gmg.h:1070-1187, with
-    the GPU kernels of cuda_gmg_kernel.h).  levels[l] is a dict with
-    A=(rp, ci, v), invd and, above the coarsest level, R and P as
-    (rp, ci, v).  The first sweep from x = 0 is x = (w invd) b."""
+    the GPU kernels of cuda_gmg_kernel.h).  levels[l] is a dict with the
+    operator A=(rp, ci, v) or, matrix-free on level 0, S=(order, dim, nc,
+    E, factor, fixed) for stencil_vmadd; invd; and above the coarsest level
+    R and P as (rp, ci, v)."""
     L = levels[l]
     last = l == len(levels) - 1
     w = coarse_omega if last else omega
-    rp, ci, v = L["A"]
-    invd = L["invd"]
-    x = w * invd * b
+    x = _lv_first(L, w, b)
     for _ in range((coarse if last else pre) - 1):
-        x = jacobi_damped(rp, ci, v, invd, w, b, x)
+        x = _lv_sweep(L, w, b, x)
     if last:
         return x
-    r = residual_sub(rp, ci, v, b, x)
+    r = _lv_residual(L, b, x)
     bc = spmv(*L["R"], r)
     xc = mg_cycle(levels, bc, pre, post, coarse, omega, coarse_omega, l + 1)
     spmv_add(*L["P"], xc, x)
     for _ in range(post):
-        x = jacobi_damped(rp, ci, v, invd, w, b, x)
+        x = _lv_sweep(L, w, b, x)
     return x
 
 
-def cg_prec(rp, ci, v, b, prec, x0=None, maxit=10000, atol=0.0, rtol=1e-8, dtol=1e6):
+def cg_prec(A, b, prec, x0=None, maxit=10000, atol=0.0, rtol=1e-8, dtol=1e6):
     """The preconditioned branch of or_cg (CG::SolvePrecond, cg.h:165-246)
-    with z = prec(r) for any callable preconditioner."""
-    n = len(rp) - 1
-    x = np.zeros(n, v.dtype) if x0 is None else np.array(x0, v.dtype)
-    r = spmv(rp, ci, v, x)
-    scaleadd(-1.0, np.ascontiguousarray(b, v.dtype), r)
+    with z = prec(r) for any callable preconditioner; A is (rp, ci, v) or a
+    callable y = A x."""
+    op = A if callable(A) else (lambda x: spmv(*A, x))
+    n = len(b)
+    x = np.zeros(n) if x0 is None else np.array(x0, np.float64)
+    r = op(x)
+    scaleadd(-1.0, np.ascontiguousarray(b, np.float64), r)
     res0 = res = float(np.sqrt(dot(r, r)))
     hist = [res]
 
@@ -545,7 +569,7 @@ def cg_prec(rp, ci, v, b, prec, x0=None, maxit=10000, atol=0.0, rtol=1e-8, dtol=
     beta = dot(r, z)
     while conv == -1:
         it += 1
-        Ap = spmv(rp, ci, v, p)
+        Ap = op(p)
         alpha = beta / dot(Ap, p)
         axpy(alpha, p, x)
         axpy(-alpha, Ap, r)
@@ -565,9 +589,8 @@ def mg_solve(levels, b, x0=None, maxit=1000, atol=0.0, rtol=1e-8, dtol=1e6, **cy
     """Multigrid as the solver (GMGBase::SolveImpl, gmg.h:979-1066) in the
     correction form x = x + cycle(r), r = b - A x term by term; the
     stopping test IterateControl::Check on ||r|| with res0 = ||b - A x0||."""
-    rp, ci, v = levels[0]["A"]
     x = np.zeros(len(b)) if x0 is None else np.array(x0, np.float64)
-    r = residual_sub(rp, ci, v, b, x)
+    r = _lv_residual(levels[0], b, x)
     res0 = res = float(np.sqrt(dot(r, r)))
 
     def check(it, res):
@@ -584,7 +607,7 @@ def mg_solve(levels, b, x0=None, maxit=1000, atol=0.0, rtol=1e-8, dtol=1e6, **cy
     while conv == -1:
         it += 1
         x = x + mg_cycle(levels, r, **cycle)
-        r = residual_sub(rp, ci, v, b, x)
+        r = _lv_residual(levels[0], b, x)
         res = float(np.sqrt(dot(r, r)))
         conv = check(it, res)
     return x, dict(status=conv, iters=it, res_init=res0, res_final=res)

@@ -84,7 +84,7 @@ def test_pcg_mg_vs_oracle(ctx, case):
         db, dx = ctx.upload(b), ctx.zeros(len(b), np.float64)
         info = M.pcg(db, dx, rtol=1e-8)
         x = dx.download()
-        xo, io = O.cg_prec(*A, b, lambda r: O.mg_cycle(levels, r, **_oracle_kw(SMOOTH[0])),
+        xo, io = O.cg_prec(A, b, lambda r: O.mg_cycle(levels, r, **_oracle_kw(SMOOTH[0])),
                            rtol=1e-8)
         assert info["status"] == 0 and io["status"] == 0
         assert abs(info["iters"] - io["iters"]) <= 2
@@ -163,3 +163,88 @@ def test_mg_refusals(ctx):
     finally:
         for m in (dA, R, P, Ac):
             m.free()
+
+
+def _setup_mf(ctx, dim, n, params=SMOOTH[0], seed=7):
+    kap = hflmp.fe_kappa(dim, n, 1.0, seed)
+    E = hflmp.fe_element(1, dim, n)
+    fixed = hflmp.fe_fixed(1, dim, n)
+    S = hflmp.StencilOperator(ctx, 1, dim, [n] * dim, E, kap, fixed)
+    M = mg.build_structured(ctx, dim, n, E, kap, None, stencil=S, **params)
+    A = hflmp.fe_system(1, dim, n, kap)
+    sizes = mg.level_sizes(n)
+    mats = mg.coarse_cell_mats(dim, n, E, kap, sizes)
+    levels = O.mg_levels(A, [mg.restriction(dim, nc) for nc in sizes[1:]],
+                         [mg.prolongation(dim, nc) for nc in sizes[1:]], mats,
+                         [hflmp.fe_cell_dofs(1, dim, nc) for nc in sizes[1:]],
+                         [hflmp.fe_fixed(1, dim, nc) for nc in sizes[1:]])
+    del levels[0]["A"]
+    levels[0]["S"] = (1, dim, [n] * dim, E, kap, fixed)
+    return S, M, levels
+
+
+@pytest.mark.parametrize("p", range(len(SMOOTH)))
+@pytest.mark.parametrize("case", [(2, 32), (3, 16), (3, 12)])
+def test_vcycle_matrix_free_bit_exact(ctx, case, p):
+    dim, n = case
+    print("vcycle_mf", case, p, flush=True)
+    S, M, levels = _setup_mf(ctx, dim, n, SMOOTH[p])
+    try:
+        inner = np.ones(S.nrow, bool)
+        inner[hflmp.fe_fixed(1, dim, n)] = False
+        r = np.random.default_rng(n + 1).uniform(-1, 1, S.nrow) * inner
+        dr, dz = ctx.upload(r), ctx.zeros(S.nrow, np.float64)
+        M.apply(dr, dz)
+        assert np.array_equal(dz.download(), O.mg_cycle(levels, r, **_oracle_kw(SMOOTH[p])))
+        dr.free(), dz.free()
+    finally:
+        M.free()
+        S.free()
+
+
+@pytest.mark.parametrize("case", [(2, 64), (3, 32)])
+def test_pcg_mg_matrix_free_vs_oracle(ctx, case):
+    dim, n = case
+    print("pcg_mg_mf", case, flush=True)
+    S, M, levels = _setup_mf(ctx, dim, n)
+    try:
+        b = hflmp.fe_rhs(1, dim, n, 5)
+        db, dx = ctx.upload(b), ctx.zeros(len(b), np.float64)
+        info = M.pcg(db, dx, rtol=1e-8)
+        x = dx.download()
+        op = lambda v: O.stencil_vmadd(*levels[0]["S"], v)
+        xo, io = O.cg_prec(op, b, lambda r: O.mg_cycle(levels, r, **_oracle_kw(SMOOTH[0])),
+                           rtol=1e-8)
+        assert info["status"] == 0 and io["status"] == 0
+        assert abs(info["iters"] - io["iters"]) <= 2
+        assert np.linalg.norm(x - xo) <= 1e-8 * np.linalg.norm(xo)
+        assert np.linalg.norm(b - op(x)) <= 2e-8 * np.linalg.norm(b)
+        # multigrid as the solver through the same hierarchy
+        dx2 = ctx.zeros(len(b), np.float64)
+        i2 = M.solve(db, dx2, rtol=1e-8, maxit=200)
+        x2, o2 = O.mg_solve(levels, b, rtol=1e-8, maxit=200, **_oracle_kw(SMOOTH[0]))
+        assert i2["status"] == 0 and abs(i2["iters"] - o2["iters"]) <= 1
+        assert np.linalg.norm(dx2.download() - x2) <= 1e-8 * np.linalg.norm(x2)
+        db.free(), dx.free(), dx2.free()
+    finally:
+        M.free()
+        S.free()
+
+
+def test_mg_matrix_free_refusals(ctx):
+    n = 8
+    kap = hflmp.fe_kappa(2, n, 1.0, 3)
+    E = hflmp.fe_element(1, 2, n)
+    fixed = hflmp.fe_fixed(1, 2, n)
+    S32 = hflmp.StencilOperator(ctx, 1, 2, [n, n], E, kap, fixed, dtype=np.float32)
+    S = hflmp.StencilOperator(ctx, 1, 2, [n, n + 2], E, np.ones(n * (n + 2)),
+                              hflmp.fe_fixed(1, 2, n)[:4])
+    try:
+        with pytest.raises(hflmp.HflmpError) as e:     # FP32 operator
+            mg.build_structured(ctx, 2, n, E, kap, None, stencil=S32)
+        assert e.value.code == hflmp.EINVAL
+        with pytest.raises(hflmp.HflmpError):          # node count differs from P's rows
+            mg.build_structured(ctx, 2, n, E, kap, None, stencil=S)
+    finally:
+        S32.free()
+        S.free()

@@ -173,10 +173,36 @@ def test_mg_pcg_and_solver_converge(dim, n):
     b = O.fe_rhs(1, dim, n, 5)
     xs = spla.spsolve(_sp(*A).tocsc(), b)
     _, jac = O.cg(*A, b)
-    x, info = O.cg_prec(*A, b, lambda r: O.mg_cycle(levels, r, 1, 1, 8, 0.8, 0.8))
+    x, info = O.cg_prec(A, b, lambda r: O.mg_cycle(levels, r, 1, 1, 8, 0.8, 0.8))
     assert info["status"] == 0 and info["iters"] * 3 < jac["iters"]
     assert np.linalg.norm(x - xs) <= 1e-6 * np.linalg.norm(xs)
     x2, info2 = O.mg_solve(levels, b, rtol=1e-8, pre=1, post=1, coarse=8, omega=0.8,
                            coarse_omega=0.8)
     assert info2["status"] == 0 and info2["iters"] < 60
     assert np.linalg.norm(x2 - xs) <= 1e-6 * np.linalg.norm(xs)
+
+
+@pytest.mark.parametrize("dim,n", [(2, 16), (3, 8)])
+def test_matrix_free_level_agrees_with_assembled(dim, n):
+    """The finest level given as the stencil (Richardson-Jacobi smoothing,
+    VectorMultAdd residual) is the same operator as the assembled level with
+    the damped Jacobi step: the cycles agree to rounding, and stay
+    symmetric."""
+    kap, E, A, _, _, levels = _hierarchy(dim, n)
+    fixed = hflmp.fe_fixed(1, dim, n)
+    mf = [dict(levels[0])] + levels[1:]
+    del mf[0]["A"]
+    mf[0]["S"] = (1, dim, [n] * dim, E, kap, fixed)
+    N = len(A[0]) - 1
+    inner = np.ones(N, bool)
+    inner[fixed] = False
+    rng = np.random.default_rng(5)
+    u, w = rng.standard_normal(N) * inner, rng.standard_normal(N) * inner
+    for kw in (dict(), dict(pre=2, post=2, coarse=3, omega=0.6, coarse_omega=0.9)):
+        a, m = O.mg_cycle(levels, u, **kw), O.mg_cycle(mf, u, **kw)
+        assert np.allclose(m, a, rtol=0, atol=1e-12 * np.abs(a).max())
+    Mu, Mw = O.mg_cycle(mf, u), O.mg_cycle(mf, w)
+    assert abs(u @ Mw - w @ Mu) <= 1e-12 * abs(u @ Mw)
+    # the stencil's residual is b - A x
+    r = O._lv_residual(mf[0], w, u)
+    assert np.allclose(r, w - O.spmv(*A, u), rtol=0, atol=1e-12)
