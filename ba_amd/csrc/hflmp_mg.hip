@@ -341,6 +341,161 @@ __global__ void __launch_bounds__(kBlock)
 
 }  // namespace
 
+// ---- a dense inverse on the coarsest level ---------------------------------
+// x = C b with C stored by columns (ct[j n + i] = C_ij): lane i sums row i
+// from 0 in ascending j, the loads of a wave contiguous.
+constexpr int kDenseBlock = 64;
+constexpr int kDenseMaxRows = 4096;
+
+__global__ void __launch_bounds__(kDenseBlock)
+    mg_dense_kernel(int n, const double *__restrict__ ct, const double *__restrict__ b,
+                    double *__restrict__ x, const int *status) {
+#pragma clang fp contract(off)
+  if (stopped(status)) return;
+  const int i = blockIdx.x * kDenseBlock + threadIdx.x;
+  if (i >= n) return;
+  double s = 0;
+#pragma unroll 8
+  for (int j = 0; j < n; ++j) s += ct[(size_t)j * n + i] * b[j];
+  x[i] = s;
+}
+
+__global__ void __launch_bounds__(kBlock)
+    mg_transpose_kernel(int n, const double *__restrict__ a, double *__restrict__ at) {
+  const long long nn = (long long)n * n;
+  for (long long k = (long long)blockIdx.x * kBlock + threadIdx.x; k < nn;
+       k += (long long)gridDim.x * kBlock) {
+    const long long i = k / n, j = k - i * n;
+    at[j * n + i] = a[k];
+  }
+}
+
+// ---- the small coarse levels in one workgroup ------------------------------
+// Levels of at most kTailMaxRows rows take microseconds of latency per
+// launch and a few per kernel for little work, so the cycle from the first
+// such level down to the coarsest and back runs as one launch of one
+// 1024-thread workgroup, the phases separated by barriers (all its memory
+// traffic stays within the workgroup).  Each phase computes exactly what
+// the per-level kernels above compute.
+constexpr int kTailBlock = 1024;
+constexpr int kTailMaxRows = 2048;
+
+struct MgLevelDev {
+  Op A, R, P;
+  const double *invd;
+  double *b, *x, *x2, *r;
+  int n;
+};
+
+// SellOp views in units of 256 rows: with 1024 threads, unit 4k covers rows
+// 1024k .. 1024k + 1023, thread t taking row 1024k + t.
+__device__ __forceinline__ void tail_first(const MgLevelDev &L, double w, const double *b,
+                                           double *x) {
+#pragma clang fp contract(off)
+  for (int i = threadIdx.x; i < L.n; i += kTailBlock) x[i] = w * L.invd[i] * b[i];
+}
+
+__device__ __forceinline__ void tail_sweep(const MgLevelDev &L, double w, double omw,
+                                           const double *b, const double *xi, double *xo) {
+#pragma clang fp contract(off)
+  for (int k = 0; k * kTailBlock < L.n; ++k) {
+    const long long unit = 4LL * k;
+    const long long r = L.A.thread_row(unit);
+    const double b0 = r < L.n ? b[r] : 0.0;
+    TileRow tr;
+    const double t = L.A.row_sum<true, true>(unit, xi, b0, tr, nullptr);
+    if (tr.has_row) xo[tr.row] = (w * L.invd[tr.row] * t) + (omw * xi[tr.row]);
+  }
+}
+
+__device__ __forceinline__ void tail_residual(const MgLevelDev &L, const double *b,
+                                              const double *x) {
+#pragma clang fp contract(off)
+  for (int k = 0; k * kTailBlock < L.n; ++k) {
+    const long long unit = 4LL * k;
+    const long long r = L.A.thread_row(unit);
+    const double b0 = r < L.n ? b[r] : 0.0;
+    TileRow tr;
+    const double s = L.A.row_sum<true>(unit, x, b0, tr, nullptr);
+    if (tr.has_row) L.r[tr.row] = s;
+  }
+}
+
+template <bool ADD>
+__device__ __forceinline__ void tail_apply(const Op &M, int nrow, const double *x, double *y) {
+  for (int k = 0; k * kTailBlock < nrow; ++k) {
+    const long long unit = 4LL * k;
+    double s = 0;
+    if (ADD) {
+      const long long r = M.thread_row(unit);
+      if (r < nrow) s = y[r];
+    }
+    TileRow tr;
+    s = M.row_sum(unit, x, s, tr, nullptr);
+    if (tr.has_row) y[tr.row] = s;
+  }
+}
+
+// The cycle from level t (right-hand side bin) to the coarsest level L-1.
+// The result of level l ends in x2 when its number of sweeps after the
+// first step is odd, else in x (the host uses the same rule).
+__global__ void __launch_bounds__(kTailBlock)
+    mg_tail_kernel(const MgLevelDev *__restrict__ lv, int t, int L, const double *bin, int pre,
+                   int post, int coarse, double w, double wc, const int *status) {
+  if (stopped(status)) return;
+  const double omw = 1.0 - w, omwc = 1.0 - wc;
+  for (int l = t; l < L - 1; ++l) {
+    const MgLevelDev Lv = lv[l];
+    const double *b = l == t ? bin : Lv.b;
+    double *x = Lv.x, *y = Lv.x2;
+    tail_first(Lv, w, b, x);
+    __syncthreads();
+    for (int k = 1; k < pre; ++k) {
+      tail_sweep(Lv, w, omw, b, x, y);
+      __syncthreads();
+      double *tmp = x;
+      x = y;
+      y = tmp;
+    }
+    tail_residual(Lv, b, x);
+    __syncthreads();
+    tail_apply<false>(Lv.R, lv[l + 1].n, Lv.r, lv[l + 1].b);
+    __syncthreads();
+  }
+  {
+    const MgLevelDev Lv = lv[L - 1];
+    const double *b = L - 1 == t ? bin : Lv.b;
+    double *x = Lv.x, *y = Lv.x2;
+    tail_first(Lv, wc, b, x);
+    __syncthreads();
+    for (int k = 1; k < coarse; ++k) {
+      tail_sweep(Lv, wc, omwc, b, x, y);
+      __syncthreads();
+      double *tmp = x;
+      x = y;
+      y = tmp;
+    }
+  }
+  for (int l = L - 2; l >= t; --l) {
+    const MgLevelDev Lv = lv[l];
+    const MgLevelDev Cv = lv[l + 1];
+    const int csw = l + 1 == L - 1 ? coarse - 1 : (pre - 1) + post;
+    const double *xc = (csw & 1) ? Cv.x2 : Cv.x;
+    const double *b = l == t ? bin : Lv.b;
+    double *x = ((pre - 1) & 1) ? Lv.x2 : Lv.x;
+    double *y = ((pre - 1) & 1) ? Lv.x : Lv.x2;
+    tail_apply<true>(Lv.P, Lv.n, xc, x);
+    __syncthreads();
+    for (int k = 0; k < post; ++k) {
+      tail_sweep(Lv, w, omw, b, x, y);
+      __syncthreads();
+      double *tmp = x;
+      x = y;
+      y = tmp;
+    }
+  }
+}
+
 struct hflmp_mg {
   int device = 0;
   int nlevels = 0;
@@ -352,6 +507,10 @@ struct hflmp_mg {
     double *invd = nullptr, *b = nullptr, *x = nullptr, *x2 = nullptr, *r = nullptr;
   } lv[kMaxLevels];
   int grid0 = 0;  // workgroups of a level-0 operator launch
+  int tail = 0;   // first level of the one-workgroup tail (nlevels: none)
+  int tail0 = 0;  // the tail chosen at creation
+  MgLevelDev *dlv = nullptr;  // device copy of the level table
+  double *cinv = nullptr;     // coarsest level's inverse by columns, or null
 };
 
 namespace {
@@ -372,6 +531,8 @@ void mg_free(hflmp_mg *m) {
     (void)hipFree(L.x2);
     (void)hipFree(L.r);
   }
+  (void)hipFree(m->dlv);
+  (void)hipFree(m->cinv);
   if (cur >= 0 && cur != m->device) (void)hipSetDevice(cur);
   delete m;
 }
@@ -434,12 +595,24 @@ double *cycle(hipStream_t s, const hflmp_mg *m, int l, const double *b, double *
   const bool coarsest = l == m->nlevels - 1;
   const double w = coarsest ? m->prm.coarse_omega : m->prm.omega;
   const unsigned ga = sell_grid(L.n);
+  const bool dense = coarsest && m->cinv;
   const int first = coarsest ? m->prm.coarse_sweeps : m->prm.pre_smooth;
-  const int swaps = (first - 1) + (coarsest ? 0 : m->prm.post_smooth);
+  const int swaps = dense ? 0 : (first - 1) + (coarsest ? 0 : m->prm.post_smooth);
+  if (l > 0 && l == m->tail) {
+    mg_tail_kernel<<<1, kTailBlock, 0, s>>>(m->dlv, l, m->nlevels, b, m->prm.pre_smooth,
+                                            m->prm.post_smooth, m->prm.coarse_sweeps,
+                                            m->prm.omega, m->prm.coarse_omega, status);
+    return (swaps & 1) ? L.x2 : L.x;
+  }
   double *x = L.x, *y = L.x2;
   if (out) {
     x = (swaps & 1) ? L.x2 : out;
     y = (swaps & 1) ? out : L.x2;
+  }
+  if (dense) {
+    mg_dense_kernel<<<(L.n + kDenseBlock - 1) / kDenseBlock, kDenseBlock, 0, s>>>(L.n, m->cinv,
+                                                                                b, x, status);
+    return x;
   }
   smooth_first(s, m, l, w, b, x, status);
   for (int k = 1; k < first; ++k) {
@@ -535,6 +708,39 @@ int mg_build(hflmp_ctx *ctx, const hflmp_stencil *S, int nlevels, const hflmp_cs
     return rc;
   }
   m->grid0 = sell_grid(m->lv[0].n);
+  // the tail: the longest run of small levels ending at the coarsest,
+  // never level 0 (it may be matrix-free and ends in the caller's buffer)
+  m->tail = nlevels;
+  while (m->tail > 1 && m->lv[m->tail - 1].n <= kTailMaxRows) --m->tail;
+  if (m->tail < nlevels) {
+    MgLevelDev h[kMaxLevels];
+    for (int l = m->tail; l < nlevels; ++l) {
+      const auto &L = m->lv[l];
+      MgLevelDev d = {};
+      d.A = sell_view<double, int>(L.A);
+      if (l + 1 < nlevels) {
+        d.R = sell_view<double, int>(L.R);
+        d.P = sell_view<double, int>(L.P);
+      }
+      d.invd = L.invd;
+      d.b = L.b;
+      d.x = L.x;
+      d.x2 = L.x2;
+      d.r = L.r;
+      d.n = L.n;
+      h[l] = d;
+    }
+    if (hipMalloc(&m->dlv, sizeof(MgLevelDev) * nlevels) != hipSuccess) {
+      mg_free(m);
+      return set_error(ctx, HFLMP_ENOMEM, "mg_create: out of device memory");
+    }
+    if (hipMemcpy(m->dlv + m->tail, h + m->tail, sizeof(MgLevelDev) * (nlevels - m->tail),
+                  hipMemcpyHostToDevice) != hipSuccess) {
+      mg_free(m);
+      return set_error(ctx, HFLMP_EHIP, "mg_create: HIP error");
+    }
+  }
+  m->tail0 = m->tail;
   *out = m;
   return HFLMP_OK;
 }
@@ -593,6 +799,35 @@ int hflmp_mg_create_stencil_f64(hflmp_ctx *ctx, const hflmp_stencil *S, int nlev
 }
 
 void hflmp_mg_destroy(hflmp_mg *m) { mg_free(m); }
+
+int hflmp_mg_set_coarse_inverse_f64(hflmp_ctx *ctx, hflmp_mg *m, const double *inv) {
+  if (!ctx) return HFLMP_EINVAL;
+  HFLMP_REQUIRE(ctx, m, "mg_set_coarse_inverse: null hierarchy");
+  HFLMP_REQUIRE(ctx, m->device == ctx->device, "mg_set_coarse_inverse: hierarchy on another "
+                                               "device");
+  const int n = m->lv[m->nlevels - 1].n;
+  if (!inv) {
+    HFLMP_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    (void)hipFree(m->cinv);
+    m->cinv = nullptr;
+    m->tail = m->tail0;
+    return HFLMP_OK;
+  }
+  HFLMP_REQUIRE(ctx, n <= kDenseMaxRows, "mg_set_coarse_inverse: coarsest level has %d rows "
+                                         "(at most %d)", n, kDenseMaxRows);
+  if (!m->cinv &&
+      hipMalloc(&m->cinv, sizeof(double) * (size_t)n * (size_t)n) != hipSuccess) {
+    m->cinv = nullptr;
+    return set_error(ctx, HFLMP_ENOMEM, "mg_set_coarse_inverse: out of device memory");
+  }
+  mg_transpose_kernel<<<grid_for((long long)n * n, 4), kBlock, 0, ctx->stream>>>(n, inv,
+                                                                                m->cinv);
+  HFLMP_LAUNCH_CHECK(ctx);
+  // the coarsest level is no longer swept: the one-workgroup tail stops
+  // above it
+  m->tail = m->nlevels;
+  return HFLMP_OK;
+}
 
 int hflmp_mg_apply_f64(hflmp_ctx *ctx, const hflmp_mg *m, const double *r, double *z) {
   if (!ctx) return HFLMP_EINVAL;

@@ -163,6 +163,7 @@ def hip_lib():
         L.hflmp_mg_create_f64.argtypes = [vp, i, vp, vp, vp, vp, vp]
         L.hflmp_mg_create_stencil_f64.argtypes = [vp, vp, i, vp, vp, vp, vp, vp]
         L.hflmp_mg_destroy.argtypes = [vp]
+        L.hflmp_mg_set_coarse_inverse_f64.argtypes = [vp, vp, vp]
         L.hflmp_mg_destroy.restype = None
         L.hflmp_mg_apply_f64.argtypes = [vp, vp, vp, vp]
         L.hflmp_mg_workspace_bytes.argtypes = [vp]
@@ -714,6 +715,19 @@ class Multigrid:
             _call(ctx, "hflmp_mg_create_f64", self.nlevels, a, r, p, C.byref(self.params),
                   C.byref(self.h))
         self.level_rows = [d.nrow for d in descs]
+
+    def set_coarse_inverse(self, C):
+        """Solve the coarsest level with the dense inverse C (host array,
+        n x n) instead of sweeps; None returns to the sweeps."""
+        if C is None:
+            _call(self.ctx, "hflmp_mg_set_coarse_inverse_f64", self.h, None)
+            return
+        d = self.ctx.upload(np.ascontiguousarray(C, np.float64))
+        try:
+            _call(self.ctx, "hflmp_mg_set_coarse_inverse_f64", self.h, d.ptr)
+            self.ctx.sync()
+        finally:
+            d.free()
 
     def apply(self, r, z):
         """z = M r (one V-cycle from zero); r and z device arrays."""

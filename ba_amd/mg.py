@@ -158,15 +158,27 @@ def coarse_cell_mats(dim, n, E, kappa, sizes):
     return out
 
 
+def dense_inverse(A):
+    """Dense inverse of a (small) device CsrMatrix, computed on the host."""
+    rp, ci, v = A.rp.download(), A.ci.download()[:A.nnz], A.val.download()[:A.nnz]
+    D = sp.csr_matrix((v, ci, rp), shape=(A.nrow, A.ncol)).toarray()
+    return np.linalg.inv(D)
+
+
 def build_structured(ctx, dim, n, E, kappa, A, nlevels=None, min_cells=2, stencil=None,
-                     **params):
+                     coarse="sweeps", **params):
     """Multigrid over the structured Q1 mesh of n^dim cells whose finest
     operator A (a device CsrMatrix) was assembled from kappa_c E with unit
     Dirichlet rows on the boundary, or is given matrix-free as `stencil`
     (a StencilOperator with local matrix E and cell factors kappa; A is
-    then None).  Returns an hflmp.Multigrid that owns the coarse
-    operators, R and P."""
+    then None).  coarse="dense" solves the coarsest level with its dense
+    inverse instead of sweeps.  Returns an hflmp.Multigrid that owns the
+    coarse operators, R and P."""
+    if coarse not in ("sweeps", "dense"):
+        raise ValueError("build_structured: coarse must be 'sweeps' or 'dense'")
     sizes = level_sizes(n, nlevels, min_cells)
+    if coarse == "dense" and len(sizes) < 2:
+        raise ValueError("build_structured: a dense coarse solve needs two levels or more")
     mats = coarse_cell_mats(dim, n, E, kappa, sizes)
     As, Rs, Ps = [A], [], []
     owned = []
@@ -179,7 +191,14 @@ def build_structured(ctx, dim, n, E, kappa, A, nlevels=None, min_cells=2, stenci
             owned += [Rs[-1], Ps[-1]]
             As.append(hflmp.assemble_q1(ctx, dim, nc, mats[l]))
             owned.append(As[-1])
-        return hflmp.Multigrid(ctx, As, Rs, Ps, owned=owned, stencil=stencil, **params)
+        M = hflmp.Multigrid(ctx, As, Rs, Ps, owned=owned, stencil=stencil, **params)
+        if coarse == "dense":
+            try:
+                M.set_coarse_inverse(dense_inverse(As[-1]))
+            except Exception:
+                M.free()
+                raise
+        return M
     except Exception:
         for m in owned:
             m.free()

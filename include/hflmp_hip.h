@@ -481,6 +481,13 @@ int hflmp_mg_create_stencil_f64(hflmp_ctx *ctx, const hflmp_stencil *S,
                                 const hflmp_csr_f64 *R, const hflmp_csr_f64 *P,
                                 const hflmp_mg_params *prm, hflmp_mg **out);
 void hflmp_mg_destroy(hflmp_mg *m);
+/* Coarsest level solved exactly: x = C b with C (DEVICE, n x n row-major,
+ * n <= 4096 rows of the coarsest level) its operator's inverse, each row
+ * summed from 0 in column order, instead of coarse_sweeps sweeps (the
+ * reference's GMG takes any LinearSolver as its coarse solver cS_,
+ * gmg.h:1159).  C is copied; NULL returns to the sweeps. */
+int hflmp_mg_set_coarse_inverse_f64(hflmp_ctx *ctx, hflmp_mg *m,
+                                    const double *inv);
 /* z = M r (one V-cycle from z = 0) */
 int hflmp_mg_apply_f64(hflmp_ctx *ctx, const hflmp_mg *m, const double *r,
                        double *z);

@@ -518,14 +518,26 @@ def _lv_first(L, w, b):
     return w * (b * L["invd"]) if "S" in L else w * L["invd"] * b
 
 
+def dense_matvec(C, b):
+    """y = C b with every row summed from 0 in ascending column order (the
+    CSR row loop over a full pattern)."""
+    n = C.shape[0]
+    rp = np.arange(0, n * n + 1, n, dtype=np.int32)
+    ci = np.tile(np.arange(n, dtype=np.int32), n)
+    return spmv(rp, ci, np.ascontiguousarray(C, np.float64).ravel(), np.asarray(b, np.float64))
+
+
 def mg_cycle(levels, b, pre=1, post=1, coarse=4, omega=0.7, coarse_omega=0.7, l=0):
     """One V-cycle from zero (GMGBase::SolveImplLevel, gmg.h:1070-1187, with
     the GPU kernels of cuda_gmg_kernel.h).  levels[l] is a dict with the
     operator A=(rp, ci, v) or, matrix-free on level 0, S=(order, dim, nc,
-    E, factor, fixed) for stencil_vmadd; invd; and above the coarsest level
-    R and P as (rp, ci, v)."""
+    E, factor, fixed) for stencil_vmadd; invd; above the coarsest level R
+    and P as (rp, ci, v); on the coarsest level optionally Cinv, a dense
+    inverse that replaces its sweeps (x = Cinv b)."""
     L = levels[l]
     last = l == len(levels) - 1
+    if last and "Cinv" in L:
+        return dense_matvec(L["Cinv"], b)
     w = coarse_omega if last else omega
     x = _lv_first(L, w, b)
     for _ in range((coarse if last else pre) - 1):

@@ -248,3 +248,60 @@ def test_mg_matrix_free_refusals(ctx):
     finally:
         S32.free()
         S.free()
+
+
+@pytest.mark.parametrize("case", [(2, 32), (3, 16)])
+def test_vcycle_dense_coarse_bit_exact(ctx, case):
+    dim, n = case
+    print("vcycle_dense", case, flush=True)
+    A, dA, M, levels = _setup(ctx, dim, n, nlevels=3)
+    try:
+        Ac = levels[-1]["A"]
+        Nc = len(Ac[0]) - 1
+        import scipy.sparse as sp
+        Cinv = np.linalg.inv(sp.csr_matrix((Ac[2], Ac[1], Ac[0]), shape=(Nc, Nc)).toarray())
+        M.set_coarse_inverse(Cinv)
+        inner = np.ones(dA.nrow, bool)
+        inner[hflmp.fe_fixed(1, dim, n)] = False
+        r = np.random.default_rng(2).uniform(-1, 1, dA.nrow) * inner
+        dr, dz = ctx.upload(r), ctx.zeros(dA.nrow, np.float64)
+        M.apply(dr, dz)
+        lv = [dict(L) for L in levels]
+        lv[-1]["Cinv"] = Cinv
+        assert np.array_equal(dz.download(), O.mg_cycle(lv, r, **_oracle_kw(SMOOTH[0])))
+        # back to the sweeps
+        M.set_coarse_inverse(None)
+        M.apply(dr, dz)
+        assert np.array_equal(dz.download(), O.mg_cycle(levels, r, **_oracle_kw(SMOOTH[0])))
+        # CG with it, built by the hierarchy builder
+        b = hflmp.fe_rhs(1, dim, n, 5)
+        kap = hflmp.fe_kappa(dim, n, 1.0, 7)
+        M2 = mg.build_structured(ctx, dim, n, hflmp.fe_element(1, dim, n), kap, dA, nlevels=3,
+                                 coarse="dense", **SMOOTH[0])
+        try:
+            db, dx = ctx.upload(b), ctx.zeros(len(b), np.float64)
+            info = M2.pcg(db, dx, rtol=1e-8)
+            xo, io = O.cg_prec(A, b, lambda v: O.mg_cycle(lv, v, **_oracle_kw(SMOOTH[0])),
+                               rtol=1e-8)
+            assert info["status"] == 0 and abs(info["iters"] - io["iters"]) <= 2
+            assert np.linalg.norm(dx.download() - xo) <= 1e-8 * np.linalg.norm(xo)
+            db.free(), dx.free()
+        finally:
+            M2.free()
+        dr.free(), dz.free()
+    finally:
+        M.free()
+        dA.free()
+
+
+def test_dense_coarse_refusal(ctx):
+    rp, ci, v = hflmp.fe_system(1, 3, 16)
+    dA = hflmp.CsrMatrix(ctx, rp, ci, v)
+    M = hflmp.Multigrid(ctx, [dA], [], [])
+    try:
+        with pytest.raises(hflmp.HflmpError) as e:   # 4913 rows > 4096
+            M.set_coarse_inverse(np.eye(dA.nrow))
+        assert e.value.code == hflmp.EINVAL
+    finally:
+        M.free()
+        dA.free()

@@ -206,3 +206,18 @@ def test_matrix_free_level_agrees_with_assembled(dim, n):
     # the stencil's residual is b - A x
     r = O._lv_residual(mf[0], w, u)
     assert np.allclose(r, w - O.spmv(*A, u), rtol=0, atol=1e-12)
+
+
+def test_dense_coarse_solve():
+    rp, ci, v = O.fe_system(1, 2, 6, O.fe_kappa(2, 6, 1.0, 4))
+    N = len(rp) - 1
+    Cinv = np.linalg.inv(_sp(rp, ci, v).toarray())
+    b = O.fe_rhs(1, 2, 6, 9)
+    y = O.dense_matvec(Cinv, b)
+    assert np.allclose(y, Cinv @ b, rtol=0, atol=1e-14 * np.abs(Cinv @ b).max())
+    # an exact coarsest solve on a single level is the exact preconditioner
+    lv = [dict(A=(rp, ci, v), invd=O.invdiag(rp, ci, v), Cinv=Cinv)]
+    x, info = O.cg_prec((rp, ci, v), b, lambda r: O.mg_cycle(lv, r), rtol=1e-10)
+    assert info["iters"] <= 2 and info["status"] == 0
+    assert np.allclose(_sp(rp, ci, v) @ x, b, rtol=0, atol=1e-10 * np.abs(b).max())
+    assert N == Cinv.shape[0]
