@@ -341,23 +341,120 @@ __global__ void __launch_bounds__(kBlock)
 
 }  // namespace
 
+// ---- transfers of the structured Q1 hierarchy without stored entries -------
+// For the hierarchy of halved structured Q1 meshes (nodes lexicographic, x
+// fastest), R and P are the Q1 interpolation with Dirichlet rows and
+// columns removed.  Their entries are products of 1 and 1/2, exact in any
+// order, so summing the same terms in the CSR rows' order (ascending
+// column index: z, then y, then x) gives results bit-identical to the
+// SELL-64 products over R and P, while reading only the vectors.
+
+// bc = R r: coarse node (I,J,K) interior sums its 3^dim fine neighbours
+template <int DIM>
+__global__ void __launch_bounds__(kBlock)
+    mg_restrict_q1_kernel(int pf, int pc, const double *__restrict__ r,
+                          double *__restrict__ bc, const int *status) {
+#pragma clang fp contract(off)
+  if (stopped(status)) return;
+  const long long nc = DIM == 3 ? (long long)pc * pc * pc : (long long)pc * pc;
+  for (long long c = (long long)blockIdx.x * kBlock + threadIdx.x; c < nc;
+       c += (long long)gridDim.x * kBlock) {
+    const int I = (int)(c % pc);
+    const long long t = c / pc;
+    const int J = (int)(t % pc), K = DIM == 3 ? (int)(t / pc) : 0;
+    const bool bnd = I == 0 || I == pc - 1 || J == 0 || J == pc - 1 ||
+                     (DIM == 3 && (K == 0 || K == pc - 1));
+    double s = 0;
+    if (!bnd) {
+      const long long f0 = 2LL * I + (long long)pf * (2LL * J + (long long)pf * 2LL * K);
+#pragma unroll
+      for (int dz = (DIM == 3 ? -1 : 0); dz <= (DIM == 3 ? 1 : 0); ++dz)
+#pragma unroll
+        for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+          for (int dx = -1; dx <= 1; ++dx) {
+            const double w = (dz ? 0.5 : 1.0) * ((dy ? 0.5 : 1.0) * (dx ? 0.5 : 1.0));
+            s += w * r[f0 + dx + (long long)pf * (dy + (long long)pf * dz)];
+          }
+    }
+    bc[c] = s;
+  }
+}
+
+// x = x + P xc: fine node (i,j,k) interior adds its interior coarse parents
+template <int DIM>
+__global__ void __launch_bounds__(kBlock)
+    mg_prolong_q1_kernel(int pf, int pc, const double *__restrict__ xc, double *__restrict__ x,
+                         const int *status) {
+#pragma clang fp contract(off)
+  if (stopped(status)) return;
+  const long long nf = DIM == 3 ? (long long)pf * pf * pf : (long long)pf * pf;
+  for (long long f = (long long)blockIdx.x * kBlock + threadIdx.x; f < nf;
+       f += (long long)gridDim.x * kBlock) {
+    const int i = (int)(f % pf);
+    const long long t = f / pf;
+    const int j = (int)(t % pf), k = DIM == 3 ? (int)(t / pf) : 0;
+    if (i == 0 || i == pf - 1 || j == 0 || j == pf - 1 || (DIM == 3 && (k == 0 || k == pf - 1)))
+      continue;
+    // parents per direction: g even -> g/2 (weight 1), odd -> (g-1)/2 and
+    // (g+1)/2 (1/2 each); parents on the coarse boundary have no column
+    const int nzp = DIM == 3 ? 1 + (k & 1) : 1, nyp = 1 + (j & 1), nxp = 1 + (i & 1);
+    const int kz0 = DIM == 3 ? k >> 1 : 0, jy0 = j >> 1, ix0 = i >> 1;
+    const double wz = DIM == 3 && (k & 1) ? 0.5 : 1.0, wy = (j & 1) ? 0.5 : 1.0,
+                 wx = (i & 1) ? 0.5 : 1.0;
+    double s = x[f];
+    for (int a = 0; a < nzp; ++a) {
+      const int K = kz0 + a;
+      if (DIM == 3 && (K == 0 || K == pc - 1)) continue;
+      for (int bq = 0; bq < nyp; ++bq) {
+        const int J = jy0 + bq;
+        if (J == 0 || J == pc - 1) continue;
+        for (int cq = 0; cq < nxp; ++cq) {
+          const int I = ix0 + cq;
+          if (I == 0 || I == pc - 1) continue;
+          const double w = (DIM == 3 ? wz : 1.0) * (wy * wx);
+          s += w * xc[I + (long long)pc * (J + (long long)pc * K)];
+        }
+      }
+    }
+    x[f] = s;
+  }
+}
+
 // ---- a dense inverse on the coarsest level ---------------------------------
-// x = C b with C stored by columns (ct[j n + i] = C_ij): lane i sums row i
-// from 0 in ascending j, the loads of a wave contiguous.
-constexpr int kDenseBlock = 64;
+// x = C b with C stored by columns (ct[j n + i] = C_ij).  The columns are
+// cut into kDenseParts blocks of ceil(n / kDenseParts); wave k of a
+// workgroup sums block k of 64 rows from 0 in ascending j (lane = row, so
+// its loads are contiguous), and the row's sum is p_0 + p_1 + ... + p_7 in
+// that order.  Eight waves per 64 rows keep enough loads in flight for a
+// matrix read once per cycle.
+constexpr int kDenseParts = 8;
+constexpr int kDenseBlock = kDenseParts * kWave;
 constexpr int kDenseMaxRows = 4096;
 
 __global__ void __launch_bounds__(kDenseBlock)
     mg_dense_kernel(int n, const double *__restrict__ ct, const double *__restrict__ b,
                     double *__restrict__ x, const int *status) {
 #pragma clang fp contract(off)
+  __shared__ double part[kDenseParts][kWave];
   if (stopped(status)) return;
-  const int i = blockIdx.x * kDenseBlock + threadIdx.x;
-  if (i >= n) return;
+  const int lane = threadIdx.x & (kWave - 1), k = threadIdx.x / kWave;
+  const int i = blockIdx.x * kWave + lane;
+  const int cb = (n + kDenseParts - 1) / kDenseParts;
+  const int j0 = k * cb, j1 = j0 + cb < n ? j0 + cb : n;
   double s = 0;
+  if (i < n) {
 #pragma unroll 8
-  for (int j = 0; j < n; ++j) s += ct[(size_t)j * n + i] * b[j];
-  x[i] = s;
+    for (int j = j0; j < j1; ++j) s += ct[(size_t)j * n + i] * b[j];
+  }
+  part[k][lane] = s;
+  __syncthreads();
+  if (k == 0 && i < n) {
+    double t = part[0][lane];
+#pragma unroll
+    for (int q = 1; q < kDenseParts; ++q) t = t + part[q][lane];
+    x[i] = t;
+  }
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -511,6 +608,8 @@ struct hflmp_mg {
   int tail0 = 0;  // the tail chosen at creation
   MgLevelDev *dlv = nullptr;  // device copy of the level table
   double *cinv = nullptr;     // coarsest level's inverse by columns, or null
+  int sdim = 0;               // structured Q1 transfers: dimension, or 0
+  int scells[kMaxLevels] = {};  // cells per direction of each level
 };
 
 namespace {
@@ -610,8 +709,8 @@ double *cycle(hipStream_t s, const hflmp_mg *m, int l, const double *b, double *
     y = (swaps & 1) ? out : L.x2;
   }
   if (dense) {
-    mg_dense_kernel<<<(L.n + kDenseBlock - 1) / kDenseBlock, kDenseBlock, 0, s>>>(L.n, m->cinv,
-                                                                                b, x, status);
+    mg_dense_kernel<<<(L.n + kWave - 1) / kWave, kDenseBlock, 0, s>>>(L.n, m->cinv, b, x,
+                                                                       status);
     return x;
   }
   smooth_first(s, m, l, w, b, x, status);
@@ -622,10 +721,21 @@ double *cycle(hipStream_t s, const hflmp_mg *m, int l, const double *b, double *
   if (coarsest) return x;
   const auto &C = m->lv[l + 1];
   residual(s, m, l, b, x, L.r, status);
-  mg_apply_kernel<false><<<sell_grid(C.n), kBlock, 0, s>>>(sell_view<double, int>(L.R), L.r,
-                                                            C.b, status);
+  const int pf = m->scells[l] + 1, pc = m->scells[l + 1] + 1;
+  if (m->sdim == 3)
+    mg_restrict_q1_kernel<3><<<grid_for(C.n, 1), kBlock, 0, s>>>(pf, pc, L.r, C.b, status);
+  else if (m->sdim == 2)
+    mg_restrict_q1_kernel<2><<<grid_for(C.n, 1), kBlock, 0, s>>>(pf, pc, L.r, C.b, status);
+  else
+    mg_apply_kernel<false><<<sell_grid(C.n), kBlock, 0, s>>>(sell_view<double, int>(L.R), L.r,
+                                                              C.b, status);
   const double *xc = cycle(s, m, l + 1, C.b, nullptr, status);
-  mg_apply_kernel<true><<<ga, kBlock, 0, s>>>(sell_view<double, int>(L.P), xc, x, status);
+  if (m->sdim == 3)
+    mg_prolong_q1_kernel<3><<<grid_for(L.n, 1), kBlock, 0, s>>>(pf, pc, xc, x, status);
+  else if (m->sdim == 2)
+    mg_prolong_q1_kernel<2><<<grid_for(L.n, 1), kBlock, 0, s>>>(pf, pc, xc, x, status);
+  else
+    mg_apply_kernel<true><<<ga, kBlock, 0, s>>>(sell_view<double, int>(L.P), xc, x, status);
   for (int k = 0; k < m->prm.post_smooth; ++k) {
     smooth_sweep(s, m, l, w, b, x, y, status);
     std::swap(x, y);
@@ -799,6 +909,30 @@ int hflmp_mg_create_stencil_f64(hflmp_ctx *ctx, const hflmp_stencil *S, int nlev
 }
 
 void hflmp_mg_destroy(hflmp_mg *m) { mg_free(m); }
+
+int hflmp_mg_set_structured_q1(hflmp_ctx *ctx, hflmp_mg *m, int dim, int ncells) {
+  if (!ctx) return HFLMP_EINVAL;
+  HFLMP_REQUIRE(ctx, m, "mg_set_structured_q1: null hierarchy");
+  if (dim == 0) {
+    m->sdim = 0;
+    return HFLMP_OK;
+  }
+  HFLMP_REQUIRE(ctx, dim == 2 || dim == 3, "mg_set_structured_q1: dim must be 2 or 3");
+  int nc = ncells;
+  for (int l = 0; l < m->nlevels; ++l) {
+    HFLMP_REQUIRE(ctx, nc >= 1 && (l + 1 == m->nlevels || nc % 2 == 0),
+                  "mg_set_structured_q1: level %d has %d cells per direction", l, nc);
+    long long nodes = 1;
+    for (int d = 0; d < dim; ++d) nodes *= (long long)(nc + 1);
+    HFLMP_REQUIRE(ctx, nodes == m->lv[l].n,
+                  "mg_set_structured_q1: level %d has %d rows, not (%d+1)^%d", l, m->lv[l].n, nc,
+                  dim);
+    m->scells[l] = nc;
+    nc /= 2;
+  }
+  m->sdim = dim;
+  return HFLMP_OK;
+}
 
 int hflmp_mg_set_coarse_inverse_f64(hflmp_ctx *ctx, hflmp_mg *m, const double *inv) {
   if (!ctx) return HFLMP_EINVAL;

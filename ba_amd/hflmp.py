@@ -164,6 +164,7 @@ def hip_lib():
         L.hflmp_mg_create_stencil_f64.argtypes = [vp, vp, i, vp, vp, vp, vp, vp]
         L.hflmp_mg_destroy.argtypes = [vp]
         L.hflmp_mg_set_coarse_inverse_f64.argtypes = [vp, vp, vp]
+        L.hflmp_mg_set_structured_q1.argtypes = [vp, vp, i, i]
         L.hflmp_mg_destroy.restype = None
         L.hflmp_mg_apply_f64.argtypes = [vp, vp, vp, vp]
         L.hflmp_mg_workspace_bytes.argtypes = [vp]
@@ -715,6 +716,11 @@ class Multigrid:
             _call(ctx, "hflmp_mg_create_f64", self.nlevels, a, r, p, C.byref(self.params),
                   C.byref(self.h))
         self.level_rows = [d.nrow for d in descs]
+
+    def set_structured(self, dim, ncells):
+        """Transfers of the structured Q1 hierarchy without reading R and P
+        (dim 0: through R and P again)."""
+        _call(self.ctx, "hflmp_mg_set_structured_q1", self.h, dim, ncells)
 
     def set_coarse_inverse(self, C):
         """Solve the coarsest level with the dense inverse C (host array,

@@ -166,14 +166,15 @@ def dense_inverse(A):
 
 
 def build_structured(ctx, dim, n, E, kappa, A, nlevels=None, min_cells=2, stencil=None,
-                     coarse="sweeps", **params):
+                     coarse="sweeps", structured_transfers=True, **params):
     """Multigrid over the structured Q1 mesh of n^dim cells whose finest
     operator A (a device CsrMatrix) was assembled from kappa_c E with unit
     Dirichlet rows on the boundary, or is given matrix-free as `stencil`
     (a StencilOperator with local matrix E and cell factors kappa; A is
     then None).  coarse="dense" solves the coarsest level with its dense
-    inverse instead of sweeps.  Returns an hflmp.Multigrid that owns the
-    coarse operators, R and P."""
+    inverse instead of sweeps; structured_transfers restricts and
+    prolongates without reading R and P (same results).  Returns an
+    hflmp.Multigrid that owns the coarse operators, R and P."""
     if coarse not in ("sweeps", "dense"):
         raise ValueError("build_structured: coarse must be 'sweeps' or 'dense'")
     sizes = level_sizes(n, nlevels, min_cells)
@@ -192,12 +193,14 @@ def build_structured(ctx, dim, n, E, kappa, A, nlevels=None, min_cells=2, stenci
             As.append(hflmp.assemble_q1(ctx, dim, nc, mats[l]))
             owned.append(As[-1])
         M = hflmp.Multigrid(ctx, As, Rs, Ps, owned=owned, stencil=stencil, **params)
-        if coarse == "dense":
-            try:
+        try:
+            if coarse == "dense":
                 M.set_coarse_inverse(dense_inverse(As[-1]))
-            except Exception:
-                M.free()
-                raise
+            if structured_transfers:
+                M.set_structured(dim, n)
+        except Exception:
+            M.free()
+            raise
         return M
     except Exception:
         for m in owned:

@@ -518,13 +518,28 @@ def _lv_first(L, w, b):
     return w * (b * L["invd"]) if "S" in L else w * L["invd"] * b
 
 
-def dense_matvec(C, b):
-    """y = C b with every row summed from 0 in ascending column order (the
-    CSR row loop over a full pattern)."""
+def dense_matvec(C, b, parts=8):
+    """y = C b as the dense coarse solve computes it: the columns cut into
+    `parts` blocks of ceil(n / parts), each block's part of a row summed from
+    0 in ascending column order (the CSR row loop over the block), and the
+    row's sum p_0 + p_1 + ... in block order (an empty block adds 0)."""
+    C = np.ascontiguousarray(C, np.float64)
+    b = np.asarray(b, np.float64)
     n = C.shape[0]
-    rp = np.arange(0, n * n + 1, n, dtype=np.int32)
-    ci = np.tile(np.arange(n, dtype=np.int32), n)
-    return spmv(rp, ci, np.ascontiguousarray(C, np.float64).ravel(), np.asarray(b, np.float64))
+    cb = (n + parts - 1) // parts
+    y = None
+    for k in range(parts):
+        j0, j1 = min(k * cb, n), min((k + 1) * cb, n)
+        w = j1 - j0
+        if w > 0:
+            rp = np.arange(0, n * w + 1, w, dtype=np.int32)
+            ci = np.tile(np.arange(w, dtype=np.int32), n)
+            p = spmv(rp, ci, np.ascontiguousarray(C[:, j0:j1]).ravel(),
+                     np.ascontiguousarray(b[j0:j1]))
+        else:
+            p = np.zeros(n)
+        y = p if y is None else y + p
+    return y
 
 
 def mg_cycle(levels, b, pre=1, post=1, coarse=4, omega=0.7, coarse_omega=0.7, l=0):

@@ -52,6 +52,9 @@ def test_vcycle_bit_exact(ctx, case, p):
         assert np.array_equal(z, O.mg_cycle(levels, r, **_oracle_kw(SMOOTH[p])))
         M.apply(dr, dz)     # the level buffers keep no state between cycles
         assert np.array_equal(dz.download(), z)
+        M.set_structured(0, 0)   # through the stored R and P: the same cycle
+        M.apply(dr, dz)
+        assert np.array_equal(dz.download(), z)
         dr.free(), dz.free()
     finally:
         M.free()
@@ -147,6 +150,10 @@ def test_mg_refusals(ctx):
             hflmp.Multigrid(ctx, [dA, dA], [R], [P])
         M = hflmp.Multigrid(ctx, [dA, Ac], [R], [P])
         try:
+            for dim, nc in ((3, 8), (2, 6), (2, 3), (4, 8)):   # not this hierarchy
+                with pytest.raises(hflmp.HflmpError):
+                    M.set_structured(dim, nc)
+            M.set_structured(2, 8)
             r = ctx.zeros(dA.nrow, np.float64)
             with pytest.raises(hflmp.HflmpError):  # z aliasing r
                 M.apply(r, r)
@@ -302,6 +309,27 @@ def test_dense_coarse_refusal(ctx):
         with pytest.raises(hflmp.HflmpError) as e:   # 4913 rows > 4096
             M.set_coarse_inverse(np.eye(dA.nrow))
         assert e.value.code == hflmp.EINVAL
+    finally:
+        M.free()
+        dA.free()
+
+
+def test_single_level_jacobi_solver(ctx):
+    """One level, k sweeps per residual check: the synchronous form of the
+    reference's AsynchronousIterationGPU (damped, n_inner = k) on one GPU."""
+    rp, ci, v = hflmp.fe_system(1, 2, 24, hflmp.fe_kappa(2, 24, 0.5, 3))
+    dA = hflmp.CsrMatrix(ctx, rp, ci, v)
+    M = hflmp.Multigrid(ctx, [dA], [], [], coarse_sweeps=5, coarse_omega=0.7)
+    try:
+        b = hflmp.fe_rhs(1, 2, 24, 8)
+        db, dx = ctx.upload(b), ctx.zeros(len(b), np.float64)
+        info = M.solve(db, dx, rtol=1e-6, maxit=5000)
+        lv = [dict(A=(rp, ci, v), invd=O.invdiag(rp, ci, v))]
+        xo, io = O.mg_solve(lv, b, rtol=1e-6, maxit=5000, coarse=5, coarse_omega=0.7)
+        assert info["status"] == 0 and io["status"] == 0
+        assert abs(info["iters"] - io["iters"]) <= 1
+        assert np.linalg.norm(dx.download() - xo) <= 1e-8 * np.linalg.norm(xo)
+        db.free(), dx.free()
     finally:
         M.free()
         dA.free()
