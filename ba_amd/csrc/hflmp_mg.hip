@@ -349,24 +349,26 @@ __global__ void __launch_bounds__(kBlock)
 // column index: z, then y, then x) gives results bit-identical to the
 // SELL-64 products over R and P, while reading only the vectors.
 
-// bc = R r: coarse node (I,J,K) interior sums its 3^dim fine neighbours
+// bc = R r: coarse node (I,J,K) interior sums its 3^dim fine neighbours.
+// Node counts stay below 2^31 (checked when the hierarchy is set), so the
+// index arithmetic is 32-bit with multiply-shift division by pf and pc.
 template <int DIM>
 __global__ void __launch_bounds__(kBlock)
-    mg_restrict_q1_kernel(int pf, int pc, const double *__restrict__ r,
+    mg_restrict_q1_kernel(int pf, int pc, DivMagic dpc, const double *__restrict__ r,
                           double *__restrict__ bc, const int *status) {
 #pragma clang fp contract(off)
   if (stopped(status)) return;
-  const long long nc = DIM == 3 ? (long long)pc * pc * pc : (long long)pc * pc;
-  for (long long c = (long long)blockIdx.x * kBlock + threadIdx.x; c < nc;
-       c += (long long)gridDim.x * kBlock) {
-    const int I = (int)(c % pc);
-    const long long t = c / pc;
-    const int J = (int)(t % pc), K = DIM == 3 ? (int)(t / pc) : 0;
+  const int nc = DIM == 3 ? pc * pc * pc : pc * pc;
+  for (int c = blockIdx.x * kBlock + threadIdx.x; c < nc; c += gridDim.x * kBlock) {
+    const int t = dpc.div(c);
+    const int I = c - t * pc;
+    const int K = DIM == 3 ? dpc.div(t) : 0;
+    const int J = t - K * pc;
     const bool bnd = I == 0 || I == pc - 1 || J == 0 || J == pc - 1 ||
                      (DIM == 3 && (K == 0 || K == pc - 1));
     double s = 0;
     if (!bnd) {
-      const long long f0 = 2LL * I + (long long)pf * (2LL * J + (long long)pf * 2LL * K);
+      const int f0 = 2 * I + pf * (2 * J + pf * 2 * K);
 #pragma unroll
       for (int dz = (DIM == 3 ? -1 : 0); dz <= (DIM == 3 ? 1 : 0); ++dz)
 #pragma unroll
@@ -374,7 +376,7 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
           for (int dx = -1; dx <= 1; ++dx) {
             const double w = (dz ? 0.5 : 1.0) * ((dy ? 0.5 : 1.0) * (dx ? 0.5 : 1.0));
-            s += w * r[f0 + dx + (long long)pf * (dy + (long long)pf * dz)];
+            s += w * r[f0 + dx + pf * (dy + pf * dz)];
           }
     }
     bc[c] = s;
@@ -384,16 +386,16 @@ __global__ void __launch_bounds__(kBlock)
 // x = x + P xc: fine node (i,j,k) interior adds its interior coarse parents
 template <int DIM>
 __global__ void __launch_bounds__(kBlock)
-    mg_prolong_q1_kernel(int pf, int pc, const double *__restrict__ xc, double *__restrict__ x,
-                         const int *status) {
+    mg_prolong_q1_kernel(int pf, int pc, DivMagic dpf, const double *__restrict__ xc,
+                         double *__restrict__ x, const int *status) {
 #pragma clang fp contract(off)
   if (stopped(status)) return;
-  const long long nf = DIM == 3 ? (long long)pf * pf * pf : (long long)pf * pf;
-  for (long long f = (long long)blockIdx.x * kBlock + threadIdx.x; f < nf;
-       f += (long long)gridDim.x * kBlock) {
-    const int i = (int)(f % pf);
-    const long long t = f / pf;
-    const int j = (int)(t % pf), k = DIM == 3 ? (int)(t / pf) : 0;
+  const int nf = DIM == 3 ? pf * pf * pf : pf * pf;
+  for (int f = blockIdx.x * kBlock + threadIdx.x; f < nf; f += gridDim.x * kBlock) {
+    const int t = dpf.div(f);
+    const int i = f - t * pf;
+    const int k = DIM == 3 ? dpf.div(t) : 0;
+    const int j = t - k * pf;
     if (i == 0 || i == pf - 1 || j == 0 || j == pf - 1 || (DIM == 3 && (k == 0 || k == pf - 1)))
       continue;
     // parents per direction: g even -> g/2 (weight 1), odd -> (g-1)/2 and
@@ -413,7 +415,7 @@ __global__ void __launch_bounds__(kBlock)
           const int I = ix0 + cq;
           if (I == 0 || I == pc - 1) continue;
           const double w = (DIM == 3 ? wz : 1.0) * (wy * wx);
-          s += w * xc[I + (long long)pc * (J + (long long)pc * K)];
+          s += w * xc[I + pc * (J + pc * K)];
         }
       }
     }
@@ -722,18 +724,21 @@ double *cycle(hipStream_t s, const hflmp_mg *m, int l, const double *b, double *
   const auto &C = m->lv[l + 1];
   residual(s, m, l, b, x, L.r, status);
   const int pf = m->scells[l] + 1, pc = m->scells[l + 1] + 1;
+  DivMagic dpf, dpc;
+  dpf.init((unsigned)pf);
+  dpc.init((unsigned)pc);
   if (m->sdim == 3)
-    mg_restrict_q1_kernel<3><<<grid_for(C.n, 1), kBlock, 0, s>>>(pf, pc, L.r, C.b, status);
+    mg_restrict_q1_kernel<3><<<grid_for(C.n, 1), kBlock, 0, s>>>(pf, pc, dpc, L.r, C.b, status);
   else if (m->sdim == 2)
-    mg_restrict_q1_kernel<2><<<grid_for(C.n, 1), kBlock, 0, s>>>(pf, pc, L.r, C.b, status);
+    mg_restrict_q1_kernel<2><<<grid_for(C.n, 1), kBlock, 0, s>>>(pf, pc, dpc, L.r, C.b, status);
   else
     mg_apply_kernel<false><<<sell_grid(C.n), kBlock, 0, s>>>(sell_view<double, int>(L.R), L.r,
                                                               C.b, status);
   const double *xc = cycle(s, m, l + 1, C.b, nullptr, status);
   if (m->sdim == 3)
-    mg_prolong_q1_kernel<3><<<grid_for(L.n, 1), kBlock, 0, s>>>(pf, pc, xc, x, status);
+    mg_prolong_q1_kernel<3><<<grid_for(L.n, 1), kBlock, 0, s>>>(pf, pc, dpf, xc, x, status);
   else if (m->sdim == 2)
-    mg_prolong_q1_kernel<2><<<grid_for(L.n, 1), kBlock, 0, s>>>(pf, pc, xc, x, status);
+    mg_prolong_q1_kernel<2><<<grid_for(L.n, 1), kBlock, 0, s>>>(pf, pc, dpf, xc, x, status);
   else
     mg_apply_kernel<true><<<ga, kBlock, 0, s>>>(sell_view<double, int>(L.P), xc, x, status);
   for (int k = 0; k < m->prm.post_smooth; ++k) {
