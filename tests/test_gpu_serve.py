@@ -215,3 +215,28 @@ def test_serve_precision_window(ctx):
         assert info["status"] == 0
         # FP64 stops on the recursive residual, mixed on the true one
         assert np.linalg.norm(b - O.spmv(rp, ci, v, x)) < 1.1e-8 * np.linalg.norm(b)
+
+
+def test_serve_tune_small_sample(ctx):
+    """The autotuner on a small mixed sample returns a usable configuration
+    (window bounds on class boundaries, a candidate chunk size) and a
+    report of every timing it based its choices on."""
+    from ba_amd import serve_tune as T
+    sample = []
+    for dim, n, seed in ((2, 24, 1), (2, 40, 2), (3, 10, 3), (3, 14, 4)):
+        kap = hflmp.fe_kappa(dim, n, 1.0, seed)
+        rp, ci, v = hflmp.fe_system(1, dim, n, kap)
+        sample.append((rp, ci, v, hflmp.fe_rhs(1, dim, n, seed + 10)))
+    base = dict(hbm_budget=1 << 30, max_active=16, chunk_iters=16, use_graphs=True,
+                mixed_min_nnz=0, inner_rtol=1e-2, mixed_max_nnz=0)
+    cfg, rep = T.tune(ctx, sample, base, rtol=1e-8, chunks=(8, 16))
+    assert cfg["chunk_iters"] in (8, 16)
+    assert cfg["mixed_min_nnz"] in (0,) + tuple(max(b, 1) for b in T.CLASS_BOUNDS[:-1])
+    assert cfg["mixed_max_nnz"] in (0,) + T.CLASS_BOUNDS[1:-1]
+    assert set(rep["chunks"]) == {8, 16}
+    for k, c in rep["classes"].items():
+        assert c["choice"] in ("fp64", "mixed") and c["fp64_s"] > 0 and c["mixed_s"] > 0
+    if any(c["choice"] == "mixed" for c in rep["classes"].values()):
+        assert "window_check" in rep
+        if not rep["window_check"]["kept"]:
+            assert cfg["mixed_min_nnz"] == 0
