@@ -816,6 +816,21 @@ def main():
                                      args.serve_rate, not args.no_serve_reference,
                                      not args.serve_no_graphs)
 
+    # CG time to 1e-8 on the 128^3 Poisson system, every solver path side
+    # by side (each measured in its leg above)
+    tt = {}
+    if "cg" in out:
+        tt["jacobi_cg_sell64_ms"] = round(out["cg"]["sell"]["seconds_to_tol"] * 1e3, 3)
+        tt["jacobi_cg_csr_ms"] = round(out["cg"]["csr"]["seconds_to_tol"] * 1e3, 3)
+    if "matrix_free" in out:
+        tt["jacobi_cg_matrix_free_ms"] = round(out["matrix_free"]["cg"]["seconds_to_tol"] * 1e3, 3)
+    if "multigrid" in out:
+        mgp = out["multigrid"]["poisson"]
+        tt["mg_pcg_sell64_ms"] = round(mgp["seconds_to_tol"] * 1e3, 3)
+        tt["mg_pcg_matrix_free_ms"] = round(
+            mgp["matrix_free_fine_level"]["seconds_to_tol"] * 1e3, 3)
+    if tt:
+        out["cg_time_to_1e-8"] = tt
     if not args.no_cpu:
         base, naive = cpu_baseline(rp, ci, v, x, args.cpu_seconds)
         out["cpu_baseline"] = base
