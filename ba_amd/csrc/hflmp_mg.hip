@@ -115,14 +115,17 @@ __global__ void __launch_bounds__(kBlock)
     x[i] = w * (b[i] * invd[i]);
 }
 
+// (every thread of the workgroup calls it: x is staged through lds)
 template <class SOp>
 __device__ __forceinline__ double st_residual(const SOp &A, long long unit,
                                               const double *__restrict__ b,
-                                              const double *__restrict__ x, TileRow &tr) {
+                                              const double *__restrict__ x, TileRow &tr,
+                                              double *lds) {
 #pragma clang fp contract(off)
+  A.stage_x(unit, x, lds);
   const long long row = A.thread_row(unit);
   const double s = row < A.nrow ? b[row] * 1.0 + 0.0 : 0.0;
-  return A.row_sum_scaled(unit, x, s, -1.0, tr);
+  return A.template row_sum_scaled<true>(unit, x, s, -1.0, tr, lds);
 }
 
 // xo = xi + w ((b - A xi) .* invd)
@@ -132,10 +135,11 @@ __global__ void __launch_bounds__(kBlock)
                        const double *__restrict__ b, const double *__restrict__ xi,
                        double *__restrict__ xo, const int *status) {
 #pragma clang fp contract(off)
+  __shared__ double lds[SOp::kLds];
   if (stopped(status)) return;
   const long long unit = xcd_tile(blockIdx.x, gridDim.x);
   TileRow tr;
-  const double r = st_residual(A, unit, b, xi, tr);
+  const double r = st_residual(A, unit, b, xi, tr, lds);
   if (tr.has_row) xo[tr.row] = xi[tr.row] + w * (r * invd[tr.row]);
 }
 
@@ -146,10 +150,11 @@ __global__ void __launch_bounds__(kBlock)
                           double *__restrict__ r, double *__restrict__ partials,
                           const int *status) {
 #pragma clang fp contract(off)
+  __shared__ double lds[SOp::kLds];
   if (stopped(status)) return;
   const long long unit = xcd_tile(blockIdx.x, gridDim.x);
   TileRow tr;
-  const double ri = st_residual(A, unit, b, x, tr);
+  const double ri = st_residual(A, unit, b, x, tr, lds);
   double rr = 0;
   if (tr.has_row) {
     r[tr.row] = ri;
@@ -166,9 +171,10 @@ __global__ void __launch_bounds__(kBlock)
     mgcg_residual_kernel(AOp A, const double *__restrict__ b, const double *__restrict__ x,
                          double *__restrict__ r) {
 #pragma clang fp contract(off)
+  __shared__ double lds[AOp::kLds];
   const long long unit = xcd_tile(blockIdx.x, gridDim.x);
   TileRow tr;
-  const double s = A.row_sum(unit, x, 0.0, tr, nullptr);
+  const double s = A.row_sum(unit, x, 0.0, tr, lds);
   if (tr.has_row) r[tr.row] = b[tr.row] - s;
 }
 
@@ -205,10 +211,11 @@ template <class AOp>
 __global__ void __launch_bounds__(kBlock)
     mgcg_spmv_kernel(AOp A, const double *__restrict__ p, double *__restrict__ Ap,
                      double *__restrict__ partials, const int *status) {
+  __shared__ double lds[AOp::kLds];
   if (stopped(status)) return;
   const long long unit = xcd_tile(blockIdx.x, gridDim.x);
   TileRow tr;
-  const double s = A.row_sum(unit, p, 0.0, tr, nullptr);
+  const double s = A.row_sum(unit, p, 0.0, tr, lds);
   double part = 0;
   if (tr.has_row) {
     Ap[tr.row] = s;

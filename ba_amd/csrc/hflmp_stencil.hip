@@ -18,7 +18,9 @@ __global__ void __launch_bounds__(kBlock)
                    const typename Op::value_type *__restrict__ x,
                    typename Op::value_type beta, typename Op::value_type *__restrict__ y) {
   using T = typename Op::value_type;
+  __shared__ T xs[Op::kLds];
   const long long unit = xcd_tile(blockIdx.x, gridDim.x);
+  op.stage_x(unit, x, xs);
   T s = 0;
   if (MODE == kStScaleAdd) {
     const long long r = op.thread_row(unit);
@@ -28,7 +30,7 @@ __global__ void __launch_bounds__(kBlock)
     if (r < op.nrow) s = y[r] * beta + T(0);
   }
   TileRow tr;
-  s = op.row_sum_scaled(unit, x, s, MODE == kStScaleAdd ? alpha : T(1), tr);
+  s = op.template row_sum_scaled<true>(unit, x, s, MODE == kStScaleAdd ? alpha : T(1), tr, xs);
   if (tr.has_row) __builtin_nontemporal_store(s, y + tr.row);
 }
 

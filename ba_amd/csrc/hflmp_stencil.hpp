@@ -17,9 +17,11 @@
 // the same order, so y is bit-identical to the cell loop without atomics.
 //
 // Data per product: x and y (8 N B each in FP64), f (8 B per cell), a bit
-// per node for the fixed dofs; no indices and no matrix values.  The x
-// values of a node's (2 order + 1)^dim neighbourhood are loaded once into
-// registers; every other operand is read through the caches.
+// per node for the fixed dofs; no indices and no matrix values.  A
+// workgroup's 256 consecutive nodes read x along (2 order + 1)^(dim - 1)
+// neighbour lines; those line segments are staged in LDS with one load per
+// value, and each node takes its (2 order + 1)^dim neighbourhood from there
+// into registers.  Every other operand is read through the caches.
 #pragma once
 
 #include "hflmp_spmv_tile.hpp"
@@ -87,12 +89,15 @@ struct DivMagic {
 template <typename T, int DIM, int ORDER, bool MULTI>
 struct StencilOp {
   using value_type = T;
-  static constexpr int kLds = 1;
   static constexpr int Q = ORDER + 1;                        // local nodes per direction
   static constexpr int NLOC = DIM == 2 ? Q * Q : Q * Q * Q;  // local nodes per cell
   static constexpr int W = 2 * ORDER + 1;                    // neighbourhood width
   static constexpr int KZ = DIM == 3 ? 2 : 1;
   static constexpr int WZ = DIM == 3 ? W : 1;
+  // staged x: one segment per (dz, dy) neighbour line of the unit's 256
+  // consecutive nodes, ORDER values wider on each side
+  static constexpr int kSeg = kBlock + 2 * ORDER;
+  static constexpr int kLds = WZ * W * kSeg;
   int nrow;
   int ncx, ncy, ncz, px, py, pz;
   DivMagic dpx, dpy;  // division by px and py
@@ -105,11 +110,40 @@ struct StencilOp {
     return unit * kBlock + threadIdx.x;
   }
 
+  // Copies the x values the unit's nodes read into lds (kLds values): for
+  // each neighbour line (dz, dy), the index range of the unit shifted by
+  // that line's offset and widened by ORDER; indices outside the vector
+  // give 0, as the buffer loads do.  All threads of the workgroup call it.
+  __device__ __forceinline__ void stage_x(long long unit, const T *__restrict__ x, T *lds) const {
+    constexpr int kIt = (kLds + kBlock - 1) / kBlock;
+    const long long i0 = unit * kBlock;
+    const long long pxy = (long long)px * py;
+    T v[kIt];
+#pragma unroll
+    for (int k = 0; k < kIt; ++k) {
+      const int e = threadIdx.x + k * kBlock;
+      const int sg = e / kSeg, p = e - sg * kSeg;
+      const int dz = sg / W, dy = sg - dz * W;
+      const long long g = i0 + (DIM == 3 ? (dz - ORDER) * pxy : 0) +
+                          (long long)(dy - ORDER) * px + p - ORDER;
+      v[k] = (e < kLds && g >= 0 && g < nrow) ? x[g] : T(0);
+    }
+#pragma unroll
+    for (int k = 0; k < kIt; ++k) {
+      const int e = threadIdx.x + k * kBlock;
+      if (e < kLds) lds[e] = v[k];
+    }
+    __syncthreads();
+  }
+
   // s + sum over the node's cells (ascending) of 0 + (alpha f_c) (E_m x_c)[local],
   // or x[row] on a fixed node.  s must not be -0 (callers start from +0 or
-  // add +0 to a scaled y).
+  // add +0 to a scaled y).  STAGED: x is read from lds, which stage_x
+  // filled (the same values, a third of the vector loads).
+  template <bool STAGED = false>
   __device__ __forceinline__ T row_sum_scaled(long long unit, const T *__restrict__ x, T s,
-                                              T alpha, TileRow &tr) const {
+                                              T alpha, TileRow &tr,
+                                              const T *lds = nullptr) const {
 #pragma clang fp contract(off)
     const long long r = thread_row(unit);
     tr.has_row = r < nrow;
@@ -148,10 +182,14 @@ struct StencilOp {
             ib + (unsigned)(((DIM == 3 ? (dz - ORDER) * pxy : 0) + (dy - ORDER) * px - ORDER) *
                             (int)sizeof(T));
 #pragma unroll
-        for (int dx = 0; dx < W; ++dx)
-          xn[dz][dy][dx] = buffer_load<T>(
-              xr, dx < ORDER ? before_start(rb + dx * (unsigned)sizeof(T))
-                             : rb + dx * (unsigned)sizeof(T));
+        for (int dx = 0; dx < W; ++dx) {
+          if (STAGED)
+            xn[dz][dy][dx] = lds[(dz * W + dy) * kSeg + (int)(r - unit * kBlock) + dx];
+          else
+            xn[dz][dy][dx] = buffer_load<T>(
+                xr, dx < ORDER ? before_start(rb + dx * (unsigned)sizeof(T))
+                               : rb + dx * (unsigned)sizeof(T));
+        }
       }
     // the upper cell (k = 1 in every direction); the others sit at fixed
     // offsets from it
@@ -237,8 +275,12 @@ struct StencilOp {
     return s;
   }
 
-  __device__ __forceinline__ T row_sum(long long unit, const T *x, T s, TileRow &tr, T *) const {
-    return row_sum_scaled(unit, x, s, T(1), tr);
+  // with x staged through lds (kLds values; every thread of the workgroup
+  // must call it)
+  __device__ __forceinline__ T row_sum(long long unit, const T *x, T s, TileRow &tr,
+                                       T *lds) const {
+    stage_x(unit, x, lds);
+    return row_sum_scaled<true>(unit, x, s, T(1), tr, lds);
   }
 
 };
