@@ -122,7 +122,6 @@ __device__ __forceinline__ double st_residual(const SOp &A, long long unit,
                                               const double *__restrict__ x, TileRow &tr,
                                               double *lds) {
 #pragma clang fp contract(off)
-  A.stage_x(unit, x, lds);
   const long long row = A.thread_row(unit);
   const double s = row < A.nrow ? b[row] * 1.0 + 0.0 : 0.0;
   return A.template row_sum_scaled<true>(unit, x, s, -1.0, tr, lds);

@@ -20,7 +20,6 @@ __global__ void __launch_bounds__(kBlock)
   using T = typename Op::value_type;
   __shared__ T xs[Op::kLds];
   const long long unit = xcd_tile(blockIdx.x, gridDim.x);
-  op.stage_x(unit, x, xs);
   T s = 0;
   if (MODE == kStScaleAdd) {
     const long long r = op.thread_row(unit);

@@ -110,46 +110,71 @@ struct StencilOp {
     return unit * kBlock + threadIdx.x;
   }
 
-  // Copies the x values the unit's nodes read into lds (kLds values): for
-  // each neighbour line (dz, dy), the index range of the unit shifted by
-  // that line's offset and widened by ORDER; indices outside the vector
-  // give 0, as the buffer loads do.  All threads of the workgroup call it.
-  __device__ __forceinline__ void stage_x(long long unit, const T *__restrict__ x, T *lds) const {
-    constexpr int kIt = (kLds + kBlock - 1) / kBlock;
-    const long long i0 = unit * kBlock;
-    const long long pxy = (long long)px * py;
-    T v[kIt];
+  // The x values the unit's nodes read, for lds (kLds values): for each
+  // neighbour line (dz, dy), the index range of the unit shifted by that
+  // line's offset and widened by ORDER; indices outside the vector give 0,
+  // as the buffer loads do.  stage_load issues the loads, stage_store
+  // writes lds and waits for the workgroup.
+  // One register per line (all lanes: the unit's 256 nodes) and one for
+  // the 2 ORDER extra values of every line (lanes 0 .. lines * 2 ORDER - 1).
+  static constexpr int kLines = WZ * W;
+  static constexpr int kStageIt = kLines + 1;
+  __device__ __forceinline__ int line_offset(int sg) const {
+    const int dz = sg / W, dy = sg - dz * W;
+    return (DIM == 3 ? (dz - ORDER) * px * py : 0) + (dy - ORDER) * px - ORDER;
+  }
+  // The loads go through a buffer descriptor over x: an index before the
+  // start (negative, so a byte offset beyond 2^31 as unsigned) or past the
+  // end reads 0 with no compare.  A lane's loads are whole lines apart, so
+  // none is fused with a neighbour (the case before_start guards).
+  __device__ __forceinline__ void stage_load(long long unit, const T *__restrict__ x,
+                                             T (&v)[kStageIt]) const {
+    const __amdgpu_buffer_rsrc_t xr = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<T *>(x), 0, nrow * (int)sizeof(T), kRsrcWord3);
+    // 32-bit index arithmetic: nodes and offsets stay below 2^31
+    const int i0 = (int)unit * kBlock;
+    const int t = (int)threadIdx.x;
 #pragma unroll
-    for (int k = 0; k < kIt; ++k) {
-      const int e = threadIdx.x + k * kBlock;
-      const int sg = e / kSeg, p = e - sg * kSeg;
-      const int dz = sg / W, dy = sg - dz * W;
-      const long long g = i0 + (DIM == 3 ? (dz - ORDER) * pxy : 0) +
-                          (long long)(dy - ORDER) * px + p - ORDER;
-      v[k] = (e < kLds && g >= 0 && g < nrow) ? x[g] : T(0);
+    for (int sg = 0; sg < kLines; ++sg) {
+      const int g = i0 + line_offset(sg) + t;
+      v[sg] = buffer_load<T>(xr, (unsigned)g * (unsigned)sizeof(T));
     }
+    const int sg = t / (2 * ORDER), q = kBlock + t - sg * (2 * ORDER);
+    const int g = i0 + line_offset(sg) + q;
+    v[kLines] = t < kLines * 2 * ORDER
+                    ? buffer_load<T>(xr, (unsigned)g * (unsigned)sizeof(T))
+                    : T(0);
+  }
+  __device__ __forceinline__ void stage_store(const T (&v)[kStageIt], T *lds) const {
+    const int t = (int)threadIdx.x;
 #pragma unroll
-    for (int k = 0; k < kIt; ++k) {
-      const int e = threadIdx.x + k * kBlock;
-      if (e < kLds) lds[e] = v[k];
+    for (int sg = 0; sg < kLines; ++sg) lds[sg * kSeg + t] = v[sg];
+    if (t < kLines * 2 * ORDER) {
+      const int sg = t / (2 * ORDER), q = kBlock + t - sg * (2 * ORDER);
+      lds[sg * kSeg + q] = v[kLines];
     }
     __syncthreads();
   }
 
   // s + sum over the node's cells (ascending) of 0 + (alpha f_c) (E_m x_c)[local],
   // or x[row] on a fixed node.  s must not be -0 (callers start from +0 or
-  // add +0 to a scaled y).  STAGED: x is read from lds, which stage_x
-  // filled (the same values, a third of the vector loads).
+  // add +0 to a scaled y).  STAGED: the workgroup stages x in lds (kLds
+  // values; every thread must call) and reads it from there: the same
+  // values with a third of the vector loads, issued together with the
+  // node's factor loads.
   template <bool STAGED = false>
   __device__ __forceinline__ T row_sum_scaled(long long unit, const T *__restrict__ x, T s,
-                                              T alpha, TileRow &tr,
-                                              const T *lds = nullptr) const {
+                                              T alpha, TileRow &tr, T *lds = nullptr) const {
 #pragma clang fp contract(off)
     const long long r = thread_row(unit);
     tr.has_row = r < nrow;
     tr.row = tr.has_row ? (int)r : 0;
-    if (!tr.has_row) return s;
-    const int i = (int)r;
+    T sv[kStageIt];
+    if (STAGED) stage_load(unit, x, sv);
+    else if (!tr.has_row) return s;
+    // (staged: a thread past the last row computes node 0's indices and
+    // loads, and returns after the workgroup's barrier)
+    const int i = tr.row;
     const int t = dpx.div(i);
     const int gx = i - t * px;
     const int gz = DIM == 3 ? dpy.div(t) : 0;
@@ -174,23 +199,21 @@ struct StencilOp {
     T fc[KZ][2][2];
     int mc[KZ][2][2];
     const unsigned ib = (unsigned)i * (unsigned)sizeof(T);
+    if (!STAGED) {
 #pragma unroll
-    for (int dz = 0; dz < WZ; ++dz)
+      for (int dz = 0; dz < WZ; ++dz)
 #pragma unroll
-      for (int dy = 0; dy < W; ++dy) {
-        const unsigned rb =
-            ib + (unsigned)(((DIM == 3 ? (dz - ORDER) * pxy : 0) + (dy - ORDER) * px - ORDER) *
-                            (int)sizeof(T));
+        for (int dy = 0; dy < W; ++dy) {
+          const unsigned rb =
+              ib + (unsigned)(((DIM == 3 ? (dz - ORDER) * pxy : 0) + (dy - ORDER) * px - ORDER) *
+                              (int)sizeof(T));
 #pragma unroll
-        for (int dx = 0; dx < W; ++dx) {
-          if (STAGED)
-            xn[dz][dy][dx] = lds[(dz * W + dy) * kSeg + (int)(r - unit * kBlock) + dx];
-          else
+          for (int dx = 0; dx < W; ++dx)
             xn[dz][dy][dx] = buffer_load<T>(
                 xr, dx < ORDER ? before_start(rb + dx * (unsigned)sizeof(T))
                                : rb + dx * (unsigned)sizeof(T));
         }
-      }
+    }
     // the upper cell (k = 1 in every direction); the others sit at fixed
     // offsets from it
     const int cu = (bx + 1) + ncx * ((by + 1) + ncy * (DIM == 3 ? bz + 1 : 0));
@@ -214,6 +237,17 @@ struct StencilOp {
         }
       }
     const unsigned fixed_word = fixed[i >> 5];
+    if (STAGED) {
+      stage_store(sv, lds);
+      if (!tr.has_row) return s;
+      const int t0 = (int)(r - unit * kBlock);
+#pragma unroll
+      for (int dz = 0; dz < WZ; ++dz)
+#pragma unroll
+        for (int dy = 0; dy < W; ++dy)
+#pragma unroll
+          for (int dx = 0; dx < W; ++dx) xn[dz][dy][dx] = lds[(dz * W + dy) * kSeg + t0 + dx];
+    }
     // cell k in a direction: c = g / ORDER - 1 + k, local position
     // a = g - ORDER c; it exists when 0 <= c < nc and a <= ORDER
     bool okc[KZ][2][2];
@@ -279,7 +313,6 @@ struct StencilOp {
   // must call it)
   __device__ __forceinline__ T row_sum(long long unit, const T *x, T s, TileRow &tr,
                                        T *lds) const {
-    stage_x(unit, x, lds);
     return row_sum_scaled<true>(unit, x, s, T(1), tr, lds);
   }
 
