@@ -382,26 +382,38 @@ def bench_mixed(hflmp, n):
     db = ctx.upload(b)
     out = {"workload": "3D Q1 log-normal diffusivity (sigma 1) %d^3, Jacobi, rtol 1e-10" % n,
            "seeds": {"kappa": KAPPA_SEED, "rhs": RHS_SEED},
+           "timing": "FP64, reliable and restart solves in turn, 3 rounds after a warm-up "
+                     "round, fastest round of each",
            "fp32_operator": "SELL-64, int16 columns" if S32.info()["col_bytes"] == 2
            else "SELL-64, int32 columns"}
-    for rep in range(2):
-        dx64 = ctx.zeros(N, np.float64)
-        i64 = hflmp.pcg_jacobi_sell(ctx, S64, invd, db, dx64, maxit=50000, rtol=1e-10)
-        if rep == 0:
-            dx64.free()
-    x64 = dx64.download()
+    # The three solves are timed in turn, three rounds after a warm-up
+    # round, and each keeps its fastest round: run-to-run drift of the
+    # device then touches all three alike.
+    def run_fp64():
+        dx = ctx.zeros(N, np.float64)
+        info = hflmp.pcg_jacobi_sell(ctx, S64, invd, db, dx, maxit=50000, rtol=1e-10)
+        x = dx.download()
+        dx.free()
+        return info, x
+
     def run_mixed(flags):
-        for rep in range(2):
-            dxm = ctx.zeros(N, np.float64)
-            im = hflmp.mpcg_jacobi_sell(ctx, S64, S32, invd32, db, dxm, rtol=1e-10,
-                                        inner_rtol=1e-4, flags=flags)
-            if rep == 0:
-                dxm.free()
+        dxm = ctx.zeros(N, np.float64)
+        info = hflmp.mpcg_jacobi_sell(ctx, S64, S32, invd32, db, dxm, rtol=1e-10,
+                                      inner_rtol=1e-4, flags=flags)
         xm = dxm.download()
         dxm.free()
-        return im, xm
-    ir, xr = run_mixed(hflmp.MP_RELIABLE)
-    im, xm = run_mixed(0)
+        return info, xm
+
+    best = {}
+    for rnd in range(4):
+        for key, fn in (("fp64", run_fp64), ("reliable", lambda: run_mixed(hflmp.MP_RELIABLE)),
+                        ("restart", lambda: run_mixed(0))):
+            info, xv = fn()
+            if rnd > 0 and (key not in best or info["seconds"] < best[key][0]["seconds"]):
+                best[key] = (info, xv)
+    i64, x64 = best["fp64"]
+    ir, xr = best["reliable"]
+    im, xm = best["restart"]
     bn = float(np.linalg.norm(b))
     out["fp64_cg"] = {"iters": i64["iters"], "seconds": round(i64["seconds"], 5),
                       "status": hflmp.STATUS.get(i64["status"], "?"),
@@ -444,7 +456,7 @@ def bench_mixed(hflmp, n):
         "true_rel_residual": float(np.linalg.norm(b - O.spmv(rp, ci, v, xg))) / bn,
         "rel_error_vs_fp64_gpu": float(np.linalg.norm(xg - x64) / np.linalg.norm(x64)),
         "speedup_vs_fp64_jacobi_cg": round(i64["seconds"] / ig["seconds"], 3)}
-    for a in (dx64, db, invd, invd32):
+    for a in (db, invd, invd32):
         a.free()
     S64.free(); S32.free(); A32.val.free(); A.free()
     ctx.close()
