@@ -488,11 +488,14 @@ void hflmp_mg_destroy(hflmp_mg *m);
  * gmg.h:1159).  C is copied; NULL returns to the sweeps. */
 int hflmp_mg_set_coarse_inverse_f64(hflmp_ctx *ctx, hflmp_mg *m,
                                     const double *inv);
-/* The levels are the halved structured Q1 meshes of ncells, ncells/2, ...
- * cells per direction (nodes lexicographic, x fastest) and R, P the Q1
- * interpolation with Dirichlet rows and columns removed (as built by
- * ba_amd/mg.py): restriction and prolongation then run without reading R
- * and P, bit-identical to the products over them.  dim 0 switches back. */
+/* No reference counterpart (the reference's GMG always applies its R and P
+ * matrices, cuda_gmg_kernel.h:89-157): the levels are the halved
+ * structured Q1 meshes of ncells, ncells/2, ... cells per direction (nodes
+ * lexicographic, x fastest) and R, P the Q1 interpolation with Dirichlet
+ * rows and columns removed (as built by ba_amd/mg.py); restriction and
+ * prolongation then run without reading R and P, bit-identical to the
+ * products over them.  dim 0 switches back.  A hierarchy runs one cycle at
+ * a time: its level vectors are shared by every call. */
 int hflmp_mg_set_structured_q1(hflmp_ctx *ctx, hflmp_mg *m, int dim,
                                int ncells);
 /* z = M r (one V-cycle from z = 0) */
