@@ -1017,7 +1017,7 @@ class McSgs:
 
 # ---- C++ lmp layer (libhflmp_lmp.so) ---------------------------------------------------
 LMP_CG, LMP_GMRES, LMP_FGMRES, LMP_MPIR, LMP_BICGSTAB = 0, 1, 2, 3, 4
-LMP_PREC_NONE, LMP_PREC_JACOBI, LMP_PREC_MCSGS = 0, 1, 2
+LMP_PREC_NONE, LMP_PREC_JACOBI, LMP_PREC_MCSGS, LMP_PREC_GMG = 0, 1, 2, 3
 
 
 class LmpOptions(C.Structure):
@@ -1051,6 +1051,11 @@ def lmp_lib():
                                       C.POINTER(LmpOptions), C.POINTER(LmpResult)]
         L.hflmp_lmp_vector_mult.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
                                             C.c_void_p, C.c_void_p, C.c_void_p]
+        L.hflmp_lmp_solve_gmg.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                          C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                          C.c_void_p, C.c_void_p, C.c_void_p,
+                                          C.POINTER(MgParams), C.c_void_p,
+                                          C.POINTER(LmpOptions), C.POINTER(LmpResult)]
         L.hflmp_lmp_last_error.restype = C.c_char_p
         ip = C.POINTER(C.c_int)
         L.hflmp_lmp_mm_read_coo.argtypes = [C.c_char_p, ip, ip, ip, C.c_void_p, C.c_void_p,
@@ -1086,6 +1091,63 @@ def lmp_solve(rp, ci, v, b, x0=None, device=0, **kw):
                                    x.ctypes.data, C.byref(o), C.byref(r))
     if rc:
         raise HflmpError("hflmp_lmp_solve: " + lmp_lib().hflmp_lmp_last_error().decode())
+    return x, r.as_dict()
+
+
+def _host_csr(m):
+    """(CsrDesc over host arrays, arrays kept alive) of a scipy matrix or a
+    (row_ptr, col, val, ncol) tuple."""
+    if isinstance(m, tuple):
+        rp, ci, v, ncol = m
+    else:
+        m = m.tocsr()
+        rp, ci, v, ncol = m.indptr, m.indices, m.data, m.shape[1]
+    rp = np.ascontiguousarray(rp, np.int32)
+    ci = np.ascontiguousarray(ci, np.int32)
+    v = np.ascontiguousarray(v, np.float64)
+    return CsrDesc(len(rp) - 1, int(ncol), int(rp[-1]), rp.ctypes.data, ci.ctypes.data,
+                   v.ctypes.data), (rp, ci, v)
+
+
+def lmp_solve_gmg(rp, ci, v, b, coarse, R, P, x0=None, coarse_inverse=None, pre_smooth=1,
+                  post_smooth=1, coarse_sweeps=4, omega=0.7, coarse_omega=0.7, device=0, **kw):
+    """Solve with a C++ solver template (kw solver=, default CG)
+    preconditioned by PreconditionerGMG: coarse[l], R[l], P[l] are host
+    matrices (scipy, or (row_ptr, col, val, ncol) tuples) of level l + 1 and
+    of the transfers between levels l and l + 1; returns (x, result dict)."""
+    if not (len(coarse) == len(R) == len(P)):
+        raise ValueError("lmp_solve_gmg: need one R and one P per coarse level")
+    rp = np.ascontiguousarray(rp, np.int32)
+    ci = np.ascontiguousarray(ci, np.int32)
+    v = np.ascontiguousarray(v, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    x = np.zeros(len(rp) - 1) if x0 is None else np.array(x0, np.float64)
+    keep = []
+
+    def descs(ms):
+        arr = (CsrDesc * max(1, len(ms)))()
+        for i, m in enumerate(ms):
+            arr[i], k = _host_csr(m)
+            keep.append(k)
+        return arr
+
+    A_, R_, P_ = descs(coarse), descs(R), descs(P)
+    cinv = None
+    if coarse_inverse is not None:
+        cinv = np.ascontiguousarray(coarse_inverse, np.float64)
+        n = coarse[-1].shape[0] if coarse and not isinstance(coarse[-1], tuple) else \
+            (len(coarse[-1][0]) - 1 if coarse else len(rp) - 1)
+        if cinv.shape != (n, n):
+            raise ValueError("lmp_solve_gmg: coarse_inverse must be %d x %d" % (n, n))
+    prm = MgParams(pre_smooth, post_smooth, coarse_sweeps, omega, coarse_omega)
+    kw.setdefault("solver", LMP_CG)
+    o = lmp_options(**kw)
+    r = LmpResult()
+    _lmp_check(lmp_lib().hflmp_lmp_solve_gmg(
+        device, len(rp) - 1, len(ci), rp.ctypes.data, ci.ctypes.data, v.ctypes.data,
+        b.ctypes.data, x.ctypes.data, len(coarse) + 1, C.addressof(A_), C.addressof(R_),
+        C.addressof(P_), C.byref(prm), None if cinv is None else cinv.ctypes.data,
+        C.byref(o), C.byref(r)), "hflmp_lmp_solve_gmg")
     return x, r.as_dict()
 
 

@@ -16,6 +16,7 @@
 #include "linear_solver/gmres.h"
 #include "linear_solver/mpir.h"
 #include "linear_solver/preconditioner_bjacobi_standard.h"
+#include "linear_solver/preconditioner_gmg.h"
 #include "platform_management.h"
 
 using namespace hiflow;
@@ -144,6 +145,20 @@ void setup_system(int device, int nrow, const int *rp, const int *ci, const doub
   A.InitCSR(rp, ci, val);
 }
 
+// host CSR arrays of one level of the hierarchy
+PreconditionerGMG<LADescriptorCoupledD>::Csr host_csr(const hflmp_csr_f64 &m, const char *what) {
+  if (m.nrow <= 0 || m.ncol <= 0 || !m.row_ptr || m.row_ptr[m.nrow] != m.nnz ||
+      (m.nnz > 0 && (!m.col || !m.val)))
+    LMP_FATAL("hflmp_lmp_solve_gmg: bad " << what << " level");
+  PreconditionerGMG<LADescriptorCoupledD>::Csr c;
+  c.nrow = m.nrow;
+  c.ncol = m.ncol;
+  c.row_ptr.assign(m.row_ptr, m.row_ptr + m.nrow + 1);
+  c.col.assign(m.col, m.col + m.nnz);
+  c.val.assign(m.val, m.val + m.nnz);
+  return c;
+}
+
 }  // namespace
 
 extern "C" {
@@ -173,6 +188,10 @@ int hflmp_lmp_solve(int device, int nrow, int nnz, const int *rp, const int *ci,
     g_last_error = "hflmp_lmp_solve: bad arguments";
     return -1;
   }
+  if (opt->precond == HFLMP_LMP_PREC_GMG) {
+    g_last_error = "hflmp_lmp_solve: multigrid needs its hierarchy (hflmp_lmp_solve_gmg)";
+    return -1;
+  }
   std::memset(res, 0, sizeof(*res));
   try {
     CoupledMatrix<double> A;
@@ -188,6 +207,65 @@ int hflmp_lmp_solve(int device, int nrow, int nnz, const int *rp, const int *ci,
       run_plain<LADescriptorCoupledS>(A, vb, vx, *opt, res);
     else
       run_plain<LADescriptorCoupledD>(A, vb, vx, *opt, res);
+    vx.interior().GetBlockValues(0, nrow, x);
+  } catch (const std::exception &e) {
+    g_last_error = e.what();
+    return -1;
+  }
+  return 0;
+}
+
+int hflmp_lmp_solve_gmg(int device, int nrow, int nnz, const int *rp, const int *ci,
+                        const double *val, const double *b, double *x, int nlevels,
+                        const hflmp_csr_f64 *coarse, const hflmp_csr_f64 *R,
+                        const hflmp_csr_f64 *P, const hflmp_mg_params *prm,
+                        const double *coarse_inverse, const hflmp_lmp_options *opt,
+                        hflmp_lmp_result *res) {
+  ThrowOnFatal guard;
+  if (!rp || !ci || !val || !b || !x || !opt || !res || !prm || nrow <= 0 || rp[nrow] != nnz ||
+      nlevels < 1 || (nlevels > 1 && (!coarse || !R || !P)) ||
+      (opt->solver != HFLMP_LMP_CG && opt->solver != HFLMP_LMP_GMRES &&
+       opt->solver != HFLMP_LMP_FGMRES && opt->solver != HFLMP_LMP_BICGSTAB)) {
+    g_last_error = "hflmp_lmp_solve_gmg: bad arguments";
+    return -1;
+  }
+  std::memset(res, 0, sizeof(*res));
+  try {
+    typedef LADescriptorCoupledD LAD;
+    std::vector<PreconditionerGMG<LAD>::Csr> Ac, Rc, Pc;
+    for (int l = 0; l + 1 < nlevels; ++l) {
+      Ac.push_back(host_csr(coarse[l], "coarse operator"));
+      Rc.push_back(host_csr(R[l], "restriction"));
+      Pc.push_back(host_csr(P[l], "prolongation"));
+    }
+    std::vector<double> cinv;
+    if (coarse_inverse) {
+      const size_t n = nlevels > 1 ? (size_t)coarse[nlevels - 2].nrow : (size_t)nrow;
+      cinv.assign(coarse_inverse, coarse_inverse + n * n);
+    }
+    CoupledMatrix<double> A;
+    setup_system(device, nrow, rp, ci, val, A);
+    CoupledVector<double> vb, vx;
+    vb.Init(nrow);
+    vx.Init(nrow);
+    vb.interior().SetBlockValues(0, nrow, b);
+    vx.interior().SetBlockValues(0, nrow, x);
+    hflmp_lmp_options o = *opt;
+    o.precond = HFLMP_LMP_PREC_GMG;
+    auto solver = make_solver<LAD>(o.solver, o);
+    PreconditionerGMG<LAD> gmg;
+    gmg.SetHierarchy(std::move(Ac), std::move(Rc), std::move(Pc), *prm, std::move(cinv));
+    solver->SetupPreconditioner(gmg);
+    solver->SetupOperator(A);
+    hip_check(hflmp_ctx_sync(hip_ctx()), "sync");
+    const auto t0 = std::chrono::steady_clock::now();
+    const LinearSolverState st = solver->Solve(vb, &vx);
+    hip_check(hflmp_ctx_sync(hip_ctx()), "sync");
+    res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    res->iters = solver->iter();
+    res->status = (int)st;
+    res->res_init = solver->res_init();
+    res->res_final = solver->res();
     vx.interior().GetBlockValues(0, nrow, x);
   } catch (const std::exception &e) {
     g_last_error = e.what();

@@ -11,6 +11,8 @@
 #ifndef HFLMP_LMP_H
 #define HFLMP_LMP_H
 
+#include "hflmp_hip.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -24,6 +26,7 @@ extern "C" {
 #define HFLMP_LMP_PREC_NONE 0
 #define HFLMP_LMP_PREC_JACOBI 1
 #define HFLMP_LMP_PREC_MCSGS 2
+#define HFLMP_LMP_PREC_GMG 3 /* hflmp_lmp_solve_gmg only */
 
 typedef struct {
   int solver;         /* HFLMP_LMP_* (CG, GMRES, FGMRES, MPIR, BICGSTAB)   */
@@ -52,6 +55,21 @@ void hflmp_lmp_default_options(hflmp_lmp_options *opt);
 int hflmp_lmp_solve(int device, int nrow, int nnz, const int *row_ptr, const int *col,
                     const double *val, const double *b, double *x,
                     const hflmp_lmp_options *opt, hflmp_lmp_result *res);
+/* Solve A x = b with opt->solver (CG, GMRES, FGMRES or BiCGSTAB; opt->precond
+ * is ignored) preconditioned by one geometric-multigrid V-cycle per
+ * application: PreconditionerGMG, the reference's GMG used as a
+ * preconditioner (linear_solver/gmg.h:1070-1187), over hflmp_mg_*.  A, b, x
+ * as in hflmp_lmp_solve (the finest level); coarse[l], R[l], P[l]
+ * (l < nlevels - 1) are HOST CSR arrays of level l + 1's operator and of the
+ * restriction / prolongation between levels l and l + 1; coarse_inverse
+ * (optional, host, n x n row-major for the coarsest level's n rows) replaces
+ * the coarsest level's sweeps. */
+int hflmp_lmp_solve_gmg(int device, int nrow, int nnz, const int *row_ptr, const int *col,
+                        const double *val, const double *b, double *x, int nlevels,
+                        const hflmp_csr_f64 *coarse, const hflmp_csr_f64 *R,
+                        const hflmp_csr_f64 *P, const hflmp_mg_params *prm,
+                        const double *coarse_inverse, const hflmp_lmp_options *opt,
+                        hflmp_lmp_result *res);
 /* y = A x through CoupledMatrix::VectorMult on the HIP platform. */
 int hflmp_lmp_vector_mult(int device, int nrow, int nnz, const int *row_ptr, const int *col,
                           const double *val, const double *x, double *y);
