@@ -63,6 +63,8 @@ struct SlotState {
   int status;  // mixed: kFresh, kIterate or the final status
   int phase;   // mixed: 0 outer step due, 1 inner CG running, 2 its p to refresh first
   int outer, max_outer, inner_total, inner_maxit;
+  int col16;  // FP64 operator: 1 if its columns are int16 offsets from the slice
+              // start (decided on the device at admission), else int32
 };
 
 struct SysDesc {
@@ -72,7 +74,7 @@ struct SysDesc {
   const long long *soff;
   const int *slen;
   const unsigned char *rlen;
-  const int *col;
+  const void *col;  // int16 or int32 by st->col16
   const double *sval;
   // FP32 SELL-64 (P = 4): the inner operator of mixed systems
   const long long *soff32;
@@ -89,14 +91,15 @@ struct SysDesc {
   SlotState *st;
 };
 
-__device__ __forceinline__ SellOp<double, int> op64(const SysDesc &d) {
-  SellOp<double, int> op;
+template <typename IT>
+__device__ __forceinline__ SellOp<double, IT> op64(const SysDesc &d) {
+  SellOp<double, IT> op;
   op.nrow = d.nrow;
   op.nslice = d.nslice;
   op.soff = d.soff;
   op.slen = d.slen;
   op.rlen = d.rlen;
-  op.col = d.col;
+  op.col = static_cast<const IT *>(d.col);
   op.val = d.sval;
   return op;
 }
@@ -261,7 +264,8 @@ __global__ void __launch_bounds__(kBlock)
   if (!mixed_outer_due(d.st)) return;
   const bool fresh = d.st->status == kFresh;
   TileRow tr;
-  const double s = op64(d).row_sum(j, d.x, 0.0, tr, nullptr);
+  const double s = d.st->col16 ? op64<short>(d).row_sum(j, d.x, 0.0, tr, nullptr)
+                               : op64<int>(d).row_sum(j, d.x, 0.0, tr, nullptr);
   double rr = 0;
   float rr32 = 0, rz32 = 0;
   if (tr.has_row) {
@@ -354,7 +358,8 @@ __global__ void __launch_bounds__(kBlock)
   TileRow tr;
   if (d.mode == kModeFp64) {
     if (d.st->s64.status != kIterate) return;
-    const double s = op64(d).row_sum(j, d.p, 0.0, tr, nullptr);
+    const double s = d.st->col16 ? op64<short>(d).row_sum(j, d.p, 0.0, tr, nullptr)
+                                 : op64<int>(d).row_sum(j, d.p, 0.0, tr, nullptr);
     double part = 0;
     if (tr.has_row) {
       d.Ap[tr.row] = s;
@@ -507,6 +512,22 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+// Clears st->col16 if an entry's column is not within int16 of its slice's
+// first row (the flag starts at 1; every writer stores the same 0).
+__global__ void __launch_bounds__(kBlock)
+    sys_band_kernel(int nrow, const int *__restrict__ rp, const int *__restrict__ ci,
+                    SlotState *st) {
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < nrow; i += gridDim.x * kBlock) {
+    const int base = (i / kSellRows) * kSellRows;
+    bool fits = true;
+    for (int k = rp[i]; k < rp[i + 1]; ++k) {
+      const int dlt = ci[k] - base;
+      fits = fits && dlt >= -32768 && dlt <= 32767;
+    }
+    if (!fits) st->col16 = 0;
+  }
+}
+
 // CSR -> SELL-64 layouts of one system (thread per row) and its inverse
 // diagonal: 1/a_ii formed in double (extract_invdiagelements,
 // lmatrix_csr_cpu.cc:2419-2442); for the FP32 operator from the FP32 value,
@@ -516,10 +537,12 @@ template <typename IT32>
 __global__ void __launch_bounds__(kBlock)
     sys_fill_kernel(int nrow, const int *__restrict__ rp, const int *__restrict__ ci,
                     const double *__restrict__ val, const long long *__restrict__ soff,
-                    int *__restrict__ col, double *__restrict__ sval, double *__restrict__ invd,
+                    const SlotState *st, void *__restrict__ col, double *__restrict__ sval,
+                    double *__restrict__ invd,
                     const long long *__restrict__ soff32, IT32 *__restrict__ col32,
                     float *__restrict__ sval32, float *__restrict__ invd32, unsigned int *bad) {
   constexpr int P = SellChunk<double>::P, P32 = SellChunk<float>::P;
+  const bool c16 = st->col16 != 0;
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < nrow; i += gridDim.x * kBlock) {
     const int s = i / kSellRows, lane = i % kSellRows;
     const long long base = soff[s] + (long long)P * lane;
@@ -530,7 +553,10 @@ __global__ void __launch_bounds__(kBlock)
       const int c = ci[k0 + j];
       const double v = val[k0 + j];
       const long long pos = base + (long long)kSellRows * P * (j / P) + (j % P);
-      col[pos] = c;
+      if (c16)
+        static_cast<short *>(col)[pos] = (short)(c - s * kSellRows);
+      else
+        static_cast<int *>(col)[pos] = c;
       sval[pos] = v;
       if (soff32) {
         const long long p32 = base32 + (long long)kSellRows * P32 * (j / P32) + (j % P32);
@@ -655,7 +681,7 @@ struct Job {
   size_t o_soff32 = 0, o_slen32 = 0, o_col32 = 0, o_sval32 = 0, o_invd32 = 0, o_r32 = 0,
          o_p32 = 0, o_z32 = 0, o_Ap32 = 0, o_c32 = 0;
   std::chrono::steady_clock::time_point t_submit, t_admit, t_done;
-  int iters = 0, status = 0, outer = 0, bad_diag = 0;
+  int iters = 0, status = 0, outer = 0, bad_diag = 0, col_bytes = 0;
   double res_init = 0, res_final = 0;
 };
 
@@ -699,7 +725,7 @@ void plan_job(Job &jb) {
   jb.o_part = take(16 * (size_t)jb.nunits);
   jb.o_soff = take(8 * ((size_t)jb.nslice + 1));
   jb.o_slen = take(4 * (size_t)jb.nslice);
-  jb.o_col = take(4 * e64);
+  jb.o_col = take(4 * e64);  // int32 width; an int16 layout uses the first half
   jb.o_sval = take(8 * e64);
   if (jb.mode == kModeFp64) {
     jb.o_invd = take(8 * n);
@@ -844,31 +870,6 @@ int admit(hflmp_server *sv, Job &jb, int slot) {
     if (!rc) rc = hflmp_fill_zero(ctx, dp(jb.o_sval32), 4 * (size_t)jb.sp32.entries);
   }
   if (rc) return rc;
-  const unsigned g = grid_for(jb.nrow, 1);
-  const int *rp = static_cast<const int *>(dp(jb.o_rp));
-  const int *ci = static_cast<const int *>(dp(jb.o_ci));
-  const double *val = static_cast<const double *>(dp(jb.o_val));
-  const long long *soff = static_cast<const long long *>(dp(jb.o_soff));
-  int *col = static_cast<int *>(dp(jb.o_col));
-  double *sval = static_cast<double *>(dp(jb.o_sval));
-  if (!mixed) {
-    sys_fill_kernel<int><<<g, kBlock, 0, s>>>(jb.nrow, rp, ci, val, soff, col, sval,
-                                              static_cast<double *>(dp(jb.o_invd)), nullptr,
-                                              nullptr, nullptr, nullptr, sv->d_bad + slot);
-  } else if (jb.col16) {
-    sys_fill_kernel<short><<<g, kBlock, 0, s>>>(
-        jb.nrow, rp, ci, val, soff, col, sval, nullptr,
-        static_cast<const long long *>(dp(jb.o_soff32)), static_cast<short *>(dp(jb.o_col32)),
-        static_cast<float *>(dp(jb.o_sval32)), static_cast<float *>(dp(jb.o_invd32)),
-        sv->d_bad + slot);
-  } else {
-    sys_fill_kernel<int><<<g, kBlock, 0, s>>>(
-        jb.nrow, rp, ci, val, soff, col, sval, nullptr,
-        static_cast<const long long *>(dp(jb.o_soff32)), static_cast<int *>(dp(jb.o_col32)),
-        static_cast<float *>(dp(jb.o_sval32)), static_cast<float *>(dp(jb.o_invd32)),
-        sv->d_bad + slot);
-  }
-  HFLMP_LAUNCH_CHECK(ctx);
   SlotState st;
   std::memset(&st, 0, sizeof(st));
   st.mode = jb.mode;
@@ -884,11 +885,41 @@ int admit(hflmp_server *sv, Job &jb, int slot) {
   st.s64.rtol = jb.rtol;
   st.s64.dtol = 1e300;
   st.s32.status = kConverged;
+  st.col16 = 1;  // until the band kernel finds an entry that does not fit
   // the pinned mirror slot is free (the slot was retired) and is only read
   // after the next state copy, which the stream orders after this one
   sv->h_states[slot] = st;
   HFLMP_HIP_CHECK(ctx, hipMemcpyAsync(sv->d_states + slot, sv->h_states + slot, sizeof(st),
                                       hipMemcpyHostToDevice, s));
+  // the FP64 layout's column width is decided on the device (no host pass
+  // over the columns); the fill and every product read the slot's flag
+  const unsigned g = grid_for(jb.nrow, 1);
+  const int *rp = static_cast<const int *>(dp(jb.o_rp));
+  const int *ci = static_cast<const int *>(dp(jb.o_ci));
+  const double *val = static_cast<const double *>(dp(jb.o_val));
+  const long long *soff = static_cast<const long long *>(dp(jb.o_soff));
+  void *col = dp(jb.o_col);
+  double *sval = static_cast<double *>(dp(jb.o_sval));
+  SlotState *dst = sv->d_states + slot;
+  if (jb.nnz > 0) sys_band_kernel<<<g, kBlock, 0, s>>>(jb.nrow, rp, ci, dst);
+  if (!mixed) {
+    sys_fill_kernel<int><<<g, kBlock, 0, s>>>(jb.nrow, rp, ci, val, soff, dst, col, sval,
+                                              static_cast<double *>(dp(jb.o_invd)), nullptr,
+                                              nullptr, nullptr, nullptr, sv->d_bad + slot);
+  } else if (jb.col16) {
+    sys_fill_kernel<short><<<g, kBlock, 0, s>>>(
+        jb.nrow, rp, ci, val, soff, dst, col, sval, nullptr,
+        static_cast<const long long *>(dp(jb.o_soff32)), static_cast<short *>(dp(jb.o_col32)),
+        static_cast<float *>(dp(jb.o_sval32)), static_cast<float *>(dp(jb.o_invd32)),
+        sv->d_bad + slot);
+  } else {
+    sys_fill_kernel<int><<<g, kBlock, 0, s>>>(
+        jb.nrow, rp, ci, val, soff, dst, col, sval, nullptr,
+        static_cast<const long long *>(dp(jb.o_soff32)), static_cast<int *>(dp(jb.o_col32)),
+        static_cast<float *>(dp(jb.o_sval32)), static_cast<float *>(dp(jb.o_invd32)),
+        sv->d_bad + slot);
+  }
+  HFLMP_LAUNCH_CHECK(ctx);
   SysDesc &d = jb.d;
   std::memset(&d, 0, sizeof(d));
   d.nrow = jb.nrow;
@@ -1134,6 +1165,7 @@ int hflmp_server_step(hflmp_server *sv, int *n_done) {
         jb.res_final = st.s64.res;
       }
       jb.bad_diag = (int)bad;
+      jb.col_bytes = st.col16 ? 2 : 4;
       jb.t_done = Clock::now();
       jb.state = kDone;
       sv->alloc.release(jb.off, jb.bytes);
@@ -1167,6 +1199,7 @@ int hflmp_server_poll(hflmp_server *sv, long long ticket, hflmp_server_result *r
   res->latency_seconds = jb.state == kDone ? sec(jb.t_submit, jb.t_done) : -1.0;
   res->precision = jb.mode;
   res->corrections = jb.outer;
+  res->col_bytes = jb.col_bytes;
   return HFLMP_OK;
 }
 

@@ -904,7 +904,7 @@ class ServerResult(C.Structure):
                 ("res_init", C.c_double), ("res_final", C.c_double),
                 ("zero_diagonals", C.c_int), ("footprint", C.c_size_t),
                 ("queued_seconds", C.c_double), ("latency_seconds", C.c_double),
-                ("precision", C.c_int), ("corrections", C.c_int)]
+                ("precision", C.c_int), ("corrections", C.c_int), ("col_bytes", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

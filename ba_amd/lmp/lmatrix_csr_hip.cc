@@ -74,7 +74,8 @@ const hflmp_sell *HIP_CSR_lMatrix<T>::sell_layout() const {
     return nullptr;
   if (!sell_) {
     const int rc = HipOps<T>::sell_create(hip_ctx(), this->num_row_, this->num_col_,
-                                          this->nnz_, row_, col_, val_, 0, &sell_);
+                                          this->nnz_, row_, col_, val_, HFLMP_SELL_COL16,
+                                          &sell_);
     if (rc == HFLMP_EFORMAT) {
       sell_ = nullptr;
       sell_refused_ = true;

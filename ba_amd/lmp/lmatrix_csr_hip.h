@@ -7,6 +7,8 @@
 // VectorMult and VectorMultAdd run through a SELL-64 copy of the matrix
 // (hflmp_sell_*: slices of 64 rows, one wave each, lane per row), built on
 // the device at the first product and refreshed after any value change.
+// Its columns are int16 offsets from each slice's first row when every
+// column fits (10 B per FP64 nonzero instead of 12), else int32.
 // Its products are bit-identical to the CSR kernels'; a pattern that does
 // not suit it (a row longer than 255 entries, heavy padding) stays on the
 // CSR kernels, and set_use_sell(false) forces them.  The copy costs about

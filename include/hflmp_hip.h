@@ -625,6 +625,9 @@ int hflmp_mpcg_jacobi_sell(hflmp_ctx *ctx, const hflmp_sell *A64,
  * < rtol ||b||, with maxit bounding each inner solve.  precision 0 at
  * submission picks the mixed mode for mixed_min_nnz <= nnz < mixed_max_nnz
  * (mixed_min_nnz 0: never; mixed_max_nnz 0: no upper bound).
+ * The FP64 SELL-64 operator holds int16 column offsets from each slice's
+ * first row when every column fits (decided on the device at admission;
+ * the footprint assumes int32), else int32 columns.
  * The caller's host arrays (row_ptr, col, val, b, x) must stay valid until
  * the ticket is done.  Rows may hold 0..255 entries. */
 typedef struct hflmp_server hflmp_server;
@@ -648,6 +651,9 @@ typedef struct {
   double latency_seconds; /* submit -> x in the caller's buffer (-1 before) */
   int precision;          /* 64 or 32 (FP32-inner / FP64-refined)          */
   int corrections;        /* mixed: outer corrections applied              */
+  int col_bytes;          /* done: column width of the FP64 operator, 2 (int16
+                             offsets, when every column is within int16 of its
+                             slice's first row) or 4                         */
 } hflmp_server_result;
 typedef struct {
   int queued, active, active_units;

@@ -143,6 +143,56 @@ def test_serve_edge_cases(ctx):
         sv.close()
 
 
+def _wide_band_system(m=200, far=35000, seed=5):
+    """SPD 2D Laplacian on m^2 nodes plus symmetric couplings between nodes
+    `far` apart, so some columns are not within int16 of their slice."""
+    import scipy.sparse as sp
+    T = sp.diags([-1.0, 2.0, -1.0], [-1, 0, 1], shape=(m, m))
+    L = sp.kron(sp.eye(m), T) + sp.kron(T, sp.eye(m))
+    N = m * m
+    idx = np.arange(0, N - far, 97)
+    Cf = sp.csr_matrix((np.full(len(idx), -0.1), (idx, idx + far)), shape=(N, N))
+    extra = np.asarray(abs(Cf).sum(axis=1)).ravel() + np.asarray(abs(Cf).sum(axis=0)).ravel()
+    A = sp.csr_matrix(L + Cf + Cf.T + sp.diags(extra))
+    A.sort_indices()
+    b = np.random.default_rng(seed).uniform(-1, 1, N)
+    return (A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data.astype(np.float64),
+            b)
+
+
+def test_serve_fp64_column_widths(ctx):
+    """The FP64 operator takes int16 columns where every column fits and
+    int32 otherwise; either way the result matches the single device solve
+    (same iterations; x to rounding, as the two reduce their dot products
+    in different orders)."""
+    narrow = request(2, 40, 12)
+    wide = _wide_band_system()
+    rp, ci = wide[0], wide[1]
+    base = (np.repeat(np.arange(len(rp) - 1), np.diff(rp)) // 64) * 64
+    assert np.abs(ci.astype(np.int64) - base).max() > 32767   # the band does not fit
+    sv = hflmp.Server(ctx, 256 * MB, max_active=4, chunk_iters=6)
+    try:
+        out = run_all(sv, [narrow, wide, narrow])
+    finally:
+        sv.close()
+    assert [i["col_bytes"] for _, i in out] == [2, 4, 2]
+    for (rp, ci, v, b), (x, info) in zip([narrow, wide], out[:2]):
+        A = hflmp.CsrMatrix(ctx, rp, ci, v)
+        S = hflmp.SellMatrix(ctx, A, hflmp.SELL_COL16)
+        invd = hflmp.invdiag(ctx, A)
+        db, dx = ctx.upload(b), ctx.zeros(A.nrow, np.float64)
+        si = hflmp.pcg_jacobi_sell(ctx, S, invd, db, dx, rtol=1e-8, maxit=100000)
+        assert S.info()["col_bytes"] == info["col_bytes"]
+        assert info["status"] == 0 and si["status"] == 0
+        assert abs(info["iters"] - si["iters"]) <= 1
+        xs = dx.download()
+        assert np.linalg.norm(x - xs) <= 1e-12 * np.linalg.norm(xs)
+        S.free(); A.free(); invd.free(); db.free(); dx.free()
+    xo, io = O.cg(*wide, prec=O.PREC_JACOBI, rtol=1e-8, maxit=100000)
+    assert abs(out[1][1]["iters"] - io["iters"]) <= 2
+    assert np.linalg.norm(out[1][0] - xo) <= 1e-8 * np.linalg.norm(xo)
+
+
 def test_serve_mixed_vs_oracle(ctx):
     reqs = [request(*m) for m in MIX]
     sv = hflmp.Server(ctx, 256 * MB, max_active=8, chunk_iters=8)
