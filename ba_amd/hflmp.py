@@ -1135,8 +1135,9 @@ def lmp_solve_gmg(rp, ci, v, b, coarse, R, P, x0=None, coarse_inverse=None, pre_
     cinv = None
     if coarse_inverse is not None:
         cinv = np.ascontiguousarray(coarse_inverse, np.float64)
-        n = coarse[-1].shape[0] if coarse and not isinstance(coarse[-1], tuple) else \
-            (len(coarse[-1][0]) - 1 if coarse else len(rp) - 1)
+        if not coarse:
+            raise ValueError("lmp_solve_gmg: a dense coarse solve needs two levels or more")
+        n = coarse[-1].shape[0] if not isinstance(coarse[-1], tuple) else len(coarse[-1][0]) - 1
         if cinv.shape != (n, n):
             raise ValueError("lmp_solve_gmg: coarse_inverse must be %d x %d" % (n, n))
     prm = MgParams(pre_smooth, post_smooth, coarse_sweeps, omega, coarse_omega)

@@ -147,9 +147,16 @@ void setup_system(int device, int nrow, const int *rp, const int *ci, const doub
 
 // host CSR arrays of one level of the hierarchy
 PreconditionerGMG<LADescriptorCoupledD>::Csr host_csr(const hflmp_csr_f64 &m, const char *what) {
-  if (m.nrow <= 0 || m.ncol <= 0 || !m.row_ptr || m.row_ptr[m.nrow] != m.nnz ||
-      (m.nnz > 0 && (!m.col || !m.val)))
+  if (m.nrow <= 0 || m.ncol <= 0 || !m.row_ptr || m.row_ptr[0] != 0 ||
+      m.row_ptr[m.nrow] != m.nnz || (m.nnz > 0 && (!m.col || !m.val)))
     LMP_FATAL("hflmp_lmp_solve_gmg: bad " << what << " level");
+  // the device gathers at these columns: check them here
+  for (int r = 0; r < m.nrow; ++r)
+    if (m.row_ptr[r + 1] < m.row_ptr[r])
+      LMP_FATAL("hflmp_lmp_solve_gmg: " << what << " row_ptr decreases at row " << r);
+  for (int k = 0; k < m.nnz; ++k)
+    if (m.col[k] < 0 || m.col[k] >= m.ncol)
+      LMP_FATAL("hflmp_lmp_solve_gmg: " << what << " column " << m.col[k] << " out of range");
   PreconditionerGMG<LADescriptorCoupledD>::Csr c;
   c.nrow = m.nrow;
   c.ncol = m.ncol;
@@ -224,6 +231,7 @@ int hflmp_lmp_solve_gmg(int device, int nrow, int nnz, const int *rp, const int 
   ThrowOnFatal guard;
   if (!rp || !ci || !val || !b || !x || !opt || !res || !prm || nrow <= 0 || rp[nrow] != nnz ||
       nlevels < 1 || (nlevels > 1 && (!coarse || !R || !P)) ||
+      (coarse_inverse && nlevels < 2) ||
       (opt->solver != HFLMP_LMP_CG && opt->solver != HFLMP_LMP_GMRES &&
        opt->solver != HFLMP_LMP_FGMRES && opt->solver != HFLMP_LMP_BICGSTAB)) {
     g_last_error = "hflmp_lmp_solve_gmg: bad arguments";
@@ -240,7 +248,7 @@ int hflmp_lmp_solve_gmg(int device, int nrow, int nnz, const int *rp, const int 
     }
     std::vector<double> cinv;
     if (coarse_inverse) {
-      const size_t n = nlevels > 1 ? (size_t)coarse[nlevels - 2].nrow : (size_t)nrow;
+      const size_t n = (size_t)coarse[nlevels - 2].nrow;
       cinv.assign(coarse_inverse, coarse_inverse + n * n);
     }
     CoupledMatrix<double> A;

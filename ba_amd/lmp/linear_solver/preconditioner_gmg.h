@@ -54,6 +54,12 @@ class PreconditionerGMG : public Preconditioner<LAD> {
   }
   int num_levels() const { return (int)coarse_.size() + 1; }
 
+  // frees the device hierarchy; the host levels stay for the next Build
+  void Clear() override {
+    release();
+    Preconditioner<LAD>::Clear();
+  }
+
  protected:
   void BuildImpl() override {
     if (!this->op_) LMP_FATAL("PreconditionerGMG: no operator");

@@ -62,8 +62,10 @@ int hflmp_lmp_solve(int device, int nrow, int nnz, const int *row_ptr, const int
  * as in hflmp_lmp_solve (the finest level); coarse[l], R[l], P[l]
  * (l < nlevels - 1) are HOST CSR arrays of level l + 1's operator and of the
  * restriction / prolongation between levels l and l + 1; coarse_inverse
- * (optional, host, n x n row-major for the coarsest level's n rows) replaces
- * the coarsest level's sweeps. */
+ * (optional, host, n x n row-major for the coarsest level's n rows; needs
+ * nlevels >= 2) replaces the coarsest level's sweeps.  The host levels are
+ * checked (row_ptr from 0 to nnz and non-decreasing, columns in [0, ncol))
+ * before anything reaches the device. */
 int hflmp_lmp_solve_gmg(int device, int nrow, int nnz, const int *row_ptr, const int *col,
                         const double *val, const double *b, double *x, int nlevels,
                         const hflmp_csr_f64 *coarse, const hflmp_csr_f64 *R,

@@ -29,6 +29,38 @@ def test_coupled_vector_mult_bit_exact():
     assert np.array_equal(hflmp.lmp_vector_mult(rp, ci, v, x), O.spmv(rp, ci, v, x))
 
 
+def test_coupled_vector_mult_wide_band_bit_exact(ctx):
+    """HIP_CSR_lMatrix's SELL-64 copy takes int16 column offsets only when
+    every column is within int16 of its slice's first row: a matrix with
+    couplings 40000 rows apart runs the int32 layout, bit-exact either way."""
+    m, far = 220, 40000
+    T = sp.diags([-1.0, 2.0, -1.0], [-1, 0, 1], shape=(m, m))
+    L = sp.kron(sp.eye(m), T) + sp.kron(T, sp.eye(m))
+    N = m * m
+    idx = np.arange(0, N - far, 53)
+    Cf = sp.csr_matrix((np.full(len(idx), -0.25), (idx, idx + far)), shape=(N, N))
+    A = sp.csr_matrix(L + Cf + Cf.T)
+    A.sort_indices()
+    rp, ci, v = A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data
+    base = (np.repeat(np.arange(N), np.diff(rp)) // 64) * 64
+    assert np.abs(ci.astype(np.int64) - base).max() > 32767
+    # the same layout choice through the C ABI
+    dA = hflmp.CsrMatrix(ctx, rp, ci, v)
+    S = hflmp.SellMatrix(ctx, dA, hflmp.SELL_COL16)
+    assert S.info()["col_bytes"] == 4
+    S.free(); dA.free()
+    x = np.random.default_rng(8).uniform(-1, 1, N)
+    assert np.array_equal(hflmp.lmp_vector_mult(rp, ci, v, x), O.spmv(rp, ci, v, x))
+    # and a narrow band of the same size takes int16 columns
+    rp2, ci2, v2 = sp.csr_matrix(L).indptr.astype(np.int32), \
+        sp.csr_matrix(L).indices.astype(np.int32), sp.csr_matrix(L).data
+    dA = hflmp.CsrMatrix(ctx, rp2, ci2, v2)
+    S = hflmp.SellMatrix(ctx, dA, hflmp.SELL_COL16)
+    assert S.info()["col_bytes"] == 2
+    S.free(); dA.free()
+    assert np.array_equal(hflmp.lmp_vector_mult(rp2, ci2, v2, x), O.spmv(rp2, ci2, v2, x))
+
+
 @pytest.mark.parametrize("prec,sigma,it_tol", [
     (hflmp.LMP_PREC_JACOBI, 0.0, 2),     # the benchmark configuration: +-2 iterations
     (hflmp.LMP_PREC_NONE, 0.0, 2),

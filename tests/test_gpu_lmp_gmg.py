@@ -101,3 +101,10 @@ def test_lmp_gmg_refusals():
         hflmp.lmp_solve_gmg(*A, b, [bad] + coarse[1:], R, P)
     with pytest.raises(hflmp.HflmpError, match="bad arguments"):
         hflmp.lmp_solve_gmg(*A, b, coarse, R, P, solver=hflmp.LMP_MPIR)
+    # a column past the level's width is refused on the host
+    badc = ci.copy()
+    badc[5] = nr
+    with pytest.raises(hflmp.HflmpError, match="out of range"):
+        hflmp.lmp_solve_gmg(*A, b, [(rp, badc, v, nr)] + coarse[1:], R, P)
+    with pytest.raises(ValueError):
+        hflmp.lmp_solve_gmg(*A, b, [], [], [], coarse_inverse=np.eye(len(b)))
