@@ -40,7 +40,7 @@ __global__ void __launch_bounds__(kBlock)
       const int other = __shfl_xor(m, o);
       m = other > m ? other : m;
     }
-    if ((threadIdx.x & (kWave - 1)) == 0) slen[i / kSellRows] = (m + P - 1) / P * P;
+    if ((threadIdx.x & (kWave - 1)) == 0) slen[i / kSellRows] = m;
   }
 }
 
@@ -93,13 +93,15 @@ template <typename T, typename IT, bool COLS>
 __global__ void __launch_bounds__(kBlock)
     sell_fill_kernel(int nrow, int P, const int *__restrict__ rp, const int *__restrict__ ci,
                      const T *__restrict__ val, const long long *__restrict__ soff,
-                     IT *__restrict__ col, T *__restrict__ sval) {
+                     const int *__restrict__ slen, IT *__restrict__ col,
+                     T *__restrict__ sval) {
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < nrow; i += gridDim.x * kBlock) {
     const int s = i / kSellRows, lane = i % kSellRows;
-    const long long base = soff[s] + (long long)P * lane;
+    const long long off = soff[s];
+    const int m = slen[s];
     const int k0 = rp[i], len = rp[i + 1] - k0;
     for (int j = 0; j < len; ++j) {
-      const long long pos = base + (long long)kSellRows * P * (j / P) + (j % P);
+      const long long pos = sell_pos(off, m, P, lane, j);
       if (COLS) {
         const int c = ci[k0 + j];
         col[pos] = (IT)(std::is_same<IT, int>::value ? c : c - s * kSellRows);
@@ -207,11 +209,11 @@ int sell_create(hflmp_ctx *ctx, int nrow, int ncol, int nnz, const int *rp, cons
   if (nnz > 0) {
     if (m->col_bytes == 2)
       sell_fill_kernel<T, short, true><<<grid_for(nrow, 1), kBlock, 0, s>>>(
-          nrow, m->chunk, rp, ci, val, m->soff, static_cast<short *>(m->col),
+          nrow, m->chunk, rp, ci, val, m->soff, m->slen, static_cast<short *>(m->col),
           static_cast<T *>(m->val));
     else
       sell_fill_kernel<T, int, true><<<grid_for(nrow, 1), kBlock, 0, s>>>(
-          nrow, m->chunk, rp, ci, val, m->soff, static_cast<int *>(m->col),
+          nrow, m->chunk, rp, ci, val, m->soff, m->slen, static_cast<int *>(m->col),
           static_cast<T *>(m->val));
   }
   if (hipGetLastError() != hipSuccess) return fail(HFLMP_EHIP, "launch failed");
@@ -230,7 +232,7 @@ int sell_set_values(hflmp_ctx *ctx, hflmp_sell *m, const int *rp, const T *val) 
   HFLMP_REQUIRE(ctx, m->device == ctx->device, "sell_set_values: matrix on another device");
   if (m->nnz == 0) return HFLMP_OK;
   sell_fill_kernel<T, int, false><<<grid_for(m->nrow, 1), kBlock, 0, ctx->stream>>>(
-      m->nrow, m->chunk, rp, nullptr, val, m->soff, nullptr, static_cast<T *>(m->val));
+      m->nrow, m->chunk, rp, nullptr, val, m->soff, m->slen, nullptr, static_cast<T *>(m->val));
   HFLMP_LAUNCH_CHECK(ctx);
   return HFLMP_OK;
 }

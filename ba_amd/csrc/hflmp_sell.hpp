@@ -3,11 +3,14 @@
 //
 // SELL-64: rows are cut into slices of 64 consecutive rows, one wave64 per
 // slice and one lane per row.  Slice s stores its rows' entries padded to
-// the slice's longest row (rounded up to P), lane-interleaved in chunks of
-// P consecutive entries of a row:
-//     entry j of row 64 s + l  is at  off[s] + 64 P (j / P) + P l + (j mod P)
+// the slice's longest row (m entries), the first F = m - m mod P of them
+// lane-interleaved in chunks of P consecutive entries of a row, the last
+// m mod P in blocks of one entry per lane:
+//     entry j < F of row 64 s + l  is at  off[s] + 64 P (j / P) + P l + (j mod P)
+//     entry j >= F                 is at  off[s] + 64 j + l
 // so each load instruction of a wave reads one contiguous 64 P-element
-// block (FP64: P = 2, 16 B per lane; FP32: P = 4, 16 B per lane), and the
+// block (FP64: P = 2, 16 B per lane; FP32: P = 4, 16 B per lane), a row of
+// odd length loads no padding after its last entry, and the
 // x gathers of a step j hit x at 64 neighbouring rows' j-th columns, which
 // for an FE band is a nearly contiguous range.  Each lane adds its row's
 // products in ascending storage order starting from zero (or from y[i] for
@@ -57,6 +60,14 @@ struct VecOf {
 
 constexpr int kSellRows = 64;                         // rows per slice (one wave)
 constexpr int kSellSlicesPerBlock = kBlock / kWave;   // 4 slices per workgroup
+
+// Position of entry j of lane `lane` in a slice whose entries start at
+// `off` and whose longest row has m entries.
+__host__ __device__ inline long long sell_pos(long long off, int m, int P, int lane, int j) {
+  const int full = m / P * P;
+  return j < full ? off + (long long)kSellRows * P * (j / P) + (long long)P * lane + (j % P)
+                  : off + (long long)kSellRows * j + lane;
+}
 
 // Workgroups for an operator over SELL-64: 256 consecutive rows each,
 // rounded up to a multiple of the XCD count.
@@ -122,17 +133,32 @@ struct SellOp {
     const int len = tr.has_row ? (int)rlen[row] : 0;
     const long long base = soff[sid];
     const int smax = slen[sid];
+    const int full = smax / P * P;  // entries in P-chunks; the rest one per block
     const IV *__restrict__ c = reinterpret_cast<const IV *>(col + base) + lane;
     const TV *__restrict__ v = reinterpret_cast<const TV *>(val + base) + lane;
     // int16 columns are offsets from the slice's first row
     const T *__restrict__ x = std::is_same<IT, int>::value ? x_in : x_in + sid * kSellRows;
     const int dcol = std::is_same<IT, int>::value ? row : lane;
-    for (int j = 0; j < smax; j += P * U) {
+    // the row's entries past the last full chunk (at most P - 1), loaded
+    // first so their latency overlaps the chunks'
+    constexpr int TM = P > 1 ? P - 1 : 1;
+    const int ntail = len - full;
+    IT ct[TM];
+    T vt[TM];
+#pragma unroll
+    for (int t = 0; t < TM; ++t) {
+      if (t < ntail) {
+        const long long q = base + (long long)kSellRows * (full + t) + lane;
+        ct[t] = __builtin_nontemporal_load(col + q);
+        vt[t] = __builtin_nontemporal_load(val + q);
+      }
+    }
+    for (int j = 0; j < full; j += P * U) {
       IV cc[U];
       TV vv[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        if (j + u * P < len) {
+        if (j + u * P < len && j + u * P < full) {
           cc[u] = __builtin_nontemporal_load(c + (j / P + u) * kWave);
           vv[u] = __builtin_nontemporal_load(v + (j / P + u) * kWave);
         }
@@ -142,18 +168,32 @@ struct SellOp {
       for (int u = 0; u < U; ++u)
 #pragma unroll
         for (int q = 0; q < P; ++q)
-          if (j + u * P + q < len) xx[u][q] = x[(int)cc[u][q]];
+          if (j + u * P + q < len && j + u * P < full) xx[u][q] = x[(int)cc[u][q]];
 #pragma unroll
       for (int u = 0; u < U; ++u)
 #pragma unroll
         for (int q = 0; q < P; ++q)
-          if (j + u * P + q < len && (!NODIAG || (int)cc[u][q] != dcol)) {
+          if (j + u * P + q < len && j + u * P < full &&
+              (!NODIAG || (int)cc[u][q] != dcol)) {
             if (SUB)
               s -= vv[u][q] * xx[u][q];
             else
               s += vv[u][q] * xx[u][q];
           }
     }
+    // the tail, in ascending order after the chunks
+    T xt[TM];
+#pragma unroll
+    for (int t = 0; t < TM; ++t)
+      if (t < ntail) xt[t] = x[(int)ct[t]];
+#pragma unroll
+    for (int t = 0; t < TM; ++t)
+      if (t < ntail && (!NODIAG || (int)ct[t] != dcol)) {
+        if (SUB)
+          s -= vt[t] * xt[t];
+        else
+          s += vt[t] * xt[t];
+      }
     return s;
   }
 };

@@ -537,29 +537,32 @@ template <typename IT32>
 __global__ void __launch_bounds__(kBlock)
     sys_fill_kernel(int nrow, const int *__restrict__ rp, const int *__restrict__ ci,
                     const double *__restrict__ val, const long long *__restrict__ soff,
-                    const SlotState *st, void *__restrict__ col, double *__restrict__ sval,
-                    double *__restrict__ invd,
-                    const long long *__restrict__ soff32, IT32 *__restrict__ col32,
-                    float *__restrict__ sval32, float *__restrict__ invd32, unsigned int *bad) {
+                    const int *__restrict__ slen, const SlotState *st, void *__restrict__ col,
+                    double *__restrict__ sval, double *__restrict__ invd,
+                    const long long *__restrict__ soff32, const int *__restrict__ slen32,
+                    IT32 *__restrict__ col32, float *__restrict__ sval32,
+                    float *__restrict__ invd32, unsigned int *bad) {
   constexpr int P = SellChunk<double>::P, P32 = SellChunk<float>::P;
   const bool c16 = st->col16 != 0;
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < nrow; i += gridDim.x * kBlock) {
     const int s = i / kSellRows, lane = i % kSellRows;
-    const long long base = soff[s] + (long long)P * lane;
-    const long long base32 = soff32 ? soff32[s] + (long long)P32 * lane : 0;
+    const long long off = soff[s];
+    const int m = slen[s];
+    const long long off32 = soff32 ? soff32[s] : 0;
+    const int m32 = soff32 ? slen32[s] : 0;
     const int k0 = rp[i], len = rp[i + 1] - k0;
     double dg = 0;
     for (int j = 0; j < len; ++j) {
       const int c = ci[k0 + j];
       const double v = val[k0 + j];
-      const long long pos = base + (long long)kSellRows * P * (j / P) + (j % P);
+      const long long pos = sell_pos(off, m, P, lane, j);
       if (c16)
         static_cast<short *>(col)[pos] = (short)(c - s * kSellRows);
       else
         static_cast<int *>(col)[pos] = c;
       sval[pos] = v;
       if (soff32) {
-        const long long p32 = base32 + (long long)kSellRows * P32 * (j / P32) + (j % P32);
+        const long long p32 = sell_pos(off32, m32, P32, lane, j);
         col32[p32] = (IT32)(std::is_same<IT32, int>::value ? c : c - s * kSellRows);
         sval32[p32] = (float)v;
       }
@@ -652,7 +655,7 @@ void plan_slices(int nrow, const int *rp, int P, SlicePlan &sp) {
       if (r >= nrow) break;
       m = std::max(m, rp[r + 1] - rp[r]);
     }
-    sp.slen[s] = (m + P - 1) / P * P;
+    sp.slen[s] = m;
     sp.soff[s] = tot;
     tot += (long long)kSellRows * sp.slen[s];
   }
@@ -898,24 +901,28 @@ int admit(hflmp_server *sv, Job &jb, int slot) {
   const int *ci = static_cast<const int *>(dp(jb.o_ci));
   const double *val = static_cast<const double *>(dp(jb.o_val));
   const long long *soff = static_cast<const long long *>(dp(jb.o_soff));
+  const int *slen = static_cast<const int *>(dp(jb.o_slen));
   void *col = dp(jb.o_col);
   double *sval = static_cast<double *>(dp(jb.o_sval));
   SlotState *dst = sv->d_states + slot;
   if (jb.nnz > 0) sys_band_kernel<<<g, kBlock, 0, s>>>(jb.nrow, rp, ci, dst);
   if (!mixed) {
-    sys_fill_kernel<int><<<g, kBlock, 0, s>>>(jb.nrow, rp, ci, val, soff, dst, col, sval,
+    sys_fill_kernel<int><<<g, kBlock, 0, s>>>(jb.nrow, rp, ci, val, soff, slen, dst, col, sval,
                                               static_cast<double *>(dp(jb.o_invd)), nullptr,
-                                              nullptr, nullptr, nullptr, sv->d_bad + slot);
+                                              nullptr, nullptr, nullptr, nullptr,
+                                              sv->d_bad + slot);
   } else if (jb.col16) {
     sys_fill_kernel<short><<<g, kBlock, 0, s>>>(
-        jb.nrow, rp, ci, val, soff, dst, col, sval, nullptr,
-        static_cast<const long long *>(dp(jb.o_soff32)), static_cast<short *>(dp(jb.o_col32)),
+        jb.nrow, rp, ci, val, soff, slen, dst, col, sval, nullptr,
+        static_cast<const long long *>(dp(jb.o_soff32)),
+        static_cast<const int *>(dp(jb.o_slen32)), static_cast<short *>(dp(jb.o_col32)),
         static_cast<float *>(dp(jb.o_sval32)), static_cast<float *>(dp(jb.o_invd32)),
         sv->d_bad + slot);
   } else {
     sys_fill_kernel<int><<<g, kBlock, 0, s>>>(
-        jb.nrow, rp, ci, val, soff, dst, col, sval, nullptr,
-        static_cast<const long long *>(dp(jb.o_soff32)), static_cast<int *>(dp(jb.o_col32)),
+        jb.nrow, rp, ci, val, soff, slen, dst, col, sval, nullptr,
+        static_cast<const long long *>(dp(jb.o_soff32)),
+        static_cast<const int *>(dp(jb.o_slen32)), static_cast<int *>(dp(jb.o_col32)),
         static_cast<float *>(dp(jb.o_sval32)), static_cast<float *>(dp(jb.o_invd32)),
         sv->d_bad + slot);
   }
