@@ -19,10 +19,11 @@ struct SellStats {
   unsigned int bad16;      // entries whose offset from the slice start needs more than 16 bits
 };
 
-// Row lengths (a byte each) and, per slice, the longest row rounded up to P.
+// Row lengths (a byte each) and, per slice, the longest row (m; where its
+// entries sit is sell_pos's business).
 // Threads walk rows in whole waves, so a wave is always one slice.
 __global__ void __launch_bounds__(kBlock)
-    sell_lengths_kernel(int nrow, int nslice, int P, const int *__restrict__ rp,
+    sell_lengths_kernel(int nrow, int nslice, const int *__restrict__ rp,
                         unsigned char *__restrict__ rlen, int *__restrict__ slen,
                         SellStats *stats) {
   const long long total = (long long)nslice * kSellRows;
@@ -88,18 +89,25 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // Scatter of row i's entries into its lane of its slice (thread per row).
-// COLS = false refreshes the values only.
+// COLS = false refreshes the values only: a row whose length differs from
+// the one stored at build time (rlen) would be placed past its lane's
+// entries, so it is skipped and counted in *changed instead.
 template <typename T, typename IT, bool COLS>
 __global__ void __launch_bounds__(kBlock)
     sell_fill_kernel(int nrow, int P, const int *__restrict__ rp, const int *__restrict__ ci,
                      const T *__restrict__ val, const long long *__restrict__ soff,
                      const int *__restrict__ slen, IT *__restrict__ col,
-                     T *__restrict__ sval) {
+                     T *__restrict__ sval, const unsigned char *__restrict__ rlen,
+                     unsigned int *changed) {
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < nrow; i += gridDim.x * kBlock) {
     const int s = i / kSellRows, lane = i % kSellRows;
     const long long off = soff[s];
     const int m = slen[s];
     const int k0 = rp[i], len = rp[i + 1] - k0;
+    if (!COLS && len != (int)rlen[i]) {
+      atomicAdd(changed, 1u);
+      continue;
+    }
     for (int j = 0; j < len; ++j) {
       const long long pos = sell_pos(off, m, P, lane, j);
       if (COLS) {
@@ -138,6 +146,7 @@ void sell_free(hflmp_sell *m) {
   (void)hipFree(m->rlen);
   (void)hipFree(m->col);
   (void)hipFree(m->val);
+  (void)hipFree(m->check);
   if (cur >= 0 && cur != m->device) (void)hipSetDevice(cur);
   delete m;
 }
@@ -172,14 +181,15 @@ int sell_create(hflmp_ctx *ctx, int nrow, int ncol, int nnz, const int *rp, cons
   if (hipMalloc(&m->soff, sizeof(long long) * ((size_t)m->nslice + 1)) != hipSuccess ||
       hipMalloc(&m->slen, sizeof(int) * (size_t)m->nslice) != hipSuccess ||
       hipMalloc(&m->rlen, (size_t)nrow) != hipSuccess ||
+      hipMalloc(&m->check, sizeof(unsigned int)) != hipSuccess ||
       hipMalloc(&stats, sizeof(SellStats)) != hipSuccess)
     return fail(HFLMP_ENOMEM, "out of device memory");
   if ((rc = hflmp_fill_zero(ctx, stats, sizeof(SellStats))) != HFLMP_OK) return fail(rc, "fill");
   const long long lanes = (long long)m->nslice * kSellRows;
   long long g = (lanes + kBlock - 1) / kBlock;
   if (g > 65536) g = 65536;
-  sell_lengths_kernel<<<(unsigned)g, kBlock, 0, s>>>(nrow, m->nslice, m->chunk, rp, m->rlen,
-                                                     m->slen, stats);
+  sell_lengths_kernel<<<(unsigned)g, kBlock, 0, s>>>(nrow, m->nslice, rp, m->rlen, m->slen,
+                                                     stats);
   sell_scan_kernel<<<1, kFinalBlock, 0, s>>>(m->nslice, m->slen, m->soff);
   if ((flags & HFLMP_SELL_COL16) && nnz > 0)
     sell_band_kernel<<<grid_for(nrow, 1), kBlock, 0, s>>>(nrow, rp, ci, stats);
@@ -210,11 +220,11 @@ int sell_create(hflmp_ctx *ctx, int nrow, int ncol, int nnz, const int *rp, cons
     if (m->col_bytes == 2)
       sell_fill_kernel<T, short, true><<<grid_for(nrow, 1), kBlock, 0, s>>>(
           nrow, m->chunk, rp, ci, val, m->soff, m->slen, static_cast<short *>(m->col),
-          static_cast<T *>(m->val));
+          static_cast<T *>(m->val), nullptr, nullptr);
     else
       sell_fill_kernel<T, int, true><<<grid_for(nrow, 1), kBlock, 0, s>>>(
           nrow, m->chunk, rp, ci, val, m->soff, m->slen, static_cast<int *>(m->col),
-          static_cast<T *>(m->val));
+          static_cast<T *>(m->val), nullptr, nullptr);
   }
   if (hipGetLastError() != hipSuccess) return fail(HFLMP_EHIP, "launch failed");
   // the stats buffer is freed only after the stream has used it
@@ -231,9 +241,22 @@ int sell_set_values(hflmp_ctx *ctx, hflmp_sell *m, const int *rp, const T *val) 
   HFLMP_REQUIRE(ctx, m->value_bytes == (int)sizeof(T), "sell_set_values: value type mismatch");
   HFLMP_REQUIRE(ctx, m->device == ctx->device, "sell_set_values: matrix on another device");
   if (m->nnz == 0) return HFLMP_OK;
-  sell_fill_kernel<T, int, false><<<grid_for(m->nrow, 1), kBlock, 0, ctx->stream>>>(
-      m->nrow, m->chunk, rp, nullptr, val, m->soff, m->slen, nullptr, static_cast<T *>(m->val));
+  hipStream_t s = ctx->stream;
+  int rc = hflmp_fill_zero(ctx, m->check, sizeof(unsigned int));
+  if (rc) return rc;
+  sell_fill_kernel<T, int, false><<<grid_for(m->nrow, 1), kBlock, 0, s>>>(
+      m->nrow, m->chunk, rp, nullptr, val, m->soff, m->slen, nullptr, static_cast<T *>(m->val),
+      m->rlen, m->check);
   HFLMP_LAUNCH_CHECK(ctx);
+  unsigned int changed = 0;
+  HFLMP_HIP_CHECK(ctx, hipMemcpyAsync(&changed, m->check, sizeof(changed),
+                                      hipMemcpyDeviceToHost, s));
+  HFLMP_HIP_CHECK(ctx, hipStreamSynchronize(s));
+  if (changed)
+    return set_error(ctx, HFLMP_EFORMAT,
+                     "sell_set_values: %u rows differ in length from the pattern the layout "
+                     "was built for (their values were not copied; rebuild the layout)",
+                     changed);
   return HFLMP_OK;
 }
 

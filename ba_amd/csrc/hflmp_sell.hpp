@@ -34,7 +34,9 @@ struct hflmp_sell {
   int value_bytes;  // 8 or 4
   long long entries;
   long long *soff;      // nslice + 1 entry offsets
-  int *slen;            // padded length of each slice (a multiple of P)
+  int *slen;            // longest row of each slice (m): the first m - m mod P entries of a
+                        // row sit in chunks of P, the rest in one-entry-per-lane blocks
+  unsigned int *check;  // device counter of rows whose length changed (set_values)
   unsigned char *rlen;  // row lengths
   void *col;
   void *val;

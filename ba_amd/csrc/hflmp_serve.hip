@@ -51,6 +51,7 @@ namespace {
 using namespace hflmp_solver;
 
 constexpr int kFresh = -2;  // admitted; initial residual not yet formed
+constexpr int kRefused = 4;  // a column index outside 0..nrow-1: never iterated
 constexpr int kModeFp64 = 64, kModeMixed = 32;
 constexpr int kMaxOuter = 50;  // refinement steps of a mixed system
 
@@ -513,18 +514,29 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // Clears st->col16 if an entry's column is not within int16 of its slice's
-// first row (the flag starts at 1; every writer stores the same 0).
+// first row (the flag starts at 1; every writer stores the same 0), and
+// marks the system refused if a column lies outside 0..nrow-1.  It runs at
+// admission before any product: every iteration kernel skips a refused
+// system, so its out-of-range columns are never gathered.  (The row
+// pointers were checked on the host at submission, so this kernel's own
+// reads stay inside the system's arrays.)
 __global__ void __launch_bounds__(kBlock)
     sys_band_kernel(int nrow, const int *__restrict__ rp, const int *__restrict__ ci,
                     SlotState *st) {
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < nrow; i += gridDim.x * kBlock) {
     const int base = (i / kSellRows) * kSellRows;
-    bool fits = true;
+    bool fits = true, inside = true;
     for (int k = rp[i]; k < rp[i + 1]; ++k) {
-      const int dlt = ci[k] - base;
+      const int c = ci[k];
+      const int dlt = c - base;
       fits = fits && dlt >= -32768 && dlt <= 32767;
+      inside = inside && (unsigned)c < (unsigned)nrow;
     }
     if (!fits) st->col16 = 0;
+    if (!inside) {
+      st->status = kRefused;
+      st->s64.status = kRefused;
+    }
   }
 }
 
@@ -636,14 +648,14 @@ class Arena {
 
 enum JobState { kQueued = 0, kActive = 1, kDone = 2 };
 
-// SELL-64 slice plan for chunk P from the host row_ptr
+// SELL-64 slice plan (slice offsets and longest rows) from the host row_ptr
 struct SlicePlan {
   std::vector<long long> soff;
   std::vector<int> slen;
   long long entries = 0;
 };
 
-void plan_slices(int nrow, const int *rp, int P, SlicePlan &sp) {
+void plan_slices(int nrow, const int *rp, SlicePlan &sp) {
   const int ns = (nrow + kSellRows - 1) / kSellRows;
   sp.slen.assign(ns, 0);
   sp.soff.assign((size_t)ns + 1, 0);
@@ -694,10 +706,10 @@ void plan_job(Job &jb) {
   jb.nunits = (jb.nrow + kBlock - 1) / kBlock;
   jb.rlen.assign(jb.nrow, 0);
   for (int r = 0; r < jb.nrow; ++r) jb.rlen[r] = (unsigned char)(jb.rp[r + 1] - jb.rp[r]);
-  plan_slices(jb.nrow, jb.rp, SellChunk<double>::P, jb.sp64);
+  plan_slices(jb.nrow, jb.rp, jb.sp64);
   jb.col16 = 0;
   if (jb.mode == kModeMixed) {
-    plan_slices(jb.nrow, jb.rp, SellChunk<float>::P, jb.sp32);
+    plan_slices(jb.nrow, jb.rp, jb.sp32);
     bool fits = jb.ci != nullptr;
     for (int r = 0; r < jb.nrow && fits; ++r) {
       const int base = (r / kSellRows) * kSellRows;
@@ -1125,10 +1137,28 @@ int hflmp_server_step(hflmp_server *sv, int *n_done) {
   if (sv->S > 0) {
     if (sv->use_graph) {
       if (!sv->graph_exec) {
+        // a graph left by a failed instantiation is freed before recapturing;
+        // the capture is always ended, and a partial graph destroyed
+        drop_graph(sv);
         HFLMP_HIP_CHECK(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         enqueue_chunk(sv, s);
-        HFLMP_HIP_CHECK(ctx, hipStreamEndCapture(s, &sv->graph));
-        HFLMP_HIP_CHECK(ctx, hipGraphInstantiate(&sv->graph_exec, sv->graph, nullptr, nullptr, 0));
+        const hipError_t launch = hipGetLastError();
+        hipGraph_t g = nullptr;
+        const hipError_t end = hipStreamEndCapture(s, &g);
+        if (end != hipSuccess || launch != hipSuccess) {
+          if (g) (void)hipGraphDestroy(g);
+          return set_error(ctx, HFLMP_EHIP, "server: graph capture failed (%s)",
+                           hipGetErrorString(end != hipSuccess ? end : launch));
+        }
+        sv->graph = g;
+        const hipError_t inst =
+            hipGraphInstantiate(&sv->graph_exec, sv->graph, nullptr, nullptr, 0);
+        if (inst != hipSuccess) {
+          sv->graph_exec = nullptr;
+          drop_graph(sv);
+          return set_error(ctx, HFLMP_EHIP, "server: graph instantiation failed (%s)",
+                           hipGetErrorString(inst));
+        }
         ++sv->graph_captures;
       }
       HFLMP_HIP_CHECK(ctx, hipGraphLaunch(sv->graph_exec, s));
@@ -1150,8 +1180,12 @@ int hflmp_server_step(hflmp_server *sv, int *n_done) {
         keep.push_back(slot);
         continue;
       }
-      HFLMP_HIP_CHECK(ctx, hipMemcpy(jb.x_out, jb.d.x, 8 * (size_t)jb.nrow,
-                                     hipMemcpyDeviceToHost));
+      const bool refused = (st.mode == kModeMixed ? st.status : st.s64.status) == kRefused;
+      if (refused)
+        std::memset(jb.x_out, 0, 8 * (size_t)jb.nrow);
+      else
+        HFLMP_HIP_CHECK(ctx, hipMemcpy(jb.x_out, jb.d.x, 8 * (size_t)jb.nrow,
+                                       hipMemcpyDeviceToHost));
       unsigned int bad = 0;
       HFLMP_HIP_CHECK(ctx, hipMemcpy(&bad, sv->d_bad + slot, sizeof(bad), hipMemcpyDeviceToHost));
       if (bad) {

@@ -95,6 +95,11 @@ def hip_lib():
             g("csr_extract_invdiag").argtypes = [vp, i, vp, vp, vp, vp, C.POINTER(i)]
             g("pcg_jacobi").argtypes = [vp, i, i, vp, vp, vp, vp, vp, vp, i, d, d, d,
                                         vp, sz, C.POINTER(SolveInfo)]
+        L.hflmp_async_workspace_bytes.restype = sz
+        L.hflmp_async_workspace_bytes.argtypes = [i, i]
+        for s in ("f64", "f32"):
+            getattr(L, "hflmp_async_jacobi_" + s).argtypes = [
+                vp, i, i, vp, vp, vp, vp, vp, vp, i, d, d, d, vp, sz, C.POINTER(SolveInfo)]
         for s in ("f64", "f32"):
             getattr(L, "hflmp_csr_spmm_" + s).argtypes = [vp, i, i, i, i, vp, vp, vp, vp, vp]
             getattr(L, "hflmp_batched_pcg_jacobi_" + s).argtypes = [
@@ -155,6 +160,13 @@ def hip_lib():
                                              C.POINTER(vp)]
         L.hflmp_mcsgs_apply_f64.argtypes = [vp, vp, vp, vp]
         L.hflmp_mcsgs_destroy.argtypes = [vp, vp]
+        for s_ in ("f64", "f32"):
+            getattr(L, "hflmp_sgs_create_" + s_).argtypes = [vp, i, i, vp, vp, vp, vp, vp, i,
+                                                              C.POINTER(vp)]
+            getattr(L, "hflmp_sgs_apply_" + s_).argtypes = [vp, vp, vp, vp]
+        L.hflmp_sgs_get_info.argtypes = [vp, vp]
+        L.hflmp_sgs_destroy.argtypes = [vp]
+        L.hflmp_sgs_destroy.restype = None
         L.hflmp_cast_f64_to_f32.argtypes = [vp, i, vp, vp]
         L.hflmp_cast_f32_to_f64.argtypes = [vp, i, vp, vp]
         L.hflmp_cast_csr_f64_to_f32.argtypes = [vp, i, vp, vp]
@@ -468,6 +480,29 @@ def pcg_jacobi(ctx, A, invd, b, x, maxit=10000, atol=0.0, rtol=1e-8, dtol=1e6, w
     finally:
         if own:
             work.free()
+    return info.as_dict()
+
+
+class AsyncParams(C.Structure):
+    _fields_ = [("block_rows", C.c_int), ("n_global", C.c_int), ("n_block", C.c_int),
+                ("n_inner", C.c_int), ("mode", C.c_int), ("w", C.c_double)]
+
+
+def async_jacobi(ctx, A, b, x, block_rows=256, n_global=1, n_block=1, n_inner=5, mode=0, w=1.0,
+                 maxit=10000, atol=0.0, rtol=1e-8, dtol=1e6):
+    """Block-asynchronous Jacobi (hflmp_async_jacobi_*): mode 0/1 solve
+    (normal/damped), 2/3 smooth; x holds x0 on entry and the result on exit."""
+    need = ctx.L.hflmp_async_workspace_bytes(A.nrow, A.dtype.itemsize)
+    work = ctx.empty(need, np.uint8)
+    prm = AsyncParams(int(block_rows), int(n_global), int(n_block), int(n_inner), int(mode),
+                      float(w))
+    info = SolveInfo()
+    try:
+        _call(ctx, "hflmp_async_jacobi_" + _sfx(A.dtype), A.nrow, A.nnz, A.rp.ptr, A.ci.ptr,
+              A.val.ptr, b.ptr, x.ptr, C.byref(prm), int(maxit), float(atol), float(rtol),
+              float(dtol), work.ptr, work.nbytes, C.byref(info))
+    finally:
+        work.free()
     return info.as_dict()
 
 
@@ -1015,9 +1050,48 @@ class McSgs:
             self.h = None
 
 
+class SgsInfo(C.Structure):
+    _fields_ = [("nrow", C.c_int), ("num_blocks", C.c_int), ("levels_forward", C.c_int),
+                ("levels_backward", C.c_int), ("launches", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Sgs:
+    """Symmetric Gauss-Seidel in the sequential row order (num_blocks = 1)
+    or its block form (hflmp_sgs_*) on a device CSR matrix.  `host_pattern`
+    (row_ptr, col) spares the download of the pattern."""
+
+    def __init__(self, ctx, A, num_blocks=1, host_pattern=None):
+        self.ctx, self.A = ctx, A
+        self.h = C.c_void_p()
+        hrp = hci = None
+        if host_pattern is not None:
+            self._hp = (np.ascontiguousarray(host_pattern[0], np.int32),
+                        np.ascontiguousarray(host_pattern[1], np.int32))
+            hrp, hci = self._hp[0].ctypes.data, self._hp[1].ctypes.data
+        _call(ctx, "hflmp_sgs_create_" + _sfx(A.dtype), A.nrow, A.nnz, A.rp.ptr, A.ci.ptr,
+              A.val.ptr, hrp, hci, int(num_blocks), C.byref(self.h))
+
+    def apply(self, r, z):
+        _call(self.ctx, "hflmp_sgs_apply_" + _sfx(self.A.dtype), self.h, r.ptr, z.ptr)
+
+    def info(self):
+        i = SgsInfo()
+        self.ctx.check(self.ctx.L.hflmp_sgs_get_info(self.h, C.byref(i)), "sgs_get_info")
+        return i.as_dict()
+
+    def destroy(self):
+        if self.h:
+            self.ctx.L.hflmp_sgs_destroy(self.h)
+            self.h = None
+
+
 # ---- C++ lmp layer (libhflmp_lmp.so) ---------------------------------------------------
 LMP_CG, LMP_GMRES, LMP_FGMRES, LMP_MPIR, LMP_BICGSTAB = 0, 1, 2, 3, 4
 LMP_PREC_NONE, LMP_PREC_JACOBI, LMP_PREC_MCSGS, LMP_PREC_GMG = 0, 1, 2, 3
+LMP_PREC_SGS, LMP_PREC_BSGS = 4, 5
 
 
 class LmpOptions(C.Structure):
@@ -1025,7 +1099,7 @@ class LmpOptions(C.Structure):
                 ("maxit", C.c_int), ("atol", C.c_double), ("rtol", C.c_double),
                 ("dtol", C.c_double), ("basis", C.c_int), ("omega", C.c_double),
                 ("mpir_max_outer", C.c_int), ("mpir_atol", C.c_double),
-                ("mpir_rtol", C.c_double), ("solver_inner", C.c_int)]
+                ("mpir_rtol", C.c_double), ("solver_inner", C.c_int), ("sgs_blocks", C.c_int)]
 
 
 class LmpResult(C.Structure):

@@ -69,6 +69,14 @@ std::unique_ptr<PreconditionerBlockJacobiStand<LAD>> make_precond(const hflmp_lm
   } else if (o.precond == HFLMP_LMP_PREC_MCSGS) {
     p.reset(new PreconditionerBlockJacobiStand<LAD>());
     p->Init_MultiColoring_SymmetricGaussSeidel(typename LAD::DataType(o.omega));
+  } else if (o.precond == HFLMP_LMP_PREC_SGS) {
+    p.reset(new PreconditionerBlockJacobiStand<LAD>());
+    p->Init_SymmetricGaussSeidel();
+  } else if (o.precond == HFLMP_LMP_PREC_BSGS) {
+    p.reset(new PreconditionerBlockJacobiStand<LAD>());
+    p->Init_BlocksSymmetricGaussSeidel(o.sgs_blocks);
+  } else if (o.precond != HFLMP_LMP_PREC_NONE) {
+    LMP_FATAL("unknown preconditioner " << o.precond);
   }
   return p;
 }
@@ -185,6 +193,7 @@ void hflmp_lmp_default_options(hflmp_lmp_options *o) {
   o->mpir_atol = 0.0;
   o->mpir_rtol = 1e-10;
   o->solver_inner = HFLMP_LMP_CG;
+  o->sgs_blocks = 1;
 }
 
 int hflmp_lmp_solve(int device, int nrow, int nnz, const int *rp, const int *ci,

@@ -92,6 +92,14 @@ struct HipOps;
     static int sell_spmv_add(hflmp_ctx *c, const hflmp_sell *m, const T *x, T *y) {          \
       return hflmp_sell_spmv_add_##S(c, m, x, y);                                            \
     }                                                                                        \
+    static int sgs_create(hflmp_ctx *c, int nr, int nnz, const int *rp, const int *ci,       \
+                          const T *v, const int *hrp, const int *hci, int nb,                \
+                          hflmp_sgs **m) {                                                   \
+      return hflmp_sgs_create_##S(c, nr, nnz, rp, ci, v, hrp, hci, nb, m);                   \
+    }                                                                                        \
+    static int sgs_apply(hflmp_ctx *c, const hflmp_sgs *m, const T *r, T *z) {               \
+      return hflmp_sgs_apply_##S(c, m, r, z);                                                \
+    }                                                                                        \
   };
 
 HFLMP_OPS(double, f64)

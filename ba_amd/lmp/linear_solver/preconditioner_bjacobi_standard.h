@@ -1,9 +1,12 @@
 // PreconditionerBlockJacobiStand<LAD> (linear_solver/
 // preconditioner_bjacobi_standard.cc:41-196): a local lPreconditioner on
-// each process's diagonal block.  Jacobi, and symmetric Gauss-Seidel in its
-// parallel multicoloured form (the sequential row sweep of the CPU path
-// has no parallel counterpart on the device; the colour ordering is the
-// reference's lpreconditioner_mc_sgs).
+// each process's diagonal block.  Jacobi; symmetric Gauss-Seidel in the
+// sequential row order (Init_SymmetricGaussSeidel, :95-101, the operator's
+// Psgauss_seidel, level-scheduled on the device with the CPU sweep's
+// results); its block form (the lPreconditioner_BlocksSymmetricGaussSeidel
+// of lpreconditioner.h:179-200, which the reference's adapter has no Init
+// for); and the multicoloured SGS of lpreconditioner_mc_sgs, a different
+// preconditioner (colour-permuted sweeps) under its own name.
 #pragma once
 
 #include <memory>
@@ -25,7 +28,16 @@ class PreconditionerBlockJacobiStand : public Preconditioner<LAD> {
     local_.reset(new lPreconditioner_Jacobi<DataType>());
     this->state_ = false;
   }
-  void Init_SymmetricGaussSeidel() { Init_MultiColoring_SymmetricGaussSeidel(DataType(1)); }
+  void Init_SymmetricGaussSeidel() {
+    local_.reset(new lPreconditioner_SymmetricGaussSeidel<DataType>());
+    this->state_ = false;
+  }
+  void Init_BlocksSymmetricGaussSeidel(int num_blocks) {
+    auto *p = new lPreconditioner_BlocksSymmetricGaussSeidel<DataType>();
+    p->Init(num_blocks);
+    local_.reset(p);
+    this->state_ = false;
+  }
   void Init_MultiColoring_SymmetricGaussSeidel(DataType omega) {
     auto *p = new lPreconditioner_MultiColoring_SymmetricGaussSeidel<DataType>();
     p->SetRelax(omega);

@@ -86,6 +86,14 @@ class lMatrix {
   // greedy natural-order colouring of the pattern
   // (CPU_CSR_lMatrix::Multicoloring, lmatrix_csr_cpu.cc:2195-2300)
   virtual void Multicoloring(int &ncolors, int **color_sizes, int **permut_index) const = 0;
+  // out = M^-1 in, M = (D + L) D^-1 (D + U), rows in the sequential order
+  // (Psgauss_seidel, lmatrix.h:419; CPU_CSR_lMatrix, lmatrix_csr_cpu.cc:495-565)
+  virtual void Psgauss_seidel(const lVector<ValueType> &invec,
+                              lVector<ValueType> *outvec) const;
+  // the same on num_blocks row ranges swept independently (BlocksPsgauss_seidel,
+  // lmatrix.h:437; lmatrix_csr_cpu.cc:568-641)
+  virtual void BlocksPsgauss_seidel(const lVector<ValueType> &invec, lVector<ValueType> *outvec,
+                                    const int num_blocks) const;
 
  protected:
   int num_row_ = 0, num_col_ = 0, nnz_ = 0;
@@ -118,6 +126,18 @@ void lMatrix<ValueType>::ReadFile(const char *filename) {
 template <typename ValueType>
 void lMatrix<ValueType>::WriteFile(const char *filename) const {
   LMP_FATAL("lMatrix::WriteFile() does not support the format " << get_matrix_format_name());
+}
+
+template <typename ValueType>
+void lMatrix<ValueType>::Psgauss_seidel(const lVector<ValueType> &, lVector<ValueType> *) const {
+  LMP_FATAL("lMatrix::Psgauss_seidel() is not implemented for " << get_matrix_format_name());
+}
+
+template <typename ValueType>
+void lMatrix<ValueType>::BlocksPsgauss_seidel(const lVector<ValueType> &, lVector<ValueType> *,
+                                              const int) const {
+  LMP_FATAL("lMatrix::BlocksPsgauss_seidel() is not implemented for "
+            << get_matrix_format_name());
 }
 
 template <typename ValueType>

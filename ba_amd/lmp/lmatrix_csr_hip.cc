@@ -55,6 +55,8 @@ HIP_CSR_lMatrix<T>::~HIP_CSR_lMatrix() {
 
 template <typename T>
 void HIP_CSR_lMatrix<T>::drop_layout() {
+  for (auto &e : sgs_) hflmp_sgs_destroy(e.second);
+  sgs_.clear();
   if (sell_) hflmp_sell_destroy(sell_);
   sell_ = nullptr;
   sell_refused_ = false;
@@ -455,6 +457,36 @@ void HIP_CSR_lMatrix<T>::extract_invdiagelements(const int start_i, const int en
   hip_check(HipOps<T>::invdiag(hip_ctx(), this->num_row_, row_, col_, val_, all.GetBuffer(), &bad),
             "extract_invdiagelements");
   h->partial_replace_subvector(0, start_i, end_i - start_i, all);
+}
+
+template <typename T>
+const hflmp_sgs *HIP_CSR_lMatrix<T>::sgs_schedule(int num_blocks) const {
+  auto it = sgs_.find(num_blocks);
+  if (it != sgs_.end()) return it->second;
+  hflmp_sgs *m = nullptr;
+  hip_check(HipOps<T>::sgs_create(hip_ctx(), this->num_row_, this->nnz_, row_, col_, val_,
+                                  h_row_.data(), h_col_.data(), num_blocks, &m),
+            "symmetric Gauss-Seidel schedule");
+  sgs_[num_blocks] = m;
+  return m;
+}
+
+template <typename T>
+void HIP_CSR_lMatrix<T>::Psgauss_seidel(const lVector<T> &invec, lVector<T> *outvec) const {
+  BlocksPsgauss_seidel(invec, outvec, 1);
+}
+
+template <typename T>
+void HIP_CSR_lMatrix<T>::BlocksPsgauss_seidel(const lVector<T> &invec, lVector<T> *outvec,
+                                              const int num_blocks) const {
+  check_vectors(invec, outvec, "BlocksPsgauss_seidel");
+  if (this->num_row_ != this->num_col_)
+    LMP_FATAL("HIP_CSR_lMatrix::BlocksPsgauss_seidel: the matrix is not square");
+  if (num_blocks < 1) LMP_FATAL("HIP_CSR_lMatrix::BlocksPsgauss_seidel: num_blocks < 1");
+  if (this->num_row_ == 0) return;
+  hip_check(HipOps<T>::sgs_apply(hip_ctx(), sgs_schedule(num_blocks), invec.GetBuffer(),
+                                 outvec->GetBuffer()),
+            "BlocksPsgauss_seidel");
 }
 
 // Natural-order first fit: row j takes the smallest colour >= 1 that no

@@ -15,6 +15,7 @@
 // the matrix's own device memory again.
 #pragma once
 
+#include <map>
 #include <vector>
 
 #include "hflmp_hip.h"
@@ -65,6 +66,12 @@ class HIP_CSR_lMatrix : public lMatrix<ValueType> {
   void extract_invdiagelements(const int start_i, const int end_i,
                                lVector<ValueType> *vec) const override;
   void Multicoloring(int &ncolors, int **color_sizes, int **permut_index) const override;
+  // Sequential and block symmetric Gauss-Seidel on the device (hflmp_sgs_*),
+  // bit-identical to the CPU sweeps; the level schedule of each block count
+  // is built at the first call and kept until the pattern changes.
+  void Psgauss_seidel(const lVector<ValueType> &invec, lVector<ValueType> *outvec) const override;
+  void BlocksPsgauss_seidel(const lVector<ValueType> &invec, lVector<ValueType> *outvec,
+                            const int num_blocks) const override;
 
   // device arrays
   const int *row_ptr() const { return row_; }
@@ -93,6 +100,9 @@ class HIP_CSR_lMatrix : public lMatrix<ValueType> {
   int *col_ = nullptr;
   ValueType *val_ = nullptr;
   std::vector<int> h_row_, h_col_;
+  const hflmp_sgs *sgs_schedule(int num_blocks) const;
+
+  mutable std::map<int, hflmp_sgs *> sgs_;  // block count -> SGS level schedule
   mutable hflmp_sell *sell_ = nullptr;
   mutable bool sell_refused_ = false;  // the pattern does not suit SELL-64
   mutable bool sell_stale_ = false;    // values changed since the copy was made

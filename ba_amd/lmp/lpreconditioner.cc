@@ -47,6 +47,56 @@ void lPreconditioner_Jacobi<T>::Clear() {
 }
 
 template <typename T>
+lPreconditioner_SymmetricGaussSeidel<T>::lPreconditioner_SymmetricGaussSeidel() {
+  this->precond_name_ = "Symmetric Gauss-Seidel";
+}
+
+template <typename T>
+lPreconditioner_SymmetricGaussSeidel<T>::~lPreconditioner_SymmetricGaussSeidel() = default;
+
+template <typename T>
+void lPreconditioner_SymmetricGaussSeidel<T>::Build() {
+  if (!this->Operator_) LMP_FATAL("lPreconditioner_SymmetricGaussSeidel::Build: no operator");
+  this->SetState(true);
+}
+
+template <typename T>
+void lPreconditioner_SymmetricGaussSeidel<T>::ApplylPreconditioner(const lVector<T> &input,
+                                                                   lVector<T> *output) {
+  if (!this->GetState()) Build();
+  this->Operator_->Psgauss_seidel(input, output);
+}
+
+template <typename T>
+lPreconditioner_BlocksSymmetricGaussSeidel<T>::lPreconditioner_BlocksSymmetricGaussSeidel() {
+  this->precond_name_ = "Block-wise Symmetric Gauss-Seidel";
+}
+
+template <typename T>
+lPreconditioner_BlocksSymmetricGaussSeidel<T>::~lPreconditioner_BlocksSymmetricGaussSeidel() =
+    default;
+
+template <typename T>
+void lPreconditioner_BlocksSymmetricGaussSeidel<T>::Init(const int num_blocks) {
+  if (num_blocks < 1) LMP_FATAL("lPreconditioner_BlocksSymmetricGaussSeidel: num_blocks < 1");
+  num_blocks_ = num_blocks;
+}
+
+template <typename T>
+void lPreconditioner_BlocksSymmetricGaussSeidel<T>::Build() {
+  if (!this->Operator_)
+    LMP_FATAL("lPreconditioner_BlocksSymmetricGaussSeidel::Build: no operator");
+  this->SetState(true);
+}
+
+template <typename T>
+void lPreconditioner_BlocksSymmetricGaussSeidel<T>::ApplylPreconditioner(
+    const lVector<T> &input, lVector<T> *output) {
+  if (!this->GetState()) Build();
+  this->Operator_->BlocksPsgauss_seidel(input, output, num_blocks_);
+}
+
+template <typename T>
 lPreconditioner_MultiColoring_SymmetricGaussSeidel<T>::
     lPreconditioner_MultiColoring_SymmetricGaussSeidel() {
   this->precond_name_ = "Multi-Coloring / Symmetric Gauss-Seidel";
@@ -111,6 +161,10 @@ void lPreconditioner_MultiColoring_SymmetricGaussSeidel<T>::Clear() {
 
 template class lPreconditioner_Jacobi<double>;
 template class lPreconditioner_Jacobi<float>;
+template class lPreconditioner_SymmetricGaussSeidel<double>;
+template class lPreconditioner_SymmetricGaussSeidel<float>;
+template class lPreconditioner_BlocksSymmetricGaussSeidel<double>;
+template class lPreconditioner_BlocksSymmetricGaussSeidel<float>;
 template class lPreconditioner_MultiColoring_SymmetricGaussSeidel<double>;
 template class lPreconditioner_MultiColoring_SymmetricGaussSeidel<float>;
 

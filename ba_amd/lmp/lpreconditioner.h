@@ -62,6 +62,40 @@ class lPreconditioner_Jacobi : public lPreconditioner<ValueType> {
   std::unique_ptr<lVector<ValueType>> inv_D_;
 };
 
+// Symmetric Gauss-Seidel in the sequential row order: the operator's
+// Psgauss_seidel (lPreconditioner_SymmetricGaussSeidel, lpreconditioner.h:
+// 165-177, lpreconditioner.cc:230-257).  Matrix-free: Build only records the
+// operator.
+template <typename ValueType>
+class lPreconditioner_SymmetricGaussSeidel : public lPreconditioner<ValueType> {
+ public:
+  lPreconditioner_SymmetricGaussSeidel();
+  ~lPreconditioner_SymmetricGaussSeidel() override;
+  void Init() override {}
+  void Build() override;
+  void ApplylPreconditioner(const lVector<ValueType> &input,
+                            lVector<ValueType> *output) override;
+};
+
+// The same on num_blocks row ranges: the operator's BlocksPsgauss_seidel
+// (lPreconditioner_BlocksSymmetricGaussSeidel, lpreconditioner.h:179-200,
+// lpreconditioner.cc:259-293); Init() means one block.
+template <typename ValueType>
+class lPreconditioner_BlocksSymmetricGaussSeidel : public lPreconditioner<ValueType> {
+ public:
+  lPreconditioner_BlocksSymmetricGaussSeidel();
+  ~lPreconditioner_BlocksSymmetricGaussSeidel() override;
+  void Init() override { Init(1); }
+  virtual void Init(const int num_blocks);
+  void Build() override;
+  void ApplylPreconditioner(const lVector<ValueType> &input,
+                            lVector<ValueType> *output) override;
+  int num_blocks() const { return num_blocks_; }
+
+ private:
+  int num_blocks_ = 1;
+};
+
 // Multicoloured SGS with relaxation omega: colouring of the operator's
 // pattern, then the colour-ordered forward sweep, diagonal scaling and
 // backward sweep on the device (lpreconditioner_mc.cc:147-723,

@@ -224,7 +224,10 @@ int hflmp_sell_create_f64(hflmp_ctx *ctx, int nrow, int ncol, int nnz,
 int hflmp_sell_create_f32(hflmp_ctx *ctx, int nrow, int ncol, int nnz,
                           const int *row_ptr, const int *col, const float *val,
                           int flags, hflmp_sell **out);
-/* New values for the same pattern (row_ptr, val: device CSR arrays) */
+/* New values for the same pattern (row_ptr, val: device CSR arrays).
+ * Synchronises the stream; HFLMP_EFORMAT if some row's length differs from
+ * the pattern the layout was built for (those rows' values are not
+ * copied: build a new layout for a new pattern). */
 int hflmp_sell_set_values_f64(hflmp_ctx *ctx, hflmp_sell *m,
                               const int *row_ptr, const double *val);
 int hflmp_sell_set_values_f32(hflmp_ctx *ctx, hflmp_sell *m,
@@ -373,6 +376,47 @@ int hflmp_mcsgs_apply_f64(hflmp_ctx *ctx, const hflmp_mcsgs *m,
                           const double *r, double *z);
 int hflmp_mcsgs_destroy(hflmp_ctx *ctx, hflmp_mcsgs *m);
 
+/* Symmetric Gauss-Seidel in the sequential row order
+ * (CPU_CSR_lMatrix::Psgauss_seidel, lmatrix_csr_cpu.cc:495-565, applied by
+ * lPreconditioner_SymmetricGaussSeidel, lpreconditioner.cc:248-257) and its
+ * block form (BlocksPsgauss_seidel, lmatrix_csr_cpu.cc:568-641, and the
+ * CUDA platform's GPUscalar_CSR_lMatrix::BlocksPsgauss_seidel,
+ * cuda/GPUscalar_csr_lmatrix.cu:314-363; applied by
+ * lPreconditioner_BlocksSymmetricGaussSeidel, lpreconditioner.cc:285-293):
+ * z = M^-1 r with M = (D + L) D^-1 (D + U).  num_blocks row ranges of
+ * nrow / num_blocks rows (the last takes the rest; num_blocks = 1 is
+ * Psgauss_seidel) are swept independently over the entries whose column
+ * lies in the range.  Per row the reference's operations in its order:
+ * forward (rows ascending) z_i = r_i, z_i -= z_c a_ij over the lower
+ * entries in storage order up to the first diagonal entry, z_i /= a_ii;
+ * then z_i *= a_ii; backward (rows descending) z_i -= z_c a_ij over every
+ * upper entry, z_i /= a_ii (last diagonal entry).  Rows run by dependency
+ * level, so z is bit-identical to the sequential sweeps.
+ * create() analyses the pattern on the host (h_row_ptr/h_col: HOST copies
+ * of it, or both NULL to download the DEVICE arrays) and keeps the DEVICE
+ * row_ptr, col and val pointers, which must outlive the object; new values
+ * are used without a rebuild.  HFLMP_EFORMAT if a row has no diagonal
+ * entry.  r and z may be the same array. */
+typedef struct hflmp_sgs hflmp_sgs;
+typedef struct {
+  int nrow, num_blocks;
+  int levels_forward, levels_backward;  /* dependency levels of the sweeps */
+  int launches;                         /* kernel launches per apply       */
+} hflmp_sgs_info;
+int hflmp_sgs_create_f64(hflmp_ctx *ctx, int nrow, int nnz, const int *row_ptr,
+                         const int *col, const double *val,
+                         const int *h_row_ptr, const int *h_col, int num_blocks,
+                         hflmp_sgs **out);
+int hflmp_sgs_create_f32(hflmp_ctx *ctx, int nrow, int nnz, const int *row_ptr,
+                         const int *col, const float *val, const int *h_row_ptr,
+                         const int *h_col, int num_blocks, hflmp_sgs **out);
+int hflmp_sgs_apply_f64(hflmp_ctx *ctx, const hflmp_sgs *m, const double *r,
+                        double *z);
+int hflmp_sgs_apply_f32(hflmp_ctx *ctx, const hflmp_sgs *m, const float *r,
+                        float *z);
+int hflmp_sgs_get_info(const hflmp_sgs *m, hflmp_sgs_info *info);
+void hflmp_sgs_destroy(hflmp_sgs *m);
+
 /* ---- device-resident Krylov solves ----------------------------------------
  * Jacobi-preconditioned CG with the arithmetic of CG::SolvePrecond
  * (linear_solver/cg.h:165-246) and the stopping rule of IterateControl::
@@ -429,6 +473,50 @@ int hflmp_pcg_jacobi_stencil_f32(hflmp_ctx *ctx, const hflmp_stencil *A,
                                  int maxit, double atol, double rtol,
                                  double dtol, void *work, size_t work_bytes,
                                  hflmp_solve_info *info);
+
+/* ---- block-asynchronous Jacobi ---------------------------------------------
+ * AsynchronousIterationGPU (linear_solver/gpu-based/async_iter_gpu.h:32-157,
+ * async_iter_gpu.cc:493-820; kernels cuda/cuda_async_iter_kernels.h:17-283)
+ * made deterministic on one device.  Rows are cut into blocks of block_rows
+ * consecutive rows (block_rows >= nrow: one block), each in the part of
+ * the reference's device-local row set.  One launch (block iteration): for
+ * every row i of block B, c_i = b_i - sum of a_ij x_j over its entries
+ * outside B (x at the start of the launch, storage order; the reference's
+ * c = b - A_offdiag x_ghost), then n_inner synchronous sweeps over B,
+ *   tmp = c_i - sum of a_ij y_j over the entries in B with j != i,
+ *   y_i = D^-1_i tmp (kernel_async_inner_iter) or, damped,
+ *   y_i = (w D^-1_i tmp) + ((1 - w) y_i) (kernel_async_inner_damped_iter),
+ * from y = x on B.  The reference's in-place updates within a device read
+ * whatever the neighbouring rows hold at the time; here every sweep of a
+ * block reads the previous sweep, so results do not depend on scheduling.
+ * Blocks of at most 2048 rows run in one workgroup each (values in LDS);
+ * larger ones sweep through global memory with the same arithmetic.
+ * mode (SetSolveMode): 0 solve, 1 solve damped -- repeat n_global * n_block
+ * launches, then the residual r_i = b_i - sum_{j != i} a_ij x_j - a_ii x_i
+ * (kernel_compute_block_squared_residual_no_offdiag), ||r|| and
+ * IterateControl::Check on the number of inner sweeps, until it stops;
+ * 2 smooth, 3 smooth damped -- the launches once, no residual.  D^-1 = 1 /
+ * the first diagonal entry; HFLMP_ENUMERIC if one is missing or zero.
+ * x holds x0 on entry.  Workspace: hflmp_async_workspace_bytes(). */
+typedef struct {
+  int block_rows;                 /* rows per block, >= 1                   */
+  int n_global, n_block, n_inner; /* SetNumIter                             */
+  int mode;                       /* 0..3 as above                          */
+  double w;                       /* damping of modes 1 and 3 (> 0)         */
+} hflmp_async_params;
+size_t hflmp_async_workspace_bytes(int nrow, int value_bytes);
+int hflmp_async_jacobi_f64(hflmp_ctx *ctx, int nrow, int nnz,
+                           const int *row_ptr, const int *col,
+                           const double *val, const double *b, double *x,
+                           const hflmp_async_params *prm, int maxit,
+                           double atol, double rtol, double dtol, void *work,
+                           size_t work_bytes, hflmp_solve_info *info);
+int hflmp_async_jacobi_f32(hflmp_ctx *ctx, int nrow, int nnz,
+                           const int *row_ptr, const int *col, const float *val,
+                           const float *b, float *x,
+                           const hflmp_async_params *prm, int maxit,
+                           double atol, double rtol, double dtol, void *work,
+                           size_t work_bytes, hflmp_solve_info *info);
 
 /* ---- geometric multigrid ----------------------------------------------------
  * GMG (linear_solver/gmg.h:979-1230, GMGBase::SolveImpl / SolveImplLevel)
@@ -643,7 +731,10 @@ typedef struct {
 typedef struct {
   int state;              /* 0 queued, 1 active, 2 done                   */
   int iters;
-  int status;             /* as hflmp_solve_info.status once done, else -1 */
+  int status;             /* as hflmp_solve_info.status once done, else -1;
+                             4: refused, a column index lies outside
+                             0..nrow-1 (checked on the device at admission
+                             before any product; x is set to 0)           */
   double res_init, res_final;
   int zero_diagonals;     /* rows whose diagonal is missing or zero        */
   size_t footprint;       /* arena bytes while active                      */

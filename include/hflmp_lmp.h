@@ -27,6 +27,11 @@ extern "C" {
 #define HFLMP_LMP_PREC_JACOBI 1
 #define HFLMP_LMP_PREC_MCSGS 2
 #define HFLMP_LMP_PREC_GMG 3 /* hflmp_lmp_solve_gmg only */
+/* Init_SymmetricGaussSeidel: Psgauss_seidel, the sequential row order */
+#define HFLMP_LMP_PREC_SGS 4
+/* lPreconditioner_BlocksSymmetricGaussSeidel: BlocksPsgauss_seidel on
+ * sgs_blocks row ranges */
+#define HFLMP_LMP_PREC_BSGS 5
 
 typedef struct {
   int solver;         /* HFLMP_LMP_* (CG, GMRES, FGMRES, MPIR, BICGSTAB)   */
@@ -39,6 +44,7 @@ typedef struct {
   int mpir_max_outer; /* MPIR: outer corrections; inner solver = `solver_inner` */
   double mpir_atol, mpir_rtol;
   int solver_inner;   /* MPIR inner solver: CG/GMRES/FGMRES/BICGSTAB       */
+  int sgs_blocks;     /* HFLMP_LMP_PREC_BSGS: row ranges (>= 1)            */
 } hflmp_lmp_options;
 
 typedef struct {

@@ -17,6 +17,7 @@
  *   ZeroRows      linear_algebra/lmp/lmatrix_csr_cpu.cc:908-927
  *   inv diag      linear_algebra/lmp/lmatrix_csr_cpu.cc:2420-2442
  *   SGS           linear_algebra/lmp/lmatrix_csr_cpu.cc:496-565
+ *   block SGS     linear_algebra/lmp/lmatrix_csr_cpu.cc:568-641
  *   colouring     linear_algebra/lmp/lmatrix_csr_cpu.cc:2195-2300
  *   MC-SGS        linear_algebra/lmp/lpreconditioner_mc.cc:449-723,
  *                 linear_algebra/lmp/lpreconditioner_mc_sgs.cc:44-90,
@@ -40,6 +41,10 @@
  *   damped Jacobi linear_algebra/lmp/cuda/cuda_async_iter_kernels.h:116-144
  *                 (one synchronous sweep); the V-cycle itself is in
  *                 oracle.py (linear_solver/gmg.h:979-1187)
+ *   async Jacobi  linear_solver/gpu-based/async_iter_gpu.cc:493-874,
+ *                 cuda_async_iter_kernels.h:17-193 (blocks of rows, each
+ *                 launch: outside couplings from the launch's input, then
+ *                 synchronous inner sweeps; deterministic)
  *
  * Problem generator: structured Q1 (2D/3D) and Q2 (2D) Poisson/diffusion on
  * the unit square/cube, assembled element by element in lexicographic cell
@@ -110,6 +115,8 @@ int or_multicoloring(int nrow, const int *rowptr, const int *col,
   void or_jacobi_apply_##S(int n, const T *invd, const T *r, T *z);           \
   void or_sgs_apply_##S(int nrow, const int *rp, const int *ci, const T *v,   \
                         const T *r, T *z);                                    \
+  void or_block_sgs_apply_##S(int nrow, const int *rp, const int *ci,         \
+                              const T *v, int num_blocks, const T *r, T *z);  \
   void or_mcsgs_apply_##S(int nrow, const int *rp, const int *ci,             \
                           const T *v, int ncolors, const int *color_sizes,    \
                           const int *perm, const int *color, T omega,         \
@@ -134,6 +141,12 @@ int or_multicoloring(int nrow, const int *rowptr, const int *col,
                                const int *dofs, const T *cell_mats,           \
                                const T *E, const T *factor, int *rp,          \
                                int *col, T *val);                             \
+  int or_async_jacobi_##S(int nrow, const int *rp, const int *ci,             \
+                          const T *v, const T *b, T *x, int block_rows,       \
+                          int n_global, int n_block, int n_inner, int mode,   \
+                          T w, int maxit, double atol, double rtol,           \
+                          double dtol, int *iters, double *res_init,          \
+                          double *res_final);                                 \
   int or_bicgstab_##S(int nrow, const int *rp, const int *ci, const T *v,     \
                       int prec, const T *invd, int ncolors, const int *csz,   \
                       const int *perm, const int *color, const T *b, T *x,    \
@@ -148,6 +161,8 @@ OR_DECL(float, s)
 #define OR_PREC_JACOBI 1
 #define OR_PREC_SGS 2
 #define OR_PREC_MCSGS 3
+/* block SGS (BlocksPsgauss_seidel); the ncolors argument carries num_blocks */
+#define OR_PREC_BSGS 4
 
 /* casts (CPUsimple_lvector.cc:638-700) */
 void or_cast_d2s(int n, const double *x, float *y);

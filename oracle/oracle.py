@@ -14,7 +14,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "lib", "libhflmp_oracle.so")
 _lib = None
 
-PREC_NONE, PREC_JACOBI, PREC_SGS, PREC_MCSGS = 0, 1, 2, 3
+PREC_NONE, PREC_JACOBI, PREC_SGS, PREC_MCSGS, PREC_BSGS = 0, 1, 2, 3, 4
 
 _i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
 _u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
@@ -75,6 +75,7 @@ def lib():
             g("invdiag").argtypes = [C.c_int, _i32p, _i32p, fp, fp]
             g("jacobi_apply").argtypes = [C.c_int, fp, fp, fp]
             g("sgs_apply").argtypes = [C.c_int, _i32p, _i32p, fp, fp, fp]
+            g("block_sgs_apply").argtypes = [C.c_int, _i32p, _i32p, fp, C.c_int, fp, fp]
             g("mcsgs_apply").argtypes = [C.c_int, _i32p, _i32p, fp, C.c_int,
                                          _i32p, _i32p, _i32p, ct, fp, fp]
             g("cg").argtypes = [C.c_int, _i32p, _i32p, fp, C.c_int, _vp,
@@ -82,6 +83,10 @@ def lib():
                                 C.c_double, C.c_double, C.c_double,
                                 C.POINTER(C.c_int), C.POINTER(C.c_double),
                                 C.POINTER(C.c_double), _vp]
+            g("async_jacobi").argtypes = [C.c_int, _i32p, _i32p, fp, fp, fp, C.c_int, C.c_int,
+                                          C.c_int, C.c_int, C.c_int, ct, C.c_int, C.c_double,
+                                          C.c_double, C.c_double, C.POINTER(C.c_int),
+                                          C.POINTER(C.c_double), C.POINTER(C.c_double)]
             g("bicgstab").argtypes = [C.c_int, _i32p, _i32p, fp, C.c_int, _vp, C.c_int, _vp,
                                       _vp, _vp, fp, fp, C.c_int, C.c_double, C.c_double,
                                       C.c_double, C.POINTER(C.c_int), C.POINTER(C.c_double),
@@ -300,6 +305,12 @@ def sgs_apply(rp, ci, v, r):
     return z
 
 
+def block_sgs_apply(rp, ci, v, num_blocks, r):
+    z = np.empty_like(r)
+    getattr(lib(), "or_block_sgs_apply_" + _sfx(v))(len(r), rp, ci, v, int(num_blocks), r, z)
+    return z
+
+
 def mcsgs_apply(rp, ci, v, coloring, r, omega=1.0):
     nc, csz, perm, color = coloring
     z = np.empty_like(r)
@@ -321,8 +332,10 @@ def cast_s2d(x):
 
 
 # ---- solvers -------------------------------------------------------------------
-def _prec_args(rp, ci, v, prec, coloring):
+def _prec_args(rp, ci, v, prec, coloring, num_blocks=1):
     invd = invdiag(rp, ci, v) if prec == PREC_JACOBI else None
+    if prec == PREC_BSGS:
+        return invd, int(num_blocks), None, None, None
     if prec == PREC_MCSGS:
         if coloring is None:
             coloring = multicoloring(rp, ci)
@@ -332,11 +345,12 @@ def _prec_args(rp, ci, v, prec, coloring):
 
 
 def cg(rp, ci, v, b, x0=None, prec=PREC_JACOBI, maxit=10000, atol=0.0,
-       rtol=1e-8, dtol=1e6, coloring=None, history=False):
-    """CG::Solve with the reference semantics; returns (x, info dict)."""
+       rtol=1e-8, dtol=1e6, coloring=None, history=False, num_blocks=1):
+    """CG::Solve with the reference semantics; returns (x, info dict).
+    num_blocks: the block count of PREC_BSGS."""
     n = len(rp) - 1
     x = np.zeros(n, v.dtype) if x0 is None else np.array(x0, v.dtype)
-    invd, nc, csz, perm, color = _prec_args(rp, ci, v, prec, coloring)
+    invd, nc, csz, perm, color = _prec_args(rp, ci, v, prec, coloring, num_blocks)
     it, r0, rf = C.c_int(), C.c_double(), C.c_double()
     hist = np.zeros(maxit + 1, np.float64) if history else None
     st = getattr(lib(), "or_cg_" + _sfx(v))(
@@ -347,6 +361,21 @@ def cg(rp, ci, v, b, x0=None, prec=PREC_JACOBI, maxit=10000, atol=0.0,
     if history:
         info["history"] = hist[: it.value + 1].copy()
     return x, info
+
+
+def async_jacobi(rp, ci, v, b, x0=None, block_rows=256, n_global=1, n_block=1, n_inner=5,
+                 mode=0, w=1.0, maxit=10000, atol=0.0, rtol=1e-8, dtol=1e6):
+    """AsynchronousIterationGPU made deterministic (blocks of rows, outside
+    couplings from each launch's input, synchronous inner sweeps); mode 0/1
+    solve (normal/damped), 2/3 smooth.  Returns (x, info dict)."""
+    n = len(rp) - 1
+    x = np.zeros(n, v.dtype) if x0 is None else np.array(x0, v.dtype)
+    it, r0, rf = C.c_int(), C.c_double(), C.c_double()
+    st = getattr(lib(), "or_async_jacobi_" + _sfx(v))(
+        n, rp, ci, v, np.ascontiguousarray(b, v.dtype), x, int(block_rows), int(n_global),
+        int(n_block), int(n_inner), int(mode), w, int(maxit), atol, rtol, dtol, C.byref(it),
+        C.byref(r0), C.byref(rf))
+    return x, dict(status=st, iters=it.value, res_init=r0.value, res_final=rf.value)
 
 
 def bicgstab(rp, ci, v, b, x0=None, prec=PREC_NONE, maxit=10000, atol=0.0, rtol=1e-8,

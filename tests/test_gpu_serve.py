@@ -290,3 +290,28 @@ def test_serve_tune_small_sample(ctx):
         assert "window_check" in rep
         if not rep["window_check"]["kept"]:
             assert cfg["mixed_min_nnz"] == 0
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+@pytest.mark.parametrize("bad_col", [-1, "nrow"])
+def test_serve_refuses_out_of_range_column(ctx, precision, bad_col):
+    """A request whose column index lies outside 0..nrow-1 is refused on the
+    device at admission (status 4, x = 0) before any product gathers with
+    it; a healthy request in the same server is solved as usual."""
+    rp, ci, v, b = request(2, 9, 31)
+    ci_bad = ci.copy()
+    k = int(rp[40]) + 1  # an entry of an interior row
+    ci_bad[k] = -1 if bad_col == -1 else len(rp) - 1
+    good = request(2, 12, 32)
+    sv = hflmp.Server(ctx, 64 * MB, max_active=4, chunk_iters=8)
+    try:
+        out = run_all(sv, [(rp, ci_bad, v, b), good], precision=precision)
+    finally:
+        sv.close()
+    (x_bad, i_bad), (x_good, i_good) = out
+    assert i_bad["status"] == 4
+    assert not x_bad.any()
+    assert i_good["status"] == 0
+    grp, gci, gv, gb = good
+    # FP64 stops on the recursive residual, mixed on the true one
+    assert np.linalg.norm(gb - O.spmv(grp, gci, gv, x_good)) < 1.1e-8 * np.linalg.norm(gb)

@@ -307,6 +307,61 @@ def test_sgs_matches_dense_factorisation():
     np.testing.assert_allclose(M @ z, r, rtol=0, atol=1e-12 * np.abs(r).max())
 
 
+def _block_sgs_dense(A, nb):
+    """Block-diagonal M of BlocksPsgauss_seidel: on each row range of
+    nrow // nb rows (the last takes the rest), (D + L_b) D^-1 (D + U_b) of
+    the entries inside the range."""
+    n = A.shape[0]
+    step = n // nb
+    M = np.zeros_like(A)
+    for b in range(nb):
+        lo, hi = b * step, (n if b == nb - 1 else (b + 1) * step)
+        if hi <= lo:
+            continue
+        Ab = A[lo:hi, lo:hi]
+        D = np.diag(np.diag(Ab))
+        M[lo:hi, lo:hi] = (D + np.tril(Ab, -1)) @ np.linalg.inv(D) @ (D + np.triu(Ab, 1))
+    return M
+
+
+@pytest.mark.parametrize("nb", [2, 3, 7, 16])
+def test_block_sgs_matches_dense_block_factorisation(nb):
+    # 2D Q1 on 7x7 cells: 64 rows, so 3, 7 and 16 blocks leave a longer last
+    # range; the couplings across ranges are dropped
+    rp, ci, v = O.fe_system(1, 2, 7, O.fe_kappa(2, 7, 1.0, 5))
+    A = dense(rp, ci, v)
+    M = _block_sgs_dense(A, nb)
+    r = np.random.default_rng(2).uniform(-1, 1, A.shape[0])
+    z = O.block_sgs_apply(rp, ci, v, nb, r)
+    np.testing.assert_allclose(M @ z, r, rtol=0, atol=1e-12 * np.abs(r).max())
+    # a wrong block boundary (the remainder given to the first range) fails
+    n = A.shape[0]
+    if n % nb:
+        step = n // nb
+        bad = np.zeros_like(A)
+        for b in range(nb):
+            lo = 0 if b == 0 else n - (nb - b) * step
+            hi = n - (nb - 1 - b) * step
+            Ab = A[lo:hi, lo:hi]
+            D = np.diag(np.diag(Ab))
+            bad[lo:hi, lo:hi] = (D + np.tril(Ab, -1)) @ np.linalg.inv(D) @ (D + np.triu(Ab, 1))
+        assert np.abs(bad @ z - r).max() > 1e-6
+
+
+def test_block_sgs_special_cases():
+    rp, ci, v = O.fe_system(1, 3, 3, O.fe_kappa(3, 3, 1.0, 6))
+    n = len(rp) - 1
+    r = np.random.default_rng(3).uniform(-1, 1, n)
+    # one block, and more blocks than rows (step 0: the last range is all
+    # rows), are the sequential sweep
+    z1 = O.sgs_apply(rp, ci, v, r)
+    assert np.array_equal(O.block_sgs_apply(rp, ci, v, 1, r), z1)
+    assert np.array_equal(O.block_sgs_apply(rp, ci, v, n + 5, r), z1)
+    # one row per block: z_i = ((r_i / a_ii) a_ii) / a_ii, operation for operation
+    d = np.diag(dense(rp, ci, v))
+    assert np.array_equal(O.block_sgs_apply(rp, ci, v, n, r), ((r / d) * d) / d)
+
+
 def test_multicoloring_first_fit():
     rp, ci, v = O.fe_system(1, 3, 5)
     nc, csz, perm, color = O.multicoloring(rp, ci)
@@ -557,3 +612,106 @@ def test_bicgstab_small_s_branch_identity():
     b = np.linspace(-1, 2, n)
     x, io = O.bicgstab(rp, ci, np.ones(n), b, rtol=1e-12)
     assert io["iters"] == 1 and io["status"] == 0 and np.array_equal(x, b)
+
+
+# ---- block-asynchronous Jacobi (AsynchronousIterationGPU, deterministic) ----
+def _async_system(n=12, seed=7):
+    rp, ci, v = O.fe_system(1, 2, n, O.fe_kappa(2, n, 1.0, seed))
+    b = O.fe_rhs(1, 2, n, seed + 1)
+    x0 = np.random.default_rng(seed).uniform(-1, 1, len(rp) - 1)
+    return rp, ci, v, b, x0
+
+
+@pytest.mark.parametrize("w", [1.0, 0.8])
+def test_async_one_block_is_repeated_sweeps(w):
+    """One block: every launch is n_inner synchronous Jacobi sweeps of the
+    whole matrix, bit for bit (or_jacobi_damped, itself pinned against
+    x + w D^-1 (b - A x))."""
+    rp, ci, v, b, x0 = _async_system()
+    invd = O.invdiag(rp, ci, v)
+    n = len(rp) - 1
+    mode = 3 if w != 1.0 else 2
+    x, info = O.async_jacobi(rp, ci, v, b, x0, block_rows=n + 10, n_global=2, n_block=3,
+                             n_inner=4, mode=mode, w=w)
+    assert info["iters"] == 24
+    y = x0.copy()
+    for _ in range(24):
+        y = O.jacobi_damped(rp, ci, v, invd, w, b, y)
+    assert np.array_equal(x, y)  # == also treats -0 and +0 alike (w = 1: 0 * x added)
+
+
+@pytest.mark.parametrize("brows", [1, 5, 40, 100])
+def test_async_one_inner_sweep_is_one_sweep(brows):
+    """n_inner = 1 is one synchronous sweep for any blocking (the outside
+    couplings are subtracted first, so only to rounding)."""
+    rp, ci, v, b, x0 = _async_system()
+    invd = O.invdiag(rp, ci, v)
+    x, _ = O.async_jacobi(rp, ci, v, b, x0, block_rows=brows, n_inner=1, mode=3, w=0.7)
+    ref = O.jacobi_damped(rp, ci, v, invd, 0.7, b, x0)
+    scale = np.abs(ref).max()
+    np.testing.assert_allclose(x, ref, rtol=0, atol=1e-14 * scale)
+
+
+def test_async_row_blocks_closed_form():
+    """block_rows = 1: every coupling is outside the block, so c_i is the
+    sweep's tmp, and k damped inner updates are y = (w d tmp) + ((1-w) y)
+    applied k times to x_i; undamped they all give d * tmp."""
+    rp, ci, v, b, x0 = _async_system(9, 3)
+    A = dense(rp, ci, v)
+    d = 1.0 / np.diag(A)
+    n = len(rp) - 1
+    tmp = np.empty(n)
+    for i in range(n):
+        t = b[i]
+        for j in range(rp[i], rp[i + 1]):
+            if ci[j] != i:
+                t -= v[j] * x0[ci[j]]
+        tmp[i] = t
+    x, _ = O.async_jacobi(rp, ci, v, b, x0, block_rows=1, n_inner=3, mode=3, w=0.6)
+    y = x0.copy()
+    for _ in range(3):
+        y = (0.6 * d * tmp) + ((1.0 - 0.6) * y)
+    assert np.array_equal(x, y)
+    x, _ = O.async_jacobi(rp, ci, v, b, x0, block_rows=1, n_inner=3, mode=2)
+    assert np.array_equal(x, d * tmp)
+    # a block that took its own row's coupling as outside (or the
+    # neighbours' as inside) would differ
+    x2, _ = O.async_jacobi(rp, ci, v, b, x0, block_rows=2, n_inner=3, mode=2)
+    assert not np.allclose(x2, d * tmp)
+
+
+def test_async_two_blocks_dense():
+    """Two blocks: each block iterates D_B^-1 (b_B - A_{B,out} x_out -
+    (A_BB - D_B) y) from y = x_B, with x_out frozen at the launch input."""
+    rp, ci, v, b, x0 = _async_system(10, 4)
+    A = dense(rp, ci, v)
+    n = A.shape[0]
+    h = n // 2
+    x, _ = O.async_jacobi(rp, ci, v, b, x0, block_rows=h + 1, n_inner=6, mode=2)
+    ref = np.empty(n)
+    for lo, hi in ((0, h + 1), (h + 1, n)):
+        B = slice(lo, hi)
+        out = np.ones(n, bool)
+        out[B] = False
+        c = b[B] - A[B][:, out] @ x0[out]
+        ABB = A[B, B]
+        D = np.diag(ABB)
+        y = x0[B].copy()
+        for _ in range(6):
+            y = (c - (ABB - np.diag(D)) @ y) / D
+        ref[B] = y
+    np.testing.assert_allclose(x, ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+
+
+def test_async_solve_converges_and_residual():
+    rp, ci, v, b, x0 = _async_system(16, 5)
+    A = dense(rp, ci, v)
+    x, info = O.async_jacobi(rp, ci, v, b, None, block_rows=64, n_inner=3, mode=1, w=0.9,
+                             rtol=1e-6, maxit=20000)
+    assert info["status"] == 0 and info["iters"] % 3 == 0
+    # the initial residual is ||b|| from x0 = 0, the final one ||b - A x||
+    assert info["res_init"] == pytest.approx(np.linalg.norm(b), rel=1e-14)
+    assert info["res_final"] == pytest.approx(np.linalg.norm(b - A @ x), rel=1e-9)
+    assert info["res_final"] < 1e-6 * info["res_init"]
+    xs = np.linalg.solve(A, b)
+    assert np.linalg.norm(x - xs) < 1e-4 * np.linalg.norm(xs)
