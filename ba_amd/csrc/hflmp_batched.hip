@@ -15,6 +15,7 @@
 // to A times column c.  Rows are walked so that the waves of one XCD cover a
 // contiguous band of rows, whose X rows stay in that XCD's L2.
 #include <chrono>
+#include <cmath>
 
 #include "hflmp_solver_common.hpp"
 
@@ -95,23 +96,130 @@ __device__ __forceinline__ T spmm_row(long long i, int k, const int *__restrict_
   return s;
 }
 
+// A wave's walk over its rows with the next row's entries loaded while the
+// current row's X rows are gathered.  The row pointers come one row further
+// ahead, so every load of a step is issued without waiting for another:
+// the step's entries (lane e holds entry e of the row, e < 64) and the
+// following row's pointer.
+template <typename T>
+struct RowPipe {
+  long long i;     // current row
+  int a, cnt;      // its first entry and length
+  int c;           // lane e: column of entry e (e < 64)
+  T v;             //          value of entry e
+  int a1, cnt1;    // the next row's first entry and length (0 past the end)
+};
+
+template <typename T>
+__device__ __forceinline__ void pipe_entries(int a, int cnt, const int *__restrict__ ci,
+                                             const T *__restrict__ val, int &c, T &v) {
+  const int lane = threadIdx.x & (kWave - 1);
+  c = 0;
+  v = T(0);
+  if (lane < cnt && lane < kWave) {
+    c = ld_stream(ci + a + lane);
+    v = ld_stream(val + a + lane);
+  }
+}
+
+__device__ __forceinline__ void pipe_ptrs(long long i, long long end, const int *__restrict__ rp,
+                                          int &a, int &cnt) {
+  a = 0;
+  cnt = 0;
+  if (i < end) {
+    a = rp[i];
+    cnt = rp[i + 1] - a;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void pipe_start(RowPipe<T> &p, const RowWalk &w,
+                                           const int *__restrict__ rp, const int *__restrict__ ci,
+                                           const T *__restrict__ val) {
+  p.i = w.first;
+  pipe_ptrs(w.first, w.end, rp, p.a, p.cnt);
+  pipe_entries(p.a, p.cnt, ci, val, p.c, p.v);
+  pipe_ptrs(w.first + w.step, w.end, rp, p.a1, p.cnt1);
+}
+
+// y_c = sum_j a_ij x[col_j][c] for lane c < k over the current row, in
+// ascending j, and the pipe advanced to the next row.  Rows of at most 32
+// entries gather all their X rows at once; longer ones in batches of 32
+// (entries past 64 through spmm_row).
+template <typename T>
+__device__ __forceinline__ T pipe_row(RowPipe<T> &p, const RowWalk &w, int k,
+                                      const int *__restrict__ rp, const int *__restrict__ ci,
+                                      const T *__restrict__ val, const T *__restrict__ X) {
+#pragma clang fp contract(off)
+  constexpr int U = 32;
+  const int lane = threadIdx.x & (kWave - 1);
+  // Load counters count in issue order, so a batch's gathers go first:
+  // waiting for them then leaves the next row's entries (issued right
+  // after them, in flight during the adds) and the pointers of the row
+  // after it pending.  Gathers come in groups of four with no lane masking
+  // (lanes >= k read column k - 1; entries past the row read the column of
+  // lane list entry 0, which pipe_entries set to 0): every load is in bounds
+  // and nothing is zeroed before its load, so nothing waits between loads.
+  int c_n = 0;
+  T v_n = 0;
+  int a2 = 0, cnt2 = 0;
+  bool fetched = false;
+  T s = 0;
+  const int lk = lane < k ? lane : k - 1;
+  if (p.cnt <= kWave) {
+    for (int base = 0; base < p.cnt; base += U) {
+      const int m = p.cnt - base;
+      T xv[U];
+#pragma unroll
+      for (int g = 0; g < U; g += 4)
+        if (g < m) {
+#pragma unroll
+          for (int u = g; u < g + 4; ++u) {
+            const int c = bcast(p.c, (base + u) & (kWave - 1));
+            xv[u] = X[(size_t)c * k + lk];
+          }
+        }
+      if (!fetched) {
+        pipe_entries(p.a1, p.cnt1, ci, val, c_n, v_n);
+        pipe_ptrs(p.i + 2 * w.step, w.end, rp, a2, cnt2);
+        fetched = true;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (u < m) s += bcast(p.v, base + u) * xv[u];
+    }
+  }
+  if (!fetched) {
+    pipe_entries(p.a1, p.cnt1, ci, val, c_n, v_n);
+    pipe_ptrs(p.i + 2 * w.step, w.end, rp, a2, cnt2);
+  }
+  if (p.cnt > kWave) s = spmm_row(p.i, k, rp, ci, val, X);
+  p.i += w.step;
+  p.a = p.a1;
+  p.cnt = p.cnt1;
+  p.c = c_n;
+  p.v = v_n;
+  p.a1 = a2;
+  p.cnt1 = cnt2;
+  return s;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
     spmm_kernel(int nrow, int k, const int *__restrict__ rp, const int *__restrict__ ci,
                 const T *__restrict__ val, const T *__restrict__ X, T *__restrict__ Y) {
   const int lane = threadIdx.x & (kWave - 1);
   const RowWalk w = row_walk(nrow);
-  for (long long i = w.first; i < w.end; i += w.step) {
-    const T s = spmm_row(i, k, rp, ci, val, X);
+  RowPipe<T> p;
+  pipe_start(p, w, rp, ci, val);
+  while (p.i < w.end) {
+    const long long i = p.i;
+    const T s = pipe_row(p, w, k, rp, ci, val, X);
     // streamed past L2, which keeps the X rows the band reuses
     if (lane < k) __builtin_nontemporal_store(s, Y + (size_t)i * k + lane);
   }
 }
 
-inline int spmm_grid(hflmp_ctx *ctx) {
-  const int cus = ctx && ctx->num_cus > 0 ? ctx->num_cus : 256;
-  return ((8 * cus + kXcds - 1) / kXcds) * kXcds;  // 8 workgroups (32 waves) per CU
-}
 
 // ---- batched Jacobi-PCG -----------------------------------------------------------
 template <typename T>
@@ -204,8 +312,11 @@ __global__ void __launch_bounds__(kBlock)
   const int lane = threadIdx.x & (kWave - 1);
   T acc[2] = {T(0), T(0)};
   const RowWalk w = row_walk(nrow);
-  for (long long i = w.first; i < w.end; i += w.step) {
-    const T s = spmm_row(i, k, rp, ci, val, X);
+  RowPipe<T> pp;
+  pipe_start(pp, w, rp, ci, val);
+  while (pp.i < w.end) {
+    const long long i = pp.i;
+    const T s = pipe_row(pp, w, k, rp, ci, val, X);
     if (lane < k) {
       const size_t e = (size_t)i * k + lane;
       const T ri = T(-1) * s + B[e];
@@ -249,8 +360,11 @@ __global__ void __launch_bounds__(kBlock)
   const int lane = threadIdx.x & (kWave - 1);
   T acc[1] = {T(0)};
   const RowWalk w = row_walk(nrow);
-  for (long long i = w.first; i < w.end; i += w.step) {
-    const T s = spmm_row(i, k, rp, ci, val, P);
+  RowPipe<T> pp;
+  pipe_start(pp, w, rp, ci, val);
+  while (pp.i < w.end) {
+    const long long i = pp.i;
+    const T s = pipe_row(pp, w, k, rp, ci, val, P);
     if (lane < k) {
       const size_t e = (size_t)i * k + lane;
       __builtin_nontemporal_store(s, AP + e);
@@ -356,6 +470,18 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+// Workgroups per CU of the row-walk kernels.  Measured on the 2D Q2 512^2
+// SpMM with 64 columns (all bit-identical): FP64 -- 8 per CU 832 us, 6 per
+// CU 956 us, 5 per CU (all resident at 92 VGPRs) 986 us, 4 per CU 875 us;
+// FP32 -- 6 per CU 490 us, 5 per CU 540 us, 8 per CU 574 us.
+inline int row_walk_blocks_per_cu(int vb) { return vb == 8 ? 8 : 6; }
+
+inline int spmm_grid(hflmp_ctx *ctx, int vb) {
+  const int cus = ctx && ctx->num_cus > 0 ? ctx->num_cus : 256;
+  const int per = row_walk_blocks_per_cu(vb);
+  return ((per * cus + kXcds - 1) / kXcds) * kXcds;
+}
+
 struct BLayout {
   size_t state, part, vec, vstride, total;
   int grid;
@@ -363,7 +489,7 @@ struct BLayout {
 
 inline BLayout b_layout(hflmp_ctx *ctx, int nrow, int k, int vb) {
   BLayout L;
-  L.grid = spmm_grid(ctx);
+  L.grid = spmm_grid(ctx, vb);
   L.state = 0;
   L.part = 4096;  // BState < 4 KiB
   L.vec = align256(L.part + 2 * (size_t)L.grid * k * vb);
@@ -424,6 +550,178 @@ int batched_pcg(hflmp_ctx *ctx, int nrow, int nnz, int k, const int *rp, const i
   return HFLMP_OK;
 }
 
+// ---- FP32-inner / FP64-refined blocked CG -------------------------------------
+// MPIR<LADescriptorCoupledD, LADescriptorCoupledS> (linear_solver/mpir.cc:
+// 52-296) applied column by column to a block of right-hand sides: per outer
+// step R = B - A X in FP64 (one SpMM over the block) with each column's
+// ||r||, the outer test per column (initial residual kept, as
+// hflmp_mpcg_jacobi), R32 = (float) R for the columns still refining (0 for
+// the others, whose inner solve then stops at once), the FP32 lockstep
+// Jacobi-PCG above on A32 C = R32 from C = 0 to inner_rtol, and
+// X += (double) C.
+
+// R = B - A X (FP64); R32 = (float) R for live columns, else 0; per-column
+// partials of r.r
+__global__ void __launch_bounds__(kBlock)
+    bmp_residual_kernel(int nrow, int k, const int *__restrict__ rp, const int *__restrict__ ci,
+                        const double *__restrict__ val, const double *__restrict__ B,
+                        const double *__restrict__ X, const int *__restrict__ live,
+                        float *__restrict__ R32, double *__restrict__ part) {
+  const int lane = threadIdx.x & (kWave - 1);
+  double acc[1] = {0.0};
+  const bool on = lane < k && live[lane];
+  const RowWalk w = row_walk(nrow);
+  RowPipe<double> pp;
+  pipe_start(pp, w, rp, ci, val);
+  while (pp.i < w.end) {
+    const long long i = pp.i;
+    const double s = pipe_row(pp, w, k, rp, ci, val, X);
+    if (lane < k) {
+      const size_t e = (size_t)i * k + lane;
+      const double ri = -1.0 * s + B[e];
+      R32[e] = on ? (float)ri : 0.0f;
+      acc[0] += ri * ri;
+    }
+  }
+  block_col_partials<double, 1>(acc, k, part);
+}
+
+__global__ void __launch_bounds__(kBlock)
+    bmp_norm_final(const double *__restrict__ part, int count, int k, double *__restrict__ rr) {
+  const double t = col_total(part, count, k, 0);
+  if (threadIdx.x == 0) rr[blockIdx.x] = t;
+}
+
+// R32 = 0 in the columns that are not live
+__global__ void __launch_bounds__(kBlock)
+    bmp_mask_kernel(long long n, int k, const int *__restrict__ live, float *__restrict__ R32) {
+  for (long long e = (long long)blockIdx.x * kBlock + threadIdx.x; e < n;
+       e += (long long)gridDim.x * kBlock)
+    if (!live[e % k]) R32[e] = 0.0f;
+}
+
+// X += (double) C for live columns
+__global__ void __launch_bounds__(kBlock)
+    bmp_correct_kernel(long long n, int k, const float *__restrict__ C,
+                       const int *__restrict__ live, double *__restrict__ X) {
+  for (long long e = (long long)blockIdx.x * kBlock + threadIdx.x; e < n;
+       e += (long long)gridDim.x * kBlock)
+    if (live[e % k]) X[e] += (double)C[e];
+}
+
+struct BmpLayout {
+  size_t inner, r32, c32, part, rr, live, total;
+};
+
+inline BmpLayout bmp_layout(hflmp_ctx *ctx, int nrow, int k) {
+  BmpLayout L;
+  const BLayout I = b_layout(ctx, nrow, k, 4);
+  const int g = spmm_grid(ctx, 8);
+  size_t o = 0;
+  L.inner = o;
+  o = align256(o + I.total);
+  L.r32 = o;
+  o = align256(o + (size_t)nrow * k * 4);
+  L.c32 = o;
+  o = align256(o + (size_t)nrow * k * 4);
+  L.part = o;
+  o = align256(o + (size_t)g * k * 8);
+  L.rr = o;
+  o = align256(o + (size_t)kMaxRhs * 8);
+  L.live = o;
+  o = align256(o + (size_t)kMaxRhs * 4);
+  L.total = o;
+  return L;
+}
+
+int batched_mpcg(hflmp_ctx *ctx, int nrow, int nnz, int k, const int *rp, const int *ci,
+                 const double *val, const float *val32, const float *invd32, const double *B,
+                 double *X, int max_outer, double atol, double rtol, int inner_maxit,
+                 double inner_rtol, void *work, size_t work_bytes, int *outer, int *inner,
+                 int *status, double *res_init, double *res_final, double *seconds) {
+  if (!ctx) return HFLMP_EINVAL;
+  HFLMP_REQUIRE(ctx, nrow > 0 && nnz > 0 && k >= 1 && k <= kMaxRhs,
+                "batched_mpcg: need nrow, nnz > 0 and 1 <= nrhs <= 64");
+  HFLMP_REQUIRE(ctx, rp && ci && val && val32 && invd32 && B && X && work,
+                "batched_mpcg: null pointer");
+  HFLMP_REQUIRE(ctx, max_outer >= 1 && atol >= 0 && rtol >= 0 && inner_maxit > 0 &&
+                         inner_rtol > 0 && inner_rtol < 1,
+                "batched_mpcg: need max_outer >= 1, inner_maxit > 0, 0 < inner_rtol < 1");
+  const BmpLayout L = bmp_layout(ctx, nrow, k);
+  HFLMP_REQUIRE(ctx, work_bytes >= L.total, "batched_mpcg: workspace %zu < %zu bytes",
+                work_bytes, L.total);
+  char *w = static_cast<char *>(work);
+  float *R32 = reinterpret_cast<float *>(w + L.r32);
+  float *C32 = reinterpret_cast<float *>(w + L.c32);
+  double *part = reinterpret_cast<double *>(w + L.part);
+  double *rr = reinterpret_cast<double *>(w + L.rr);
+  int *live = reinterpret_cast<int *>(w + L.live);
+  hipStream_t s = ctx->stream;
+  const int g = spmm_grid(ctx, 8);
+  const long long nk = (long long)nrow * k;
+  const auto t0 = std::chrono::steady_clock::now();
+  int h_live[kMaxRhs], h_out[kMaxRhs], h_in[kMaxRhs];
+  double h_rr[kMaxRhs], r0[kMaxRhs], rcur[kMaxRhs];
+  for (int c = 0; c < kMaxRhs; ++c) {
+    h_live[c] = c < k;
+    h_out[c] = h_in[c] = 0;
+  }
+  for (int step = 0;; ++step) {
+    HFLMP_HIP_CHECK(ctx, hipMemcpyAsync(live, h_live, sizeof(int) * kMaxRhs,
+                                        hipMemcpyHostToDevice, s));
+    bmp_residual_kernel<<<g, kBlock, 0, s>>>(nrow, k, rp, ci, val, B, X, live, R32, part);
+    bmp_norm_final<<<k, kBlock, 0, s>>>(part, g, k, rr);
+    HFLMP_LAUNCH_CHECK(ctx);
+    HFLMP_HIP_CHECK(ctx, hipMemcpyAsync(h_rr, rr, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+    HFLMP_HIP_CHECK(ctx, hipStreamSynchronize(s));
+    int n_live = 0;
+    for (int c = 0; c < k; ++c) {
+      if (!h_live[c]) continue;
+      const double res = std::sqrt(h_rr[c]);
+      if (step == 0) r0[c] = res;
+      rcur[c] = res;
+      // the outer test of MPIR with the initial residual kept
+      if (res <= atol || res < rtol * r0[c] || !(res == res) || h_out[c] >= max_outer) {
+        h_live[c] = 0;
+        continue;
+      }
+      ++n_live;
+    }
+    if (!n_live) break;
+    // the columns that stopped in this test: R32 = 0, so their inner solve
+    // stops at once (and X is no longer corrected)
+    HFLMP_HIP_CHECK(ctx, hipMemcpyAsync(live, h_live, sizeof(int) * kMaxRhs,
+                                        hipMemcpyHostToDevice, s));
+    bmp_mask_kernel<<<grid_for(nk, 4), kBlock, 0, s>>>(nk, k, live, R32);
+    int rc = hflmp_fill_zero(ctx, C32, sizeof(float) * (size_t)nk);
+    if (rc) return rc;
+    int it[kMaxRhs];
+    rc = batched_pcg<float>(ctx, nrow, nnz, k, rp, ci, val32, invd32, R32, C32, inner_maxit,
+                            0.0, inner_rtol, 1e6, w + L.inner, L.r32 - L.inner, it, nullptr,
+                            nullptr, nullptr, nullptr);
+    if (rc) return rc;
+    bmp_correct_kernel<<<grid_for(nk, 4), kBlock, 0, s>>>(nk, k, C32, live, X);
+    HFLMP_LAUNCH_CHECK(ctx);
+    for (int c = 0; c < k; ++c)
+      if (h_live[c]) {
+        ++h_out[c];
+        h_in[c] += it[c];
+      }
+  }
+  HFLMP_HIP_CHECK(ctx, hipStreamSynchronize(s));
+  const auto t1 = std::chrono::steady_clock::now();
+  for (int c = 0; c < k; ++c) {
+    const bool conv = rcur[c] <= atol || rcur[c] < rtol * r0[c];
+    if (outer) outer[c] = h_out[c];
+    if (inner) inner[c] = h_in[c];
+    if (status) status[c] = conv ? kConverged : (rcur[c] == rcur[c] ? kMaxIts : kBreakdown);
+    if (res_init) res_init[c] = r0[c];
+    if (res_final) res_final[c] = rcur[c];
+  }
+  if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+  return HFLMP_OK;
+}
+
 template <typename T>
 int spmm(hflmp_ctx *ctx, int nrow, int ncol, int nnz, int k, const int *rp, const int *ci,
          const T *val, const T *X, T *Y) {
@@ -432,7 +730,7 @@ int spmm(hflmp_ctx *ctx, int nrow, int ncol, int nnz, int k, const int *rp, cons
                 "spmm: bad sizes (1 <= nrhs <= 64)");
   if (nrow == 0) return HFLMP_OK;
   HFLMP_REQUIRE(ctx, rp && Y && (nnz == 0 || (ci && val && X)), "spmm: null pointer");
-  spmm_kernel<T><<<spmm_grid(ctx), kBlock, 0, ctx->stream>>>(nrow, k, rp, ci, val, X, Y);
+  spmm_kernel<T><<<spmm_grid(ctx, (int)sizeof(T)), kBlock, 0, ctx->stream>>>(nrow, k, rp, ci, val, X, Y);
   HFLMP_LAUNCH_CHECK(ctx);
   return HFLMP_OK;
 }
@@ -466,6 +764,23 @@ int hflmp_batched_pcg_jacobi_f64(hflmp_ctx *ctx, int nrow, int nnz, int nrhs, co
   return batched_pcg<double>(ctx, nrow, nnz, nrhs, rp, ci, val, invdiag, B, X, maxit, atol,
                              rtol, dtol, work, work_bytes, iters, status, res_init, res_final,
                              seconds);
+}
+
+size_t hflmp_batched_mpcg_workspace_bytes(hflmp_ctx *ctx, int nrow, int nrhs) {
+  if (nrow <= 0 || nrhs < 1 || nrhs > kMaxRhs) return 0;
+  return bmp_layout(ctx, nrow, nrhs).total;
+}
+
+int hflmp_batched_mpcg_jacobi(hflmp_ctx *ctx, int nrow, int nnz, int nrhs, const int *rp,
+                              const int *ci, const double *val, const float *val32,
+                              const float *invdiag32, const double *B, double *X, int max_outer,
+                              double atol, double rtol, int inner_maxit, double inner_rtol,
+                              void *work, size_t work_bytes, int *outer_iters,
+                              int *inner_iters, int *status, double *res_init,
+                              double *res_final, double *seconds) {
+  return batched_mpcg(ctx, nrow, nnz, nrhs, rp, ci, val, val32, invdiag32, B, X, max_outer,
+                      atol, rtol, inner_maxit, inner_rtol, work, work_bytes, outer_iters,
+                      inner_iters, status, res_init, res_final, seconds);
 }
 
 int hflmp_batched_pcg_jacobi_f32(hflmp_ctx *ctx, int nrow, int nnz, int nrhs, const int *rp,

@@ -100,6 +100,10 @@ def hip_lib():
         for s in ("f64", "f32"):
             getattr(L, "hflmp_async_jacobi_" + s).argtypes = [
                 vp, i, i, vp, vp, vp, vp, vp, vp, i, d, d, d, vp, sz, C.POINTER(SolveInfo)]
+        L.hflmp_batched_mpcg_workspace_bytes.restype = sz
+        L.hflmp_batched_mpcg_workspace_bytes.argtypes = [vp, i, i]
+        L.hflmp_batched_mpcg_jacobi.argtypes = [vp, i, i, i, vp, vp, vp, vp, vp, vp, vp, i, d, d,
+                                                i, d, vp, sz, vp, vp, vp, vp, vp, C.POINTER(d)]
         for s in ("f64", "f32"):
             getattr(L, "hflmp_csr_spmm_" + s).argtypes = [vp, i, i, i, i, vp, vp, vp, vp, vp]
             getattr(L, "hflmp_batched_pcg_jacobi_" + s).argtypes = [
@@ -510,6 +514,35 @@ def spmm(ctx, A, X, Y, nrhs):
     """Y = A X for row-major blocks of nrhs <= 64 vectors (device arrays)."""
     _call(ctx, "hflmp_csr_spmm_" + _sfx(A.dtype), A.nrow, A.ncol, A.nnz, nrhs, A.rp.ptr,
           A.ci.ptr, A.val.ptr, X.ptr, Y.ptr)
+
+
+def batched_mpcg_jacobi(ctx, A, A32, invd32, B, X, nrhs, max_outer=50, atol=0.0, rtol=1e-8,
+                        inner_maxit=100000, inner_rtol=1e-4, work=None):
+    """FP32-inner / FP64-refined Jacobi-CG on nrhs right-hand sides (MPIR per
+    column; A FP64, A32 its FP32 copy sharing the pattern, invd32 its inverse
+    diagonal); returns per-column corrections / inner iterations / status /
+    residuals and the time."""
+    need = ctx.L.hflmp_batched_mpcg_workspace_bytes(ctx.h, A.nrow, nrhs)
+    own = work is None
+    if own:
+        work = ctx.empty(need, np.uint8)
+    out = np.zeros(nrhs, np.int32)
+    inn = np.zeros(nrhs, np.int32)
+    st = np.zeros(nrhs, np.int32)
+    r0 = np.zeros(nrhs)
+    rf = np.zeros(nrhs)
+    sec = C.c_double()
+    try:
+        _call(ctx, "hflmp_batched_mpcg_jacobi", A.nrow, A.nnz, nrhs, A.rp.ptr, A.ci.ptr,
+              A.val.ptr, A32.val.ptr, invd32.ptr, B.ptr, X.ptr, int(max_outer), float(atol),
+              float(rtol), int(inner_maxit), float(inner_rtol), work.ptr, work.nbytes,
+              out.ctypes.data, inn.ctypes.data, st.ctypes.data, r0.ctypes.data,
+              rf.ctypes.data, C.byref(sec))
+    finally:
+        if own:
+            work.free()
+    return dict(outer=out, inner_iters=inn, status=st, res_init=r0, res_final=rf,
+                seconds=sec.value)
 
 
 def batched_pcg_jacobi(ctx, A, invd, B, X, nrhs, maxit=10000, atol=0.0, rtol=1e-8, dtol=1e6,

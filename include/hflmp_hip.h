@@ -640,6 +640,30 @@ int hflmp_batched_pcg_jacobi_f32(hflmp_ctx *ctx, int nrow, int nnz, int nrhs,
                                  int *status, double *res_init,
                                  double *res_final, double *seconds);
 
+/* The same refinement for a block of nrhs <= 64 right-hand sides (B, X
+ * row-major FP64; no reference counterpart for the block): MPIR<
+ * LADescriptorCoupledD, LADescriptorCoupledS> (linear_solver/mpir.cc:
+ * 52-296) per column.  Each outer step forms R = B - A X in FP64 (one SpMM
+ * over the block) and every column's ||r||; a column stops when ||r|| <=
+ * atol or < rtol ||r_0|| (its initial residual) or after max_outer
+ * corrections; the others get R32 = (float) R and the FP32 lockstep
+ * Jacobi-PCG of hflmp_batched_pcg_jacobi_f32 (val32, invdiag32) solves
+ * A32 C = R32 from C = 0 to inner_rtol (at most inner_maxit iterations),
+ * then X += (double) C.  Per-column outputs (host arrays of nrhs, any may
+ * be NULL): corrections, inner iterations over all corrections, status,
+ * initial and final FP64 residual norms.  Workspace:
+ * hflmp_batched_mpcg_workspace_bytes(). */
+size_t hflmp_batched_mpcg_workspace_bytes(hflmp_ctx *ctx, int nrow, int nrhs);
+int hflmp_batched_mpcg_jacobi(hflmp_ctx *ctx, int nrow, int nnz, int nrhs,
+                              const int *row_ptr, const int *col,
+                              const double *val, const float *val32,
+                              const float *invdiag32, const double *B,
+                              double *X, int max_outer, double atol,
+                              double rtol, int inner_maxit, double inner_rtol,
+                              void *work, size_t work_bytes, int *outer_iters,
+                              int *inner_iters, int *status, double *res_init,
+                              double *res_final, double *seconds);
+
 /* FP32-inner / FP64-refined Jacobi-CG: the refinement of
  * MPIR<LADescriptorCoupledD, LADescriptorCoupledS> (linear_solver/mpir.cc:
  * 52-296) with every step on the device.  Per outer step: r = b - A x and

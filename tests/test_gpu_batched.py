@@ -80,3 +80,50 @@ def test_batched_pcg_fp32(ctx):
         xo, io = O.cg(rp, ci, v32, np.ascontiguousarray(B[:, c]), prec=O.PREC_JACOBI, rtol=1e-5)
         assert abs(info["iters"][c] - io["iters"]) <= 2
         assert np.linalg.norm(X[:, c] - xo) <= 1e-4 * np.linalg.norm(xo)
+
+
+@pytest.mark.parametrize("lengths", [(32, 33, 64, 65, 0, 1, 31)])
+def test_spmm_batch_boundaries(ctx, lengths):
+    # rows at the edges of the gather batches (32 entries at once, a second
+    # batch up to 64, the generic loop past 64) and empty rows
+    from tests.helpers import random_csr
+    L = np.array(lengths * 20)
+    rp, ci, v = random_csr(len(L), 300, L, seed=4)
+    k = 64
+    X = np.random.default_rng(5).uniform(-1, 1, (300, k))
+    A = hflmp.CsrMatrix(ctx, rp, ci, v, ncol=300)
+    dX, dY = ctx.upload(X.ravel()), ctx.empty(len(L) * k, np.float64)
+    hflmp.spmm(ctx, A, dX, dY, k)
+    Y = dY.download().reshape(len(L), k)
+    for c in (0, 31, 63):
+        assert np.array_equal(Y[:, c], O.spmv(rp, ci, v, np.ascontiguousarray(X[:, c])))
+
+
+@pytest.mark.parametrize("k", [16, 5])
+def test_batched_mixed_vs_mpir_oracle(ctx, k):
+    """FP32-inner / FP64-refined blocked CG against the oracle's MPIR
+    (or_mpir_ds: FP64 residual, FP32 Jacobi-PCG correction) column by column:
+    corrections within 1, the true FP64 residual below 1e-8, and x within
+    1e-6 of the oracle's and of the FP64 CPU CG solve."""
+    rp, ci, v = hflmp.fe_system(2, 2, 24, hflmp.fe_kappa(2, 24, 1.0, 41))
+    N = len(rp) - 1
+    B = np.stack([hflmp.fe_rhs(2, 2, 24, 100 + c) for c in range(k)], axis=1)
+    A = hflmp.CsrMatrix(ctx, rp, ci, v)
+    A32 = A.astype_f32()
+    invd32 = hflmp.invdiag(ctx, A32)
+    dB, dX = ctx.upload(np.ascontiguousarray(B).ravel()), ctx.zeros(N * k, np.float64)
+    info = hflmp.batched_mpcg_jacobi(ctx, A, A32, invd32, dB, dX, k, rtol=1e-8,
+                                     inner_rtol=1e-4)
+    X = dX.download().reshape(N, k)
+    for c in range(k):
+        b = np.ascontiguousarray(B[:, c])
+        xo, oo = O.mpir(rp, ci, v, b, max_outer=50, rtol=1e-8, inner_rtol=1e-4,
+                        inner_maxit=100000)
+        assert info["status"][c] == 0
+        assert abs(int(info["outer"][c]) - oo["outer"]) <= 1, (info["outer"][c], oo)
+        xg = np.ascontiguousarray(X[:, c])
+        r = b - O.spmv(rp, ci, v, xg)
+        assert np.linalg.norm(r) < 1e-8 * np.linalg.norm(b)
+        assert np.linalg.norm(xg - xo) <= 1e-6 * np.linalg.norm(xo)
+        xc, _ = O.cg(rp, ci, v, b, prec=O.PREC_JACOBI, rtol=1e-10, maxit=100000)
+        assert np.linalg.norm(xg - xc) <= 1e-6 * np.linalg.norm(xc)
