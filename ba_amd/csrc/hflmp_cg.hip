@@ -22,6 +22,7 @@
 // matrix-free StencilOp, hflmp_stencil.hpp); each gives every thread of a
 // workgroup one row.  The CSR and SELL-64 products are identical.
 #include <chrono>
+#include <mutex>
 
 #include "hflmp_sell.hpp"
 #include "hflmp_solver_common.hpp"
@@ -164,16 +165,20 @@ __global__ void __launch_bounds__(kBlock)
 // The partial-sum area of the workspace is sized for the CSR tile grid,
 // which is never smaller than the SELL grid (R <= 256 rows per tile vs 256
 // rows per SELL workgroup), so one workspace size serves both operators.
+// `layout` (optional) places the state, partials and vectors differently
+// in `work` (the grids and so the results are the same); `events` false
+// polls without creating events (run_polled_sync).
 template <typename Op, typename T = typename Op::value_type>
 int pcg_jacobi(hflmp_ctx *ctx, const Op &op, int nnz, const T *invd, const T *b, T *x,
                int maxit, double atol, double rtol, double dtol, void *work,
-               size_t work_bytes, hflmp_solve_info *info) {
+               size_t work_bytes, hflmp_solve_info *info, const SolverLayout *layout = nullptr,
+               bool events = true) {
   const int nrow = op.nrow;
   HFLMP_REQUIRE(ctx, nrow > 0 && nnz > 0, "pcg: empty system");
   HFLMP_REQUIRE(ctx, invd && b && x && work && info, "pcg: null pointer");
   HFLMP_REQUIRE(ctx, maxit > 0 && atol >= 0 && rtol >= 0 && dtol > 1,
                 "pcg: bad control parameters (maxit>0, atol>=0, rtol>=0, dtol>1)");
-  const SolverLayout L = solver_layout(nrow, nnz, sizeof(T), 4);
+  const SolverLayout L = layout ? *layout : solver_layout(nrow, nnz, sizeof(T), 4);
   const int grid1 = op_grid(op);
   HFLMP_REQUIRE(ctx, work_bytes >= L.total, "pcg: workspace %zu < %zu bytes", work_bytes,
                 L.total);
@@ -203,7 +208,8 @@ int pcg_jacobi(hflmp_ctx *ctx, const Op &op, int nnz, const T *invd, const T *b,
     cg_beta_final<T><<<1, kFinalBlock, 0, s>>>(st, partials, g2);
     cg_p_kernel<T><<<g3, kBlock, 0, s>>>(nrow, z, p, st);
   };
-  int rc = run_polled(ctx, maxit, &st->status, iteration);
+  int rc = events ? run_polled(ctx, maxit, &st->status, iteration)
+                  : run_polled_sync(ctx, maxit, &st->status, iteration);
   if (rc) return rc;
   CgState<T> h;
   HFLMP_HIP_CHECK(ctx, hipMemcpyAsync(&h, st, sizeof(h), hipMemcpyDeviceToHost, s));
@@ -538,6 +544,95 @@ int pcg_jacobi_sell(hflmp_ctx *ctx, const hflmp_sell *A, const T *invd, const T 
                     work, work_bytes, info);
 }
 
+// ---- the allocation-free Jacobi-PCG -----------------------------------------
+// For callers that cannot allocate device memory (code under HIP graph
+// capture, serving loops, harnesses that forbid hipMalloc): the workspace of
+// pcg_jacobi over a SELL-64 operator lives in `scratch`, a static device
+// array of the library sized for kNoAllocMaxRows rows.  Every GPU holds its
+// own copy at its own address, resolved once per device by
+// hipGetSymbolAddress; the array's name is never passed to a kernel.  All
+// calls on one GPU share that GPU's copy, so calls that overlap on one GPU
+// (two streams, two threads) race.  Nothing in it needs clearing: each call
+// writes the state, partials and vectors before reading them.
+constexpr int kNoAllocMaxRows = 1 << 18;
+constexpr int kMaxDevices = 64;
+
+SolverLayout noalloc_layout(int nrow) {
+  // the grids of solver_layout / sell_grid for this row count; only the
+  // partial area is sized for the SELL grid instead of the CSR tile grid
+  SolverLayout L;
+  L.R = 0;
+  L.grid1 = sell_grid(nrow);
+  long long g2 = ((long long)nrow + 4LL * kBlock - 1) / (4LL * kBlock);
+  if (g2 > kMaxRedBlocks) g2 = kMaxRedBlocks;
+  if (g2 < 1) g2 = 1;
+  L.grid2 = (int)g2;
+  const int gmax = L.grid1 > L.grid2 ? L.grid1 : L.grid2;
+  L.state = 0;
+  L.partials = 256;
+  L.vec = align256(L.partials + 2 * (size_t)gmax * sizeof(double));
+  L.vstride = align256((size_t)nrow * sizeof(double));
+  L.total = L.vec + 4 * L.vstride;
+  return L;
+}
+
+constexpr size_t kNoAllocBytes =
+    256 + (2 * (size_t)(((kNoAllocMaxRows + kBlock - 1) / kBlock + kXcds - 1) / kXcds * kXcds) *
+               sizeof(double) +
+           255) / 256 * 256 +
+    4 * (((size_t)kNoAllocMaxRows * sizeof(double) + 255) / 256 * 256);
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// The address of `scratch` on the context's device (resolved on the first
+// call for that device, with the device current).
+int scratch_on_device(hflmp_ctx *ctx, void **out) {
+  HFLMP_REQUIRE(ctx, ctx->device >= 0 && ctx->device < kMaxDevices,
+                "pcg_noalloc: device %d beyond %d", ctx->device, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[ctx->device]) {
+    int cur = -1;
+    HFLMP_HIP_CHECK(ctx, hipGetDevice(&cur));
+    if (cur != ctx->device) HFLMP_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != ctx->device) (void)hipSetDevice(cur);
+    if (e != hipSuccess || !p)
+      return set_error(ctx, HFLMP_EHIP, "pcg_noalloc: hipGetSymbolAddress(scratch): %s",
+                       hipGetErrorString(e));
+    g_scratch_ptr[ctx->device] = p;
+  }
+  *out = g_scratch_ptr[ctx->device];
+  return HFLMP_OK;
+}
+
+int pcg_jacobi_sell_noalloc(hflmp_ctx *ctx, const hflmp_sell *A, const double *invd,
+                            const double *b, double *x, int maxit, double atol, double rtol,
+                            double dtol, hflmp_solve_info *info) {
+  if (!ctx) return HFLMP_EINVAL;
+  HFLMP_REQUIRE(ctx, A != nullptr, "pcg_noalloc: null matrix");
+  HFLMP_REQUIRE(ctx, A->value_bytes == 8, "pcg_noalloc: the matrix must hold FP64 values");
+  HFLMP_REQUIRE(ctx, A->device == ctx->device, "pcg_noalloc: matrix on another device");
+  HFLMP_REQUIRE(ctx, A->nrow <= kNoAllocMaxRows,
+                "pcg_noalloc: %d rows exceed the %d rows its static workspace holds "
+                "(use hflmp_pcg_jacobi_sell_f64 with a workspace)",
+                A->nrow, kNoAllocMaxRows);
+  const SolverLayout L = noalloc_layout(A->nrow);
+  HFLMP_REQUIRE(ctx, L.total <= kNoAllocBytes, "pcg_noalloc: workspace %zu > %zu bytes",
+                L.total, kNoAllocBytes);
+  void *work = nullptr;
+  int rc = scratch_on_device(ctx, &work);
+  if (rc) return rc;
+  if (A->col_bytes == 2)
+    return pcg_jacobi(ctx, sell_view<double, short>(A), A->nnz, invd, b, x, maxit, atol, rtol,
+                      dtol, work, kNoAllocBytes, info, &L, false);
+  return pcg_jacobi(ctx, sell_view<double, int>(A), A->nnz, invd, b, x, maxit, atol, rtol,
+                    dtol, work, kNoAllocBytes, info, &L, false);
+}
+
 // One-entry rows give 256-row tiles in solver_layout, the stencil grid.
 inline int stencil_layout_nnz(long long nnode) { return (int)nnode; }
 
@@ -592,6 +687,12 @@ int hflmp_pcg_jacobi_sell_f64(hflmp_ctx *ctx, const hflmp_sell *A, const double 
                               hflmp_solve_info *info) {
   return pcg_jacobi_sell<double>(ctx, A, invd, b, x, maxit, atol, rtol, dtol, work, work_bytes,
                                  info);
+}
+
+int hflmp_pcg_jacobi_sell_noalloc_f64(hflmp_ctx *ctx, const hflmp_sell *A, const double *invd,
+                                      const double *b, double *x, int maxit, double atol,
+                                      double rtol, double dtol, hflmp_solve_info *info) {
+  return pcg_jacobi_sell_noalloc(ctx, A, invd, b, x, maxit, atol, rtol, dtol, info);
 }
 
 int hflmp_pcg_jacobi_sell_f32(hflmp_ctx *ctx, const hflmp_sell *A, const float *invd,

@@ -159,4 +159,27 @@ int run_polled(hflmp_ctx *ctx, int maxit, const int *status, F &&iteration, int 
   return HFLMP_OK;
 }
 
+// The same loop without events (nothing is created): each chunk of `poll`
+// iterations is followed by a copy of the status word into the context's
+// pinned staging and a wait for the stream, so the device idles while the
+// host reads the status and queues the next chunk.
+template <typename F>
+int run_polled_sync(hflmp_ctx *ctx, int maxit, const int *status, F &&iteration, int poll = 16,
+                    int iterate_value = kIterate) {
+  hipStream_t s = ctx->stream;
+  int *slot = reinterpret_cast<int *>(static_cast<char *>(ctx->host_pinned) + 1152);
+  int launched = 0;
+  while (launched < maxit) {
+    const int m = (maxit - launched) < poll ? (maxit - launched) : poll;
+    for (int it = 0; it < m; ++it) iteration();
+    launched += m;
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(slot, status, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      return set_error(ctx, HFLMP_EHIP, "solver: HIP error while iterating");
+    if (slot[0] != iterate_value) break;
+  }
+  return HFLMP_OK;
+}
+
 }  // namespace hflmp_solver

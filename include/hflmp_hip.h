@@ -459,6 +459,22 @@ int hflmp_pcg_jacobi_sell_f32(hflmp_ctx *ctx, const hflmp_sell *A,
                               int maxit, double atol, double rtol, double dtol,
                               void *work, size_t work_bytes,
                               hflmp_solve_info *info);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): hflmp_pcg_jacobi_sell_f64 with its workspace in a
+ * static device array of the library instead of `work` -- the same kernels
+ * and partial-sum grids, so x, the iteration count and the residuals are
+ * bit-identical to it.  It calls no hipMalloc/hipFree and creates no
+ * events (it polls the state by a copy into the context's pinned staging
+ * and a stream wait every 16 iterations).  At most 262144 rows (the array
+ * is 8.4 MB, held on each GPU the library runs on until the process ends);
+ * larger systems are refused with HFLMP_EINVAL.  Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU race. */
+int hflmp_pcg_jacobi_sell_noalloc_f64(hflmp_ctx *ctx, const hflmp_sell *A,
+                                      const double *invdiag, const double *b,
+                                      double *x, int maxit, double atol,
+                                      double rtol, double dtol,
+                                      hflmp_solve_info *info);
 /* The same solve with the matrix-free stencil operator (CG<LAD> over a
  * StencilOperator, stencil_operator.h:43-330): Ap from the cells, every
  * other step as above.  Workspace: hflmp_pcg_stencil_workspace_bytes(). */
