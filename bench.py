@@ -68,6 +68,58 @@ def host_threads():
         return os.cpu_count() or 1
 
 
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo) and its logical CPU count."""
+    name = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    name = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    if not name:
+        import platform
+        name = platform.processor() or "unknown"
+    return {"model": name, "logical_cpus": os.cpu_count(), "threads_used": host_threads()}
+
+
+def cpu_cg(rp, ci, v, b, rtol, gpu_x=None, gpu_iters=None, threads=None):
+    """Jacobi-CG on the host along the reference's CPU path: CG::SolvePrecond
+    (linear_solver/cg.h:165-246) over the CPUopenmp platform, i.e. the SpMV
+    row loop on every host thread (CPUopenmp_CSR_lMatrix::VectorMult,
+    openmp/CPUopenmp_csr_lmatrix.cc:145-165) and the vector operations on one
+    thread (CPUopenmp_lvector.cc's omp simd loops).  Timed once, from x = 0;
+    with the GPU's solution, the iteration difference and the relative
+    error of the GPU x against this solve."""
+    from oracle import oracle as O
+    th = threads or host_threads()
+    t0 = time.perf_counter()
+    x, info = O.cg(rp, ci, v, b, prec=O.PREC_JACOBI, rtol=rtol, maxit=100000, threads=th)
+    el = time.perf_counter() - t0
+    out = {"seconds_to_tol": round(el, 4), "iters": info["iters"],
+           "converged": info["status"] == 0, "spmv_threads": th, "vector_threads": 1,
+           "ms_per_iter": round(el * 1e3 / max(info["iters"], 1), 3)}
+    if gpu_x is not None:
+        out["gpu_iters"] = gpu_iters
+        out["iter_diff_gpu_vs_cpu"] = abs(gpu_iters - info["iters"])
+        out["rel_error_gpu_x_vs_cpu_x"] = float(np.linalg.norm(gpu_x - x) /
+                                               max(np.linalg.norm(x), 1e-300))
+    return out, x
+
+
+def cpu_solve_for_parity(rp, ci, v, b, rtol):
+    """A CPU FP64 Jacobi-CG solve used only as the parity reference of a
+    GPU result (not timed): SpMV and vector operations on every host thread
+    (the dot products sum per-thread blocks)."""
+    from oracle import oracle as O
+    th = host_threads()
+    x, info = O.cg(rp, ci, v, b, prec=O.PREC_JACOBI, rtol=rtol, maxit=100000, threads=th,
+                   blas_threads=th)
+    return x, info
+
+
 def cpu_baseline(rp, ci, v, x, budget_s):
     """Oracle SpMV (reference row-order semantics) on the host cores:
     OpenMP over rows with every available thread, repeated for about
@@ -164,6 +216,7 @@ def bench_cg(hflmp, ctx, A, S, Sw, b):
     invd = hflmp.invdiag(ctx, A)
     db = ctx.upload(b)
     res = {"precond": "jacobi", "rtol": 1e-8}
+    xs = None
     for name, run in (("sell", lambda dx: hflmp.pcg_jacobi_sell(ctx, S, invd, db, dx,
                                                                 maxit=20000, rtol=1e-8)),
                       ("sell_int32", lambda dx: hflmp.pcg_jacobi_sell(ctx, Sw, invd, db, dx,
@@ -173,12 +226,50 @@ def bench_cg(hflmp, ctx, A, S, Sw, b):
         for rep in range(2):
             dx = ctx.zeros(N, np.float64)
             info = run(dx)
+            if name == "sell" and rep == 1:
+                xs = dx.download()
             dx.free()
         res[name] = cg_record(info, N, nnz)
     res["sell"]["layout"] = "SELL-64, int16 columns" if S.info()["col_bytes"] == 2 \
         else "SELL-64, int32 columns"
     db.free(); invd.free()
-    return res
+    return res, xs
+
+
+def bench_c1(hflmp, n=256, seed=20241201):
+    """C1: 2D Q1 Poisson 256^2 (66k DoFs, 9-point pattern), seeded uniform
+    RHS, Jacobi-CG to 1e-8: the reference's CPU path (threaded SpMV,
+    one-thread vectors) and the sequential CPUsimple path timed on the host,
+    the device-resident solve on the GPU (second of two solves timed), with
+    the iteration difference and the x error of the GPU against the CPU."""
+    from oracle import oracle as O
+    rp, ci, v = hflmp.fe_system(1, 2, n)
+    b = hflmp.fe_rhs(1, 2, n, seed)
+    N = len(rp) - 1
+    ctx = hflmp.Context(0)
+    A = hflmp.CsrMatrix(ctx, rp, ci, v)
+    S = hflmp.SellMatrix(ctx, A, hflmp.SELL_COL16)
+    invd = hflmp.invdiag(ctx, A)
+    db = ctx.upload(b)
+    for rep in range(2):
+        dx = ctx.zeros(N, np.float64)
+        gi = hflmp.pcg_jacobi_sell(ctx, S, invd, db, dx, maxit=20000, rtol=1e-8)
+        xg = dx.download()
+        dx.free()
+    S.free(); invd.free(); db.free(); A.free()
+    ctx.close()
+    cpu, _ = cpu_cg(rp, ci, v, b, 1e-8, xg, gi["iters"])
+    t0 = time.perf_counter()
+    _, i1 = O.cg(rp, ci, v, b, prec=O.PREC_JACOBI, rtol=1e-8, maxit=20000)
+    t1 = time.perf_counter() - t0
+    return {"workload": "2D Q1 Poisson %d^2 cells (%d DoFs, %d nonzeros), FP64, Jacobi-CG "
+                        "to 1e-8, RHS seed %d" % (n, N, len(ci), seed),
+            "gpu": {"iters": gi["iters"], "seconds_to_tol": round(gi["seconds"], 5),
+                    "operator": "SELL-64, int16 columns"},
+            "cpu_reference_path": cpu,
+            "cpu_single_thread": {"seconds_to_tol": round(t1, 4), "iters": i1["iters"],
+                                  "what": "CPUsimple platform: every operation on one thread"},
+            "speedup_gpu_vs_cpu_reference_path": round(cpu["seconds_to_tol"] / gi["seconds"], 2)}
 
 
 def stencil_bytes(N, ncell, vb=8):
@@ -486,19 +577,44 @@ def bench_mixed(hflmp, n):
         a.free()
     S64.free(); S64w.free(); S32.free(); A32.val.free(); A.free()
     ctx.close()
+    # every result against the FP64 CPU solve of the same system
+    t0 = time.perf_counter()
+    xc, ic = cpu_solve_for_parity(rp, ci, v, b, 1e-10)
+    nc = float(np.linalg.norm(xc))
+    out["fp64_cpu_solve"] = {
+        "what": "FP64 Jacobi-CG to 1e-10 on the host (all threads), the reference solution "
+                "for the errors below", "iters": ic["iters"],
+        "seconds": round(time.perf_counter() - t0, 2),
+        "true_rel_residual": float(np.linalg.norm(b - O.spmv(rp, ci, v, xc))) / bn}
+    out["rel_error_vs_fp64_cpu"] = {
+        "fp64_gpu_cg": float(np.linalg.norm(x64 - xc)) / nc,
+        "mixed_restart": float(np.linalg.norm(xm - xc)) / nc,
+        "mixed_reliable": float(np.linalg.norm(xr - xc)) / nc,
+        "fp64_mg_pcg": float(np.linalg.norm(xg - xc)) / nc}
     return out
 
 
-def bench_batched(hflmp, k=64, n=512):
-    """C4: 2D Q2 512^2, 64 right-hand sides at once (lockstep Jacobi-PCG,
-    rtol 1e-8) vs the same right-hand sides solved one at a time; FP64 and
-    FP32 values / vectors."""
+def bench_batched(hflmp, k=64, n=512, ncheck_cpu=2):
+    """C4: 2D Q2 512^2, 64 right-hand sides at once (lockstep Jacobi-PCG)
+    vs the same right-hand sides solved one at a time: FP64 to 1e-8, plain
+    FP32 to 1e-5, and FP32-inner / FP64-refined (MPIR per column) to 1e-8.
+    Every column's solution is compared with the CPU FP64 solve for
+    `ncheck_cpu` sampled columns."""
     rp, ci, v = hflmp.fe_system(2, 2, n)
     N, nnz = len(rp) - 1, len(ci)
     B = np.stack([hflmp.fe_rhs(2, 2, n, BATCH_SEED0 + c) for c in range(k)], axis=1)
+    cols_cpu = [int(c) for c in np.linspace(0, k - 1, ncheck_cpu).round()]
+    xcpu = {c: cpu_solve_for_parity(rp, ci, v, np.ascontiguousarray(B[:, c]), 1e-8)[0]
+            for c in cols_cpu}
+
+    def cpu_err(X):
+        return max(float(np.linalg.norm(X[:, c] - xcpu[c]) / np.linalg.norm(xcpu[c]))
+                   for c in cols_cpu)
+
     ctx = hflmp.Context(0)
-    out = {"workload": "2D Q2 Poisson %d^2, %d right-hand sides, Jacobi, rtol 1e-8" % (n, k),
-           "nrow": N, "nnz": nnz}
+    out = {"workload": "2D Q2 Poisson %d^2, %d right-hand sides, Jacobi" % (n, k),
+           "nrow": N, "nnz": nnz,
+           "cpu_parity_sample": "columns %s solved on the host in FP64 to 1e-8" % cols_cpu}
     for dt, name in ((np.float64, "fp64"), (np.float32, "fp32")):
         rtol = 1e-8 if dt == np.float64 else 1e-5
         A = hflmp.CsrMatrix(ctx, rp, ci, v.astype(dt))
@@ -510,6 +626,7 @@ def bench_batched(hflmp, k=64, n=512):
         t_mm = time_launches(ctx, lambda: hflmp.spmm(ctx, A, dB, dX, k), 20)
         ctx.check(ctx.L.hflmp_fill_zero(ctx.h, dX.ptr, dX.nbytes), "fill")
         info = hflmp.batched_pcg_jacobi(ctx, A, invd, dB, dX, k, rtol=rtol, maxit=50000)
+        X = dX.download().reshape(N, k).astype(np.float64)
         # one at a time: the same columns through the single-vector solver
         S = hflmp.SellMatrix(ctx, A, hflmp.SELL_COL16)
         singles, t_single = [], 0.0
@@ -524,6 +641,8 @@ def bench_batched(hflmp, k=64, n=512):
         t_single_all = t_single / ncheck * k
         out[name] = {"rtol": rtol, "spmm_us": round(t_mm * 1e6, 2),
                      "spmm_GBps": round(spmm_bytes(N, nnz, k, vb) / t_mm / 1e9, 2),
+                     "spmm_frac_hbm": round(spmm_bytes(N, nnz, k, vb) / t_mm / 1e9
+                                            / HBM_PEAK_GBS, 4),
                      "iters_max": int(info["iters"].max()), "iters_min": int(info["iters"].min()),
                      "all_converged": bool((info["status"] == 0).all()),
                      "seconds_batched": round(info["seconds"], 4),
@@ -532,8 +651,37 @@ def bench_batched(hflmp, k=64, n=512):
                      "one_at_a_time_sample": "%d of the %d columns solved singly "
                                              "(SELL operator), scaled to %d" % (ncheck, k, k),
                      "single_iters": singles,
-                     "speedup_batched": round(t_single_all / info["seconds"], 3)}
+                     "speedup_batched": round(t_single_all / info["seconds"], 3),
+                     "max_rel_error_vs_fp64_cpu": cpu_err(X)}
         S.free(); dB.free(); dX.free(); invd.free(); A.free()
+    # FP32-inner / FP64-refined: FP64 residuals and outer test, FP32 lockstep
+    # inner CG, to 1e-8 on every column
+    A = hflmp.CsrMatrix(ctx, rp, ci, v)
+    A32 = A.astype_f32()
+    invd32 = hflmp.invdiag(ctx, A32)
+    dB = ctx.upload(B.ravel())
+    dX = ctx.zeros(N * k, np.float64)
+    for rep in range(2):
+        ctx.check(ctx.L.hflmp_fill_zero(ctx.h, dX.ptr, dX.nbytes), "fill")
+        mi = hflmp.batched_mpcg_jacobi(ctx, A, A32, invd32, dB, dX, k, rtol=1e-8,
+                                       inner_rtol=1e-4)
+    X = dX.download().reshape(N, k)
+    bn = np.linalg.norm(B, axis=0)
+    from oracle import oracle as O
+    tr = max(float(np.linalg.norm(B[:, c] - O.spmv(rp, ci, v, np.ascontiguousarray(X[:, c])))
+                   / bn[c]) for c in cols_cpu)
+    out["fp32_inner_fp64_refined"] = {
+        "rtol": 1e-8, "inner_rtol": 1e-4,
+        "outer_max": int(mi["outer"].max()), "inner_iters_max": int(mi["inner_iters"].max()),
+        "all_converged": bool((mi["status"] == 0).all()),
+        "seconds_batched": round(mi["seconds"], 4),
+        "solves_per_s_batched": round(k / mi["seconds"], 3),
+        "speedup_vs_fp64_batched": round(out["fp64"]["seconds_batched"] / mi["seconds"], 3),
+        "speedup_vs_fp64_one_at_a_time": round(out["fp64"]["seconds_one_at_a_time_est"]
+                                               / mi["seconds"], 3),
+        "max_true_rel_residual_cpu_sample": tr,
+        "max_rel_error_vs_fp64_cpu": cpu_err(X)}
+    dB.free(); dX.free(); invd32.free(); A32.val.free(); A.free()
     ctx.close()
     return out
 
@@ -560,6 +708,27 @@ def make_request(hflmp, spec):
     return rp, ci, v, hflmp.fe_rhs(1, dim, n, bs)
 
 
+class RequestFeed:
+    """The stream's requests generated on host threads shortly before they
+    are submitted (at most `ahead` past the last one taken), so the whole
+    stream is never held in memory at once."""
+
+    def __init__(self, hflmp, specs, threads=8, ahead=48):
+        from concurrent.futures import ThreadPoolExecutor
+        self.hflmp, self.specs, self.ahead = hflmp, specs, ahead
+        self.ex = ThreadPoolExecutor(threads)
+        self.futs, self.nxt = {}, 0
+
+    def get(self, k):
+        while self.nxt < len(self.specs) and self.nxt <= k + self.ahead:
+            self.futs[self.nxt] = self.ex.submit(make_request, self.hflmp, self.specs[self.nxt])
+            self.nxt += 1
+        return self.futs.pop(k).result()
+
+    def close(self):
+        self.ex.shutdown(wait=True)
+
+
 def fifo_latencies(arrivals, service):
     """Completion - arrival of a one-at-a-time server taking requests in
     arrival order with the measured service times."""
@@ -570,176 +739,193 @@ def fifo_latencies(arrivals, service):
     return np.array(out)
 
 
-def run_server(hflmp, ctx, reqs, arrivals, cfg, rtol, precision=64):
-    """Replay requests into a server at their arrival times (all at t = 0
-    when arrivals is None); returns per-request (x, info, latency)."""
-    sv = hflmp.Server(ctx, cfg["hbm_budget"], cfg["max_active"], cfg["chunk_iters"],
-                      cfg["use_graphs"], cfg.get("mixed_min_nnz", 0), cfg.get("inner_rtol", 1e-2),
-                      cfg.get("mixed_max_nnz", 0))
+def server_of(hflmp, ctx, cfg):
+    return hflmp.Server(ctx, cfg["hbm_budget"], cfg["max_active"], cfg["chunk_iters"],
+                        cfg["use_graphs"], cfg.get("mixed_min_nnz", 0),
+                        cfg.get("inner_rtol", 1e-2), cfg.get("mixed_max_nnz", 0))
+
+
+def run_server_stream(hflmp, ctx, feed_specs, arrivals, cfg, rtol, precision, keep,
+                      max_queued=32):
+    """The stream through one server, its requests generated as it runs
+    (RequestFeed).  arrivals None: each request is submitted as soon as
+    fewer than `max_queued` wait for admission (the server never idles for
+    lack of work: its capacity); else at its arrival time (latency from
+    arrival to the result's return).  Keeps x only for the request indices
+    in `keep`; returns (per-request info, kept x by index, latencies,
+    makespan, counters)."""
+    feed = RequestFeed(hflmp, feed_specs)
+    sv = server_of(hflmp, ctx, cfg)
     try:
-        n = len(reqs)
-        tickets, lat, out = [None] * n, [0.0] * n, [None] * n
+        n = len(feed_specs)
+        infos, lat, xs = [None] * n, np.zeros(n), {}
         by_ticket = {}
-        outstanding = set()
         nxt = 0
         t0 = time.perf_counter()
-        while nxt < n or outstanding:
+        while nxt < n or by_ticket:
             now = time.perf_counter() - t0
-            while nxt < n and (arrivals is None or arrivals[nxt] <= now):
-                t = sv.submit(*reqs[nxt], rtol=rtol, precision=precision)
-                tickets[nxt] = t
-                by_ticket[t] = nxt
-                outstanding.add(t)
+            while nxt < n and ((arrivals is None and sv.stats()["queued"] < max_queued)
+                               or (arrivals is not None and arrivals[nxt] <= now)):
+                r = feed.get(nxt)
+                by_ticket[sv.submit(*r, rtol=rtol, precision=precision)] = nxt
                 nxt += 1
-            if not outstanding:
-                time.sleep(max(0.0, min(arrivals[nxt] - now, 0.01)))
+                now = time.perf_counter() - t0
+            if not by_ticket:
+                time.sleep(max(0.0, min(arrivals[nxt] - now, 0.005)))
                 continue
             if sv.step():
                 now = time.perf_counter() - t0
-                for t in list(outstanding):
+                for t in list(by_ticket):
                     if sv.poll(t)["state"] == 2:
-                        k = by_ticket[t]
+                        k = by_ticket.pop(t)
                         x, info = sv.take(t)
-                        out[k] = (x, info)
+                        infos[k] = info
                         lat[k] = now - (0.0 if arrivals is None else arrivals[k])
-                        outstanding.discard(t)
+                        if k in keep:
+                            xs[k] = x
         makespan = time.perf_counter() - t0
         stats = sv.stats()
     finally:
         sv.close()
-    return out, np.array(lat), makespan, stats
+        feed.close()
+    return infos, xs, lat, makespan, stats
 
 
-def bench_serve(hflmp, nreq, seed, rate, ref_style, graphs=True, ntune=32):
-    """C5: a stream of FE solves (Jacobi-CG to 1e-8) through the solve
-    server vs one-at-a-time device-resident FP64 CG (and the reference's
-    host-driven CG<LAD> on this platform), throughput with every request
-    present at t = 0, latency under Poisson arrivals.  The server's
-    precision per size class and its step length come from the autotuner
-    run on a separate tuning sample; an FP64-only server run is reported
-    beside it."""
-    from concurrent.futures import ThreadPoolExecutor
+def run_singles(hflmp, ctx, reqs, rtol):
+    """One at a time, device-resident FP64 Jacobi-CG per request: upload,
+    SELL-64 build (int16 columns where they fit), Jacobi, solve, download."""
+    service, out, solve_s = [], [], 0.0
+    t0 = time.perf_counter()
+    for rp, ci, v, b in reqs:
+        ts = time.perf_counter()
+        A = hflmp.CsrMatrix(ctx, rp, ci, v)
+        S = hflmp.SellMatrix(ctx, A, hflmp.SELL_COL16)
+        invd = hflmp.invdiag(ctx, A)
+        db, dx = ctx.upload(b), ctx.zeros(A.nrow, np.float64)
+        info = hflmp.pcg_jacobi_sell(ctx, S, invd, db, dx, rtol=rtol, maxit=100000)
+        solve_s += info["seconds"]
+        x = dx.download()
+        S.free(); A.free(); invd.free(); db.free(); dx.free()
+        service.append(time.perf_counter() - ts)
+        out.append((x, info))
+    return service, out, solve_s, time.perf_counter() - t0
+
+
+def bench_serve(hflmp, nreq, seed, rate, ref_style, graphs=True, ntune=32, sample_every=20,
+                ref_every=100, load=0.7):
+    """C5: a stream of `nreq` FE solves (Jacobi-CG to 1e-8) with Poisson
+    arrivals through the solve server.  Requests are generated on host
+    threads as the server runs (never the whole stream at once).
+      * capacity: every request present as soon as the server can queue it;
+      * latency: the same stream submitted at its Poisson arrival times, at
+        `load` x the measured capacity (or `rate`);
+      * one at a time (device-resident FP64 CG) on every `sample_every`-th
+        request of the stream, and its FIFO latencies on that sub-stream at
+        the same offered rate; the reference's host-driven CG<LAD> on every
+        `ref_every`-th request;
+      * parity on the sampled requests: server x vs the single solve and the
+        true residual b - A x (computed after the timed runs).
+    The server's precision window and step length come from the autotuner
+    run on a separate tuning sample."""
     from ba_amd import serve_tune
     from oracle import oracle as O
     rtol = 1e-8
     specs, gaps = serve_stream(nreq, seed)
     tspecs, _ = serve_stream(ntune, seed + 1)
-    t0 = time.perf_counter()
-    with ThreadPoolExecutor(8) as ex:
-        reqs = list(ex.map(lambda s: make_request(hflmp, s), specs))
-        treqs = list(ex.map(lambda s: make_request(hflmp, s), tspecs))
-    t_gen = time.perf_counter() - t0
-    total_nnz = int(sum(len(r[1]) for r in reqs))
     ctx = hflmp.Context(0)
-    def run_singles():
-        """One at a time, device-resident CG (upload, SELL-64 build,
-        Jacobi, solve, download)."""
-        service, singles, solve_s = [], [], 0.0
-        t0 = time.perf_counter()
-        for rp, ci, v, b in reqs:
-            ts = time.perf_counter()
-            A = hflmp.CsrMatrix(ctx, rp, ci, v)
-            S = hflmp.SellMatrix(ctx, A, hflmp.SELL_COL16)
-            invd = hflmp.invdiag(ctx, A)
-            db, dx = ctx.upload(b), ctx.zeros(A.nrow, np.float64)
-            info = hflmp.pcg_jacobi_sell(ctx, S, invd, db, dx, rtol=rtol, maxit=100000)
-            solve_s += info["seconds"]
-            x = dx.download()
-            S.free(); A.free(); invd.free(); db.free(); dx.free()
-            service.append(time.perf_counter() - ts)
-            singles.append((x, info))
-        return service, singles, solve_s, time.perf_counter() - t0
-
+    treqs = [make_request(hflmp, s) for s in tspecs]
     base = {"hbm_budget": 32 << 30, "max_active": 256, "chunk_iters": 16, "use_graphs": graphs,
             "mixed_min_nnz": 0, "inner_rtol": 1e-2, "mixed_max_nnz": 0}
     t0 = time.perf_counter()
     cfg, tune_report = serve_tune.tune(ctx, treqs, base, rtol)
     t_tune = time.perf_counter() - t0
-    # the single solves, the FP64 server and the tuned server are each run
-    # twice, in turn, and keep their faster run: drift of the device between
-    # runs (a few percent) then touches all three alike
-    best = {}
-    for rnd in range(2):
-        for key, fn in (("single", run_singles),
-                        ("fp64", lambda: run_server(hflmp, ctx, reqs, None, base, rtol, 64)),
-                        ("tuned", lambda: run_server(hflmp, ctx, reqs, None, cfg, rtol, 0))):
-            r = fn()
-            if key not in best or r[3 if key == "single" else 2] < \
-                    best[key][3 if key == "single" else 2]:
-                best[key] = r
-    service, singles, solve_s, t_single = best["single"]
-    out64, _, makespan64, _ = best["fp64"]
-    out, _, makespan, stats = best["tuned"]
-    true_res = [float(np.linalg.norm(b - O.spmv(rp, ci, v, x)) / max(np.linalg.norm(b), 1e-300))
-                for (rp, ci, v, b), (x, _) in zip(reqs, out)]
-    err_auto = [float(np.linalg.norm(x - xs) / max(np.linalg.norm(xs), 1e-300))
-                for (x, _), (xs, _) in zip(out, singles)]
-    errs = [float(np.linalg.norm(x - xs) / max(np.linalg.norm(xs), 1e-300))
-            for (x, _), (xs, _) in zip(out64, singles)]
-    itd = [abs(i["iters"] - s["iters"]) for (_, i), (_, s) in zip(out64, singles)]
-    # algorithmic bytes of every FP64 CG iteration of the stream (fused iteration)
-    moved = sum(i["iters"] * cg_iter_bytes(len(r[0]) - 1, len(r[1]))
-                for (_, i), r in zip(out64, reqs))
-    nmixed = sum(1 for _, i in out if i["precision"] == 32)
-    res = {"stream": {"requests": nreq, "seed": seed, "rtol": rtol, "precision": "fp64",
+    del treqs
+    sample = list(range(0, nreq, sample_every))
+    keep = set(sample)
+    # capacity: the whole stream, the server never short of queued work
+    infos, xs, _, makespan, stats = run_server_stream(hflmp, ctx, specs, None, cfg, rtol, 0,
+                                                      keep)
+    capacity = nreq / makespan
+    # latency under Poisson arrivals
+    lam = rate if rate else load * capacity
+    arrivals = np.cumsum(gaps) / lam
+    linfos, _, lat, mk2, _ = run_server_stream(hflmp, ctx, specs, arrivals, cfg, rtol, 0, set())
+    # one at a time on the sample (requests regenerated; identical by seed)
+    sreqs = [make_request(hflmp, specs[k]) for k in sample]
+    service, singles, solve_s, t_single = run_singles(hflmp, ctx, sreqs, rtol)
+    single_rate = len(sample) / t_single
+    # FIFO on the sample's own arrivals, rescaled to the same offered rate
+    sub_arr = arrivals[sample] / sample_every
+    base_lat = fifo_latencies(sub_arr, service)
+    # parity on the sample
+    err = [float(np.linalg.norm(xs[k] - xsgl) / max(np.linalg.norm(xsgl), 1e-300))
+           for k, (xsgl, _) in zip(sample, singles)]
+    itd = [abs(infos[k]["iters"] - si["iters"]) for k, (_, si) in zip(sample, singles)
+           if infos[k]["precision"] == 64]
+    true_res = [float(np.linalg.norm(b - O.spmv(rp, ci, v, xs[k])) /
+                      max(np.linalg.norm(b), 1e-300))
+                for k, (rp, ci, v, b) in zip(sample, sreqs)]
+    fp64_bytes = sum(si["iters"] * cg_iter_bytes(len(r[0]) - 1, len(r[1]))
+                     for (_, si), r in zip(singles, sreqs))
+    nmixed = sum(1 for i in infos if i["precision"] == 32)
+    res = {"stream": {"requests": nreq, "seed": seed, "rtol": rtol,
                       "mix": "70% 2D Q1 n log-uniform [16,1024], 30% 3D Q1 n log-uniform "
-                             "[16,96]; log-normal diffusivity sigma 1; uniform RHS",
-                      "total_nnz": total_nnz, "gen_seconds": round(t_gen, 2),
-                      "fp64_iterations_total": int(sum(i["iters"] for _, i in out64)),
-                      "fp64_cg_bytes_total": int(moved)},
-           "timing": "single solves, FP64 server and tuned server each run twice in turn; "
-                     "the faster run of each",
+                             "[16,96]; log-normal diffusivity sigma 1; uniform RHS; "
+                             "generated on 8 host threads while the server runs",
+                      "arrivals": "Poisson (exponential gaps, seed %d)" % seed},
            "autotune": {"sample": "%d requests of the same mix, seed %d" % (ntune, seed + 1),
                         "seconds": round(t_tune, 2), "report": tune_report},
-           "server": {"config": cfg, "solves_per_s": round(nreq / makespan, 2),
+           "server": {"config": cfg, "solves_per_s": round(capacity, 2),
                       "makespan_s": round(makespan, 3), "steps": stats["steps"],
                       "graph_captures": stats["graph_captures"],
                       "mixed_requests": nmixed,
-                      "all_converged": all(i["status"] == 0 for _, i in out),
-                      "max_true_rel_residual": max(true_res),
-                      "max_rel_diff_x_vs_fp64_single": max(err_auto)},
-           "server_fp64_only": {"config": base, "solves_per_s": round(nreq / makespan64, 2),
-                                "makespan_s": round(makespan64, 3),
-                                "effective_GBps": round(moved / makespan64 / 1e9, 1),
-                                "all_converged": all(i["status"] == 0 for _, i in out64)},
-           "one_at_a_time_device": {"solves_per_s": round(nreq / t_single, 2),
-                                    "makespan_s": round(t_single, 3),
+                      "all_converged": all(i["status"] == 0 for i in infos),
+                      "iterations_total": int(sum(i["iters"] for i in infos)),
+                      "what": "capacity: every request queued as soon as fewer than 32 "
+                              "wait for admission"},
+           "latency": {"rate_per_s": round(lam, 2), "offered_load": round(lam / capacity, 3),
+                       "server_p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
+                       "server_p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2),
+                       "server_max_ms": round(float(lat.max()) * 1e3, 2),
+                       "server_solves_per_s": round(nreq / mk2, 2),
+                       "all_converged": all(i["status"] == 0 for i in linfos),
+                       "one_at_a_time_p50_ms": round(float(np.percentile(base_lat, 50)) * 1e3,
+                                                     2),
+                       "one_at_a_time_p99_ms": round(float(np.percentile(base_lat, 99)) * 1e3,
+                                                     2),
+                       "note": "server measured live over all %d requests; one at a time: "
+                               "FIFO on its measured service times over the sampled "
+                               "sub-stream at the same offered rate" % nreq},
+           "one_at_a_time_device": {"sample": "every %dth request of the stream (%d)"
+                                              % (sample_every, len(sample)),
+                                    "solves_per_s": round(single_rate, 2),
+                                    "seconds": round(t_single, 3),
                                     "solve_loop_seconds": round(solve_s, 3),
-                                    "effective_GBps": round(moved / t_single / 1e9, 1),
-                                    "all_converged": all(s["status"] == 0 for _, s in singles)},
-           "server_fp64_vs_single": {"max_rel_diff_x": max(errs), "max_iter_diff": max(itd)}}
-    res["speedup_vs_one_at_a_time_device"] = round(t_single / makespan, 3)
-    res["speedup_fp64_server_vs_one_at_a_time_device"] = round(t_single / makespan64, 3)
+                                    "effective_GBps_fp64_cg_bytes":
+                                        round(fp64_bytes / t_single / 1e9, 1),
+                                    "all_converged": all(s["status"] == 0
+                                                         for _, s in singles)},
+           "parity_on_sample": {"max_rel_diff_x_vs_single": max(err),
+                                "max_iter_diff_fp64": max(itd) if itd else None,
+                                "max_true_rel_residual": max(true_res)}}
+    res["speedup_vs_one_at_a_time_device"] = round(capacity / single_rate, 3)
     if ref_style:
-        t0 = time.perf_counter()
-        ok = True
-        rservice = []
-        for rp, ci, v, b in reqs:
+        rsample = list(range(0, nreq, ref_every))
+        ok, t_ref = True, 0.0
+        for k in rsample:
+            rp, ci, v, b = make_request(hflmp, specs[k])
             ts = time.perf_counter()
             _, r = hflmp.lmp_solve(rp, ci, v, b, solver=0, precond=1, rtol=rtol, maxit=100000)
-            rservice.append(time.perf_counter() - ts)
+            t_ref += time.perf_counter() - ts
             ok = ok and r["status"] == 0
-        t_ref = time.perf_counter() - t0
         res["one_at_a_time_reference_style"] = {
-            "solves_per_s": round(nreq / t_ref, 2), "makespan_s": round(t_ref, 3),
-            "all_converged": ok,
+            "sample": "every %dth request (%d)" % (ref_every, len(rsample)),
+            "solves_per_s": round(len(rsample) / t_ref, 2), "all_converged": ok,
             "what": "CG<LAD>::SolvePrecond with the block-Jacobi preconditioner over "
                     "CoupledMatrix/CoupledVector on the HIP platform (host-driven, one "
                     "synchronising dot per reduction)"}
-        res["speedup_vs_one_at_a_time_reference_style"] = round(t_ref / makespan, 3)
-    # latency under Poisson arrivals at `rate` (default: 60% of the server's capacity)
-    lam = rate if rate else 0.6 * nreq / makespan
-    arrivals = np.cumsum(gaps) / lam
-    _, lat, mk2, _ = run_server(hflmp, ctx, reqs, arrivals, cfg, rtol, 0)
-    base = fifo_latencies(arrivals, service)
-    res["latency"] = {"rate_per_s": round(lam, 2),
-                      "server_p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
-                      "server_p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2),
-                      "server_solves_per_s": round(nreq / mk2, 2),
-                      "one_at_a_time_p50_ms": round(float(np.percentile(base, 50)) * 1e3, 2),
-                      "one_at_a_time_p99_ms": round(float(np.percentile(base, 99)) * 1e3, 2),
-                      "note": "server measured live; one-at-a-time latencies from its measured "
-                              "per-request service times in arrival order (FIFO)"}
+        res["speedup_vs_one_at_a_time_reference_style"] = round(
+            capacity * t_ref / len(rsample), 3)
     ctx.close()
     return res
 
@@ -760,7 +946,8 @@ def main():
     ap.add_argument("--no-assembly", action="store_true")
     ap.add_argument("--no-mg", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--serve-requests", type=int, default=200)
+    ap.add_argument("--no-c1", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=10000)
     ap.add_argument("--serve-seed", type=int, default=20241205)
     ap.add_argument("--serve-rate", type=float, default=0.0,
                     help="Poisson arrival rate for the latency run (default: 60%% of the "
@@ -875,8 +1062,9 @@ def main():
                          "max_abs_diff": float(np.max(np.abs(y - yref)))}
     dz.free(); dw.free()
 
+    x_cg = None
     if not args.no_cg:
-        out["cg"] = bench_cg(hflmp, ctx, A, S16, S, b)
+        out["cg"], x_cg = bench_cg(hflmp, ctx, A, S16, S, b)
     S16.free()
     if not args.no_stencil:
         out["matrix_free"] = bench_stencil(hflmp, ctx, n, x, b, max(args.steps // 2, 1),
@@ -918,8 +1106,20 @@ def main():
         out["cg_time_to_1e-8"] = tt
     if not args.no_cpu:
         base, naive = cpu_baseline(rp, ci, v, x, args.cpu_seconds)
+        base["cpu"] = cpu_model()
         out["cpu_baseline"] = base
         out["cpu_baseline_naive"] = naive
+        if "cg" in out:
+            # CG time to 1e-8 at 128^3 along the reference's CPU path, and
+            # the GPU's int16 SELL-64 solve against it
+            g = out["cg"]["sell"]
+            ccg, _ = cpu_cg(rp, ci, v, b, 1e-8, x_cg, g["iters"])
+            ccg["speedup_gpu_sell_int16_vs_cpu"] = round(ccg["seconds_to_tol"] /
+                                                         g["seconds_to_tol"], 2)
+            out["cpu_baseline_cg"] = ccg
+            tt["cpu_reference_path_ms"] = round(ccg["seconds_to_tol"] * 1e3, 1)
+        if not args.no_c1:
+            out["c1_2d_q1_256"] = bench_c1(hflmp)
     print(json.dumps(out), flush=True)
 
 

@@ -170,6 +170,14 @@ void or_cast_s2d(int n, const float *x, double *y);
 
 /* Mixed-precision refinement: FP64 residual/update/norm, FP32 Jacobi-PCG
  * correction solve.  Returns 0 on success. */
+/* threads of the solvers' SpMV row loop (1 = sequential; same results) */
+void or_set_solver_threads(int n);
+int or_get_solver_threads(void);
+/* threads of or_cg_*'s vector operations (1 = sequential; dot then sums
+ * per-thread blocks: deterministic, not the sequential bits) */
+void or_set_blas_threads(int n);
+int or_get_blas_threads(void);
+
 int or_mpir_ds(int nrow, const int *rp, const int *ci, const double *vd,
                const float *vs, const double *b, double *x, int max_outer,
                double atol, double rtol, int inner_maxit, double inner_rtol,

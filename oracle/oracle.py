@@ -54,6 +54,10 @@ def lib():
                                          C.c_double]
         L.or_multicoloring.argtypes = [C.c_int, _i32p, _i32p, C.POINTER(C.c_int),
                                        _i32p, _i32p, _i32p]
+        L.or_set_solver_threads.argtypes = [C.c_int]
+        L.or_get_solver_threads.restype = C.c_int
+        L.or_set_blas_threads.argtypes = [C.c_int]
+        L.or_get_blas_threads.restype = C.c_int
         L.or_cast_d2s.argtypes = [C.c_int, _f64p, _f32p]
         L.or_cast_s2d.argtypes = [C.c_int, _f32p, _f64p]
         for s, fp, ct in (("d", _f64p, C.c_double), ("s", _f32p, C.c_float)):
@@ -344,10 +348,39 @@ def _prec_args(rp, ci, v, prec, coloring, num_blocks=1):
     return invd, 0, None, None, None
 
 
+class _SolverThreads:
+    """For the duration of a call: the solvers' SpMV row loop on `n` threads
+    (CPUopenmp_CSR_lMatrix::VectorMult; the same per-row sums, so the same
+    results as one thread) and the CG's vector operations on `nb` threads
+    (1: sequential, the reference's CPU path; more: a faster checker whose
+    dot products sum per-thread blocks, so not the sequential bits)."""
+
+    def __init__(self, n, nb=1):
+        self.n, self.nb = max(int(n), 1), max(int(nb), 1)
+
+    def __enter__(self):
+        self.prev = (lib().or_get_solver_threads(), lib().or_get_blas_threads())
+        lib().or_set_solver_threads(self.n)
+        lib().or_set_blas_threads(self.nb)
+
+    def __exit__(self, *exc):
+        lib().or_set_solver_threads(self.prev[0])
+        lib().or_set_blas_threads(self.prev[1])
+
+
 def cg(rp, ci, v, b, x0=None, prec=PREC_JACOBI, maxit=10000, atol=0.0,
-       rtol=1e-8, dtol=1e6, coloring=None, history=False, num_blocks=1):
+       rtol=1e-8, dtol=1e6, coloring=None, history=False, num_blocks=1, threads=1,
+       blas_threads=1):
     """CG::Solve with the reference semantics; returns (x, info dict).
-    num_blocks: the block count of PREC_BSGS."""
+    num_blocks: the block count of PREC_BSGS; threads: threads of the SpMV
+    row loop (the CPUopenmp platform's matrix; same results); blas_threads:
+    threads of the vector operations (see _SolverThreads)."""
+    with _SolverThreads(threads, blas_threads):
+        return _cg(rp, ci, v, b, x0, prec, maxit, atol, rtol, dtol, coloring, history,
+                   num_blocks)
+
+
+def _cg(rp, ci, v, b, x0, prec, maxit, atol, rtol, dtol, coloring, history, num_blocks):
     n = len(rp) - 1
     x = np.zeros(n, v.dtype) if x0 is None else np.array(x0, v.dtype)
     invd, nc, csz, perm, color = _prec_args(rp, ci, v, prec, coloring, num_blocks)
@@ -408,7 +441,12 @@ def gmres(rp, ci, v, b, x0=None, prec=PREC_NONE, basis=100, flexible=False,
 
 
 def mpir(rp, ci, vd, b, x0=None, max_outer=50, atol=0.0, rtol=1e-10,
-         inner_maxit=10000, inner_rtol=1e-4):
+         inner_maxit=10000, inner_rtol=1e-4, threads=1, blas_threads=1):
+    with _SolverThreads(threads, blas_threads):
+        return _mpir(rp, ci, vd, b, x0, max_outer, atol, rtol, inner_maxit, inner_rtol)
+
+
+def _mpir(rp, ci, vd, b, x0, max_outer, atol, rtol, inner_maxit, inner_rtol):
     n = len(rp) - 1
     x = np.zeros(n, np.float64) if x0 is None else np.array(x0, np.float64)
     vs = vd.astype(np.float32)

@@ -18,6 +18,24 @@ enum {
   OR_FAIL_MAXIT = 4
 };
 
+/* Threads of the row loop of the solvers' SpMV (or_cg_*, or_mpir_ds).  1:
+ * CPUsimple_CSR_lMatrix::VectorMult; more: CPUopenmp_CSR_lMatrix::VectorMult
+ * (openmp/CPUopenmp_csr_lmatrix.cc:145-165), the same per-row sums, so the
+ * same bits.  The vector operations stay on one thread, as the CPUopenmp
+ * platform's (openmp/CPUopenmp_lvector.cc:314-370 and :397-540 are omp simd
+ * loops, not parallel ones). */
+static int or_solver_threads = 1;
+/* Threads of the CG's vector operations (or_cg_*; 1 = sequential). */
+#define OR_MAX_BLAS_THREADS 256
+static int or_blas_threads = 1;
+
+void or_set_solver_threads(int n) { or_solver_threads = n > 1 ? n : 1; }
+int or_get_solver_threads(void) { return or_solver_threads; }
+void or_set_blas_threads(int n) {
+  or_blas_threads = n < 1 ? 1 : n > OR_MAX_BLAS_THREADS ? OR_MAX_BLAS_THREADS : n;
+}
+int or_get_blas_threads(void) { return or_blas_threads; }
+
 static int cmp_int_or(const void *a, const void *b) {
   const int x = *(const int *)a, y = *(const int *)b;
   return (x > y) - (x < y);
@@ -56,7 +74,7 @@ int or_mpir_ds(int nrow, const int *rp, const int *ci, const double *vd,
   float *invd = (float *)malloc(sizeof(float) * nn);
   or_invdiag_s(n, rp, ci, vs, invd);
 
-  or_spmv_d(n, rp, ci, vd, x, resh);
+  or_spmv_solver_d(n, rp, ci, vd, x, resh);
   or_scaleadd_d(n, -1.0, b, resh);
   double res = or_nrm2_d(n, resh);
   const double res0 = res;
@@ -72,7 +90,7 @@ int or_mpir_ds(int nrow, const int *rp, const int *ci, const double *vd,
     inner_total += it;
     or_cast_s2d(n, corl, corh);
     or_axpy_d(n, 1.0, corh, x);
-    or_spmv_d(n, rp, ci, vd, x, resh);
+    or_spmv_solver_d(n, rp, ci, vd, x, resh);
     or_scaleadd_d(n, -1.0, b, resh);
     res = or_nrm2_d(n, resh);
   }

@@ -419,6 +419,25 @@ def test_cg_vs_scipy_and_dense(prec):
     assert abs(np.linalg.norm(b - A @ x) - info["res_final"]) <= 1e-9 * info["res_init"]
 
 
+def test_cg_threaded_paths():
+    # the CPUopenmp matrix (threaded SpMV row loop): the same bits; threaded
+    # vector operations: dot products summed by blocks, so the iterates
+    # differ by rounding only
+    rp, ci, v = O.fe_system(1, 2, 48, O.fe_kappa(2, 48, 1.0, 4))
+    b = O.fe_rhs(1, 2, 48, 6)
+    x1, i1 = O.cg(rp, ci, v, b, rtol=1e-10)
+    x2, i2 = O.cg(rp, ci, v, b, rtol=1e-10, threads=4)
+    assert np.array_equal(x1, x2) and i1 == i2
+    x3, i3 = O.cg(rp, ci, v, b, rtol=1e-10, threads=4, blas_threads=3)
+    assert abs(i3["iters"] - i1["iters"]) <= 2
+    assert np.linalg.norm(x3 - x1) <= 1e-10 * np.linalg.norm(x1)
+    xm, im = O.mpir(rp, ci, v, b, rtol=1e-10)
+    xm2, im2 = O.mpir(rp, ci, v, b, rtol=1e-10, threads=4)
+    assert np.array_equal(xm, xm2) and im == im2
+    # the settings are restored after each call
+    assert O.lib().or_get_solver_threads() == 1 and O.lib().or_get_blas_threads() == 1
+
+
 def test_cg_stopping_rules():
     rp, ci, v = O.fe_system(1, 2, 16)
     b = O.fe_rhs(1, 2, 16, 2)
