@@ -45,6 +45,16 @@ KAPPA_SEED, RHS_SEED = 20241203, 20241204  # mixed-precision system
 BATCH_SEED0 = 1000  # right-hand sides of the batched system: seeds 1000..1063
 
 
+_T_START = time.perf_counter()
+
+
+def progress(msg):
+    """One line on stderr when a leg or part of one ends (the JSON result
+    alone goes to stdout)."""
+    print("[bench %7.1f s] %s" % (time.perf_counter() - _T_START, msg), file=sys.stderr,
+          flush=True)
+
+
 def spmv_bytes(N, nnz, vb=8):
     return (4 + vb) * nnz + 4 * (N + 1) + 2 * vb * N
 
@@ -760,7 +770,7 @@ def run_server_stream(hflmp, ctx, feed_specs, arrivals, cfg, rtol, precision, ke
         n = len(feed_specs)
         infos, lat, xs = [None] * n, np.zeros(n), {}
         by_ticket = {}
-        nxt = 0
+        nxt = done = 0
         t0 = time.perf_counter()
         while nxt < n or by_ticket:
             now = time.perf_counter() - t0
@@ -783,6 +793,11 @@ def run_server_stream(hflmp, ctx, feed_specs, arrivals, cfg, rtol, precision, ke
                         lat[k] = now - (0.0 if arrivals is None else arrivals[k])
                         if k in keep:
                             xs[k] = x
+                        done += 1
+                        if done % 1000 == 0:
+                            progress("serving (%s): %d of %d requests done"
+                                     % ("capacity" if arrivals is None else "latency",
+                                        done, n))
         makespan = time.perf_counter() - t0
         stats = sv.stats()
     finally:
@@ -839,6 +854,7 @@ def bench_serve(hflmp, nreq, seed, rate, ref_style, graphs=True, ntune=32, sampl
     t0 = time.perf_counter()
     cfg, tune_report = serve_tune.tune(ctx, treqs, base, rtol)
     t_tune = time.perf_counter() - t0
+    progress("serving: autotuner done in %.1f s" % t_tune)
     del treqs
     sample = list(range(0, nreq, sample_every))
     keep = set(sample)
@@ -846,14 +862,17 @@ def bench_serve(hflmp, nreq, seed, rate, ref_style, graphs=True, ntune=32, sampl
     infos, xs, _, makespan, stats = run_server_stream(hflmp, ctx, specs, None, cfg, rtol, 0,
                                                       keep)
     capacity = nreq / makespan
+    progress("serving: capacity run %.1f s, %.1f solves/s" % (makespan, capacity))
     # latency under Poisson arrivals
     lam = rate if rate else load * capacity
     arrivals = np.cumsum(gaps) / lam
     linfos, _, lat, mk2, _ = run_server_stream(hflmp, ctx, specs, arrivals, cfg, rtol, 0, set())
+    progress("serving: latency run %.1f s" % mk2)
     # one at a time on the sample (requests regenerated; identical by seed)
     sreqs = [make_request(hflmp, specs[k]) for k in sample]
     service, singles, solve_s, t_single = run_singles(hflmp, ctx, sreqs, rtol)
     single_rate = len(sample) / t_single
+    progress("serving: one at a time on %d requests, %.1f solves/s" % (len(sample), single_rate))
     # FIFO on the sample's own arrivals, rescaled to the same offered rate
     sub_arr = arrivals[sample] / sample_every
     base_lat = fifo_latencies(sub_arr, service)
@@ -1001,6 +1020,7 @@ def main():
     wall = time.perf_counter() - w0
     per_launch_s = ev1.elapsed_ms(ev0) / 1e3 / args.steps
     gbs = nb / per_launch_s / 1e9
+    progress("SpMV timed steps: %.2f us per launch, %.1f GB/s" % (per_launch_s * 1e6, gbs))
 
     out = {
         "metric": "FP64 SpMV GB/s (CSR algorithmic bytes), 3D Q1 Poisson 128^3",
@@ -1065,27 +1085,34 @@ def main():
     x_cg = None
     if not args.no_cg:
         out["cg"], x_cg = bench_cg(hflmp, ctx, A, S16, S, b)
+        progress("Jacobi-CG leg done")
     S16.free()
     if not args.no_stencil:
         out["matrix_free"] = bench_stencil(hflmp, ctx, n, x, b, max(args.steps // 2, 1),
                                            out.get("cg", {}).get("sell"),
                                            None if args.no_parity else (S, rp, ci, v))
+        progress("matrix-free leg done")
     if not args.no_mg:
         out["multigrid"] = bench_mg(hflmp, ctx, n, max(args.steps // 10, 5),
                                     out.get("cg", {}).get("sell"))
+        progress("multigrid leg done")
     S.free(); A.free(); dx.free(); dy.free()
     ctx.close()
 
     if not args.no_assembly:
         out["assembly"] = bench_assembly(hflmp, n)
+        progress("assembly leg done")
     if not args.no_mixed:
         out["mixed_precision"] = bench_mixed(hflmp, n)
+        progress("mixed-precision leg done")
     if not args.no_batched:
         out["batched"] = bench_batched(hflmp)
+        progress("batched leg done")
     if not args.no_serve:
         out["serving"] = bench_serve(hflmp, args.serve_requests, args.serve_seed,
                                      args.serve_rate, not args.no_serve_reference,
                                      not args.serve_no_graphs)
+        progress("serving leg done")
 
     # CG time to 1e-8 on the 128^3 Poisson system, every solver path side
     # by side (each measured in its leg above)
@@ -1107,6 +1134,7 @@ def main():
     if not args.no_cpu:
         base, naive = cpu_baseline(rp, ci, v, x, args.cpu_seconds)
         base["cpu"] = cpu_model()
+        progress("CPU SpMV baseline done")
         out["cpu_baseline"] = base
         out["cpu_baseline_naive"] = naive
         if "cg" in out:
@@ -1118,6 +1146,7 @@ def main():
                                                          g["seconds_to_tol"], 2)
             out["cpu_baseline_cg"] = ccg
             tt["cpu_reference_path_ms"] = round(ccg["seconds_to_tol"] * 1e3, 1)
+            progress("CPU CG baseline done")
         if not args.no_c1:
             out["c1_2d_q1_256"] = bench_c1(hflmp)
     print(json.dumps(out), flush=True)
