@@ -7,12 +7,15 @@
 //         p = (gamma/beta) p + z;  beta = gamma
 // with the stopping rule of IterateControl::Check (common/iterate_control.cc:
 // 80-107) on the recursive residual.  Every scalar stays on the device.
-// One iteration is five launches:
+// One iteration is three launches (five for systems of a few hundred rows,
+// whose partial area is too small for the folded form):
 //   K1  SpMV Ap = A p, per-workgroup partials of p.Ap
-//   F1  one workgroup: p.Ap, alpha
+//  (F1  one workgroup: p.Ap, alpha)
 //   K2  x += alpha p, r -= alpha Ap, z = D^-1 r, partials of r.r and r.z
-//   F2  one workgroup: res, stopping test, beta
+//  (F2  one workgroup: res, stopping test, beta)
 //   K3  p = beta p + z
+// In the three-launch form F1 runs at the start of every workgroup of K2
+// and F2 at the start of every workgroup of K3.
 // Every kernel returns at once when the state is no longer "iterate", so
 // the host queues iterations ahead and reads the state every `poll`
 // iterations only.
@@ -162,6 +165,186 @@ __global__ void __launch_bounds__(kBlock)
     p[i] = beta * p[i] + z[i];
 }
 
+// ---- three launches per iteration ------------------------------------------------
+// The same arithmetic with F1 and F2 folded into their consumers: every
+// workgroup of K2 sums K1's partials (p.Ap) and every workgroup of K3 sums
+// K2's (r.r, r.z), in the same fixed order, so all of them compute the same
+// alpha, beta and stopping decision, and workgroup 0 records them in the
+// state.  K1 runs a grid that fits on the device at once, each workgroup
+// walking several units and keeping one partial, so the partial counts stay
+// small (about 1280 and 2 x 1024 on gfx950) and every workgroup's re-read
+// of them is an L2 hit.  Fields written by workgroup 0 are never read by
+// the other workgroups of the same kernel: K2 leaves r.z and the iteration
+// count in rz_prev / iter_prev for K3, whose workgroup 0 writes rz / iter.
+
+// Operators whose K1 walks several units per workgroup.  The matrix-free
+// stencil keeps one unit per workgroup (cg_spmv_kernel over its whole grid,
+// one partial per unit): inside a walk its 3D kernel needs more than twice
+// the registers.
+template <typename Op>
+struct UnitWalk : std::true_type {};
+template <typename T, int DIM, int ORDER, bool MULTI>
+struct UnitWalk<StencilOp<T, DIM, ORDER, MULTI>> : std::false_type {};
+
+// K1: Ap = A p over all units; one partial of p.Ap per workgroup.  At least
+// five waves per SIMD, so that every SELL-64 FP64 form (int16 or int32
+// columns) runs the same grid and the sums, and so the iterates, do not
+// depend on the column width.
+template <typename Op, typename T = typename Op::value_type>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5)))
+    cg3_spmv_kernel(Op op, int vgrid, const T *__restrict__ p, T *__restrict__ Ap,
+                    const CgState<T> *st, T *__restrict__ part) {
+  __shared__ T prod[Op::kLds];
+  if (st->status != kIterate) return;
+  T acc = 0;
+  // gridDim.x is a multiple of the XCD count: unit vb runs on XCD vb % 8
+  for (int vb = blockIdx.x; vb < vgrid; vb += gridDim.x) {
+    // operators that stage x in LDS (CSR tiles, stencil) write it before
+    // their own barrier: wait until the previous unit's reads are done
+    if (Op::kLds > 1 && vb != (int)blockIdx.x) __syncthreads();
+    const long long tile = xcd_tile(vb, vgrid);
+    TileRow tr;
+    const T s = op.row_sum(tile, p, T(0), tr, prod);
+    if (tr.has_row) {
+      __builtin_nontemporal_store(s, Ap + tr.row);
+      acc += s * p[tr.row];
+    }
+  }
+  block_partial1(acc, part);
+}
+
+// K2: alpha from K1's partials; x += alpha p; r -= alpha Ap; z = D^-1 r;
+// partials of r.r and r.z.  Four elements per thread per step, their loads
+// issued together.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    cg3_update_kernel(int n, int g1, const T *__restrict__ invd, const T *__restrict__ p,
+                      const T *__restrict__ Ap, T *__restrict__ x, T *__restrict__ r,
+                      T *__restrict__ z, CgState<T> *st, const T *__restrict__ part1,
+                      T *__restrict__ part2) {
+  __shared__ T lds[kBlock / kWave];
+  if (st->status != kIterate) return;
+  const T pAp = block_total(part1, g1, 1, 0, lds);
+  const T rzc = st->rz;
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  if (lead) {
+    st->pAp = pAp;
+    st->rz_prev = rzc;
+    st->iter_prev = st->iter;
+  }
+  if (pAp == T(0) || !(pAp == pAp)) {
+    if (lead) st->status = kBreakdown;
+    return;
+  }
+  const T alpha = rzc / pAp;
+  if (lead) st->alpha = alpha;
+  const T malpha = -alpha;
+  T rr = 0, rz = 0;
+  const long long stride = (long long)gridDim.x * kBlock;
+  for (long long i0 = (long long)blockIdx.x * kBlock + threadIdx.x; i0 < n; i0 += 4 * stride) {
+    T pv[4], av[4], xv[4], rv[4], dv[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const long long i = i0 + q * stride;
+      if (i < n) {
+        pv[q] = p[i];
+        av[q] = Ap[i];
+        xv[q] = x[i];
+        rv[q] = r[i];
+        dv[q] = invd[i];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const long long i = i0 + q * stride;
+      if (i < n) {
+        x[i] = xv[q] + alpha * pv[q];
+        const T ri = rv[q] + malpha * av[q];
+        const T zi = ri * dv[q];
+        r[i] = ri;
+        z[i] = zi;
+        rr += ri * ri;
+        rz += ri * zi;
+      }
+    }
+  }
+  block_partial2(rr, rz, part2);
+}
+
+// K3: res = sqrt(r.r), stopping test and beta from K2's partials; p = beta p + z
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    cg3_p_kernel(int n, int g2, const T *__restrict__ z, T *__restrict__ p, CgState<T> *st,
+                 const T *__restrict__ part2) {
+  __shared__ T lds[kBlock / kWave];
+  if (st->status != kIterate) return;
+  const T rr = block_total(part2, g2, 2, 0, lds);
+  const T rz = block_total(part2, g2, 2, 1, lds);
+  const int iter = st->iter_prev + 1;
+  const T res = sqrt(rr);
+  const int sv = check_state(iter, (double)res, (double)st->res0, st->maxit, st->atol,
+                             st->rtol, st->dtol);
+  const T beta = rz / st->rz_prev;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->iter = iter;
+    st->rr = rr;
+    st->res = res;
+    if (sv == kIterate) {
+      st->beta = beta;
+      st->rz = rz;
+    }
+    st->status = sv;
+  }
+  if (sv != kIterate) return;
+  const long long stride = (long long)gridDim.x * kBlock;
+  for (long long i0 = (long long)blockIdx.x * kBlock + threadIdx.x; i0 < n; i0 += 4 * stride) {
+    T pv[4], zv[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const long long i = i0 + q * stride;
+      if (i < n) {
+        pv[q] = p[i];
+        zv[q] = z[i];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const long long i = i0 + q * stride;
+      if (i < n) p[i] = beta * pv[q] + zv[q];
+    }
+  }
+}
+
+// Grids of the three-launch iteration for an operator over `vgrid` units
+// (a multiple of the XCD count) and a partial area of 2 * gmax values: K1
+// on as many workgroups as are resident at once (a multiple of the XCD
+// count), K2 and K3 on at most min(grid2, 1024).  g1 = 0: the partial area
+// is too small (systems of a few hundred rows); use the five-launch form.
+struct Cg3Grids {
+  int g1, g2;
+};
+
+template <typename Op>
+Cg3Grids cg3_grids(hflmp_ctx *ctx, int vgrid, int grid2, int gmax) {
+  long long g1 = vgrid;
+  if constexpr (UnitWalk<Op>::value) {
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &occ, reinterpret_cast<const void *>(&cg3_spmv_kernel<Op>), kBlock, 0) !=
+            hipSuccess ||
+        occ < 1)
+      occ = 1;
+    const int cus = ctx->num_cus > 0 ? ctx->num_cus : 256;
+    g1 = (long long)occ * cus / kXcds * kXcds;
+    if (g1 > vgrid) g1 = vgrid;
+  }
+  int g2 = grid2 < 1024 ? grid2 : 1024;
+  const long long avail = 2LL * gmax - g1;
+  if (g2 > avail / 2) g2 = (int)(avail / 2);
+  if (g1 < kXcds || g2 < 1) return {0, 0};
+  return {(int)g1, g2};
+}
+
 // The partial-sum area of the workspace is sized for the CSR tile grid,
 // which is never smaller than the SELL grid (R <= 256 rows per tile vs 256
 // rows per SELL workgroup), so one workspace size serves both operators.
@@ -201,7 +384,22 @@ int pcg_jacobi(hflmp_ctx *ctx, const Op &op, int nnz, const T *invd, const T *b,
   cg_init_final<T><<<1, kFinalBlock, 0, s>>>(st, partials, grid1);
   HFLMP_LAUNCH_CHECK(ctx);
 
+  // the partial area holds 2 * max(L.grid1, L.grid2) values, at least 2 *
+  // max(grid1, grid2); sizing by the latter keeps the grids, and so the
+  // results, the same for every layout of the same system
+  const Cg3Grids G = cg3_grids<Op>(ctx, grid1, L.grid2, grid1 > L.grid2 ? grid1 : L.grid2);
+  T *part2 = partials + G.g1;
   auto iteration = [&]() {
+    if (G.g1) {
+      if constexpr (UnitWalk<Op>::value)
+        cg3_spmv_kernel<Op><<<G.g1, kBlock, 0, s>>>(op, grid1, p, Ap, st, partials);
+      else
+        cg_spmv_kernel<Op><<<G.g1, kBlock, 0, s>>>(op, p, Ap, st, partials);
+      cg3_update_kernel<T><<<G.g2, kBlock, 0, s>>>(nrow, G.g1, invd, p, Ap, x, r, z, st,
+                                                   partials, part2);
+      cg3_p_kernel<T><<<G.g2, kBlock, 0, s>>>(nrow, G.g2, z, p, st, part2);
+      return;
+    }
     cg_spmv_kernel<Op><<<grid1, kBlock, 0, s>>>(op, p, Ap, st, partials);
     cg_alpha_final<T><<<1, kFinalBlock, 0, s>>>(st, partials, grid1);
     cg_update_kernel<T><<<g2, kBlock, 0, s>>>(nrow, invd, p, Ap, x, r, z, st, partials);
@@ -389,6 +587,9 @@ int mpcg_run(hflmp_ctx *ctx, const Op64 &op64, const Op32 &op32, const MpLayout 
   float *c = reinterpret_cast<float *>(w + L.vec + 4 * L.vstride);
   hipStream_t s = ctx->stream;
   const unsigned g3 = grid_for(nrow, 4);
+  // the FP32 part32 area holds 2 * gmax floats, gmax >= grid32 and grid2
+  const int gmax32 = (int)((L.vec - L.part32) / (2 * sizeof(float)));
+  const Cg3Grids G = cg3_grids<Op32>(ctx, grid32, L.grid2, gmax32);
   int *host_status = reinterpret_cast<int *>(static_cast<char *>(ctx->host_pinned) + 2048);
   int outer = 0, inner_total = 0;
   bool last_breakdown = false;
@@ -409,6 +610,13 @@ int mpcg_run(hflmp_ctx *ctx, const Op64 &op64, const Op32 &op32, const MpLayout 
     int rc = hflmp_fill_zero(ctx, c, sizeof(float) * (size_t)nrow);
     if (rc) return rc;
     auto iteration = [&]() {
+      if (G.g1) {
+        cg3_spmv_kernel<Op32><<<G.g1, kBlock, 0, s>>>(op32, grid32, p, Ap, st, part32);
+        cg3_update_kernel<float><<<G.g2, kBlock, 0, s>>>(nrow, G.g1, invd32, p, Ap, c, r, z,
+                                                         st, part32, part32 + G.g1);
+        cg3_p_kernel<float><<<G.g2, kBlock, 0, s>>>(nrow, G.g2, z, p, st, part32 + G.g1);
+        return;
+      }
       cg_spmv_kernel<Op32><<<grid32, kBlock, 0, s>>>(op32, p, Ap, st, part32);
       cg_alpha_final<float><<<1, kFinalBlock, 0, s>>>(st, part32, grid32);
       cg_update_kernel<float><<<L.grid2, kBlock, 0, s>>>(nrow, invd32, p, Ap, c, r, z, st,

@@ -26,6 +26,11 @@ struct CgState {
   int status;  // read by the host poll together with `maxit`
   int maxit;
   double atol, rtol, dtol;
+  // the three-launch iteration: r.z and the iteration count as the update
+  // kernel found them, read by the p kernel while its workgroup 0 writes
+  // rz and iter
+  T rz_prev;
+  int iter_prev;
 };
 
 // IterateControl::Check (common/iterate_control.cc:80-107); a NaN residual
@@ -71,12 +76,43 @@ __device__ __forceinline__ void block_partial2(T a, T b, T *partials) {
   }
 }
 
+// Sum of `count` strided partials src[k*stride + off] over one workgroup of
+// kBlock threads, returned to every thread: the same fixed order in every
+// workgroup, so all workgroups of a kernel get the same value.
+template <typename T>
+__device__ __forceinline__ T block_total(const T *__restrict__ src, int count, int stride,
+                                         int off, T *lds) {
+  constexpr int F = 8;
+  T a[F];
+#pragma unroll
+  for (int f = 0; f < F; ++f) a[f] = 0;
+  for (int k0 = threadIdx.x; k0 < count; k0 += F * kBlock) {
+    T v[F];
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+      const int kk = k0 + f * kBlock;
+      v[f] = kk < count ? src[(size_t)kk * stride + off] : T(0);
+    }
+#pragma unroll
+    for (int f = 0; f < F; ++f) a[f] += v[f];
+  }
+  const T v = wave_sum(((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7])));
+  __syncthreads();  // a previous call may still be reading lds
+  if ((threadIdx.x & (kWave - 1)) == 0) lds[threadIdx.x / kWave] = v;
+  __syncthreads();
+  T s = 0;
+#pragma unroll
+  for (int w = 0; w < kBlock / kWave; ++w) s += lds[w];
+  return s;
+}
+
 template <typename T>
 __global__ void init_state_kernel(CgState<T> *st, int maxit, double atol, double rtol,
                                   double dtol) {
   if (threadIdx.x == 0) {
     CgState<T> s;
-    s.rz = s.pAp = s.alpha = s.beta = s.rr = s.res = s.res0 = T(0);
+    s.rz = s.pAp = s.alpha = s.beta = s.rr = s.res = s.res0 = s.rz_prev = T(0);
+    s.iter_prev = 0;
     s.iter = 0;
     s.status = kIterate;
     s.maxit = maxit;

@@ -104,6 +104,13 @@ def hip_lib():
         L.hflmp_batched_mpcg_workspace_bytes.argtypes = [vp, i, i]
         L.hflmp_batched_mpcg_jacobi.argtypes = [vp, i, i, i, vp, vp, vp, vp, vp, vp, vp, i, d, d,
                                                 i, d, vp, sz, vp, vp, vp, vp, vp, C.POINTER(d)]
+        L.hflmp_spmm_plan_create.argtypes = [vp, i, i, i, vp, vp, i, i, i, vp]
+        L.hflmp_spmm_plan_destroy.argtypes = [vp]
+        L.hflmp_spmm_plan_destroy.restype = None
+        L.hflmp_spmm_plan_info.argtypes = [vp, C.POINTER(i), C.POINTER(i),
+                                           C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
+        for s in ("f64", "f32"):
+            getattr(L, "hflmp_plan_spmm_" + s).argtypes = [vp, vp, vp, vp, vp, vp, i]
         for s in ("f64", "f32"):
             getattr(L, "hflmp_csr_spmm_" + s).argtypes = [vp, i, i, i, i, vp, vp, vp, vp, vp]
             getattr(L, "hflmp_batched_pcg_jacobi_" + s).argtypes = [
@@ -514,6 +521,40 @@ def spmm(ctx, A, X, Y, nrhs):
     """Y = A X for row-major blocks of nrhs <= 64 vectors (device arrays)."""
     _call(ctx, "hflmp_csr_spmm_" + _sfx(A.dtype), A.nrow, A.ncol, A.nnz, nrhs, A.rp.ptr,
           A.ci.ptr, A.val.ptr, X.ptr, Y.ptr)
+
+
+class SpmmPlan:
+    """Row-block layout of a device CSR matrix for Y = A X (hflmp_spmm_plan_*):
+    each block's distinct columns listed once, its X rows staged in LDS.
+    Blocks are sized for products of nrhs vectors of value_bytes."""
+
+    def __init__(self, ctx, A, nrhs, value_bytes=None, lds_budget=0):
+        self.ctx = ctx
+        self.nrow, self.ncol, self.nnz = A.nrow, A.ncol, A.nnz
+        self.h = C.c_void_p()
+        vb = A.dtype.itemsize if value_bytes is None else value_bytes
+        _call(ctx, "hflmp_spmm_plan_create", A.nrow, A.ncol, A.nnz, A.rp.ptr, A.ci.ptr,
+              int(nrhs), int(vb), int(lds_budget), C.byref(self.h))
+
+    def info(self):
+        r, u = C.c_int(), C.c_int()
+        n, b = C.c_longlong(), C.c_longlong()
+        rc = self.ctx.L.hflmp_spmm_plan_info(self.h, C.byref(r), C.byref(u), C.byref(n),
+                                             C.byref(b))
+        if rc:
+            raise HflmpError("hflmp_spmm_plan_info failed (%d)" % rc)
+        return {"rows_per_block": r.value, "max_cols_per_block": u.value,
+                "list_entries": n.value, "device_bytes": b.value}
+
+    def spmm(self, A, X, Y, nrhs):
+        """Y = A X through the plan (A: the matrix it was built from)."""
+        _call(self.ctx, "hflmp_plan_spmm_" + _sfx(A.dtype), self.h, A.rp.ptr, A.val.ptr,
+              X.ptr, Y.ptr, int(nrhs))
+
+    def free(self):
+        if self.h:
+            self.ctx.L.hflmp_spmm_plan_destroy(self.h)
+            self.h = C.c_void_p()
 
 
 def batched_mpcg_jacobi(ctx, A, A32, invd32, B, X, nrhs, max_outer=50, atol=0.0, rtol=1e-8,

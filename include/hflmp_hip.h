@@ -81,7 +81,11 @@ int hflmp_scatter_f64(hflmp_ctx *ctx, int n, const int *idx, const double *src,
                       double *dst);
 int hflmp_scatter_f32(hflmp_ctx *ctx, int n, const int *idx, const float *src,
                       float *dst);
-/* dst[idx[k]] += src[k]  (add_values, lvector.h:76-83) */
+/* dst[idx[k]] += src[k]  (add_values, lvector.h:76-83).  The adds are
+ * device atomics: with distinct indices the result equals the sequential
+ * loop's; where an index repeats, the order of its additions is not fixed,
+ * so the sum may differ from the sequential one in the last bits (and from
+ * run to run).  Integer-valued data and single hits are exact. */
 int hflmp_scatter_add_f64(hflmp_ctx *ctx, int n, const int *idx,
                           const double *src, double *dst);
 int hflmp_scatter_add_f32(hflmp_ctx *ctx, int n, const int *idx,
@@ -631,6 +635,35 @@ int hflmp_csr_spmm_f64(hflmp_ctx *ctx, int nrow, int ncol, int nnz, int nrhs,
 int hflmp_csr_spmm_f32(hflmp_ctx *ctx, int nrow, int ncol, int nnz, int nrhs,
                        const int *row_ptr, const int *col, const float *val,
                        const float *X, float *Y);
+/* Row-block layout for the same product (no reference counterpart): rows
+ * in blocks of R consecutive rows, each block's distinct columns listed
+ * once in ascending order and every entry's column given as its position
+ * in that list.  A workgroup loads a block's X rows into LDS once and all
+ * the block's entries read them there, the next block's rows loading while
+ * one is computed.  R is the largest of 64, 32, 16, 8, 4 whose blocks keep
+ * to 64 (FP64) or 128 (FP32) distinct columns within a window of 65,536
+ * columns, 1024 entries,
+ * and lds_budget bytes of LDS (0 = 64 KiB, at most 160 KiB) for products
+ * of nrhs vectors of value_bytes (4 or 8); HFLMP_EFORMAT if none does,
+ * HFLMP_EINVAL for a column outside [0, ncol).  The plan keeps no reference
+ * to row_ptr/col; the products take the matrix's row_ptr and values, which
+ * must be those the plan was built from, and any nrhs <= 64 whose LDS need
+ * stays within 160 KiB.  Products are bit-identical to hflmp_csr_spmm_*. */
+typedef struct hflmp_spmm_plan hflmp_spmm_plan;
+int hflmp_spmm_plan_create(hflmp_ctx *ctx, int nrow, int ncol, int nnz,
+                           const int *row_ptr, const int *col, int nrhs,
+                           int value_bytes, int lds_budget,
+                           hflmp_spmm_plan **plan);
+void hflmp_spmm_plan_destroy(hflmp_spmm_plan *plan);
+int hflmp_spmm_plan_info(const hflmp_spmm_plan *plan, int *rows_per_block,
+                         int *max_cols_per_block, long long *list_entries,
+                         long long *device_bytes);
+int hflmp_plan_spmm_f64(hflmp_ctx *ctx, const hflmp_spmm_plan *plan,
+                        const int *row_ptr, const double *val,
+                        const double *X, double *Y, int nrhs);
+int hflmp_plan_spmm_f32(hflmp_ctx *ctx, const hflmp_spmm_plan *plan,
+                        const int *row_ptr, const float *val, const float *X,
+                        float *Y, int nrhs);
 /* Jacobi-PCG on nrhs systems with the same matrix, advanced in lockstep:
  * every column follows the single-vector iteration with its own alpha,
  * beta and stopping test and stops updating when it converges.  Per-column

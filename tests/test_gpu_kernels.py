@@ -229,6 +229,76 @@ def test_gather_scatter(ctx):
     assert np.array_equal(dst.download(), 2 * ref)
 
 
+def test_scatter_add_duplicate_indices(ctx):
+    # add_values (lvector.h:76-83) adds in index order; the device adds with
+    # atomics, so with duplicate indices the order of the additions is not
+    # fixed.  Values that are small integers make every order give the
+    # exact sum, which is what this checks (distinct indices: bit-exact).
+    rng = np.random.default_rng(11)
+    n, m = 5000, 700
+    idx = rng.integers(0, m, n).astype(np.int32)
+    idx[:50] = 3  # one destination hit 50+ times
+    src = rng.integers(-64, 65, n).astype(np.float64)
+    dst0 = rng.integers(-1000, 1000, m).astype(np.float64)
+    ref = dst0.copy()
+    for k in range(n):
+        ref[idx[k]] += src[k]
+    di, ds, dd = ctx.upload(idx), ctx.upload(src), ctx.upload(dst0)
+    ctx.check(ctx.L.hflmp_scatter_add_f64(ctx.h, n, di.ptr, ds.ptr, dd.ptr))
+    assert np.array_equal(dd.download(), ref)
+    d32 = ctx.upload(dst0.astype(np.float32))
+    s32 = ctx.upload(src.astype(np.float32))
+    ctx.check(ctx.L.hflmp_scatter_add_f32(ctx.h, n, di.ptr, s32.ptr, d32.ptr))
+    assert np.array_equal(d32.download(), ref.astype(np.float32))
+    for a in (di, ds, dd, d32, s32):
+        a.free()
+
+
+def test_rot(ctx):
+    # Rot (lvector.h:306): x' = c x + s y, y' = c y - s x with the old x, y
+    rng = np.random.default_rng(12)
+    n = 100003
+    x, y = rng.uniform(-1, 1, n), rng.uniform(-1, 1, n)
+    c, s_ = 0.6, 0.8
+    dx, dy = ctx.upload(x), ctx.upload(y)
+    ctx.check(ctx.L.hflmp_rot_f64(ctx.h, n, dx.ptr, dy.ptr, c, s_))
+    np.testing.assert_allclose(dx.download(), c * x + s_ * y, rtol=0, atol=4e-16)
+    np.testing.assert_allclose(dy.download(), c * y - s_ * x, rtol=0, atol=4e-16)
+    # c = 1, s = 0 is the identity, exactly; c = 0, s = 1 swaps with a sign
+    ctx.check(ctx.L.hflmp_rot_f64(ctx.h, n, dx.ptr, dy.ptr, 1.0, 0.0))
+    x1, y1 = dx.download(), dy.download()
+    ctx.check(ctx.L.hflmp_rot_f64(ctx.h, n, dx.ptr, dy.ptr, 0.0, 1.0))
+    assert np.array_equal(dx.download(), y1) and np.array_equal(dy.download(), -x1)
+    x32, y32 = x.astype(np.float32), y.astype(np.float32)
+    fx, fy = ctx.upload(x32), ctx.upload(y32)
+    ctx.check(ctx.L.hflmp_rot_f32(ctx.h, n, fx.ptr, fy.ptr, c, s_))
+    np.testing.assert_allclose(fx.download(), c * x32.astype(np.float64) + s_ * y32,
+                               rtol=0, atol=3e-7)
+    for a in (dx, dy, fx, fy):
+        a.free()
+
+
+def test_csr_diag_ptr(ctx):
+    # get_diag_ptr: position of a_ii, -1 where the row has no diagonal
+    from tests.helpers import random_csr
+    rng = np.random.default_rng(13)
+    lengths = rng.integers(0, 12, 400)
+    rp, ci, v = random_csr(400, 400, lengths, seed=14)
+    drp, dci, ddp = ctx.upload(rp), ctx.upload(ci), ctx.empty(400, np.int32)
+    ctx.check(ctx.L.hflmp_csr_diag_ptr(ctx.h, 400, drp.ptr, dci.ptr, ddp.ptr))
+    ref, missing = O.diag_ptr(rp, ci)
+    got = ddp.download()
+    assert np.array_equal(got, ref)
+    assert missing == int(np.sum(got < 0)) and missing > 0
+    rp2, ci2, v2 = hflmp.fe_system(1, 2, 20)
+    d2, c2, p2 = ctx.upload(rp2), ctx.upload(ci2), ctx.empty(len(rp2) - 1, np.int32)
+    ctx.check(ctx.L.hflmp_csr_diag_ptr(ctx.h, len(rp2) - 1, d2.ptr, c2.ptr, p2.ptr))
+    got2 = p2.download()
+    assert np.array_equal(ci2[got2], np.arange(len(rp2) - 1))
+    for a in (drp, dci, ddp, d2, c2, p2):
+        a.free()
+
+
 def test_invdiag_and_zero_diag(ctx):
     rp, ci, v = hflmp.fe_system(2, 2, 17, hflmp.fe_kappa(2, 17, 1.0, 3))
     A = hflmp.CsrMatrix(ctx, rp, ci, v)
