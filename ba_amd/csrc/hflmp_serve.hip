@@ -7,9 +7,12 @@
 //
 // Memory.  One device arena of the HBM budget is allocated at creation.  A
 // request is admitted (first come, first served) when a slot is free and
-// its footprint fits in the arena; its CSR arrays, SELL-64 layout(s), inverse
-// diagonal, right-hand side and Krylov vectors then occupy one arena block
-// until its result has been copied back.
+// its footprint fits in the arena; its SELL-64 layout(s), inverse diagonal,
+// right-hand side, Krylov vectors and CSR arrays then occupy one arena
+// block.  The CSR arrays, at the block's end, are read only by the
+// admission kernels that build the SELL-64 layout: that part of the block
+// returns to the arena at once; the rest stays until the result has been
+// copied back.
 //
 // Batching.  The rows of every active system are cut into units of 256
 // consecutive rows (one workgroup, four SELL-64 slices).  One launch of each
@@ -686,10 +689,12 @@ struct Job {
   SlicePlan sp64, sp32;
   std::vector<unsigned char> rlen;
   size_t bytes = 0, off = 0;
+  size_t kept = 0;  // bytes of the block still held (all but the CSR arrays once admitted)
   int slot = -1;
   int state = kQueued;
   SysDesc d{};
-  // device-side block layout (offsets inside the job's arena block; 0 = absent)
+  // device-side block layout (offsets inside the job's arena block; arrays a
+  // precision does not use keep offset 0 and are never read)
   size_t o_rp = 0, o_ci = 0, o_val = 0, o_rlen = 0, o_b = 0, o_x = 0, o_part = 0;
   size_t o_soff = 0, o_slen = 0, o_col = 0, o_sval = 0, o_invd = 0, o_r = 0, o_p = 0, o_z = 0,
          o_Ap = 0;
@@ -731,9 +736,6 @@ void plan_job(Job &jb) {
   };
   const size_t n = (size_t)jb.nrow;
   const size_t e64 = (size_t)jb.sp64.entries;
-  jb.o_rp = take(4 * (n + 1));
-  jb.o_ci = take(4 * (size_t)jb.nnz);
-  jb.o_val = take(8 * (size_t)jb.nnz);
   jb.o_rlen = take(n);
   jb.o_b = take(8 * n);
   jb.o_x = take(8 * n);
@@ -761,7 +763,13 @@ void plan_job(Job &jb) {
     jb.o_Ap32 = take(4 * n);
     jb.o_c32 = take(4 * n);
   }
+  // the CSR arrays last: read only at admission, then returned to the
+  // arena, where they join the free space after the block
+  jb.o_rp = take(4 * (n + 1));
+  jb.o_ci = take(4 * (size_t)jb.nnz);
+  jb.o_val = take(8 * (size_t)jb.nnz);
   jb.bytes = o;
+  jb.kept = o;
 }
 
 }  // namespace
@@ -791,6 +799,8 @@ struct hflmp_server {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   long long steps = 0, graph_captures = 0;
+  int peak_active = 0;
+  size_t peak_in_use = 0;
 };
 
 namespace {
@@ -1125,11 +1135,19 @@ int hflmp_server_step(hflmp_server *sv, int *n_done) {
       sv->alloc.release(jb.off, jb.bytes);
       return rc;
     }
+    // The CSR arrays at the block's end are read only by the admission
+    // kernels just queued (band check, SELL-64 fill); every later use of
+    // that range (another system's uploads and kernels) is queued on the
+    // same stream, after them.
+    sv->alloc.release(jb.off + jb.o_rp, jb.bytes - jb.o_rp);
+    jb.kept = jb.o_rp;
     sv->slot_job[slot] = jb.ticket;
     sv->order.push_back(slot);
     sv->queue.pop_front();
     sv->set_changed = true;
   }
+  sv->peak_active = std::max(sv->peak_active, (int)sv->order.size());
+  sv->peak_in_use = std::max(sv->peak_in_use, sv->budget - sv->alloc.total_free());
   if (sv->set_changed) {
     int rc = rebuild_tables(sv);
     if (rc) return rc;
@@ -1209,7 +1227,7 @@ int hflmp_server_step(hflmp_server *sv, int *n_done) {
       jb.col_bytes = st.col16 ? 2 : 4;
       jb.t_done = Clock::now();
       jb.state = kDone;
-      sv->alloc.release(jb.off, jb.bytes);
+      sv->alloc.release(jb.off, jb.kept);
       sv->slot_job[slot] = -1;
       sv->set_changed = true;
       ++done;
@@ -1262,6 +1280,8 @@ int hflmp_server_stats(const hflmp_server *sv, hflmp_server_stats_t *out) {
   out->arena_largest_free = sv->alloc.largest();
   out->steps = sv->steps;
   out->graph_captures = sv->graph_captures;
+  out->peak_active = sv->peak_active;
+  out->peak_arena_in_use = sv->peak_in_use;
   return HFLMP_OK;
 }
 

@@ -1022,7 +1022,8 @@ class ServerResult(C.Structure):
 class ServerStats(C.Structure):
     _fields_ = [("queued", C.c_int), ("active", C.c_int), ("active_units", C.c_int),
                 ("arena_free", C.c_size_t), ("arena_largest_free", C.c_size_t),
-                ("steps", C.c_longlong), ("graph_captures", C.c_longlong)]
+                ("steps", C.c_longlong), ("graph_captures", C.c_longlong),
+                ("peak_active", C.c_int), ("peak_arena_in_use", C.c_size_t)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

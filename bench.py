@@ -771,19 +771,27 @@ def run_server_stream(hflmp, ctx, feed_specs, arrivals, cfg, rtol, precision, ke
         infos, lat, xs = [None] * n, np.zeros(n), {}
         by_ticket = {}
         nxt = done = 0
+        t_feed = t_submit = t_step = 0.0
         t0 = time.perf_counter()
         while nxt < n or by_ticket:
             now = time.perf_counter() - t0
             while nxt < n and ((arrivals is None and sv.stats()["queued"] < max_queued)
                                or (arrivals is not None and arrivals[nxt] <= now)):
+                tw = time.perf_counter()
                 r = feed.get(nxt)
+                tg = time.perf_counter()
                 by_ticket[sv.submit(*r, rtol=rtol, precision=precision)] = nxt
+                t_feed += tg - tw
+                t_submit += time.perf_counter() - tg
                 nxt += 1
                 now = time.perf_counter() - t0
             if not by_ticket:
                 time.sleep(max(0.0, min(arrivals[nxt] - now, 0.005)))
                 continue
-            if sv.step():
+            ts = time.perf_counter()
+            stepped = sv.step()
+            t_step += time.perf_counter() - ts
+            if stepped:
                 now = time.perf_counter() - t0
                 for t in list(by_ticket):
                     if sv.poll(t)["state"] == 2:
@@ -800,6 +808,8 @@ def run_server_stream(hflmp, ctx, feed_specs, arrivals, cfg, rtol, precision, ke
                                         done, n))
         makespan = time.perf_counter() - t0
         stats = sv.stats()
+        stats["host_seconds"] = {"waiting_for_generated_requests": round(t_feed, 3),
+                                 "submit": round(t_submit, 3), "step": round(t_step, 3)}
     finally:
         sv.close()
         feed.close()
@@ -897,6 +907,9 @@ def bench_serve(hflmp, nreq, seed, rate, ref_style, graphs=True, ntune=32, sampl
            "server": {"config": cfg, "solves_per_s": round(capacity, 2),
                       "makespan_s": round(makespan, 3), "steps": stats["steps"],
                       "graph_captures": stats["graph_captures"],
+                      "host_seconds": stats["host_seconds"],
+                      "peak_active": stats["peak_active"],
+                      "peak_arena_in_use_GB": round(stats["peak_arena_in_use"] / 1e9, 3),
                       "mixed_requests": nmixed,
                       "all_converged": all(i["status"] == 0 for i in infos),
                       "iterations_total": int(sum(i["iters"] for i in infos)),

@@ -810,7 +810,10 @@ typedef struct {
                              before any product; x is set to 0)           */
   double res_init, res_final;
   int zero_diagonals;     /* rows whose diagonal is missing or zero        */
-  size_t footprint;       /* arena bytes while active                      */
+  size_t footprint;       /* arena bytes that admission takes; the CSR
+                             arrays' share (12 nnz + 4 (nrow+1) bytes,
+                             rounded) returns to the arena as soon as the
+                             SELL-64 layout is built                       */
   double queued_seconds;  /* submit -> admission (-1 while queued)         */
   double latency_seconds; /* submit -> x in the caller's buffer (-1 before) */
   int precision;          /* 64 or 32 (FP32-inner / FP64-refined)          */
@@ -823,6 +826,8 @@ typedef struct {
   int queued, active, active_units;
   size_t arena_free, arena_largest_free;
   long long steps, graph_captures;
+  int peak_active;          /* most systems active at once so far          */
+  size_t peak_arena_in_use; /* most arena bytes held at once so far        */
 } hflmp_server_stats_t;
 int hflmp_server_create(hflmp_ctx *ctx, const hflmp_server_config *cfg,
                         hflmp_server **out);

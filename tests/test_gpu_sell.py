@@ -169,3 +169,52 @@ def test_mpcg_sell_reaches_target(ctx, flags):
     assert np.linalg.norm(x - xo) <= 1e-8 * np.linalg.norm(xo)
     S64.free()
     S32.free()
+
+
+def _host_threads():
+    import os
+    try:
+        return max(1, min(16, len(os.sched_getaffinity(0))))
+    except AttributeError:
+        return 1
+
+
+@pytest.mark.parametrize("flags", [0, hflmp.SELL_COL16])
+def test_sell_128cube_full_size_vs_oracle(ctx, flags):
+    # the headline system at its stated size: 2,146,689 rows, 57,066,625
+    # nonzeros; every entry of y bit-equal to the oracle's CSR loop
+    rp, ci, v = hflmp.fe_system(1, 3, 128)
+    A = hflmp.CsrMatrix(ctx, rp, ci, v)
+    S = hflmp.SellMatrix(ctx, A, flags)
+    x = np.random.default_rng(21).uniform(-1, 1, A.nrow)
+    dx, dy = ctx.upload(x), ctx.empty(A.nrow, np.float64)
+    try:
+        S.spmv(dx, dy)
+        assert np.array_equal(dy.download(), O.spmv(rp, ci, v, x))
+        assert S.info()["col_bytes"] == (2 if flags else 4)
+    finally:
+        dx.free(); dy.free(); S.free(); A.free()
+
+
+def test_pcg_sell_128cube_vs_oracle_cg(ctx):
+    # Jacobi-CG to 1e-8 on the 128^3 Poisson system: the device solve
+    # (SELL-64, int16 columns) against the oracle's CG::SolvePrecond on the
+    # host (its SpMV row loop on host threads: the same bits); iterations
+    # within 2 and the solution within 1e-8
+    n = 128
+    rp, ci, v = hflmp.fe_system(1, 3, n)
+    b = hflmp.fe_rhs(1, 3, n, 20241202)
+    A = hflmp.CsrMatrix(ctx, rp, ci, v)
+    S = hflmp.SellMatrix(ctx, A, hflmp.SELL_COL16)
+    invd = hflmp.invdiag(ctx, A)
+    db, dx = ctx.upload(b), ctx.zeros(A.nrow, np.float64)
+    try:
+        info = hflmp.pcg_jacobi_sell(ctx, S, invd, db, dx, rtol=1e-8, maxit=2000)
+        xo, io = O.cg(rp, ci, v, b, prec=O.PREC_JACOBI, rtol=1e-8, maxit=2000,
+                      threads=_host_threads())
+        assert info["status"] == 0 and io["status"] == 0
+        assert abs(info["iters"] - io["iters"]) <= 2, (info["iters"], io["iters"])
+        x = dx.download()
+        assert np.linalg.norm(x - xo) <= 1e-8 * np.linalg.norm(xo)
+    finally:
+        db.free(); dx.free(); invd.free(); S.free(); A.free()

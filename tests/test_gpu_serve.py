@@ -82,11 +82,20 @@ def test_serve_alone_equals_mix(ctx, graphs):
     assert st["active"] == 0 and st["queued"] == 0
 
 
+def csr_share(rp, ci):
+    """Arena bytes of a system's CSR arrays (row_ptr, col, FP64 values, each
+    rounded to 256 B), returned to the arena once it is admitted."""
+    n, nnz = len(rp) - 1, len(ci)
+    return ((4 * (n + 1) + 255) // 256 + (4 * nnz + 255) // 256 + (8 * nnz + 255) // 256) * 256
+
+
 def test_serve_admission_by_footprint(ctx):
     reqs = [request(2, 100, 40 + k) for k in range(6)]
     foot = hflmp.server_footprint(reqs[0][0], reqs[0][1])
-    # room for two systems at a time
-    budget = max(2 * foot + foot // 2, MB)
+    held = foot - csr_share(reqs[0][0], reqs[0][1])
+    # room for two systems at a time: one admitted (its CSR share returned)
+    # and a second one's whole footprint, but not a third
+    budget = max(held + foot + held // 2, MB)
     sv = hflmp.Server(ctx, budget, max_active=16, chunk_iters=16)
     try:
         tickets = [sv.submit(*r) for r in reqs]
@@ -315,3 +324,33 @@ def test_serve_refuses_out_of_range_column(ctx, precision, bad_col):
     grp, gci, gv, gb = good
     # FP64 stops on the recursive residual, mixed on the true one
     assert np.linalg.norm(gb - O.spmv(grp, gci, gv, x_good)) < 1.1e-8 * np.linalg.norm(gb)
+
+
+def test_serve_csr_share_returns_to_the_arena(ctx):
+    # Two copies of one system under a budget that holds one whole block
+    # plus all of a second but its CSR share: the first system's CSR arrays
+    # go back to the arena once its SELL-64 layout is built, so the second
+    # is admitted in the same step and both run at once, with the results
+    # they have alone.
+    rp, ci, v, b = request(2, 96, 31)
+    foot = hflmp.server_footprint(rp, ci)
+    csr = csr_share(rp, ci)
+    budget = 2 * foot - csr // 2
+    assert budget < 2 * foot and budget >= 2 * foot - csr
+    sv = hflmp.Server(ctx, budget, max_active=4, chunk_iters=8)
+    try:
+        out = run_all(sv, [(rp, ci, v, b), (rp, ci, v, b)])
+        st = sv.stats()
+    finally:
+        sv.close()
+    assert st["peak_active"] == 2
+    assert st["peak_arena_in_use"] <= 2 * foot - csr
+    (x1, i1), (x2, i2) = out
+    assert i1["status"] == 0 and i2["status"] == 0
+    assert np.array_equal(x1, x2) and i1["iters"] == i2["iters"]
+    sv1 = hflmp.Server(ctx, budget, max_active=4, chunk_iters=8)
+    try:
+        (xa, ia), = run_all(sv1, [(rp, ci, v, b)])
+    finally:
+        sv1.close()
+    assert np.array_equal(xa, x1) and ia["iters"] == i1["iters"]
