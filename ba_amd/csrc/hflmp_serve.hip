@@ -801,6 +801,9 @@ struct hflmp_server {
   long long steps = 0, graph_captures = 0;
   int peak_active = 0;
   size_t peak_in_use = 0;
+  // host time of step(): admission (uploads, layout kernels queued), table
+  // rebuild and graph capture, waiting for the step's kernels
+  double admit_seconds = 0, capture_seconds = 0, wait_seconds = 0;
 };
 
 namespace {
@@ -1119,6 +1122,7 @@ int hflmp_server_step(hflmp_server *sv, int *n_done) {
   hflmp_ctx *ctx = sv->ctx;
   hipStream_t s = ctx->stream;
   int done = 0;
+  const auto t_begin = Clock::now();
   // admit first come, first served while a slot is free and the block fits
   while (!sv->queue.empty()) {
     int slot = -1;
@@ -1148,6 +1152,8 @@ int hflmp_server_step(hflmp_server *sv, int *n_done) {
   }
   sv->peak_active = std::max(sv->peak_active, (int)sv->order.size());
   sv->peak_in_use = std::max(sv->peak_in_use, sv->budget - sv->alloc.total_free());
+  const auto t_admitted = Clock::now();
+  sv->admit_seconds += std::chrono::duration<double>(t_admitted - t_begin).count();
   if (sv->set_changed) {
     int rc = rebuild_tables(sv);
     if (rc) return rc;
@@ -1179,6 +1185,7 @@ int hflmp_server_step(hflmp_server *sv, int *n_done) {
         }
         ++sv->graph_captures;
       }
+      sv->capture_seconds += std::chrono::duration<double>(Clock::now() - t_admitted).count();
       HFLMP_HIP_CHECK(ctx, hipGraphLaunch(sv->graph_exec, s));
     } else {
       enqueue_chunk(sv, s);
@@ -1188,7 +1195,9 @@ int hflmp_server_step(hflmp_server *sv, int *n_done) {
     HFLMP_HIP_CHECK(ctx, hipMemcpyAsync(sv->h_states, sv->d_states,
                                         sizeof(SlotState) * sv->max_active,
                                         hipMemcpyDeviceToHost, s));
+    const auto t_wait = Clock::now();
     HFLMP_HIP_CHECK(ctx, hipStreamSynchronize(s));
+    sv->wait_seconds += std::chrono::duration<double>(Clock::now() - t_wait).count();
     // retire the systems that stopped
     std::vector<int> keep;
     for (int slot : sv->order) {
@@ -1282,6 +1291,9 @@ int hflmp_server_stats(const hflmp_server *sv, hflmp_server_stats_t *out) {
   out->graph_captures = sv->graph_captures;
   out->peak_active = sv->peak_active;
   out->peak_arena_in_use = sv->peak_in_use;
+  out->admit_seconds = sv->admit_seconds;
+  out->capture_seconds = sv->capture_seconds;
+  out->wait_seconds = sv->wait_seconds;
   return HFLMP_OK;
 }
 

@@ -1023,7 +1023,9 @@ class ServerStats(C.Structure):
     _fields_ = [("queued", C.c_int), ("active", C.c_int), ("active_units", C.c_int),
                 ("arena_free", C.c_size_t), ("arena_largest_free", C.c_size_t),
                 ("steps", C.c_longlong), ("graph_captures", C.c_longlong),
-                ("peak_active", C.c_int), ("peak_arena_in_use", C.c_size_t)]
+                ("peak_active", C.c_int), ("peak_arena_in_use", C.c_size_t),
+                ("admit_seconds", C.c_double), ("capture_seconds", C.c_double),
+                ("wait_seconds", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -604,7 +604,7 @@ def bench_mixed(hflmp, n):
     return out
 
 
-def bench_batched(hflmp, k=64, n=512, ncheck_cpu=2):
+def bench_batched(hflmp, k=64, n=512, ncheck_cpu=2, spmm_only=False):
     """C4: 2D Q2 512^2, 64 right-hand sides at once (lockstep Jacobi-PCG)
     vs the same right-hand sides solved one at a time: FP64 to 1e-8, plain
     FP32 to 1e-5, and FP32-inner / FP64-refined (MPIR per column) to 1e-8.
@@ -613,6 +613,8 @@ def bench_batched(hflmp, k=64, n=512, ncheck_cpu=2):
     rp, ci, v = hflmp.fe_system(2, 2, n)
     N, nnz = len(rp) - 1, len(ci)
     B = np.stack([hflmp.fe_rhs(2, 2, n, BATCH_SEED0 + c) for c in range(k)], axis=1)
+    if spmm_only:
+        return bench_spmm_products(hflmp, rp, ci, v, B, k)
     cols_cpu = [int(c) for c in np.linspace(0, k - 1, ncheck_cpu).round()]
     xcpu = {c: cpu_solve_for_parity(rp, ci, v, np.ascontiguousarray(B[:, c]), 1e-8)[0]
             for c in cols_cpu}
@@ -634,6 +636,15 @@ def bench_batched(hflmp, k=64, n=512, ncheck_cpu=2):
         dX = ctx.zeros(N * k, dt)
         vb = np.dtype(dt).itemsize
         t_mm = time_launches(ctx, lambda: hflmp.spmm(ctx, A, dB, dX, k), 20)
+        # the same product through the row-block layout (X rows staged in LDS)
+        Y0 = dX.download()
+        P = hflmp.SpmmPlan(ctx, A, k)
+        t_plan = time_launches(ctx, lambda: P.spmm(A, dB, dX, k), 20)
+        plan_rep = {"us": round(t_plan * 1e6, 2),
+                    "GBps": round(spmm_bytes(N, nnz, k, vb) / t_plan / 1e9, 2),
+                    "bit_identical_to_csr_spmm": bool(np.array_equal(dX.download(), Y0)),
+                    **P.info()}
+        P.free()
         ctx.check(ctx.L.hflmp_fill_zero(ctx.h, dX.ptr, dX.nbytes), "fill")
         info = hflmp.batched_pcg_jacobi(ctx, A, invd, dB, dX, k, rtol=rtol, maxit=50000)
         X = dX.download().reshape(N, k).astype(np.float64)
@@ -653,6 +664,7 @@ def bench_batched(hflmp, k=64, n=512, ncheck_cpu=2):
                      "spmm_GBps": round(spmm_bytes(N, nnz, k, vb) / t_mm / 1e9, 2),
                      "spmm_frac_hbm": round(spmm_bytes(N, nnz, k, vb) / t_mm / 1e9
                                             / HBM_PEAK_GBS, 4),
+                     "spmm_row_block_layout": plan_rep,
                      "iters_max": int(info["iters"].max()), "iters_min": int(info["iters"].min()),
                      "all_converged": bool((info["status"] == 0).all()),
                      "seconds_batched": round(info["seconds"], 4),
@@ -692,6 +704,32 @@ def bench_batched(hflmp, k=64, n=512, ncheck_cpu=2):
         "max_true_rel_residual_cpu_sample": tr,
         "max_rel_error_vs_fp64_cpu": cpu_err(X)}
     dB.free(); dX.free(); invd32.free(); A32.val.free(); A.free()
+    ctx.close()
+    return out
+
+
+def bench_spmm_products(hflmp, rp, ci, v, B, k, reps=20):
+    """Only the block products of the batched leg: Y = A X for the 64
+    right-hand sides, FP64 and FP32, through the CSR SpMM and through the
+    row-block layout (bit-identity checked)."""
+    N, nnz = len(rp) - 1, len(ci)
+    ctx = hflmp.Context(0)
+    out = {"workload": "2D Q2 Poisson 512^2, Y = A X for %d right-hand sides" % k}
+    for dt, name in ((np.float64, "fp64"), (np.float32, "fp32")):
+        vb = np.dtype(dt).itemsize
+        A = hflmp.CsrMatrix(ctx, rp, ci, v.astype(dt))
+        dX, dY = ctx.upload(B.astype(dt).ravel()), ctx.empty(N * k, dt)
+        t_mm = time_launches(ctx, lambda: hflmp.spmm(ctx, A, dX, dY, k), reps)
+        Y0 = dY.download()
+        P = hflmp.SpmmPlan(ctx, A, k)
+        t_plan = time_launches(ctx, lambda: P.spmm(A, dX, dY, k), reps)
+        nb = spmm_bytes(N, nnz, k, vb)
+        out[name] = {"bytes": nb, "csr_us": round(t_mm * 1e6, 2),
+                     "csr_frac_hbm": round(nb / t_mm / 1e9 / HBM_PEAK_GBS, 4),
+                     "row_block_us": round(t_plan * 1e6, 2),
+                     "row_block_frac_hbm": round(nb / t_plan / 1e9 / HBM_PEAK_GBS, 4),
+                     "bit_identical": bool(np.array_equal(dY.download(), Y0)), **P.info()}
+        P.free(); dX.free(); dY.free(); A.free()
     ctx.close()
     return out
 
@@ -809,7 +847,10 @@ def run_server_stream(hflmp, ctx, feed_specs, arrivals, cfg, rtol, precision, ke
         makespan = time.perf_counter() - t0
         stats = sv.stats()
         stats["host_seconds"] = {"waiting_for_generated_requests": round(t_feed, 3),
-                                 "submit": round(t_submit, 3), "step": round(t_step, 3)}
+                                 "submit": round(t_submit, 3), "step": round(t_step, 3),
+                                 "step_admit": round(stats["admit_seconds"], 3),
+                                 "step_tables_and_graphs": round(stats["capture_seconds"], 3),
+                                 "step_wait_for_kernels": round(stats["wait_seconds"], 3)}
     finally:
         sv.close()
         feed.close()
@@ -979,6 +1020,8 @@ def main():
     ap.add_argument("--no-mg", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-c1", action="store_true")
+    ap.add_argument("--batched-spmm-only", action="store_true",
+                    help="in the batched leg, only the block products (no solves)")
     ap.add_argument("--serve-requests", type=int, default=10000)
     ap.add_argument("--serve-seed", type=int, default=20241205)
     ap.add_argument("--serve-rate", type=float, default=0.0,
@@ -1119,7 +1162,7 @@ def main():
         out["mixed_precision"] = bench_mixed(hflmp, n)
         progress("mixed-precision leg done")
     if not args.no_batched:
-        out["batched"] = bench_batched(hflmp)
+        out["batched"] = bench_batched(hflmp, spmm_only=args.batched_spmm_only)
         progress("batched leg done")
     if not args.no_serve:
         out["serving"] = bench_serve(hflmp, args.serve_requests, args.serve_seed,

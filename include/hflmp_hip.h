@@ -828,6 +828,10 @@ typedef struct {
   long long steps, graph_captures;
   int peak_active;          /* most systems active at once so far          */
   size_t peak_arena_in_use; /* most arena bytes held at once so far        */
+  /* host seconds spent in step() so far: admitting (uploads, layout
+   * kernels queued), rebuilding tables and capturing graphs, and waiting
+   * for the steps' kernels */
+  double admit_seconds, capture_seconds, wait_seconds;
 } hflmp_server_stats_t;
 int hflmp_server_create(hflmp_ctx *ctx, const hflmp_server_config *cfg,
                         hflmp_server **out);
