@@ -46,6 +46,7 @@
 #include <map>
 #include <vector>
 
+#include "hflmp_cg3.hpp"
 #include "hflmp_sell.hpp"
 #include "hflmp_solver_common.hpp"
 
@@ -96,7 +97,7 @@ struct SysDesc {
 };
 
 template <typename IT>
-__device__ __forceinline__ SellOp<double, IT> op64(const SysDesc &d) {
+__host__ __device__ __forceinline__ SellOp<double, IT> op64(const SysDesc &d) {
   SellOp<double, IT> op;
   op.nrow = d.nrow;
   op.nslice = d.nslice;
@@ -678,6 +679,14 @@ void plan_slices(int nrow, const int *rp, SlicePlan &sp) {
   sp.entries = tot;
 }
 
+// A solo system's partial sums: one pair per unit of its SELL-64 grid for
+// the start, then the three-launch iteration's (at most 1280 of K1 and
+// 2 x 1024 of K2 on gfx950).
+inline size_t solo_part_values(int nrow) {
+  const size_t g = 2 * (size_t)sell_grid(nrow);
+  return g > 4096 ? g : 4096;
+}
+
 struct Job {
   long long ticket = 0;
   int nrow = 0, nnz = 0, nslice = 0, nunits = 0, mode = kModeFp64, col16 = 0;
@@ -690,6 +699,8 @@ struct Job {
   std::vector<unsigned char> rlen;
   size_t bytes = 0, off = 0;
   size_t kept = 0;  // bytes of the block still held (all but the CSR arrays once admitted)
+  bool solo = false;  // FP64, at least solo_min_nnz nonzeros: runs alone on the solo stream
+  int col16_64 = 0;   // solo: the FP64 layout's column width as the device decided it
   int slot = -1;
   int state = kQueued;
   SysDesc d{};
@@ -739,7 +750,7 @@ void plan_job(Job &jb) {
   jb.o_rlen = take(n);
   jb.o_b = take(8 * n);
   jb.o_x = take(8 * n);
-  jb.o_part = take(16 * (size_t)jb.nunits);
+  jb.o_part = take(8 * (jb.solo ? solo_part_values(jb.nrow) : 2 * (size_t)jb.nunits));
   jb.o_soff = take(8 * ((size_t)jb.nslice + 1));
   jb.o_slen = take(4 * (size_t)jb.nslice);
   jb.o_col = take(4 * e64);  // int32 width; an int16 layout uses the first half
@@ -801,6 +812,14 @@ struct hflmp_server {
   long long steps = 0, graph_captures = 0;
   int peak_active = 0;
   size_t peak_in_use = 0;
+  // systems that run alone: the single-system three-launch iteration on a
+  // second stream, beside the grouped launches of the others
+  int solo_min_nnz = 0;
+  hipStream_t solo_stream = nullptr;
+  hipEvent_t ev_admitted = nullptr, ev_solo_done = nullptr;
+  std::vector<int> solo;        // active solo slots
+  SlotState *h_admit = nullptr; // pinned: a solo system's state read at admission
+  long long solo_runs = 0;
   // host time of step(): admission (uploads, layout kernels queued), table
   // rebuild and graph capture, waiting for the step's kernels
   double admit_seconds = 0, capture_seconds = 0, wait_seconds = 0;
@@ -834,6 +853,42 @@ void enqueue_chunk(hflmp_server *sv, hipStream_t s) {
     grp_update_kernel<<<gU, kBlock, 0, s>>>(sv->d_sys, sv->d_unit_sys, sv->U);
     grp_beta_final<<<gS, kBlock, 0, s>>>(sv->d_sys);
     grp_p_kernel<<<gU, kBlock, 0, s>>>(sv->d_sys, sv->d_unit_sys, sv->U);
+  }
+}
+
+// A solo system runs the single-system solver's kernels on its arena block
+// (hflmp_cg3.hpp): the start from x = 0, then chunks of the three-launch
+// iteration, with the grids hflmp_pcg_jacobi_sell_f64 chooses for the same
+// system, so its iterates are that solver's, bit for bit.
+template <typename IT>
+void solo_start(const Job &jb, hipStream_t s) {
+  const SellOp<double, IT> op = op64<IT>(jb.d);
+  CgState<double> *st = &jb.d.st->s64;
+  const int grid1 = op_grid(op);
+  init_state_kernel<double><<<1, kBlock, 0, s>>>(st, jb.maxit, jb.atol, jb.rtol, 1e300);
+  cg_init_kernel<SellOp<double, IT>><<<grid1, kBlock, 0, s>>>(
+      op, jb.d.invd, jb.d.b, jb.d.x, jb.d.r, jb.d.z, jb.d.p, jb.d.part);
+  cg_init_final<double><<<1, kFinalBlock, 0, s>>>(st, jb.d.part, grid1);
+}
+
+template <typename IT>
+void solo_chunk(hflmp_ctx *ctx, const Job &jb, hipStream_t s, int iters) {
+  const SellOp<double, IT> op = op64<IT>(jb.d);
+  CgState<double> *st = &jb.d.st->s64;
+  const int grid1 = op_grid(op);
+  long long g2 = ((long long)jb.nrow + 4LL * kBlock - 1) / (4LL * kBlock);
+  if (g2 > kMaxRedBlocks) g2 = kMaxRedBlocks;
+  if (g2 < 1) g2 = 1;
+  const Cg3Grids G = cg3_grids<SellOp<double, IT>>(
+      ctx, grid1, (int)g2, (int)(solo_part_values(jb.nrow) / 2));
+  double *part2 = jb.d.part + G.g1;
+  for (int it = 0; it < iters; ++it) {
+    cg3_spmv_kernel<SellOp<double, IT>><<<G.g1, kBlock, 0, s>>>(op, grid1, jb.d.p, jb.d.Ap, st,
+                                                                 jb.d.part);
+    cg3_update_kernel<double><<<G.g2, kBlock, 0, s>>>(jb.nrow, G.g1, jb.d.invd, jb.d.p,
+                                                      jb.d.Ap, jb.d.x, jb.d.r, jb.d.z, st,
+                                                      jb.d.part, part2);
+    cg3_p_kernel<double><<<G.g2, kBlock, 0, s>>>(jb.nrow, G.g2, jb.d.z, jb.d.p, st, part2);
   }
 }
 
@@ -992,6 +1047,71 @@ int admit(hflmp_server *sv, Job &jb, int slot) {
   return HFLMP_OK;
 }
 
+// The grouped systems' chunk on the context stream: a replay of the
+// captured graph (captured again after the set changed), or direct launches.
+int launch_group(hflmp_server *sv) {
+  hflmp_ctx *ctx = sv->ctx;
+  hipStream_t s = ctx->stream;
+  const auto t0 = Clock::now();
+  if (!sv->use_graph) {
+    enqueue_chunk(sv, s);
+    HFLMP_LAUNCH_CHECK(ctx);
+    return HFLMP_OK;
+  }
+  if (!sv->graph_exec) {
+    // a graph left by a failed instantiation is freed before recapturing;
+    // the capture is always ended, and a partial graph destroyed
+    drop_graph(sv);
+    HFLMP_HIP_CHECK(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    enqueue_chunk(sv, s);
+    const hipError_t launch = hipGetLastError();
+    hipGraph_t g = nullptr;
+    const hipError_t end = hipStreamEndCapture(s, &g);
+    if (end != hipSuccess || launch != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return set_error(ctx, HFLMP_EHIP, "server: graph capture failed (%s)",
+                       hipGetErrorString(end != hipSuccess ? end : launch));
+    }
+    sv->graph = g;
+    const hipError_t inst = hipGraphInstantiate(&sv->graph_exec, sv->graph, nullptr, nullptr, 0);
+    if (inst != hipSuccess) {
+      sv->graph_exec = nullptr;
+      drop_graph(sv);
+      return set_error(ctx, HFLMP_EHIP, "server: graph instantiation failed (%s)",
+                       hipGetErrorString(inst));
+    }
+    ++sv->graph_captures;
+  }
+  sv->capture_seconds += std::chrono::duration<double>(Clock::now() - t0).count();
+  HFLMP_HIP_CHECK(ctx, hipGraphLaunch(sv->graph_exec, s));
+  return HFLMP_OK;
+}
+
+// After a solo system's admission kernels: read its state back once (the
+// FP64 layout's column width, decided by the band kernel, and whether the
+// system was refused), then queue its start on the solo stream after the
+// admission work (x = 0 first).
+int start_solo(hflmp_server *sv, Job &jb, int slot) {
+  hflmp_ctx *ctx = sv->ctx;
+  hipStream_t s = ctx->stream;
+  HFLMP_HIP_CHECK(ctx, hipMemcpyAsync(sv->h_admit, sv->d_states + slot, sizeof(SlotState),
+                                      hipMemcpyDeviceToHost, s));
+  HFLMP_HIP_CHECK(ctx, hipStreamSynchronize(s));
+  jb.col16_64 = sv->h_admit->col16;
+  if (sv->h_admit->s64.status == kRefused) return HFLMP_OK;  // retired at the next step
+  int rc = hflmp_fill_zero(ctx, jb.d.x, 8 * (size_t)jb.nrow);
+  if (rc) return rc;
+  HFLMP_HIP_CHECK(ctx, hipEventRecord(sv->ev_admitted, s));
+  HFLMP_HIP_CHECK(ctx, hipStreamWaitEvent(sv->solo_stream, sv->ev_admitted, 0));
+  if (jb.col16_64)
+    solo_start<short>(jb, sv->solo_stream);
+  else
+    solo_start<int>(jb, sv->solo_stream);
+  HFLMP_LAUNCH_CHECK(ctx);
+  ++sv->solo_runs;
+  return HFLMP_OK;
+}
+
 bool slot_running(const SlotState &st) {
   if (st.mode == kModeMixed) return st.status == kIterate || st.status == kFresh;
   return st.s64.status == kIterate || st.s64.status == kFresh;
@@ -1025,6 +1145,7 @@ int hflmp_server_create(hflmp_ctx *ctx, const hflmp_server_config *cfg, hflmp_se
   // residual on badly conditioned systems; frequent replacement keeps the
   // reliable updates effective
   sv->inner_rtol = cfg->inner_rtol > 0 ? cfg->inner_rtol : 1e-2;
+  sv->solo_min_nnz = cfg->solo_min_nnz;
   auto fail = [&](const char *what) {
     hflmp_server_destroy(sv);
     return set_error(ctx, HFLMP_ENOMEM, "server_create: %s", what);
@@ -1033,8 +1154,14 @@ int hflmp_server_create(hflmp_ctx *ctx, const hflmp_server_config *cfg, hflmp_se
   if (hipMalloc(&sv->d_states, sizeof(SlotState) * sv->max_active) != hipSuccess ||
       hipMalloc(&sv->d_sys, sizeof(SysDesc) * sv->max_active) != hipSuccess ||
       hipMalloc(&sv->d_bad, sizeof(unsigned int) * sv->max_active) != hipSuccess ||
-      hipHostMalloc(&sv->h_states, sizeof(SlotState) * sv->max_active) != hipSuccess)
+      hipHostMalloc(&sv->h_states, sizeof(SlotState) * sv->max_active) != hipSuccess ||
+      hipHostMalloc(&sv->h_admit, sizeof(SlotState)) != hipSuccess)
     return fail("table allocation failed");
+  // the solo stream keeps the default priority (as every stream here)
+  if (hipStreamCreateWithFlags(&sv->solo_stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&sv->ev_admitted, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&sv->ev_solo_done, hipEventDisableTiming) != hipSuccess)
+    return fail("stream or event creation failed");
   std::memset(sv->h_states, 0, sizeof(SlotState) * sv->max_active);
   sv->alloc.reset(sv->budget);
   sv->slot_job.assign(sv->max_active, -1);
@@ -1046,7 +1173,12 @@ int hflmp_server_create(hflmp_ctx *ctx, const hflmp_server_config *cfg, hflmp_se
 
 void hflmp_server_destroy(hflmp_server *sv) {
   if (!sv) return;
+  if (sv->solo_stream) (void)hipStreamSynchronize(sv->solo_stream);
   if (sv->ctx) (void)hipStreamSynchronize(sv->ctx->stream);
+  if (sv->solo_stream) (void)hipStreamDestroy(sv->solo_stream);
+  if (sv->ev_admitted) (void)hipEventDestroy(sv->ev_admitted);
+  if (sv->ev_solo_done) (void)hipEventDestroy(sv->ev_solo_done);
+  if (sv->h_admit) (void)hipHostFree(sv->h_admit);
   drop_graph(sv);
   (void)hipFree(sv->arena);
   (void)hipFree(sv->d_states);
@@ -1106,6 +1238,7 @@ int hflmp_server_submit(hflmp_server *sv, int nrow, int nnz, const int *row_ptr,
                     ? 32
                     : 64;
   jb.mode = precision == 32 ? kModeMixed : kModeFp64;
+  jb.solo = jb.mode == kModeFp64 && sv->solo_min_nnz > 0 && nnz >= sv->solo_min_nnz;
   plan_job(jb);
   HFLMP_REQUIRE(ctx, jb.bytes <= sv->budget,
                 "server_submit: footprint %zu B exceeds the HBM budget %zu B", jb.bytes,
@@ -1139,18 +1272,26 @@ int hflmp_server_step(hflmp_server *sv, int *n_done) {
       sv->alloc.release(jb.off, jb.bytes);
       return rc;
     }
+    if (jb.solo) {
+      rc = start_solo(sv, jb, slot);
+      if (rc) {
+        sv->alloc.release(jb.off, jb.bytes);
+        return rc;
+      }
+    }
     // The CSR arrays at the block's end are read only by the admission
     // kernels just queued (band check, SELL-64 fill); every later use of
     // that range (another system's uploads and kernels) is queued on the
-    // same stream, after them.
+    // same stream, after them.  (A solo system's start on the solo stream
+    // reads its layout, not these arrays.)
     sv->alloc.release(jb.off + jb.o_rp, jb.bytes - jb.o_rp);
     jb.kept = jb.o_rp;
     sv->slot_job[slot] = jb.ticket;
-    sv->order.push_back(slot);
+    (jb.solo ? sv->solo : sv->order).push_back(slot);
     sv->queue.pop_front();
-    sv->set_changed = true;
+    if (!jb.solo) sv->set_changed = true;
   }
-  sv->peak_active = std::max(sv->peak_active, (int)sv->order.size());
+  sv->peak_active = std::max(sv->peak_active, (int)(sv->order.size() + sv->solo.size()));
   sv->peak_in_use = std::max(sv->peak_in_use, sv->budget - sv->alloc.total_free());
   const auto t_admitted = Clock::now();
   sv->admit_seconds += std::chrono::duration<double>(t_admitted - t_begin).count();
@@ -1158,38 +1299,24 @@ int hflmp_server_step(hflmp_server *sv, int *n_done) {
     int rc = rebuild_tables(sv);
     if (rc) return rc;
   }
-  if (sv->S > 0) {
-    if (sv->use_graph) {
-      if (!sv->graph_exec) {
-        // a graph left by a failed instantiation is freed before recapturing;
-        // the capture is always ended, and a partial graph destroyed
-        drop_graph(sv);
-        HFLMP_HIP_CHECK(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        enqueue_chunk(sv, s);
-        const hipError_t launch = hipGetLastError();
-        hipGraph_t g = nullptr;
-        const hipError_t end = hipStreamEndCapture(s, &g);
-        if (end != hipSuccess || launch != hipSuccess) {
-          if (g) (void)hipGraphDestroy(g);
-          return set_error(ctx, HFLMP_EHIP, "server: graph capture failed (%s)",
-                           hipGetErrorString(end != hipSuccess ? end : launch));
-        }
-        sv->graph = g;
-        const hipError_t inst =
-            hipGraphInstantiate(&sv->graph_exec, sv->graph, nullptr, nullptr, 0);
-        if (inst != hipSuccess) {
-          sv->graph_exec = nullptr;
-          drop_graph(sv);
-          return set_error(ctx, HFLMP_EHIP, "server: graph instantiation failed (%s)",
-                           hipGetErrorString(inst));
-        }
-        ++sv->graph_captures;
+  if (sv->S > 0 || !sv->solo.empty()) {
+    if (sv->S > 0) {
+      int rc = launch_group(sv);
+      if (rc) return rc;
+    }
+    if (!sv->solo.empty()) {
+      // the solo systems' chunks on their stream, beside the group's; the
+      // state copy waits for both
+      for (int slot : sv->solo) {
+        const Job &jb = sv->jobs[sv->slot_job[slot]];
+        if (jb.col16_64)
+          solo_chunk<short>(ctx, jb, sv->solo_stream, sv->chunk);
+        else
+          solo_chunk<int>(ctx, jb, sv->solo_stream, sv->chunk);
       }
-      sv->capture_seconds += std::chrono::duration<double>(Clock::now() - t_admitted).count();
-      HFLMP_HIP_CHECK(ctx, hipGraphLaunch(sv->graph_exec, s));
-    } else {
-      enqueue_chunk(sv, s);
       HFLMP_LAUNCH_CHECK(ctx);
+      HFLMP_HIP_CHECK(ctx, hipEventRecord(sv->ev_solo_done, sv->solo_stream));
+      HFLMP_HIP_CHECK(ctx, hipStreamWaitEvent(s, sv->ev_solo_done, 0));
     }
     ++sv->steps;
     HFLMP_HIP_CHECK(ctx, hipMemcpyAsync(sv->h_states, sv->d_states,
@@ -1199,49 +1326,52 @@ int hflmp_server_step(hflmp_server *sv, int *n_done) {
     HFLMP_HIP_CHECK(ctx, hipStreamSynchronize(s));
     sv->wait_seconds += std::chrono::duration<double>(Clock::now() - t_wait).count();
     // retire the systems that stopped
-    std::vector<int> keep;
-    for (int slot : sv->order) {
-      const SlotState &st = sv->h_states[slot];
-      Job &jb = sv->jobs[sv->slot_job[slot]];
-      if (slot_running(st)) {
-        keep.push_back(slot);
-        continue;
+    for (std::vector<int> *list : {&sv->order, &sv->solo}) {
+      std::vector<int> keep;
+      for (int slot : *list) {
+        const SlotState &st = sv->h_states[slot];
+        Job &jb = sv->jobs[sv->slot_job[slot]];
+        if (slot_running(st)) {
+          keep.push_back(slot);
+          continue;
+        }
+        const bool refused = (st.mode == kModeMixed ? st.status : st.s64.status) == kRefused;
+        if (refused)
+          std::memset(jb.x_out, 0, 8 * (size_t)jb.nrow);
+        else
+          HFLMP_HIP_CHECK(ctx, hipMemcpy(jb.x_out, jb.d.x, 8 * (size_t)jb.nrow,
+                                         hipMemcpyDeviceToHost));
+        unsigned int bad = 0;
+        HFLMP_HIP_CHECK(ctx,
+                        hipMemcpy(&bad, sv->d_bad + slot, sizeof(bad), hipMemcpyDeviceToHost));
+        if (bad) {
+          const int rc = hflmp_fill_zero(ctx, sv->d_bad + slot, sizeof(bad));
+          if (rc) return rc;
+        }
+        if (st.mode == kModeMixed) {
+          jb.iters = st.inner_total;
+          jb.status = st.status;
+          jb.outer = st.outer;
+          jb.res_init = st.res0;
+          jb.res_final = st.res;
+        } else {
+          jb.iters = st.s64.iter;
+          jb.status = st.s64.status;
+          jb.outer = 0;
+          jb.res_init = st.s64.res0;
+          jb.res_final = st.s64.res;
+        }
+        jb.bad_diag = (int)bad;
+        jb.col_bytes = st.col16 ? 2 : 4;
+        jb.t_done = Clock::now();
+        jb.state = kDone;
+        sv->alloc.release(jb.off, jb.kept);
+        sv->slot_job[slot] = -1;
+        if (list == &sv->order) sv->set_changed = true;
+        ++done;
       }
-      const bool refused = (st.mode == kModeMixed ? st.status : st.s64.status) == kRefused;
-      if (refused)
-        std::memset(jb.x_out, 0, 8 * (size_t)jb.nrow);
-      else
-        HFLMP_HIP_CHECK(ctx, hipMemcpy(jb.x_out, jb.d.x, 8 * (size_t)jb.nrow,
-                                       hipMemcpyDeviceToHost));
-      unsigned int bad = 0;
-      HFLMP_HIP_CHECK(ctx, hipMemcpy(&bad, sv->d_bad + slot, sizeof(bad), hipMemcpyDeviceToHost));
-      if (bad) {
-        const int rc = hflmp_fill_zero(ctx, sv->d_bad + slot, sizeof(bad));
-        if (rc) return rc;
-      }
-      if (st.mode == kModeMixed) {
-        jb.iters = st.inner_total;
-        jb.status = st.status;
-        jb.outer = st.outer;
-        jb.res_init = st.res0;
-        jb.res_final = st.res;
-      } else {
-        jb.iters = st.s64.iter;
-        jb.status = st.s64.status;
-        jb.outer = 0;
-        jb.res_init = st.s64.res0;
-        jb.res_final = st.s64.res;
-      }
-      jb.bad_diag = (int)bad;
-      jb.col_bytes = st.col16 ? 2 : 4;
-      jb.t_done = Clock::now();
-      jb.state = kDone;
-      sv->alloc.release(jb.off, jb.kept);
-      sv->slot_job[slot] = -1;
-      sv->set_changed = true;
-      ++done;
+      list->swap(keep);
     }
-    sv->order.swap(keep);
   }
   if (n_done) *n_done = done;
   return HFLMP_OK;
@@ -1283,7 +1413,7 @@ int hflmp_server_release(hflmp_server *sv, long long ticket) {
 int hflmp_server_stats(const hflmp_server *sv, hflmp_server_stats_t *out) {
   if (!sv || !out) return HFLMP_EINVAL;
   out->queued = (int)sv->queue.size();
-  out->active = (int)sv->order.size();
+  out->active = (int)(sv->order.size() + sv->solo.size());
   out->active_units = sv->U;
   out->arena_free = sv->alloc.total_free();
   out->arena_largest_free = sv->alloc.largest();
@@ -1291,6 +1421,7 @@ int hflmp_server_stats(const hflmp_server *sv, hflmp_server_stats_t *out) {
   out->graph_captures = sv->graph_captures;
   out->peak_active = sv->peak_active;
   out->peak_arena_in_use = sv->peak_in_use;
+  out->solo_runs = sv->solo_runs;
   out->admit_seconds = sv->admit_seconds;
   out->capture_seconds = sv->capture_seconds;
   out->wait_seconds = sv->wait_seconds;

@@ -1005,7 +1005,7 @@ def mpcg_jacobi_sell(ctx, S64, S32, invd32, b, x, max_outer=50, atol=0.0, rtol=1
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget", C.c_size_t), ("max_active", C.c_int), ("chunk_iters", C.c_int),
                 ("use_graphs", C.c_int), ("mixed_min_nnz", C.c_int), ("inner_rtol", C.c_double),
-                ("mixed_max_nnz", C.c_int)]
+                ("mixed_max_nnz", C.c_int), ("solo_min_nnz", C.c_int)]
 
 
 class ServerResult(C.Structure):
@@ -1025,7 +1025,7 @@ class ServerStats(C.Structure):
                 ("steps", C.c_longlong), ("graph_captures", C.c_longlong),
                 ("peak_active", C.c_int), ("peak_arena_in_use", C.c_size_t),
                 ("admit_seconds", C.c_double), ("capture_seconds", C.c_double),
-                ("wait_seconds", C.c_double)]
+                ("wait_seconds", C.c_double), ("solo_runs", C.c_longlong)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -1044,11 +1044,11 @@ class Server:
     its result is taken."""
 
     def __init__(self, ctx, hbm_budget, max_active=64, chunk_iters=16, use_graphs=True,
-                 mixed_min_nnz=0, inner_rtol=1e-2, mixed_max_nnz=0):
+                 mixed_min_nnz=0, inner_rtol=1e-2, mixed_max_nnz=0, solo_min_nnz=0):
         self.ctx = ctx
         cfg = ServerConfig(int(hbm_budget), int(max_active), int(chunk_iters),
                            1 if use_graphs else 0, int(mixed_min_nnz), float(inner_rtol),
-                           int(mixed_max_nnz))
+                           int(mixed_max_nnz), int(solo_min_nnz))
         self.h = C.c_void_p()
         ctx.check(ctx.L.hflmp_server_create(ctx.h, C.byref(cfg), C.byref(self.h)),
                   "server_create")

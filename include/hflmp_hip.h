@@ -789,6 +789,12 @@ int hflmp_mpcg_jacobi_sell(hflmp_ctx *ctx, const hflmp_sell *A64,
  * The FP64 SELL-64 operator holds int16 column offsets from each slice's
  * first row when every column fits (decided on the device at admission;
  * the footprint assumes int32), else int32 columns.
+ * FP64 systems with at least solo_min_nnz nonzeros run alone instead: the
+ * single-system iteration of hflmp_pcg_jacobi_sell_f64 (three launches per
+ * iteration, its grids) on a second stream of the server, concurrently
+ * with the grouped launches, chunk_iters iterations per step; their
+ * iterates are that solver's bit for bit (with dtol unbounded), and their
+ * footprint holds a partial-sum area of max(2 x units, 4096) values.
  * The caller's host arrays (row_ptr, col, val, b, x) must stay valid until
  * the ticket is done.  Rows may hold 0..255 entries. */
 typedef struct hflmp_server hflmp_server;
@@ -800,6 +806,8 @@ typedef struct {
   int mixed_min_nnz;    /* precision 0: mixed from this many nonzeros (0: off) */
   double inner_rtol;    /* mixed: residual drop between replacements (0: 1e-2) */
   int mixed_max_nnz;    /* precision 0: mixed below this many nonzeros (0: no bound) */
+  int solo_min_nnz;     /* FP64 systems from this many nonzeros run alone
+                           on the second stream (0: none)                 */
 } hflmp_server_config;
 typedef struct {
   int state;              /* 0 queued, 1 active, 2 done                   */
@@ -832,11 +840,13 @@ typedef struct {
    * kernels queued), rebuilding tables and capturing graphs, and waiting
    * for the steps' kernels */
   double admit_seconds, capture_seconds, wait_seconds;
+  long long solo_runs; /* systems started alone so far (solo_min_nnz)     */
 } hflmp_server_stats_t;
 int hflmp_server_create(hflmp_ctx *ctx, const hflmp_server_config *cfg,
                         hflmp_server **out);
 void hflmp_server_destroy(hflmp_server *sv);
-/* footprint for precision 64 or 32 (col may be NULL: int32 columns assumed) */
+/* footprint for precision 64 or 32 (col may be NULL: int32 columns
+ * assumed), for a system in the grouped launches */
 size_t hflmp_server_footprint(int nrow, int nnz, const int *row_ptr,
                               const int *col, int precision);
 int hflmp_server_submit(hflmp_server *sv, int nrow, int nnz, const int *row_ptr,

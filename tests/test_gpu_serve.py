@@ -301,18 +301,19 @@ def test_serve_tune_small_sample(ctx):
             assert cfg["mixed_min_nnz"] == 0
 
 
-@pytest.mark.parametrize("precision", [64, 32])
+@pytest.mark.parametrize("precision,solo", [(64, 0), (32, 0), (64, 1)])
 @pytest.mark.parametrize("bad_col", [-1, "nrow"])
-def test_serve_refuses_out_of_range_column(ctx, precision, bad_col):
+def test_serve_refuses_out_of_range_column(ctx, precision, solo, bad_col):
     """A request whose column index lies outside 0..nrow-1 is refused on the
     device at admission (status 4, x = 0) before any product gathers with
-    it; a healthy request in the same server is solved as usual."""
+    it; a healthy request in the same server is solved as usual (grouped,
+    or alone with solo_min_nnz = 1)."""
     rp, ci, v, b = request(2, 9, 31)
     ci_bad = ci.copy()
     k = int(rp[40]) + 1  # an entry of an interior row
     ci_bad[k] = -1 if bad_col == -1 else len(rp) - 1
     good = request(2, 12, 32)
-    sv = hflmp.Server(ctx, 64 * MB, max_active=4, chunk_iters=8)
+    sv = hflmp.Server(ctx, 64 * MB, max_active=4, chunk_iters=8, solo_min_nnz=solo)
     try:
         out = run_all(sv, [(rp, ci_bad, v, b), good], precision=precision)
     finally:
@@ -354,3 +355,77 @@ def test_serve_csr_share_returns_to_the_arena(ctx):
     finally:
         sv1.close()
     assert np.array_equal(xa, x1) and ia["iters"] == i1["iters"]
+
+
+def _single_sell_solve(ctx, rp, ci, v, b, rtol=1e-8, maxit=100000):
+    """The single-system device solve over SELL-64 (int16 columns where the
+    band fits), with the server's divergence bound (none)."""
+    A = hflmp.CsrMatrix(ctx, rp, ci, v)
+    S = hflmp.SellMatrix(ctx, A, hflmp.SELL_COL16)
+    invd = hflmp.invdiag(ctx, A)
+    db, dx = ctx.upload(b), ctx.zeros(A.nrow, np.float64)
+    try:
+        info = hflmp.pcg_jacobi_sell(ctx, S, invd, db, dx, rtol=rtol, maxit=maxit, dtol=1e300)
+        return dx.download(), info, S.info()["col_bytes"]
+    finally:
+        S.free(); A.free(); invd.free(); db.free(); dx.free()
+
+
+@pytest.mark.parametrize("graphs", [True, False])
+def test_serve_solo_equals_single_solver(ctx, graphs):
+    """FP64 systems with at least solo_min_nnz nonzeros run alone, on the
+    server's second stream, with the single solver's kernels and grids:
+    iterations, residuals and x are those of hflmp_pcg_jacobi_sell_f64 bit
+    for bit (int16 and int32 columns), while the smaller systems of the same
+    staggered mix run grouped and still match the oracle."""
+    big = [request(3, 24, 61), request(2, 150, 62), _wide_band_system()]
+    small = [request(*m) for m in MIX[:4]]
+    thr = min(len(r[1]) for r in big)
+    assert max(len(r[1]) for r in small) < thr
+    reqs = small[:2] + big[:2] + small[2:] + big[2:]
+    sv = hflmp.Server(ctx, 512 * MB, max_active=8, chunk_iters=8, use_graphs=graphs,
+                      solo_min_nnz=thr)
+    try:
+        out = run_all(sv, reqs, submit_every=2)
+        st = sv.stats()
+    finally:
+        sv.close()
+    assert st["solo_runs"] == 3
+    assert st["active"] == 0 and st["queued"] == 0
+    for (rp, ci, v, b), (x, info) in zip(reqs, out):
+        assert info["status"] == 0
+        if len(ci) >= thr:
+            xs, si, cb = _single_sell_solve(ctx, rp, ci, v, b)
+            assert info["col_bytes"] == cb
+            assert info["iters"] == si["iters"]
+            assert info["res_final"] == si["res_final"] and info["res_init"] == si["res_init"]
+            assert np.array_equal(x, xs)
+        else:
+            xo, io = O.cg(rp, ci, v, b, prec=O.PREC_JACOBI, rtol=1e-8, maxit=100000)
+            assert abs(info["iters"] - io["iters"]) <= 2, (info["iters"], io["iters"])
+            assert np.linalg.norm(x - xo) <= 1e-8 * np.linalg.norm(xo)
+    assert [i["col_bytes"] for _, i in out if i["col_bytes"] == 4] == [4]
+
+
+def test_serve_solo_alone_equals_mix(ctx):
+    """A solo system's result does not depend on what else the server runs:
+    alone, beside grouped systems, or beside another solo system, with
+    another chunk length."""
+    probe = request(2, 150, 62)
+    sv = hflmp.Server(ctx, 256 * MB, max_active=1, chunk_iters=5, solo_min_nnz=100000)
+    try:
+        (x_alone, i_alone), = run_all(sv, [probe])
+    finally:
+        sv.close()
+    reqs = [request(*m) for m in MIX[:3]] + [probe, request(3, 24, 61)] + \
+        [request(*m) for m in MIX[3:6]]
+    sv = hflmp.Server(ctx, 512 * MB, max_active=6, chunk_iters=7, solo_min_nnz=100000)
+    try:
+        out = run_all(sv, reqs, submit_every=3)
+        st = sv.stats()
+    finally:
+        sv.close()
+    assert st["solo_runs"] == 2
+    x_mix, i_mix = out[3]
+    assert i_mix["iters"] == i_alone["iters"]
+    assert np.array_equal(x_mix, x_alone)
