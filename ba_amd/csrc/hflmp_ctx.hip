@@ -21,6 +21,15 @@ __global__ void __launch_bounds__(kBlock) fill_zero_kernel(unsigned char *p, siz
 
 }  // namespace
 
+int fill_zero_on(hflmp_ctx *ctx, void *dptr, size_t bytes, hipStream_t s) {
+  if (bytes == 0) return HFLMP_OK;
+  HFLMP_REQUIRE(ctx, dptr, "fill_zero: null pointer");
+  const unsigned g = grid_for((long long)((bytes + 15) / 16), 4);
+  fill_zero_kernel<<<g, kBlock, 0, s>>>(static_cast<unsigned char *>(dptr), bytes);
+  HFLMP_LAUNCH_CHECK(ctx);
+  return HFLMP_OK;
+}
+
 extern "C" {
 
 int hflmp_device_count(int *count) {
@@ -190,12 +199,7 @@ int hflmp_memcpy_d2d(hflmp_ctx *ctx, void *dst, const void *src, size_t bytes) {
 
 int hflmp_fill_zero(hflmp_ctx *ctx, void *dptr, size_t bytes) {
   if (!ctx) return HFLMP_EINVAL;
-  if (bytes == 0) return HFLMP_OK;
-  HFLMP_REQUIRE(ctx, dptr, "fill_zero: null pointer");
-  const unsigned g = grid_for((long long)((bytes + 15) / 16), 4);
-  fill_zero_kernel<<<g, kBlock, 0, ctx->stream>>>(static_cast<unsigned char *>(dptr), bytes);
-  HFLMP_LAUNCH_CHECK(ctx);
-  return HFLMP_OK;
+  return fill_zero_on(ctx, dptr, bytes, ctx->stream);
 }
 
 }  // extern "C"

@@ -67,6 +67,9 @@ inline int set_error(hflmp_ctx *ctx, int code, const char *fmt, ...) {
 
 inline int check_ctx(hflmp_ctx *ctx) { return ctx ? HFLMP_OK : HFLMP_EINVAL; }
 
+// hflmp_fill_zero on a given stream (errors recorded in ctx).
+int fill_zero_on(hflmp_ctx *ctx, void *dptr, size_t bytes, hipStream_t s);
+
 // Grid for an elementwise kernel that handles `per_thread` items/thread.
 inline unsigned grid_for(long long n, int per_thread) {
   long long g = (n + (long long)kBlock * per_thread - 1) / ((long long)kBlock * per_thread);

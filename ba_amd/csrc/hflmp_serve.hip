@@ -24,13 +24,20 @@
 // not depend on which other systems share the launches: a result is
 // bit-identical alone and in any mix.
 //
-// Steps.  hflmp_server_step admits what fits, then runs one step: the
+// Steps.  hflmp_server_step runs one chunk of the active systems: the
 // initial residuals of new FP64 systems, the FP64 outer step (x += c,
 // r = b - A x, r32 = (float) r, outer test) of the mixed systems whose inner
 // solve has reached its tolerance, and `chunk` iterations -- captured once
-// per active set as a hipGraph and replayed while the set is unchanged.  It
-// then reads the states and retires the systems that stopped (x copied to
-// the caller's buffer).
+// per active set as a hipGraph and replayed while the set is unchanged.
+// While the chunk runs, queued systems are admitted on a second stream
+// (uploads from the caller's host arrays, layout kernels).  Then the states
+// are read, the systems that stopped retire (x copied to the caller's
+// buffer), and the admitted systems join from the next chunk on.
+//
+// Solo systems.  FP64 systems from solo_min_nnz nonzeros run the
+// single-system three-launch iteration (hflmp_cg3.hpp) on a third stream,
+// beside the grouped launches, with the single solver's grids: their
+// iterates are that solver's bit for bit.
 //
 // Mixed systems use reliable updates: the outer step replaces the inner
 // CG's recursive residual by the FP64 true residual and the correction c
@@ -44,6 +51,7 @@
 #include <cstring>
 #include <deque>
 #include <map>
+#include <thread>
 #include <vector>
 
 #include "hflmp_cg3.hpp"
@@ -793,7 +801,12 @@ struct hflmp_server {
   int max_active = 0, chunk = 0, use_graph = 1, mixed_min_nnz = 0, mixed_max_nnz = 0;
   double inner_rtol = 1e-2;
   SlotState *d_states = nullptr;  // [max_active]
-  SlotState *h_states = nullptr;  // pinned mirror
+  SlotState *h_states = nullptr;  // pinned mirror, written by each chunk's state copy
+  SlotState *h_init = nullptr;    // pinned: each slot's initial state at admission
+  hipStream_t admit_stream = nullptr;  // admission work, beside the chunk in flight
+  char *stage = nullptr;               // pinned staging for admission uploads
+  size_t stage_cap = 0, stage_used = 0;
+  std::vector<int> joining;            // admitted slots that join after the chunk
   SysDesc *d_sys = nullptr;       // [max_active]
   int *d_unit_sys = nullptr;
   size_t unit_cap = 0;
@@ -822,7 +835,7 @@ struct hflmp_server {
   long long solo_runs = 0;
   // host time of step(): admission (uploads, layout kernels queued), table
   // rebuild and graph capture, waiting for the step's kernels
-  double admit_seconds = 0, capture_seconds = 0, wait_seconds = 0;
+  double admit_seconds = 0, capture_seconds = 0, wait_seconds = 0, join_seconds = 0;
 };
 
 namespace {
@@ -927,30 +940,84 @@ int rebuild_tables(hflmp_server *sv) {
   return HFLMP_OK;
 }
 
+// Admission work on the admission stream, which runs beside the step in
+// flight: that step's launches never touch this slot or block (both were
+// free when its tables were built).
+// Host copy into the pinned staging area, split over a few threads for
+// large arrays (one thread reaches about a third of what PCIe takes).
+void host_copy(char *dst, const char *src, size_t bytes) {
+  constexpr size_t kSplitBytes = 8u << 20;
+  constexpr int kThreads = 4;
+  if (bytes < kSplitBytes) {
+    std::memcpy(dst, src, bytes);
+    return;
+  }
+  std::thread th[kThreads - 1];
+  const size_t part = (bytes / kThreads + 4095) & ~(size_t)4095;
+  for (int t = 1; t < kThreads; ++t) {
+    const size_t a = std::min(bytes, part * t), b = std::min(bytes, part * (t + 1));
+    th[t - 1] = std::thread([=] { std::memcpy(dst + a, src + a, b - a); });
+  }
+  std::memcpy(dst, src, std::min(bytes, part));
+  for (auto &x : th) x.join();
+}
+
+// Queues an upload of the caller's (pageable) host array on the admission
+// stream through the pinned staging area, so the host never waits for the
+// admission kernels queued before it.  When the area is full the host
+// waits for the admission stream once and starts again at its beginning.
+int stage_upload(hflmp_server *sv, void *dst, const void *src, size_t bytes) {
+  hflmp_ctx *ctx = sv->ctx;
+  constexpr size_t kMinPiece = 4u << 20;
+  const char *p = static_cast<const char *>(src);
+  char *d = static_cast<char *>(dst);
+  while (bytes) {
+    size_t room = sv->stage_cap - sv->stage_used;
+    if (room < bytes && room < kMinPiece) {
+      HFLMP_HIP_CHECK(ctx, hipStreamSynchronize(sv->admit_stream));
+      sv->stage_used = 0;
+      room = sv->stage_cap;
+    }
+    const size_t piece = std::min(bytes, room);
+    char *h = sv->stage + sv->stage_used;
+    host_copy(h, p, piece);
+    HFLMP_HIP_CHECK(ctx, hipMemcpyAsync(d, h, piece, hipMemcpyHostToDevice, sv->admit_stream));
+    sv->stage_used = std::min(sv->stage_cap, align256(sv->stage_used + piece));
+    p += piece;
+    d += piece;
+    bytes -= piece;
+  }
+  return HFLMP_OK;
+}
+
 int admit(hflmp_server *sv, Job &jb, int slot) {
   hflmp_ctx *ctx = sv->ctx;
-  hipStream_t s = ctx->stream;
+  hipStream_t s = sv->admit_stream;
   char *base = sv->arena + jb.off;
   auto dp = [&](size_t o) { return static_cast<void *>(base + o); };
   const size_t n = (size_t)jb.nrow;
   const bool mixed = jb.mode == kModeMixed;
+  int rc = HFLMP_OK;
   auto up = [&](size_t o, const void *src, size_t bytes) {
-    return bytes ? hipMemcpyAsync(dp(o), src, bytes, hipMemcpyHostToDevice, s) : hipSuccess;
+    if (!rc) rc = stage_upload(sv, dp(o), src, bytes);
   };
-  HFLMP_HIP_CHECK(ctx, up(jb.o_rp, jb.rp, 4 * (n + 1)));
-  HFLMP_HIP_CHECK(ctx, up(jb.o_ci, jb.ci, 4 * (size_t)jb.nnz));
-  HFLMP_HIP_CHECK(ctx, up(jb.o_val, jb.val, 8 * (size_t)jb.nnz));
-  HFLMP_HIP_CHECK(ctx, up(jb.o_rlen, jb.rlen.data(), n));
-  HFLMP_HIP_CHECK(ctx, up(jb.o_b, jb.b, 8 * n));
-  HFLMP_HIP_CHECK(ctx, up(jb.o_soff, jb.sp64.soff.data(), 8 * jb.sp64.soff.size()));
-  HFLMP_HIP_CHECK(ctx, up(jb.o_slen, jb.sp64.slen.data(), 4 * jb.sp64.slen.size()));
-  int rc = hflmp_fill_zero(ctx, dp(jb.o_col), 4 * (size_t)jb.sp64.entries);
-  if (!rc) rc = hflmp_fill_zero(ctx, dp(jb.o_sval), 8 * (size_t)jb.sp64.entries);
+  up(jb.o_rp, jb.rp, 4 * (n + 1));
+  up(jb.o_ci, jb.ci, 4 * (size_t)jb.nnz);
+  up(jb.o_val, jb.val, 8 * (size_t)jb.nnz);
+  up(jb.o_rlen, jb.rlen.data(), n);
+  up(jb.o_b, jb.b, 8 * n);
+  up(jb.o_soff, jb.sp64.soff.data(), 8 * jb.sp64.soff.size());
+  up(jb.o_slen, jb.sp64.slen.data(), 4 * jb.sp64.slen.size());
+  if (mixed) {
+    up(jb.o_soff32, jb.sp32.soff.data(), 8 * jb.sp32.soff.size());
+    up(jb.o_slen32, jb.sp32.slen.data(), 4 * jb.sp32.slen.size());
+  }
+  if (!rc) rc = fill_zero_on(ctx, dp(jb.o_col), 4 * (size_t)jb.sp64.entries, s);
+  if (!rc) rc = fill_zero_on(ctx, dp(jb.o_sval), 8 * (size_t)jb.sp64.entries, s);
+  if (!rc) rc = fill_zero_on(ctx, sv->d_bad + slot, sizeof(unsigned int), s);
   if (!rc && mixed) {
-    HFLMP_HIP_CHECK(ctx, up(jb.o_soff32, jb.sp32.soff.data(), 8 * jb.sp32.soff.size()));
-    HFLMP_HIP_CHECK(ctx, up(jb.o_slen32, jb.sp32.slen.data(), 4 * jb.sp32.slen.size()));
-    rc = hflmp_fill_zero(ctx, dp(jb.o_col32), (jb.col16 ? 2 : 4) * (size_t)jb.sp32.entries);
-    if (!rc) rc = hflmp_fill_zero(ctx, dp(jb.o_sval32), 4 * (size_t)jb.sp32.entries);
+    rc = fill_zero_on(ctx, dp(jb.o_col32), (jb.col16 ? 2 : 4) * (size_t)jb.sp32.entries, s);
+    if (!rc) rc = fill_zero_on(ctx, dp(jb.o_sval32), 4 * (size_t)jb.sp32.entries, s);
   }
   if (rc) return rc;
   SlotState st;
@@ -969,10 +1036,10 @@ int admit(hflmp_server *sv, Job &jb, int slot) {
   st.s64.dtol = 1e300;
   st.s32.status = kConverged;
   st.col16 = 1;  // until the band kernel finds an entry that does not fit
-  // the pinned mirror slot is free (the slot was retired) and is only read
-  // after the next state copy, which the stream orders after this one
-  sv->h_states[slot] = st;
-  HFLMP_HIP_CHECK(ctx, hipMemcpyAsync(sv->d_states + slot, sv->h_states + slot, sizeof(st),
+  // staged in the slot's own pinned entry (the state mirror may be written
+  // by the running step's state copy meanwhile)
+  sv->h_init[slot] = st;
+  HFLMP_HIP_CHECK(ctx, hipMemcpyAsync(sv->d_states + slot, sv->h_init + slot, sizeof(st),
                                       hipMemcpyHostToDevice, s));
   // the FP64 layout's column width is decided on the device (no host pass
   // over the columns); the fill and every product read the slot's flag
@@ -1093,13 +1160,14 @@ int launch_group(hflmp_server *sv) {
 // admission work (x = 0 first).
 int start_solo(hflmp_server *sv, Job &jb, int slot) {
   hflmp_ctx *ctx = sv->ctx;
-  hipStream_t s = ctx->stream;
+  hipStream_t s = sv->admit_stream;
   HFLMP_HIP_CHECK(ctx, hipMemcpyAsync(sv->h_admit, sv->d_states + slot, sizeof(SlotState),
                                       hipMemcpyDeviceToHost, s));
   HFLMP_HIP_CHECK(ctx, hipStreamSynchronize(s));
+  sv->stage_used = 0;
   jb.col16_64 = sv->h_admit->col16;
   if (sv->h_admit->s64.status == kRefused) return HFLMP_OK;  // retired at the next step
-  int rc = hflmp_fill_zero(ctx, jb.d.x, 8 * (size_t)jb.nrow);
+  int rc = fill_zero_on(ctx, jb.d.x, 8 * (size_t)jb.nrow, s);
   if (rc) return rc;
   HFLMP_HIP_CHECK(ctx, hipEventRecord(sv->ev_admitted, s));
   HFLMP_HIP_CHECK(ctx, hipStreamWaitEvent(sv->solo_stream, sv->ev_admitted, 0));
@@ -1116,6 +1184,163 @@ bool slot_running(const SlotState &st) {
   if (st.mode == kModeMixed) return st.status == kIterate || st.status == kFresh;
   return st.s64.status == kIterate || st.s64.status == kFresh;
 }
+
+// One step's work, in this order:
+//   launch_chunk:   the active systems' chunk (the grouped launches on the
+//                   context stream, the solo systems' on the solo stream),
+//                   then the copy of all states to the host;
+//   admit_queued:   admission of queued systems on the admission stream,
+//                   while the chunk runs (host uploads, layout kernels);
+//   retire_stopped: wait for the chunk, retire the systems that stopped;
+//   join_admitted:  wait for the admission work; the admitted systems join
+//                   the active set and run from the next chunk on.
+// Slots and blocks freed by retire_stopped are not in the next chunk's
+// tables, so the next admission may reuse them while that chunk runs.
+int launch_chunk(hflmp_server *sv, bool *launched) {
+  hflmp_ctx *ctx = sv->ctx;
+  hipStream_t s = ctx->stream;
+  *launched = false;
+  if (sv->order.empty() && sv->solo.empty()) return HFLMP_OK;
+  const auto t0 = Clock::now();
+  if (sv->set_changed) {
+    int rc = rebuild_tables(sv);
+    if (rc) return rc;
+  }
+  if (sv->S > 0) {
+    int rc = launch_group(sv);
+    if (rc) return rc;
+  }
+  sv->capture_seconds += std::chrono::duration<double>(Clock::now() - t0).count();
+  if (!sv->solo.empty()) {
+    // the solo systems' chunks on their stream, beside the group's; the
+    // state copy waits for both
+    for (int slot : sv->solo) {
+      const Job &jb = sv->jobs[sv->slot_job[slot]];
+      if (jb.col16_64)
+        solo_chunk<short>(ctx, jb, sv->solo_stream, sv->chunk);
+      else
+        solo_chunk<int>(ctx, jb, sv->solo_stream, sv->chunk);
+    }
+    HFLMP_LAUNCH_CHECK(ctx);
+    HFLMP_HIP_CHECK(ctx, hipEventRecord(sv->ev_solo_done, sv->solo_stream));
+    HFLMP_HIP_CHECK(ctx, hipStreamWaitEvent(s, sv->ev_solo_done, 0));
+  }
+  ++sv->steps;
+  HFLMP_HIP_CHECK(ctx, hipMemcpyAsync(sv->h_states, sv->d_states,
+                                      sizeof(SlotState) * sv->max_active,
+                                      hipMemcpyDeviceToHost, s));
+  *launched = true;
+  return HFLMP_OK;
+}
+
+// Admits first come, first served while a slot is free and the block fits.
+int admit_queued(hflmp_server *sv) {
+  const auto t0 = Clock::now();
+  while (!sv->queue.empty()) {
+    int slot = -1;
+    for (int k = 0; k < sv->max_active; ++k)
+      if (sv->slot_job[k] < 0) {
+        slot = k;
+        break;
+      }
+    if (slot < 0) break;
+    Job &jb = sv->jobs[sv->queue.front()];
+    if (!sv->alloc.alloc(jb.bytes, &jb.off)) break;
+    int rc = admit(sv, jb, slot);
+    if (!rc && jb.solo) rc = start_solo(sv, jb, slot);
+    if (rc) {
+      sv->alloc.release(jb.off, jb.bytes);
+      return rc;
+    }
+    // The CSR arrays at the block's end are read only by the admission
+    // kernels just queued (band check, SELL-64 fill); every later use of
+    // that range (another system's uploads and kernels) is queued on the
+    // same stream, after them.  (A solo system's start on the solo stream
+    // reads its layout, not these arrays.)
+    sv->alloc.release(jb.off + jb.o_rp, jb.bytes - jb.o_rp);
+    jb.kept = jb.o_rp;
+    sv->slot_job[slot] = jb.ticket;
+    sv->joining.push_back(slot);
+    sv->queue.pop_front();
+  }
+  sv->admit_seconds += std::chrono::duration<double>(Clock::now() - t0).count();
+  return HFLMP_OK;
+}
+
+// Waits for the chunk and retires the systems that stopped: x to the
+// caller's buffer, block back to the arena, slot freed.
+int retire_stopped(hflmp_server *sv, int *n_done) {
+  hflmp_ctx *ctx = sv->ctx;
+  const auto t_wait = Clock::now();
+  HFLMP_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  sv->wait_seconds += std::chrono::duration<double>(Clock::now() - t_wait).count();
+  for (std::vector<int> *list : {&sv->order, &sv->solo}) {
+    std::vector<int> keep;
+    for (int slot : *list) {
+      const SlotState &st = sv->h_states[slot];
+      Job &jb = sv->jobs[sv->slot_job[slot]];
+      if (slot_running(st)) {
+        keep.push_back(slot);
+        continue;
+      }
+      const bool refused = (st.mode == kModeMixed ? st.status : st.s64.status) == kRefused;
+      if (refused)
+        std::memset(jb.x_out, 0, 8 * (size_t)jb.nrow);
+      else
+        HFLMP_HIP_CHECK(ctx, hipMemcpy(jb.x_out, jb.d.x, 8 * (size_t)jb.nrow,
+                                       hipMemcpyDeviceToHost));
+      // zeroed again by the slot's next admission
+      unsigned int bad = 0;
+      HFLMP_HIP_CHECK(ctx,
+                      hipMemcpy(&bad, sv->d_bad + slot, sizeof(bad), hipMemcpyDeviceToHost));
+      if (st.mode == kModeMixed) {
+        jb.iters = st.inner_total;
+        jb.status = st.status;
+        jb.outer = st.outer;
+        jb.res_init = st.res0;
+        jb.res_final = st.res;
+      } else {
+        jb.iters = st.s64.iter;
+        jb.status = st.s64.status;
+        jb.outer = 0;
+        jb.res_init = st.s64.res0;
+        jb.res_final = st.s64.res;
+      }
+      jb.bad_diag = (int)bad;
+      jb.col_bytes = st.col16 ? 2 : 4;
+      jb.t_done = Clock::now();
+      jb.state = kDone;
+      sv->alloc.release(jb.off, jb.kept);
+      sv->slot_job[slot] = -1;
+      if (list == &sv->order) sv->set_changed = true;
+      ++*n_done;
+    }
+    list->swap(keep);
+  }
+  return HFLMP_OK;
+}
+
+int join_admitted(hflmp_server *sv) {
+  hflmp_ctx *ctx = sv->ctx;
+  if (sv->joining.empty()) return HFLMP_OK;
+  const auto t0 = Clock::now();
+  HFLMP_HIP_CHECK(ctx, hipStreamSynchronize(sv->admit_stream));
+  sv->stage_used = 0;
+  sv->join_seconds += std::chrono::duration<double>(Clock::now() - t0).count();
+  for (int slot : sv->joining) {
+    if (sv->jobs[sv->slot_job[slot]].solo) {
+      sv->solo.push_back(slot);
+    } else {
+      sv->order.push_back(slot);
+      sv->set_changed = true;
+    }
+  }
+  sv->joining.clear();
+  sv->peak_active = std::max(sv->peak_active, (int)(sv->order.size() + sv->solo.size()));
+  sv->peak_in_use = std::max(sv->peak_in_use, sv->budget - sv->alloc.total_free());
+  return HFLMP_OK;
+}
+
 
 }  // namespace
 
@@ -1155,26 +1380,32 @@ int hflmp_server_create(hflmp_ctx *ctx, const hflmp_server_config *cfg, hflmp_se
       hipMalloc(&sv->d_sys, sizeof(SysDesc) * sv->max_active) != hipSuccess ||
       hipMalloc(&sv->d_bad, sizeof(unsigned int) * sv->max_active) != hipSuccess ||
       hipHostMalloc(&sv->h_states, sizeof(SlotState) * sv->max_active) != hipSuccess ||
+      hipHostMalloc(&sv->h_init, sizeof(SlotState) * sv->max_active) != hipSuccess ||
       hipHostMalloc(&sv->h_admit, sizeof(SlotState)) != hipSuccess)
     return fail("table allocation failed");
-  // the solo stream keeps the default priority (as every stream here)
-  if (hipStreamCreateWithFlags(&sv->solo_stream, hipStreamNonBlocking) != hipSuccess ||
+  sv->stage_cap = std::min(std::max(sv->budget / 8, (size_t)16 << 20), (size_t)256 << 20);
+  if (hipHostMalloc(&sv->stage, sv->stage_cap) != hipSuccess)
+    return fail("staging allocation failed");
+  // the admission and solo streams keep the default priority (as every
+  // stream here)
+  if (hipStreamCreateWithFlags(&sv->admit_stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&sv->solo_stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&sv->ev_admitted, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&sv->ev_solo_done, hipEventDisableTiming) != hipSuccess)
     return fail("stream or event creation failed");
   std::memset(sv->h_states, 0, sizeof(SlotState) * sv->max_active);
   sv->alloc.reset(sv->budget);
   sv->slot_job.assign(sv->max_active, -1);
-  if (hflmp_fill_zero(ctx, sv->d_bad, sizeof(unsigned int) * sv->max_active))
-    return fail("fill");
   *out = sv;
   return HFLMP_OK;
 }
 
 void hflmp_server_destroy(hflmp_server *sv) {
   if (!sv) return;
+  if (sv->admit_stream) (void)hipStreamSynchronize(sv->admit_stream);
   if (sv->solo_stream) (void)hipStreamSynchronize(sv->solo_stream);
   if (sv->ctx) (void)hipStreamSynchronize(sv->ctx->stream);
+  if (sv->admit_stream) (void)hipStreamDestroy(sv->admit_stream);
   if (sv->solo_stream) (void)hipStreamDestroy(sv->solo_stream);
   if (sv->ev_admitted) (void)hipEventDestroy(sv->ev_admitted);
   if (sv->ev_solo_done) (void)hipEventDestroy(sv->ev_solo_done);
@@ -1186,6 +1417,8 @@ void hflmp_server_destroy(hflmp_server *sv) {
   (void)hipFree(sv->d_unit_sys);
   (void)hipFree(sv->d_bad);
   if (sv->h_states) (void)hipHostFree(sv->h_states);
+  if (sv->h_init) (void)hipHostFree(sv->h_init);
+  if (sv->stage) (void)hipHostFree(sv->stage);
   delete sv;
 }
 
@@ -1252,126 +1485,16 @@ int hflmp_server_submit(hflmp_server *sv, int nrow, int nnz, const int *row_ptr,
 
 int hflmp_server_step(hflmp_server *sv, int *n_done) {
   if (!sv) return HFLMP_EINVAL;
-  hflmp_ctx *ctx = sv->ctx;
-  hipStream_t s = ctx->stream;
   int done = 0;
-  const auto t_begin = Clock::now();
-  // admit first come, first served while a slot is free and the block fits
-  while (!sv->queue.empty()) {
-    int slot = -1;
-    for (int k = 0; k < sv->max_active; ++k)
-      if (sv->slot_job[k] < 0) {
-        slot = k;
-        break;
-      }
-    if (slot < 0) break;
-    Job &jb = sv->jobs[sv->queue.front()];
-    if (!sv->alloc.alloc(jb.bytes, &jb.off)) break;
-    int rc = admit(sv, jb, slot);
-    if (rc) {
-      sv->alloc.release(jb.off, jb.bytes);
-      return rc;
-    }
-    if (jb.solo) {
-      rc = start_solo(sv, jb, slot);
-      if (rc) {
-        sv->alloc.release(jb.off, jb.bytes);
-        return rc;
-      }
-    }
-    // The CSR arrays at the block's end are read only by the admission
-    // kernels just queued (band check, SELL-64 fill); every later use of
-    // that range (another system's uploads and kernels) is queued on the
-    // same stream, after them.  (A solo system's start on the solo stream
-    // reads its layout, not these arrays.)
-    sv->alloc.release(jb.off + jb.o_rp, jb.bytes - jb.o_rp);
-    jb.kept = jb.o_rp;
-    sv->slot_job[slot] = jb.ticket;
-    (jb.solo ? sv->solo : sv->order).push_back(slot);
-    sv->queue.pop_front();
-    if (!jb.solo) sv->set_changed = true;
-  }
-  sv->peak_active = std::max(sv->peak_active, (int)(sv->order.size() + sv->solo.size()));
-  sv->peak_in_use = std::max(sv->peak_in_use, sv->budget - sv->alloc.total_free());
-  const auto t_admitted = Clock::now();
-  sv->admit_seconds += std::chrono::duration<double>(t_admitted - t_begin).count();
-  if (sv->set_changed) {
-    int rc = rebuild_tables(sv);
+  // A step with nothing to run admits, and runs the admitted systems at once.
+  for (int pass = 0; pass < 2; ++pass) {
+    bool launched = false;
+    int rc = launch_chunk(sv, &launched);
+    if (!rc) rc = admit_queued(sv);
+    if (!rc && launched) rc = retire_stopped(sv, &done);
+    if (!rc) rc = join_admitted(sv);
     if (rc) return rc;
-  }
-  if (sv->S > 0 || !sv->solo.empty()) {
-    if (sv->S > 0) {
-      int rc = launch_group(sv);
-      if (rc) return rc;
-    }
-    if (!sv->solo.empty()) {
-      // the solo systems' chunks on their stream, beside the group's; the
-      // state copy waits for both
-      for (int slot : sv->solo) {
-        const Job &jb = sv->jobs[sv->slot_job[slot]];
-        if (jb.col16_64)
-          solo_chunk<short>(ctx, jb, sv->solo_stream, sv->chunk);
-        else
-          solo_chunk<int>(ctx, jb, sv->solo_stream, sv->chunk);
-      }
-      HFLMP_LAUNCH_CHECK(ctx);
-      HFLMP_HIP_CHECK(ctx, hipEventRecord(sv->ev_solo_done, sv->solo_stream));
-      HFLMP_HIP_CHECK(ctx, hipStreamWaitEvent(s, sv->ev_solo_done, 0));
-    }
-    ++sv->steps;
-    HFLMP_HIP_CHECK(ctx, hipMemcpyAsync(sv->h_states, sv->d_states,
-                                        sizeof(SlotState) * sv->max_active,
-                                        hipMemcpyDeviceToHost, s));
-    const auto t_wait = Clock::now();
-    HFLMP_HIP_CHECK(ctx, hipStreamSynchronize(s));
-    sv->wait_seconds += std::chrono::duration<double>(Clock::now() - t_wait).count();
-    // retire the systems that stopped
-    for (std::vector<int> *list : {&sv->order, &sv->solo}) {
-      std::vector<int> keep;
-      for (int slot : *list) {
-        const SlotState &st = sv->h_states[slot];
-        Job &jb = sv->jobs[sv->slot_job[slot]];
-        if (slot_running(st)) {
-          keep.push_back(slot);
-          continue;
-        }
-        const bool refused = (st.mode == kModeMixed ? st.status : st.s64.status) == kRefused;
-        if (refused)
-          std::memset(jb.x_out, 0, 8 * (size_t)jb.nrow);
-        else
-          HFLMP_HIP_CHECK(ctx, hipMemcpy(jb.x_out, jb.d.x, 8 * (size_t)jb.nrow,
-                                         hipMemcpyDeviceToHost));
-        unsigned int bad = 0;
-        HFLMP_HIP_CHECK(ctx,
-                        hipMemcpy(&bad, sv->d_bad + slot, sizeof(bad), hipMemcpyDeviceToHost));
-        if (bad) {
-          const int rc = hflmp_fill_zero(ctx, sv->d_bad + slot, sizeof(bad));
-          if (rc) return rc;
-        }
-        if (st.mode == kModeMixed) {
-          jb.iters = st.inner_total;
-          jb.status = st.status;
-          jb.outer = st.outer;
-          jb.res_init = st.res0;
-          jb.res_final = st.res;
-        } else {
-          jb.iters = st.s64.iter;
-          jb.status = st.s64.status;
-          jb.outer = 0;
-          jb.res_init = st.s64.res0;
-          jb.res_final = st.s64.res;
-        }
-        jb.bad_diag = (int)bad;
-        jb.col_bytes = st.col16 ? 2 : 4;
-        jb.t_done = Clock::now();
-        jb.state = kDone;
-        sv->alloc.release(jb.off, jb.kept);
-        sv->slot_job[slot] = -1;
-        if (list == &sv->order) sv->set_changed = true;
-        ++done;
-      }
-      list->swap(keep);
-    }
+    if (launched || (sv->order.empty() && sv->solo.empty())) break;
   }
   if (n_done) *n_done = done;
   return HFLMP_OK;
@@ -1425,6 +1548,7 @@ int hflmp_server_stats(const hflmp_server *sv, hflmp_server_stats_t *out) {
   out->admit_seconds = sv->admit_seconds;
   out->capture_seconds = sv->capture_seconds;
   out->wait_seconds = sv->wait_seconds;
+  out->join_seconds = sv->join_seconds;
   return HFLMP_OK;
 }
 

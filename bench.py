@@ -850,20 +850,27 @@ def run_server_stream(hflmp, ctx, feed_specs, arrivals, cfg, rtol, precision, ke
                                  "submit": round(t_submit, 3), "step": round(t_step, 3),
                                  "step_admit": round(stats["admit_seconds"], 3),
                                  "step_tables_and_graphs": round(stats["capture_seconds"], 3),
-                                 "step_wait_for_kernels": round(stats["wait_seconds"], 3)}
+                                 "step_wait_for_kernels": round(stats["wait_seconds"], 3),
+                                 "step_wait_for_admission": round(stats["join_seconds"], 3)}
     finally:
         sv.close()
         feed.close()
     return infos, xs, lat, makespan, stats
 
 
-def run_singles(hflmp, ctx, reqs, rtol):
+def run_singles(hflmp, ctx, specs, rtol):
     """One at a time, device-resident FP64 Jacobi-CG per request: upload,
-    SELL-64 build (int16 columns where they fit), Jacobi, solve, download."""
-    service, out, solve_s = [], [], 0.0
+    SELL-64 build (int16 columns where they fit), Jacobi, solve, download.
+    The requests come from the same generator as the server's (RequestFeed,
+    8 host threads working ahead of the solves), so both runs share the
+    host with the generation.  Returns the requests too (for parity)."""
+    service, out, reqs, solve_s = [], [], [], 0.0
+    feed = RequestFeed(hflmp, specs)
     t0 = time.perf_counter()
-    for rp, ci, v, b in reqs:
+    for k in range(len(specs)):
         ts = time.perf_counter()
+        rp, ci, v, b = feed.get(k)
+        reqs.append((rp, ci, v, b))
         A = hflmp.CsrMatrix(ctx, rp, ci, v)
         S = hflmp.SellMatrix(ctx, A, hflmp.SELL_COL16)
         invd = hflmp.invdiag(ctx, A)
@@ -874,7 +881,9 @@ def run_singles(hflmp, ctx, reqs, rtol):
         S.free(); A.free(); invd.free(); db.free(); dx.free()
         service.append(time.perf_counter() - ts)
         out.append((x, info))
-    return service, out, solve_s, time.perf_counter() - t0
+    t_all = time.perf_counter() - t0
+    feed.close()
+    return service, out, solve_s, t_all, reqs
 
 
 def bench_serve(hflmp, nreq, seed, rate, ref_style, graphs=True, ntune=32, sample_every=20,
@@ -919,9 +928,9 @@ def bench_serve(hflmp, nreq, seed, rate, ref_style, graphs=True, ntune=32, sampl
     arrivals = np.cumsum(gaps) / lam
     linfos, _, lat, mk2, _ = run_server_stream(hflmp, ctx, specs, arrivals, cfg, rtol, 0, set())
     progress("serving: latency run %.1f s" % mk2)
-    # one at a time on the sample (requests regenerated; identical by seed)
-    sreqs = [make_request(hflmp, specs[k]) for k in sample]
-    service, singles, solve_s, t_single = run_singles(hflmp, ctx, sreqs, rtol)
+    # one at a time on the sample (requests generated again, identical by seed)
+    service, singles, solve_s, t_single, sreqs = run_singles(
+        hflmp, ctx, [specs[k] for k in sample], rtol)
     single_rate = len(sample) / t_single
     progress("serving: one at a time on %d requests, %.1f solves/s" % (len(sample), single_rate))
     # FIFO on the sample's own arrivals, rescaled to the same offered rate
@@ -969,8 +978,9 @@ def bench_serve(hflmp, nreq, seed, rate, ref_style, graphs=True, ntune=32, sampl
                        "note": "server measured live over all %d requests; one at a time: "
                                "FIFO on its measured service times over the sampled "
                                "sub-stream at the same offered rate" % nreq},
-           "one_at_a_time_device": {"sample": "every %dth request of the stream (%d)"
-                                              % (sample_every, len(sample)),
+           "one_at_a_time_device": {"sample": "every %dth request of the stream (%d), "
+                                              "from the same 8-thread generator as the "
+                                              "server's" % (sample_every, len(sample)),
                                     "solves_per_s": round(single_rate, 2),
                                     "seconds": round(t_single, 3),
                                     "solve_loop_seconds": round(solve_s, 3),

@@ -774,11 +774,13 @@ int hflmp_mpcg_jacobi_sell(hflmp_ctx *ctx, const hflmp_sell *A64,
  * system waits in a first-come-first-served queue until a slot
  * is free and its footprint (hflmp_server_footprint: CSR, SELL-64 copy,
  * inverse diagonal(s), b, x and the Krylov vectors) fits.  hflmp_server_step
- * admits what fits, forms new systems' initial residuals, advances every
- * active system by chunk_iters iterations in shared launches (one launch
- * per kernel for all systems, captured as a hipGraph per active set when
- * use_graphs), and retires the systems that stopped: their x is copied to
- * the caller's buffer.  Each system keeps its own scalars and partial sums
+ * advances every active system by chunk_iters iterations in shared
+ * launches (one launch per kernel for all systems, captured as a hipGraph
+ * per active set when use_graphs); while they run it admits what fits
+ * (uploads and layout kernels on a second stream); then it retires the
+ * systems that stopped (their x is copied to the caller's buffer), and the
+ * admitted systems join, starting with the next step's chunk (at once when
+ * nothing else was active).  Each system keeps its own scalars and partial sums
  * in a fixed order, so its result is bit-identical alone and in any mix.
  * Solves start from x0 = 0.  FP64 solves stop as IterateControl::Check
  * (common/iterate_control.cc:80-107) on the recursive residual; mixed
@@ -791,7 +793,7 @@ int hflmp_mpcg_jacobi_sell(hflmp_ctx *ctx, const hflmp_sell *A64,
  * the footprint assumes int32), else int32 columns.
  * FP64 systems with at least solo_min_nnz nonzeros run alone instead: the
  * single-system iteration of hflmp_pcg_jacobi_sell_f64 (three launches per
- * iteration, its grids) on a second stream of the server, concurrently
+ * iteration, its grids) on a third stream of the server, concurrently
  * with the grouped launches, chunk_iters iterations per step; their
  * iterates are that solver's bit for bit (with dtol unbounded), and their
  * footprint holds a partial-sum area of max(2 x units, 4096) values.
@@ -837,10 +839,11 @@ typedef struct {
   int peak_active;          /* most systems active at once so far          */
   size_t peak_arena_in_use; /* most arena bytes held at once so far        */
   /* host seconds spent in step() so far: admitting (uploads, layout
-   * kernels queued), rebuilding tables and capturing graphs, and waiting
-   * for the steps' kernels */
+   * kernels queued; while the chunk runs), rebuilding tables, capturing
+   * graphs and launching, and waiting for the chunks' kernels */
   double admit_seconds, capture_seconds, wait_seconds;
   long long solo_runs; /* systems started alone so far (solo_min_nnz)     */
+  double join_seconds; /* waiting for admission work after the chunk     */
 } hflmp_server_stats_t;
 int hflmp_server_create(hflmp_ctx *ctx, const hflmp_server_config *cfg,
                         hflmp_server **out);
