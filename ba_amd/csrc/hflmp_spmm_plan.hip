@@ -1,15 +1,22 @@
-// Building the row-block layout of hflmp_spmm_plan.hpp on the device, and
+// Building the block layout of hflmp_spmm_plan.hpp on the device, and
 // Y = A X through it.
 //
-// Build: per block of R rows, a bitmap of the block's column window
-// [cmin, cmax] (at most kPlanBits columns) is set from its entries; the
-// set bits in order are the block's distinct columns, and an entry's
-// position in the list is the number of set bits below its column (a
-// prefix count over the bitmap words plus a popcount in its word).  A first
-// pass counts each block's distinct columns and entries; the host picks the
-// largest R of 64, 32, 16, 8, 4 whose largest block fits the walk's limits
-// and the LDS budget, forms the offsets, and a second pass writes the lists
-// and the entry positions.
+// Geometry: the line length L is read off a sample of rows in the middle
+// of the matrix (the positive column offsets form a band 1..w around the
+// diagonal and, on a structured mesh, the next band starts at L - w).
+// Candidates are runs of 4..64 consecutive rows (one line of L = nrow) and
+// a x nl patches (a = 2..16 rows of nl = 2..8 lines of L rows, a multiple
+// of 4 and at most 64 rows); the host keeps the one with the fewest staged
+// X rows in total that fits the walk's limits and the LDS budget (one
+// counting pass each).
+//
+// Build: per block, a bitmap of the block's column window [cmin, cmax]
+// (at most kPlanBits columns) is set from its entries; the set bits in
+// order are the block's distinct columns, and an entry's position in the
+// list is the number of set bits below its column (a prefix count over the
+// bitmap words plus a popcount in its word).  A first pass counts each
+// block's distinct columns and entries, the host forms the offsets of the
+// chosen geometry, and a second pass writes the lists and the positions.
 #include <algorithm>
 #include <climits>
 #include <vector>
@@ -55,39 +62,54 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int *lds, int *total)
   return before + incl - v;
 }
 
+// Calls f(j) for the CSR entries of block b (its runs in order), each
+// entry on one thread of the workgroup.
+template <typename F>
+__device__ __forceinline__ void for_block_entries(const PlanGeom &g, int b,
+                                                  const int *__restrict__ rp, F &&f) {
+  for (int t = 0; t < g.nl; ++t) {
+    const PlanRun run = plan_run(g, b, t);
+    const int j1 = rp[run.r0 + run.n];
+    for (int j = rp[run.r0] + threadIdx.x; j < j1; j += kBlock) f(j);
+  }
+}
+
 struct BlockSpan {
-  int j0, j1, cmin, cmax;
+  int ne, cmin, cmax;
 };
 
-// The block's entry range and column window; every thread gets the same.
-__device__ __forceinline__ BlockSpan block_span(int b, int nrow, int R, const int *__restrict__ rp,
+// The block's entry count and column window; every thread gets the same.
+__device__ __forceinline__ BlockSpan block_span(const PlanGeom &g, int b,
+                                                const int *__restrict__ rp,
                                                 const int *__restrict__ ci, int *lds) {
   BlockSpan s;
-  const int r0 = b * R;
-  const int r1 = r0 + R < nrow ? r0 + R : nrow;
-  s.j0 = rp[r0];
-  s.j1 = rp[r1];
+  s.ne = 0;
+  for (int t = 0; t < g.nl; ++t) {
+    const PlanRun run = plan_run(g, b, t);
+    s.ne += rp[run.r0 + run.n] - rp[run.r0];
+  }
   int mn = INT_MAX, mx = INT_MIN;
-  for (int j = s.j0 + threadIdx.x; j < s.j1; j += kBlock) {
+  for_block_entries(g, b, rp, [&](int j) {
     const int c = ci[j];
     mn = c < mn ? c : mn;
     mx = c > mx ? c : mx;
-  }
+  });
   s.cmin = block_min_max(mn, true, lds);
   s.cmax = block_min_max(mx, false, lds);
   return s;
 }
 
 // The block's column bitmap over [cmin, cmin + 32 nwords).
-__device__ __forceinline__ void block_bitmap(const BlockSpan &s, int nwords,
+__device__ __forceinline__ void block_bitmap(const PlanGeom &g, int b, const BlockSpan &s,
+                                             int nwords, const int *__restrict__ rp,
                                              const int *__restrict__ ci, unsigned *bits) {
   __syncthreads();
   for (int w = threadIdx.x; w < nwords; w += kBlock) bits[w] = 0u;
   __syncthreads();
-  for (int j = s.j0 + threadIdx.x; j < s.j1; j += kBlock) {
+  for_block_entries(g, b, rp, [&](int j) {
     const int d = ci[j] - s.cmin;
     atomicOr(&bits[d >> 5], 1u << (d & 31));
-  }
+  });
   __syncthreads();
 }
 
@@ -95,15 +117,15 @@ __device__ __forceinline__ void block_bitmap(const BlockSpan &s, int nwords,
 // bitmap; ecnt[b]: its entries; bad: entries whose column is outside
 // [0, ncol).
 __global__ void __launch_bounds__(kBlock)
-    plan_count_kernel(int nrow, int ncol, int R, int nblk, const int *__restrict__ rp,
+    plan_count_kernel(PlanGeom g, int ncol, int nblk, const int *__restrict__ rp,
                       const int *__restrict__ ci, int *__restrict__ ucnt,
                       int *__restrict__ ecnt, unsigned int *__restrict__ bad) {
   __shared__ unsigned bits[kPlanWords];
   __shared__ int lds[kBlock / kWave];
   for (int b = blockIdx.x; b < nblk; b += gridDim.x) {
-    const BlockSpan s = block_span(b, nrow, R, rp, ci, lds);
-    if (threadIdx.x == 0) ecnt[b] = s.j1 - s.j0;
-    if (s.j1 <= s.j0) {
+    const BlockSpan s = block_span(g, b, rp, ci, lds);
+    if (threadIdx.x == 0) ecnt[b] = s.ne;
+    if (s.ne <= 0) {
       if (threadIdx.x == 0) ucnt[b] = 0;
       continue;
     }
@@ -119,7 +141,7 @@ __global__ void __launch_bounds__(kBlock)
       continue;
     }
     const int nwords = (s.cmax - s.cmin) / 32 + 1;
-    block_bitmap(s, nwords, ci, bits);
+    block_bitmap(g, b, s, nwords, rp, ci, bits);
     int cnt = 0;
     for (int w = threadIdx.x; w < nwords; w += kBlock) cnt += __popc(bits[w]);
     int total;
@@ -130,7 +152,7 @@ __global__ void __launch_bounds__(kBlock)
 
 // The lists and the entry positions (blocks with a valid count only).
 __global__ void __launch_bounds__(kBlock)
-    plan_fill_kernel(int nrow, int R, int nblk, const int *__restrict__ rp,
+    plan_fill_kernel(PlanGeom g, int nblk, const int *__restrict__ rp,
                      const int *__restrict__ ci, const int *__restrict__ uoff,
                      int *__restrict__ ucol, int *__restrict__ lidx) {
   __shared__ unsigned bits[kPlanWords];
@@ -138,10 +160,10 @@ __global__ void __launch_bounds__(kBlock)
   __shared__ int lds[kBlock / kWave];
   constexpr int per = kPlanWords / kBlock;  // words per thread in the scan
   for (int b = blockIdx.x; b < nblk; b += gridDim.x) {
-    const BlockSpan s = block_span(b, nrow, R, rp, ci, lds);
-    if (s.j1 <= s.j0) continue;
+    const BlockSpan s = block_span(g, b, rp, ci, lds);
+    if (s.ne <= 0) continue;
     const int nwords = (s.cmax - s.cmin) / 32 + 1;
-    block_bitmap(s, nwords, ci, bits);
+    block_bitmap(g, b, s, nwords, rp, ci, bits);
     // prefix counts: thread t owns words [t per, t per + per)
     int c[per], sum = 0;
 #pragma unroll
@@ -169,11 +191,11 @@ __global__ void __launch_bounds__(kBlock)
         m &= m - 1u;
       }
     }
-    for (int j = s.j0 + threadIdx.x; j < s.j1; j += kBlock) {
+    for_block_entries(g, b, rp, [&](int j) {
       const int d = ci[j] - s.cmin;
       const int w = d >> 5;
       lidx[j] = pre[w] + __popc(bits[w] & ((1u << (d & 31)) - 1u));
-    }
+    });
   }
 }
 
@@ -182,9 +204,8 @@ __global__ void __launch_bounds__(kBlock)
     plan_spmm_kernel(PlanView pv, int k, const int *__restrict__ rp, const T *__restrict__ val,
                      const T *__restrict__ X, T *__restrict__ Y) {
   extern __shared__ __align__(16) unsigned char plan_lds[];
-  const int lane = threadIdx.x & (kWave - 1);
-  plan_walk<T, KC>(pv, k, rp, val, X, plan_lds, [&](int i, T s) {
-    if (lane < k) __builtin_nontemporal_store(s, Y + (size_t)i * k + lane);
+  plan_walk<T, KC>(pv, k, rp, val, X, plan_lds, [&](int i, int c, T s) {
+    if (c < k) __builtin_nontemporal_store(s, Y + (size_t)i * k + c);
   });
 }
 
@@ -200,6 +221,46 @@ void free_plan(hflmp_spmm_plan *p) {
   delete p;
 }
 
+// The line length of a structured numbering, read off up to 64 rows in the
+// middle of the matrix: the positive column offsets 1..w of the band around
+// the diagonal, then the first offset d1 > w + 1 of the next band, which
+// is centred on L = d1 + w.  0 when there is no second band.
+int detect_line(hflmp_ctx *ctx, int nrow, const int *rp, const int *ci, int *L) {
+  *L = 0;
+  const int r0 = nrow / 2, nr = std::min(64, nrow - r0);
+  if (nr <= 0) return HFLMP_OK;
+  std::vector<int> hrp(nr + 1);
+  HFLMP_HIP_CHECK(ctx, hipMemcpyAsync(hrp.data(), rp + r0, sizeof(int) * (nr + 1),
+                                      hipMemcpyDeviceToHost, ctx->stream));
+  HFLMP_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  const int ne = hrp[nr] - hrp[0];
+  if (ne <= 0) return HFLMP_OK;
+  std::vector<int> hci(ne);
+  HFLMP_HIP_CHECK(ctx, hipMemcpyAsync(hci.data(), ci + hrp[0], sizeof(int) * (size_t)ne,
+                                      hipMemcpyDeviceToHost, ctx->stream));
+  HFLMP_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  std::vector<long long> d;
+  for (int r = 0; r < nr; ++r)
+    for (int j = hrp[r] - hrp[0]; j < hrp[r + 1] - hrp[0]; ++j)
+      if (hci[j] > r0 + r) d.push_back((long long)hci[j] - (r0 + r));
+  std::sort(d.begin(), d.end());
+  d.erase(std::unique(d.begin(), d.end()), d.end());
+  long long w = 0;
+  size_t i = 0;
+  while (i < d.size() && d[i] == w + 1) w = d[i++];
+  if (i == d.size()) return HFLMP_OK;
+  const long long line = d[i] + w;
+  if (line >= 2 && line <= nrow / 2) *L = (int)line;
+  return HFLMP_OK;
+}
+
+struct PlanChoice {
+  PlanGeom g;
+  int nblk, umax, emax;
+  long long nuniq;
+  std::vector<int> cnt;
+};
+
 int create_plan(hflmp_ctx *ctx, int nrow, int ncol, int nnz, const int *rp, const int *ci,
                 int nrhs, int vb, int lds_budget, hflmp_spmm_plan **out) {
   if (!ctx) return HFLMP_EINVAL;
@@ -212,7 +273,19 @@ int create_plan(hflmp_ctx *ctx, int nrow, int ncol, int nnz, const int *rp, cons
   if (lds_budget <= 0) lds_budget = 64 * 1024;
   HFLMP_REQUIRE(ctx, lds_budget <= 160 * 1024, "spmm_plan: lds_budget above 160 KiB");
   hipStream_t s = ctx->stream;
-  const int max_nblk = (nrow + 3) / 4;
+  int L = 0;
+  int rc = detect_line(ctx, nrow, rp, ci, &L);
+  if (rc) return rc;
+  // candidates: runs of consecutive rows, then patches of lines of L rows
+  std::vector<PlanGeom> cand;
+  for (int r : {64, 32, 16, 8, 4}) cand.push_back(plan_geom(nrow, nrow, r, 1));
+  // (patches of a multiple of 4 rows: the four waves take equal shares)
+  if (L > 0)
+    for (int nl = 2; nl <= kPlanMaxRuns; ++nl)
+      for (int a = 2; a <= 16 && a <= L && a * nl <= 64; ++a)
+        if ((a * nl) % 4 == 0) cand.push_back(plan_geom(nrow, L, a, nl));
+  int max_nblk = 0;
+  for (const PlanGeom &g : cand) max_nblk = std::max(max_nblk, plan_nblk(g));
   int *d_cnt = nullptr;
   unsigned int *d_bad = nullptr;
   HFLMP_HIP_CHECK(ctx, hipMalloc(&d_cnt, sizeof(int) * 2 * ((size_t)max_nblk + 1)));
@@ -225,16 +298,17 @@ int create_plan(hflmp_ctx *ctx, int nrow, int ncol, int nnz, const int *rp, cons
     (void)hipFree(d_cnt);
     (void)hipFree(d_bad);
   };
+  PlanChoice best;
+  best.nblk = 0;
   std::vector<int> cnt, ecnt;
-  int R = 0, nblk = 0, umax = 0, emax = 0;
-  for (int r : {64, 32, 16, 8, 4}) {
-    const int nb = (nrow + r - 1) / r;
+  for (const PlanGeom &g : cand) {
+    const int nb = plan_nblk(g);
     if (hipMemsetAsync(d_bad, 0, sizeof(unsigned int), s) != hipSuccess) {
       release();
       return set_error(ctx, HFLMP_EHIP, "spmm_plan: memset failed");
     }
-    const int g = nb < 4096 ? nb : 4096;
-    plan_count_kernel<<<g, kBlock, 0, s>>>(nrow, ncol, r, nb, rp, ci, d_cnt, d_ecnt, d_bad);
+    const int grid = nb < 4096 ? nb : 4096;
+    plan_count_kernel<<<grid, kBlock, 0, s>>>(g, ncol, nb, rp, ci, d_cnt, d_ecnt, d_bad);
     cnt.assign(nb, 0);
     ecnt.assign(nb, 0);
     unsigned int bad = 0;
@@ -257,27 +331,36 @@ int create_plan(hflmp_ctx *ctx, int nrow, int ncol, int nnz, const int *rp, cons
     const int mn = *std::min_element(cnt.begin(), cnt.end());
     const int mx = *std::max_element(cnt.begin(), cnt.end());
     const int ex = *std::max_element(ecnt.begin(), ecnt.end());
+    const int R = g.a * g.nl;
     if (mn < 0 || mx > plan_max_cols(vb) || ex > kPlanMaxEntries ||
-        plan_lds_bytes(r, mx, ex, nrhs, vb) > (size_t)lds_budget)
+        plan_lds_bytes(R, mx, ex, nrhs, vb) > (size_t)lds_budget)
       continue;
-    R = r;
-    nblk = nb;
-    umax = mx;
-    emax = ex;
-    break;
+    long long tot = 0;
+    for (int c : cnt) tot += c;
+    // fewest staged X rows; on a tie, more rows per block
+    if (best.nblk == 0 || tot < best.nuniq ||
+        (tot == best.nuniq && R > best.g.a * best.g.nl)) {
+      best.g = g;
+      best.nblk = nb;
+      best.umax = mx;
+      best.emax = ex;
+      best.nuniq = tot;
+      best.cnt = cnt;
+    }
   }
-  if (!R) {
+  if (best.nblk == 0) {
     release();
     return set_error(ctx, HFLMP_EFORMAT,
                      "spmm_plan: no block of 4..64 rows keeps to %d distinct columns within a "
                      "%d-column window, %d entries and %d bytes of LDS for %d x %d-byte rows",
                      plan_max_cols(vb), kPlanBits, kPlanMaxEntries, lds_budget, nrhs, vb);
   }
+  const int nblk = best.nblk;
   std::vector<int> off(nblk + 1);
   long long acc = 0;
   for (int b = 0; b < nblk; ++b) {
     off[b] = (int)acc;
-    acc += cnt[b];
+    acc += best.cnt[b];
   }
   off[nblk] = (int)acc;
   auto *p = new hflmp_spmm_plan();
@@ -285,10 +368,13 @@ int create_plan(hflmp_ctx *ctx, int nrow, int ncol, int nnz, const int *rp, cons
   p->nrow = nrow;
   p->ncol = ncol;
   p->nnz = nnz;
-  p->R = R;
+  p->L = best.g.L;
+  p->a = best.g.a;
+  p->nl = best.g.nl;
+  p->R = best.g.a * best.g.nl;
   p->nblk = nblk;
-  p->umax = umax;
-  p->emax = emax;
+  p->umax = best.umax;
+  p->emax = best.emax;
   p->nrhs = nrhs;
   p->value_bytes = vb;
   p->nuniq = acc;
@@ -300,14 +386,14 @@ int create_plan(hflmp_ctx *ctx, int nrow, int ncol, int nnz, const int *rp, cons
     release();
     return set_error(ctx, HFLMP_ENOMEM, "spmm_plan: allocation failed");
   }
-  const int g = nblk < 4096 ? nblk : 4096;
+  const int grid = nblk < 4096 ? nblk : 4096;
   if (hipMemcpyAsync(p->uoff, off.data(), sizeof(int) * ((size_t)nblk + 1),
                      hipMemcpyHostToDevice, s) != hipSuccess) {
     free_plan(p);
     release();
     return set_error(ctx, HFLMP_EHIP, "spmm_plan: upload failed");
   }
-  plan_fill_kernel<<<g, kBlock, 0, s>>>(nrow, R, nblk, rp, ci, p->uoff, p->ucol, p->lidx);
+  plan_fill_kernel<<<grid, kBlock, 0, s>>>(best.g, nblk, rp, ci, p->uoff, p->ucol, p->lidx);
   if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
     free_plan(p);
     release();
@@ -365,6 +451,15 @@ int hflmp_spmm_plan_info(const hflmp_spmm_plan *plan, int *rows_per_block,
   if (list_entries) *list_entries = plan->nuniq;
   if (device_bytes)
     *device_bytes = 4LL * (plan->nblk + 1) + 4LL * plan->nuniq + 4LL * plan->nnz;
+  return HFLMP_OK;
+}
+
+int hflmp_spmm_plan_geometry(const hflmp_spmm_plan *plan, int *line_rows, int *run_rows,
+                             int *runs) {
+  if (!plan) return HFLMP_EINVAL;
+  if (line_rows) *line_rows = plan->L;
+  if (run_rows) *run_rows = plan->a;
+  if (runs) *runs = plan->nl;
   return HFLMP_OK;
 }
 

@@ -1,17 +1,21 @@
 // Row-block layout for Y = A X over a row-major block of k <= 64 vectors,
 // and the device walk that applies it.
 //
-// Rows are cut into blocks of R consecutive rows.  For each block the
-// distinct columns of its entries are listed once, in ascending order
-// (ucol[uoff[b] .. uoff[b+1])), and every entry j of the CSR arrays gets the
-// position of its column in that list (lidx[j]).  A workgroup loads a
-// block's X rows into LDS once -- each a contiguous run of k values -- and
-// every entry of the block then reads its X row from LDS.  For an FE
-// stiffness matrix a block of 8 rows of the 2D Q2 pattern has ~25 entries
-// per row over ~55 distinct columns, so each X row crosses L2 -> CU about
-// 3 times less often than with one gather per entry, and with the blocks of
-// one XCD walked in order the X rows shared by neighbouring blocks stay in
-// that XCD's L2 between their loads.
+// Rows are grouped into blocks, each the rows of up to 8 runs of
+// consecutive rows taken from consecutive "lines" of L rows: block b of
+// line group g (lines g nl .. g nl + nl - 1) and segment c holds, from each
+// of its nl lines, the rows at positions c a .. c a + a - 1.  With L the
+// line length of a structured mesh's numbering this is an a x nl patch of
+// the mesh; with nl = 1 and L = nrow it is a run of a consecutive rows.
+// For each block the distinct columns of its entries are listed once, in
+// ascending order (ucol[uoff[b] .. uoff[b+1])), and every entry j of the
+// CSR arrays gets the position of its column in that list (lidx[j]).  A
+// workgroup loads a block's X rows into LDS once -- each a contiguous run
+// of k values -- and every entry of the block then reads its X row from
+// LDS.  A 4 x 4 patch of the 2D Q2 pattern has 16 rows over at most 64
+// distinct columns (4 staged X rows per row), where a run of 8 consecutive
+// rows has about 55 (7 per row); neighbouring blocks are walked in order
+// by one XCD, so the X rows they share stay in its L2.
 //
 // Each column c of a row is summed in ascending storage order from zero
 // (s += a_ij * x_j, no contraction), the order of the single-vector CSR
@@ -23,16 +27,46 @@
 #include "hflmp_internal.hpp"
 
 // Limits of the pipelined walk: distinct columns and entries of one block.
-// A wave holds the next block's X rows in registers: 16 of them in FP64,
+// A wave holds the next block's X rows in registers: 20 of them in FP64,
 // 32 in FP32.
-__host__ __device__ constexpr int plan_max_cols(int vb) { return vb == 8 ? 64 : 128; }
+__host__ __device__ constexpr int plan_max_cols(int vb) { return vb == 8 ? 80 : 128; }
 constexpr int kPlanMaxEntries = 1024;  // 4 per thread
+
+// Block geometry: lines of L rows, blocks of nl runs (one per line) of at
+// most a rows; nseg blocks per line group.
+struct PlanGeom {
+  int nrow, L, a, nl, nseg;
+};
+
+constexpr int kPlanMaxRuns = 8;
+
+inline PlanGeom plan_geom(int nrow, int L, int a, int nl) {
+  return PlanGeom{nrow, L, a, nl, (L + a - 1) / a};
+}
+inline int plan_nblk(const PlanGeom &g) {
+  const long long lines = ((long long)g.nrow + g.L - 1) / g.L;
+  return (int)((lines + g.nl - 1) / g.nl * g.nseg);
+}
+
+// Run t (< nl) of block b: rows [r0, r0 + n), clipped to the matrix.
+struct PlanRun {
+  int r0, n;
+};
+__host__ __device__ __forceinline__ PlanRun plan_run(const PlanGeom &g, int b, int t) {
+  const int grp = b / g.nseg, c = b - grp * g.nseg;
+  const long long s = ((long long)grp * g.nl + t) * g.L + (long long)c * g.a;
+  const int w = g.a < g.L - c * g.a ? g.a : g.L - c * g.a;
+  const long long lo = s < g.nrow ? s : g.nrow;
+  const long long hi = s + w < g.nrow ? s + w : g.nrow;
+  return PlanRun{(int)lo, (int)(hi - lo)};
+}
 
 struct hflmp_spmm_plan {
   int device;
   int nrow, ncol, nnz;
-  int R;            // rows per block
-  int nblk;         // ceil(nrow / R)
+  int L, a, nl;     // block geometry (PlanGeom)
+  int R;            // rows per block, at most (a nl)
+  int nblk;
   int umax;         // most distinct columns of one block
   int emax;         // most entries of one block
   int nrhs, value_bytes;  // the products the block size was chosen for
@@ -43,20 +77,24 @@ struct hflmp_spmm_plan {
 };
 
 struct PlanView {
-  int nrow, nnz, R, nblk, umax, emax;
+  PlanGeom g;
+  int nnz, R, nblk, umax, emax;
   const int *__restrict__ uoff;
   const int *__restrict__ ucol;
   const int *__restrict__ lidx;
 };
 
 inline PlanView plan_view(const hflmp_spmm_plan *p) {
-  return PlanView{p->nrow, p->nnz, p->R, p->nblk, p->umax, p->emax, p->uoff, p->ucol, p->lidx};
+  return PlanView{plan_geom(p->nrow, p->L, p->a, p->nl), p->nnz, p->R, p->nblk, p->umax,
+                  p->emax, p->uoff, p->ucol, p->lidx};
 }
 
 // LDS of the walk: the X rows of one block, then two buffers of the
-// block's entries (column positions, values), column list and row pointers.
+// block's description: entries (values, column positions), column list,
+// local row pointers and global row numbers.
 __host__ __device__ inline size_t plan_meta_bytes(int R, int umax, int emax, int vb) {
-  const size_t b = (size_t)emax * (4 + vb) + (size_t)umax * 4 + (size_t)(R + 1) * 4;
+  const size_t b =
+      (size_t)emax * (4 + vb) + (size_t)umax * 4 + (size_t)(R + 1) * 4 + (size_t)R * 4;
   return (b + 15) & ~size_t(15);
 }
 __host__ __device__ inline size_t plan_lds_bytes(int R, int umax, int emax, int k, int vb) {
@@ -85,7 +123,8 @@ struct PlanMeta {
   int *li;  // entries' column positions
   T *v;     // entries' values
   int *uc;  // distinct columns
-  int *rp;  // row pointers of the block's rows (R + 1)
+  int *rp;  // local row pointers of the block's rows (R + 1), from 0
+  int *gr;  // global row of each of the block's rows (R)
 };
 
 template <typename T>
@@ -95,47 +134,59 @@ __device__ __forceinline__ PlanMeta<T> plan_meta_at(unsigned char *base, const P
   m.li = reinterpret_cast<int *>(base + (size_t)pv.emax * sizeof(T));
   m.uc = m.li + pv.emax;
   m.rp = m.uc + pv.umax;
+  m.gr = m.rp + pv.R + 1;
   return m;
 }
 
-// A block's header: lanes 0..3 of one register hold rp[r0], rp[r0 + nr],
-// uoff[b], uoff[b + 1].  Loaded a stage before the block's description,
+// A block's header, one register: lane t < 8 holds rp at the start of run
+// t, lane 8 + t rp at its end (runs t >= nl: rp[0] twice), lanes 16 and 17
+// uoff[b] and uoff[b + 1].  Loaded a stage before the block's description,
 // as a vector load, so that nothing waits on it in the stage that issues it.
 __device__ __forceinline__ int plan_hdr_load(int b, const PlanView &pv,
                                              const int *__restrict__ rp) {
   const int lane = threadIdx.x & (kWave - 1);
-  const int r0 = b * pv.R;
-  const int r1 = r0 + pv.R < pv.nrow ? r0 + pv.R : pv.nrow;
-  const int *p = lane == 1 ? rp + r1 : lane == 2 ? pv.uoff + b : lane == 3 ? pv.uoff + b + 1
-                                                                          : rp + r0;
+  const int t = lane & 7;
+  const int *p = rp;
+  if (lane < 16 && t < pv.g.nl) {
+    const PlanRun run = plan_run(pv.g, b, t);
+    p = rp + run.r0 + (lane < 8 ? 0 : run.n);
+  } else if (lane == 16 || lane == 17) {
+    p = pv.uoff + b + (lane - 16);
+  }
   return *p;
 }
 
-// The uniform values of a block, from its header register.
+// The uniform values of a block: its number, entries, column list and rows.
 struct PlanHdr {
-  int r0, nr, j0, ne, u0, U;
+  int b, hreg, ne, u0, U, nr;
 };
 
 __device__ __forceinline__ PlanHdr plan_hdr(int hreg, int b, const PlanView &pv) {
   PlanHdr h;
-  h.r0 = b * pv.R;
-  h.nr = h.r0 + pv.R < pv.nrow ? pv.R : pv.nrow - h.r0;
-  h.j0 = __builtin_amdgcn_readlane(hreg, 0);
-  h.ne = __builtin_amdgcn_readlane(hreg, 1) - h.j0;
-  h.u0 = __builtin_amdgcn_readlane(hreg, 2);
-  h.U = __builtin_amdgcn_readlane(hreg, 3) - h.u0;
+  h.b = b;
+  h.hreg = hreg;
+  int ne = 0, nr = 0;
+  for (int t = 0; t < pv.g.nl; ++t) {
+    ne += __builtin_amdgcn_readlane(hreg, 8 + t) - __builtin_amdgcn_readlane(hreg, t);
+    nr += plan_run(pv.g, b, t).n;
+  }
+  h.ne = ne;
+  h.nr = nr;
+  h.u0 = __builtin_amdgcn_readlane(hreg, 16);
+  h.U = __builtin_amdgcn_readlane(hreg, 17) - h.u0;
   return h;
 }
 
 // A block's description on its way from memory to LDS: thread t holds
-// entries t, t+256, ... (at most 4), column t of the list and row pointer
-// t.  Every load is issued (addresses past the block's end are clamped
-// into it), so the number of loads in flight does not depend on the block.
+// entries t, t+256, ... (at most 4) of the block's runs laid end to end,
+// column t of the list, and the local row pointer and global row of row t.
+// Every load is issued (addresses past the block's end are clamped into
+// it), so the number of loads in flight does not depend on the block.
 template <typename T>
 struct PlanMetaRegs {
   int li[4];
   T v[4];
-  int uc, rp;
+  int uc, rp, gr;
 };
 
 template <typename T>
@@ -143,13 +194,39 @@ __device__ __forceinline__ void plan_meta_load(PlanMetaRegs<T> &m, const PlanHdr
                                                const PlanView &pv, const int *__restrict__ rp,
                                                const T *__restrict__ val) {
   const int t = threadIdx.x;
-  m.rp = rp[h.r0 + (t < h.nr ? t : h.nr)];
+  // Local entry e lies in the last run that starts at or before it, and
+  // has CSR index e + base for that run's base; local row t likewise has
+  // global row t + rbase and its first entry at local index rp[row] + ebase.
+  // (Runs are in row order, and an empty run starts where the next one does.)
+  int base[4] = {0, 0, 0, 0};
+  int rbase = 0, ebase = 0;
+  int eacc = 0, racc = 0;
+  for (int r = 0; r < pv.g.nl; ++r) {
+    const int j = __builtin_amdgcn_readlane(h.hreg, r);
+    const int c = __builtin_amdgcn_readlane(h.hreg, 8 + r) - j;
+    const PlanRun run = plan_run(pv.g, h.b, r);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (t + q * kBlock >= eacc) base[q] = j - eacc;
+    if (t >= racc) {
+      rbase = run.r0 - racc;
+      ebase = eacc - j;
+    }
+    eacc += c;
+    racc += run.n;
+  }
+  // rows past the block's last take its last row (loaded, never stored)
+  const int tr = t < h.nr ? t : (h.nr > 0 ? h.nr - 1 : 0);
+  m.gr = h.nr > 0 ? tr + rbase : 0;
+  m.rp = h.nr > 0 ? rp[m.gr] + ebase : 0;
   m.uc = pv.ucol[h.u0 + (t < h.U ? t : (h.U > 0 ? h.U - 1 : 0))];
-  const int last = h.ne > 0 ? h.j0 + h.ne - 1 : (h.j0 < pv.nnz ? h.j0 : pv.nnz - 1);
+  // entries past the block's last read its first run's first entry
+  const int j0 = __builtin_amdgcn_readlane(h.hreg, 0);
+  const int jpad = j0 < pv.nnz ? j0 : pv.nnz - 1;
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    const int e = h.j0 + t + q * kBlock;
-    const int j = e < last ? e : last;
+    const int e = t + q * kBlock;
+    const int j = e < h.ne ? e + base[q] : jpad;
     m.li[q] = __builtin_nontemporal_load(pv.lidx + j);
     m.v[q] = __builtin_nontemporal_load(val + j);
   }
@@ -159,7 +236,11 @@ template <typename T>
 __device__ __forceinline__ void plan_meta_store(const PlanMetaRegs<T> &m, const PlanHdr &h,
                                                 const PlanMeta<T> &d) {
   const int t = threadIdx.x;
-  if (t <= h.nr) d.rp[t] = m.rp;
+  if (t < h.nr) {
+    d.rp[t] = m.rp;
+    d.gr[t] = m.gr;
+  }
+  if (t == h.nr) d.rp[t] = h.ne;
   if (t < h.U) d.uc[t] = m.uc;
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
@@ -181,8 +262,8 @@ __device__ __forceinline__ void plan_meta_store(const PlanMetaRegs<T> &m, const 
 // issued a whole step before.  Per row, the entries' positions and values
 // are read from LDS (the same address across the wave) and their X values
 // from the staged rows, eight in flight, added in storage order.
-// epi(i, s) gets row i's sum for column `lane` (lanes >= k hold a copy of
-// column k - 1 and must not store).
+// epi(i, c, s) gets row i's sum s for column c = lane (c >= k: a copy of
+// column k - 1, not to be stored).
 // KC > 0: k is KC, known at compile time (the X-row offsets in LDS are
 // then shifts); KC = 0: k from the argument.
 template <typename T, int KC, typename Epi>
@@ -260,7 +341,7 @@ __device__ __forceinline__ void plan_walk(const PlanView &pv, int k_arg,
     // block m from LDS
     const PlanMeta<T> md = meta(buf);
     for (int r = wid; r < H0.nr; r += nw) {
-      const int a = md.rp[r] - H0.j0, e = md.rp[r + 1] - H0.j0;
+      const int a = md.rp[r], e = md.rp[r + 1];
       T s = 0;
       int j = a;
       for (; j + 8 <= e; j += 8) {
@@ -277,7 +358,7 @@ __device__ __forceinline__ void plan_walk(const PlanView &pv, int k_arg,
         for (int q = 0; q < 8; ++q) s += vv[q] * xv[q];
       }
       for (; j < e; ++j) s += md.v[j] * xs[md.li[j] * k + lk];
-      epi(H0.r0 + r, s);
+      epi(md.gr[r], lane, s);
     }
     __syncthreads();
     H0 = H1;

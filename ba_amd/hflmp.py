@@ -109,6 +109,7 @@ def hip_lib():
         L.hflmp_spmm_plan_destroy.restype = None
         L.hflmp_spmm_plan_info.argtypes = [vp, C.POINTER(i), C.POINTER(i),
                                            C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
+        L.hflmp_spmm_plan_geometry.argtypes = [vp, C.POINTER(i), C.POINTER(i), C.POINTER(i)]
         for s in ("f64", "f32"):
             getattr(L, "hflmp_plan_spmm_" + s).argtypes = [vp, vp, vp, vp, vp, vp, i]
         for s in ("f64", "f32"):
@@ -525,8 +526,10 @@ def spmm(ctx, A, X, Y, nrhs):
 
 class SpmmPlan:
     """Row-block layout of a device CSR matrix for Y = A X (hflmp_spmm_plan_*):
-    each block's distinct columns listed once, its X rows staged in LDS.
-    Blocks are sized for products of nrhs vectors of value_bytes."""
+    each block (a run of consecutive rows, or a patch of runs on consecutive
+    lines of the numbering) has its distinct columns listed once and its X
+    rows staged in LDS.  Blocks are sized for products of nrhs vectors of
+    value_bytes."""
 
     def __init__(self, ctx, A, nrhs, value_bytes=None, lds_budget=0):
         self.ctx = ctx
@@ -543,8 +546,13 @@ class SpmmPlan:
                                              C.byref(b))
         if rc:
             raise HflmpError("hflmp_spmm_plan_info failed (%d)" % rc)
+        L, a, nl = C.c_int(), C.c_int(), C.c_int()
+        rc = self.ctx.L.hflmp_spmm_plan_geometry(self.h, C.byref(L), C.byref(a), C.byref(nl))
+        if rc:
+            raise HflmpError("hflmp_spmm_plan_geometry failed (%d)" % rc)
         return {"rows_per_block": r.value, "max_cols_per_block": u.value,
-                "list_entries": n.value, "device_bytes": b.value}
+                "list_entries": n.value, "device_bytes": b.value,
+                "line_rows": L.value, "run_rows": a.value, "runs": nl.value}
 
     def spmm(self, A, X, Y, nrhs):
         """Y = A X through the plan (A: the matrix it was built from)."""

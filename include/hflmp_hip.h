@@ -636,19 +636,23 @@ int hflmp_csr_spmm_f32(hflmp_ctx *ctx, int nrow, int ncol, int nnz, int nrhs,
                        const int *row_ptr, const int *col, const float *val,
                        const float *X, float *Y);
 /* Row-block layout for the same product (no reference counterpart): rows
- * in blocks of R consecutive rows, each block's distinct columns listed
- * once in ascending order and every entry's column given as its position
- * in that list.  A workgroup loads a block's X rows into LDS once and all
- * the block's entries read them there, the next block's rows loading while
- * one is computed.  R is the largest of 64, 32, 16, 8, 4 whose blocks keep
- * to 64 (FP64) or 128 (FP32) distinct columns within a window of 65,536
- * columns, 1024 entries,
- * and lds_budget bytes of LDS (0 = 64 KiB, at most 160 KiB) for products
- * of nrhs vectors of value_bytes (4 or 8); HFLMP_EFORMAT if none does,
- * HFLMP_EINVAL for a column outside [0, ncol).  The plan keeps no reference
- * to row_ptr/col; the products take the matrix's row_ptr and values, which
- * must be those the plan was built from, and any nrhs <= 64 whose LDS need
- * stays within 160 KiB.  Products are bit-identical to hflmp_csr_spmm_*. */
+ * in blocks, each block's distinct columns listed once in ascending order
+ * and every entry's column given as its position in that list.  A
+ * workgroup loads a block's X rows into LDS once and all the block's
+ * entries read them there, the next block's rows loading while one is
+ * computed.  A block is a run of a consecutive rows (a = 64..4), or an
+ * a x nl patch: the rows at positions c a .. c a + a - 1 of nl consecutive
+ * lines of L rows, where L is read off the pattern (the offset of the
+ * band next to the diagonal band, as a structured mesh's numbering gives).
+ * Of the geometries whose blocks keep to 80 (FP64) or 128 (FP32) distinct
+ * columns within a window of 65,536 columns, 1024 entries, and lds_budget
+ * bytes of LDS (0 = 64 KiB, at most 160 KiB) for products of nrhs vectors
+ * of value_bytes (4 or 8), the plan takes the one with the fewest staged X
+ * rows; HFLMP_EFORMAT if none fits, HFLMP_EINVAL for a column outside
+ * [0, ncol).  The plan keeps no reference to row_ptr/col; the products
+ * take the matrix's row_ptr and values, which must be those the plan was
+ * built from, and any nrhs <= 64 whose LDS need stays within 160 KiB.
+ * Products are bit-identical to hflmp_csr_spmm_*. */
 typedef struct hflmp_spmm_plan hflmp_spmm_plan;
 int hflmp_spmm_plan_create(hflmp_ctx *ctx, int nrow, int ncol, int nnz,
                            const int *row_ptr, const int *col, int nrhs,
@@ -658,6 +662,10 @@ void hflmp_spmm_plan_destroy(hflmp_spmm_plan *plan);
 int hflmp_spmm_plan_info(const hflmp_spmm_plan *plan, int *rows_per_block,
                          int *max_cols_per_block, long long *list_entries,
                          long long *device_bytes);
+/* the chosen geometry: line length L (nrow for runs), rows per run a, runs
+ * (lines) per block nl */
+int hflmp_spmm_plan_geometry(const hflmp_spmm_plan *plan, int *line_rows,
+                             int *run_rows, int *runs);
 int hflmp_plan_spmm_f64(hflmp_ctx *ctx, const hflmp_spmm_plan *plan,
                         const int *row_ptr, const double *val,
                         const double *X, double *Y, int nrhs);

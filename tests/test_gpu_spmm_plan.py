@@ -46,13 +46,48 @@ def _check(ctx, rp, ci, v, ncol, k, dtype, budget=0):
 def test_plan_fe_q2_columns_bit_exact(ctx, k, dtype):
     rp, ci, v = hflmp.fe_system(2, 2, 40, hflmp.fe_kappa(2, 40, 1.0, 3))
     info = _check(ctx, rp, ci, v, len(rp) - 1, k, dtype)
-    assert info["rows_per_block"] in (4, 8, 16, 32, 64)
+    assert 4 <= info["rows_per_block"] <= 64
+    assert info["rows_per_block"] == info["run_rows"] * info["runs"]
     assert info["max_cols_per_block"] <= 128
 
 
-def test_plan_fe_q1_3d(ctx):
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_plan_fe_q1_3d(ctx, dtype):
     rp, ci, v = hflmp.fe_system(1, 3, 20)
-    _check(ctx, rp, ci, v, len(rp) - 1, 64, np.float64)
+    info = _check(ctx, rp, ci, v, len(rp) - 1, 64, dtype)
+    if dtype == np.float32:
+        # up to 128 columns per block: patches of a plane of the numbering,
+        # whose lines of 21 nodes are found
+        assert info["line_rows"] == 21 and info["runs"] > 1
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_plan_q2_patches(ctx, dtype):
+    # 2D Q2 on 40^2 cells: lines of 81 nodes (neither 81 nor the 81 lines
+    # divide into whole patches), so edge patches are clipped at the line
+    # end and the last line group is partial
+    rp, ci, v = hflmp.fe_system(2, 2, 40, hflmp.fe_kappa(2, 40, 1.0, 5))
+    info = _check(ctx, rp, ci, v, len(rp) - 1, 64, dtype)
+    assert info["line_rows"] == 81 and info["runs"] > 1
+    assert info["rows_per_block"] == info["run_rows"] * info["runs"]
+    # fewer staged X rows than runs of consecutive rows would need
+    assert info["list_entries"] < 4 * (len(rp) - 1)
+
+
+def _shifted(rp, ci, v, m):
+    """m rows holding only their diagonal, then the matrix (indices + m):
+    the lines of the numbering no longer start at a multiple of L."""
+    rp2 = np.concatenate([np.arange(m, dtype=np.int32), rp + m]).astype(np.int32)
+    ci2 = np.concatenate([np.arange(m, dtype=np.int32), ci + m]).astype(np.int32)
+    v2 = np.concatenate([np.full(m, 2.0), v])
+    return rp2, ci2, v2
+
+
+@pytest.mark.parametrize("m", [3, 40])
+def test_plan_patches_shifted_lines(ctx, m):
+    rp, ci, v = _shifted(*hflmp.fe_system(2, 2, 24), m)
+    info = _check(ctx, rp, ci, v, len(rp) - 1, 48, np.float64)
+    assert info["line_rows"] == 49
 
 
 @pytest.mark.parametrize("budget", [8192, 16384, 160 * 1024])
