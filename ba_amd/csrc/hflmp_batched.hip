@@ -18,6 +18,7 @@
 #include <cmath>
 
 #include "hflmp_solver_common.hpp"
+#include "hflmp_spmm_plan.hpp"
 
 namespace {
 
@@ -374,6 +375,29 @@ __global__ void __launch_bounds__(kBlock)
   block_col_partials<T, 1>(acc, k, part);
 }
 
+// The same through a block layout (hflmp_spmm_plan.hpp): AP = A P with
+// per-column partials of p.Ap, one set per workgroup of the walk.
+template <typename T, int KC>
+__global__ void __launch_bounds__(kBlock)
+    bcg_plan_spmm_kernel(PlanView pv, int k, const int *__restrict__ rp,
+                         const T *__restrict__ val, const T *__restrict__ P,
+                         T *__restrict__ AP, const BState<T> *st, T *__restrict__ part) {
+  extern __shared__ __align__(16) unsigned char plan_lds[];
+  if (!st->any_iterate) return;
+  T acc[1] = {T(0)};
+  const int lane = threadIdx.x & (kWave - 1);
+  const int lk = lane < k ? lane : k - 1;
+  plan_walk<T, KC>(
+      pv, k, rp, val, P, plan_lds, [&](int i) { return P[(size_t)i * k + lk]; },
+      [&](int i, int c, T s, T p) {
+        if (c < k) {
+          __builtin_nontemporal_store(s, AP + (size_t)i * k + c);
+          acc[0] += s * p;
+        }
+      });
+  block_col_partials<T, 1>(acc, k, part);
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
     bcg_alpha_final(BState<T> *st, const T *__restrict__ part, int count, int k) {
@@ -498,11 +522,14 @@ inline BLayout b_layout(hflmp_ctx *ctx, int nrow, int k, int vb) {
   return L;
 }
 
+// plan: the products of the iteration go through this block layout of the
+// matrix (null: the CSR row walk).
 template <typename T>
 int batched_pcg(hflmp_ctx *ctx, int nrow, int nnz, int k, const int *rp, const int *ci,
                 const T *val, const T *invd, const T *B, T *X, int maxit, double atol,
                 double rtol, double dtol, void *work, size_t work_bytes, int *iters,
-                int *status, double *res_init, double *res_final, double *seconds) {
+                int *status, double *res_init, double *res_final, double *seconds,
+                const hflmp_spmm_plan *plan = nullptr) {
   if (!ctx) return HFLMP_EINVAL;
   HFLMP_REQUIRE(ctx, nrow > 0 && nnz > 0 && k >= 1 && k <= kMaxRhs,
                 "batched_pcg: need nrow, nnz > 0 and 1 <= nrhs <= 64");
@@ -521,14 +548,40 @@ int batched_pcg(hflmp_ctx *ctx, int nrow, int nnz, int k, const int *rp, const i
   T *AP = reinterpret_cast<T *>(w + L.vec + 3 * L.vstride);
   hipStream_t s = ctx->stream;
   const int g = L.grid;
+  // the block layout's walk: its grid (at most the partial area's) and LDS
+  int gp = g;
+  size_t lds = 0;
+  auto plan_kern = k == 64 ? &bcg_plan_spmm_kernel<T, 64> : &bcg_plan_spmm_kernel<T, 0>;
+  if (plan) {
+    HFLMP_REQUIRE(ctx, plan->nrow == nrow && plan->nnz == nnz && plan->device == ctx->device,
+                  "batched_pcg: the plan was built for another matrix or device");
+    HFLMP_REQUIRE(ctx, plan->umax <= plan_max_cols((int)sizeof(T)),
+                  "batched_pcg: the plan's blocks have up to %d columns, above %d for %zu-byte "
+                  "values", plan->umax, plan_max_cols((int)sizeof(T)), sizeof(T));
+    lds = plan_lds_bytes(plan->R, plan->umax, plan->emax, k, (int)sizeof(T));
+    HFLMP_REQUIRE(ctx, lds <= 128 * 1024,
+                  "batched_pcg: %zu bytes of LDS for %d x %zu-byte rows; build the plan for "
+                  "these rows", lds, k, sizeof(T));
+    if (lds > 64 * 1024)
+      HFLMP_HIP_CHECK(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(plan_kern),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               (int)lds));
+    gp = plan_grid(ctx, reinterpret_cast<const void *>(plan_kern), plan->nblk, lds);
+    if (gp > g) gp = g / kXcds * kXcds;
+  }
   const auto t0 = std::chrono::steady_clock::now();
   binit_state_kernel<T><<<1, kMaxRhs, 0, s>>>(st, k, maxit, atol, rtol, dtol);
   bcg_init_kernel<T><<<g, kBlock, 0, s>>>(nrow, k, rp, ci, val, invd, B, X, R, Z, P, part);
   bcg_init_final<T><<<k, kBlock, 0, s>>>(st, part, g, k);
   HFLMP_LAUNCH_CHECK(ctx);
   auto iteration = [&]() {
-    bcg_spmm_kernel<T><<<g, kBlock, 0, s>>>(nrow, k, rp, ci, val, P, AP, st, part);
-    bcg_alpha_final<T><<<k, kBlock, 0, s>>>(st, part, g, k);
+    if (plan) {
+      plan_kern<<<gp, kBlock, lds, s>>>(plan_view(plan), k, rp, val, P, AP, st, part);
+      bcg_alpha_final<T><<<k, kBlock, 0, s>>>(st, part, gp, k);
+    } else {
+      bcg_spmm_kernel<T><<<g, kBlock, 0, s>>>(nrow, k, rp, ci, val, P, AP, st, part);
+      bcg_alpha_final<T><<<k, kBlock, 0, s>>>(st, part, g, k);
+    }
     bcg_update_kernel<T><<<g, kBlock, 0, s>>>(nrow, k, invd, P, AP, X, R, Z, st, part);
     bcg_beta_final<T><<<k, kBlock, 0, s>>>(st, part, g, k);
     bcg_p_kernel<T><<<g, kBlock, 0, s>>>(nrow, k, Z, P, st);
@@ -781,6 +834,32 @@ int hflmp_batched_mpcg_jacobi(hflmp_ctx *ctx, int nrow, int nnz, int nrhs, const
   return batched_mpcg(ctx, nrow, nnz, nrhs, rp, ci, val, val32, invdiag32, B, X, max_outer,
                       atol, rtol, inner_maxit, inner_rtol, work, work_bytes, outer_iters,
                       inner_iters, status, res_init, res_final, seconds);
+}
+
+int hflmp_batched_pcg_jacobi_plan_f64(hflmp_ctx *ctx, const hflmp_spmm_plan *plan, int nrow,
+                                      int nnz, int nrhs, const int *rp, const int *ci,
+                                      const double *val, const double *invdiag,
+                                      const double *B, double *X, int maxit, double atol,
+                                      double rtol, double dtol, void *work, size_t work_bytes,
+                                      int *iters, int *status, double *res_init,
+                                      double *res_final, double *seconds) {
+  if (!plan) return ctx ? set_error(ctx, HFLMP_EINVAL, "batched_pcg: null plan") : HFLMP_EINVAL;
+  return batched_pcg<double>(ctx, nrow, nnz, nrhs, rp, ci, val, invdiag, B, X, maxit, atol,
+                             rtol, dtol, work, work_bytes, iters, status, res_init, res_final,
+                             seconds, plan);
+}
+
+int hflmp_batched_pcg_jacobi_plan_f32(hflmp_ctx *ctx, const hflmp_spmm_plan *plan, int nrow,
+                                      int nnz, int nrhs, const int *rp, const int *ci,
+                                      const float *val, const float *invdiag, const float *B,
+                                      float *X, int maxit, double atol, double rtol,
+                                      double dtol, void *work, size_t work_bytes, int *iters,
+                                      int *status, double *res_init, double *res_final,
+                                      double *seconds) {
+  if (!plan) return ctx ? set_error(ctx, HFLMP_EINVAL, "batched_pcg: null plan") : HFLMP_EINVAL;
+  return batched_pcg<float>(ctx, nrow, nnz, nrhs, rp, ci, val, invdiag, B, X, maxit, atol,
+                            rtol, dtol, work, work_bytes, iters, status, res_init, res_final,
+                            seconds, plan);
 }
 
 int hflmp_batched_pcg_jacobi_f32(hflmp_ctx *ctx, int nrow, int nnz, int nrhs, const int *rp,

@@ -204,9 +204,11 @@ __global__ void __launch_bounds__(kBlock)
     plan_spmm_kernel(PlanView pv, int k, const int *__restrict__ rp, const T *__restrict__ val,
                      const T *__restrict__ X, T *__restrict__ Y) {
   extern __shared__ __align__(16) unsigned char plan_lds[];
-  plan_walk<T, KC>(pv, k, rp, val, X, plan_lds, [&](int i, int c, T s) {
-    if (c < k) __builtin_nontemporal_store(s, Y + (size_t)i * k + c);
-  });
+  plan_walk<T, KC>(
+      pv, k, rp, val, X, plan_lds, [](int) { return T(0); },
+      [&](int i, int c, T s, T) {
+        if (c < k) __builtin_nontemporal_store(s, Y + (size_t)i * k + c);
+      });
 }
 
 void free_plan(hflmp_spmm_plan *p) {
@@ -418,13 +420,13 @@ int plan_spmm(hflmp_ctx *ctx, const hflmp_spmm_plan *p, const int *rp, const T *
   HFLMP_REQUIRE(ctx, lds <= 160 * 1024,
                 "plan_spmm: %zu bytes of LDS for %d x %zu-byte rows; build the plan for "
                 "these rows", lds, k, sizeof(T));
-  const int g = plan_grid(ctx->num_cus, p->nblk, lds);
   // 64 vectors, the common block width, with the row stride known to the compiler
   auto kern = k == 64 ? &plan_spmm_kernel<T, 64> : &plan_spmm_kernel<T, 0>;
   if (lds > 64 * 1024)
     HFLMP_HIP_CHECK(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
                                              hipFuncAttributeMaxDynamicSharedMemorySize,
                                              (int)lds));
+  const int g = plan_grid(ctx, reinterpret_cast<const void *>(kern), p->nblk, lds);
   kern<<<g, kBlock, lds, ctx->stream>>>(plan_view(p), k, rp, val, X, Y);
   HFLMP_LAUNCH_CHECK(ctx);
   return HFLMP_OK;

@@ -102,12 +102,16 @@ __host__ __device__ inline size_t plan_lds_bytes(int R, int umax, int emax, int 
   return x + 2 * plan_meta_bytes(R, umax, emax, vb);
 }
 
-// Workgroups of a walk: as many as fit on the CUs with `lds` bytes each (at
-// most 8 per CU), a multiple of the XCD count, and no more than the blocks
-// rounded up to that multiple.
-inline int plan_grid(int num_cus, int nblk, size_t lds) {
-  const int cus = num_cus > 0 ? num_cus : 256;
-  int per_cu = lds > 0 ? (int)(160 * 1024 / lds) : 8;
+// Workgroups of a walk: as many as are resident at once on the CUs (the
+// kernel's occupancy with `lds` dynamic bytes on top of its static LDS and
+// registers; at most 8 per CU), a multiple of the XCD count, and no more
+// than the blocks rounded up to that multiple.  A grid beyond what is
+// resident would run a second partial round of workgroups.
+inline int plan_grid(hflmp_ctx *ctx, const void *kern, int nblk, size_t lds) {
+  const int cus = ctx->num_cus > 0 ? ctx->num_cus : 256;
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, lds) != hipSuccess)
+    per_cu = lds > 0 ? (int)(160 * 1024 / lds) : 8;
   if (per_cu > 8) per_cu = 8;
   if (per_cu < 1) per_cu = 1;
   long long g = (long long)per_cu * cus / kXcds * kXcds;
@@ -262,15 +266,17 @@ __device__ __forceinline__ void plan_meta_store(const PlanMetaRegs<T> &m, const 
 // issued a whole step before.  Per row, the entries' positions and values
 // are read from LDS (the same address across the wave) and their X values
 // from the staged rows, eight in flight, added in storage order.
-// epi(i, c, s) gets row i's sum s for column c = lane (c >= k: a copy of
-// column k - 1, not to be stored).
+// pre(i) is called for row i before its sum (a value the epilogue needs,
+// loaded early so that its latency hides behind the row's entries); epi(i,
+// c, s, u) gets row i's sum s for column c = lane (c >= k: a copy of
+// column k - 1, not to be stored) and u = pre(i).
 // KC > 0: k is KC, known at compile time (the X-row offsets in LDS are
 // then shifts); KC = 0: k from the argument.
-template <typename T, int KC, typename Epi>
+template <typename T, int KC, typename Pre, typename Epi>
 __device__ __forceinline__ void plan_walk(const PlanView &pv, int k_arg,
                                           const int *__restrict__ rp, const T *__restrict__ val,
                                           const T *__restrict__ X, unsigned char *lds,
-                                          Epi &&epi) {
+                                          Pre &&pre, Epi &&epi) {
 #pragma clang fp contract(off)
   const int k = KC > 0 ? KC : k_arg;
   constexpr int nw = kBlock / kWave;
@@ -342,6 +348,8 @@ __device__ __forceinline__ void plan_walk(const PlanView &pv, int k_arg,
     const PlanMeta<T> md = meta(buf);
     for (int r = wid; r < H0.nr; r += nw) {
       const int a = md.rp[r], e = md.rp[r + 1];
+      const int gi = md.gr[r];
+      const T u = pre(gi);
       T s = 0;
       int j = a;
       for (; j + 8 <= e; j += 8) {
@@ -358,7 +366,7 @@ __device__ __forceinline__ void plan_walk(const PlanView &pv, int k_arg,
         for (int q = 0; q < 8; ++q) s += vv[q] * xv[q];
       }
       for (; j < e; ++j) s += md.v[j] * xs[md.li[j] * k + lk];
-      epi(md.gr[r], lane, s);
+      epi(gi, lane, s, u);
     }
     __syncthreads();
     H0 = H1;

@@ -117,6 +117,9 @@ def hip_lib():
             getattr(L, "hflmp_batched_pcg_jacobi_" + s).argtypes = [
                 vp, i, i, i, vp, vp, vp, vp, vp, vp, i, d, d, d, vp, sz, vp, vp, vp, vp,
                 C.POINTER(d)]
+            getattr(L, "hflmp_batched_pcg_jacobi_plan_" + s).argtypes = [
+                vp, vp, i, i, i, vp, vp, vp, vp, vp, vp, i, d, d, d, vp, sz, vp, vp, vp, vp,
+                C.POINTER(d)]
         for s in ("f64", "f32"):
             getattr(L, "hflmp_sell_create_" + s).argtypes = [vp, i, i, i, vp, vp, vp, i, vp]
             getattr(L, "hflmp_sell_set_values_" + s).argtypes = [vp, vp, vp, vp]
@@ -595,9 +598,11 @@ def batched_mpcg_jacobi(ctx, A, A32, invd32, B, X, nrhs, max_outer=50, atol=0.0,
 
 
 def batched_pcg_jacobi(ctx, A, invd, B, X, nrhs, maxit=10000, atol=0.0, rtol=1e-8, dtol=1e6,
-                       work=None):
+                       work=None, plan=None):
     """Lockstep Jacobi-PCG on nrhs right-hand sides (row-major device blocks
-    B, X); returns per-column iterations/status/residuals and the time."""
+    B, X); returns per-column iterations/status/residuals and the time.
+    plan: a SpmmPlan of A for nrhs vectors, through which the iteration's
+    products go (hflmp_batched_pcg_jacobi_plan_*)."""
     need = ctx.L.hflmp_batched_pcg_workspace_bytes(ctx.h, A.nrow, nrhs, A.dtype.itemsize)
     own = work is None
     if own:
@@ -607,11 +612,14 @@ def batched_pcg_jacobi(ctx, A, invd, B, X, nrhs, maxit=10000, atol=0.0, rtol=1e-
     r0 = np.zeros(nrhs)
     rf = np.zeros(nrhs)
     sec = C.c_double()
+    args = (A.nrow, A.nnz, nrhs, A.rp.ptr, A.ci.ptr, A.val.ptr, invd.ptr, B.ptr, X.ptr, maxit,
+            atol, rtol, dtol, work.ptr, work.nbytes, it.ctypes.data, st.ctypes.data,
+            r0.ctypes.data, rf.ctypes.data, C.byref(sec))
     try:
-        _call(ctx, "hflmp_batched_pcg_jacobi_" + _sfx(A.dtype), A.nrow, A.nnz, nrhs, A.rp.ptr,
-              A.ci.ptr, A.val.ptr, invd.ptr, B.ptr, X.ptr, maxit, atol, rtol, dtol, work.ptr,
-              work.nbytes, it.ctypes.data, st.ctypes.data, r0.ctypes.data, rf.ctypes.data,
-              C.byref(sec))
+        if plan is None:
+            _call(ctx, "hflmp_batched_pcg_jacobi_" + _sfx(A.dtype), *args)
+        else:
+            _call(ctx, "hflmp_batched_pcg_jacobi_plan_" + _sfx(A.dtype), plan.h, *args)
     finally:
         if own:
             work.free()

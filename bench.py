@@ -638,16 +638,28 @@ def bench_batched(hflmp, k=64, n=512, ncheck_cpu=2, spmm_only=False):
         t_mm = time_launches(ctx, lambda: hflmp.spmm(ctx, A, dB, dX, k), 20)
         # the same product through the row-block layout (X rows staged in LDS)
         Y0 = dX.download()
+        ctx.sync()
+        tp0 = time.perf_counter()
         P = hflmp.SpmmPlan(ctx, A, k)
+        t_build = time.perf_counter() - tp0
         t_plan = time_launches(ctx, lambda: P.spmm(A, dB, dX, k), 20)
         plan_rep = {"us": round(t_plan * 1e6, 2),
                     "GBps": round(spmm_bytes(N, nnz, k, vb) / t_plan / 1e9, 2),
+                    "frac_hbm": round(spmm_bytes(N, nnz, k, vb) / t_plan / 1e9 / HBM_PEAK_GBS,
+                                      4),
                     "bit_identical_to_csr_spmm": bool(np.array_equal(dX.download(), Y0)),
-                    **P.info()}
-        P.free()
+                    "build_s": round(t_build, 3), **P.info()}
+        # the lockstep solve with its products through the block layout
+        # (the plan built once, outside the timed solve), then through the
+        # CSR walk for comparison
         ctx.check(ctx.L.hflmp_fill_zero(ctx.h, dX.ptr, dX.nbytes), "fill")
-        info = hflmp.batched_pcg_jacobi(ctx, A, invd, dB, dX, k, rtol=rtol, maxit=50000)
+        info = hflmp.batched_pcg_jacobi(ctx, A, invd, dB, dX, k, rtol=rtol, maxit=50000,
+                                        plan=P)
         X = dX.download().reshape(N, k).astype(np.float64)
+        ctx.check(ctx.L.hflmp_fill_zero(ctx.h, dX.ptr, dX.nbytes), "fill")
+        info_csr = hflmp.batched_pcg_jacobi(ctx, A, invd, dB, dX, k, rtol=rtol, maxit=50000)
+        X_csr = dX.download().reshape(N, k).astype(np.float64)
+        P.free()
         # one at a time: the same columns through the single-vector solver
         S = hflmp.SellMatrix(ctx, A, hflmp.SELL_COL16)
         singles, t_single = [], 0.0
@@ -669,6 +681,11 @@ def bench_batched(hflmp, k=64, n=512, ncheck_cpu=2, spmm_only=False):
                      "all_converged": bool((info["status"] == 0).all()),
                      "seconds_batched": round(info["seconds"], 4),
                      "solves_per_s_batched": round(k / info["seconds"], 3),
+                     "products": "block layout (spmm_row_block_layout)",
+                     "seconds_batched_csr_products": round(info_csr["seconds"], 4),
+                     "csr_products_iters_max": int(info_csr["iters"].max()),
+                     "max_rel_diff_vs_csr_products": float(
+                         np.linalg.norm(X - X_csr) / np.linalg.norm(X_csr)),
                      "seconds_one_at_a_time_est": round(t_single_all, 4),
                      "one_at_a_time_sample": "%d of the %d columns solved singly "
                                              "(SELL operator), scaled to %d" % (ncheck, k, k),

@@ -696,6 +696,29 @@ int hflmp_batched_pcg_jacobi_f32(hflmp_ctx *ctx, int nrow, int nnz, int nrhs,
                                  void *work, size_t work_bytes, int *iters,
                                  int *status, double *res_init,
                                  double *res_final, double *seconds);
+/* The same iteration with its products Y = A P through a block layout of
+ * the matrix (hflmp_spmm_plan_create on row_ptr/col with this nrhs and
+ * value size): the products are bit-identical to the CSR walk's, the
+ * per-column p.Ap partial sums are added in the layout's block order (the
+ * iterates match the CSR form's to rounding).  Same workspace. */
+int hflmp_batched_pcg_jacobi_plan_f64(hflmp_ctx *ctx, const hflmp_spmm_plan *plan,
+                                      int nrow, int nnz, int nrhs,
+                                      const int *row_ptr, const int *col,
+                                      const double *val, const double *invdiag,
+                                      const double *B, double *X, int maxit,
+                                      double atol, double rtol, double dtol,
+                                      void *work, size_t work_bytes, int *iters,
+                                      int *status, double *res_init,
+                                      double *res_final, double *seconds);
+int hflmp_batched_pcg_jacobi_plan_f32(hflmp_ctx *ctx, const hflmp_spmm_plan *plan,
+                                      int nrow, int nnz, int nrhs,
+                                      const int *row_ptr, const int *col,
+                                      const float *val, const float *invdiag,
+                                      const float *B, float *X, int maxit,
+                                      double atol, double rtol, double dtol,
+                                      void *work, size_t work_bytes, int *iters,
+                                      int *status, double *res_init,
+                                      double *res_final, double *seconds);
 
 /* The same refinement for a block of nrhs <= 64 right-hand sides (B, X
  * row-major FP64; no reference counterpart for the block): MPIR<

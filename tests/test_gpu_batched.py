@@ -43,8 +43,10 @@ def test_spmm_long_rows(ctx):
         assert np.array_equal(Y[:, c], O.spmv(rp, ci, v, np.ascontiguousarray(X[:, c])))
 
 
-@pytest.mark.parametrize("k", [64, 5])
-def test_batched_pcg_vs_per_column_oracle(ctx, k):
+@pytest.mark.parametrize("k,planned", [(64, False), (5, False), (64, True), (5, True)])
+def test_batched_pcg_vs_per_column_oracle(ctx, k, planned):
+    # planned: the iteration's products through the block layout (patches
+    # of the 97-node lines of this Q2 mesh)
     order, dim, n = 2, 2, 48
     rp, ci, v = hflmp.fe_system(order, dim, n)
     N = len(rp) - 1
@@ -53,7 +55,12 @@ def test_batched_pcg_vs_per_column_oracle(ctx, k):
     A = hflmp.CsrMatrix(ctx, rp, ci, v)
     invd = hflmp.invdiag(ctx, A)
     dB, dX = ctx.upload(B.ravel()), ctx.zeros(N * k, np.float64)
-    info = hflmp.batched_pcg_jacobi(ctx, A, invd, dB, dX, k, rtol=1e-8, maxit=5000)
+    P = hflmp.SpmmPlan(ctx, A, k) if planned else None
+    if planned:
+        assert P.info()["line_rows"] == 97 and P.info()["runs"] > 1
+    info = hflmp.batched_pcg_jacobi(ctx, A, invd, dB, dX, k, rtol=1e-8, maxit=5000, plan=P)
+    if P is not None:
+        P.free()
     X = dX.download().reshape(N, k)
     for c in range(k):
         b = np.ascontiguousarray(B[:, c])
@@ -65,7 +72,8 @@ def test_batched_pcg_vs_per_column_oracle(ctx, k):
     assert info["iters"][3] == 0
 
 
-def test_batched_pcg_fp32(ctx):
+@pytest.mark.parametrize("planned", [False, True])
+def test_batched_pcg_fp32(ctx, planned):
     rp, ci, v = hflmp.fe_system(1, 2, 40)
     N = len(rp) - 1
     k = 16
@@ -74,7 +82,10 @@ def test_batched_pcg_fp32(ctx):
     A = hflmp.CsrMatrix(ctx, rp, ci, v32)
     invd = hflmp.invdiag(ctx, A)
     dB, dX = ctx.upload(B.ravel()), ctx.zeros(N * k, np.float32)
-    info = hflmp.batched_pcg_jacobi(ctx, A, invd, dB, dX, k, rtol=1e-5, maxit=5000)
+    P = hflmp.SpmmPlan(ctx, A, k) if planned else None
+    info = hflmp.batched_pcg_jacobi(ctx, A, invd, dB, dX, k, rtol=1e-5, maxit=5000, plan=P)
+    if P is not None:
+        P.free()
     X = dX.download().reshape(N, k)
     for c in (0, 7, 15):
         xo, io = O.cg(rp, ci, v32, np.ascontiguousarray(B[:, c]), prec=O.PREC_JACOBI, rtol=1e-5)
@@ -127,3 +138,29 @@ def test_batched_mixed_vs_mpir_oracle(ctx, k):
         assert np.linalg.norm(xg - xo) <= 1e-6 * np.linalg.norm(xo)
         xc, _ = O.cg(rp, ci, v, b, prec=O.PREC_JACOBI, rtol=1e-10, maxit=100000)
         assert np.linalg.norm(xg - xc) <= 1e-6 * np.linalg.norm(xc)
+
+
+def test_batched_pcg_plan_matches_csr_form(ctx):
+    # the same solve with the products through the block layout: equal
+    # iteration counts and x to rounding (only the order of the p.Ap partial
+    # sums differs); a plan built for another matrix is refused
+    rp, ci, v = hflmp.fe_system(2, 2, 40)
+    N, k = len(rp) - 1, 64
+    B = np.stack([hflmp.fe_rhs(2, 2, 40, 200 + c) for c in range(k)], axis=1)
+    A = hflmp.CsrMatrix(ctx, rp, ci, v)
+    invd = hflmp.invdiag(ctx, A)
+    dB = ctx.upload(B.ravel())
+    dX1, dX2 = ctx.zeros(N * k, np.float64), ctx.zeros(N * k, np.float64)
+    i1 = hflmp.batched_pcg_jacobi(ctx, A, invd, dB, dX1, k, rtol=1e-8, maxit=20000)
+    P = hflmp.SpmmPlan(ctx, A, k)
+    i2 = hflmp.batched_pcg_jacobi(ctx, A, invd, dB, dX2, k, rtol=1e-8, maxit=20000, plan=P)
+    assert (i1["status"] == 0).all() and (i2["status"] == 0).all()
+    assert np.abs(i1["iters"] - i2["iters"]).max() <= 1
+    X1, X2 = dX1.download().reshape(N, k), dX2.download().reshape(N, k)
+    assert np.linalg.norm(X1 - X2) <= 1e-9 * np.linalg.norm(X1)
+    rp2, ci2, v2 = hflmp.fe_system(2, 2, 30)
+    A2 = hflmp.CsrMatrix(ctx, rp2, ci2, v2)
+    P2 = hflmp.SpmmPlan(ctx, A2, k)
+    with pytest.raises(hflmp.HflmpError):
+        hflmp.batched_pcg_jacobi(ctx, A, invd, dB, dX2, k, rtol=1e-8, maxit=5000, plan=P2)
+    P.free(); P2.free(); A2.free(); A.free()
