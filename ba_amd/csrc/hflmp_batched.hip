@@ -687,11 +687,14 @@ inline BmpLayout bmp_layout(hflmp_ctx *ctx, int nrow, int k) {
   return L;
 }
 
+// plan32: the inner FP32 solves' products through this block layout (null:
+// the CSR walk).
 int batched_mpcg(hflmp_ctx *ctx, int nrow, int nnz, int k, const int *rp, const int *ci,
                  const double *val, const float *val32, const float *invd32, const double *B,
                  double *X, int max_outer, double atol, double rtol, int inner_maxit,
                  double inner_rtol, void *work, size_t work_bytes, int *outer, int *inner,
-                 int *status, double *res_init, double *res_final, double *seconds) {
+                 int *status, double *res_init, double *res_final, double *seconds,
+                 const hflmp_spmm_plan *plan32 = nullptr) {
   if (!ctx) return HFLMP_EINVAL;
   HFLMP_REQUIRE(ctx, nrow > 0 && nnz > 0 && k >= 1 && k <= kMaxRhs,
                 "batched_mpcg: need nrow, nnz > 0 and 1 <= nrhs <= 64");
@@ -751,7 +754,7 @@ int batched_mpcg(hflmp_ctx *ctx, int nrow, int nnz, int k, const int *rp, const 
     int it[kMaxRhs];
     rc = batched_pcg<float>(ctx, nrow, nnz, k, rp, ci, val32, invd32, R32, C32, inner_maxit,
                             0.0, inner_rtol, 1e6, w + L.inner, L.r32 - L.inner, it, nullptr,
-                            nullptr, nullptr, nullptr);
+                            nullptr, nullptr, nullptr, plan32);
     if (rc) return rc;
     bmp_correct_kernel<<<grid_for(nk, 4), kBlock, 0, s>>>(nk, k, C32, live, X);
     HFLMP_LAUNCH_CHECK(ctx);
@@ -860,6 +863,21 @@ int hflmp_batched_pcg_jacobi_plan_f32(hflmp_ctx *ctx, const hflmp_spmm_plan *pla
   return batched_pcg<float>(ctx, nrow, nnz, nrhs, rp, ci, val, invdiag, B, X, maxit, atol,
                             rtol, dtol, work, work_bytes, iters, status, res_init, res_final,
                             seconds, plan);
+}
+
+int hflmp_batched_mpcg_jacobi_plan(hflmp_ctx *ctx, const hflmp_spmm_plan *plan32, int nrow,
+                                   int nnz, int nrhs, const int *rp, const int *ci,
+                                   const double *val, const float *val32,
+                                   const float *invdiag32, const double *B, double *X,
+                                   int max_outer, double atol, double rtol, int inner_maxit,
+                                   double inner_rtol, void *work, size_t work_bytes,
+                                   int *outer_iters, int *inner_iters, int *status,
+                                   double *res_init, double *res_final, double *seconds) {
+  if (!plan32)
+    return ctx ? set_error(ctx, HFLMP_EINVAL, "batched_mpcg: null plan") : HFLMP_EINVAL;
+  return batched_mpcg(ctx, nrow, nnz, nrhs, rp, ci, val, val32, invdiag32, B, X, max_outer,
+                      atol, rtol, inner_maxit, inner_rtol, work, work_bytes, outer_iters,
+                      inner_iters, status, res_init, res_final, seconds, plan32);
 }
 
 int hflmp_batched_pcg_jacobi_f32(hflmp_ctx *ctx, int nrow, int nnz, int nrhs, const int *rp,

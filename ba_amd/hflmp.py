@@ -104,6 +104,9 @@ def hip_lib():
         L.hflmp_batched_mpcg_workspace_bytes.argtypes = [vp, i, i]
         L.hflmp_batched_mpcg_jacobi.argtypes = [vp, i, i, i, vp, vp, vp, vp, vp, vp, vp, i, d, d,
                                                 i, d, vp, sz, vp, vp, vp, vp, vp, C.POINTER(d)]
+        L.hflmp_batched_mpcg_jacobi_plan.argtypes = [vp, vp, i, i, i, vp, vp, vp, vp, vp, vp,
+                                                     vp, i, d, d, i, d, vp, sz, vp, vp, vp, vp,
+                                                     vp, C.POINTER(d)]
         L.hflmp_spmm_plan_create.argtypes = [vp, i, i, i, vp, vp, i, i, i, vp]
         L.hflmp_spmm_plan_destroy.argtypes = [vp]
         L.hflmp_spmm_plan_destroy.restype = None
@@ -569,11 +572,12 @@ class SpmmPlan:
 
 
 def batched_mpcg_jacobi(ctx, A, A32, invd32, B, X, nrhs, max_outer=50, atol=0.0, rtol=1e-8,
-                        inner_maxit=100000, inner_rtol=1e-4, work=None):
+                        inner_maxit=100000, inner_rtol=1e-4, work=None, plan32=None):
     """FP32-inner / FP64-refined Jacobi-CG on nrhs right-hand sides (MPIR per
     column; A FP64, A32 its FP32 copy sharing the pattern, invd32 its inverse
     diagonal); returns per-column corrections / inner iterations / status /
-    residuals and the time."""
+    residuals and the time.  plan32: a SpmmPlan of A32 (value_bytes 4) for
+    the inner solves' products."""
     need = ctx.L.hflmp_batched_mpcg_workspace_bytes(ctx.h, A.nrow, nrhs)
     own = work is None
     if own:
@@ -584,12 +588,15 @@ def batched_mpcg_jacobi(ctx, A, A32, invd32, B, X, nrhs, max_outer=50, atol=0.0,
     r0 = np.zeros(nrhs)
     rf = np.zeros(nrhs)
     sec = C.c_double()
+    args = (A.nrow, A.nnz, nrhs, A.rp.ptr, A.ci.ptr, A.val.ptr, A32.val.ptr, invd32.ptr, B.ptr,
+            X.ptr, int(max_outer), float(atol), float(rtol), int(inner_maxit),
+            float(inner_rtol), work.ptr, work.nbytes, out.ctypes.data, inn.ctypes.data,
+            st.ctypes.data, r0.ctypes.data, rf.ctypes.data, C.byref(sec))
     try:
-        _call(ctx, "hflmp_batched_mpcg_jacobi", A.nrow, A.nnz, nrhs, A.rp.ptr, A.ci.ptr,
-              A.val.ptr, A32.val.ptr, invd32.ptr, B.ptr, X.ptr, int(max_outer), float(atol),
-              float(rtol), int(inner_maxit), float(inner_rtol), work.ptr, work.nbytes,
-              out.ctypes.data, inn.ctypes.data, st.ctypes.data, r0.ctypes.data,
-              rf.ctypes.data, C.byref(sec))
+        if plan32 is None:
+            _call(ctx, "hflmp_batched_mpcg_jacobi", *args)
+        else:
+            _call(ctx, "hflmp_batched_mpcg_jacobi_plan", plan32.h, *args)
     finally:
         if own:
             work.free()

@@ -743,6 +743,21 @@ int hflmp_batched_mpcg_jacobi(hflmp_ctx *ctx, int nrow, int nnz, int nrhs,
                               void *work, size_t work_bytes, int *outer_iters,
                               int *inner_iters, int *status, double *res_init,
                               double *res_final, double *seconds);
+/* The same with the inner FP32 solves' products through a block layout
+ * of the FP32 matrix (hflmp_spmm_plan_create with value_bytes 4 and this
+ * nrhs), as hflmp_batched_pcg_jacobi_plan_f32. */
+int hflmp_batched_mpcg_jacobi_plan(hflmp_ctx *ctx, const hflmp_spmm_plan *plan32,
+                                   int nrow, int nnz, int nrhs,
+                                   const int *row_ptr, const int *col,
+                                   const double *val, const float *val32,
+                                   const float *invdiag32, const double *B,
+                                   double *X, int max_outer, double atol,
+                                   double rtol, int inner_maxit,
+                                   double inner_rtol, void *work,
+                                   size_t work_bytes, int *outer_iters,
+                                   int *inner_iters, int *status,
+                                   double *res_init, double *res_final,
+                                   double *seconds);
 
 /* FP32-inner / FP64-refined Jacobi-CG: the refinement of
  * MPIR<LADescriptorCoupledD, LADescriptorCoupledS> (linear_solver/mpir.cc:

@@ -110,12 +110,13 @@ def test_spmm_batch_boundaries(ctx, lengths):
         assert np.array_equal(Y[:, c], O.spmv(rp, ci, v, np.ascontiguousarray(X[:, c])))
 
 
-@pytest.mark.parametrize("k", [16, 5])
-def test_batched_mixed_vs_mpir_oracle(ctx, k):
+@pytest.mark.parametrize("k,planned", [(16, False), (5, False), (16, True)])
+def test_batched_mixed_vs_mpir_oracle(ctx, k, planned):
     """FP32-inner / FP64-refined blocked CG against the oracle's MPIR
     (or_mpir_ds: FP64 residual, FP32 Jacobi-PCG correction) column by column:
     corrections within 1, the true FP64 residual below 1e-8, and x within
-    1e-6 of the oracle's and of the FP64 CPU CG solve."""
+    1e-6 of the oracle's and of the FP64 CPU CG solve (planned: the inner
+    products through the FP32 block layout)."""
     rp, ci, v = hflmp.fe_system(2, 2, 24, hflmp.fe_kappa(2, 24, 1.0, 41))
     N = len(rp) - 1
     B = np.stack([hflmp.fe_rhs(2, 2, 24, 100 + c) for c in range(k)], axis=1)
@@ -123,8 +124,11 @@ def test_batched_mixed_vs_mpir_oracle(ctx, k):
     A32 = A.astype_f32()
     invd32 = hflmp.invdiag(ctx, A32)
     dB, dX = ctx.upload(np.ascontiguousarray(B).ravel()), ctx.zeros(N * k, np.float64)
+    P = hflmp.SpmmPlan(ctx, A32, k, 4) if planned else None
     info = hflmp.batched_mpcg_jacobi(ctx, A, A32, invd32, dB, dX, k, rtol=1e-8,
-                                     inner_rtol=1e-4)
+                                     inner_rtol=1e-4, plan32=P)
+    if P is not None:
+        P.free()
     X = dX.download().reshape(N, k)
     for c in range(k):
         b = np.ascontiguousarray(B[:, c])
