@@ -83,11 +83,12 @@ struct SlotState {
 struct SysDesc {
   int nrow, nslice, nunits, unit0;
   int mode, col16;  // mixed: FP32 columns are int16 offsets from the slice start
+  int col16_64;     // FP64 layout: int16 column offsets (as the device decided at admission)
   // FP64 SELL-64 (P = 2): the operator of FP64 systems and of the residuals
   const long long *soff;
   const int *slen;
   const unsigned char *rlen;
-  const void *col;  // int16 or int32 by st->col16
+  const void *col;  // int16 or int32 by col16_64
   const double *sval;
   // FP32 SELL-64 (P = 4): the inner operator of mixed systems
   const long long *soff32;
@@ -277,8 +278,8 @@ __global__ void __launch_bounds__(kBlock)
   if (!mixed_outer_due(d.st)) return;
   const bool fresh = d.st->status == kFresh;
   TileRow tr;
-  const double s = d.st->col16 ? op64<short>(d).row_sum(j, d.x, 0.0, tr, nullptr)
-                               : op64<int>(d).row_sum(j, d.x, 0.0, tr, nullptr);
+  const double s = d.col16_64 ? op64<short>(d).row_sum(j, d.x, 0.0, tr, nullptr)
+                              : op64<int>(d).row_sum(j, d.x, 0.0, tr, nullptr);
   double rr = 0;
   float rr32 = 0, rz32 = 0;
   if (tr.has_row) {
@@ -369,10 +370,13 @@ __global__ void __launch_bounds__(kBlock)
   int j;
   if (!unit_of(sys, unit_sys, U, d, j)) return;
   TileRow tr;
+  // the row sums' loads go out before the state is known; a stopped
+  // system's results are dropped
   if (d.mode == kModeFp64) {
-    if (d.st->s64.status != kIterate) return;
-    const double s = d.st->col16 ? op64<short>(d).row_sum(j, d.p, 0.0, tr, nullptr)
-                                 : op64<int>(d).row_sum(j, d.p, 0.0, tr, nullptr);
+    const int status = d.st->s64.status;
+    const double s = d.col16_64 ? op64<short>(d).row_sum(j, d.p, 0.0, tr, nullptr)
+                                : op64<int>(d).row_sum(j, d.p, 0.0, tr, nullptr);
+    if (status != kIterate) return;
     double part = 0;
     if (tr.has_row) {
       d.Ap[tr.row] = s;
@@ -380,9 +384,10 @@ __global__ void __launch_bounds__(kBlock)
     }
     unit_partial1(part, d.part, j);
   } else {
-    if (!mixed_inner_live(d.st)) return;
+    const bool live = mixed_inner_live(d.st);
     const float s = d.col16 ? op32<short>(d).row_sum(j, d.p32, 0.0f, tr, nullptr)
                             : op32<int>(d).row_sum(j, d.p32, 0.0f, tr, nullptr);
+    if (!live) return;
     float part = 0;
     if (tr.has_row) {
       d.Ap32[tr.row] = s;
@@ -1267,8 +1272,44 @@ int admit_queued(hflmp_server *sv) {
   return HFLMP_OK;
 }
 
-// Waits for the chunk and retires the systems that stopped: x to the
-// caller's buffer, block back to the arena, slot freed.
+// Retires one system: x to the caller's buffer (0 if refused), its
+// results from its final state, block back to the arena, slot freed.
+int retire_one(hflmp_server *sv, int slot, const SlotState &st, int *n_done) {
+  hflmp_ctx *ctx = sv->ctx;
+  Job &jb = sv->jobs[sv->slot_job[slot]];
+  const bool refused = (st.mode == kModeMixed ? st.status : st.s64.status) == kRefused;
+  if (refused)
+    std::memset(jb.x_out, 0, 8 * (size_t)jb.nrow);
+  else
+    HFLMP_HIP_CHECK(ctx,
+                    hipMemcpy(jb.x_out, jb.d.x, 8 * (size_t)jb.nrow, hipMemcpyDeviceToHost));
+  // zeroed again by the slot's next admission
+  unsigned int bad = 0;
+  HFLMP_HIP_CHECK(ctx, hipMemcpy(&bad, sv->d_bad + slot, sizeof(bad), hipMemcpyDeviceToHost));
+  if (st.mode == kModeMixed) {
+    jb.iters = st.inner_total;
+    jb.status = st.status;
+    jb.outer = st.outer;
+    jb.res_init = st.res0;
+    jb.res_final = st.res;
+  } else {
+    jb.iters = st.s64.iter;
+    jb.status = st.s64.status;
+    jb.outer = 0;
+    jb.res_init = st.s64.res0;
+    jb.res_final = st.s64.res;
+  }
+  jb.bad_diag = (int)bad;
+  jb.col_bytes = st.col16 ? 2 : 4;
+  jb.t_done = Clock::now();
+  jb.state = kDone;
+  sv->alloc.release(jb.off, jb.kept);
+  sv->slot_job[slot] = -1;
+  ++*n_done;
+  return HFLMP_OK;
+}
+
+// Waits for the chunk and retires the systems that stopped.
 int retire_stopped(hflmp_server *sv, int *n_done) {
   hflmp_ctx *ctx = sv->ctx;
   const auto t_wait = Clock::now();
@@ -1278,56 +1319,46 @@ int retire_stopped(hflmp_server *sv, int *n_done) {
     std::vector<int> keep;
     for (int slot : *list) {
       const SlotState &st = sv->h_states[slot];
-      Job &jb = sv->jobs[sv->slot_job[slot]];
       if (slot_running(st)) {
         keep.push_back(slot);
         continue;
       }
-      const bool refused = (st.mode == kModeMixed ? st.status : st.s64.status) == kRefused;
-      if (refused)
-        std::memset(jb.x_out, 0, 8 * (size_t)jb.nrow);
-      else
-        HFLMP_HIP_CHECK(ctx, hipMemcpy(jb.x_out, jb.d.x, 8 * (size_t)jb.nrow,
-                                       hipMemcpyDeviceToHost));
-      // zeroed again by the slot's next admission
-      unsigned int bad = 0;
-      HFLMP_HIP_CHECK(ctx,
-                      hipMemcpy(&bad, sv->d_bad + slot, sizeof(bad), hipMemcpyDeviceToHost));
-      if (st.mode == kModeMixed) {
-        jb.iters = st.inner_total;
-        jb.status = st.status;
-        jb.outer = st.outer;
-        jb.res_init = st.res0;
-        jb.res_final = st.res;
-      } else {
-        jb.iters = st.s64.iter;
-        jb.status = st.s64.status;
-        jb.outer = 0;
-        jb.res_init = st.s64.res0;
-        jb.res_final = st.s64.res;
-      }
-      jb.bad_diag = (int)bad;
-      jb.col_bytes = st.col16 ? 2 : 4;
-      jb.t_done = Clock::now();
-      jb.state = kDone;
-      sv->alloc.release(jb.off, jb.kept);
-      sv->slot_job[slot] = -1;
+      const int rc = retire_one(sv, slot, st, n_done);
+      if (rc) return rc;
       if (list == &sv->order) sv->set_changed = true;
-      ++*n_done;
     }
     list->swap(keep);
   }
   return HFLMP_OK;
 }
 
-int join_admitted(hflmp_server *sv) {
+// A system refused at admission (a column out of range) retires here and
+// never enters the tables: the grouped kernels issue their gathers before
+// they read a system's state.
+int join_admitted(hflmp_server *sv, int *n_done) {
   hflmp_ctx *ctx = sv->ctx;
   if (sv->joining.empty()) return HFLMP_OK;
   const auto t0 = Clock::now();
+  // each admitted system's state as admission left it: the FP64 layout's
+  // column width that the band kernel decided (the grouped kernels take it
+  // from the system table rather than from the state)
+  for (int slot : sv->joining)
+    HFLMP_HIP_CHECK(ctx, hipMemcpyAsync(sv->h_init + slot, sv->d_states + slot,
+                                        sizeof(SlotState), hipMemcpyDeviceToHost,
+                                        sv->admit_stream));
   HFLMP_HIP_CHECK(ctx, hipStreamSynchronize(sv->admit_stream));
   sv->stage_used = 0;
   sv->join_seconds += std::chrono::duration<double>(Clock::now() - t0).count();
-  for (int slot : sv->joining) {
+  std::vector<int> joining;
+  joining.swap(sv->joining);
+  for (int slot : joining) {
+    const SlotState &st = sv->h_init[slot];
+    if ((st.mode == kModeMixed ? st.status : st.s64.status) == kRefused) {
+      const int rc = retire_one(sv, slot, st, n_done);
+      if (rc) return rc;
+      continue;
+    }
+    sv->jobs[sv->slot_job[slot]].d.col16_64 = st.col16;
     if (sv->jobs[sv->slot_job[slot]].solo) {
       sv->solo.push_back(slot);
     } else {
@@ -1335,7 +1366,6 @@ int join_admitted(hflmp_server *sv) {
       sv->set_changed = true;
     }
   }
-  sv->joining.clear();
   sv->peak_active = std::max(sv->peak_active, (int)(sv->order.size() + sv->solo.size()));
   sv->peak_in_use = std::max(sv->peak_in_use, sv->budget - sv->alloc.total_free());
   return HFLMP_OK;
@@ -1492,7 +1522,7 @@ int hflmp_server_step(hflmp_server *sv, int *n_done) {
     int rc = launch_chunk(sv, &launched);
     if (!rc) rc = admit_queued(sv);
     if (!rc && launched) rc = retire_stopped(sv, &done);
-    if (!rc) rc = join_admitted(sv);
+    if (!rc) rc = join_admitted(sv, &done);
     if (rc) return rc;
     if (launched || (sv->order.empty() && sv->solo.empty())) break;
   }
