@@ -78,6 +78,7 @@ struct SlotState {
   int outer, max_outer, inner_total, inner_maxit;
   int col16;  // FP64 operator: 1 if its columns are int16 offsets from the slice
               // start (decided on the device at admission), else int32
+  unsigned int bad;  // rows whose diagonal is missing or zero (counted at admission)
 };
 
 struct SysDesc {
@@ -796,6 +797,29 @@ void plan_job(Job &jb) {
   jb.kept = o;
 }
 
+// One of the two table sets that chunks run on in turn: the system and
+// unit tables on the device (uploaded from pinned staging), the chunk's
+// captured graph over them, and the pinned copy of all states taken after
+// the chunk.  A set is rebuilt only when the chunk that last used it has
+// finished.
+struct TableSet {
+  long long version = -1;  // the version of the active set these tables hold
+  int S = 0, U = 0;
+  SysDesc *d_sys = nullptr, *h_sys = nullptr;  // [max_active]
+  int *d_unit = nullptr, *h_unit = nullptr;    // [unit_cap]
+  size_t unit_cap = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  SlotState *h_states = nullptr;  // [max_active]
+  hipEvent_t done = nullptr;      // recorded after the state copy
+};
+
+// A retired system's arena block, released once no chunk in flight lists it.
+struct Deferred {
+  int slot;
+  size_t off, bytes;
+};
+
 }  // namespace
 
 struct hflmp_server {
@@ -806,27 +830,28 @@ struct hflmp_server {
   int max_active = 0, chunk = 0, use_graph = 1, mixed_min_nnz = 0, mixed_max_nnz = 0;
   double inner_rtol = 1e-2;
   SlotState *d_states = nullptr;  // [max_active]
-  SlotState *h_states = nullptr;  // pinned mirror, written by each chunk's state copy
   SlotState *h_init = nullptr;    // pinned: each slot's initial state at admission
   hipStream_t admit_stream = nullptr;  // admission work, beside the chunk in flight
   char *stage = nullptr;               // pinned staging for admission uploads
   size_t stage_cap = 0, stage_used = 0;
+  hipStream_t result_stream = nullptr; // retired systems' x to the host
+  char *rstage = nullptr;              // pinned staging for those copies
+  size_t rstage_cap = 0;
   std::vector<int> joining;            // admitted slots that join after the chunk
-  SysDesc *d_sys = nullptr;       // [max_active]
-  int *d_unit_sys = nullptr;
-  size_t unit_cap = 0;
-  unsigned int *d_bad = nullptr;
-  std::vector<SysDesc> h_sys;
-  std::vector<int> h_unit_sys;
   std::vector<long long> slot_job;  // slot -> ticket, -1 free
   std::vector<int> order;           // active slots in table order
   std::deque<long long> queue;
   std::map<long long, Job> jobs;
   long long next_ticket = 1;
-  bool set_changed = true;
-  int U = 0, S = 0;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t graph_exec = nullptr;
+  // Chunk c runs on table set c % 2.  A step queues the next chunk before
+  // it waits for the one before, so the device always has a chunk queued
+  // while the host retires, admits and returns to the caller.  The active
+  // set's version changes when grouped systems join or leave.
+  long long version = 0;
+  TableSet sets[2];
+  long long launched = 0, completed = 0;  // chunks queued / waited for
+  std::vector<long long> first_chunk;     // per slot: the first chunk that lists it
+  std::vector<Deferred> deferred;         // retired, still listed by the chunk in flight
   long long steps = 0, graph_captures = 0;
   int peak_active = 0;
   size_t peak_in_use = 0;
@@ -841,36 +866,37 @@ struct hflmp_server {
   // host time of step(): admission (uploads, layout kernels queued), table
   // rebuild and graph capture, waiting for the step's kernels
   double admit_seconds = 0, capture_seconds = 0, wait_seconds = 0, join_seconds = 0;
+  double retire_seconds = 0;
 };
 
 namespace {
 
 using Clock = std::chrono::steady_clock;
 
-void drop_graph(hflmp_server *sv) {
-  if (sv->graph_exec) (void)hipGraphExecDestroy(sv->graph_exec);
-  if (sv->graph) (void)hipGraphDestroy(sv->graph);
-  sv->graph_exec = nullptr;
-  sv->graph = nullptr;
+void drop_graph(TableSet &t) {
+  if (t.graph_exec) (void)hipGraphExecDestroy(t.graph_exec);
+  if (t.graph) (void)hipGraphDestroy(t.graph);
+  t.graph_exec = nullptr;
+  t.graph = nullptr;
 }
 
-// The work of one step on the stream: starts of fresh FP64 systems, outer
+// The work of one chunk on the stream: starts of fresh FP64 systems, outer
 // steps of mixed systems, then `chunk` iterations.
-void enqueue_chunk(hflmp_server *sv, hipStream_t s) {
-  const unsigned gU = (unsigned)(((sv->U + kXcds - 1) / kXcds) * kXcds);
-  const unsigned gS = (unsigned)sv->S;
-  grp_init_kernel<<<gU, kBlock, 0, s>>>(sv->d_sys, sv->d_unit_sys, sv->U);
-  grp_init_final<<<gS, kBlock, 0, s>>>(sv->d_sys);
-  grp_mix_correct_kernel<<<gU, kBlock, 0, s>>>(sv->d_sys, sv->d_unit_sys, sv->U);
-  grp_mix_residual_kernel<<<gU, kBlock, 0, s>>>(sv->d_sys, sv->d_unit_sys, sv->U);
-  grp_mix_residual_final<<<gS, kBlock, 0, s>>>(sv->d_sys);
-  grp_mix_prefresh_kernel<<<gU, kBlock, 0, s>>>(sv->d_sys, sv->d_unit_sys, sv->U);
+void enqueue_chunk(const hflmp_server *sv, const TableSet &t, hipStream_t s) {
+  const unsigned gU = (unsigned)(((t.U + kXcds - 1) / kXcds) * kXcds);
+  const unsigned gS = (unsigned)t.S;
+  grp_init_kernel<<<gU, kBlock, 0, s>>>(t.d_sys, t.d_unit, t.U);
+  grp_init_final<<<gS, kBlock, 0, s>>>(t.d_sys);
+  grp_mix_correct_kernel<<<gU, kBlock, 0, s>>>(t.d_sys, t.d_unit, t.U);
+  grp_mix_residual_kernel<<<gU, kBlock, 0, s>>>(t.d_sys, t.d_unit, t.U);
+  grp_mix_residual_final<<<gS, kBlock, 0, s>>>(t.d_sys);
+  grp_mix_prefresh_kernel<<<gU, kBlock, 0, s>>>(t.d_sys, t.d_unit, t.U);
   for (int it = 0; it < sv->chunk; ++it) {
-    grp_spmv_kernel<<<gU, kBlock, 0, s>>>(sv->d_sys, sv->d_unit_sys, sv->U);
-    grp_alpha_final<<<gS, kBlock, 0, s>>>(sv->d_sys);
-    grp_update_kernel<<<gU, kBlock, 0, s>>>(sv->d_sys, sv->d_unit_sys, sv->U);
-    grp_beta_final<<<gS, kBlock, 0, s>>>(sv->d_sys);
-    grp_p_kernel<<<gU, kBlock, 0, s>>>(sv->d_sys, sv->d_unit_sys, sv->U);
+    grp_spmv_kernel<<<gU, kBlock, 0, s>>>(t.d_sys, t.d_unit, t.U);
+    grp_alpha_final<<<gS, kBlock, 0, s>>>(t.d_sys);
+    grp_update_kernel<<<gU, kBlock, 0, s>>>(t.d_sys, t.d_unit, t.U);
+    grp_beta_final<<<gS, kBlock, 0, s>>>(t.d_sys);
+    grp_p_kernel<<<gU, kBlock, 0, s>>>(t.d_sys, t.d_unit, t.U);
   }
 }
 
@@ -910,38 +936,45 @@ void solo_chunk(hflmp_ctx *ctx, const Job &jb, hipStream_t s, int iters) {
   }
 }
 
-int rebuild_tables(hflmp_server *sv) {
+// Builds table set t for the current active set (grouped systems in table
+// order) in its pinned staging and queues the uploads on the context
+// stream.  The chunk that last ran on t has finished, so neither its
+// staging nor its device tables are in use.
+int rebuild_tables(hflmp_server *sv, TableSet &t) {
   hflmp_ctx *ctx = sv->ctx;
-  sv->h_sys.clear();
-  sv->h_unit_sys.clear();
-  int unit0 = 0;
-  for (int slot : sv->order) {
-    Job &jb = sv->jobs[sv->slot_job[slot]];
-    jb.d.unit0 = unit0;
-    sv->h_sys.push_back(jb.d);
-    for (int u = 0; u < jb.nunits; ++u) sv->h_unit_sys.push_back((int)sv->h_sys.size() - 1);
-    unit0 += jb.nunits;
-  }
-  sv->S = (int)sv->h_sys.size();
-  sv->U = unit0;
-  if ((size_t)sv->U > sv->unit_cap) {
-    if (sv->d_unit_sys) (void)hipFree(sv->d_unit_sys);
-    sv->d_unit_sys = nullptr;
-    sv->unit_cap = std::max((size_t)sv->U, 2 * sv->unit_cap);
-    if (hipMalloc(&sv->d_unit_sys, sizeof(int) * sv->unit_cap) != hipSuccess) {
-      sv->unit_cap = 0;
+  int U = 0;
+  for (int slot : sv->order) U += sv->jobs[sv->slot_job[slot]].nunits;
+  if ((size_t)U > t.unit_cap) {
+    if (t.d_unit) (void)hipFree(t.d_unit);
+    if (t.h_unit) (void)hipHostFree(t.h_unit);
+    t.d_unit = nullptr;
+    t.h_unit = nullptr;
+    t.unit_cap = std::max((size_t)U, 2 * t.unit_cap);
+    if (hipMalloc(&t.d_unit, sizeof(int) * t.unit_cap) != hipSuccess ||
+        hipHostMalloc(&t.h_unit, sizeof(int) * t.unit_cap) != hipSuccess) {
+      t.unit_cap = 0;
       return set_error(ctx, HFLMP_ENOMEM, "server: unit table allocation failed");
     }
   }
-  if (sv->S) {
-    HFLMP_HIP_CHECK(ctx, hipMemcpyAsync(sv->d_sys, sv->h_sys.data(), sizeof(SysDesc) * sv->S,
-                                        hipMemcpyHostToDevice, ctx->stream));
-    HFLMP_HIP_CHECK(ctx, hipMemcpyAsync(sv->d_unit_sys, sv->h_unit_sys.data(),
-                                        sizeof(int) * sv->U, hipMemcpyHostToDevice,
-                                        ctx->stream));
+  int S = 0, unit0 = 0;
+  for (int slot : sv->order) {
+    Job &jb = sv->jobs[sv->slot_job[slot]];
+    jb.d.unit0 = unit0;
+    t.h_sys[S] = jb.d;
+    for (int u = 0; u < jb.nunits; ++u) t.h_unit[unit0 + u] = S;
+    unit0 += jb.nunits;
+    ++S;
   }
-  drop_graph(sv);
-  sv->set_changed = false;
+  t.S = S;
+  t.U = U;
+  if (S) {
+    HFLMP_HIP_CHECK(ctx, hipMemcpyAsync(t.d_sys, t.h_sys, sizeof(SysDesc) * S,
+                                        hipMemcpyHostToDevice, ctx->stream));
+    HFLMP_HIP_CHECK(ctx, hipMemcpyAsync(t.d_unit, t.h_unit, sizeof(int) * U,
+                                        hipMemcpyHostToDevice, ctx->stream));
+  }
+  drop_graph(t);
+  t.version = sv->version;
   return HFLMP_OK;
 }
 
@@ -1019,7 +1052,6 @@ int admit(hflmp_server *sv, Job &jb, int slot) {
   }
   if (!rc) rc = fill_zero_on(ctx, dp(jb.o_col), 4 * (size_t)jb.sp64.entries, s);
   if (!rc) rc = fill_zero_on(ctx, dp(jb.o_sval), 8 * (size_t)jb.sp64.entries, s);
-  if (!rc) rc = fill_zero_on(ctx, sv->d_bad + slot, sizeof(unsigned int), s);
   if (!rc && mixed) {
     rc = fill_zero_on(ctx, dp(jb.o_col32), (jb.col16 ? 2 : 4) * (size_t)jb.sp32.entries, s);
     if (!rc) rc = fill_zero_on(ctx, dp(jb.o_sval32), 4 * (size_t)jb.sp32.entries, s);
@@ -1062,21 +1094,21 @@ int admit(hflmp_server *sv, Job &jb, int slot) {
     sys_fill_kernel<int><<<g, kBlock, 0, s>>>(jb.nrow, rp, ci, val, soff, slen, dst, col, sval,
                                               static_cast<double *>(dp(jb.o_invd)), nullptr,
                                               nullptr, nullptr, nullptr, nullptr,
-                                              sv->d_bad + slot);
+                                              &sv->d_states[slot].bad);
   } else if (jb.col16) {
     sys_fill_kernel<short><<<g, kBlock, 0, s>>>(
         jb.nrow, rp, ci, val, soff, slen, dst, col, sval, nullptr,
         static_cast<const long long *>(dp(jb.o_soff32)),
         static_cast<const int *>(dp(jb.o_slen32)), static_cast<short *>(dp(jb.o_col32)),
         static_cast<float *>(dp(jb.o_sval32)), static_cast<float *>(dp(jb.o_invd32)),
-        sv->d_bad + slot);
+        &sv->d_states[slot].bad);
   } else {
     sys_fill_kernel<int><<<g, kBlock, 0, s>>>(
         jb.nrow, rp, ci, val, soff, slen, dst, col, sval, nullptr,
         static_cast<const long long *>(dp(jb.o_soff32)),
         static_cast<const int *>(dp(jb.o_slen32)), static_cast<int *>(dp(jb.o_col32)),
         static_cast<float *>(dp(jb.o_sval32)), static_cast<float *>(dp(jb.o_invd32)),
-        sv->d_bad + slot);
+        &sv->d_states[slot].bad);
   }
   HFLMP_LAUNCH_CHECK(ctx);
   SysDesc &d = jb.d;
@@ -1121,21 +1153,20 @@ int admit(hflmp_server *sv, Job &jb, int slot) {
 
 // The grouped systems' chunk on the context stream: a replay of the
 // captured graph (captured again after the set changed), or direct launches.
-int launch_group(hflmp_server *sv) {
+int launch_group(hflmp_server *sv, TableSet &t) {
   hflmp_ctx *ctx = sv->ctx;
   hipStream_t s = ctx->stream;
-  const auto t0 = Clock::now();
   if (!sv->use_graph) {
-    enqueue_chunk(sv, s);
+    enqueue_chunk(sv, t, s);
     HFLMP_LAUNCH_CHECK(ctx);
     return HFLMP_OK;
   }
-  if (!sv->graph_exec) {
+  if (!t.graph_exec) {
     // a graph left by a failed instantiation is freed before recapturing;
     // the capture is always ended, and a partial graph destroyed
-    drop_graph(sv);
+    drop_graph(t);
     HFLMP_HIP_CHECK(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    enqueue_chunk(sv, s);
+    enqueue_chunk(sv, t, s);
     const hipError_t launch = hipGetLastError();
     hipGraph_t g = nullptr;
     const hipError_t end = hipStreamEndCapture(s, &g);
@@ -1144,18 +1175,17 @@ int launch_group(hflmp_server *sv) {
       return set_error(ctx, HFLMP_EHIP, "server: graph capture failed (%s)",
                        hipGetErrorString(end != hipSuccess ? end : launch));
     }
-    sv->graph = g;
-    const hipError_t inst = hipGraphInstantiate(&sv->graph_exec, sv->graph, nullptr, nullptr, 0);
+    t.graph = g;
+    const hipError_t inst = hipGraphInstantiate(&t.graph_exec, t.graph, nullptr, nullptr, 0);
     if (inst != hipSuccess) {
-      sv->graph_exec = nullptr;
-      drop_graph(sv);
+      t.graph_exec = nullptr;
+      drop_graph(t);
       return set_error(ctx, HFLMP_EHIP, "server: graph instantiation failed (%s)",
                        hipGetErrorString(inst));
     }
     ++sv->graph_captures;
   }
-  sv->capture_seconds += std::chrono::duration<double>(Clock::now() - t0).count();
-  HFLMP_HIP_CHECK(ctx, hipGraphLaunch(sv->graph_exec, s));
+  HFLMP_HIP_CHECK(ctx, hipGraphLaunch(t.graph_exec, s));
   return HFLMP_OK;
 }
 
@@ -1191,28 +1221,36 @@ bool slot_running(const SlotState &st) {
 }
 
 // One step's work, in this order:
-//   launch_chunk:   the active systems' chunk (the grouped launches on the
-//                   context stream, the solo systems' on the solo stream),
-//                   then the copy of all states to the host;
-//   admit_queued:   admission of queued systems on the admission stream,
-//                   while the chunk runs (host uploads, layout kernels);
-//   retire_stopped: wait for the chunk, retire the systems that stopped;
-//   join_admitted:  wait for the admission work; the admitted systems join
-//                   the active set and run from the next chunk on.
-// Slots and blocks freed by retire_stopped are not in the next chunk's
-// tables, so the next admission may reuse them while that chunk runs.
+//   launch_chunk:  queue the active systems' next chunk (the grouped
+//                  launches on the context stream, the solo systems' on the
+//                  solo stream), then the copy of all states to the host,
+//                  behind the chunk still in flight;
+//   admit_queued:  admission of queued systems on the admission stream,
+//                  while the chunks run (host uploads, layout kernels);
+//   wait_oldest:   wait for the older chunk, retire the systems that
+//                  stopped in it;
+//   join_admitted: wait for the admission work; the admitted systems join
+//                  the active set and run from the next chunk queued.
+// A slot or block is released only when no queued chunk lists it, so an
+// admission never writes memory that a chunk in flight reads.
+// Queues chunk `launched` on its table set (rebuilt if the active set has
+// changed since the set was last built), the solo systems' chunks, the copy
+// of all states into the set's pinned buffer and the set's event.  At most
+// one chunk is in flight when this is called, and it ran on the other set.
 int launch_chunk(hflmp_server *sv, bool *launched) {
   hflmp_ctx *ctx = sv->ctx;
   hipStream_t s = ctx->stream;
   *launched = false;
   if (sv->order.empty() && sv->solo.empty()) return HFLMP_OK;
+  HFLMP_REQUIRE(ctx, sv->launched - sv->completed <= 1, "server: two chunks already in flight");
+  TableSet &t = sv->sets[sv->launched & 1];
   const auto t0 = Clock::now();
-  if (sv->set_changed) {
-    int rc = rebuild_tables(sv);
+  if (t.version != sv->version) {
+    int rc = rebuild_tables(sv, t);
     if (rc) return rc;
   }
-  if (sv->S > 0) {
-    int rc = launch_group(sv);
+  if (t.S > 0) {
+    int rc = launch_group(sv, t);
     if (rc) return rc;
   }
   sv->capture_seconds += std::chrono::duration<double>(Clock::now() - t0).count();
@@ -1230,10 +1268,15 @@ int launch_chunk(hflmp_server *sv, bool *launched) {
     HFLMP_HIP_CHECK(ctx, hipEventRecord(sv->ev_solo_done, sv->solo_stream));
     HFLMP_HIP_CHECK(ctx, hipStreamWaitEvent(s, sv->ev_solo_done, 0));
   }
-  ++sv->steps;
-  HFLMP_HIP_CHECK(ctx, hipMemcpyAsync(sv->h_states, sv->d_states,
+  for (const std::vector<int> *list : {&sv->order, &sv->solo})
+    for (int slot : *list)
+      if (sv->first_chunk[slot] < 0) sv->first_chunk[slot] = sv->launched;
+  HFLMP_HIP_CHECK(ctx, hipMemcpyAsync(t.h_states, sv->d_states,
                                       sizeof(SlotState) * sv->max_active,
                                       hipMemcpyDeviceToHost, s));
+  HFLMP_HIP_CHECK(ctx, hipEventRecord(t.done, s));
+  ++sv->launched;
+  ++sv->steps;
   *launched = true;
   return HFLMP_OK;
 }
@@ -1272,62 +1315,116 @@ int admit_queued(hflmp_server *sv) {
   return HFLMP_OK;
 }
 
-// Retires one system: x to the caller's buffer (0 if refused), its
-// results from its final state, block back to the arena, slot freed.
-int retire_one(hflmp_server *sv, int slot, const SlotState &st, int *n_done) {
+// Retires systems: x to the callers' buffers (0 if refused) and the
+// results from their final states.  The x of all of them go device ->
+// pinned staging as one queue of asynchronous copies on the result stream
+// (beside the chunk in flight, which no longer writes a stopped system's
+// x), one wait, then host copies; what does not fit the staging area is
+// copied directly.  Their slots and arena blocks are released by the
+// caller (release_slot), when no chunk in flight lists them.
+int retire_systems(hflmp_server *sv, const std::vector<int> &slots,
+                   const std::vector<const SlotState *> &states, int *n_done) {
   hflmp_ctx *ctx = sv->ctx;
-  Job &jb = sv->jobs[sv->slot_job[slot]];
-  const bool refused = (st.mode == kModeMixed ? st.status : st.s64.status) == kRefused;
-  if (refused)
-    std::memset(jb.x_out, 0, 8 * (size_t)jb.nrow);
-  else
-    HFLMP_HIP_CHECK(ctx,
-                    hipMemcpy(jb.x_out, jb.d.x, 8 * (size_t)jb.nrow, hipMemcpyDeviceToHost));
-  // zeroed again by the slot's next admission
-  unsigned int bad = 0;
-  HFLMP_HIP_CHECK(ctx, hipMemcpy(&bad, sv->d_bad + slot, sizeof(bad), hipMemcpyDeviceToHost));
-  if (st.mode == kModeMixed) {
-    jb.iters = st.inner_total;
-    jb.status = st.status;
-    jb.outer = st.outer;
-    jb.res_init = st.res0;
-    jb.res_final = st.res;
-  } else {
-    jb.iters = st.s64.iter;
-    jb.status = st.s64.status;
-    jb.outer = 0;
-    jb.res_init = st.s64.res0;
-    jb.res_final = st.s64.res;
+  const size_t none = ~(size_t)0;
+  std::vector<size_t> at(slots.size(), none);
+  size_t used = 0;
+  for (size_t i = 0; i < slots.size(); ++i) {
+    const SlotState &st = *states[i];
+    const Job &jb = sv->jobs[sv->slot_job[slots[i]]];
+    const bool refused = (st.mode == kModeMixed ? st.status : st.s64.status) == kRefused;
+    const size_t bytes = 8 * (size_t)jb.nrow;
+    if (refused || used + bytes > sv->rstage_cap) continue;
+    HFLMP_HIP_CHECK(ctx, hipMemcpyAsync(sv->rstage + used, jb.d.x, bytes,
+                                        hipMemcpyDeviceToHost, sv->result_stream));
+    at[i] = used;
+    used = align256(used + bytes);
   }
-  jb.bad_diag = (int)bad;
-  jb.col_bytes = st.col16 ? 2 : 4;
-  jb.t_done = Clock::now();
-  jb.state = kDone;
-  sv->alloc.release(jb.off, jb.kept);
-  sv->slot_job[slot] = -1;
-  ++*n_done;
+  if (used) HFLMP_HIP_CHECK(ctx, hipStreamSynchronize(sv->result_stream));
+  for (size_t i = 0; i < slots.size(); ++i) {
+    const SlotState &st = *states[i];
+    Job &jb = sv->jobs[sv->slot_job[slots[i]]];
+    const bool refused = (st.mode == kModeMixed ? st.status : st.s64.status) == kRefused;
+    const size_t bytes = 8 * (size_t)jb.nrow;
+    if (refused)
+      std::memset(jb.x_out, 0, bytes);
+    else if (at[i] != none)
+      host_copy(reinterpret_cast<char *>(jb.x_out), sv->rstage + at[i], bytes);
+    else
+      HFLMP_HIP_CHECK(ctx, hipMemcpy(jb.x_out, jb.d.x, bytes, hipMemcpyDeviceToHost));
+    if (st.mode == kModeMixed) {
+      jb.iters = st.inner_total;
+      jb.status = st.status;
+      jb.outer = st.outer;
+      jb.res_init = st.res0;
+      jb.res_final = st.res;
+    } else {
+      jb.iters = st.s64.iter;
+      jb.status = st.s64.status;
+      jb.outer = 0;
+      jb.res_init = st.s64.res0;
+      jb.res_final = st.s64.res;
+    }
+    jb.bad_diag = (int)st.bad;
+    jb.col_bytes = st.col16 ? 2 : 4;
+    jb.t_done = Clock::now();
+    jb.state = kDone;
+    ++*n_done;
+  }
   return HFLMP_OK;
 }
 
-// Waits for the chunk and retires the systems that stopped.
-int retire_stopped(hflmp_server *sv, int *n_done) {
+void release_slot(hflmp_server *sv, const Deferred &r) {
+  sv->alloc.release(r.off, r.bytes);
+  sv->slot_job[r.slot] = -1;
+  sv->first_chunk[r.slot] = -1;
+}
+
+Deferred slot_block(hflmp_server *sv, int slot) {
+  const Job &jb = sv->jobs[sv->slot_job[slot]];
+  return Deferred{slot, jb.off, jb.kept};
+}
+
+// Waits for the oldest chunk in flight (k) and retires the systems that
+// stopped in it.  Systems retired at the previous wait were still listed by
+// chunk k; now that it has finished their slots and blocks are released.
+// The systems retired now stay listed by the chunk queued after k (their
+// kernels skip them), so their release waits for it -- or happens at once
+// when no chunk is in flight.
+int wait_oldest(hflmp_server *sv, int *n_done) {
   hflmp_ctx *ctx = sv->ctx;
+  const long long k = sv->completed;
+  TableSet &t = sv->sets[k & 1];
   const auto t_wait = Clock::now();
-  HFLMP_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  HFLMP_HIP_CHECK(ctx, hipEventSynchronize(t.done));
   sv->wait_seconds += std::chrono::duration<double>(Clock::now() - t_wait).count();
+  ++sv->completed;
+  for (const Deferred &r : sv->deferred) release_slot(sv, r);
+  sv->deferred.clear();
+  std::vector<int> out;
+  std::vector<const SlotState *> out_st;
   for (std::vector<int> *list : {&sv->order, &sv->solo}) {
     std::vector<int> keep;
     for (int slot : *list) {
-      const SlotState &st = sv->h_states[slot];
-      if (slot_running(st)) {
+      // a system that joined after chunk k was queued is not in its states
+      const SlotState &st = t.h_states[slot];
+      if (sv->first_chunk[slot] < 0 || sv->first_chunk[slot] > k || slot_running(st)) {
         keep.push_back(slot);
         continue;
       }
-      const int rc = retire_one(sv, slot, st, n_done);
-      if (rc) return rc;
-      if (list == &sv->order) sv->set_changed = true;
+      out.push_back(slot);
+      out_st.push_back(&st);
+      if (list == &sv->order) ++sv->version;
     }
     list->swap(keep);
+  }
+  const auto t_ret = Clock::now();
+  const int rc = retire_systems(sv, out, out_st, n_done);
+  if (rc) return rc;
+  sv->retire_seconds += std::chrono::duration<double>(Clock::now() - t_ret).count();
+  for (int slot : out) sv->deferred.push_back(slot_block(sv, slot));
+  if (sv->launched == sv->completed) {
+    for (const Deferred &r : sv->deferred) release_slot(sv, r);
+    sv->deferred.clear();
   }
   return HFLMP_OK;
 }
@@ -1354,16 +1451,18 @@ int join_admitted(hflmp_server *sv, int *n_done) {
   for (int slot : joining) {
     const SlotState &st = sv->h_init[slot];
     if ((st.mode == kModeMixed ? st.status : st.s64.status) == kRefused) {
-      const int rc = retire_one(sv, slot, st, n_done);
+      const int rc = retire_systems(sv, {slot}, {&st}, n_done);
       if (rc) return rc;
+      release_slot(sv, slot_block(sv, slot));  // in no chunk's tables
       continue;
     }
     sv->jobs[sv->slot_job[slot]].d.col16_64 = st.col16;
+    sv->first_chunk[slot] = -1;
     if (sv->jobs[sv->slot_job[slot]].solo) {
       sv->solo.push_back(slot);
     } else {
       sv->order.push_back(slot);
-      sv->set_changed = true;
+      ++sv->version;
     }
   }
   sv->peak_active = std::max(sv->peak_active, (int)(sv->order.size() + sv->solo.size()));
@@ -1407,25 +1506,31 @@ int hflmp_server_create(hflmp_ctx *ctx, const hflmp_server_config *cfg, hflmp_se
   };
   if (hipMalloc(&sv->arena, sv->budget) != hipSuccess) return fail("arena allocation failed");
   if (hipMalloc(&sv->d_states, sizeof(SlotState) * sv->max_active) != hipSuccess ||
-      hipMalloc(&sv->d_sys, sizeof(SysDesc) * sv->max_active) != hipSuccess ||
-      hipMalloc(&sv->d_bad, sizeof(unsigned int) * sv->max_active) != hipSuccess ||
-      hipHostMalloc(&sv->h_states, sizeof(SlotState) * sv->max_active) != hipSuccess ||
       hipHostMalloc(&sv->h_init, sizeof(SlotState) * sv->max_active) != hipSuccess ||
       hipHostMalloc(&sv->h_admit, sizeof(SlotState)) != hipSuccess)
     return fail("table allocation failed");
+  for (TableSet &t : sv->sets)
+    if (hipMalloc(&t.d_sys, sizeof(SysDesc) * sv->max_active) != hipSuccess ||
+        hipHostMalloc(&t.h_sys, sizeof(SysDesc) * sv->max_active) != hipSuccess ||
+        hipHostMalloc(&t.h_states, sizeof(SlotState) * sv->max_active) != hipSuccess ||
+        hipEventCreateWithFlags(&t.done, hipEventDisableTiming) != hipSuccess)
+      return fail("table allocation failed");
   sv->stage_cap = std::min(std::max(sv->budget / 8, (size_t)16 << 20), (size_t)256 << 20);
-  if (hipHostMalloc(&sv->stage, sv->stage_cap) != hipSuccess)
+  sv->rstage_cap = sv->stage_cap;
+  if (hipHostMalloc(&sv->stage, sv->stage_cap) != hipSuccess ||
+      hipHostMalloc(&sv->rstage, sv->rstage_cap) != hipSuccess)
     return fail("staging allocation failed");
   // the admission and solo streams keep the default priority (as every
   // stream here)
   if (hipStreamCreateWithFlags(&sv->admit_stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&sv->result_stream, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&sv->solo_stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&sv->ev_admitted, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&sv->ev_solo_done, hipEventDisableTiming) != hipSuccess)
     return fail("stream or event creation failed");
-  std::memset(sv->h_states, 0, sizeof(SlotState) * sv->max_active);
   sv->alloc.reset(sv->budget);
   sv->slot_job.assign(sv->max_active, -1);
+  sv->first_chunk.assign(sv->max_active, -1);
   *out = sv;
   return HFLMP_OK;
 }
@@ -1433,22 +1538,29 @@ int hflmp_server_create(hflmp_ctx *ctx, const hflmp_server_config *cfg, hflmp_se
 void hflmp_server_destroy(hflmp_server *sv) {
   if (!sv) return;
   if (sv->admit_stream) (void)hipStreamSynchronize(sv->admit_stream);
+  if (sv->result_stream) (void)hipStreamSynchronize(sv->result_stream);
   if (sv->solo_stream) (void)hipStreamSynchronize(sv->solo_stream);
   if (sv->ctx) (void)hipStreamSynchronize(sv->ctx->stream);
   if (sv->admit_stream) (void)hipStreamDestroy(sv->admit_stream);
+  if (sv->result_stream) (void)hipStreamDestroy(sv->result_stream);
   if (sv->solo_stream) (void)hipStreamDestroy(sv->solo_stream);
   if (sv->ev_admitted) (void)hipEventDestroy(sv->ev_admitted);
   if (sv->ev_solo_done) (void)hipEventDestroy(sv->ev_solo_done);
   if (sv->h_admit) (void)hipHostFree(sv->h_admit);
-  drop_graph(sv);
+  for (TableSet &t : sv->sets) {
+    drop_graph(t);
+    if (t.d_sys) (void)hipFree(t.d_sys);
+    if (t.d_unit) (void)hipFree(t.d_unit);
+    if (t.h_sys) (void)hipHostFree(t.h_sys);
+    if (t.h_unit) (void)hipHostFree(t.h_unit);
+    if (t.h_states) (void)hipHostFree(t.h_states);
+    if (t.done) (void)hipEventDestroy(t.done);
+  }
   (void)hipFree(sv->arena);
   (void)hipFree(sv->d_states);
-  (void)hipFree(sv->d_sys);
-  (void)hipFree(sv->d_unit_sys);
-  (void)hipFree(sv->d_bad);
-  if (sv->h_states) (void)hipHostFree(sv->h_states);
   if (sv->h_init) (void)hipHostFree(sv->h_init);
   if (sv->stage) (void)hipHostFree(sv->stage);
+  if (sv->rstage) (void)hipHostFree(sv->rstage);
   delete sv;
 }
 
@@ -1517,11 +1629,15 @@ int hflmp_server_step(hflmp_server *sv, int *n_done) {
   if (!sv) return HFLMP_EINVAL;
   int done = 0;
   // A step with nothing to run admits, and runs the admitted systems at once.
+  // Two chunks in flight after the launch: wait for the older one.  One
+  // chunk in flight that this step did not queue (nothing left to run):
+  // wait for it.  One chunk just queued and none before: leave it running.
   for (int pass = 0; pass < 2; ++pass) {
     bool launched = false;
     int rc = launch_chunk(sv, &launched);
     if (!rc) rc = admit_queued(sv);
-    if (!rc && launched) rc = retire_stopped(sv, &done);
+    const long long in_flight = sv->launched - sv->completed;
+    if (!rc && (in_flight >= 2 || (!launched && in_flight == 1))) rc = wait_oldest(sv, &done);
     if (!rc) rc = join_admitted(sv, &done);
     if (rc) return rc;
     if (launched || (sv->order.empty() && sv->solo.empty())) break;
@@ -1567,7 +1683,7 @@ int hflmp_server_stats(const hflmp_server *sv, hflmp_server_stats_t *out) {
   if (!sv || !out) return HFLMP_EINVAL;
   out->queued = (int)sv->queue.size();
   out->active = (int)(sv->order.size() + sv->solo.size());
-  out->active_units = sv->U;
+  out->active_units = sv->launched > 0 ? sv->sets[(sv->launched - 1) & 1].U : 0;
   out->arena_free = sv->alloc.total_free();
   out->arena_largest_free = sv->alloc.largest();
   out->steps = sv->steps;
@@ -1579,6 +1695,7 @@ int hflmp_server_stats(const hflmp_server *sv, hflmp_server_stats_t *out) {
   out->capture_seconds = sv->capture_seconds;
   out->wait_seconds = sv->wait_seconds;
   out->join_seconds = sv->join_seconds;
+  out->retire_seconds = sv->retire_seconds;
   return HFLMP_OK;
 }
 

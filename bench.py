@@ -871,7 +871,8 @@ def run_server_stream(hflmp, ctx, feed_specs, arrivals, cfg, rtol, precision, ke
                                  "step_admit": round(stats["admit_seconds"], 3),
                                  "step_tables_and_graphs": round(stats["capture_seconds"], 3),
                                  "step_wait_for_kernels": round(stats["wait_seconds"], 3),
-                                 "step_wait_for_admission": round(stats["join_seconds"], 3)}
+                                 "step_wait_for_admission": round(stats["join_seconds"], 3),
+                                 "step_retire": round(stats["retire_seconds"], 3)}
     finally:
         sv.close()
         feed.close()

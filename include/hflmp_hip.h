@@ -890,6 +890,7 @@ typedef struct {
   double admit_seconds, capture_seconds, wait_seconds;
   long long solo_runs; /* systems started alone so far (solo_min_nnz)     */
   double join_seconds; /* waiting for admission work after the chunk     */
+  double retire_seconds; /* copying retired systems' x to the callers      */
 } hflmp_server_stats_t;
 int hflmp_server_create(hflmp_ctx *ctx, const hflmp_server_config *cfg,
                         hflmp_server **out);
