@@ -820,13 +820,15 @@ int hflmp_mpcg_jacobi_sell(hflmp_ctx *ctx, const hflmp_sell *A64,
  * system waits in a first-come-first-served queue until a slot
  * is free and its footprint (hflmp_server_footprint: CSR, SELL-64 copy,
  * inverse diagonal(s), b, x and the Krylov vectors) fits.  hflmp_server_step
- * advances every active system by chunk_iters iterations in shared
- * launches (one launch per kernel for all systems, captured as a hipGraph
- * per active set when use_graphs); while they run it admits what fits
- * (uploads and layout kernels on a second stream); then it retires the
- * systems that stopped (their x is copied to the caller's buffer), and the
- * admitted systems join, starting with the next step's chunk (at once when
- * nothing else was active).  Each system keeps its own scalars and partial sums
+ * queues the next chunk of chunk_iters iterations of every active system
+ * in shared launches (one launch per kernel for all systems, captured as a
+ * hipGraph per active set when use_graphs) behind the chunk still running;
+ * while they run it admits what fits (uploads and layout kernels on a
+ * second stream); then it waits for the older chunk and retires the
+ * systems that stopped in it (their x is copied to the caller's buffer),
+ * and the admitted systems join, starting with the next chunk queued.  So
+ * a step returns with one chunk in flight, and a system's result arrives
+ * in the step after the chunk in which it stopped.  Each system keeps its own scalars and partial sums
  * in a fixed order, so its result is bit-identical alone and in any mix.
  * Solves start from x0 = 0.  FP64 solves stop as IterateControl::Check
  * (common/iterate_control.cc:80-107) on the recursive residual; mixed
