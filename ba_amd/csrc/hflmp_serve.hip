@@ -612,7 +612,39 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+// Copies nwords 4-byte words between device memory and coherent pinned host
+// memory (both ends 16-byte aligned), 16 bytes per access where it can.
+// The per-chunk transfers on the context stream (table uploads, the state
+// copy) go through this kernel rather than the copy engine: a copy-engine
+// transfer queued behind a running chunk waits for the chunk, and every
+// transfer queued after it on the engine -- the retirement copies on the
+// result stream -- waits with it.
+__global__ void __launch_bounds__(kBlock)
+    xfer_kernel(const unsigned *__restrict__ src, unsigned *__restrict__ dst, size_t nwords) {
+  const size_t n4 = nwords / 4;
+  const size_t stride = (size_t)gridDim.x * kBlock;
+  const size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x;
+  const uint4 *s4 = reinterpret_cast<const uint4 *>(src);
+  uint4 *d4 = reinterpret_cast<uint4 *>(dst);
+  for (size_t i = t; i < n4; i += stride) d4[i] = s4[i];
+  for (size_t i = 4 * n4 + t; i < nwords; i += stride) dst[i] = src[i];
+}
+
 // ---- host side ------------------------------------------------------------------
+
+// Queues xfer_kernel for `bytes` (a multiple of 4) on stream s.
+int xfer(hflmp_ctx *ctx, void *dst, const void *src, size_t bytes, hipStream_t s) {
+  if (!bytes) return HFLMP_OK;
+  HFLMP_REQUIRE(ctx, bytes % 4 == 0 && reinterpret_cast<uintptr_t>(dst) % 16 == 0 &&
+                         reinterpret_cast<uintptr_t>(src) % 16 == 0,
+                "server: transfer of %zu bytes is not word-sized and 16-byte aligned", bytes);
+  const size_t nwords = bytes / 4;
+  const unsigned g = (unsigned)std::min<size_t>((nwords / 4 + kBlock - 1) / kBlock + 1, 1024);
+  xfer_kernel<<<g, kBlock, 0, s>>>(static_cast<const unsigned *>(src),
+                                   static_cast<unsigned *>(dst), nwords);
+  HFLMP_LAUNCH_CHECK(ctx);
+  return HFLMP_OK;
+}
 
 // First-fit allocator over the arena with coalescing of free neighbours.
 class Arena {
@@ -798,10 +830,10 @@ void plan_job(Job &jb) {
 }
 
 // One of the two table sets that chunks run on in turn: the system and
-// unit tables on the device (uploaded from pinned staging), the chunk's
-// captured graph over them, and the pinned copy of all states taken after
-// the chunk.  A set is rebuilt only when the chunk that last used it has
-// finished.
+// unit tables on the device (uploaded from coherent pinned staging), the
+// chunk's captured graph over them, and the coherent pinned copy of all
+// states taken after the chunk.  A set is rebuilt only when the chunk that
+// last used it has finished.
 struct TableSet {
   long long version = -1;  // the version of the active set these tables hold
   int S = 0, U = 0;
@@ -951,7 +983,8 @@ int rebuild_tables(hflmp_server *sv, TableSet &t) {
     t.h_unit = nullptr;
     t.unit_cap = std::max((size_t)U, 2 * t.unit_cap);
     if (hipMalloc(&t.d_unit, sizeof(int) * t.unit_cap) != hipSuccess ||
-        hipHostMalloc(&t.h_unit, sizeof(int) * t.unit_cap) != hipSuccess) {
+        hipHostMalloc(&t.h_unit, sizeof(int) * t.unit_cap, hipHostMallocCoherent) !=
+            hipSuccess) {
       t.unit_cap = 0;
       return set_error(ctx, HFLMP_ENOMEM, "server: unit table allocation failed");
     }
@@ -968,10 +1001,9 @@ int rebuild_tables(hflmp_server *sv, TableSet &t) {
   t.S = S;
   t.U = U;
   if (S) {
-    HFLMP_HIP_CHECK(ctx, hipMemcpyAsync(t.d_sys, t.h_sys, sizeof(SysDesc) * S,
-                                        hipMemcpyHostToDevice, ctx->stream));
-    HFLMP_HIP_CHECK(ctx, hipMemcpyAsync(t.d_unit, t.h_unit, sizeof(int) * U,
-                                        hipMemcpyHostToDevice, ctx->stream));
+    int rc = xfer(ctx, t.d_sys, t.h_sys, sizeof(SysDesc) * S, ctx->stream);
+    if (!rc) rc = xfer(ctx, t.d_unit, t.h_unit, sizeof(int) * U, ctx->stream);
+    if (rc) return rc;
   }
   drop_graph(t);
   t.version = sv->version;
@@ -1271,9 +1303,10 @@ int launch_chunk(hflmp_server *sv, bool *launched) {
   for (const std::vector<int> *list : {&sv->order, &sv->solo})
     for (int slot : *list)
       if (sv->first_chunk[slot] < 0) sv->first_chunk[slot] = sv->launched;
-  HFLMP_HIP_CHECK(ctx, hipMemcpyAsync(t.h_states, sv->d_states,
-                                      sizeof(SlotState) * sv->max_active,
-                                      hipMemcpyDeviceToHost, s));
+  {
+    const int rc = xfer(ctx, t.h_states, sv->d_states, sizeof(SlotState) * sv->max_active, s);
+    if (rc) return rc;
+  }
   HFLMP_HIP_CHECK(ctx, hipEventRecord(t.done, s));
   ++sv->launched;
   ++sv->steps;
@@ -1511,8 +1544,10 @@ int hflmp_server_create(hflmp_ctx *ctx, const hflmp_server_config *cfg, hflmp_se
     return fail("table allocation failed");
   for (TableSet &t : sv->sets)
     if (hipMalloc(&t.d_sys, sizeof(SysDesc) * sv->max_active) != hipSuccess ||
-        hipHostMalloc(&t.h_sys, sizeof(SysDesc) * sv->max_active) != hipSuccess ||
-        hipHostMalloc(&t.h_states, sizeof(SlotState) * sv->max_active) != hipSuccess ||
+        hipHostMalloc(&t.h_sys, sizeof(SysDesc) * sv->max_active, hipHostMallocCoherent) !=
+            hipSuccess ||
+        hipHostMalloc(&t.h_states, sizeof(SlotState) * sv->max_active,
+                      hipHostMallocCoherent) != hipSuccess ||
         hipEventCreateWithFlags(&t.done, hipEventDisableTiming) != hipSuccess)
       return fail("table allocation failed");
   sv->stage_cap = std::min(std::max(sv->budget / 8, (size_t)16 << 20), (size_t)256 << 20);
