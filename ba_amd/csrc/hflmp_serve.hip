@@ -85,6 +85,8 @@ struct SysDesc {
   int nrow, nslice, nunits, unit0;
   int mode, col16;  // mixed: FP32 columns are int16 offsets from the slice start
   int col16_64;     // FP64 layout: int16 column offsets (as the device decided at admission)
+  int grp0, gsz;    // K2's groups of gsz units (kGroup64 or kQuad): the system's first
+  int quad0;        // K3's groups of kQuad units: the system's first
   // FP64 SELL-64 (P = 2): the operator of FP64 systems and of the residuals
   const long long *soff;
   const int *slen;
@@ -191,16 +193,77 @@ __device__ __forceinline__ float *part32(const SysDesc &d) {
   return reinterpret_cast<float *>(d.part);
 }
 
-// Unit of the launch -> (system descriptor, local unit).  Units of the
-// launch are spread over the XCDs in contiguous bands.
+// Unit of the launch -> (system descriptor, local unit).  The launch covers
+// units u0 .. u0 + U - 1 of the table, spread over the XCDs in contiguous
+// bands.
 __device__ __forceinline__ bool unit_of(const SysDesc *__restrict__ sys,
                                         const int *__restrict__ unit_sys, int U, SysDesc &d,
-                                        int &j) {
+                                        int &j, int u0 = 0) {
   const long long u = xcd_tile(blockIdx.x, gridDim.x);
   if (u >= U) return false;
-  d = sys[unit_sys[u]];
-  j = (int)u - d.unit0;
+  d = sys[unit_sys[u0 + u]];
+  j = (int)(u0 + u) - d.unit0;
   return true;
+}
+
+// The vector kernels of the iteration (K2, K3) run a workgroup over a group
+// of consecutive units of one system (at most kQuad units, one row per unit
+// per thread, their loads issued together).  A workgroup's life is mostly
+// the latency of its dependent loads (table, descriptor, state, data), so
+// more units per workgroup move more bytes per life.  K3 takes kQuad units
+// for every system; K2 takes kQuad for FP32 systems and kGroup64 for FP64
+// ones: their one-unit workgroups already run the update near the
+// bandwidth that its mix of streams reaches, and four units ran it slower.
+// Group of the launch -> (system descriptor, local group).
+constexpr int kQuad = 4;
+#ifndef HFLMP_GROUP64
+#define HFLMP_GROUP64 1
+#endif
+constexpr int kGroup64 = HFLMP_GROUP64;
+static_assert(kGroup64 >= 1 && kGroup64 <= kQuad, "group size");
+__device__ __forceinline__ bool quad_of(const SysDesc *__restrict__ sys,
+                                        const int *__restrict__ grp_sys, int G, SysDesc &d,
+                                        int &k, bool quads = false) {
+  const long long q = xcd_tile(blockIdx.x, gridDim.x);
+  if (q >= G) return false;
+  d = sys[grp_sys[q]];
+  k = (int)q - (quads ? d.quad0 : d.grp0);
+  return true;
+}
+
+// Per-unit sums of a[q] and b[q] over the workgroup, stored at
+// part[2 (j0 + q)], part[2 (j0 + q) + 1] for the units j0 + q < jend: the
+// same wave sums and the same order over the waves as unit_partial2 forms
+// for one unit, so the partials do not depend on the grouping.
+template <typename T>
+__device__ __forceinline__ void quad_partial2(const T (&a)[kQuad], const T (&b)[kQuad],
+                                              T *part, int j0, int jend) {
+  __shared__ T lds[kQuad][2][kBlock / kWave];
+  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  // only the group's own units (cross-lane sums are not free)
+  const int nq = jend - j0;
+#pragma unroll
+  for (int q = 0; q < kQuad; ++q) {
+    if (q >= nq) break;
+    const T sa = wave_sum(a[q]);
+    const T sb = wave_sum(b[q]);
+    if (lane == 0) {
+      lds[q][0][wid] = sa;
+      lds[q][1][wid] = sb;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < kQuad && j0 + (int)threadIdx.x < jend) {
+    const int q = threadIdx.x;
+    T sa = 0, sb = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / kWave; ++w) {
+      sa += lds[q][0][w];
+      sb += lds[q][1][w];
+    }
+    part[2 * (j0 + q)] = sa;
+    part[2 * (j0 + q) + 1] = sb;
+  }
 }
 
 __device__ __forceinline__ bool mixed_outer_due(const SlotState *st) {
@@ -364,38 +427,48 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // ---- the iteration, both precisions -------------------------------------------
-// K1: Ap = A p, partial p.Ap per unit
+// K1: Ap = A p, partial p.Ap per unit.  One launch per form over the
+// table's units of that form (FP64 with int16 columns, FP64 with int32
+// columns, FP32: the table lists them in that order), so that each form has
+// its own register budget: eight waves per SIMD for FP32, five for FP64
+// (six spill, and run slower).  The row sums' loads go
+// out before the state is known; a stopped system's results are dropped.
+template <typename IT>
 __global__ void __launch_bounds__(kBlock)
-    grp_spmv_kernel(const SysDesc *__restrict__ sys, const int *__restrict__ unit_sys, int U) {
+    grp_spmv64_kernel(const SysDesc *__restrict__ sys, const int *__restrict__ unit_sys, int u0,
+                      int U) {
   SysDesc d;
   int j;
-  if (!unit_of(sys, unit_sys, U, d, j)) return;
+  if (!unit_of(sys, unit_sys, U, d, j, u0)) return;
   TileRow tr;
-  // the row sums' loads go out before the state is known; a stopped
-  // system's results are dropped
-  if (d.mode == kModeFp64) {
-    const int status = d.st->s64.status;
-    const double s = d.col16_64 ? op64<short>(d).row_sum(j, d.p, 0.0, tr, nullptr)
-                                : op64<int>(d).row_sum(j, d.p, 0.0, tr, nullptr);
-    if (status != kIterate) return;
-    double part = 0;
-    if (tr.has_row) {
-      d.Ap[tr.row] = s;
-      part = s * d.p[tr.row];
-    }
-    unit_partial1(part, d.part, j);
-  } else {
-    const bool live = mixed_inner_live(d.st);
-    const float s = d.col16 ? op32<short>(d).row_sum(j, d.p32, 0.0f, tr, nullptr)
-                            : op32<int>(d).row_sum(j, d.p32, 0.0f, tr, nullptr);
-    if (!live) return;
-    float part = 0;
-    if (tr.has_row) {
-      d.Ap32[tr.row] = s;
-      part = s * d.p32[tr.row];
-    }
-    unit_partial1(part, part32(d), j);
+  const int status = d.st->s64.status;
+  const double s = op64<IT>(d).row_sum(j, d.p, 0.0, tr, nullptr);
+  if (status != kIterate) return;
+  double part = 0;
+  if (tr.has_row) {
+    d.Ap[tr.row] = s;
+    part = s * d.p[tr.row];
   }
+  unit_partial1(part, d.part, j);
+}
+
+__global__ void __launch_bounds__(kBlock)
+    grp_spmv32_kernel(const SysDesc *__restrict__ sys, const int *__restrict__ unit_sys, int u0,
+                      int U) {
+  SysDesc d;
+  int j;
+  if (!unit_of(sys, unit_sys, U, d, j, u0)) return;
+  TileRow tr;
+  const bool live = mixed_inner_live(d.st);
+  const float s = d.col16 ? op32<short>(d).row_sum(j, d.p32, 0.0f, tr, nullptr)
+                          : op32<int>(d).row_sum(j, d.p32, 0.0f, tr, nullptr);
+  if (!live) return;
+  float part = 0;
+  if (tr.has_row) {
+    d.Ap32[tr.row] = s;
+    part = s * d.p32[tr.row];
+  }
+  unit_partial1(part, part32(d), j);
 }
 
 // F1: alpha = r.z / p.Ap, one workgroup per system
@@ -427,42 +500,78 @@ __global__ void __launch_bounds__(kBlock) grp_alpha_final(const SysDesc *__restr
   }
 }
 
-// K2: x += alpha p; r -= alpha Ap; z = D^-1 r; partials r.r, r.z
-// (mixed systems: c in place of x, all FP32)
+// K2: x += alpha p; r -= alpha Ap; z = D^-1 r; partials r.r, r.z per unit
+// (mixed systems: c in place of x, all FP32).  One workgroup per unit group;
+// the loads go out before the state is read.
+template <typename T>
+__device__ __forceinline__ void quad_update(long long i0, int n, T alpha, const T *__restrict__ p,
+                                            const T *__restrict__ Ap, T *__restrict__ x,
+                                            T *__restrict__ r, T *__restrict__ z,
+                                            const T (&pv)[kQuad], const T (&av)[kQuad],
+                                            const T (&xv)[kQuad], const T (&rv)[kQuad],
+                                            const T (&dv)[kQuad], T (&rr)[kQuad],
+                                            T (&rz)[kQuad]) {
+  const T malpha = -alpha;
+#pragma unroll
+  for (int q = 0; q < kQuad; ++q) {
+    const long long i = i0 + q * kBlock;
+    rr[q] = 0;
+    rz[q] = 0;
+    if (i < n) {
+      x[i] = xv[q] + alpha * pv[q];
+      const T ri = rv[q] + malpha * av[q];
+      const T zi = ri * dv[q];
+      r[i] = ri;
+      z[i] = zi;
+      rr[q] = ri * ri;
+      rz[q] = ri * zi;
+    }
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void quad_load5(long long i0, int n, const T *__restrict__ p,
+                                           const T *__restrict__ Ap, const T *__restrict__ x,
+                                           const T *__restrict__ r, const T *__restrict__ invd,
+                                           T (&pv)[kQuad], T (&av)[kQuad], T (&xv)[kQuad],
+                                           T (&rv)[kQuad], T (&dv)[kQuad]) {
+#pragma unroll
+  for (int q = 0; q < kQuad; ++q) {
+    const long long i = i0 + q * kBlock;
+    if (i < n) {
+      pv[q] = p[i];
+      av[q] = Ap[i];
+      xv[q] = x[i];
+      rv[q] = r[i];
+      dv[q] = invd[i];
+    }
+  }
+}
+
 __global__ void __launch_bounds__(kBlock)
-    grp_update_kernel(const SysDesc *__restrict__ sys, const int *__restrict__ unit_sys, int U) {
+    grp_update_kernel(const SysDesc *__restrict__ sys, const int *__restrict__ grp_sys, int G) {
   SysDesc d;
-  int j;
-  if (!unit_of(sys, unit_sys, U, d, j)) return;
-  const long long i = (long long)j * kBlock + threadIdx.x;
+  int k;
+  if (!quad_of(sys, grp_sys, G, d, k)) return;
+  // rows of the group: i0 + q kBlock < iend (q < gsz follows from the bound)
+  const int g = d.gsz;
+  const long long i0 = (long long)k * g * kBlock + threadIdx.x;
+  const int iend = (int)std::min((long long)d.nrow, (long long)(k + 1) * g * kBlock);
+  const int jend = std::min(d.nunits, g * (k + 1));
   if (d.mode == kModeFp64) {
+    double pv[kQuad], av[kQuad], xv[kQuad], rv[kQuad], dv[kQuad], rr[kQuad], rz[kQuad];
+    quad_load5(i0, iend, d.p, d.Ap, d.x, d.r, d.invd, pv, av, xv, rv, dv);
     if (d.st->s64.status != kIterate) return;
-    const double alpha = d.st->s64.alpha, malpha = -alpha;
-    double rr = 0, rz = 0;
-    if (i < d.nrow) {
-      d.x[i] += alpha * d.p[i];
-      const double ri = d.r[i] + malpha * d.Ap[i];
-      const double zi = ri * d.invd[i];
-      d.r[i] = ri;
-      d.z[i] = zi;
-      rr = ri * ri;
-      rz = ri * zi;
-    }
-    unit_partial2(rr, rz, d.part, j);
+    quad_update(i0, iend, d.st->s64.alpha, d.p, d.Ap, d.x, d.r, d.z, pv, av, xv, rv, dv, rr,
+                rz);
+    quad_partial2(rr, rz, d.part, g * k, jend);
   } else {
+    float pv[kQuad], av[kQuad], xv[kQuad], rv[kQuad], dv[kQuad], rr[kQuad], rz[kQuad];
+    quad_load5(i0, iend, d.p32, d.Ap32, d.c32, d.r32, d.invd32, pv, av, xv, rv, dv);
     if (!mixed_inner_live(d.st)) return;
-    const float alpha = d.st->s32.alpha, malpha = -alpha;
-    float rr = 0, rz = 0;
-    if (i < d.nrow) {
-      d.c32[i] += alpha * d.p32[i];
-      const float ri = d.r32[i] + malpha * d.Ap32[i];
-      const float zi = ri * d.invd32[i];
-      d.r32[i] = ri;
-      d.z32[i] = zi;
-      rr = ri * ri;
-      rz = ri * zi;
-    }
-    unit_partial2(rr, rz, part32(d), j);
+    quad_update(i0, iend, d.st->s32.alpha, d.p32, d.Ap32, d.c32, d.r32, d.z32, pv, av, xv,
+                rv, dv, rr, rz);
+    quad_partial2(rr, rz, part32(d), g * k, jend);
   }
 }
 
@@ -513,21 +622,48 @@ __global__ void __launch_bounds__(kBlock) grp_beta_final(const SysDesc *__restri
   }
 }
 
-// K3: p = beta p + z
+// K3: p = beta p + z; one workgroup per unit group, the loads before the state
+template <typename T>
+__device__ __forceinline__ void quad_load2(long long i0, int n, const T *__restrict__ p,
+                                           const T *__restrict__ z, T (&pv)[kQuad],
+                                           T (&zv)[kQuad]) {
+#pragma unroll
+  for (int q = 0; q < kQuad; ++q) {
+    const long long i = i0 + q * kBlock;
+    if (i < n) {
+      pv[q] = p[i];
+      zv[q] = z[i];
+    }
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void quad_p(long long i0, int n, T beta, T *__restrict__ p,
+                                       const T (&pv)[kQuad], const T (&zv)[kQuad]) {
+#pragma unroll
+  for (int q = 0; q < kQuad; ++q) {
+    const long long i = i0 + q * kBlock;
+    if (i < n) p[i] = beta * pv[q] + zv[q];
+  }
+}
+
 __global__ void __launch_bounds__(kBlock)
-    grp_p_kernel(const SysDesc *__restrict__ sys, const int *__restrict__ unit_sys, int U) {
+    grp_p_kernel(const SysDesc *__restrict__ sys, const int *__restrict__ quad_sys, int Q) {
   SysDesc d;
-  int j;
-  if (!unit_of(sys, unit_sys, U, d, j)) return;
-  const long long i = (long long)j * kBlock + threadIdx.x;
+  int k;
+  if (!quad_of(sys, quad_sys, Q, d, k, true)) return;
+  const long long i0 = (long long)k * kQuad * kBlock + threadIdx.x;
+  const int iend = (int)std::min((long long)d.nrow, (long long)(k + 1) * kQuad * kBlock);
   if (d.mode == kModeFp64) {
+    double pv[kQuad], zv[kQuad];
+    quad_load2(i0, iend, d.p, d.z, pv, zv);
     if (d.st->s64.status != kIterate) return;
-    const double beta = d.st->s64.beta;
-    if (i < d.nrow) d.p[i] = beta * d.p[i] + d.z[i];
+    quad_p(i0, iend, d.st->s64.beta, d.p, pv, zv);
   } else {
+    float pv[kQuad], zv[kQuad];
+    quad_load2(i0, iend, d.p32, d.z32, pv, zv);
     if (!mixed_inner_live(d.st)) return;
-    const float beta = d.st->s32.beta;
-    if (i < d.nrow) d.p32[i] = beta * d.p32[i] + d.z32[i];
+    quad_p(i0, iend, d.st->s32.beta, d.p32, pv, zv);
   }
 }
 
@@ -838,8 +974,14 @@ struct TableSet {
   long long version = -1;  // the version of the active set these tables hold
   int S = 0, U = 0;
   SysDesc *d_sys = nullptr, *h_sys = nullptr;  // [max_active]
-  int *d_unit = nullptr, *h_unit = nullptr;    // [unit_cap]
+  int *d_unit = nullptr, *h_unit = nullptr;    // [unit_cap]: FP64 systems' units first
   size_t unit_cap = 0;
+  int U16 = 0, U64 = 0;  // units of FP64 systems with int16 columns; of all FP64 systems
+  int G = 0, Q = 0;                            // K2's unit groups; K3's quads
+  int *d_grp = nullptr, *h_grp = nullptr;      // [grp_cap]: system of each group
+  size_t grp_cap = 0;
+  int *d_quad = nullptr, *h_quad = nullptr;    // [quad_cap]: system of each quad
+  size_t quad_cap = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   SlotState *h_states = nullptr;  // [max_active]
@@ -915,8 +1057,10 @@ void drop_graph(TableSet &t) {
 // The work of one chunk on the stream: starts of fresh FP64 systems, outer
 // steps of mixed systems, then `chunk` iterations.
 void enqueue_chunk(const hflmp_server *sv, const TableSet &t, hipStream_t s) {
-  const unsigned gU = (unsigned)(((t.U + kXcds - 1) / kXcds) * kXcds);
+  auto grid = [](int n) { return (unsigned)(((n + kXcds - 1) / kXcds) * kXcds); };
+  const unsigned gU = grid(t.U), gG = grid(t.G), gQ = grid(t.Q);
   const unsigned gS = (unsigned)t.S;
+  const int U32 = t.U - t.U64;
   grp_init_kernel<<<gU, kBlock, 0, s>>>(t.d_sys, t.d_unit, t.U);
   grp_init_final<<<gS, kBlock, 0, s>>>(t.d_sys);
   grp_mix_correct_kernel<<<gU, kBlock, 0, s>>>(t.d_sys, t.d_unit, t.U);
@@ -924,11 +1068,16 @@ void enqueue_chunk(const hflmp_server *sv, const TableSet &t, hipStream_t s) {
   grp_mix_residual_final<<<gS, kBlock, 0, s>>>(t.d_sys);
   grp_mix_prefresh_kernel<<<gU, kBlock, 0, s>>>(t.d_sys, t.d_unit, t.U);
   for (int it = 0; it < sv->chunk; ++it) {
-    grp_spmv_kernel<<<gU, kBlock, 0, s>>>(t.d_sys, t.d_unit, t.U);
+    const int U32i = t.U64 - t.U16;
+    if (t.U16)
+      grp_spmv64_kernel<short><<<grid(t.U16), kBlock, 0, s>>>(t.d_sys, t.d_unit, 0, t.U16);
+    if (U32i)
+      grp_spmv64_kernel<int><<<grid(U32i), kBlock, 0, s>>>(t.d_sys, t.d_unit, t.U16, U32i);
+    if (U32) grp_spmv32_kernel<<<grid(U32), kBlock, 0, s>>>(t.d_sys, t.d_unit, t.U64, U32);
     grp_alpha_final<<<gS, kBlock, 0, s>>>(t.d_sys);
-    grp_update_kernel<<<gU, kBlock, 0, s>>>(t.d_sys, t.d_unit, t.U);
+    grp_update_kernel<<<gG, kBlock, 0, s>>>(t.d_sys, t.d_grp, t.G);
     grp_beta_final<<<gS, kBlock, 0, s>>>(t.d_sys);
-    grp_p_kernel<<<gU, kBlock, 0, s>>>(t.d_sys, t.d_unit, t.U);
+    grp_p_kernel<<<gQ, kBlock, 0, s>>>(t.d_sys, t.d_quad, t.Q);
   }
 }
 
@@ -969,40 +1118,80 @@ void solo_chunk(hflmp_ctx *ctx, const Job &jb, hipStream_t s, int iters) {
 }
 
 // Builds table set t for the current active set (grouped systems in table
-// order) in its pinned staging and queues the uploads on the context
-// stream.  The chunk that last ran on t has finished, so neither its
-// staging nor its device tables are in use.
+// order; the units by SpMV form, FP64 int16, FP64 int32, FP32, for the
+// per-form SpMV launches; every system's unit groups and quads) in its pinned
+// staging and queues the uploads on the context stream.  The chunk that
+// last ran on t has finished, so neither its staging nor its device tables
+// are in use.
+int grow_table(hflmp_ctx *ctx, int **d, int **h, size_t *cap, size_t need) {
+  if (need <= *cap) return HFLMP_OK;
+  if (*d) (void)hipFree(*d);
+  if (*h) (void)hipHostFree(*h);
+  *d = nullptr;
+  *h = nullptr;
+  *cap = std::max(need, 2 * *cap);
+  if (hipMalloc(d, sizeof(int) * *cap) != hipSuccess ||
+      hipHostMalloc(h, sizeof(int) * *cap, hipHostMallocCoherent) != hipSuccess) {
+    *cap = 0;
+    return set_error(ctx, HFLMP_ENOMEM, "server: unit table allocation failed");
+  }
+  return HFLMP_OK;
+}
+
 int rebuild_tables(hflmp_server *sv, TableSet &t) {
   hflmp_ctx *ctx = sv->ctx;
-  int U = 0;
-  for (int slot : sv->order) U += sv->jobs[sv->slot_job[slot]].nunits;
-  if ((size_t)U > t.unit_cap) {
-    if (t.d_unit) (void)hipFree(t.d_unit);
-    if (t.h_unit) (void)hipHostFree(t.h_unit);
-    t.d_unit = nullptr;
-    t.h_unit = nullptr;
-    t.unit_cap = std::max((size_t)U, 2 * t.unit_cap);
-    if (hipMalloc(&t.d_unit, sizeof(int) * t.unit_cap) != hipSuccess ||
-        hipHostMalloc(&t.h_unit, sizeof(int) * t.unit_cap, hipHostMallocCoherent) !=
-            hipSuccess) {
-      t.unit_cap = 0;
-      return set_error(ctx, HFLMP_ENOMEM, "server: unit table allocation failed");
-    }
-  }
-  int S = 0, unit0 = 0;
+  int U = 0, G = 0, Q = 0;
   for (int slot : sv->order) {
     Job &jb = sv->jobs[sv->slot_job[slot]];
-    jb.d.unit0 = unit0;
-    t.h_sys[S] = jb.d;
-    for (int u = 0; u < jb.nunits; ++u) t.h_unit[unit0 + u] = S;
-    unit0 += jb.nunits;
+    jb.d.gsz = jb.d.mode == kModeFp64 ? kGroup64 : kQuad;
+    U += jb.nunits;
+    G += (jb.nunits + jb.d.gsz - 1) / jb.d.gsz;
+    Q += (jb.nunits + kQuad - 1) / kQuad;
+  }
+  int rc = grow_table(ctx, &t.d_unit, &t.h_unit, &t.unit_cap, (size_t)U);
+  if (!rc) rc = grow_table(ctx, &t.d_grp, &t.h_grp, &t.grp_cap, (size_t)G);
+  if (!rc) rc = grow_table(ctx, &t.d_quad, &t.h_quad, &t.quad_cap, (size_t)Q);
+  if (rc) return rc;
+  int S = 0, unit0 = 0, grp0 = 0, quad0 = 0;
+  for (int slot : sv->order) {
+    Job &jb = sv->jobs[sv->slot_job[slot]];
+    jb.d.grp0 = grp0;
+    const int ng = (jb.nunits + jb.d.gsz - 1) / jb.d.gsz;
+    for (int k = 0; k < ng; ++k) t.h_grp[grp0 + k] = S;
+    grp0 += ng;
+    jb.d.quad0 = quad0;
+    const int nq = (jb.nunits + kQuad - 1) / kQuad;
+    for (int k = 0; k < nq; ++k) t.h_quad[quad0 + k] = S;
+    quad0 += nq;
     ++S;
   }
+  // form of a system's SpMV: 0 FP64 int16 columns, 1 FP64 int32, 2 FP32
+  auto form = [](const SysDesc &d) { return d.mode != kModeFp64 ? 2 : d.col16_64 ? 0 : 1; };
+  for (int pass = 0; pass < 3; ++pass) {
+    S = 0;
+    for (int slot : sv->order) {
+      Job &jb = sv->jobs[sv->slot_job[slot]];
+      if (form(jb.d) == pass) {
+        jb.d.unit0 = unit0;
+        for (int u = 0; u < jb.nunits; ++u) t.h_unit[unit0 + u] = S;
+        unit0 += jb.nunits;
+      }
+      ++S;
+    }
+    if (pass == 0) t.U16 = unit0;
+    if (pass == 1) t.U64 = unit0;
+  }
+  S = 0;
+  for (int slot : sv->order) t.h_sys[S++] = sv->jobs[sv->slot_job[slot]].d;
   t.S = S;
   t.U = U;
+  t.G = G;
+  t.Q = Q;
   if (S) {
-    int rc = xfer(ctx, t.d_sys, t.h_sys, sizeof(SysDesc) * S, ctx->stream);
+    rc = xfer(ctx, t.d_sys, t.h_sys, sizeof(SysDesc) * S, ctx->stream);
     if (!rc) rc = xfer(ctx, t.d_unit, t.h_unit, sizeof(int) * U, ctx->stream);
+    if (!rc) rc = xfer(ctx, t.d_grp, t.h_grp, sizeof(int) * G, ctx->stream);
+    if (!rc) rc = xfer(ctx, t.d_quad, t.h_quad, sizeof(int) * Q, ctx->stream);
     if (rc) return rc;
   }
   drop_graph(t);
@@ -1588,6 +1777,10 @@ void hflmp_server_destroy(hflmp_server *sv) {
     if (t.d_unit) (void)hipFree(t.d_unit);
     if (t.h_sys) (void)hipHostFree(t.h_sys);
     if (t.h_unit) (void)hipHostFree(t.h_unit);
+    if (t.d_grp) (void)hipFree(t.d_grp);
+    if (t.h_grp) (void)hipHostFree(t.h_grp);
+    if (t.d_quad) (void)hipFree(t.d_quad);
+    if (t.h_quad) (void)hipHostFree(t.h_quad);
     if (t.h_states) (void)hipHostFree(t.h_states);
     if (t.done) (void)hipEventDestroy(t.done);
   }

@@ -1,20 +1,19 @@
 """Autotuner for the solve server (hflmp_server_*).
 
-Per size class (nonzeros of the system) it chooses the precision mode --
-FP64 CG or the FP32-inner / FP64-refined mode -- by timing the class's
-systems from a tuning sample through the server in both modes; the run of consecutive
-classes where the mixed mode wins that saves the most time gives the window
-[`mixed_min_nnz`, `mixed_max_nnz`), kept only if the whole sample, solved
-with the window, beats the sample in FP64.
-Nonzeros rather than rows separate the classes because the mixed mode's
-gain follows the bytes per iteration: a 3D Q1 row (27 entries) gains more
-than a 2D one (9 entries), and large 3D systems need few iterations, so
-the restarts of the inner solves cost little.  Then it
-chooses the iterations per step (the step is the unit of admission,
-retirement and graph replay) among candidates by the sample's throughput.
+It chooses the precision window [`mixed_min_nnz`, `mixed_max_nnz`) -- the
+systems solved by the FP32-inner / FP64-refined mode rather than FP64 CG --
+by timing the whole tuning sample through the server under each candidate
+window, and then the iterations per step (the step is the unit of
+admission, retirement and graph replay) among candidates by the sample's
+throughput.  The sample must be large enough to load the server: the mixed
+mode's gain is a bandwidth gain (fewer bytes per iteration), which a few
+systems at a time do not show, since their launches' latency hides it.
+Candidate windows start at size-class boundaries (nonzeros of the system,
+which the bytes per iteration follow) and have no upper bound: the larger
+the system, the more of its time is bandwidth.
 
-The tuning sample is solved for real (same tolerance), so a mode that does
-not converge for a class is never chosen for it.
+The tuning sample is solved for real (same tolerance), so a window under
+which a system does not converge is never chosen.
 """
 import time
 
@@ -54,28 +53,27 @@ def _run(ctx, reqs, cfg, precision, rtol):
         sv.close()
 
 
-MARGIN = 0.05  # a candidate replaces the default only when this much faster
-# The precision window must beat FP64 on the whole sample by more: it adds
-# per-system outer steps to launches shared with every other system, and on
-# a sample much smaller than the stream a 5% gain has not carried over (a
-# 5.4% gain on a 32-request sample was a 10% loss on a 200-request stream
-# of the same mix).
+MARGIN = 0.05  # a chunk length replaces the default only when this much faster
+# A window replaces FP64 only when the whole sample gains more than this: it
+# adds per-system outer steps to launches shared with every other system.
 WINDOW_MARGIN = 0.10
+# candidate windows (lo, hi): FP64 only, then mixed from each class boundary
+WINDOWS = ((0, 0),) + tuple((max(b, 1), 0) for b in reversed(CLASS_BOUNDS[:-1]))
 
 
-def mixed_window(wins, gain):
-    """(lo, hi): the run of consecutive size classes lo..hi, all with
-    wins[k], whose summed gain[k] (seconds saved by the mixed mode) is
-    largest; None when no class wins."""
-    best, best_gain = None, 0.0
-    for lo in sorted(wins):
-        total, k = 0.0, lo
-        while wins.get(k):
-            total += gain[k]
-            if best is None or total > best_gain:
-                best, best_gain = (lo, k), total
-            k += 1
-    return best
+def choose_window(times, margin=WINDOW_MARGIN):
+    """The window with the shortest time among those whose sample
+    converged (times: window -> seconds, None where a system failed), if it
+    beats FP64 (window (0, 0)) by more than `margin` or FP64 failed;
+    otherwise FP64."""
+    ok = {w: t for w, t in times.items() if t is not None}
+    if not ok:
+        return (0, 0)
+    best = min(ok, key=ok.get)
+    t64 = ok.get((0, 0))
+    if t64 is None or ok[best] < (1 - margin) * t64:
+        return best
+    return (0, 0)
 
 
 def _best(ctx, reqs, cfg, precision, rtol):
@@ -84,56 +82,31 @@ def _best(ctx, reqs, cfg, precision, rtol):
     return min(t1, t2), ok1 and ok2
 
 
-def tune(ctx, sample, base_cfg, rtol=1e-8, chunks=(8, 16, 32)):
-    """sample: list of (rp, ci, val, b).  Returns (config, report).  Every
-    timing is the better of two runs, and a choice departs from the default
-    (FP64, base_cfg's chunk_iters) only on a gain of more than MARGIN."""
+def tune(ctx, sample, base_cfg, rtol=1e-8, chunks=(8, 16, 32), windows=WINDOWS):
+    """sample: list of (rp, ci, val, b), submitted at once (enough systems
+    to load the server).  Returns (config, report).  Every timing is the
+    better of two runs; the window departs from FP64 on a gain of more than
+    WINDOW_MARGIN, the chunk length from base_cfg's on more than MARGIN."""
     cfg = dict(base_cfg)
-    classes = {}
-    for r in sample:
-        classes.setdefault(size_class(len(r[1])), []).append(r)
-    report = {"classes": {}}
-    mixed_wins = {}
-    for k in sorted(classes):
-        reqs = classes[k]
-        _run(ctx, reqs[:1], cfg, 64, rtol)  # warm the kernels
-        t64, ok64 = _best(ctx, reqs, cfg, 64, rtol)
-        t32, ok32 = _best(ctx, reqs, cfg, 32, rtol)
-        mixed_wins[k] = ok32 and (t32 < (1 - MARGIN) * t64 or not ok64)
-        report["classes"][k] = {"nnz": [CLASS_BOUNDS[k], CLASS_BOUNDS[k + 1]],
-                                "systems": len(reqs), "fp64_s": round(t64, 4),
-                                "mixed_s": round(t32, 4),
-                                "choice": "mixed" if mixed_wins[k] else "fp64"}
-    lo_hi = mixed_window(mixed_wins, {k: report["classes"][k]["fp64_s"] -
-                                      report["classes"][k]["mixed_s"] for k in classes})
-    if lo_hi is None:
-        cfg["mixed_min_nnz"], cfg["mixed_max_nnz"] = 0, 0
-    else:
-        lo, hi = lo_hi
-        cfg["mixed_min_nnz"] = max(CLASS_BOUNDS[lo], 1)
-        cfg["mixed_max_nnz"] = 0 if hi == len(CLASS_BOUNDS) - 2 else CLASS_BOUNDS[hi + 1]
-        # A class timed alone can favour the mixed mode while the mix does
-        # not (its outer steps add work to steps shared with every other
-        # system): keep the window only if the whole sample gains too.
-        fp64_cfg = dict(cfg, mixed_min_nnz=0, mixed_max_nnz=0)
-        t_win, ok_win = _best(ctx, sample, cfg, 0, rtol)
-        t_64, ok_64 = _best(ctx, sample, fp64_cfg, 0, rtol)
-        keep = ok_win and (t_win < (1 - WINDOW_MARGIN) * t_64 or not ok_64)
-        report["window_check"] = {"with_window_s": round(t_win, 4), "fp64_s": round(t_64, 4),
-                                  "kept": bool(keep)}
-        if not keep:
-            cfg["mixed_min_nnz"], cfg["mixed_max_nnz"] = 0, 0
-    report["chunks"] = {}
+    _run(ctx, sample[:1], cfg, 64, rtol)  # warm the kernels
+    report = {"sample_systems": len(sample), "windows": {}}
     times = {}
+    for lo, hi in windows:
+        t, ok = _best(ctx, sample, dict(cfg, mixed_min_nnz=lo, mixed_max_nnz=hi), 0, rtol)
+        report["windows"]["[%d, %d)" % (lo, hi)] = {"seconds": round(t, 4), "converged": ok}
+        times[(lo, hi)] = t if ok else None
+    cfg["mixed_min_nnz"], cfg["mixed_max_nnz"] = choose_window(times)
+    report["chunks"] = {}
+    ctimes = {}
     for c in sorted(set(chunks) | {cfg["chunk_iters"]}):
         t, ok = _best(ctx, sample, dict(cfg, chunk_iters=c), 0, rtol)
         report["chunks"][c] = round(t, 4)
         if ok:
-            times[c] = t
-    if times:
-        c_best = min(times, key=times.get)
-        t_def = times.get(cfg["chunk_iters"])
-        if t_def is None or times[c_best] < (1 - MARGIN) * t_def:
+            ctimes[c] = t
+    if ctimes:
+        c_best = min(ctimes, key=ctimes.get)
+        t_def = ctimes.get(cfg["chunk_iters"])
+        if t_def is None or ctimes[c_best] < (1 - MARGIN) * t_def:
             cfg["chunk_iters"] = c_best
     report["mixed_min_nnz"] = cfg["mixed_min_nnz"]
     report["mixed_max_nnz"] = cfg["mixed_max_nnz"]

@@ -907,7 +907,7 @@ def run_singles(hflmp, ctx, specs, rtol):
     return service, out, solve_s, t_all, reqs
 
 
-def bench_serve(hflmp, nreq, seed, rate, ref_style, graphs=True, ntune=32, sample_every=20,
+def bench_serve(hflmp, nreq, seed, rate, ref_style, graphs=True, ntune=160, sample_every=20,
                 ref_every=100, load=0.7):
     """C5: a stream of `nreq` FE solves (Jacobi-CG to 1e-8) with Poisson
     arrivals through the solve server.  Requests are generated on host
@@ -929,7 +929,11 @@ def bench_serve(hflmp, nreq, seed, rate, ref_style, graphs=True, ntune=32, sampl
     specs, gaps = serve_stream(nreq, seed)
     tspecs, _ = serve_stream(ntune, seed + 1)
     ctx = hflmp.Context(0)
-    treqs = [make_request(hflmp, s) for s in tspecs]
+    # generated on the stream's 8 host threads; held while the tuner runs
+    # (160 requests of the mix: about 4.3 GB)
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(8) as ex:
+        treqs = list(ex.map(lambda sp: make_request(hflmp, sp), tspecs))
     base = {"hbm_budget": 32 << 30, "max_active": 256, "chunk_iters": 16, "use_graphs": graphs,
             "mixed_min_nnz": 0, "inner_rtol": 1e-2, "mixed_max_nnz": 0}
     t0 = time.perf_counter()
@@ -962,6 +966,10 @@ def bench_serve(hflmp, nreq, seed, rate, ref_style, graphs=True, ntune=32, sampl
            for k, (xsgl, _) in zip(sample, singles)]
     itd = [abs(infos[k]["iters"] - si["iters"]) for k, (_, si) in zip(sample, singles)
            if infos[k]["precision"] == 64]
+    # FP64 requests follow the single solve's arithmetic; mixed ones meet
+    # the same tolerance on the true residual by another path
+    e64 = [e for e, k in zip(err, sample) if infos[k]["precision"] == 64]
+    e32 = [e for e, k in zip(err, sample) if infos[k]["precision"] == 32]
     true_res = [float(np.linalg.norm(b - O.spmv(rp, ci, v, xs[k])) /
                       max(np.linalg.norm(b), 1e-300))
                 for k, (rp, ci, v, b) in zip(sample, sreqs)]
@@ -973,7 +981,8 @@ def bench_serve(hflmp, nreq, seed, rate, ref_style, graphs=True, ntune=32, sampl
                              "[16,96]; log-normal diffusivity sigma 1; uniform RHS; "
                              "generated on 8 host threads while the server runs",
                       "arrivals": "Poisson (exponential gaps, seed %d)" % seed},
-           "autotune": {"sample": "%d requests of the same mix, seed %d" % (ntune, seed + 1),
+           "autotune": {"sample": "%d requests of the same mix, seed %d, submitted at once"
+                                  % (ntune, seed + 1),
                         "seconds": round(t_tune, 2), "report": tune_report},
            "server": {"config": cfg, "solves_per_s": round(capacity, 2),
                       "makespan_s": round(makespan, 3), "steps": stats["steps"],
@@ -1009,9 +1018,12 @@ def bench_serve(hflmp, nreq, seed, rate, ref_style, graphs=True, ntune=32, sampl
                                         round(fp64_bytes / t_single / 1e9, 1),
                                     "all_converged": all(s["status"] == 0
                                                          for _, s in singles)},
-           "parity_on_sample": {"max_rel_diff_x_vs_single": max(err),
-                                "max_iter_diff_fp64": max(itd) if itd else None,
-                                "max_true_rel_residual": max(true_res)}}
+           "parity_on_sample": {
+               "fp64_requests": len(e64), "mixed_requests": len(e32),
+               "max_rel_diff_x_vs_single_fp64": max(e64) if e64 else None,
+               "max_rel_diff_x_vs_single_mixed": max(e32) if e32 else None,
+               "max_iter_diff_fp64": max(itd) if itd else None,
+               "max_true_rel_residual": max(true_res)}}
     res["speedup_vs_one_at_a_time_device"] = round(capacity / single_rate, 3)
     if ref_style:
         rsample = list(range(0, nreq, ref_every))

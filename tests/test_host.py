@@ -85,15 +85,19 @@ def test_bench_refuses_multi_gpu():
     assert r.returncode != 0 and "single" in r.stderr
 
 
-def test_serve_tune_mixed_window():
+def test_serve_tune_choose_window():
     from ba_amd import serve_tune as T
-    # the larger of two winning runs is chosen; a losing class splits runs
-    wins = {0: True, 1: False, 2: True, 3: True, 4: False}
-    gain = {0: 0.5, 1: -0.1, 2: 0.2, 3: 0.4, 4: -0.3}
-    assert T.mixed_window(wins, gain) == (2, 3)
-    # a winning class with a small gain at the edge still joins the run
-    assert T.mixed_window({1: True, 2: True}, {1: 0.01, 2: 0.3}) == (1, 2)
-    assert T.mixed_window({0: False, 1: False}, {0: -1, 1: -1}) is None
-    # a class missing from the sample ends a run
-    assert T.mixed_window({0: True, 2: True}, {0: 0.1, 2: 0.3}) == (2, 2)
+    w64, big, all_ = (0, 0), (1 << 20, 0), (1, 0)
+    # the fastest window, when it beats FP64 by more than the margin
+    assert T.choose_window({w64: 10.0, big: 7.9, all_: 7.5}) == all_
+    # a gain within the margin keeps FP64
+    assert T.choose_window({w64: 10.0, big: 9.5, all_: 9.2}) == w64
+    # a window whose sample did not converge is never chosen
+    assert T.choose_window({w64: 10.0, big: None, all_: 8.5}) == all_
+    assert T.choose_window({w64: 10.0, all_: None}) == w64
+    # FP64 failing: the best converged window, whatever the margin
+    assert T.choose_window({w64: None, big: 9.9}) == big
+    assert T.choose_window({w64: None, big: None}) == w64
+    assert T.WINDOWS[0] == w64 and set(T.WINDOWS[1:]) == {
+        (1, 0), (1 << 16, 0), (1 << 20, 0), (8 << 20, 0)}
     assert [T.size_class(n) for n in (0, 65535, 65536, 1 << 20, 9 << 20)] == [0, 0, 1, 2, 3]
