@@ -97,6 +97,83 @@ struct CsrTileOp {
   }
 };
 
+// One lane's row of a SELL-64 slice whose entries start at col / val (the
+// slice's first entry): s + sum of a_ij x_j over the row's len entries in
+// ascending storage order (SUB: s - a_ij x_j; NODIAG: the entry whose
+// column is dcol skipped).  smax: the slice's entries per lane.  The loads
+// cover the row's own entries, or with LOAD_ALL every entry position of the
+// slice (all allocated: 64 x smax entries), so that they need not wait for
+// len; the sum is the same either way.
+template <typename T, typename IT, bool SUB, bool NODIAG, bool LOAD_ALL>
+__device__ __forceinline__ T sell_slice_sum(const IT *__restrict__ col, const T *__restrict__ val,
+                                            int smax, int len, int lane,
+                                            const T *__restrict__ x, int dcol, T s) {
+#pragma clang fp contract(off)
+  constexpr int P = SellChunk<T>::P;
+  constexpr int U = SellChunk<T>::U;
+  typedef typename VecOf<T, P>::type TV;
+  typedef typename VecOf<IT, P>::type IV;
+  const int full = smax / P * P;  // entries in P-chunks; the rest one per block
+  const IV *__restrict__ c = reinterpret_cast<const IV *>(col) + lane;
+  const TV *__restrict__ v = reinterpret_cast<const TV *>(val) + lane;
+  // the row's entries past the last full chunk (at most P - 1), loaded
+  // first so their latency overlaps the chunks'
+  constexpr int TM = P > 1 ? P - 1 : 1;
+  const int ntail = len - full;
+  const int nload = LOAD_ALL ? smax - full : ntail;
+  IT ct[TM];
+  T vt[TM];
+#pragma unroll
+  for (int t = 0; t < TM; ++t) {
+    if (t < nload) {
+      const long long q = (long long)kSellRows * (full + t) + lane;
+      ct[t] = __builtin_nontemporal_load(col + q);
+      vt[t] = __builtin_nontemporal_load(val + q);
+    }
+  }
+  for (int j = 0; j < full; j += P * U) {
+    IV cc[U];
+    TV vv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if ((LOAD_ALL || j + u * P < len) && j + u * P < full) {
+        cc[u] = __builtin_nontemporal_load(c + (j / P + u) * kWave);
+        vv[u] = __builtin_nontemporal_load(v + (j / P + u) * kWave);
+      }
+    }
+    T xx[U][P];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int q = 0; q < P; ++q)
+        if (j + u * P + q < len && j + u * P < full) xx[u][q] = x[(int)cc[u][q]];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int q = 0; q < P; ++q)
+        if (j + u * P + q < len && j + u * P < full && (!NODIAG || (int)cc[u][q] != dcol)) {
+          if (SUB)
+            s -= vv[u][q] * xx[u][q];
+          else
+            s += vv[u][q] * xx[u][q];
+        }
+  }
+  // the tail, in ascending order after the chunks
+  T xt[TM];
+#pragma unroll
+  for (int t = 0; t < TM; ++t)
+    if (t < ntail) xt[t] = x[(int)ct[t]];
+#pragma unroll
+  for (int t = 0; t < TM; ++t)
+    if (t < ntail && (!NODIAG || (int)ct[t] != dcol)) {
+      if (SUB)
+        s -= vt[t] * xt[t];
+      else
+        s += vt[t] * xt[t];
+    }
+  return s;
+}
+
 // SELL-64: unit = four slices (256 consecutive rows), wave w of the
 // workgroup runs slice 4 unit + w.  No LDS and no barrier.
 template <typename T, typename IT = int>
@@ -120,9 +197,6 @@ struct SellOp {
   template <bool SUB = false, bool NODIAG = false>
   __device__ __forceinline__ T row_sum(long long unit, const T *x_in, T s, TileRow &tr,
                                        T *) const {
-#pragma clang fp contract(off)
-    typedef typename VecOf<T, P>::type TV;
-    typedef typename VecOf<IT, P>::type IV;
     tr.has_row = false;
     tr.row = 0;
     const long long sl = unit * kSellSlicesPerBlock + (threadIdx.x >> 6);
@@ -135,68 +209,11 @@ struct SellOp {
     const int len = tr.has_row ? (int)rlen[row] : 0;
     const long long base = soff[sid];
     const int smax = slen[sid];
-    const int full = smax / P * P;  // entries in P-chunks; the rest one per block
-    const IV *__restrict__ c = reinterpret_cast<const IV *>(col + base) + lane;
-    const TV *__restrict__ v = reinterpret_cast<const TV *>(val + base) + lane;
     // int16 columns are offsets from the slice's first row
     const T *__restrict__ x = std::is_same<IT, int>::value ? x_in : x_in + sid * kSellRows;
     const int dcol = std::is_same<IT, int>::value ? row : lane;
-    // the row's entries past the last full chunk (at most P - 1), loaded
-    // first so their latency overlaps the chunks'
-    constexpr int TM = P > 1 ? P - 1 : 1;
-    const int ntail = len - full;
-    IT ct[TM];
-    T vt[TM];
-#pragma unroll
-    for (int t = 0; t < TM; ++t) {
-      if (t < ntail) {
-        const long long q = base + (long long)kSellRows * (full + t) + lane;
-        ct[t] = __builtin_nontemporal_load(col + q);
-        vt[t] = __builtin_nontemporal_load(val + q);
-      }
-    }
-    for (int j = 0; j < full; j += P * U) {
-      IV cc[U];
-      TV vv[U];
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        if (j + u * P < len && j + u * P < full) {
-          cc[u] = __builtin_nontemporal_load(c + (j / P + u) * kWave);
-          vv[u] = __builtin_nontemporal_load(v + (j / P + u) * kWave);
-        }
-      }
-      T xx[U][P];
-#pragma unroll
-      for (int u = 0; u < U; ++u)
-#pragma unroll
-        for (int q = 0; q < P; ++q)
-          if (j + u * P + q < len && j + u * P < full) xx[u][q] = x[(int)cc[u][q]];
-#pragma unroll
-      for (int u = 0; u < U; ++u)
-#pragma unroll
-        for (int q = 0; q < P; ++q)
-          if (j + u * P + q < len && j + u * P < full &&
-              (!NODIAG || (int)cc[u][q] != dcol)) {
-            if (SUB)
-              s -= vv[u][q] * xx[u][q];
-            else
-              s += vv[u][q] * xx[u][q];
-          }
-    }
-    // the tail, in ascending order after the chunks
-    T xt[TM];
-#pragma unroll
-    for (int t = 0; t < TM; ++t)
-      if (t < ntail) xt[t] = x[(int)ct[t]];
-#pragma unroll
-    for (int t = 0; t < TM; ++t)
-      if (t < ntail && (!NODIAG || (int)ct[t] != dcol)) {
-        if (SUB)
-          s -= vt[t] * xt[t];
-        else
-          s += vt[t] * xt[t];
-      }
-    return s;
+    return sell_slice_sum<T, IT, SUB, NODIAG, false>(col + base, val + base, smax, len, lane, x,
+                                                      dcol, s);
   }
 };
 

@@ -427,48 +427,98 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // ---- the iteration, both precisions -------------------------------------------
-// K1: Ap = A p, partial p.Ap per unit.  One launch per form over the
-// table's units of that form (FP64 with int16 columns, FP64 with int32
-// columns, FP32: the table lists them in that order), so that each form has
-// its own register budget: eight waves per SIMD for FP32, five for FP64
-// (six spill, and run slower).  The row sums' loads go
-// out before the state is known; a stopped system's results are dropped.
-template <typename IT>
-__global__ void __launch_bounds__(kBlock)
-    grp_spmv64_kernel(const SysDesc *__restrict__ sys, const int *__restrict__ unit_sys, int u0,
-                      int U) {
-  SysDesc d;
-  int j;
-  if (!unit_of(sys, unit_sys, U, d, j, u0)) return;
-  TileRow tr;
-  const int status = d.st->s64.status;
-  const double s = op64<IT>(d).row_sum(j, d.p, 0.0, tr, nullptr);
-  if (status != kIterate) return;
-  double part = 0;
-  if (tr.has_row) {
-    d.Ap[tr.row] = s;
-    part = s * d.p[tr.row];
-  }
-  unit_partial1(part, d.part, j);
+// ---- K1: Ap = A p, partial p.Ap per unit --------------------------------------
+// A grouped-SpMV unit's slices, gathered on the device when the tables are
+// built, so that the SpMV's waves reach their entries one dependent load
+// after the launch (the record) instead of three (unit table, system
+// descriptor, slice offsets).
+struct UnitRec {
+  const void *col, *val;  // the unit's first slice in its form's column / value arrays
+  const void *p;          // the system's search direction (p, or p32 of a mixed system)
+  void *Ap;
+  const unsigned char *rlen;
+  void *part;             // the system's partials
+  const SlotState *st;
+  int row0, nrow, j, pad;
+  int2 sl[kSellSlicesPerBlock];  // slice w: entry offset from slice 0, entries per lane
+};
+
+// The SpMV forms, in the order the unit table lists their units.
+constexpr int kForms = 4;  // FP64 int16 columns, FP64 int32, FP32 int16, FP32 int32
+__host__ __device__ inline int spmv_form(const SysDesc &d) {
+  return d.mode == kModeFp64 ? (d.col16_64 ? 0 : 1) : (d.col16 ? 2 : 3);
 }
 
 __global__ void __launch_bounds__(kBlock)
-    grp_spmv32_kernel(const SysDesc *__restrict__ sys, const int *__restrict__ unit_sys, int u0,
-                      int U) {
-  SysDesc d;
-  int j;
-  if (!unit_of(sys, unit_sys, U, d, j, u0)) return;
-  TileRow tr;
-  const bool live = mixed_inner_live(d.st);
-  const float s = d.col16 ? op32<short>(d).row_sum(j, d.p32, 0.0f, tr, nullptr)
-                          : op32<int>(d).row_sum(j, d.p32, 0.0f, tr, nullptr);
-  if (!live) return;
-  float part = 0;
-  if (tr.has_row) {
-    d.Ap32[tr.row] = s;
-    part = s * d.p32[tr.row];
+    grp_unitrec_kernel(const SysDesc *__restrict__ sys, const int *__restrict__ unit_sys, int U,
+                       UnitRec *__restrict__ rec) {
+  for (int u = blockIdx.x * kBlock + threadIdx.x; u < U; u += gridDim.x * kBlock) {
+    const SysDesc &d = sys[unit_sys[u]];
+    const bool f64 = d.mode == kModeFp64;
+    const long long *soff = f64 ? d.soff : d.soff32;
+    const int *slen = f64 ? d.slen : d.slen32;
+    const int form = spmv_form(d);
+    const long long csz = (form == 0 || form == 2) ? 2 : 4;
+    const int j = u - d.unit0;
+    const int s0 = j * kSellSlicesPerBlock;  // < nslice: nunits = ceil(nslice / 4)
+    const long long base = soff[s0];
+    UnitRec r;
+    r.col = static_cast<const char *>(f64 ? d.col : d.col32) + base * csz;
+    r.val = f64 ? static_cast<const void *>(d.sval + base)
+                : static_cast<const void *>(d.sval32 + base);
+    r.p = f64 ? static_cast<const void *>(d.p) : static_cast<const void *>(d.p32);
+    r.Ap = f64 ? static_cast<void *>(d.Ap) : static_cast<void *>(d.Ap32);
+    r.rlen = d.rlen;
+    r.part = d.part;
+    r.st = d.st;
+    r.row0 = s0 * kSellRows;
+    r.nrow = d.nrow;
+    r.j = j;
+    r.pad = 0;
+#pragma unroll
+    for (int w = 0; w < kSellSlicesPerBlock; ++w) {
+      const int sid = s0 + w;
+      r.sl[w] = sid < d.nslice ? make_int2((int)(soff[sid] - base), slen[sid]) : make_int2(0, 0);
+    }
+    rec[u] = r;
   }
-  unit_partial1(part, part32(d), j);
+}
+
+// One launch per form over the table's units of that form, so that each
+// has its own register budget.  The loads of the state, the row's length,
+// p at the row and the slice's entries go out together; the sums are those
+// of SellOp::row_sum (same order, same predicates), and a stopped system's
+// results are dropped.
+template <typename T, typename IT>
+__global__ void __launch_bounds__(kBlock)
+    grp_spmv_kernel(const UnitRec *__restrict__ rec, int u0, int U) {
+  const long long u = xcd_tile(blockIdx.x, gridDim.x);
+  if (u >= U) return;
+  const UnitRec &r = rec[u0 + u];
+  const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int lane = threadIdx.x & (kWave - 1);
+  const int srow = r.row0 + w * kSellRows;  // the slice's first row
+  const int row = srow + lane;
+  const bool has_row = row < r.nrow;
+  const T *__restrict__ p = static_cast<const T *>(r.p);
+  const int2 sl = r.sl[w];
+  const bool live = std::is_same<T, double>::value ? r.st->s64.status == kIterate
+                                                   : mixed_inner_live(r.st);
+  const int len = has_row ? (int)r.rlen[row] : 0;
+  const T prow = has_row ? p[row] : T(0);
+  // int16 columns are offsets from the slice's first row
+  const T *__restrict__ x = std::is_same<IT, int>::value ? p : p + srow;
+  const int dcol = std::is_same<IT, int>::value ? row : lane;
+  const T s = sell_slice_sum<T, IT, false, false, true>(
+      static_cast<const IT *>(r.col) + sl.x, static_cast<const T *>(r.val) + sl.x, sl.y, len,
+      lane, x, dcol, T(0));
+  if (!live) return;
+  T part = 0;
+  if (has_row) {
+    static_cast<T *>(r.Ap)[row] = s;
+    part = s * prow;
+  }
+  unit_partial1(part, static_cast<T *>(r.part), r.j);
 }
 
 // F1: alpha = r.z / p.Ap, one workgroup per system
@@ -976,7 +1026,9 @@ struct TableSet {
   SysDesc *d_sys = nullptr, *h_sys = nullptr;  // [max_active]
   int *d_unit = nullptr, *h_unit = nullptr;    // [unit_cap]: FP64 systems' units first
   size_t unit_cap = 0;
-  int U16 = 0, U64 = 0;  // units of FP64 systems with int16 columns; of all FP64 systems
+  int ubeg[kForms + 1] = {};  // the units of SpMV form f: ubeg[f] .. ubeg[f + 1] - 1
+  UnitRec *d_rec = nullptr;   // [rec_cap]: every unit's record, built on the device
+  size_t rec_cap = 0;
   int G = 0, Q = 0;                            // K2's unit groups; K3's quads
   int *d_grp = nullptr, *h_grp = nullptr;      // [grp_cap]: system of each group
   size_t grp_cap = 0;
@@ -1060,7 +1112,7 @@ void enqueue_chunk(const hflmp_server *sv, const TableSet &t, hipStream_t s) {
   auto grid = [](int n) { return (unsigned)(((n + kXcds - 1) / kXcds) * kXcds); };
   const unsigned gU = grid(t.U), gG = grid(t.G), gQ = grid(t.Q);
   const unsigned gS = (unsigned)t.S;
-  const int U32 = t.U - t.U64;
+  const int *ub = t.ubeg;
   grp_init_kernel<<<gU, kBlock, 0, s>>>(t.d_sys, t.d_unit, t.U);
   grp_init_final<<<gS, kBlock, 0, s>>>(t.d_sys);
   grp_mix_correct_kernel<<<gU, kBlock, 0, s>>>(t.d_sys, t.d_unit, t.U);
@@ -1068,12 +1120,18 @@ void enqueue_chunk(const hflmp_server *sv, const TableSet &t, hipStream_t s) {
   grp_mix_residual_final<<<gS, kBlock, 0, s>>>(t.d_sys);
   grp_mix_prefresh_kernel<<<gU, kBlock, 0, s>>>(t.d_sys, t.d_unit, t.U);
   for (int it = 0; it < sv->chunk; ++it) {
-    const int U32i = t.U64 - t.U16;
-    if (t.U16)
-      grp_spmv64_kernel<short><<<grid(t.U16), kBlock, 0, s>>>(t.d_sys, t.d_unit, 0, t.U16);
-    if (U32i)
-      grp_spmv64_kernel<int><<<grid(U32i), kBlock, 0, s>>>(t.d_sys, t.d_unit, t.U16, U32i);
-    if (U32) grp_spmv32_kernel<<<grid(U32), kBlock, 0, s>>>(t.d_sys, t.d_unit, t.U64, U32);
+    if (ub[1] > ub[0])
+      grp_spmv_kernel<double, short><<<grid(ub[1] - ub[0]), kBlock, 0, s>>>(t.d_rec, ub[0],
+                                                                              ub[1] - ub[0]);
+    if (ub[2] > ub[1])
+      grp_spmv_kernel<double, int><<<grid(ub[2] - ub[1]), kBlock, 0, s>>>(t.d_rec, ub[1],
+                                                                            ub[2] - ub[1]);
+    if (ub[3] > ub[2])
+      grp_spmv_kernel<float, short><<<grid(ub[3] - ub[2]), kBlock, 0, s>>>(t.d_rec, ub[2],
+                                                                             ub[3] - ub[2]);
+    if (ub[4] > ub[3])
+      grp_spmv_kernel<float, int><<<grid(ub[4] - ub[3]), kBlock, 0, s>>>(t.d_rec, ub[3],
+                                                                           ub[4] - ub[3]);
     grp_alpha_final<<<gS, kBlock, 0, s>>>(t.d_sys);
     grp_update_kernel<<<gG, kBlock, 0, s>>>(t.d_sys, t.d_grp, t.G);
     grp_beta_final<<<gS, kBlock, 0, s>>>(t.d_sys);
@@ -1118,8 +1176,8 @@ void solo_chunk(hflmp_ctx *ctx, const Job &jb, hipStream_t s, int iters) {
 }
 
 // Builds table set t for the current active set (grouped systems in table
-// order; the units by SpMV form, FP64 int16, FP64 int32, FP32, for the
-// per-form SpMV launches; every system's unit groups and quads) in its pinned
+// order; the units by SpMV form, for the per-form SpMV launches, and their
+// records; every system's unit groups and quads) in its pinned
 // staging and queues the uploads on the context stream.  The chunk that
 // last ran on t has finished, so neither its staging nor its device tables
 // are in use.
@@ -1148,6 +1206,15 @@ int rebuild_tables(hflmp_server *sv, TableSet &t) {
     G += (jb.nunits + jb.d.gsz - 1) / jb.d.gsz;
     Q += (jb.nunits + kQuad - 1) / kQuad;
   }
+  if ((size_t)U > t.rec_cap) {
+    if (t.d_rec) (void)hipFree(t.d_rec);
+    t.d_rec = nullptr;
+    t.rec_cap = std::max((size_t)U, 2 * t.rec_cap);
+    if (hipMalloc(&t.d_rec, sizeof(UnitRec) * t.rec_cap) != hipSuccess) {
+      t.rec_cap = 0;
+      return set_error(ctx, HFLMP_ENOMEM, "server: unit record allocation failed");
+    }
+  }
   int rc = grow_table(ctx, &t.d_unit, &t.h_unit, &t.unit_cap, (size_t)U);
   if (!rc) rc = grow_table(ctx, &t.d_grp, &t.h_grp, &t.grp_cap, (size_t)G);
   if (!rc) rc = grow_table(ctx, &t.d_quad, &t.h_quad, &t.quad_cap, (size_t)Q);
@@ -1165,22 +1232,20 @@ int rebuild_tables(hflmp_server *sv, TableSet &t) {
     quad0 += nq;
     ++S;
   }
-  // form of a system's SpMV: 0 FP64 int16 columns, 1 FP64 int32, 2 FP32
-  auto form = [](const SysDesc &d) { return d.mode != kModeFp64 ? 2 : d.col16_64 ? 0 : 1; };
-  for (int pass = 0; pass < 3; ++pass) {
+  for (int f = 0; f < kForms; ++f) {
+    t.ubeg[f] = unit0;
     S = 0;
     for (int slot : sv->order) {
       Job &jb = sv->jobs[sv->slot_job[slot]];
-      if (form(jb.d) == pass) {
+      if (spmv_form(jb.d) == f) {
         jb.d.unit0 = unit0;
         for (int u = 0; u < jb.nunits; ++u) t.h_unit[unit0 + u] = S;
         unit0 += jb.nunits;
       }
       ++S;
     }
-    if (pass == 0) t.U16 = unit0;
-    if (pass == 1) t.U64 = unit0;
   }
+  t.ubeg[kForms] = unit0;
   S = 0;
   for (int slot : sv->order) t.h_sys[S++] = sv->jobs[sv->slot_job[slot]].d;
   t.S = S;
@@ -1193,6 +1258,9 @@ int rebuild_tables(hflmp_server *sv, TableSet &t) {
     if (!rc) rc = xfer(ctx, t.d_grp, t.h_grp, sizeof(int) * G, ctx->stream);
     if (!rc) rc = xfer(ctx, t.d_quad, t.h_quad, sizeof(int) * Q, ctx->stream);
     if (rc) return rc;
+    const unsigned g = (unsigned)std::min<long long>(((long long)U + kBlock - 1) / kBlock, 4096);
+    grp_unitrec_kernel<<<g, kBlock, 0, ctx->stream>>>(t.d_sys, t.d_unit, U, t.d_rec);
+    HFLMP_LAUNCH_CHECK(ctx);
   }
   drop_graph(t);
   t.version = sv->version;
@@ -1777,6 +1845,7 @@ void hflmp_server_destroy(hflmp_server *sv) {
     if (t.d_unit) (void)hipFree(t.d_unit);
     if (t.h_sys) (void)hipHostFree(t.h_sys);
     if (t.h_unit) (void)hipHostFree(t.h_unit);
+    if (t.d_rec) (void)hipFree(t.d_rec);
     if (t.d_grp) (void)hipFree(t.d_grp);
     if (t.h_grp) (void)hipHostFree(t.h_grp);
     if (t.d_quad) (void)hipFree(t.d_quad);
