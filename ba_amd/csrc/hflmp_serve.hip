@@ -76,15 +76,15 @@ struct SlotState {
   int status;  // mixed: kFresh, kIterate or the final status
   int phase;   // mixed: 0 outer step due, 1 inner CG running, 2 its p to refresh first
   int outer, max_outer, inner_total, inner_maxit;
-  int col16;  // FP64 operator: 1 if its columns are int16 offsets from the slice
-              // start (decided on the device at admission), else int32
+  int col16;  // both operators: 1 if their columns are int16 offsets from the
+              // slice start (decided on the device at admission), else int32
   unsigned int bad;  // rows whose diagonal is missing or zero (counted at admission)
 };
 
 struct SysDesc {
   int nrow, nslice, nunits, unit0;
   int mode, col16;  // mixed: FP32 columns are int16 offsets from the slice start
-  int col16_64;     // FP64 layout: int16 column offsets (as the device decided at admission)
+  int col16_64;     // FP64 layout: the same (both as the device decided at admission)
   int grp0, gsz;    // K2's groups of gsz units (kGroup64 or kQuad): the system's first
   int quad0;        // K3's groups of kQuad units: the system's first
   // FP64 SELL-64 (P = 2): the operator of FP64 systems and of the residuals
@@ -748,15 +748,16 @@ __global__ void __launch_bounds__(kBlock)
 // diagonal: 1/a_ii formed in double (extract_invdiagelements,
 // lmatrix_csr_cpu.cc:2419-2442); for the FP32 operator from the FP32 value,
 // as the cast matrix's own extraction gives.  0 where the diagonal is
-// missing or zero, counted in *bad.
-template <typename IT32>
+// missing or zero, counted in *bad.  Both layouts take the column width
+// that sys_band_kernel decided (st->col16): int16 offsets from the slice's
+// first row where every entry fits, else int32 columns.
 __global__ void __launch_bounds__(kBlock)
     sys_fill_kernel(int nrow, const int *__restrict__ rp, const int *__restrict__ ci,
                     const double *__restrict__ val, const long long *__restrict__ soff,
                     const int *__restrict__ slen, const SlotState *st, void *__restrict__ col,
                     double *__restrict__ sval, double *__restrict__ invd,
                     const long long *__restrict__ soff32, const int *__restrict__ slen32,
-                    IT32 *__restrict__ col32, float *__restrict__ sval32,
+                    void *__restrict__ col32, float *__restrict__ sval32,
                     float *__restrict__ invd32, unsigned int *bad) {
   constexpr int P = SellChunk<double>::P, P32 = SellChunk<float>::P;
   const bool c16 = st->col16 != 0;
@@ -779,7 +780,10 @@ __global__ void __launch_bounds__(kBlock)
       sval[pos] = v;
       if (soff32) {
         const long long p32 = sell_pos(off32, m32, P32, lane, j);
-        col32[p32] = (IT32)(std::is_same<IT32, int>::value ? c : c - s * kSellRows);
+        if (c16)
+          static_cast<short *>(col32)[p32] = (short)(c - s * kSellRows);
+        else
+          static_cast<int *>(col32)[p32] = c;
         sval32[p32] = (float)v;
       }
       if (c == i && dg == 0.0) dg = v;
@@ -921,7 +925,7 @@ inline size_t solo_part_values(int nrow) {
 
 struct Job {
   long long ticket = 0;
-  int nrow = 0, nnz = 0, nslice = 0, nunits = 0, mode = kModeFp64, col16 = 0;
+  int nrow = 0, nnz = 0, nslice = 0, nunits = 0, mode = kModeFp64;
   const int *rp = nullptr, *ci = nullptr;
   const double *val = nullptr, *b = nullptr;
   double *x_out = nullptr;
@@ -955,22 +959,7 @@ void plan_job(Job &jb) {
   jb.rlen.assign(jb.nrow, 0);
   for (int r = 0; r < jb.nrow; ++r) jb.rlen[r] = (unsigned char)(jb.rp[r + 1] - jb.rp[r]);
   plan_slices(jb.nrow, jb.rp, jb.sp64);
-  jb.col16 = 0;
-  if (jb.mode == kModeMixed) {
-    plan_slices(jb.nrow, jb.rp, jb.sp32);
-    bool fits = jb.ci != nullptr;
-    for (int r = 0; r < jb.nrow && fits; ++r) {
-      const int base = (r / kSellRows) * kSellRows;
-      for (int k = jb.rp[r]; k < jb.rp[r + 1]; ++k) {
-        const int dlt = jb.ci[k] - base;
-        if (dlt < -32768 || dlt > 32767) {
-          fits = false;
-          break;
-        }
-      }
-    }
-    jb.col16 = fits ? 1 : 0;
-  }
+  if (jb.mode == kModeMixed) plan_slices(jb.nrow, jb.rp, jb.sp32);
   size_t o = 0;
   auto take = [&](size_t bytes) {
     const size_t at = o;
@@ -997,7 +986,7 @@ void plan_job(Job &jb) {
     const size_t e32 = (size_t)jb.sp32.entries;
     jb.o_soff32 = take(8 * ((size_t)jb.nslice + 1));
     jb.o_slen32 = take(4 * (size_t)jb.nslice);
-    jb.o_col32 = take((jb.col16 ? 2 : 4) * e32);
+    jb.o_col32 = take(4 * e32);  // as o_col
     jb.o_sval32 = take(4 * e32);
     jb.o_invd32 = take(4 * n);
     jb.o_r32 = take(4 * n);
@@ -1037,7 +1026,8 @@ struct TableSet {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   SlotState *h_states = nullptr;  // [max_active]
-  hipEvent_t done = nullptr;      // recorded after the state copy
+  hipEvent_t start = nullptr;     // recorded before the chunk's work (timing)
+  hipEvent_t done = nullptr;      // recorded after the state copy (timing)
 };
 
 // A retired system's arena block, released once no chunk in flight lists it.
@@ -1093,6 +1083,7 @@ struct hflmp_server {
   // rebuild and graph capture, waiting for the step's kernels
   double admit_seconds = 0, capture_seconds = 0, wait_seconds = 0, join_seconds = 0;
   double retire_seconds = 0;
+  double device_seconds = 0;  // the chunks' own device time (events)
 };
 
 namespace {
@@ -1342,7 +1333,7 @@ int admit(hflmp_server *sv, Job &jb, int slot) {
   if (!rc) rc = fill_zero_on(ctx, dp(jb.o_col), 4 * (size_t)jb.sp64.entries, s);
   if (!rc) rc = fill_zero_on(ctx, dp(jb.o_sval), 8 * (size_t)jb.sp64.entries, s);
   if (!rc && mixed) {
-    rc = fill_zero_on(ctx, dp(jb.o_col32), (jb.col16 ? 2 : 4) * (size_t)jb.sp32.entries, s);
+    rc = fill_zero_on(ctx, dp(jb.o_col32), 4 * (size_t)jb.sp32.entries, s);
     if (!rc) rc = fill_zero_on(ctx, dp(jb.o_sval32), 4 * (size_t)jb.sp32.entries, s);
   }
   if (rc) return rc;
@@ -1380,22 +1371,15 @@ int admit(hflmp_server *sv, Job &jb, int slot) {
   SlotState *dst = sv->d_states + slot;
   if (jb.nnz > 0) sys_band_kernel<<<g, kBlock, 0, s>>>(jb.nrow, rp, ci, dst);
   if (!mixed) {
-    sys_fill_kernel<int><<<g, kBlock, 0, s>>>(jb.nrow, rp, ci, val, soff, slen, dst, col, sval,
-                                              static_cast<double *>(dp(jb.o_invd)), nullptr,
-                                              nullptr, nullptr, nullptr, nullptr,
-                                              &sv->d_states[slot].bad);
-  } else if (jb.col16) {
-    sys_fill_kernel<short><<<g, kBlock, 0, s>>>(
-        jb.nrow, rp, ci, val, soff, slen, dst, col, sval, nullptr,
-        static_cast<const long long *>(dp(jb.o_soff32)),
-        static_cast<const int *>(dp(jb.o_slen32)), static_cast<short *>(dp(jb.o_col32)),
-        static_cast<float *>(dp(jb.o_sval32)), static_cast<float *>(dp(jb.o_invd32)),
-        &sv->d_states[slot].bad);
+    sys_fill_kernel<<<g, kBlock, 0, s>>>(jb.nrow, rp, ci, val, soff, slen, dst, col, sval,
+                                         static_cast<double *>(dp(jb.o_invd)), nullptr,
+                                         nullptr, nullptr, nullptr, nullptr,
+                                         &sv->d_states[slot].bad);
   } else {
-    sys_fill_kernel<int><<<g, kBlock, 0, s>>>(
+    sys_fill_kernel<<<g, kBlock, 0, s>>>(
         jb.nrow, rp, ci, val, soff, slen, dst, col, sval, nullptr,
         static_cast<const long long *>(dp(jb.o_soff32)),
-        static_cast<const int *>(dp(jb.o_slen32)), static_cast<int *>(dp(jb.o_col32)),
+        static_cast<const int *>(dp(jb.o_slen32)), dp(jb.o_col32),
         static_cast<float *>(dp(jb.o_sval32)), static_cast<float *>(dp(jb.o_invd32)),
         &sv->d_states[slot].bad);
   }
@@ -1406,7 +1390,6 @@ int admit(hflmp_server *sv, Job &jb, int slot) {
   d.nslice = jb.nslice;
   d.nunits = jb.nunits;
   d.mode = jb.mode;
-  d.col16 = jb.col16;
   d.soff = soff;
   d.slen = static_cast<const int *>(dp(jb.o_slen));
   d.rlen = static_cast<const unsigned char *>(dp(jb.o_rlen));
@@ -1534,6 +1517,7 @@ int launch_chunk(hflmp_server *sv, bool *launched) {
   HFLMP_REQUIRE(ctx, sv->launched - sv->completed <= 1, "server: two chunks already in flight");
   TableSet &t = sv->sets[sv->launched & 1];
   const auto t0 = Clock::now();
+  HFLMP_HIP_CHECK(ctx, hipEventRecord(t.start, s));
   if (t.version != sv->version) {
     int rc = rebuild_tables(sv, t);
     if (rc) return rc;
@@ -1687,6 +1671,8 @@ int wait_oldest(hflmp_server *sv, int *n_done) {
   const auto t_wait = Clock::now();
   HFLMP_HIP_CHECK(ctx, hipEventSynchronize(t.done));
   sv->wait_seconds += std::chrono::duration<double>(Clock::now() - t_wait).count();
+  float ms = 0;
+  if (hipEventElapsedTime(&ms, t.start, t.done) == hipSuccess) sv->device_seconds += 1e-3 * ms;
   ++sv->completed;
   for (const Deferred &r : sv->deferred) release_slot(sv, r);
   sv->deferred.clear();
@@ -1746,7 +1732,9 @@ int join_admitted(hflmp_server *sv, int *n_done) {
       release_slot(sv, slot_block(sv, slot));  // in no chunk's tables
       continue;
     }
+    // both layouts' column width, as the band kernel decided it
     sv->jobs[sv->slot_job[slot]].d.col16_64 = st.col16;
+    sv->jobs[sv->slot_job[slot]].d.col16 = st.col16;
     sv->first_chunk[slot] = -1;
     if (sv->jobs[sv->slot_job[slot]].solo) {
       sv->solo.push_back(slot);
@@ -1805,7 +1793,7 @@ int hflmp_server_create(hflmp_ctx *ctx, const hflmp_server_config *cfg, hflmp_se
             hipSuccess ||
         hipHostMalloc(&t.h_states, sizeof(SlotState) * sv->max_active,
                       hipHostMallocCoherent) != hipSuccess ||
-        hipEventCreateWithFlags(&t.done, hipEventDisableTiming) != hipSuccess)
+        hipEventCreate(&t.start) != hipSuccess || hipEventCreate(&t.done) != hipSuccess)
       return fail("table allocation failed");
   sv->stage_cap = std::min(std::max(sv->budget / 8, (size_t)16 << 20), (size_t)256 << 20);
   sv->rstage_cap = sv->stage_cap;
@@ -1851,6 +1839,7 @@ void hflmp_server_destroy(hflmp_server *sv) {
     if (t.d_quad) (void)hipFree(t.d_quad);
     if (t.h_quad) (void)hipHostFree(t.h_quad);
     if (t.h_states) (void)hipHostFree(t.h_states);
+    if (t.start) (void)hipEventDestroy(t.start);
     if (t.done) (void)hipEventDestroy(t.done);
   }
   (void)hipFree(sv->arena);
@@ -1993,6 +1982,7 @@ int hflmp_server_stats(const hflmp_server *sv, hflmp_server_stats_t *out) {
   out->wait_seconds = sv->wait_seconds;
   out->join_seconds = sv->join_seconds;
   out->retire_seconds = sv->retire_seconds;
+  out->chunk_device_seconds = sv->device_seconds;
   return HFLMP_OK;
 }
 

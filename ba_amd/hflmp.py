@@ -1049,7 +1049,8 @@ class ServerStats(C.Structure):
                 ("peak_active", C.c_int), ("peak_arena_in_use", C.c_size_t),
                 ("admit_seconds", C.c_double), ("capture_seconds", C.c_double),
                 ("wait_seconds", C.c_double), ("solo_runs", C.c_longlong),
-                ("join_seconds", C.c_double), ("retire_seconds", C.c_double)]
+                ("join_seconds", C.c_double), ("retire_seconds", C.c_double),
+                ("chunk_device_seconds", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

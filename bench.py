@@ -873,6 +873,9 @@ def run_server_stream(hflmp, ctx, feed_specs, arrivals, cfg, rtol, precision, ke
                                  "step_wait_for_kernels": round(stats["wait_seconds"], 3),
                                  "step_wait_for_admission": round(stats["join_seconds"], 3),
                                  "step_retire": round(stats["retire_seconds"], 3)}
+        # the device's own time in the grouped chunks (HIP events): the rest
+        # of the makespan the device waited for the host
+        stats["device_seconds_in_chunks"] = round(stats["chunk_device_seconds"], 3)
     finally:
         sv.close()
         feed.close()

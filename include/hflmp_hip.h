@@ -836,9 +836,10 @@ int hflmp_mpcg_jacobi_sell(hflmp_ctx *ctx, const hflmp_sell *A64,
  * < rtol ||b||, with maxit bounding each inner solve.  precision 0 at
  * submission picks the mixed mode for mixed_min_nnz <= nnz < mixed_max_nnz
  * (mixed_min_nnz 0: never; mixed_max_nnz 0: no upper bound).
- * The FP64 SELL-64 operator holds int16 column offsets from each slice's
- * first row when every column fits (decided on the device at admission;
- * the footprint assumes int32), else int32 columns.
+ * Both SELL-64 operators (FP64, and the FP32 one of a mixed system) hold
+ * int16 column offsets from each slice's first row when every column fits
+ * (decided on the device at admission; the footprint assumes int32), else
+ * int32 columns.
  * FP64 systems with at least solo_min_nnz nonzeros run alone instead: the
  * single-system iteration of hflmp_pcg_jacobi_sell_f64 (three launches per
  * iteration, its grids) on a third stream of the server, concurrently
@@ -893,12 +894,17 @@ typedef struct {
   long long solo_runs; /* systems started alone so far (solo_min_nnz)     */
   double join_seconds; /* waiting for admission work after the chunk     */
   double retire_seconds; /* copying retired systems' x to the callers      */
+  /* device seconds of the grouped chunks so far (HIP events around each
+   * chunk's work on the context stream: table uploads and its launches);
+   * against the host's clock it gives the time the device spent idle
+   * between chunks */
+  double chunk_device_seconds;
 } hflmp_server_stats_t;
 int hflmp_server_create(hflmp_ctx *ctx, const hflmp_server_config *cfg,
                         hflmp_server **out);
 void hflmp_server_destroy(hflmp_server *sv);
-/* footprint for precision 64 or 32 (col may be NULL: int32 columns
- * assumed), for a system in the grouped launches */
+/* footprint for precision 64 or 32, for a system in the grouped launches
+ * (int32 columns assumed; col is not read and may be NULL) */
 size_t hflmp_server_footprint(int nrow, int nnz, const int *row_ptr,
                               const int *col, int precision);
 int hflmp_server_submit(hflmp_server *sv, int nrow, int nnz, const int *row_ptr,
