@@ -813,6 +813,50 @@ def server_of(hflmp, ctx, cfg):
                         cfg.get("inner_rtol", 1e-2), cfg.get("mixed_max_nnz", 0))
 
 
+def bench_serve_full_load(hflmp, dim=2, n=255, count=256, graphs=True):
+    """The solve server at full load: `count` copies of one FE system (2D
+    Q1 n x n by default: 65,536 rows, 586,756 nonzeros) submitted at once
+    and solved to 1e-8, in FP64 and in the mixed mode.  Reports the
+    grouped chunks' device time (HIP events) per CG iteration and the
+    bandwidth it implies, from the iteration's algorithmic bytes per
+    system: FP64 10 nnz + 113 N (int16-column SELL-64 SpMV with the row
+    length, Ap, p at the row and the gathered p once; the x/r/z update 64 N;
+    the p update 24 N), the mixed mode's FP32 inner iteration 6 nnz + 57 N."""
+    kap = hflmp.fe_kappa(dim, n, 1.0, 7)
+    rp, ci, v = hflmp.fe_system(1, dim, n, kap)
+    b = hflmp.fe_rhs(1, dim, n, 8)
+    N, nnz = len(rp) - 1, len(ci)
+    ctx = hflmp.Context(0)
+    res = {"systems": count, "system": "%dD Q1 %d^%d: %d rows, %d nonzeros" % (dim, n, dim, N, nnz),
+           "rows_in_all": count * N}
+    for name, prec, per_iter in (("fp64", 64, 10 * nnz + 113 * N),
+                                 ("mixed", 32, 6 * nnz + 57 * N)):
+        sv = hflmp.Server(ctx, 32 << 30, 256, 16, graphs, 0, 1e-2, 0)
+        try:
+            t0 = time.perf_counter()
+            tickets = [sv.submit(rp, ci, v, b, rtol=1e-8, precision=prec) for _ in range(count)]
+            left, iters, ok = set(tickets), 0, True
+            while left:
+                if sv.step():
+                    for t in list(left):
+                        if sv.poll(t)["state"] == 2:
+                            _, info = sv.take(t)
+                            iters = max(iters, info["iters"])
+                            ok = ok and info["status"] == 0
+                            left.discard(t)
+            wall = time.perf_counter() - t0
+            dev = sv.stats()["chunk_device_seconds"]
+        finally:
+            sv.close()
+        res[name] = {"iterations": iters, "converged": ok, "seconds": round(wall, 4),
+                     "device_seconds_in_chunks": round(dev, 4),
+                     "device_us_per_iteration": round(dev / iters * 1e6, 2),
+                     "GBps": round(count * per_iter * iters / dev / 1e9, 1),
+                     "bytes_per_iteration_per_system": per_iter}
+    ctx.close()
+    return res
+
+
 def run_server_stream(hflmp, ctx, feed_specs, arrivals, cfg, rtol, precision, keep,
                       max_queued=32):
     """The stream through one server, its requests generated as it runs
@@ -991,6 +1035,7 @@ def bench_serve(hflmp, nreq, seed, rate, ref_style, graphs=True, ntune=160, samp
                       "makespan_s": round(makespan, 3), "steps": stats["steps"],
                       "graph_captures": stats["graph_captures"],
                       "host_seconds": stats["host_seconds"],
+                      "device_seconds_in_chunks": stats["device_seconds_in_chunks"],
                       "peak_active": stats["peak_active"],
                       "peak_arena_in_use_GB": round(stats["peak_arena_in_use"] / 1e9, 3),
                       "mixed_requests": nmixed,
@@ -1074,6 +1119,8 @@ def main():
                     help="Poisson arrival rate for the latency run (default: 60%% of the "
                          "server's measured capacity)")
     ap.add_argument("--no-serve-reference", action="store_true")
+    ap.add_argument("--serve-full-load-only", action="store_true",
+                    help="in the serving leg, only the full-load run (256 systems at once)")
     ap.add_argument("--serve-no-graphs", action="store_true",
                     help="run the server's steps as direct launches instead of graph replay")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -1210,11 +1257,14 @@ def main():
     if not args.no_batched:
         out["batched"] = bench_batched(hflmp, spmm_only=args.batched_spmm_only)
         progress("batched leg done")
-    if not args.no_serve:
+    if not args.no_serve and not args.serve_full_load_only:
         out["serving"] = bench_serve(hflmp, args.serve_requests, args.serve_seed,
                                      args.serve_rate, not args.no_serve_reference,
                                      not args.serve_no_graphs)
         progress("serving leg done")
+    if not args.no_serve:
+        out["serving_full_load"] = bench_serve_full_load(hflmp, graphs=not args.serve_no_graphs)
+        progress("serving at full load done")
 
     # CG time to 1e-8 on the 128^3 Poisson system, every solver path side
     # by side (each measured in its leg above)
