@@ -48,9 +48,11 @@
 // inner iteration moves FP32 bytes.
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
 #include <cstring>
 #include <deque>
 #include <map>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -1052,6 +1054,21 @@ struct hflmp_server {
   size_t stage_cap = 0, stage_used = 0;
   hipStream_t result_stream = nullptr; // retired systems' x to the host
   char *rstage = nullptr;              // pinned staging for those copies
+  // The admission thread: the uploads and layout kernels of admitted
+  // systems (admit_upload), in the order they were admitted, so that their
+  // host copies overlap the calling thread's wait for the chunk.
+  std::thread admitter;
+  std::mutex am;
+  std::condition_variable awake, aidle;
+  std::deque<std::pair<const Job *, int>> apending;  // (job, slot) not yet taken
+  int abusy = 0;          // taken and not finished
+  bool astop = false;
+  int arc = HFLMP_OK;     // the thread's first failure
+  hflmp_ctx actx;         // the thread's copy of the context (its own error text)
+  // per slot: an event after the slot's admission work on the admission
+  // stream, and whether it has been recorded (guarded by am)
+  std::vector<hipEvent_t> jev;
+  std::vector<char> aready;
   size_t rstage_cap = 0;
   std::vector<int> joining;            // admitted slots that join after the chunk
   std::vector<long long> slot_job;  // slot -> ticket, -1 free
@@ -1284,8 +1301,7 @@ void host_copy(char *dst, const char *src, size_t bytes) {
 // stream through the pinned staging area, so the host never waits for the
 // admission kernels queued before it.  When the area is full the host
 // waits for the admission stream once and starts again at its beginning.
-int stage_upload(hflmp_server *sv, void *dst, const void *src, size_t bytes) {
-  hflmp_ctx *ctx = sv->ctx;
+int stage_upload(hflmp_server *sv, hflmp_ctx *ctx, void *dst, const void *src, size_t bytes) {
   constexpr size_t kMinPiece = 4u << 20;
   const char *p = static_cast<const char *>(src);
   char *d = static_cast<char *>(dst);
@@ -1308,8 +1324,13 @@ int stage_upload(hflmp_server *sv, void *dst, const void *src, size_t bytes) {
   return HFLMP_OK;
 }
 
-int admit(hflmp_server *sv, Job &jb, int slot) {
-  hflmp_ctx *ctx = sv->ctx;
+// The admission work of one system on the admission stream: uploads
+// through the staging area, zero fills, its initial state and the layout
+// kernels.  Runs on the server's admission thread (ctx: the thread's copy
+// of the context, for its own error text), or on the calling thread for a
+// solo system.  Touches only the system's block, the staging area and the
+// slot's pinned initial state.
+int admit_upload(hflmp_server *sv, hflmp_ctx *ctx, const Job &jb, int slot) {
   hipStream_t s = sv->admit_stream;
   char *base = sv->arena + jb.off;
   auto dp = [&](size_t o) { return static_cast<void *>(base + o); };
@@ -1317,7 +1338,7 @@ int admit(hflmp_server *sv, Job &jb, int slot) {
   const bool mixed = jb.mode == kModeMixed;
   int rc = HFLMP_OK;
   auto up = [&](size_t o, const void *src, size_t bytes) {
-    if (!rc) rc = stage_upload(sv, dp(o), src, bytes);
+    if (!rc) rc = stage_upload(sv, ctx, dp(o), src, bytes);
   };
   up(jb.o_rp, jb.rp, 4 * (n + 1));
   up(jb.o_ci, jb.ci, 4 * (size_t)jb.nnz);
@@ -1384,17 +1405,26 @@ int admit(hflmp_server *sv, Job &jb, int slot) {
         &sv->d_states[slot].bad);
   }
   HFLMP_LAUNCH_CHECK(ctx);
+  return HFLMP_OK;
+}
+
+// The host side of an admission, on the calling thread: the system's
+// descriptor over its block, its slot and state.
+void admit_describe(hflmp_server *sv, Job &jb, int slot) {
+  char *base = sv->arena + jb.off;
+  auto dp = [&](size_t o) { return static_cast<void *>(base + o); };
+  const bool mixed = jb.mode == kModeMixed;
   SysDesc &d = jb.d;
   std::memset(&d, 0, sizeof(d));
   d.nrow = jb.nrow;
   d.nslice = jb.nslice;
   d.nunits = jb.nunits;
   d.mode = jb.mode;
-  d.soff = soff;
+  d.soff = static_cast<const long long *>(dp(jb.o_soff));
   d.slen = static_cast<const int *>(dp(jb.o_slen));
   d.rlen = static_cast<const unsigned char *>(dp(jb.o_rlen));
-  d.col = col;
-  d.sval = sval;
+  d.col = dp(jb.o_col);
+  d.sval = static_cast<double *>(dp(jb.o_sval));
   d.b = static_cast<const double *>(dp(jb.o_b));
   d.x = static_cast<double *>(dp(jb.o_x));
   d.part = static_cast<double *>(dp(jb.o_part));
@@ -1420,7 +1450,6 @@ int admit(hflmp_server *sv, Job &jb, int slot) {
   jb.slot = slot;
   jb.state = kActive;
   jb.t_admit = Clock::now();
-  return HFLMP_OK;
 }
 
 // The grouped systems' chunk on the context stream: a replay of the
@@ -1555,6 +1584,44 @@ int launch_chunk(hflmp_server *sv, bool *launched) {
   return HFLMP_OK;
 }
 
+void admitter_run(hflmp_server *sv) {
+  (void)hipSetDevice(sv->actx.device);
+  for (;;) {
+    std::pair<const Job *, int> w;
+    bool failed;
+    {
+      std::unique_lock<std::mutex> l(sv->am);
+      sv->awake.wait(l, [sv] { return sv->astop || !sv->apending.empty(); });
+      if (sv->apending.empty()) return;
+      w = sv->apending.front();
+      sv->apending.pop_front();
+      ++sv->abusy;
+      failed = sv->arc != HFLMP_OK;
+    }
+    // after a failure the server reports it at the next join; later
+    // admissions are not started
+    int rc = failed ? HFLMP_OK : admit_upload(sv, &sv->actx, *w.first, w.second);
+    if (!failed && !rc && hipEventRecord(sv->jev[w.second], sv->admit_stream) != hipSuccess)
+      rc = set_error(&sv->actx, HFLMP_EHIP, "server: admission event not recorded");
+    std::lock_guard<std::mutex> g(sv->am);
+    if (rc && sv->arc == HFLMP_OK) sv->arc = rc;
+    if (!failed && !rc) sv->aready[w.second] = 1;
+    if (--sv->abusy == 0 && sv->apending.empty()) sv->aidle.notify_all();
+  }
+}
+
+// Waits until the admission thread has queued all admission work posted so
+// far; returns its first failure (with its message in the context).
+int admitter_drain(hflmp_server *sv) {
+  std::unique_lock<std::mutex> l(sv->am);
+  sv->aidle.wait(l, [sv] { return sv->abusy == 0 && sv->apending.empty(); });
+  if (sv->arc != HFLMP_OK) {
+    std::memcpy(sv->ctx->last_error, sv->actx.last_error, sizeof(sv->ctx->last_error));
+    return sv->arc;
+  }
+  return HFLMP_OK;
+}
+
 // Admits first come, first served while a slot is free and the block fits.
 int admit_queued(hflmp_server *sv) {
   const auto t0 = Clock::now();
@@ -1568,11 +1635,28 @@ int admit_queued(hflmp_server *sv) {
     if (slot < 0) break;
     Job &jb = sv->jobs[sv->queue.front()];
     if (!sv->alloc.alloc(jb.bytes, &jb.off)) break;
-    int rc = admit(sv, jb, slot);
-    if (!rc && jb.solo) rc = start_solo(sv, jb, slot);
-    if (rc) {
-      sv->alloc.release(jb.off, jb.bytes);
-      return rc;
+    admit_describe(sv, jb, slot);
+    if (jb.solo) {
+      // started at once after its admission work: admitted on this thread,
+      // after the admission thread's work
+      int rc = admitter_drain(sv);
+      if (!rc) rc = admit_upload(sv, sv->ctx, jb, slot);
+      if (!rc) rc = start_solo(sv, jb, slot);
+      if (!rc && hipEventRecord(sv->jev[slot], sv->admit_stream) != hipSuccess)
+        rc = set_error(sv->ctx, HFLMP_EHIP, "server: admission event not recorded");
+      if (rc) {
+        sv->alloc.release(jb.off, jb.bytes);
+        return rc;
+      }
+      std::lock_guard<std::mutex> g(sv->am);
+      sv->aready[slot] = 1;
+    } else {
+      {
+        std::lock_guard<std::mutex> g(sv->am);
+        sv->aready[slot] = 0;
+        sv->apending.emplace_back(&jb, slot);
+      }
+      sv->awake.notify_one();
     }
     // The CSR arrays at the block's end are read only by the admission
     // kernels just queued (band check, SELL-64 fill); every later use of
@@ -1712,19 +1796,49 @@ int join_admitted(hflmp_server *sv, int *n_done) {
   hflmp_ctx *ctx = sv->ctx;
   if (sv->joining.empty()) return HFLMP_OK;
   const auto t0 = Clock::now();
-  // each admitted system's state as admission left it: the FP64 layout's
-  // column width that the band kernel decided (the grouped kernels take it
-  // from the system table rather than from the state)
-  for (int slot : sv->joining)
+  // With systems running, only those whose admission work has finished
+  // join (the others at a later step), so the next chunk is never held up
+  // by admission; with nothing running, wait for all of them.
+  const bool block = sv->order.empty() && sv->solo.empty() && sv->launched == sv->completed;
+  if (block) {
+    const int rc = admitter_drain(sv);
+    if (rc) return rc;
+  }
+  std::vector<int> cand, ready, later;
+  {
+    std::lock_guard<std::mutex> g(sv->am);
+    if (sv->arc != HFLMP_OK) {
+      std::memcpy(ctx->last_error, sv->actx.last_error, sizeof(ctx->last_error));
+      return sv->arc;
+    }
+    for (int slot : sv->joining) (sv->aready[slot] ? cand : later).push_back(slot);
+  }
+  for (int slot : cand) {
+    if (block) {
+      HFLMP_HIP_CHECK(ctx, hipEventSynchronize(sv->jev[slot]));
+    } else {
+      const hipError_t e = hipEventQuery(sv->jev[slot]);
+      if (e == hipErrorNotReady) {
+        later.push_back(slot);
+        continue;
+      }
+      HFLMP_HIP_CHECK(ctx, e);
+    }
+    ready.push_back(slot);
+  }
+  sv->joining.swap(later);
+  // each ready system's state as admission left it: the column width that
+  // the band kernel decided (the grouped kernels take it from the system
+  // table rather than from the state), refusal; read on the result stream,
+  // which holds no other work now (the admission stream may hold later
+  // systems' work)
+  for (int slot : ready)
     HFLMP_HIP_CHECK(ctx, hipMemcpyAsync(sv->h_init + slot, sv->d_states + slot,
                                         sizeof(SlotState), hipMemcpyDeviceToHost,
-                                        sv->admit_stream));
-  HFLMP_HIP_CHECK(ctx, hipStreamSynchronize(sv->admit_stream));
-  sv->stage_used = 0;
+                                        sv->result_stream));
+  if (!ready.empty()) HFLMP_HIP_CHECK(ctx, hipStreamSynchronize(sv->result_stream));
   sv->join_seconds += std::chrono::duration<double>(Clock::now() - t0).count();
-  std::vector<int> joining;
-  joining.swap(sv->joining);
-  for (int slot : joining) {
+  for (int slot : ready) {
     const SlotState &st = sv->h_init[slot];
     if ((st.mode == kModeMixed ? st.status : st.s64.status) == kRefused) {
       const int rc = retire_systems(sv, {slot}, {&st}, n_done);
@@ -1811,12 +1925,31 @@ int hflmp_server_create(hflmp_ctx *ctx, const hflmp_server_config *cfg, hflmp_se
   sv->alloc.reset(sv->budget);
   sv->slot_job.assign(sv->max_active, -1);
   sv->first_chunk.assign(sv->max_active, -1);
+  sv->jev.assign(sv->max_active, nullptr);
+  sv->aready.assign(sv->max_active, 0);
+  for (hipEvent_t &e : sv->jev)
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+      return fail("event creation failed");
+  sv->actx = *ctx;
+  sv->actx.last_error[0] = 0;
+  try {
+    sv->admitter = std::thread(admitter_run, sv);
+  } catch (...) {
+    return fail("admission thread not started");
+  }
   *out = sv;
   return HFLMP_OK;
 }
 
 void hflmp_server_destroy(hflmp_server *sv) {
   if (!sv) return;
+  // the admission thread finishes the work already posted, then ends
+  {
+    std::lock_guard<std::mutex> g(sv->am);
+    sv->astop = true;
+  }
+  sv->awake.notify_all();
+  if (sv->admitter.joinable()) sv->admitter.join();
   if (sv->admit_stream) (void)hipStreamSynchronize(sv->admit_stream);
   if (sv->result_stream) (void)hipStreamSynchronize(sv->result_stream);
   if (sv->solo_stream) (void)hipStreamSynchronize(sv->solo_stream);
@@ -1825,6 +1958,8 @@ void hflmp_server_destroy(hflmp_server *sv) {
   if (sv->result_stream) (void)hipStreamDestroy(sv->result_stream);
   if (sv->solo_stream) (void)hipStreamDestroy(sv->solo_stream);
   if (sv->ev_admitted) (void)hipEventDestroy(sv->ev_admitted);
+  for (hipEvent_t e : sv->jev)
+    if (e) (void)hipEventDestroy(e);
   if (sv->ev_solo_done) (void)hipEventDestroy(sv->ev_solo_done);
   if (sv->h_admit) (void)hipHostFree(sv->h_admit);
   for (TableSet &t : sv->sets) {
