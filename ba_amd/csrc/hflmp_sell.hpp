@@ -174,6 +174,94 @@ __device__ __forceinline__ T sell_slice_sum(const IT *__restrict__ col, const T 
   return s;
 }
 
+// Two slices at once, A and B, each lane one row of each: the sums that
+// sell_slice_sum<T, IT, false, false, true> forms for either (the same
+// order within each row), with both slices' entry loads issued before
+// either's gathers, so that a wave keeps twice the bytes in flight.
+// smaxB = 0: no slice B (sB stays as given).  U: chunk loads in flight
+// per slice and step (the order of the sums does not depend on it).
+template <typename T, typename IT, int U = SellChunk<T>::U>
+__device__ __forceinline__ void sell_slice_sum2(const IT *__restrict__ colA,
+                                                const T *__restrict__ valA, int smaxA, int lenA,
+                                                const T *__restrict__ xA,
+                                                const IT *__restrict__ colB,
+                                                const T *__restrict__ valB, int smaxB, int lenB,
+                                                const T *__restrict__ xB, int lane, T &sA, T &sB) {
+#pragma clang fp contract(off)
+  constexpr int P = SellChunk<T>::P;
+  typedef typename VecOf<T, P>::type TV;
+  typedef typename VecOf<IT, P>::type IV;
+  const int fullA = smaxA / P * P, fullB = smaxB / P * P;
+  const IV *__restrict__ cA = reinterpret_cast<const IV *>(colA) + lane;
+  const TV *__restrict__ vA = reinterpret_cast<const TV *>(valA) + lane;
+  const IV *__restrict__ cB = reinterpret_cast<const IV *>(colB) + lane;
+  const TV *__restrict__ vB = reinterpret_cast<const TV *>(valB) + lane;
+  constexpr int TM = P > 1 ? P - 1 : 1;
+  const int ntA = lenA - fullA, ntB = lenB - fullB;
+  const int nlA = smaxA - fullA, nlB = smaxB - fullB;
+  IT ctA[TM], ctB[TM];
+  T vtA[TM], vtB[TM];
+#pragma unroll
+  for (int t = 0; t < TM; ++t) {
+    if (t < nlA) {
+      const long long q = (long long)kSellRows * (fullA + t) + lane;
+      ctA[t] = __builtin_nontemporal_load(colA + q);
+      vtA[t] = __builtin_nontemporal_load(valA + q);
+    }
+    if (t < nlB) {
+      const long long q = (long long)kSellRows * (fullB + t) + lane;
+      ctB[t] = __builtin_nontemporal_load(colB + q);
+      vtB[t] = __builtin_nontemporal_load(valB + q);
+    }
+  }
+  const int full = fullA > fullB ? fullA : fullB;
+  for (int j = 0; j < full; j += P * U) {
+    IV ccA[U], ccB[U];
+    TV vvA[U], vvB[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (j + u * P < fullA) {
+        ccA[u] = __builtin_nontemporal_load(cA + (j / P + u) * kWave);
+        vvA[u] = __builtin_nontemporal_load(vA + (j / P + u) * kWave);
+      }
+      if (j + u * P < fullB) {
+        ccB[u] = __builtin_nontemporal_load(cB + (j / P + u) * kWave);
+        vvB[u] = __builtin_nontemporal_load(vB + (j / P + u) * kWave);
+      }
+    }
+    T xxA[U][P], xxB[U][P];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int q = 0; q < P; ++q) {
+        if (j + u * P + q < lenA && j + u * P < fullA) xxA[u][q] = xA[(int)ccA[u][q]];
+        if (j + u * P + q < lenB && j + u * P < fullB) xxB[u][q] = xB[(int)ccB[u][q]];
+      }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int q = 0; q < P; ++q)
+        if (j + u * P + q < lenA && j + u * P < fullA) sA += vvA[u][q] * xxA[u][q];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int q = 0; q < P; ++q)
+        if (j + u * P + q < lenB && j + u * P < fullB) sB += vvB[u][q] * xxB[u][q];
+  }
+  T xtA[TM], xtB[TM];
+#pragma unroll
+  for (int t = 0; t < TM; ++t) {
+    if (t < ntA) xtA[t] = xA[(int)ctA[t]];
+    if (t < ntB) xtB[t] = xB[(int)ctB[t]];
+  }
+#pragma unroll
+  for (int t = 0; t < TM; ++t)
+    if (t < ntA) sA += vtA[t] * xtA[t];
+#pragma unroll
+  for (int t = 0; t < TM; ++t)
+    if (t < ntB) sB += vtB[t] * xtB[t];
+}
+
 // SELL-64: unit = four slices (256 consecutive rows), wave w of the
 // workgroup runs slice 4 unit + w.  No LDS and no barrier.
 template <typename T, typename IT = int>

@@ -523,6 +523,68 @@ __global__ void __launch_bounds__(kBlock)
   unit_partial1(part, static_cast<T *>(r.part), r.j);
 }
 
+// The FP32 forms run two units per workgroup: each wave sums slice w of
+// both (sell_slice_sum2), so that it keeps twice the bytes in flight (the
+// FP32 wave's loads are half the FP64 one's).  The units may belong to
+// different systems; each keeps its own partial, summed as unit_partial1
+// sums it (wave sums, then the waves in order).
+template <typename T, typename IT>
+__global__ void __launch_bounds__(kBlock)
+    grp_spmv_pair_kernel(const UnitRec *__restrict__ rec, int u0, int U) {
+  const long long pr = xcd_tile(blockIdx.x, gridDim.x);
+  if (pr >= (U + 1) / 2) return;
+  const int ua = u0 + 2 * (int)pr;
+  const bool hasB = 2 * (int)pr + 1 < U;
+  const UnitRec &ra = rec[ua];
+  const UnitRec &rb = rec[hasB ? ua + 1 : ua];
+  const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int lane = threadIdx.x & (kWave - 1);
+  const int srowA = ra.row0 + w * kSellRows, srowB = rb.row0 + w * kSellRows;
+  const int rowA = srowA + lane, rowB = srowB + lane;
+  const bool hA = rowA < ra.nrow, hB = hasB && rowB < rb.nrow;
+  const T *__restrict__ pA = static_cast<const T *>(ra.p);
+  const T *__restrict__ pB = static_cast<const T *>(rb.p);
+  const int2 slA = ra.sl[w];
+  const int2 slB = hasB ? rb.sl[w] : make_int2(0, 0);
+  const bool liveA = mixed_inner_live(ra.st), liveB = hasB && mixed_inner_live(rb.st);
+  const int lenA = hA ? (int)ra.rlen[rowA] : 0, lenB = hB ? (int)rb.rlen[rowB] : 0;
+  const T prowA = hA ? pA[rowA] : T(0), prowB = hB ? pB[rowB] : T(0);
+  const T *__restrict__ xA = std::is_same<IT, int>::value ? pA : pA + srowA;
+  const T *__restrict__ xB = std::is_same<IT, int>::value ? pB : pB + srowB;
+  T sA = 0, sB = 0;
+  sell_slice_sum2<T, IT, 2>(static_cast<const IT *>(ra.col) + slA.x,
+                         static_cast<const T *>(ra.val) + slA.x, slA.y, lenA, xA,
+                         static_cast<const IT *>(rb.col) + slB.x,
+                         static_cast<const T *>(rb.val) + slB.x, slB.y, lenB, xB, lane, sA, sB);
+  T partA = 0, partB = 0;
+  if (liveA && hA) {
+    static_cast<T *>(ra.Ap)[rowA] = sA;
+    partA = sA * prowA;
+  }
+  if (liveB && hB) {
+    static_cast<T *>(rb.Ap)[rowB] = sB;
+    partB = sB * prowB;
+  }
+  __shared__ T lds[2][kBlock / kWave];
+  partA = wave_sum(partA);
+  partB = wave_sum(partB);
+  if (lane == 0) {
+    lds[0][w] = partA;
+    lds[1][w] = partB;
+  }
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    const int h = threadIdx.x;
+    if (h == 0 ? liveA : liveB) {
+      T t = 0;
+#pragma unroll
+      for (int k = 0; k < kBlock / kWave; ++k) t += lds[h][k];
+      const UnitRec &r = h == 0 ? ra : rb;
+      static_cast<T *>(r.part)[r.j] = t;
+    }
+  }
+}
+
 // F1: alpha = r.z / p.Ap, one workgroup per system
 __global__ void __launch_bounds__(kBlock) grp_alpha_final(const SysDesc *__restrict__ sys) {
   const SysDesc d = sys[blockIdx.x];
@@ -1135,11 +1197,11 @@ void enqueue_chunk(const hflmp_server *sv, const TableSet &t, hipStream_t s) {
       grp_spmv_kernel<double, int><<<grid(ub[2] - ub[1]), kBlock, 0, s>>>(t.d_rec, ub[1],
                                                                             ub[2] - ub[1]);
     if (ub[3] > ub[2])
-      grp_spmv_kernel<float, short><<<grid(ub[3] - ub[2]), kBlock, 0, s>>>(t.d_rec, ub[2],
-                                                                             ub[3] - ub[2]);
+      grp_spmv_pair_kernel<float, short><<<grid((ub[3] - ub[2] + 1) / 2), kBlock, 0, s>>>(
+          t.d_rec, ub[2], ub[3] - ub[2]);
     if (ub[4] > ub[3])
-      grp_spmv_kernel<float, int><<<grid(ub[4] - ub[3]), kBlock, 0, s>>>(t.d_rec, ub[3],
-                                                                           ub[4] - ub[3]);
+      grp_spmv_pair_kernel<float, int><<<grid((ub[4] - ub[3] + 1) / 2), kBlock, 0, s>>>(
+          t.d_rec, ub[3], ub[4] - ub[3]);
     grp_alpha_final<<<gS, kBlock, 0, s>>>(t.d_sys);
     grp_update_kernel<<<gG, kBlock, 0, s>>>(t.d_sys, t.d_grp, t.G);
     grp_beta_final<<<gS, kBlock, 0, s>>>(t.d_sys);

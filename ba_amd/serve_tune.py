@@ -3,9 +3,10 @@
 It chooses the precision window [`mixed_min_nnz`, `mixed_max_nnz`) -- the
 systems solved by the FP32-inner / FP64-refined mode rather than FP64 CG --
 by timing the whole tuning sample through the server under each candidate
-window, and then the iterations per step (the step is the unit of
-admission, retirement and graph replay) among candidates by the sample's
-throughput.  The sample must be large enough to load the server: the mixed
+window; with a mixed window, the inner tolerance (how far the FP32 inner
+solve drives its residual before the FP64 one replaces it); then the
+iterations per step (the step is the unit of admission, retirement and
+graph replay) among candidates by the sample's throughput.  The sample must be large enough to load the server: the mixed
 mode's gain is a bandwidth gain (fewer bytes per iteration), which a few
 systems at a time do not show, since their launches' latency hides it.
 Candidate windows start at size-class boundaries (nonzeros of the system,
@@ -57,6 +58,10 @@ MARGIN = 0.05  # a chunk length replaces the default only when this much faster
 # A window replaces FP64 only when the whole sample gains more than this: it
 # adds per-system outer steps to launches shared with every other system.
 WINDOW_MARGIN = 0.10
+# The inner tolerance of a mixed window changes the same kernels' iteration
+# counts, not which kernels run: a smaller gain carries over (stream probes:
+# 0.1 against 0.01 took 2.5-3% off the device time, every time).
+IRT_MARGIN = 0.02
 # candidate windows (lo, hi): FP64 only, then mixed from each class boundary
 WINDOWS = ((0, 0),) + tuple((max(b, 1), 0) for b in reversed(CLASS_BOUNDS[:-1]))
 
@@ -82,11 +87,14 @@ def _best(ctx, reqs, cfg, precision, rtol):
     return min(t1, t2), ok1 and ok2
 
 
-def tune(ctx, sample, base_cfg, rtol=1e-8, chunks=(8, 16, 32), windows=WINDOWS):
+def tune(ctx, sample, base_cfg, rtol=1e-8, chunks=(8, 16, 32), windows=WINDOWS,
+         inner_rtols=(1e-1,)):
     """sample: list of (rp, ci, val, b), submitted at once (enough systems
     to load the server).  Returns (config, report).  Every timing is the
     better of two runs; the window departs from FP64 on a gain of more than
-    WINDOW_MARGIN, the chunk length from base_cfg's on more than MARGIN."""
+    WINDOW_MARGIN, the inner tolerance (with a mixed window) from
+    base_cfg's on more than IRT_MARGIN, the chunk length on more than
+    MARGIN."""
     cfg = dict(base_cfg)
     _run(ctx, sample[:1], cfg, 64, rtol)  # warm the kernels
     report = {"sample_systems": len(sample), "windows": {}}
@@ -96,6 +104,20 @@ def tune(ctx, sample, base_cfg, rtol=1e-8, chunks=(8, 16, 32), windows=WINDOWS):
         report["windows"]["[%d, %d)" % (lo, hi)] = {"seconds": round(t, 4), "converged": ok}
         times[(lo, hi)] = t if ok else None
     cfg["mixed_min_nnz"], cfg["mixed_max_nnz"] = choose_window(times)
+    if cfg["mixed_min_nnz"] > 0:
+        # how far the FP32 inner solve drives its residual before the FP64
+        # one replaces it: replacing more often costs outer steps and keeps
+        # the FP32 recursion closer to the true residual (fewer inner
+        # iterations on this mix)
+        t_def = times[(cfg["mixed_min_nnz"], cfg["mixed_max_nnz"])]
+        report["inner_rtol"] = {str(cfg["inner_rtol"]): round(t_def, 4)}
+        for irt in inner_rtols:
+            if irt == cfg["inner_rtol"]:
+                continue
+            t, ok = _best(ctx, sample, dict(cfg, inner_rtol=irt), 0, rtol)
+            report["inner_rtol"][str(irt)] = round(t, 4)
+            if ok and t < (1 - IRT_MARGIN) * t_def:
+                cfg["inner_rtol"], t_def = irt, t
     report["chunks"] = {}
     ctimes = {}
     for c in sorted(set(chunks) | {cfg["chunk_iters"]}):
@@ -110,5 +132,6 @@ def tune(ctx, sample, base_cfg, rtol=1e-8, chunks=(8, 16, 32), windows=WINDOWS):
             cfg["chunk_iters"] = c_best
     report["mixed_min_nnz"] = cfg["mixed_min_nnz"]
     report["mixed_max_nnz"] = cfg["mixed_max_nnz"]
+    report["inner_rtol_chosen"] = cfg["inner_rtol"]
     report["chunk_iters"] = cfg["chunk_iters"]
     return cfg, report

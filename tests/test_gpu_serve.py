@@ -295,6 +295,9 @@ def test_serve_tune_small_sample(ctx):
     assert len(rep["windows"]) == len(T.WINDOWS)
     for w in rep["windows"].values():
         assert w["seconds"] > 0 and w["converged"]
+    assert cfg["inner_rtol"] in (1e-2, 1e-1) and rep["inner_rtol_chosen"] == cfg["inner_rtol"]
+    if cfg["mixed_min_nnz"] > 0:
+        assert set(rep["inner_rtol"]) == {"0.01", "0.1"}
 
 
 @pytest.mark.parametrize("precision,solo", [(64, 0), (32, 0), (64, 1)])
