@@ -823,12 +823,15 @@ int hflmp_mpcg_jacobi_sell(hflmp_ctx *ctx, const hflmp_sell *A64,
  * queues the next chunk of chunk_iters iterations of every active system
  * in shared launches (one launch per kernel for all systems, captured as a
  * hipGraph per active set when use_graphs) behind the chunk still running;
- * while they run it admits what fits (uploads and layout kernels on a
- * second stream); then it waits for the older chunk and retires the
- * systems that stopped in it (their x is copied to the caller's buffer),
- * and the admitted systems join, starting with the next chunk queued.  So
- * a step returns with one chunk in flight, and a system's result arrives
- * in the step after the chunk in which it stopped.  Each system keeps its own scalars and partial sums
+ * while they run it admits what fits (the server's admission thread does
+ * the uploads and queues the layout kernels on a second stream); then it
+ * waits for the older chunk and retires the systems that stopped in it
+ * (their x is copied to the caller's buffer), and the admitted systems
+ * whose admission work has finished join, starting with the next chunk
+ * queued (the others at a later step; with nothing running, step waits
+ * for them).  So a step returns with one chunk in flight, and a system's
+ * result arrives in the step after the chunk in which it stopped.  A
+ * server is driven from one thread at a time.  Each system keeps its own scalars and partial sums
  * in a fixed order, so its result is bit-identical alone and in any mix.
  * Solves start from x0 = 0.  FP64 solves stop as IterateControl::Check
  * (common/iterate_control.cc:80-107) on the recursive residual; mixed
@@ -887,12 +890,14 @@ typedef struct {
   long long steps, graph_captures;
   int peak_active;          /* most systems active at once so far          */
   size_t peak_arena_in_use; /* most arena bytes held at once so far        */
-  /* host seconds spent in step() so far: admitting (uploads, layout
-   * kernels queued; while the chunk runs), rebuilding tables, capturing
-   * graphs and launching, and waiting for the chunks' kernels */
+  /* host seconds spent in step() so far: admitting (slots, blocks and
+   * descriptors on the calling thread; the uploads and layout kernels are
+   * the admission thread's), rebuilding tables, capturing graphs and
+   * launching, and waiting for the chunks' kernels */
   double admit_seconds, capture_seconds, wait_seconds;
   long long solo_runs; /* systems started alone so far (solo_min_nnz)     */
-  double join_seconds; /* waiting for admission work after the chunk     */
+  double join_seconds; /* joining admitted systems (waiting for their
+                        * admission work only when nothing runs)          */
   double retire_seconds; /* copying retired systems' x to the callers      */
   /* device seconds of the grouped chunks so far (HIP events around each
    * chunk's work on the context stream: table uploads and its launches);
