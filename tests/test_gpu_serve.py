@@ -202,6 +202,41 @@ def test_serve_fp64_column_widths(ctx):
     assert np.linalg.norm(out[1][0] - xo) <= 1e-8 * np.linalg.norm(xo)
 
 
+def test_serve_mixed_column_widths(ctx):
+    """A mixed system's FP32 operator takes int32 columns where its band
+    does not fit int16 (decided on the device, as for the FP64 operator).
+    In a mix with int16 systems, submitted one per step so that the
+    admission thread's work and the joins interleave with running chunks,
+    its result is bit-identical to the same system solved alone, and every
+    system meets the tolerance on the true residual.  The chunks' device
+    time is reported, within the run's wall time."""
+    import time
+    narrow = request(2, 40, 12)
+    wide = _wide_band_system()
+    sv = hflmp.Server(ctx, 256 * MB, max_active=1, chunk_iters=6)
+    try:
+        (x_alone, i_alone), = run_all(sv, [wide], precision=32)
+    finally:
+        sv.close()
+    assert i_alone["col_bytes"] == 4
+    reqs = [narrow, wide, narrow, request(2, 33, 4)]
+    sv = hflmp.Server(ctx, 256 * MB, max_active=4, chunk_iters=6)
+    t0 = time.perf_counter()
+    try:
+        out = run_all(sv, reqs, submit_every=1, precision=32)
+        st = sv.stats()
+    finally:
+        sv.close()
+    wall = time.perf_counter() - t0
+    assert [i["col_bytes"] for _, i in out] == [2, 4, 2, 2]
+    x_mix, i_mix = out[1]
+    assert i_mix["iters"] == i_alone["iters"] and np.array_equal(x_mix, x_alone)
+    for (rp, ci, v, b), (x, info) in zip(reqs, out):
+        assert info["status"] == 0 and info["precision"] == 32
+        assert np.linalg.norm(b - O.spmv(rp, ci, v, x)) < 1e-8 * np.linalg.norm(b)
+    assert 0 < st["chunk_device_seconds"] <= wall
+
+
 def test_serve_mixed_vs_oracle(ctx):
     reqs = [request(*m) for m in MIX]
     sv = hflmp.Server(ctx, 256 * MB, max_active=8, chunk_iters=8)
